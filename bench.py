@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark of the MRFMap BP inference path on one MI355X.
+
+Workload "real": 13 synthetic keyframes of 640x480 (f=525, c=(319.5,239.5)),
+1 cm voxels in a 4x4x3 m room with 12 boxes and 4 spheres (seed 2), full-ray
+brick activation, FP32 messages.  A step is one BP iteration: every camera's
+ray-message pass + epilogue + the FP64 residual, with all keyframes, lists
+and map state resident in HBM.
+
+Prints one JSON line: updates/s over the timed steps, the time-to-converged
+(max marginal change < 1e-4, from "keyframes uploaded", traversal included),
+the roofline of the ray-message kernel and the CPU baseline (the FP64 oracle
+on a bounded sample of the same workload, one host core).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="real")
+    ap.add_argument("--msg", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-4)
+    return ap.parse_args()
+
+
+def cpu_baseline(wl):
+    """FP64 scalar oracle on keyframes 0 and 6 of the same workload, every 4th
+    pixel row; one BP iteration, timed on one host core."""
+    import numpy as np
+
+    import oracle
+    from mrfmap_amd import synth
+
+    p, frames, _ = synth.make_workload(wl, frames=13)
+    sub = []
+    for k in (0, 6):
+        pose, K, d = frames[k]
+        dd = np.zeros_like(d)
+        dd[::4] = d[::4]
+        sub.append((pose, K, dd))
+    o = oracle.Oracle(p.K, p.rows, p.cols, p.res, p.prior, p.dims, p.const_sigmasq_poly, p.const_bias_poly,
+                      lookup_n=p.lookup_n, activation=p.activation)
+    for pose, K, d in sub:
+        o.add_camera(pose, d, K)
+    o.prepare()
+    t0 = time.perf_counter()
+    st = o.perform_inference(1)
+    dt = time.perf_counter() - t0
+    return {"value": st["updates_per_iter"] / dt, "unit": "ray-msg updates/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle BP, 1 iteration over keyframes 0 and 6 of '{wl}' with every 4th pixel row "
+                      f"({st['updates_per_iter']} updates, {dt:.2f} s)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU split is described in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+    from mrfmap_amd import GVDBInference, _lib, synth
+
+    msg = {"f32": _lib.MSG_F32, "f16": _lib.MSG_F16, "bf16": _lib.MSG_BF16}[a.msg]
+    sbytes = 4 if msg == _lib.MSG_F32 else 2
+    t0 = time.perf_counter()
+    p, frames, _ = synth.make_workload(a.workload, msg_dtype=msg)
+    t_gen = time.perf_counter() - t0
+    print(f"[bench] generated {len(frames)} keyframes in {t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    def build():
+        g = GVDBInference(params=p)
+        for pose, K, depth in frames:
+            g.add_camera_with_depth(pose, depth, K)
+        return g
+
+    # time-to-converged from a fresh map (geometry build included)
+    g = build()
+    conv = g.perform_inference(max_iters=200, tol=a.tol)
+    print(f"[bench] converged: {conv['iters']} iters residual {conv['residual']:.2e} "
+          f"total {conv['t_total_ms']:.1f} ms (geometry {conv['t_geometry_ms']:.1f} ms)", file=sys.stderr, flush=True)
+
+    # throughput: W warmup steps then exactly K timed steps on resident data
+    if a.warmup > 0:
+        g.perform_inference(max_iters=a.warmup)
+    t1 = time.perf_counter()
+    st = g.perform_inference(max_iters=a.steps)
+    dt = time.perf_counter() - t1
+    assert st["iters"] == a.steps
+    n_upd = st["updates_per_iter"]
+    ncam = len(frames)
+    value = n_upd * a.steps / dt
+    launches = a.steps * ncam
+    avg_launch_ms = st["t_ray_kernel_ms"] / launches
+    bytes_per_launch = n_upd / ncam * (2 * sbytes + 4 + 8)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    print(f"[bench] {a.steps} steps in {dt * 1e3:.1f} ms; ray kernel {st['t_ray_kernel_ms']:.1f} ms, "
+          f"epilogue {st['t_epilogue_ms']:.1f} ms", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(a.workload)
+        print(f"[bench] cpu baseline {cpu['value']:.3e} updates/s", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "ray-msg updates/s; time-to-converged map (<=1e-3 vs FP64), warm vs plain BP",
+        "value": value,
+        "unit": "ray-msg updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.msg,
+        "data": "synthetic",
+        "config": {
+            "workload": f"{a.workload}: {ncam} keyframes {p.cols}x{p.rows} f={float(p.K[0][0])}, "
+                        f"{p.res * 100:.0f} cm voxels, {list(map(float, p.dims))} m room, full-ray activation",
+            "updates_per_step": n_upd,
+            "pairs_per_step": st["pairs_per_iter"],
+            "rays": st["rays"],
+            "list_records": st["list_records"],
+        },
+        "time_to_converged_ms": conv["t_total_ms"],
+        "iters_to_converge": conv["iters"],
+        "converged_residual": conv["residual"],
+        "geometry_ms": conv["t_geometry_ms"],
+        "roofline": {
+            "kernel": "k_ray_messages",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": avg_launch_ms,
+            "alg_bytes_per_update": 2 * sbytes + 4 + 8,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

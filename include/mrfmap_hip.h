@@ -1,0 +1,150 @@
+/*
+ * libmrfmap_hip — C ABI of the MI355X BP inference path of MRFMap.
+ *
+ * The library replaces the GVDB/CUDA hot path behind the reference's
+ * GVDBInference object (include/mrfmap/GVDBInference.h:20-103): keyframes
+ * (pose + depth) are added, perform_inference runs loopy belief propagation
+ * over the per-ray forward-sensor factors on one GPU, and the occupancy
+ * marginals are read back brick by brick.  All entry points are synchronous
+ * on return, take plain pointers and sizes, never throw, and return 0 on
+ * success or a negative MRF_E_* code (text via mrf_last_error).  A context is
+ * not thread-safe: one caller thread per context, as in the reference
+ * (GVDBInference.cpp:201 has no lock either).
+ *
+ * Conventions:
+ *   pose  : 4x4 camera-to-world in the optical frame (x right, y down,
+ *           z forward), float[16], row-major unless MRF_COL_MAJOR is given
+ *           (Eigen::MatrixXf default storage is column-major).
+ *   depth : rows x cols float32 metric planar depth; NaN, inf, 0 and
+ *           negatives mark invalid pixels (GVDBImage.cpp:32 maps NaN to -1;
+ *           gvdbkernels.cu:1170-1173 skips d <= 0).  Row-major unless
+ *           MRF_COL_MAJOR.
+ *   map   : voxel v = p/res + (dims_x/2, dims_y/2, 0)/res
+ *           (GVDBCamera.cpp:13-18); bricks are 8^3 voxels; brick rows are
+ *           returned in slot order, voxels inside a brick in x*64 + y*8 + z
+ *           order (gvdbkernels.cu:372).
+ */
+#ifndef MRFMAP_HIP_H
+#define MRFMAP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRF_OK 0
+#define MRF_E_INVALID (-1) /* bad argument or shape */
+#define MRF_E_HIP (-2)     /* HIP runtime error */
+#define MRF_E_NOMEM (-3)   /* device allocation failed */
+#define MRF_E_SHAPE (-4)   /* output buffer shape mismatch (GVDBMRFMap.cpp:174-177 throws) */
+#define MRF_E_RANGE (-5)   /* camera id out of range */
+#define MRF_E_STATE (-6)   /* call not valid in the current state */
+#define MRF_E_LIMIT (-7)   /* a size limit of the implementation is exceeded */
+
+#define MRF_ROW_MAJOR 0
+#define MRF_COL_MAJOR 1
+
+/* Which bricks join the MRF. */
+#define MRF_ACT_ENDPOINT 0 /* bricks within 1 voxel of a depth endpoint (GVDBMRFMap.cpp:117-135) */
+#define MRF_ACT_FULL_RAY 1 /* every brick a truncated ray crosses */
+
+/* Storage type of the per-camera ray->voxel messages M_c. */
+#define MRF_MSG_F32 0
+#define MRF_MSG_F16 1
+#define MRF_MSG_BF16 2
+
+typedef struct mrf_ctx mrf_ctx;
+
+/* Mirrors gvdb_params_t (include/mrfmap/GVDBParams.h:11-215). */
+typedef struct mrf_params {
+  float K[9];                  /* intrinsics, row-major 3x3 */
+  int rows, cols;              /* image size */
+  float res;                   /* voxel size (m) */
+  float prior;                 /* prior occupancy probability */
+  float dims[3];               /* map extent (m) */
+  float occ_thresh, acc_thresh;
+  int max_iters;               /* BP iterations when perform_inference gets max_iters <= 0 */
+  int lookup_n;                /* polynomial coefficients per lookup cell (2 or 3) */
+  float const_sigmasq_poly[3]; /* sigma(z) = p0 + p1 z + p2 z^2 (used as sigma, not sigma^2) */
+  float const_bias_poly[3];    /* bias(z) likewise */
+  const float* depth_sigma_lookup; /* optional ceil(rows/20)*ceil(cols/20)*lookup_n table */
+  const float* depth_bias_lookup;  /* optional, same shape; NULL -> constant polynomials */
+  int activation;              /* MRF_ACT_* */
+  int msg_dtype;               /* MRF_MSG_* */
+  int mask_848_border;         /* blank 24/20 px borders of 848x480 frames (GVDBImage.cpp:35-40) */
+} mrf_params;
+
+/* Per-call statistics of mrf_perform_inference. */
+typedef struct mrf_stats {
+  int iters;                   /* BP iterations run */
+  int converged;               /* residual fell below tol */
+  double residual;             /* max |delta marginal| of the last iteration (FP64 reduction) */
+  int64_t updates_per_iter;    /* ray-msg updates per iteration (voxel incidences) */
+  int64_t pairs_per_iter;      /* (camera, touched voxel) pairs per iteration */
+  int64_t rays;                /* rays with at least one incidence */
+  int64_t list_records;        /* incidence records incl. gap records */
+  double t_geometry_ms;        /* activation + traversal + pair build (0 if cached) */
+  double t_iters_ms;           /* wall time of all iterations (device events) */
+  double t_ray_kernel_ms;      /* summed device time of the ray-message kernel */
+  double t_epilogue_ms;        /* summed device time of the epilogue kernel */
+  double t_total_ms;           /* wall time of the whole call */
+} mrf_stats;
+
+/* gvdb_params_t() defaults (GVDBParams.h:32-55). */
+void mrf_params_default(mrf_params* p);
+
+/* Replaces GVDBInference(create_context, use_gl) + the process-global
+ * gvdb_params_ (GVDBInference.h:36-44, GVDBMRFMap.cpp:23-93). */
+int mrf_create(const mrf_params* p, mrf_ctx** out);
+/* Replaces ~GVDBInference (GVDBInference.h:46-61). */
+void mrf_destroy(mrf_ctx* ctx);
+
+/* Replaces GVDBInference::add_camera_with_depth (GVDBInference.cpp:58-64). */
+int mrf_add_camera_with_depth(mrf_ctx* ctx, const float pose[16], const float* depth, int rows, int cols,
+                              int layout, uint32_t* cam_id);
+/* Same with per-camera intrinsics (K row-major 3x3) for uneven keyframes; no
+ * counterpart in the reference, whose GVDBImage shares one gvdb_params_.K. */
+int mrf_add_camera_with_depth_k(mrf_ctx* ctx, const float pose[16], const float K[9], const float* depth,
+                                int rows, int cols, int layout, uint32_t* cam_id);
+/* Replaces GVDBInference::set_pose (GVDBInference.cpp:66-82): the camera's
+ * rays are re-traversed at the next perform_inference. */
+int mrf_set_pose(mrf_ctx* ctx, uint32_t cam_id, const float pose[16], int layout);
+int mrf_num_cameras(mrf_ctx* ctx, uint32_t* n);
+
+/* Replaces GVDBInference::perform_inference[_ids] (GVDBInference.cpp:84-178).
+ * ids == NULL runs every camera.  max_iters <= 0 uses params.max_iters.
+ * tol > 0 stops early once the max marginal change of an iteration is below
+ * tol.  Per-camera messages are reset at every call, beliefs persist
+ * (GVDBInference.cpp:128-134). */
+int mrf_perform_inference(mrf_ctx* ctx, const uint32_t* ids, int n_ids, int max_iters, double tol,
+                          mrf_stats* out);
+
+/* Number of active bricks (rows of get_occupied_bricks / get_alphas). */
+int mrf_num_bricks(mrf_ctx* ctx, int64_t* n);
+/* Replaces GVDBMRFMap::get_occupied_bricks(0) (GVDBMRFMap.cpp:137-152):
+ * brick centres in voxel units, n x 3 row-major. */
+int mrf_get_occupied_bricks(mrf_ctx* ctx, float* centers, int64_t n);
+/* Replaces GVDBMRFMap::get_alphas (GVDBMRFMap.cpp:154-208): rows x cols must
+ * be num_bricks x 512 (else MRF_E_SHAPE); voxels never updated read 0. */
+int mrf_get_alphas(mrf_ctx* ctx, float* out, int64_t rows, int64_t cols);
+/* Belief log-odds B per voxel, same shape/order as mrf_get_alphas.  No
+ * counterpart (the reference keeps B in a half2 atlas channel). */
+int mrf_get_beliefs(mrf_ctx* ctx, float* out, int64_t rows, int64_t cols);
+
+/* Traversal lists of one camera (for parity tests and tools).  No
+ * counterpart: GVDB recomputed the DDA in every pass.  Ray records are
+ * 6 x uint32 {off, n, z0 bits, inv_norm bits, depth bits, cell | ran_out<<31};
+ * incidence records are 2 x uint32 {voxel or 0xFFFFFFFF for a gap, length bits}.
+ * Call with NULL buffers to get the sizes.  Valid after perform_inference
+ * (or mrf_prepare). */
+int mrf_prepare(mrf_ctx* ctx);
+int mrf_get_traversal(mrf_ctx* ctx, uint32_t cam_id, int64_t* n_rays, int64_t* n_records, uint32_t* rays,
+                      uint32_t* records);
+
+const char* mrf_last_error(mrf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

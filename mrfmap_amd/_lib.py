@@ -1,0 +1,104 @@
+"""ctypes bindings of libmrfmap_hip.so (C ABI in include/mrfmap_hip.h).
+
+The library is built in-tree by ``mrfmap_amd.build``.  There is no CPU
+fallback: if the shared library is missing or no HIP device is present the
+calls fail loudly.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmrfmap_hip.so")
+
+MRF_OK = 0
+ERRORS = {-1: "MRF_E_INVALID", -2: "MRF_E_HIP", -3: "MRF_E_NOMEM", -4: "MRF_E_SHAPE", -5: "MRF_E_RANGE",
+          -6: "MRF_E_STATE", -7: "MRF_E_LIMIT"}
+ROW_MAJOR, COL_MAJOR = 0, 1
+ACT_ENDPOINT, ACT_FULL_RAY = 0, 1
+MSG_F32, MSG_F16, MSG_BF16 = 0, 1, 2
+
+# every symbol include/mrfmap_hip.h declares
+EXPORTS = [
+    "mrf_params_default", "mrf_create", "mrf_destroy", "mrf_add_camera_with_depth",
+    "mrf_add_camera_with_depth_k", "mrf_set_pose", "mrf_num_cameras", "mrf_perform_inference",
+    "mrf_num_bricks", "mrf_get_occupied_bricks", "mrf_get_alphas", "mrf_get_beliefs", "mrf_prepare",
+    "mrf_get_traversal", "mrf_last_error",
+]
+
+
+class MrfParams(ctypes.Structure):
+    _fields_ = [
+        ("K", ctypes.c_float * 9), ("rows", ctypes.c_int), ("cols", ctypes.c_int),
+        ("res", ctypes.c_float), ("prior", ctypes.c_float), ("dims", ctypes.c_float * 3),
+        ("occ_thresh", ctypes.c_float), ("acc_thresh", ctypes.c_float), ("max_iters", ctypes.c_int),
+        ("lookup_n", ctypes.c_int), ("const_sigmasq_poly", ctypes.c_float * 3),
+        ("const_bias_poly", ctypes.c_float * 3), ("depth_sigma_lookup", ctypes.c_void_p),
+        ("depth_bias_lookup", ctypes.c_void_p), ("activation", ctypes.c_int), ("msg_dtype", ctypes.c_int),
+        ("mask_848_border", ctypes.c_int),
+    ]
+
+
+class MrfStats(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int), ("converged", ctypes.c_int), ("residual", ctypes.c_double),
+        ("updates_per_iter", ctypes.c_int64), ("pairs_per_iter", ctypes.c_int64), ("rays", ctypes.c_int64),
+        ("list_records", ctypes.c_int64), ("t_geometry_ms", ctypes.c_double), ("t_iters_ms", ctypes.c_double),
+        ("t_ray_kernel_ms", ctypes.c_double), ("t_epilogue_ms", ctypes.c_double),
+        ("t_total_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MrfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libmrfmap_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m mrfmap_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.mrf_params_default.argtypes = [ctypes.POINTER(MrfParams)]
+    L.mrf_params_default.restype = None
+    L.mrf_create.argtypes = [ctypes.POINTER(MrfParams), ctypes.POINTER(P)]
+    L.mrf_destroy.argtypes = [P]
+    L.mrf_destroy.restype = None
+    L.mrf_add_camera_with_depth.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p]
+    L.mrf_add_camera_with_depth_k.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p]
+    L.mrf_set_pose.argtypes = [P, ctypes.c_uint32, P, ctypes.c_int]
+    L.mrf_num_cameras.argtypes = [P, u32p]
+    L.mrf_perform_inference.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.POINTER(MrfStats)]
+    L.mrf_num_bricks.argtypes = [P, i64p]
+    L.mrf_get_occupied_bricks.argtypes = [P, P, ctypes.c_int64]
+    L.mrf_get_alphas.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64]
+    L.mrf_get_beliefs.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64]
+    L.mrf_prepare.argtypes = [P]
+    L.mrf_get_traversal.argtypes = [P, ctypes.c_uint32, i64p, i64p, P, P]
+    L.mrf_last_error.argtypes = [P]
+    L.mrf_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc, h=None):
+    if rc != MRF_OK:
+        msg = ""
+        if h:
+            m = load().mrf_last_error(h)
+            msg = m.decode() if m else ""
+        raise MrfError(rc, msg)
+    return rc

@@ -1,0 +1,701 @@
+// C ABI of libmrfmap_hip: context, keyframes, geometry build and the BP
+// schedule.  Host orchestration only; the kernels live in traverse.hip,
+// bp.hip and scan.hip.
+//
+// Schedule (GVDBInference::perform_inference_ids, src/GVDBInference.cpp:100-178):
+//   reset M_c of the requested cameras;
+//   repeat max_iters times (or until the FP64 residual drops below tol):
+//     for every camera c in order (Gauss-Seidel):
+//       ray kernel (pre + infinity term + post fused)  -> S
+//       epilogue over c's (voxel, C) pairs             -> B, M_c
+//     residual = max |marginal(B) - marginal(B_prev)|
+// The per-camera summed chord length C (render_cumulative_length_kernel,
+// recomputed every camera pass in the reference) depends on geometry only
+// and is built once per traversal.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mrfmap_hip.h"
+#include "mrf_kernels.h"
+
+using namespace mrf;
+
+namespace {
+
+struct Camera {
+  float pose[16];
+  float K[9];
+  int rows = 0, cols = 0;
+  float* d_depth = nullptr;
+  RayRec* d_rays = nullptr;
+  uint32_t nrays = 0;
+  uint2* d_inc = nullptr;
+  uint64_t nrec = 0, nupd = 0;
+  uint32_t* d_pair_vox = nullptr;
+  float* d_pair_c = nullptr;
+  uint32_t npairs = 0;
+};
+
+size_t msg_size(int t) { return t == MRF_MSG_F32 ? 4 : 2; }
+
+}  // namespace
+
+struct mrf_ctx {
+  mrf_params p;
+  std::vector<float> sig_tab, bia_tab;
+  bool user_tables = false;
+  int tab_rows = 0, tab_cols = 0;
+  float *d_sig = nullptr, *d_bia = nullptr;
+  GridGeom g;
+  int nbricks = 0;
+  float prior_logodds = 0.f;
+  hipStream_t stream = nullptr;
+
+  std::vector<int32_t> h_slot;   // brick directory: linear brick index -> slot or -1
+  std::vector<int32_t> slot_lin; // slot -> linear brick index
+  int32_t* d_slot = nullptr;
+  uint8_t* d_bflags = nullptr;
+  size_t cap_slots = 0;
+
+  float* d_B = nullptr;
+  float* d_A = nullptr;   // marginal at the end of the previous iteration
+  float* d_S = nullptr;   // ray->voxel accumulator of the camera being processed
+  uint8_t* d_upd = nullptr;
+  unsigned long long* d_cfix = nullptr;
+  float* d_out = nullptr; // readback staging
+  void* d_M = nullptr;
+  size_t M_bytes = 0;
+  size_t M_stride = 0;    // elements per camera
+
+  // scratch
+  uint32_t *d_counts = nullptr, *d_has = nullptr, *d_offs = nullptr, *d_rayidx = nullptr;
+  size_t cap_pix = 0;
+  uint64_t* d_scan_tmp = nullptr;
+  size_t cap_scan = 0;
+  uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
+  uint32_t *d_bcnt = nullptr, *d_boff = nullptr;
+  size_t cap_blocks = 0;
+
+  std::vector<Camera> cams;
+  bool geom_dirty = true;
+  std::string err;
+};
+
+namespace {
+
+int fail(mrf_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                         \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return fail(ctx, MRF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+template <typename T>
+int dalloc(mrf_ctx* c, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, MRF_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+  }
+  return MRF_OK;
+}
+
+template <typename T>
+void dfree(T** p) {
+  if (*p) (void)hipFree((void*)*p);
+  *p = nullptr;
+}
+
+void to_row_major(const float* in, int rows, int cols, int layout, std::vector<float>& out) {
+  out.resize((size_t)rows * cols);
+  if (layout == MRF_COL_MAJOR) {
+    for (int r = 0; r < rows; ++r)
+      for (int c = 0; c < cols; ++c) out[(size_t)r * cols + c] = in[(size_t)c * rows + r];
+  } else {
+    std::memcpy(out.data(), in, out.size() * sizeof(float));
+  }
+}
+
+CamGeom make_geom(const mrf_ctx* c, const Camera& cam) {
+  CamGeom g;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) g.R[i * 3 + j] = (double)cam.pose[i * 4 + j];
+    g.t[i] = (double)cam.pose[i * 4 + 3];
+  }
+  g.fx = cam.K[0];
+  g.fy = cam.K[4];
+  g.cx = cam.K[2];
+  g.cy = cam.K[5];
+  g.rows = cam.rows;
+  g.cols = cam.cols;
+  g.lookup_u = (cam.cols + kLookupPx - 1) / kLookupPx;
+  g.mask848 = c->p.mask_848_border;
+  return g;
+}
+
+// (Re)build the sensor-model tables so that every camera's lookup cells exist.
+int ensure_tables(mrf_ctx* c, int rows, int cols) {
+  const int ln = c->p.lookup_n;
+  if (c->user_tables) {
+    if (rows != c->p.rows || cols != c->p.cols)
+      return fail(c, MRF_E_INVALID, "per-pixel lookup tables require every image to be rows x cols of the params");
+    return MRF_OK;
+  }
+  if (c->d_sig && rows <= c->tab_rows && cols <= c->tab_cols) return MRF_OK;
+  c->tab_rows = std::max(rows, c->tab_rows);
+  c->tab_cols = std::max(cols, c->tab_cols);
+  const size_t cells = (size_t)((c->tab_rows + kLookupPx - 1) / kLookupPx) * ((c->tab_cols + kLookupPx - 1) / kLookupPx);
+  c->sig_tab.assign(cells * ln, 0.f);
+  c->bia_tab.assign(cells * ln, 0.f);
+  for (size_t i = 0; i < cells; ++i)
+    for (int j = 0; j < ln; ++j) {
+      c->sig_tab[i * ln + j] = c->p.const_sigmasq_poly[j];
+      c->bia_tab[i * ln + j] = c->p.const_bias_poly[j];
+    }
+  dfree(&c->d_sig);
+  dfree(&c->d_bia);
+  int rc;
+  if ((rc = dalloc(c, &c->d_sig, c->sig_tab.size())) || (rc = dalloc(c, &c->d_bia, c->bia_tab.size()))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_sig, c->sig_tab.data(), c->sig_tab.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_bia, c->bia_tab.data(), c->bia_tab.size() * 4, hipMemcpyHostToDevice));
+  return MRF_OK;
+}
+
+int ensure_pixel_scratch(mrf_ctx* c, size_t npix) {
+  if (npix <= c->cap_pix) return MRF_OK;
+  dfree(&c->d_counts);
+  dfree(&c->d_has);
+  dfree(&c->d_offs);
+  dfree(&c->d_rayidx);
+  int rc;
+  if ((rc = dalloc(c, &c->d_counts, npix)) || (rc = dalloc(c, &c->d_has, npix)) ||
+      (rc = dalloc(c, &c->d_offs, npix)) || (rc = dalloc(c, &c->d_rayidx, npix)))
+    return rc;
+  c->cap_pix = npix;
+  return MRF_OK;
+}
+
+int ensure_scan(mrf_ctx* c, size_t n) {
+  const size_t w = scan_tmp_words(n);
+  if (w <= c->cap_scan) return MRF_OK;
+  dfree(&c->d_scan_tmp);
+  int rc = dalloc(c, &c->d_scan_tmp, w);
+  if (rc) return rc;
+  c->cap_scan = w;
+  return MRF_OK;
+}
+
+uint64_t read_total(mrf_ctx* c) {
+  uint64_t v = 0;
+  (void)hipMemcpyAsync(&v, c->d_total, sizeof(v), hipMemcpyDeviceToHost, c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  return v;
+}
+
+// Grow the voxel pool to hold nslots bricks, preserving B and the update flags.
+int ensure_pool(mrf_ctx* c, size_t nslots) {
+  if (nslots <= c->cap_slots) return MRF_OK;
+  size_t cap = std::max(nslots, c->cap_slots + c->cap_slots / 2);
+  const size_t nv = cap * kBrickVox, old = c->cap_slots * kBrickVox;
+  float *B = nullptr, *A = nullptr, *S = nullptr, *O = nullptr;
+  uint8_t* U = nullptr;
+  unsigned long long* F = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &B, nv)) || (rc = dalloc(c, &A, nv)) || (rc = dalloc(c, &S, nv)) ||
+      (rc = dalloc(c, &O, nv)) || (rc = dalloc(c, &U, nv)) || (rc = dalloc(c, &F, nv))) {
+    dfree(&B); dfree(&A); dfree(&S); dfree(&O); dfree(&U); dfree(&F);
+    return rc;
+  }
+  launch_fill_f32(B, c->prior_logodds, nv, c->stream);
+  launch_zero_words(S, nv * 4, c->stream);
+  launch_zero_words(F, nv * 8, c->stream);
+  launch_fill_u8(U, 0, nv, c->stream);
+  if (old) {
+    HIPCHK(c, hipMemcpyAsync(B, c->d_B, old * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(U, c->d_upd, old, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_S); dfree(&c->d_out); dfree(&c->d_upd); dfree(&c->d_cfix);
+  c->d_B = B; c->d_A = A; c->d_S = S; c->d_out = O; c->d_upd = U; c->d_cfix = F;
+  c->cap_slots = cap;
+  const size_t nb = pair_blocks(nv);
+  if (nb > c->cap_blocks) {
+    dfree(&c->d_bcnt);
+    dfree(&c->d_boff);
+    if ((rc = dalloc(c, &c->d_bcnt, nb)) || (rc = dalloc(c, &c->d_boff, nb))) return rc;
+    c->cap_blocks = nb;
+  }
+  return ensure_scan(c, nb);
+}
+
+void free_camera_lists(Camera& cam) {
+  dfree(&cam.d_rays);
+  dfree(&cam.d_inc);
+  dfree(&cam.d_pair_vox);
+  dfree(&cam.d_pair_c);
+  cam.nrays = 0;
+  cam.nrec = cam.nupd = 0;
+  cam.npairs = 0;
+}
+
+__global__ void k_count_live(const uint2* __restrict__ inc, uint64_t n, unsigned long long* out) {
+  __shared__ unsigned long long sh[256];
+  unsigned long long c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    c += inc[i].x != kGapVox;
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, sh[0]);
+}
+
+int traverse_camera(mrf_ctx* c, Camera& cam) {
+  free_camera_lists(cam);
+  const CamGeom cg = make_geom(c, cam);
+  const size_t npix = (size_t)cam.rows * cam.cols;
+  int rc;
+  if ((rc = ensure_pixel_scratch(c, npix)) || (rc = ensure_scan(c, npix))) return rc;
+  launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, c->stream);
+  launch_exclusive_scan(c->d_counts, c->d_offs, npix, c->d_scan_tmp, c->d_total, c->stream);
+  const uint64_t nrec = read_total(c);
+  launch_exclusive_scan(c->d_has, c->d_rayidx, npix, c->d_scan_tmp, c->d_total, c->stream);
+  const uint64_t nrays = read_total(c);
+  HIPCHK(c, hipGetLastError());
+  if (nrec >= 0xFFFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^32 incidence records in one camera");
+  cam.nrec = nrec;
+  cam.nrays = (uint32_t)nrays;
+  if ((rc = dalloc(c, &cam.d_inc, nrec)) || (rc = dalloc(c, &cam.d_rays, nrays))) return rc;
+  launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_rayidx, cam.d_inc, cam.d_rays,
+              c->stream);
+  // live incidences
+  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 8, c->stream));
+  if (nrec) {
+    unsigned g = (unsigned)std::min<uint64_t>((nrec + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, c->stream, cam.d_inc, nrec,
+                       (unsigned long long*)(c->d_total + 2));
+  }
+  uint64_t live = 0;
+  HIPCHK(c, hipMemcpyAsync(&live, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  cam.nupd = live;
+  // (voxel, C) pairs
+  const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
+  launch_cum_len(cam.d_rays, cam.nrays, cam.d_inc, c->d_cfix, c->stream);
+  launch_build_pairs(c->d_cfix, nvox, c->d_bcnt, c->d_boff, c->d_scan_tmp, c->d_total, nullptr, nullptr, c->stream);
+  const uint64_t np = read_total(c);
+  cam.npairs = (uint32_t)np;
+  if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np))) return rc;
+  launch_build_pairs(c->d_cfix, nvox, c->d_bcnt, c->d_boff, c->d_scan_tmp, c->d_total, cam.d_pair_vox,
+                     cam.d_pair_c, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipGetLastError());
+  return MRF_OK;
+}
+
+int prepare(mrf_ctx* c) {
+  if (!c->geom_dirty) return MRF_OK;
+  int rc;
+  // 1. activation
+  HIPCHK(c, hipMemsetAsync(c->d_bflags, 0, (size_t)c->nbricks, c->stream));
+  for (Camera& cam : c->cams) {
+    const CamGeom cg = make_geom(c, cam);
+    if (c->p.activation == MRF_ACT_FULL_RAY)
+      launch_mark_rays(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_bflags, c->stream);
+    else
+      launch_mark_endpoints(cg, c->g, cam.d_depth, c->d_bflags, c->stream);
+  }
+  std::vector<uint8_t> flags((size_t)c->nbricks);
+  HIPCHK(c, hipMemcpyAsync(flags.data(), c->d_bflags, flags.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipGetLastError());
+  // 2. slots for new bricks, ascending linear brick index
+  bool grew = false;
+  for (int lin = 0; lin < c->nbricks; ++lin) {
+    if (flags[lin] && c->h_slot[lin] < 0) {
+      c->h_slot[lin] = (int32_t)c->slot_lin.size();
+      c->slot_lin.push_back(lin);
+      grew = true;
+    }
+  }
+  if ((rc = ensure_pool(c, c->slot_lin.size()))) return rc;
+  if (grew || true)
+    HIPCHK(c, hipMemcpyAsync(c->d_slot, c->h_slot.data(), c->h_slot.size() * 4, hipMemcpyHostToDevice, c->stream));
+  // 3. traversal lists and pairs of every camera
+  for (Camera& cam : c->cams)
+    if ((rc = traverse_camera(c, cam))) return rc;
+  // 4. message storage: one dense pool-sized array per camera
+  const size_t stride = c->slot_lin.size() * (size_t)kBrickVox;
+  const size_t need = std::max<size_t>(1, stride * c->cams.size() * msg_size(c->p.msg_dtype));
+  if (need > c->M_bytes) {
+    if (c->d_M) (void)hipFree(c->d_M);
+    c->d_M = nullptr;
+    c->M_bytes = 0;
+    if (hipMalloc(&c->d_M, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, MRF_E_NOMEM, "message storage allocation failed");
+    }
+    c->M_bytes = need;
+  }
+  c->M_stride = stride;
+  c->geom_dirty = false;
+  return MRF_OK;
+}
+
+int add_camera(mrf_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols,
+               int layout, uint32_t* cam_id) {
+  if (!pose || !depth || rows <= 0 || cols <= 0 || (layout != MRF_ROW_MAJOR && layout != MRF_COL_MAJOR))
+    return fail(c, MRF_E_INVALID, "add_camera: invalid argument");
+  if (K[0] <= 0 || K[4] <= 0) return fail(c, MRF_E_INVALID, "add_camera: focal lengths must be positive");
+  int rc = ensure_tables(c, rows, cols);
+  if (rc) return rc;
+  Camera cam;
+  if (layout == MRF_COL_MAJOR) {
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) cam.pose[i * 4 + j] = pose[j * 4 + i];
+  } else {
+    std::memcpy(cam.pose, pose, sizeof(cam.pose));
+  }
+  std::memcpy(cam.K, K, sizeof(cam.K));
+  cam.rows = rows;
+  cam.cols = cols;
+  std::vector<float> rm;
+  to_row_major(depth, rows, cols, layout, rm);
+  if ((rc = dalloc(c, &cam.d_depth, rm.size()))) return rc;
+  HIPCHK(c, hipMemcpy(cam.d_depth, rm.data(), rm.size() * 4, hipMemcpyHostToDevice));
+  c->cams.push_back(cam);
+  c->geom_dirty = true;
+  if (cam_id) *cam_id = (uint32_t)(c->cams.size() - 1);
+  return MRF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mrf_params_default(mrf_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  const float hfov = 0.01f;
+  p->res = 0.01f;
+  p->occ_thresh = 0.1f;
+  p->acc_thresh = 0.5f;
+  p->prior = 0.01f;
+  p->max_iters = 1;
+  p->dims[0] = p->dims[1] = p->dims[2] = 1.0f;
+  p->rows = 1;
+  p->cols = 1;
+  p->K[0] = p->K[4] = (p->cols / 2.0f) / tanf(hfov / 2.0f);
+  p->K[2] = p->cols * 0.5f;
+  p->K[5] = p->rows * 0.5f;
+  p->K[8] = 1.0f;
+  p->lookup_n = 3;
+  p->const_sigmasq_poly[0] = 1.5f * p->res;
+  p->activation = MRF_ACT_ENDPOINT;
+  p->msg_dtype = MRF_MSG_F32;
+  p->mask_848_border = 1;
+}
+
+int mrf_create(const mrf_params* p, mrf_ctx** out) {
+  if (!p || !out) return MRF_E_INVALID;
+  *out = nullptr;
+  if (!(p->res > 0) || !(p->prior > 0 && p->prior < 1) || (p->lookup_n != 2 && p->lookup_n != 3) ||
+      p->activation < 0 || p->activation > 1 || p->msg_dtype < 0 || p->msg_dtype > 2)
+    return MRF_E_INVALID;
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) {
+    (void)hipGetLastError();
+    return MRF_E_HIP;
+  }
+  mrf_ctx* c = new mrf_ctx();
+  c->p = *p;
+  c->p.depth_sigma_lookup = nullptr;
+  c->p.depth_bias_lookup = nullptr;
+  int mv[3];
+  for (int a = 0; a < 3; ++a) {
+    mv[a] = (int)std::lround((double)p->dims[a] / (double)p->res);
+    if (mv[a] <= 0 || mv[a] > 2048) {
+      delete c;
+      return MRF_E_LIMIT;
+    }
+  }
+  GridGeom& g = c->g;
+  g.mx = mv[0];
+  g.my = mv[1];
+  g.mz = mv[2];
+  g.nx = (mv[0] + 7) & ~7;
+  g.ny = (mv[1] + 7) & ~7;
+  g.nz = (mv[2] + 7) & ~7;
+  g.bx = g.nx >> kBrickLog;
+  g.by = g.ny >> kBrickLog;
+  g.bz = g.nz >> kBrickLog;
+  g.res = (double)p->res;
+  g.off[0] = 0.5 * (double)mv[0];
+  g.off[1] = 0.5 * (double)mv[1];
+  g.off[2] = 0.0;
+  g.lookup_n = p->lookup_n;
+  c->nbricks = g.bx * g.by * g.bz;
+  c->prior_logodds = logf(p->prior) - logf(1.0f - p->prior);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return MRF_E_HIP;
+  }
+  c->h_slot.assign((size_t)c->nbricks, -1);
+  int rc;
+  if ((rc = dalloc(c, &c->d_slot, (size_t)c->nbricks)) || (rc = dalloc(c, &c->d_bflags, (size_t)c->nbricks)) ||
+      (rc = dalloc(c, &c->d_total, 4))) {
+    mrf_destroy(c);
+    return rc;
+  }
+  if (p->depth_sigma_lookup && p->depth_bias_lookup) {
+    const size_t cells = (size_t)((p->rows + kLookupPx - 1) / kLookupPx) * ((p->cols + kLookupPx - 1) / kLookupPx);
+    c->sig_tab.assign(p->depth_sigma_lookup, p->depth_sigma_lookup + cells * p->lookup_n);
+    c->bia_tab.assign(p->depth_bias_lookup, p->depth_bias_lookup + cells * p->lookup_n);
+    c->user_tables = true;
+    c->tab_rows = p->rows;
+    c->tab_cols = p->cols;
+    if ((rc = dalloc(c, &c->d_sig, c->sig_tab.size())) || (rc = dalloc(c, &c->d_bia, c->bia_tab.size()))) {
+      mrf_destroy(c);
+      return rc;
+    }
+    if (hipMemcpy(c->d_sig, c->sig_tab.data(), c->sig_tab.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_bia, c->bia_tab.data(), c->bia_tab.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      mrf_destroy(c);
+      return MRF_E_HIP;
+    }
+  }
+  *out = c;
+  return MRF_OK;
+}
+
+void mrf_destroy(mrf_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (Camera& cam : c->cams) {
+    free_camera_lists(cam);
+    dfree(&cam.d_depth);
+  }
+  dfree(&c->d_sig); dfree(&c->d_bia); dfree(&c->d_slot); dfree(&c->d_bflags);
+  dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_S); dfree(&c->d_upd); dfree(&c->d_cfix); dfree(&c->d_out);
+  if (c->d_M) (void)hipFree(c->d_M);
+  dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
+  dfree(&c->d_scan_tmp); dfree(&c->d_total); dfree(&c->d_bcnt); dfree(&c->d_boff);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int mrf_add_camera_with_depth(mrf_ctx* c, const float pose[16], const float* depth, int rows, int cols, int layout,
+                              uint32_t* cam_id) {
+  if (!c) return MRF_E_INVALID;
+  if (rows != c->p.rows || cols != c->p.cols)
+    return fail(c, MRF_E_INVALID, "depth image must be params.rows x params.cols");
+  return add_camera(c, pose, c->p.K, depth, rows, cols, layout, cam_id);
+}
+
+int mrf_add_camera_with_depth_k(mrf_ctx* c, const float pose[16], const float K[9], const float* depth, int rows,
+                                int cols, int layout, uint32_t* cam_id) {
+  if (!c || !K) return MRF_E_INVALID;
+  return add_camera(c, pose, K, depth, rows, cols, layout, cam_id);
+}
+
+int mrf_set_pose(mrf_ctx* c, uint32_t id, const float pose[16], int layout) {
+  if (!c || !pose) return MRF_E_INVALID;
+  if (id >= c->cams.size()) return fail(c, MRF_E_RANGE, "set_pose: camera id out of range");
+  Camera& cam = c->cams[id];
+  if (layout == MRF_COL_MAJOR) {
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) cam.pose[i * 4 + j] = pose[j * 4 + i];
+  } else {
+    std::memcpy(cam.pose, pose, sizeof(cam.pose));
+  }
+  c->geom_dirty = true;
+  return MRF_OK;
+}
+
+int mrf_num_cameras(mrf_ctx* c, uint32_t* n) {
+  if (!c || !n) return MRF_E_INVALID;
+  *n = (uint32_t)c->cams.size();
+  return MRF_OK;
+}
+
+int mrf_prepare(mrf_ctx* c) {
+  if (!c) return MRF_E_INVALID;
+  return prepare(c);
+}
+
+int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_iters, double tol, mrf_stats* st) {
+  if (!c) return MRF_E_INVALID;
+  auto t0 = std::chrono::steady_clock::now();
+  mrf_stats s;
+  std::memset(&s, 0, sizeof(s));
+  std::vector<uint32_t> sel;
+  if (ids) {
+    for (int i = 0; i < n_ids; ++i) {
+      if (ids[i] >= c->cams.size()) return fail(c, MRF_E_RANGE, "perform_inference: camera id out of range");
+      sel.push_back(ids[i]);
+    }
+  } else {
+    for (uint32_t i = 0; i < c->cams.size(); ++i) sel.push_back(i);
+  }
+  if (max_iters <= 0) max_iters = c->p.max_iters;
+  const bool dirty = c->geom_dirty;
+  int rc = prepare(c);
+  if (rc) return rc;
+  auto t1 = std::chrono::steady_clock::now();
+  if (dirty) s.t_geometry_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
+  const size_t esz = msg_size(c->p.msg_dtype);
+  // reset M_c of the selected cameras (GVDBInference.cpp:128-134)
+  for (uint32_t id : sel) launch_zero_words((char*)c->d_M + (size_t)id * c->M_stride * esz, c->M_stride * esz, c->stream);
+  launch_marginals(c->d_B, c->d_A, nvox, c->stream);
+  for (const Camera& cam : c->cams) {
+    s.updates_per_iter += (int64_t)cam.nupd;
+    s.pairs_per_iter += cam.npairs;
+    s.rays += cam.nrays;
+    s.list_records += (int64_t)cam.nrec;
+  }
+  const size_t ncam = c->cams.size();
+  std::vector<hipEvent_t> ev(4 * ncam + 2);
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  double tray = 0, tepi = 0, titer = 0;
+  int it = 0;
+  double resid = 0;
+  bool conv = false;
+  for (; it < max_iters; ++it) {
+    HIPCHK(c, hipEventRecord(ev[4 * ncam], c->stream));
+    for (size_t ci = 0; ci < ncam; ++ci) {
+      Camera& cam = c->cams[ci];
+      MsgLaunch a;
+      a.rays = cam.d_rays;
+      a.nrays = cam.nrays;
+      a.inc = cam.d_inc;
+      a.B = c->d_B;
+      a.M = (const char*)c->d_M + ci * c->M_stride * esz;
+      a.S = c->d_S;
+      a.sig = c->d_sig;
+      a.bia = c->d_bia;
+      a.lookup_n = c->p.lookup_n;
+      a.msg_type = c->p.msg_dtype;
+      HIPCHK(c, hipEventRecord(ev[4 * ci + 0], c->stream));
+      launch_ray_messages(a, c->stream);
+      HIPCHK(c, hipEventRecord(ev[4 * ci + 1], c->stream));
+      launch_epilogue(cam.d_pair_vox, cam.d_pair_c, cam.npairs, c->d_B, (char*)c->d_M + ci * c->M_stride * esz,
+                      c->p.msg_dtype, c->d_S, c->d_upd, c->stream);
+      HIPCHK(c, hipEventRecord(ev[4 * ci + 2], c->stream));
+    }
+    HIPCHK(c, hipMemsetAsync(c->d_total + 1, 0, 8, c->stream));
+    launch_residual(c->d_B, c->d_A, nvox, (unsigned long long*)(c->d_total + 1), c->stream);
+    HIPCHK(c, hipEventRecord(ev[4 * ncam + 1], c->stream));
+    uint64_t bits = 0;
+    HIPCHK(c, hipMemcpyAsync(&bits, c->d_total + 1, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    std::memcpy(&resid, &bits, 8);
+    float ms = 0;
+    for (size_t ci = 0; ci < ncam; ++ci) {
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[4 * ci + 0], ev[4 * ci + 1]));
+      tray += ms;
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[4 * ci + 1], ev[4 * ci + 2]));
+      tepi += ms;
+    }
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[4 * ncam], ev[4 * ncam + 1]));
+    titer += ms;
+    if (tol > 0 && resid < tol) {
+      conv = true;
+      ++it;
+      break;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  s.iters = it;
+  s.converged = conv ? 1 : 0;
+  s.residual = resid;
+  s.t_ray_kernel_ms = tray;
+  s.t_epilogue_ms = tepi;
+  s.t_iters_ms = titer;
+  s.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (st) *st = s;
+  return MRF_OK;
+}
+
+int mrf_num_bricks(mrf_ctx* c, int64_t* n) {
+  if (!c || !n) return MRF_E_INVALID;
+  int rc = prepare(c);
+  if (rc) return rc;
+  *n = (int64_t)c->slot_lin.size();
+  return MRF_OK;
+}
+
+int mrf_get_occupied_bricks(mrf_ctx* c, float* centers, int64_t n) {
+  if (!c || (!centers && n)) return MRF_E_INVALID;
+  int rc = prepare(c);
+  if (rc) return rc;
+  if (n != (int64_t)c->slot_lin.size()) return fail(c, MRF_E_SHAPE, "get_occupied_bricks: wrong row count");
+  for (size_t s = 0; s < c->slot_lin.size(); ++s) {
+    const int lin = c->slot_lin[s];
+    const int bz = lin % c->g.bz, by = (lin / c->g.bz) % c->g.by, bx = lin / (c->g.bz * c->g.by);
+    centers[3 * s + 0] = (float)(bx * 8 + 4);
+    centers[3 * s + 1] = (float)(by * 8 + 4);
+    centers[3 * s + 2] = (float)(bz * 8 + 4);
+  }
+  return MRF_OK;
+}
+
+static int read_pool(mrf_ctx* c, float* out, int64_t rows, int64_t cols, bool alphas) {
+  if (!c || (!out && rows)) return MRF_E_INVALID;
+  int rc = prepare(c);
+  if (rc) return rc;
+  if (rows != (int64_t)c->slot_lin.size() || cols != kBrickVox)
+    return fail(c, MRF_E_SHAPE, "Dimensions for the alphas array are incorrect!");
+  const size_t n = (size_t)rows * kBrickVox;
+  if (n == 0) return MRF_OK;
+  if (alphas) {
+    launch_alphas(c->d_B, c->d_upd, c->d_out, n, c->stream);
+    HIPCHK(c, hipMemcpyAsync(out, c->d_out, n * 4, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    HIPCHK(c, hipMemcpyAsync(out, c->d_B, n * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipGetLastError());
+  return MRF_OK;
+}
+
+int mrf_get_alphas(mrf_ctx* c, float* out, int64_t rows, int64_t cols) { return read_pool(c, out, rows, cols, true); }
+int mrf_get_beliefs(mrf_ctx* c, float* out, int64_t rows, int64_t cols) {
+  return read_pool(c, out, rows, cols, false);
+}
+
+int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_records, uint32_t* rays,
+                      uint32_t* records) {
+  if (!c) return MRF_E_INVALID;
+  if (id >= c->cams.size()) return fail(c, MRF_E_RANGE, "get_traversal: camera id out of range");
+  int rc = prepare(c);
+  if (rc) return rc;
+  const Camera& cam = c->cams[id];
+  if (n_rays) *n_rays = cam.nrays;
+  if (n_records) *n_records = (int64_t)cam.nrec;
+  if (rays && cam.nrays) HIPCHK(c, hipMemcpy(rays, cam.d_rays, (size_t)cam.nrays * sizeof(RayRec), hipMemcpyDeviceToHost));
+  if (records && cam.nrec) HIPCHK(c, hipMemcpy(records, cam.d_inc, cam.nrec * sizeof(uint2), hipMemcpyDeviceToHost));
+  return MRF_OK;
+}
+
+const char* mrf_last_error(mrf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+}  // extern "C"

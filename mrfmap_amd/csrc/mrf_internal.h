@@ -1,0 +1,58 @@
+// Internal types shared by the HIP kernels and the host orchestration of
+// libmrfmap_hip.  Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mrf {
+
+// Constants of the live reference path (include/mrfmap/GVDBCommon.cuh:3-11).
+constexpr float kFEps = 1e-4f;         // F_EPS: probability clamp
+constexpr float kZMax = 10.0f;         // Z_MAX
+constexpr int kLookupPx = 20;          // LOOKUP_PX: sensor-model lookup cell (pixels)
+constexpr int kLookupMax = 3096;       // LOOKUP_NUM
+constexpr float kCumLenMin = 0.1f;     // epilogue gate on a camera's summed chord length
+constexpr float kMsgNormEps = 1e-7f;   // 0.001 * F_EPS in the message normaliser
+
+// Fixed-point scale of the integer DDA: origins and directions are kept in
+// units of 2^-kFix voxels.
+constexpr int kFix = 16;
+constexpr double kFixScale = 65536.0;
+
+constexpr int kBrickLog = 3;           // 8^3 voxels per brick
+constexpr int kBrickVox = 512;
+constexpr uint32_t kGapVox = 0xFFFFFFFFu;  // incidence record that only advances z
+
+// Per-ray record of one camera's traversal list.
+struct RayRec {
+  uint32_t off;        // first incidence record
+  uint32_t n;          // number of incidence records (voxels + gap records)
+  float z0;            // planar depth at the entry of the first record (m)
+  float inv_norm;      // metres of planar depth per voxel of chord length
+  float depth;         // measured depth d (m)
+  uint32_t cell_flags; // sensor-model lookup offset (bits 0..30) | kRanOut
+};
+constexpr uint32_t kRanOut = 0x80000000u;  // ray left the active space before truncation
+
+// Camera geometry in the form the kernels consume (all double so that the
+// host oracle and the device agree bit for bit on the fixed-point setup).
+struct CamGeom {
+  double R[9];      // camera-to-world rotation, row-major
+  double t[3];      // camera centre (m)
+  double fx, fy, cx, cy;
+  int rows, cols;
+  int lookup_u;     // ceil(cols / 20)
+  int mask848;      // apply the 848x480 border mask
+};
+
+struct GridGeom {
+  int nx, ny, nz;          // voxel grid (multiple of 8)
+  int mx, my, mz;          // map extent in voxels (endpoint gating)
+  int bx, by, bz;          // brick grid
+  double res;              // voxel size (m)
+  double off[3];           // world->voxel offset (voxels)
+  int lookup_n;            // 2 or 3 coefficients per cell
+};
+
+}  // namespace mrf

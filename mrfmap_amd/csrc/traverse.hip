@@ -1,0 +1,366 @@
+// Ray traversal for the BP inference path: pixel ray set-up, integer 3-D DDA
+// over the brick grid, brick activation and emission of per-ray incidence
+// lists.
+//
+// Replaces GVDB's hierarchical DDA `rayCast` + PREPARE_DDA_LEAF / NEXT_DDA /
+// STEP_DDA as driven by render_cumulative_length_kernel and
+// render_factors_to_node_kernel (src/cuda/gvdbkernels.cu:1164-1226,
+// 1276-1302), and the endpoint activation of GVDBCamera::reprojectAndActivate
+// -> GVDBMRFMap::activateSpace (src/GVDBCamera.cpp:70-112,
+// src/GVDBMRFMap.cpp:117-135).
+//
+// Traversal rule (the contract the CPU oracle restates):
+//  * ray of pixel (x, y): origin O = t/res + off, direction per metre of planar
+//    depth D = R ((x-cx)/fx, (y-cy)/fy, 1) / res, both in voxel units and
+//    rounded to fixed point Oi = rint(O 2^16), Di = rint(D 2^16);
+//  * the ray is walked from max(0, box entry) through the voxel grid; every
+//    voxel boundary crossing is the exact rational z = (k 2^16 - Oi_a) / Di_a,
+//    compared by 64-bit cross multiplication, ties step all tied axes at once;
+//  * a voxel is emitted when its brick is active, with chord length
+//    l = (z_exit - z_entry) |D| evaluated in IEEE double from the rationals;
+//    a run of skipped voxels after the first emission becomes one gap record;
+//  * at the exit of every active brick the walk stops if
+//    z + bias(z) >= d + 3 sigma(z) (src/cuda/gvdbkernels.cu:567-570, 1035-1037);
+//    leaving the grid first marks the ray kRanOut (the "infinity" term,
+//    src/cuda/gvdbkernels.cu:1196-1208).
+// The file is compiled with -ffp-contract=off so each double operation is a
+// single correctly rounded IEEE operation, as in the host oracle.
+
+#include "mrf_internal.h"
+#include "mrf_kernels.h"
+
+namespace mrf {
+
+namespace {
+
+struct Rat {
+  int64_t n, d;  // value n / d with d > 0
+};
+
+__device__ __forceinline__ bool rat_lt(const Rat& a, const Rat& b) { return a.n * b.d < b.n * a.d; }
+
+__device__ __forceinline__ int64_t floor_div(int64_t x, int64_t y, bool* exact) {
+  int64_t q = x / y, r = x % y;
+  if (r != 0 && x < 0) q -= 1;
+  *exact = (r == 0);
+  return q;
+}
+
+struct RaySetup {
+  int64_t Oi[3], Di[3];
+  double norm;  // |D| in voxels per metre of planar depth
+};
+
+// Returns false for rays whose fixed-point set-up would leave the exact range
+// of the 64-bit comparisons (|Oi| >= 2^29 or |Di| >= 2^26).
+__device__ bool setup_ray(const CamGeom& c, const GridGeom& g, int x, int y, RaySetup& rs) {
+  double ux = ((double)x - c.cx) / c.fx;
+  double uy = ((double)y - c.cy) / c.fy;
+  double n2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    double dw = c.R[a * 3 + 0] * ux;
+    dw = dw + c.R[a * 3 + 1] * uy;
+    dw = dw + c.R[a * 3 + 2];
+    double D = dw / g.res;
+    double O = c.t[a] / g.res;
+    O = O + g.off[a];
+    double oi = rint(O * kFixScale), di = rint(D * kFixScale);
+    if (!(fabs(oi) < 536870912.0) || !(fabs(di) < 67108864.0)) return false;
+    rs.Oi[a] = (int64_t)oi;
+    rs.Di[a] = (int64_t)di;
+    double dd = (double)rs.Di[a] / kFixScale;
+    n2 = n2 + dd * dd;
+  }
+  rs.norm = sqrt(n2);
+  return rs.norm > 0.0;
+}
+
+__device__ __forceinline__ double poly_eval(const float* c, int n, double s) {
+  if (n == 3) {
+    double p = (s * s) * (double)c[2];
+    p = p + s * (double)c[1];
+    return p + (double)c[0];
+  }
+  double p = s * (double)c[1];
+  return p + (double)c[0];
+}
+
+__device__ __forceinline__ bool pixel_valid(const CamGeom& c, int x, int y, float d) {
+  if (!(d > 0.0f)) return false;  // NaN, 0 and negatives are invalid
+  if (c.mask848 && c.cols == 848 && c.rows == 480 &&
+      (x < 24 || x >= c.cols - 24 || y < 20 || y >= c.rows - 20))
+    return false;
+  return true;
+}
+
+__device__ __forceinline__ uint32_t lookup_offset(const CamGeom& c, int lookup_n, int x, int y) {
+  int v = c.rows - 1 - y;  // the reference indexes its tables in the vertically flipped image
+  return (uint32_t)((v / kLookupPx) * (c.lookup_u * lookup_n) + (x / kLookupPx) * lookup_n);
+}
+
+enum WalkMode { kMark = 0, kCount = 1, kEmit = 2 };
+
+struct WalkOut {
+  uint32_t n;
+  float z0;
+  bool ran_out;
+};
+
+template <int MODE>
+__device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, const float* sig,
+                        const float* bia, const int32_t* __restrict__ slot,
+                        uint8_t* __restrict__ bflags, uint2* __restrict__ out) {
+  WalkOut wo = {0u, 0.0f, true};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  Rat zin = {0, 1}, zout = {0, 1};
+  bool has_out = false;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t Oi = rs.Oi[a], Di = rs.Di[a], top = (int64_t)N[a] << kFix;
+    if (Di == 0) {
+      if (Oi < 0 || Oi >= top) return wo;
+      continue;
+    }
+    Rat lo, hi;
+    if (Di > 0) {
+      lo = {-Oi, Di};
+      hi = {top - Oi, Di};
+    } else {
+      lo = {Oi - top, -Di};
+      hi = {Oi, -Di};
+    }
+    if (rat_lt(zin, lo)) zin = lo;
+    if (!has_out || rat_lt(hi, zout)) {
+      zout = hi;
+      has_out = true;
+    }
+  }
+  if (!has_out || !rat_lt(zin, zout)) return wo;
+
+  int ix[3], sg[3];
+  Rat t[3];
+  for (int a = 0; a < 3; ++a) {
+    const int64_t Oi = rs.Oi[a], Di = rs.Di[a];
+    bool exact;
+    int64_t X = Oi * zin.d + zin.n * Di;
+    int64_t q = floor_div(X, zin.d << kFix, &exact);
+    if (Di < 0 && exact) q -= 1;
+    if (q < 0) q = 0;
+    if (q > N[a] - 1) q = N[a] - 1;
+    ix[a] = (int)q;
+    sg[a] = Di > 0 ? 1 : (Di < 0 ? -1 : 0);
+    if (Di > 0)
+      t[a] = {(((int64_t)ix[a] + 1) << kFix) - Oi, Di};
+    else if (Di < 0)
+      t[a] = {Oi - ((int64_t)ix[a] << kFix), -Di};
+    else
+      t[a] = {1, 0};
+  }
+
+  const int lookup_n = g.lookup_n;
+  Rat zc = zin;
+  double zc_d = (double)zc.n / (double)zc.d;
+  double last_exit = 0.0;
+  bool any = false, skipped = false;
+  uint32_t cnt = 0;
+  int32_t bslot = 0;
+  auto brick_lookup = [&](void) -> int32_t {
+    const int lin = ((ix[0] >> kBrickLog) * g.by + (ix[1] >> kBrickLog)) * g.bz + (ix[2] >> kBrickLog);
+    if (MODE == kMark) {
+      bflags[lin] = 1;
+      return 0;
+    }
+    return slot[lin];
+  };
+  bslot = brick_lookup();
+  const int max_steps = g.nx + g.ny + g.nz + 3;
+  for (int step = 0; step < max_steps; ++step) {
+    int best = -1;
+    for (int a = 0; a < 3; ++a) {
+      if (sg[a] == 0) continue;
+      if (best < 0 || rat_lt(t[a], t[best])) best = a;
+    }
+    const Rat zn = t[best];
+    const double zn_d = (double)zn.n / (double)zn.d;
+    const bool active = bslot >= 0;
+    if (active) {
+      if (MODE != kMark) {
+        if (any && skipped) {
+          if (MODE == kEmit) {
+            float gl = (float)((zc_d - last_exit) * rs.norm);
+            out[cnt] = make_uint2(kGapVox, __float_as_uint(gl));
+          }
+          ++cnt;
+          skipped = false;
+        }
+        if (!any) {
+          wo.z0 = (float)zc_d;
+          any = true;
+        }
+        if (MODE == kEmit) {
+          const uint32_t local = (uint32_t)((((ix[0] & 7) << 3) | (ix[1] & 7)) << 3 | (ix[2] & 7));
+          const uint32_t vox = (uint32_t)bslot * kBrickVox + local;
+          float len = (float)((zn_d - zc_d) * rs.norm);
+          out[cnt] = make_uint2(vox, __float_as_uint(len));
+        }
+        ++cnt;
+        last_exit = zn_d;
+      }
+    } else if (any) {
+      skipped = true;
+    }
+    int nx[3] = {ix[0], ix[1], ix[2]};
+    for (int a = 0; a < 3; ++a) {
+      if (sg[a] != 0 && !rat_lt(zn, t[a])) {
+        nx[a] += sg[a];
+        t[a].n += ((int64_t)1 << kFix);
+      }
+    }
+    const bool leaving = nx[0] < 0 || nx[0] >= N[0] || nx[1] < 0 || nx[1] >= N[1] || nx[2] < 0 ||
+                         nx[2] >= N[2];
+    const bool brick_change = leaving || ((nx[0] >> kBrickLog) != (ix[0] >> kBrickLog)) ||
+                              ((nx[1] >> kBrickLog) != (ix[1] >> kBrickLog)) ||
+                              ((nx[2] >> kBrickLog) != (ix[2] >> kBrickLog));
+    if (brick_change && active) {
+      const float* sc = sig;
+      const float* bc = bia;
+      const double lhs = zn_d + poly_eval(bc, lookup_n, zn_d);
+      const double rhs = depth + 3.0 * poly_eval(sc, lookup_n, zn_d);
+      if (lhs >= rhs) {
+        wo.ran_out = false;
+        break;
+      }
+    }
+    if (leaving) break;
+    ix[0] = nx[0];
+    ix[1] = nx[1];
+    ix[2] = nx[2];
+    zc = zn;
+    zc_d = zn_d;
+    if (brick_change) bslot = brick_lookup();
+  }
+  wo.n = cnt;
+  return wo;
+}
+
+struct PixelCtx {
+  bool ok;
+  int x, y;
+  float d;
+  RaySetup rs;
+  uint32_t cell;
+};
+
+__device__ __forceinline__ PixelCtx load_pixel(const CamGeom& c, const GridGeom& g, const float* depth,
+                                               int pix) {
+  PixelCtx p;
+  p.ok = false;
+  p.x = pix % c.cols;
+  p.y = pix / c.cols;
+  p.d = depth[pix];
+  if (!pixel_valid(c, p.x, p.y, p.d)) return p;
+  if (!setup_ray(c, g, p.x, p.y, p.rs)) return p;
+  p.cell = lookup_offset(c, g.lookup_n, p.x, p.y);
+  p.ok = true;
+  return p;
+}
+
+__global__ void k_mark_rays(CamGeom c, GridGeom g, const float* __restrict__ depth,
+                            const float* __restrict__ sig, const float* __restrict__ bia,
+                            uint8_t* __restrict__ bflags) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= c.rows * c.cols) return;
+  PixelCtx p = load_pixel(c, g, depth, pix);
+  if (!p.ok) return;
+  walk<kMark>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, nullptr, bflags, nullptr);
+}
+
+__global__ void k_mark_endpoints(CamGeom c, GridGeom g, const float* __restrict__ depth,
+                                 uint8_t* __restrict__ bflags) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= c.rows * c.cols) return;
+  PixelCtx p = load_pixel(c, g, depth, pix);
+  if (!p.ok) return;
+  const int M[3] = {g.mx, g.my, g.mz};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    double P = (double)p.d * (double)p.rs.Di[a];
+    P = (double)p.rs.Oi[a] + P;
+    P = P / kFixScale;
+    if (!(P >= 0.0 && P < (double)M[a])) return;
+    int l = (int)floor(P - 1.0), h = (int)floor(P + 1.0);
+    if (l < 0) l = 0;
+    if (h > N[a] - 1) h = N[a] - 1;
+    lo[a] = l >> kBrickLog;
+    hi[a] = h >> kBrickLog;
+  }
+  for (int i = lo[0]; i <= hi[0]; ++i)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int k = lo[2]; k <= hi[2]; ++k) bflags[(i * g.by + j) * g.bz + k] = 1;
+}
+
+__global__ void k_count(CamGeom c, GridGeom g, const float* __restrict__ depth,
+                        const float* __restrict__ sig, const float* __restrict__ bia,
+                        const int32_t* __restrict__ slot, uint32_t* __restrict__ counts,
+                        uint32_t* __restrict__ has) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= c.rows * c.cols) return;
+  PixelCtx p = load_pixel(c, g, depth, pix);
+  uint32_t n = 0;
+  if (p.ok) n = walk<kCount>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, nullptr).n;
+  counts[pix] = n;
+  has[pix] = n > 0 ? 1u : 0u;
+}
+
+__global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth,
+                       const float* __restrict__ sig, const float* __restrict__ bia,
+                       const int32_t* __restrict__ slot, const uint32_t* __restrict__ offs,
+                       const uint32_t* __restrict__ ray_idx, uint2* __restrict__ inc,
+                       RayRec* __restrict__ rays) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= c.rows * c.cols) return;
+  PixelCtx p = load_pixel(c, g, depth, pix);
+  if (!p.ok) return;
+  const uint32_t off = offs[pix];
+  WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, inc + off);
+  if (wo.n == 0) return;
+  RayRec r;
+  r.off = off;
+  r.n = wo.n;
+  r.z0 = wo.z0;
+  r.inv_norm = (float)(1.0 / p.rs.norm);
+  r.depth = p.d;
+  r.cell_flags = p.cell | (wo.ran_out ? kRanOut : 0u);
+  rays[ray_idx[pix]] = r;
+}
+
+}  // namespace
+
+void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
+                      const float* bia, uint8_t* bflags, hipStream_t s) {
+  const int n = c.rows * c.cols;
+  hipLaunchKernelGGL(k_mark_rays, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, bflags);
+}
+
+void launch_mark_endpoints(const CamGeom& c, const GridGeom& g, const float* depth, uint8_t* bflags,
+                           hipStream_t s) {
+  const int n = c.rows * c.cols;
+  hipLaunchKernelGGL(k_mark_endpoints, dim3((n + 255) / 256), dim3(256), 0, s, c, g, depth, bflags);
+}
+
+void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
+                  const float* bia, const int32_t* slot, uint32_t* counts, uint32_t* has, hipStream_t s) {
+  const int n = c.rows * c.cols;
+  hipLaunchKernelGGL(k_count, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, counts,
+                     has);
+}
+
+void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
+                 const float* bia, const int32_t* slot, const uint32_t* offs, const uint32_t* ray_idx,
+                 uint2* inc, RayRec* rays, hipStream_t s) {
+  const int n = c.rows * c.cols;
+  hipLaunchKernelGGL(k_emit, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, offs,
+                     ray_idx, inc, rays);
+}
+
+}  // namespace mrf

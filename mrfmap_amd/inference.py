@@ -1,0 +1,144 @@
+"""GVDBInference: Python surface of the BP inference path.
+
+Mirrors the reference's pybind11 class (src/GVDBPyModules.cpp:64-83) and the
+C++ class it wraps (include/mrfmap/GVDBInference.h:20-103) for the hot path:
+``add_camera``, ``add_camera_with_depth``, ``set_pose``, ``perform_inference``,
+``perform_inference_ids``, ``get_occupied_bricks``, ``get_indices_and_alphas``.
+Every call goes through the C ABI of libmrfmap_hip.so; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .params import global_params
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class GVDBImage:
+    """A depth keyframe (include/mrfmap/GVDBImage.h).  The reference uploads
+    it to the GPU on construction; here it is kept on the host until the
+    image is attached to a camera."""
+
+    def __init__(self, image, params=None):
+        self.params = params or global_params()
+        self.depth = np.ascontiguousarray(np.asarray(image, dtype=np.float32))
+        if self.depth.shape != (self.params.rows, self.params.cols):
+            raise ValueError(f"depth image shape {self.depth.shape} != ({self.params.rows}, {self.params.cols})")
+
+    def get_reproj_points(self):
+        """Camera-frame points of valid pixels (GVDBImage.cpp:44-60, with fy for
+        the vertical axis and without the reference's dropped last point)."""
+        K = np.asarray(self.params.K, dtype=np.float64)
+        v, u = np.nonzero(self.depth > 0)
+        d = self.depth[v, u].astype(np.float64)
+        return np.stack([(u - K[0, 2]) / K[0, 0] * d, (v - K[1, 2]) / K[1, 1] * d, d], axis=1).astype(np.float32)
+
+
+class GVDBInference:
+    def __init__(self, create_context=True, use_gl=False, params=None):
+        self.params = (params or global_params()).copy()
+        self._L = _lib.load()
+        self._cp = self.params.to_c()
+        self._h = ctypes.c_void_p()
+        _lib.check(self._L.mrf_create(ctypes.byref(self._cp), ctypes.byref(self._h)))
+        self.last_stats = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mrf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ keyframes
+    def add_camera_with_depth(self, pose, depth, K=None):
+        pose = np.ascontiguousarray(np.asarray(pose, dtype=np.float32).reshape(4, 4))
+        depth = np.ascontiguousarray(np.asarray(depth, dtype=np.float32))
+        cid = ctypes.c_uint32()
+        if K is None:
+            rc = self._L.mrf_add_camera_with_depth(self._h, _ptr(pose), _ptr(depth), depth.shape[0],
+                                                   depth.shape[1], _lib.ROW_MAJOR, ctypes.byref(cid))
+        else:
+            Kc = np.ascontiguousarray(np.asarray(K, dtype=np.float32).reshape(3, 3))
+            rc = self._L.mrf_add_camera_with_depth_k(self._h, _ptr(pose), _ptr(Kc), _ptr(depth), depth.shape[0],
+                                                     depth.shape[1], _lib.ROW_MAJOR, ctypes.byref(cid))
+        _lib.check(rc, self._h)
+        return cid.value
+
+    def add_camera(self, pose, image):
+        return self.add_camera_with_depth(pose, image.depth)
+
+    def set_pose(self, cam_id, pose):
+        pose = np.ascontiguousarray(np.asarray(pose, dtype=np.float32).reshape(4, 4))
+        _lib.check(self._L.mrf_set_pose(self._h, int(cam_id), _ptr(pose), _lib.ROW_MAJOR), self._h)
+
+    def num_cameras(self):
+        n = ctypes.c_uint32()
+        _lib.check(self._L.mrf_num_cameras(self._h, ctypes.byref(n)), self._h)
+        return n.value
+
+    # ------------------------------------------------------------ inference
+    def perform_inference(self, max_iters=0, tol=0.0):
+        return self.perform_inference_ids(None, False, max_iters=max_iters, tol=tol)
+
+    def perform_inference_ids(self, cam_ids, dry_run=False, max_iters=0, tol=0.0):
+        st = _lib.MrfStats()
+        if cam_ids is None:
+            rc = self._L.mrf_perform_inference(self._h, None, 0, int(max_iters), float(tol), ctypes.byref(st))
+        else:
+            ids = np.ascontiguousarray(cam_ids, dtype=np.uint32)
+            rc = self._L.mrf_perform_inference(self._h, _ptr(ids), len(ids), int(max_iters), float(tol),
+                                               ctypes.byref(st))
+        _lib.check(rc, self._h)
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def prepare(self):
+        _lib.check(self._L.mrf_prepare(self._h), self._h)
+
+    # ------------------------------------------------------------ readback
+    def num_bricks(self):
+        n = ctypes.c_int64()
+        _lib.check(self._L.mrf_num_bricks(self._h, ctypes.byref(n)), self._h)
+        return n.value
+
+    def get_occupied_bricks(self, lev=0):
+        if lev != 0:
+            raise ValueError("only level 0 (8^3 bricks) exists in the brick grid")
+        n = self.num_bricks()
+        out = np.zeros((n, 3), dtype=np.float32)
+        _lib.check(self._L.mrf_get_occupied_bricks(self._h, _ptr(out), n), self._h)
+        return out
+
+    def get_indices_and_alphas(self, out):
+        """Fill ``out`` (num_bricks x 512 float32, C-contiguous) with the marginals."""
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float32 and out.flags.c_contiguous and out.ndim == 2):
+            raise TypeError("out must be a C-contiguous float32 2-D array")
+        _lib.check(self._L.mrf_get_alphas(self._h, _ptr(out), out.shape[0], out.shape[1]), self._h)
+        return out
+
+    def get_alphas(self):
+        out = np.zeros((self.num_bricks(), 512), dtype=np.float32)
+        return self.get_indices_and_alphas(out)
+
+    def get_beliefs(self):
+        out = np.zeros((self.num_bricks(), 512), dtype=np.float32)
+        _lib.check(self._L.mrf_get_beliefs(self._h, _ptr(out), out.shape[0], out.shape[1]), self._h)
+        return out
+
+    def get_traversal(self, cam_id):
+        nr, nc = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self._L.mrf_get_traversal(self._h, int(cam_id), ctypes.byref(nr), ctypes.byref(nc), None, None),
+                   self._h)
+        rays = np.zeros((nr.value, 6), dtype=np.uint32)
+        recs = np.zeros((nc.value, 2), dtype=np.uint32)
+        _lib.check(self._L.mrf_get_traversal(self._h, int(cam_id), None, None, _ptr(rays), _ptr(recs)), self._h)
+        return rays, recs

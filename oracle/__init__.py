@@ -1,0 +1,260 @@
+"""CPU oracle of the MRFMap BP inference path — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this package, and only as the checker / CPU baseline.  The
+product (``mrfmap_amd``) never imports it.
+
+It wraps ``liboracle.so`` (oracle.c, octomap_eq.c; scalar C, FP64), a
+restatement of the reference's live BP math:
+  traversal  GVDB HDDA driven by src/cuda/gvdbkernels.cu:519-571, 689-1038,
+             1164-1226, 1276-1302 (GVDB not vendored: rule defined here)
+  messages   src/cuda/gvdbkernels.cu:46-72, 689-1038, 1196-1208
+  epilogue   src/cuda/gvdbkernels.cu:457-479
+  schedule   src/GVDBInference.cpp:100-178
+  readback   src/GVDBMRFMap.cpp:137-208, src/cuda/gvdbkernels.cu:360-373
+  octomap    src/GVDBOctomapWrapper.cpp:3-43 + OctoMap's published insertion
+
+The reference itself (C++/CUDA + GVDB fetched over the network) cannot be
+built in this image, so there is no ``oracle/_ref``.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+SOURCES = ["oracle.c", "octomap_eq.c"]
+
+
+def build(verbose=False):
+    """Compile liboracle.so with gcc (IEEE double, no FMA contraction)."""
+    tmp = LIB + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math",
+           *[os.path.join(HERE, s) for s in SOURCES], "-o", tmp, "-lm"]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+_lib = None
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("K", ctypes.c_float * 9), ("rows", ctypes.c_int), ("cols", ctypes.c_int),
+        ("res", ctypes.c_float), ("prior", ctypes.c_float), ("dims", ctypes.c_float * 3),
+        ("lookup_n", ctypes.c_int), ("sigma_poly", ctypes.c_float * 3), ("bias_poly", ctypes.c_float * 3),
+        ("activation", ctypes.c_int), ("mask_848_border", ctypes.c_int),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.c_void_p
+        f64p = ctypes.POINTER(ctypes.c_double)
+        L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(P)]
+        L.orc_destroy.argtypes = [P]
+        L.orc_add_camera.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int]
+        L.orc_set_pose.argtypes = [P, ctypes.c_int, P]
+        L.orc_prepare.argtypes = [P]
+        L.orc_num_bricks.argtypes = [P]
+        L.orc_num_bricks.restype = ctypes.c_int64
+        L.orc_brick_coords.argtypes = [P, P]
+        L.orc_get_traversal.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64), P, P]
+        L.orc_perform_inference.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p,
+                                            ctypes.POINTER(ctypes.c_int64)]
+        L.orc_get_alphas.argtypes = [P, P]
+        L.orc_get_beliefs.argtypes = [P, P]
+        L.orc_ray_messages.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_double, P, P, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
+        L.orc_ray_messages.restype = ctypes.c_double
+        L.orc_ray_messages_bruteforce.argtypes = [ctypes.c_int, P, P, ctypes.c_double, P]
+        L.orc_depth_psi.argtypes = [ctypes.c_double, ctypes.c_double, P, P, ctypes.c_int]
+        L.orc_depth_psi.restype = ctypes.c_double
+        L.orc_walk_ray.argtypes = [P, P, P, ctypes.c_double, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int,
+                                   P, P, P, ctypes.POINTER(ctypes.c_int)]
+        L.oct_create.argtypes = [ctypes.c_double, P, P, ctypes.c_double, ctypes.POINTER(P)]
+        L.oct_destroy.argtypes = [P]
+        L.oct_insert_point_cloud.argtypes = [P, P, ctypes.c_int64, P]
+        L.oct_insert_point_cloud.restype = ctypes.c_int64
+        L.oct_num_nodes.argtypes = [P]
+        L.oct_num_nodes.restype = ctypes.c_int64
+        L.oct_logodds_at.argtypes = [P, P]
+        L.oct_logodds_at.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """FP64 scalar BP over the same keyframes the GPU path sees."""
+
+    def __init__(self, K, rows, cols, res, prior, dims, sigma_poly=None, bias_poly=None, lookup_n=3,
+                 activation=1, mask_848_border=1):
+        p = OrcParams()
+        p.K[:] = [float(v) for v in np.asarray(K, dtype=np.float32).reshape(9)]
+        p.rows, p.cols = int(rows), int(cols)
+        p.res, p.prior = float(res), float(prior)
+        p.dims[:] = [float(v) for v in dims]
+        p.lookup_n = int(lookup_n)
+        sp = sigma_poly if sigma_poly is not None else [1.5 * res, 0.0, 0.0]
+        bp = bias_poly if bias_poly is not None else [0.0, 0.0, 0.0]
+        p.sigma_poly[:] = [float(v) for v in list(sp) + [0.0] * (3 - len(sp))]
+        p.bias_poly[:] = [float(v) for v in list(bp) + [0.0] * (3 - len(bp))]
+        p.activation = int(activation)
+        p.mask_848_border = int(mask_848_border)
+        self._h = ctypes.c_void_p()
+        rc = lib().orc_create(ctypes.byref(p), ctypes.byref(self._h))
+        if rc != 0:
+            raise ValueError(f"orc_create failed ({rc})")
+        self.K = np.asarray(K, dtype=np.float32).reshape(9)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_destroy(h)
+            self._h = None
+
+    def add_camera(self, pose, depth, K=None):
+        pose = np.ascontiguousarray(pose, dtype=np.float32).reshape(16)
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        Kc = self.K if K is None else np.ascontiguousarray(K, dtype=np.float32).reshape(9)
+        rc = lib().orc_add_camera(self._h, _ptr(pose), _ptr(Kc), _ptr(depth), depth.shape[0], depth.shape[1])
+        if rc < 0:
+            raise ValueError(f"orc_add_camera failed ({rc})")
+        return rc
+
+    def set_pose(self, cam, pose):
+        pose = np.ascontiguousarray(pose, dtype=np.float32).reshape(16)
+        if lib().orc_set_pose(self._h, int(cam), _ptr(pose)) != 0:
+            raise IndexError(cam)
+
+    def prepare(self):
+        lib().orc_prepare(self._h)
+
+    def num_bricks(self):
+        return int(lib().orc_num_bricks(self._h))
+
+    def brick_coords(self):
+        out = np.zeros((self.num_bricks(), 3), dtype=np.int32)
+        lib().orc_brick_coords(self._h, _ptr(out))
+        return out
+
+    def traversal(self, cam):
+        nr, nc = ctypes.c_int64(), ctypes.c_int64()
+        lib().orc_get_traversal(self._h, int(cam), ctypes.byref(nr), ctypes.byref(nc), None, None)
+        rays = np.zeros((nr.value, 6), dtype=np.uint32)
+        recs = np.zeros((nc.value, 2), dtype=np.uint32)
+        lib().orc_get_traversal(self._h, int(cam), None, None, _ptr(rays), _ptr(recs))
+        return rays, recs
+
+    def perform_inference(self, max_iters, tol=0.0, ids=None):
+        res = ctypes.c_double()
+        upd = ctypes.c_int64()
+        if ids is not None:
+            ida = np.ascontiguousarray(ids, dtype=np.int32)
+            it = lib().orc_perform_inference(self._h, _ptr(ida), len(ida), int(max_iters), float(tol),
+                                             ctypes.byref(res), ctypes.byref(upd))
+        else:
+            it = lib().orc_perform_inference(self._h, None, 0, int(max_iters), float(tol), ctypes.byref(res),
+                                             ctypes.byref(upd))
+        if it < 0:
+            raise RuntimeError(f"orc_perform_inference failed ({it})")
+        return {"iters": it, "residual": res.value, "updates_per_iter": upd.value}
+
+    def alphas(self):
+        out = np.zeros((self.num_bricks(), 512), dtype=np.float64)
+        lib().orc_get_alphas(self._h, _ptr(out))
+        return out
+
+    def beliefs(self):
+        out = np.zeros((self.num_bricks(), 512), dtype=np.float64)
+        lib().orc_get_beliefs(self._h, _ptr(out))
+        return out
+
+
+def ray_messages(nin, length, s, d, sigma_poly, bias_poly=(0.0, 0.0, 0.0), live=None, lookup_n=3,
+                 ran_out=False, psi_inf=1.0, mode=0):
+    """Outgoing messages p_i of one ray; returns (p, T)."""
+    nin, length, s = _f64(nin), _f64(length), _f64(s)
+    n = len(nin)
+    live = np.ones(n, dtype=np.int32) if live is None else np.ascontiguousarray(live, dtype=np.int32)
+    sg = _f64(list(sigma_poly) + [0.0] * (3 - len(sigma_poly)))
+    bi = _f64(list(bias_poly) + [0.0] * (3 - len(bias_poly)))
+    p = np.zeros(n, dtype=np.float64)
+    T = lib().orc_ray_messages(n, _ptr(nin), _ptr(length), _ptr(s), _ptr(live), float(d), _ptr(sg), _ptr(bi),
+                               int(lookup_n), int(bool(ran_out)), float(psi_inf), int(mode), _ptr(p))
+    return p, T
+
+
+def ray_messages_bruteforce(nin, psi, psi_inf):
+    nin, psi = _f64(nin), _f64(psi)
+    p = np.zeros(len(nin), dtype=np.float64)
+    if lib().orc_ray_messages_bruteforce(len(nin), _ptr(nin), _ptr(psi), float(psi_inf), _ptr(p)) != 0:
+        raise ValueError("n out of range")
+    return p
+
+
+def depth_psi(s, d, sigma_poly, bias_poly=(0.0, 0.0, 0.0), lookup_n=3):
+    sg = _f64(list(sigma_poly) + [0.0] * (3 - len(sigma_poly)))
+    bi = _f64(list(bias_poly) + [0.0] * (3 - len(bias_poly)))
+    return lib().orc_depth_psi(float(s), float(d), _ptr(sg), _ptr(bi), int(lookup_n))
+
+
+def walk_ray(O, D, N, depth=0.0, trunc=False, sigma_poly=(0.015, 0, 0), bias_poly=(0, 0, 0), lookup_n=3, cap=100000):
+    O, D = _f64(O), _f64(D)
+    Na = np.ascontiguousarray(N, dtype=np.int32)
+    sg, bi = _f64(sigma_poly), _f64(bias_poly)
+    vox = np.zeros((cap, 3), dtype=np.int32)
+    ln = np.zeros(cap, dtype=np.float64)
+    zx = np.zeros(cap, dtype=np.float64)
+    ro = ctypes.c_int()
+    m = lib().orc_walk_ray(_ptr(O), _ptr(D), _ptr(Na), float(depth), int(bool(trunc)), _ptr(sg), _ptr(bi),
+                           int(lookup_n), cap, _ptr(vox), _ptr(ln), _ptr(zx), ctypes.byref(ro))
+    return vox[:m], ln[:m], zx[:m], bool(ro.value)
+
+
+class OctomapEq:
+    """OctoMap-equivalent insertion (CPU comparator; not OctoMap itself)."""
+
+    def __init__(self, res, dims, occ_thresh=0.5):
+        bmin = _f64([-dims[0] * 0.5, -dims[1] * 0.5, 0.0])
+        bmax = _f64([dims[0] * 0.5, dims[1] * 0.5, dims[2]])
+        self._h = ctypes.c_void_p()
+        if lib().oct_create(float(res), _ptr(bmin), _ptr(bmax), float(occ_thresh), ctypes.byref(self._h)) != 0:
+            raise ValueError("oct_create failed")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().oct_destroy(h)
+            self._h = None
+
+    def insert_point_cloud(self, pts, origin):
+        pts = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        o = _f64(origin)
+        return int(lib().oct_insert_point_cloud(self._h, _ptr(pts), len(pts), _ptr(o)))
+
+    def num_nodes(self):
+        return int(lib().oct_num_nodes(self._h))
+
+    def logodds_at(self, p):
+        return lib().oct_logodds_at(self._h, _ptr(_f64(p)))

@@ -1,0 +1,835 @@
+/*
+ * CPU oracle of the MRFMap BP inference path — TEST INFRASTRUCTURE ONLY
+ * (see oracle.h).  Scalar, FP64, single-threaded.
+ *
+ * The traversal is deliberately written differently from the GPU's
+ * incremental DDA: all boundary crossings of the ray inside the grid are
+ * collected per axis as exact rationals, merged, and each interval between
+ * consecutive distinct crossings is assigned the voxel that contains its
+ * midpoint (exact 128-bit arithmetic).  The float outputs (chord lengths,
+ * entry depth) are then derived by the same IEEE formulas the traversal
+ * contract fixes, so GPU and oracle lists must agree bit for bit.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FIX 16
+#define FIXS 65536.0
+#define F_EPS 1e-4
+#define GAP 0xFFFFFFFFu
+#define LOOKUP_PX 20
+#define RAN_OUT 0x80000000u
+
+typedef __int128 i128;
+typedef struct {
+  int64_t n, d; /* n / d, d > 0 */
+} rat;
+
+static int rlt(rat a, rat b) { return (i128)a.n * b.d < (i128)b.n * a.d; }
+static int req(rat a, rat b) { return (i128)a.n * b.d == (i128)b.n * a.d; }
+static double rdbl(rat a) { return (double)a.n / (double)a.d; }
+
+/* floor(a / b) for b > 0 */
+static i128 floordiv128(i128 a, i128 b) {
+  i128 q = a / b;
+  if ((a % b != 0) && (a < 0)) q -= 1;
+  return q;
+}
+
+static double poly(const double c[3], int n, double s) {
+  /* same operation order as the traversal contract */
+  if (n == 3) {
+    double p = (s * s) * c[2];
+    p = p + s * c[1];
+    return p + c[0];
+  }
+  double p = s * c[1];
+  return p + c[0];
+}
+
+double orc_depth_psi(double s, double d, const double sig[3], const double bia[3], int lookup_n) {
+  /* depth_potential, gvdbkernels.cu:57-72: N(d; s + bias(s), (3 sigma(s))^2) */
+  double diff = fabs(d - (s + poly(bia, lookup_n, s)));
+  double sg3 = 3.0 * poly(sig, lookup_n, s);
+  double u = diff / sg3;
+  return 0.398942 * exp(-0.5 * u * u) / sg3;
+}
+
+/* ------------------------------------------------------------------ walk */
+
+typedef struct {
+  int32_t v[3];
+  rat z0, z1;
+} seg_t;
+
+typedef struct {
+  seg_t* s;
+  int n, cap;
+} segv;
+
+static void segv_push(segv* sv, const int32_t v[3], rat a, rat b) {
+  if (sv->n == sv->cap) {
+    sv->cap = sv->cap ? 2 * sv->cap : 256;
+    sv->s = (seg_t*)realloc(sv->s, sizeof(seg_t) * sv->cap);
+  }
+  seg_t* e = &sv->s[sv->n++];
+  memcpy(e->v, v, sizeof(e->v));
+  e->z0 = a;
+  e->z1 = b;
+}
+
+/* All voxels pierced by the ray inside the grid, in order (merge + midpoint). */
+static int ray_segments(const int64_t Oi[3], const int64_t Di[3], const int N[3], segv* out) {
+  out->n = 0;
+  rat zin = {0, 1}, zout = {0, 1};
+  int has_out = 0;
+  for (int a = 0; a < 3; ++a) {
+    int64_t top = (int64_t)N[a] << FIX;
+    if (Di[a] == 0) {
+      if (Oi[a] < 0 || Oi[a] >= top) return 0;
+      continue;
+    }
+    rat lo, hi;
+    if (Di[a] > 0) {
+      lo.n = -Oi[a]; lo.d = Di[a];
+      hi.n = top - Oi[a]; hi.d = Di[a];
+    } else {
+      lo.n = Oi[a] - top; lo.d = -Di[a];
+      hi.n = Oi[a]; hi.d = -Di[a];
+    }
+    if (rlt(zin, lo)) zin = lo;
+    if (!has_out || rlt(hi, zout)) {
+      zout = hi;
+      has_out = 1;
+    }
+  }
+  if (!has_out || !rlt(zin, zout)) return 0;
+  /* crossings strictly inside (zin, zout), per axis in increasing z */
+  rat* cr[3] = {NULL, NULL, NULL};
+  int nc[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a) {
+    if (Di[a] == 0) continue;
+    cr[a] = (rat*)malloc(sizeof(rat) * (size_t)(N[a] + 1));
+    for (int kk = 1; kk < N[a]; ++kk) {
+      int k = Di[a] > 0 ? kk : N[a] - kk;
+      rat z;
+      if (Di[a] > 0) {
+        z.n = ((int64_t)k << FIX) - Oi[a];
+        z.d = Di[a];
+      } else {
+        z.n = Oi[a] - ((int64_t)k << FIX);
+        z.d = -Di[a];
+      }
+      if (rlt(zin, z) && rlt(z, zout)) cr[a][nc[a]++] = z;
+    }
+  }
+  /* merge distinct breakpoints */
+  int i0 = 0, i1 = 0, i2 = 0;
+  rat prev = zin;
+  for (;;) {
+    rat best;
+    int found = 0;
+    if (i0 < nc[0]) { best = cr[0][i0]; found = 1; }
+    if (i1 < nc[1] && (!found || rlt(cr[1][i1], best))) { best = cr[1][i1]; found = 1; }
+    if (i2 < nc[2] && (!found || rlt(cr[2][i2], best))) { best = cr[2][i2]; found = 1; }
+    rat next = found ? best : zout;
+    /* voxel containing the midpoint of [prev, next] */
+    int32_t v[3];
+    i128 den = (i128)2 * prev.d * next.d;
+    i128 num = (i128)prev.n * next.d + (i128)next.n * prev.d; /* mid = num / den */
+    for (int a = 0; a < 3; ++a) {
+      i128 coord = (i128)Oi[a] * den + num * (i128)Di[a]; /* in 2^-FIX units times den */
+      v[a] = (int32_t)floordiv128(coord, den << FIX);
+    }
+    segv_push(out, v, prev, next);
+    if (!found) break;
+    if (i0 < nc[0] && req(cr[0][i0], next)) ++i0;
+    if (i1 < nc[1] && req(cr[1][i1], next)) ++i1;
+    if (i2 < nc[2] && req(cr[2][i2], next)) ++i2;
+    prev = next;
+  }
+  for (int a = 0; a < 3; ++a) free(cr[a]);
+  return out->n;
+}
+
+/* Truncated, activation-filtered walk.  slot == NULL: every brick active and
+ * visited bricks are flagged in mark[].  Emits records via the callbacks. */
+typedef struct {
+  uint32_t* rec;   /* 2 words per record (may be NULL for counting) */
+  double* len;     /* exact chord length per record */
+  double* zex;     /* exact exit depth per record */
+  int32_t* vxyz;   /* voxel coordinates (3 per record, gap = -1) */
+  int64_t n, cap;
+} recbuf;
+
+static void rb_push(recbuf* b, uint32_t vox, float lf, double len, double zex, const int32_t* v) {
+  if (b->n == b->cap) {
+    b->cap = b->cap ? 2 * b->cap : 1024;
+    b->rec = (uint32_t*)realloc(b->rec, sizeof(uint32_t) * 2 * (size_t)b->cap);
+    b->len = (double*)realloc(b->len, sizeof(double) * (size_t)b->cap);
+    b->zex = (double*)realloc(b->zex, sizeof(double) * (size_t)b->cap);
+    b->vxyz = (int32_t*)realloc(b->vxyz, sizeof(int32_t) * 3 * (size_t)b->cap);
+  }
+  uint32_t lb;
+  memcpy(&lb, &lf, 4);
+  b->rec[2 * b->n] = vox;
+  b->rec[2 * b->n + 1] = lb;
+  b->len[b->n] = len;
+  b->zex[b->n] = zex;
+  if (v) {
+    memcpy(&b->vxyz[3 * b->n], v, 12);
+  } else {
+    b->vxyz[3 * b->n] = b->vxyz[3 * b->n + 1] = b->vxyz[3 * b->n + 2] = -1;
+  }
+  b->n++;
+}
+
+typedef struct {
+  int bx, by, bz;
+  const int32_t* slot; /* NULL: all active, mark visited */
+  uint8_t* mark;
+} brickmap;
+
+static int brick_lin(const brickmap* bm, const int32_t v[3]) {
+  return ((v[0] >> 3) * bm->by + (v[1] >> 3)) * bm->bz + (v[2] >> 3);
+}
+
+/* returns number of records emitted; *z0f, *ran_out set */
+static int64_t walk_filtered(const segv* sv, const brickmap* bm, double norm, double depth, const double sig[3],
+                             const double bia[3], int lookup_n, recbuf* out, float* z0f, int* ran_out) {
+  int64_t start = out ? out->n : 0, cnt = 0;
+  int any = 0, skipped = 0;
+  double last_exit = 0.0;
+  *ran_out = 1;
+  *z0f = 0.f;
+  for (int j = 0; j < sv->n; ++j) {
+    const seg_t* e = &sv->s[j];
+    int lin = brick_lin(bm, e->v);
+    int active;
+    if (!bm->slot) {
+      bm->mark[lin] = 1;
+      active = 1;
+    } else {
+      active = bm->slot[lin] >= 0;
+    }
+    double zc = rdbl(e->z0), zn = rdbl(e->z1);
+    if (active && bm->slot) {
+      if (any && skipped) {
+        double gl = (zc - last_exit) * norm;
+        if (out) rb_push(out, GAP, (float)gl, gl, zc, NULL);
+        ++cnt;
+        skipped = 0;
+      }
+      if (!any) {
+        *z0f = (float)zc;
+        any = 1;
+      }
+      double l = (zn - zc) * norm;
+      uint32_t local = (uint32_t)(((e->v[0] & 7) * 8 + (e->v[1] & 7)) * 8 + (e->v[2] & 7));
+      uint32_t vox = (uint32_t)bm->slot[lin] * 512u + local;
+      if (out) rb_push(out, vox, (float)l, l, zn, e->v);
+      ++cnt;
+      last_exit = zn;
+    } else if (!active && any) {
+      skipped = 1;
+    }
+    int brick_exit = (j + 1 == sv->n) || brick_lin(bm, sv->s[j + 1].v) != lin;
+    if (brick_exit && active) {
+      double lhs = zn + poly(bia, lookup_n, zn);
+      double rhs = depth + 3.0 * poly(sig, lookup_n, zn);
+      if (lhs >= rhs) {
+        *ran_out = 0;
+        break;
+      }
+    }
+  }
+  (void)start;
+  return cnt;
+}
+
+/* ------------------------------------------------------------ context */
+
+typedef struct {
+  float pose[16], K[9];
+  double R[9], t[3], fx, fy, cx, cy;
+  int rows, cols, lookup_u;
+  float* depth;
+  /* traversal */
+  int64_t nrays;
+  uint32_t* rays; /* 6 words per ray */
+  recbuf rb;
+  /* pairs */
+  int64_t npairs;
+  uint32_t* pair_vox;
+  double* pair_c;
+  double* M;
+} ocam;
+
+struct orc_ctx {
+  orc_params p;
+  int N[3], Mx[3], B3[3], nbricks;
+  double res, off[3], prior_lo;
+  double sig[3], bia[3];
+  int32_t* slot;
+  int32_t* slot_lin;
+  int64_t nslots;
+  double *B, *A;
+  uint8_t* upd;
+  int64_t pool_cap;
+  ocam* cams;
+  int ncams, capcams;
+  int dirty;
+};
+
+int orc_create(const orc_params* p, orc_ctx** out) {
+  if (!p || !out || !(p->res > 0) || (p->lookup_n != 2 && p->lookup_n != 3)) return -1;
+  orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
+  c->p = *p;
+  c->res = (double)p->res;
+  for (int a = 0; a < 3; ++a) {
+    c->Mx[a] = (int)lround((double)p->dims[a] / (double)p->res);
+    if (c->Mx[a] <= 0 || c->Mx[a] > 2048) {
+      free(c);
+      return -7;
+    }
+    c->N[a] = (c->Mx[a] + 7) & ~7;
+    c->B3[a] = c->N[a] >> 3;
+    c->sig[a] = p->sigma_poly[a];
+    c->bia[a] = p->bias_poly[a];
+  }
+  c->off[0] = 0.5 * c->Mx[0];
+  c->off[1] = 0.5 * c->Mx[1];
+  c->off[2] = 0.0;
+  c->nbricks = c->B3[0] * c->B3[1] * c->B3[2];
+  c->slot = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->nbricks);
+  for (int i = 0; i < c->nbricks; ++i) c->slot[i] = -1;
+  c->slot_lin = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->nbricks);
+  /* logit of the prior in float, as the reference computes it (GVDBMRFMap.cpp:51) */
+  c->prior_lo = (double)(logf(p->prior) - logf(1.0f - p->prior));
+  c->dirty = 1;
+  *out = c;
+  return 0;
+}
+
+static void free_cam_lists(ocam* k) {
+  free(k->rays);
+  free(k->rb.rec);
+  free(k->rb.len);
+  free(k->rb.zex);
+  free(k->rb.vxyz);
+  memset(&k->rb, 0, sizeof(k->rb));
+  k->rays = NULL;
+  k->nrays = 0;
+  free(k->pair_vox);
+  free(k->pair_c);
+  k->pair_vox = NULL;
+  k->pair_c = NULL;
+  k->npairs = 0;
+}
+
+void orc_destroy(orc_ctx* c) {
+  if (!c) return;
+  for (int i = 0; i < c->ncams; ++i) {
+    free_cam_lists(&c->cams[i]);
+    free(c->cams[i].depth);
+    free(c->cams[i].M);
+  }
+  free(c->cams);
+  free(c->slot);
+  free(c->slot_lin);
+  free(c->B);
+  free(c->A);
+  free(c->upd);
+  free(c);
+}
+
+static void set_pose(ocam* k, const float pose[16]) {
+  memcpy(k->pose, pose, sizeof(k->pose));
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) k->R[i * 3 + j] = (double)pose[i * 4 + j];
+    k->t[i] = (double)pose[i * 4 + 3];
+  }
+}
+
+int orc_add_camera(orc_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols) {
+  if (!c || !pose || !K || !depth || rows <= 0 || cols <= 0) return -1;
+  if (c->ncams == c->capcams) {
+    c->capcams = c->capcams ? 2 * c->capcams : 16;
+    c->cams = (ocam*)realloc(c->cams, sizeof(ocam) * (size_t)c->capcams);
+  }
+  ocam* k = &c->cams[c->ncams++];
+  memset(k, 0, sizeof(*k));
+  set_pose(k, pose);
+  memcpy(k->K, K, sizeof(k->K));
+  k->fx = K[0];
+  k->fy = K[4];
+  k->cx = K[2];
+  k->cy = K[5];
+  k->rows = rows;
+  k->cols = cols;
+  k->lookup_u = (cols + LOOKUP_PX - 1) / LOOKUP_PX;
+  k->depth = (float*)malloc(sizeof(float) * (size_t)rows * cols);
+  memcpy(k->depth, depth, sizeof(float) * (size_t)rows * cols);
+  c->dirty = 1;
+  return c->ncams - 1;
+}
+
+int orc_set_pose(orc_ctx* c, int cam, const float pose[16]) {
+  if (!c || cam < 0 || cam >= c->ncams) return -5;
+  set_pose(&c->cams[cam], pose);
+  c->dirty = 1;
+  return 0;
+}
+
+static int pixel_ok(const orc_ctx* c, const ocam* k, int x, int y, float d) {
+  if (!(d > 0.0f)) return 0;
+  if (c->p.mask_848_border && k->cols == 848 && k->rows == 480 &&
+      (x < 24 || x >= k->cols - 24 || y < 20 || y >= k->rows - 20))
+    return 0;
+  return 1;
+}
+
+static int setup_ray(const orc_ctx* c, const ocam* k, int x, int y, int64_t Oi[3], int64_t Di[3], double* norm) {
+  double ux = ((double)x - k->cx) / k->fx;
+  double uy = ((double)y - k->cy) / k->fy;
+  double n2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    double dw = k->R[a * 3 + 0] * ux;
+    dw = dw + k->R[a * 3 + 1] * uy;
+    dw = dw + k->R[a * 3 + 2];
+    double D = dw / c->res;
+    double O = k->t[a] / c->res;
+    O = O + c->off[a];
+    double oi = rint(O * FIXS), di = rint(D * FIXS);
+    if (!(fabs(oi) < 536870912.0) || !(fabs(di) < 67108864.0)) return 0;
+    Oi[a] = (int64_t)oi;
+    Di[a] = (int64_t)di;
+    double dd = (double)Di[a] / FIXS;
+    n2 = n2 + dd * dd;
+  }
+  *norm = sqrt(n2);
+  return *norm > 0.0;
+}
+
+static void ensure_pool(orc_ctx* c) {
+  int64_t need = c->nslots * 512;
+  if (need <= c->pool_cap) return;
+  c->B = (double*)realloc(c->B, sizeof(double) * (size_t)need);
+  c->A = (double*)realloc(c->A, sizeof(double) * (size_t)need);
+  c->upd = (uint8_t*)realloc(c->upd, (size_t)need);
+  for (int64_t i = c->pool_cap; i < need; ++i) {
+    c->B[i] = c->prior_lo;
+    c->A[i] = 0;
+    c->upd[i] = 0;
+  }
+  c->pool_cap = need;
+}
+
+static int cmp_u32(const void* a, const void* b) {
+  uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+  return x < y ? -1 : x > y;
+}
+
+int orc_prepare(orc_ctx* c) {
+  if (!c) return -1;
+  if (!c->dirty) return 0;
+  const int ln = c->p.lookup_n;
+  uint8_t* mark = (uint8_t*)calloc((size_t)c->nbricks, 1);
+  brickmap bm = {c->B3[0], c->B3[1], c->B3[2], NULL, mark};
+  segv sv = {0};
+  /* activation */
+  for (int ci = 0; ci < c->ncams; ++ci) {
+    ocam* k = &c->cams[ci];
+    for (int y = 0; y < k->rows; ++y)
+      for (int x = 0; x < k->cols; ++x) {
+        float d = k->depth[(size_t)y * k->cols + x];
+        int64_t Oi[3], Di[3];
+        double norm;
+        if (!pixel_ok(c, k, x, y, d) || !setup_ray(c, k, x, y, Oi, Di, &norm)) continue;
+        if (c->p.activation == 1) {
+          ray_segments(Oi, Di, c->N, &sv);
+          float z0;
+          int ro;
+          walk_filtered(&sv, &bm, norm, (double)d, c->sig, c->bia, ln, NULL, &z0, &ro);
+        } else {
+          int lo[3], hi[3], ok = 1;
+          for (int a = 0; a < 3 && ok; ++a) {
+            double P = (double)d * (double)Di[a];
+            P = (double)Oi[a] + P;
+            P = P / FIXS;
+            if (!(P >= 0.0 && P < (double)c->Mx[a])) {
+              ok = 0;
+              break;
+            }
+            int l = (int)floor(P - 1.0), h = (int)floor(P + 1.0);
+            if (l < 0) l = 0;
+            if (h > c->N[a] - 1) h = c->N[a] - 1;
+            lo[a] = l >> 3;
+            hi[a] = h >> 3;
+          }
+          if (!ok) continue;
+          for (int i = lo[0]; i <= hi[0]; ++i)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+              for (int kk = lo[2]; kk <= hi[2]; ++kk) mark[(i * c->B3[1] + j) * c->B3[2] + kk] = 1;
+        }
+      }
+  }
+  for (int lin = 0; lin < c->nbricks; ++lin)
+    if (mark[lin] && c->slot[lin] < 0) {
+      c->slot[lin] = (int32_t)c->nslots;
+      c->slot_lin[c->nslots++] = lin;
+    }
+  free(mark);
+  ensure_pool(c);
+  /* traversal */
+  bm.slot = c->slot;
+  bm.mark = NULL;
+  double* csum = (double*)calloc((size_t)c->nslots * 512, sizeof(double));
+  uint32_t* touched = NULL;
+  int64_t ntouched = 0, captouched = 0;
+  for (int ci = 0; ci < c->ncams; ++ci) {
+    ocam* k = &c->cams[ci];
+    free_cam_lists(k);
+    int64_t cap = 1024;
+    k->rays = (uint32_t*)malloc(sizeof(uint32_t) * 6 * (size_t)cap);
+    for (int y = 0; y < k->rows; ++y)
+      for (int x = 0; x < k->cols; ++x) {
+        float d = k->depth[(size_t)y * k->cols + x];
+        int64_t Oi[3], Di[3];
+        double norm;
+        if (!pixel_ok(c, k, x, y, d) || !setup_ray(c, k, x, y, Oi, Di, &norm)) continue;
+        ray_segments(Oi, Di, c->N, &sv);
+        int64_t off = k->rb.n;
+        float z0;
+        int ro;
+        int64_t n = walk_filtered(&sv, &bm, norm, (double)d, c->sig, c->bia, ln, &k->rb, &z0, &ro);
+        if (n == 0) continue;
+        if (k->nrays == cap) {
+          cap *= 2;
+          k->rays = (uint32_t*)realloc(k->rays, sizeof(uint32_t) * 6 * (size_t)cap);
+        }
+        uint32_t* r = &k->rays[6 * k->nrays++];
+        int v = k->rows - 1 - y;
+        uint32_t cell = (uint32_t)((v / LOOKUP_PX) * (k->lookup_u * ln) + (x / LOOKUP_PX) * ln);
+        float inv = (float)(1.0 / norm);
+        r[0] = (uint32_t)off;
+        r[1] = (uint32_t)n;
+        memcpy(&r[2], &z0, 4);
+        memcpy(&r[3], &inv, 4);
+        memcpy(&r[4], &d, 4);
+        r[5] = cell | (ro ? RAN_OUT : 0u);
+      }
+    /* pairs: summed chord length per touched voxel */
+    ntouched = 0;
+    for (int64_t i = 0; i < k->rb.n; ++i) {
+      uint32_t vox = k->rb.rec[2 * i];
+      if (vox == GAP) continue;
+      if (csum[vox] == 0.0) {
+        if (ntouched == captouched) {
+          captouched = captouched ? 2 * captouched : 4096;
+          touched = (uint32_t*)realloc(touched, sizeof(uint32_t) * (size_t)captouched);
+        }
+        touched[ntouched++] = vox;
+      }
+      csum[vox] += k->rb.len[i];
+    }
+    qsort(touched, (size_t)ntouched, sizeof(uint32_t), cmp_u32);
+    k->npairs = ntouched;
+    k->pair_vox = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(ntouched ? ntouched : 1));
+    k->pair_c = (double*)malloc(sizeof(double) * (size_t)(ntouched ? ntouched : 1));
+    for (int64_t i = 0; i < ntouched; ++i) {
+      k->pair_vox[i] = touched[i];
+      k->pair_c[i] = csum[touched[i]];
+      csum[touched[i]] = 0.0;
+    }
+    free(k->M);
+    k->M = (double*)calloc((size_t)c->nslots * 512, sizeof(double));
+  }
+  free(touched);
+  free(csum);
+  free(sv.s);
+  c->dirty = 0;
+  return 0;
+}
+
+int64_t orc_num_bricks(orc_ctx* c) {
+  if (!c || orc_prepare(c)) return -1;
+  return c->nslots;
+}
+
+int orc_brick_coords(orc_ctx* c, int32_t* xyz) {
+  if (!c || orc_prepare(c)) return -1;
+  for (int64_t s = 0; s < c->nslots; ++s) {
+    int lin = c->slot_lin[s];
+    xyz[3 * s + 2] = lin % c->B3[2];
+    xyz[3 * s + 1] = (lin / c->B3[2]) % c->B3[1];
+    xyz[3 * s + 0] = lin / (c->B3[2] * c->B3[1]);
+  }
+  return 0;
+}
+
+int orc_get_traversal(orc_ctx* c, int cam, int64_t* n_rays, int64_t* n_records, uint32_t* rays,
+                      uint32_t* records) {
+  if (!c || cam < 0 || cam >= c->ncams) return -5;
+  if (orc_prepare(c)) return -1;
+  ocam* k = &c->cams[cam];
+  if (n_rays) *n_rays = k->nrays;
+  if (n_records) *n_records = k->rb.n;
+  if (rays) memcpy(rays, k->rays, sizeof(uint32_t) * 6 * (size_t)k->nrays);
+  if (records) memcpy(records, k->rb.rec, sizeof(uint32_t) * 2 * (size_t)k->rb.n);
+  return 0;
+}
+
+/* ------------------------------------------------------------ messages */
+
+static double* scratch_buf(int n) {
+  static __thread double* buf = NULL;
+  static __thread int cap = 0;
+  if (n > cap) {
+    cap = n * 2;
+    buf = (double*)realloc(buf, sizeof(double) * 3 * (size_t)cap);
+  }
+  return buf;
+}
+
+double orc_ray_messages(int n, const double* nin, const double* len, const double* s, const int* live, double d,
+                        const double sig[3], const double bia[3], int lookup_n, int ran_out, double psi_inf,
+                        int mode, double* p) {
+  if (n <= 0) return 0.0;
+  double* buf = scratch_buf(n);
+  double *tau = buf, *vpsi = buf + n, *R = buf + 2 * n;
+  double L = 0.0, T = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!live[i]) {
+      tau[i] = vpsi[i] = 0.0;
+      continue;
+    }
+    double V;
+    if (mode == 0) { /* visibility includes voxel i itself, weighted by length (gvdbkernels.cu:781-783) */
+      L += len[i] * log(nin[i]);
+      V = exp(L);
+    } else { /* textbook: product over j < i, unit weights */
+      V = exp(L);
+      L += log(nin[i]);
+    }
+    double ps = orc_depth_psi(s[i], d, sig, bia, lookup_n);
+    vpsi[i] = V * ps;
+    tau[i] = vpsi[i] * (1.0 - nin[i]);
+    T += tau[i];
+  }
+  double tinf = 0.0;
+  if (mode == 0) {
+    /* gvdbkernels.cu:1196-1208: only when the walk ran out, and only if d lies beyond */
+    if (ran_out) {
+      double sn = s[n - 1];
+      if (d > sn + poly(bia, lookup_n, sn)) tinf = exp(L) * psi_inf;
+    }
+  } else {
+    tinf = exp(L) * psi_inf;
+  }
+  T += tinf;
+  /* suffix sums R_i = sum_{j>i} tau_j + tinf */
+  double acc = tinf;
+  for (int i = n - 1; i >= 0; --i) {
+    R[i] = acc;
+    acc += tau[i];
+  }
+  double P = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!live[i]) {
+      p[i] = 0.0;
+      continue;
+    }
+    double mu1 = P + vpsi[i];
+    double mu0 = P + R[i] / nin[i];
+    double q;
+    if (mode == 0) {
+      q = mu1 / (mu1 + mu0 + 1e-7);
+      q = q < F_EPS ? F_EPS : (q > 1.0 - F_EPS ? 1.0 - F_EPS : q);
+    } else {
+      q = mu1 / (mu1 + mu0);
+    }
+    p[i] = q;
+    P += tau[i];
+  }
+  return T;
+}
+
+int orc_ray_messages_bruteforce(int n, const double* nin, const double* psi, double psi_inf, double* p) {
+  if (n <= 0 || n > 20) return -1;
+  double* m1 = (double*)calloc((size_t)n, sizeof(double));
+  double* m0 = (double*)calloc((size_t)n, sizeof(double));
+  for (uint32_t o = 0; o < (1u << n); ++o) {
+    int first = -1;
+    for (int k = 0; k < n; ++k)
+      if (o >> k & 1u) {
+        first = k;
+        break;
+      }
+    double f = first < 0 ? psi_inf : psi[first];
+    for (int i = 0; i < n; ++i) {
+      double w = f;
+      for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        w *= (o >> j & 1u) ? (1.0 - nin[j]) : nin[j];
+      }
+      if (o >> i & 1u)
+        m1[i] += w;
+      else
+        m0[i] += w;
+    }
+  }
+  for (int i = 0; i < n; ++i) p[i] = m1[i] / (m1[i] + m0[i]);
+  free(m1);
+  free(m0);
+  return 0;
+}
+
+/* ------------------------------------------------------------ BP schedule */
+
+static double marginal(double b) {
+  double neg = 1.0 / (1.0 + exp(b));
+  neg = neg < F_EPS ? F_EPS : (neg > 1.0 - F_EPS ? 1.0 - F_EPS : neg);
+  return 1.0 - neg;
+}
+
+static double clampd(double x) { return x < F_EPS ? F_EPS : (x > 1.0 - F_EPS ? 1.0 - F_EPS : x); }
+
+int orc_perform_inference(orc_ctx* c, const int* ids, int n_ids, int max_iters, double tol, double* residual,
+                          int64_t* updates_per_iter) {
+  if (!c) return -1;
+  if (orc_prepare(c)) return -1;
+  const int ln = c->p.lookup_n;
+  const int64_t nv = c->nslots * 512;
+  if (ids) {
+    for (int i = 0; i < n_ids; ++i) {
+      if (ids[i] < 0 || ids[i] >= c->ncams) return -5;
+      memset(c->cams[ids[i]].M, 0, sizeof(double) * (size_t)nv);
+    }
+  } else {
+    for (int i = 0; i < c->ncams; ++i) memset(c->cams[i].M, 0, sizeof(double) * (size_t)nv);
+  }
+  int64_t upd = 0;
+  for (int ci = 0; ci < c->ncams; ++ci)
+    for (int64_t i = 0; i < c->cams[ci].rb.n; ++i) upd += c->cams[ci].rb.rec[2 * i] != GAP;
+  if (updates_per_iter) *updates_per_iter = upd;
+  for (int64_t v = 0; v < nv; ++v) c->A[v] = marginal(c->B[v]);
+  double* S = (double*)calloc((size_t)nv, sizeof(double));
+  int maxn = 1;
+  for (int ci = 0; ci < c->ncams; ++ci)
+    for (int64_t r = 0; r < c->cams[ci].nrays; ++r)
+      if ((int)c->cams[ci].rays[6 * r + 1] > maxn) maxn = (int)c->cams[ci].rays[6 * r + 1];
+  double* nin = (double*)malloc(sizeof(double) * (size_t)maxn);
+  double* pout = (double*)malloc(sizeof(double) * (size_t)maxn);
+  int* live = (int*)malloc(sizeof(int) * (size_t)maxn);
+  double res = 0.0;
+  int it;
+  for (it = 0; it < max_iters; ++it) {
+    for (int ci = 0; ci < c->ncams; ++ci) {
+      ocam* k = &c->cams[ci];
+      for (int64_t r = 0; r < k->nrays; ++r) {
+        const uint32_t* rr = &k->rays[6 * r];
+        const int64_t off = rr[0];
+        const int n = (int)rr[1];
+        float d;
+        memcpy(&d, &rr[4], 4);
+        const int ro = (rr[5] & RAN_OUT) != 0;
+        for (int i = 0; i < n; ++i) {
+          uint32_t vox = k->rb.rec[2 * (off + i)];
+          live[i] = vox != GAP;
+          nin[i] = live[i] ? clampd(1.0 / (1.0 + exp(c->B[vox] - k->M[vox]))) : 1.0;
+        }
+        orc_ray_messages(n, nin, k->rb.len + off, k->rb.zex + off, live, (double)d, c->sig, c->bia, ln, ro, 1.0,
+                         0, pout);
+        for (int i = 0; i < n; ++i)
+          if (live[i]) S[k->rb.rec[2 * (off + i)]] += pout[i] * k->rb.len[off + i];
+      }
+      for (int64_t q = 0; q < k->npairs; ++q) {
+        const uint32_t v = k->pair_vox[q];
+        const double C = k->pair_c[q];
+        const double s = S[v];
+        S[v] = 0.0;
+        if (C > 0.1) {
+          double a = clampd(s / C);
+          double m = log(a) - log(1.0 - a);
+          c->B[v] = (c->B[v] + m) - k->M[v];
+          k->M[v] = m;
+          c->upd[v] = 1;
+        }
+      }
+    }
+    res = 0.0;
+    for (int64_t v = 0; v < nv; ++v) {
+      double a = marginal(c->B[v]);
+      double dl = fabs(a - c->A[v]);
+      if (dl > res) res = dl;
+      c->A[v] = a;
+    }
+    if (tol > 0 && res < tol) {
+      ++it;
+      break;
+    }
+  }
+  free(S);
+  free(nin);
+  free(pout);
+  free(live);
+  if (residual) *residual = res;
+  return it;
+}
+
+int orc_get_alphas(orc_ctx* c, double* out) {
+  if (!c || orc_prepare(c)) return -1;
+  for (int64_t v = 0; v < c->nslots * 512; ++v) out[v] = c->upd[v] ? marginal(c->B[v]) : 0.0;
+  return 0;
+}
+
+int orc_get_beliefs(orc_ctx* c, double* out) {
+  if (!c || orc_prepare(c)) return -1;
+  memcpy(out, c->B, sizeof(double) * (size_t)(c->nslots * 512));
+  return 0;
+}
+
+int orc_walk_ray(const double O[3], const double D[3], const int N[3], double depth, int trunc, const double sig[3],
+                 const double bia[3], int lookup_n, int cap, int32_t* vox, double* len, double* zexit,
+                 int* ran_out) {
+  int64_t Oi[3], Di[3];
+  double n2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    Oi[a] = (int64_t)rint(O[a] * FIXS);
+    Di[a] = (int64_t)rint(D[a] * FIXS);
+    double dd = (double)Di[a] / FIXS;
+    n2 = n2 + dd * dd;
+  }
+  double norm = sqrt(n2);
+  segv sv = {0};
+  ray_segments(Oi, Di, N, &sv);
+  int m = 0, ro = 1;
+  int B3[3] = {(N[0] + 7) >> 3, (N[1] + 7) >> 3, (N[2] + 7) >> 3};
+  for (int j = 0; j < sv.n && m < cap; ++j) {
+    const seg_t* e = &sv.s[j];
+    double zc = rdbl(e->z0), zn = rdbl(e->z1);
+    memcpy(&vox[3 * m], e->v, 12);
+    len[m] = (zn - zc) * norm;
+    zexit[m] = zn;
+    ++m;
+    if (trunc) {
+      int lin = ((e->v[0] >> 3) * B3[1] + (e->v[1] >> 3)) * B3[2] + (e->v[2] >> 3);
+      int brick_exit = (j + 1 == sv.n);
+      if (!brick_exit) {
+        const int32_t* w = sv.s[j + 1].v;
+        brick_exit = (((w[0] >> 3) * B3[1] + (w[1] >> 3)) * B3[2] + (w[2] >> 3)) != lin;
+      }
+      if (brick_exit && zn + poly(bia, lookup_n, zn) >= depth + 3.0 * poly(sig, lookup_n, zn)) {
+        ro = 0;
+        break;
+      }
+    }
+  }
+  free(sv.s);
+  if (ran_out) *ran_out = ro;
+  return m;
+}

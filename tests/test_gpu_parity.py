@@ -1,0 +1,107 @@
+"""GPU parity: libmrfmap_hip (through its C ABI) against the CPU oracle.
+
+Traversal lists must match bit for bit; marginals must agree with the FP64
+BP within 1e-3 max-abs (the tolerance of the mixed-precision target; FP32
+messages land far inside it).
+"""
+import numpy as np
+import pytest
+
+from mrfmap_amd import _lib, synth
+
+from helpers import decimate, gpu_for, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+MARG_TOL = 1e-3
+
+
+def _compare_traversal(g, o, ncam):
+    tot = 0
+    for c in range(ncam):
+        gr, gi = g.get_traversal(c)
+        orr, oi = o.traversal(c)
+        assert gr.shape == orr.shape, f"cam {c}: ray count {gr.shape} vs {orr.shape}"
+        assert gi.shape == oi.shape, f"cam {c}: record count {gi.shape} vs {oi.shape}"
+        bad = np.nonzero((gr != orr).any(axis=1))[0]
+        assert len(bad) == 0, f"cam {c}: {len(bad)} ray records differ, first {bad[:3]}: {gr[bad[:1]]} vs {orr[bad[:1]]}"
+        badi = np.nonzero((gi != oi).any(axis=1))[0]
+        assert len(badi) == 0, f"cam {c}: {len(badi)} incidence records differ, first {badi[:3]}"
+        tot += len(gi)
+    return tot
+
+
+@pytest.mark.parametrize("activation", [_lib.ACT_FULL_RAY, _lib.ACT_ENDPOINT])
+def test_traversal_bit_exact_small(activation):
+    p, frames, _ = synth.make_workload("small", activation=activation)
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    np.testing.assert_array_equal(g.get_occupied_bricks(), (o.brick_coords() * 8 + 4).astype(np.float32))
+    n = _compare_traversal(g, o, len(frames))
+    assert n > 1_000_000
+
+
+def test_traversal_bit_exact_real_shape():
+    """640x480, f=525, 1 cm voxels: two keyframes, every 3rd pixel."""
+    p, frames, _ = synth.make_workload("real", frames=13)
+    frames = [(pose, K, decimate(d, 3)) for (pose, K, d) in frames[:13:6]]
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    n = _compare_traversal(g, o, len(frames))
+    assert n > 5_000_000
+
+
+@pytest.mark.parametrize("activation", [_lib.ACT_FULL_RAY, _lib.ACT_ENDPOINT])
+def test_bp_marginals_small(activation):
+    p, frames, _ = synth.make_workload("small", activation=activation)
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    st = g.perform_inference(max_iters=5)
+    so = o.perform_inference(5)
+    assert st["iters"] == 5 and so["iters"] == 5
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    ga, oa = g.get_alphas(), o.alphas()
+    assert ga.shape == oa.shape
+    err = np.abs(ga.astype(np.float64) - oa)
+    print(f"max-abs marginal error {err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e}")
+    assert err.max() <= MARG_TOL
+    # voxels never updated read 0 on both sides
+    np.testing.assert_array_equal(ga == 0, oa == 0)
+    assert abs(st["residual"] - so["residual"]) < 1e-3
+
+
+def test_bp_repeated_calls_keep_beliefs():
+    """Beliefs persist across perform_inference calls, messages are reset
+    (GVDBInference.cpp:128-134)."""
+    p, frames, _ = synth.make_workload("small")
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    for _ in range(2):
+        g.perform_inference(max_iters=2)
+        o.perform_inference(2)
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas()).max()
+    assert err <= MARG_TOL
+
+
+def test_get_alphas_shape_error():
+    p, frames, _ = synth.make_workload("small", frames=1)
+    g = gpu_for(p, frames)
+    g.perform_inference(max_iters=1)
+    n = g.num_bricks()
+    with pytest.raises(_lib.MrfError) as e:
+        g.get_indices_and_alphas(np.zeros((n + 1, 512), dtype=np.float32))
+    assert e.value.code == -4
+
+
+def test_bad_camera_id():
+    p, frames, _ = synth.make_workload("small", frames=1)
+    g = gpu_for(p, frames)
+    with pytest.raises(_lib.MrfError) as e:
+        g.set_pose(5, np.eye(4))
+    assert e.value.code == -5
