@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("MRF_OFFLOAD_ARCH", "gfx950")
-HIP_SOURCES = ["api.hip", "traverse.hip", "bp.hip", "scan.hip"]
+HIP_SOURCES = ["api.hip", "traverse.hip", "bp.hip", "scan.hip", "geometry.hip"]
 # every double operation of the traversal must be one IEEE operation
 NO_CONTRACT = {"traverse.hip"}
 HIP_FLAGS = [

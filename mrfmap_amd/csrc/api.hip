@@ -39,8 +39,11 @@ struct Camera {
   uint2* d_inc = nullptr;
   uint64_t nrec = 0, nupd = 0;
   uint32_t* d_pair_vox = nullptr;
+  uint32_t* d_pair_off = nullptr;
   float* d_pair_c = nullptr;
   uint32_t npairs = 0;
+  uint64_t* d_heads = nullptr;   // head bitmask over sorted positions
+  uint32_t* d_tile_h0 = nullptr; // heads before each kTile tile
 };
 
 size_t msg_size(int t) { return t == MRF_MSG_F32 ? 4 : 2; }
@@ -66,13 +69,25 @@ struct mrf_ctx {
 
   float* d_B = nullptr;
   float* d_A = nullptr;   // marginal at the end of the previous iteration
-  float* d_S = nullptr;   // ray->voxel accumulator of the camera being processed
   uint8_t* d_upd = nullptr;
-  unsigned long long* d_cfix = nullptr;
   float* d_out = nullptr; // readback staging
   void* d_M = nullptr;
   size_t M_bytes = 0;
   size_t M_stride = 0;    // elements per camera
+  void* d_X = nullptr;    // segment array of the camera being processed
+  size_t X_bytes = 0;
+  // per-pair / per-tile scratch of a camera pass
+  float *d_apair = nullptr, *d_Spair = nullptr;
+  size_t cap_pairs = 0;
+  uint32_t *d_pf_q = nullptr, *d_pl_q = nullptr;
+  float *d_pf_v = nullptr, *d_pl_v = nullptr, *d_mid = nullptr;
+  size_t cap_tiles = 0;
+  // segment-build scratch
+  uint32_t *d_keys_in = nullptr, *d_keys_out = nullptr, *d_vals_in = nullptr, *d_vals_out = nullptr;
+  uint32_t *d_heads = nullptr, *d_hscan = nullptr;
+  size_t cap_sort = 0;
+  void* d_sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
 
   // scratch
   uint32_t *d_counts = nullptr, *d_has = nullptr, *d_offs = nullptr, *d_rayidx = nullptr;
@@ -80,8 +95,6 @@ struct mrf_ctx {
   uint64_t* d_scan_tmp = nullptr;
   size_t cap_scan = 0;
   uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
-  uint32_t *d_bcnt = nullptr, *d_boff = nullptr;
-  size_t cap_blocks = 0;
 
   std::vector<Camera> cams;
   bool geom_dirty = true;
@@ -212,42 +225,61 @@ int ensure_pool(mrf_ctx* c, size_t nslots) {
   if (nslots <= c->cap_slots) return MRF_OK;
   size_t cap = std::max(nslots, c->cap_slots + c->cap_slots / 2);
   const size_t nv = cap * kBrickVox, old = c->cap_slots * kBrickVox;
-  float *B = nullptr, *A = nullptr, *S = nullptr, *O = nullptr;
+  float *B = nullptr, *A = nullptr, *O = nullptr;
   uint8_t* U = nullptr;
-  unsigned long long* F = nullptr;
   int rc;
-  if ((rc = dalloc(c, &B, nv)) || (rc = dalloc(c, &A, nv)) || (rc = dalloc(c, &S, nv)) ||
-      (rc = dalloc(c, &O, nv)) || (rc = dalloc(c, &U, nv)) || (rc = dalloc(c, &F, nv))) {
-    dfree(&B); dfree(&A); dfree(&S); dfree(&O); dfree(&U); dfree(&F);
+  if ((rc = dalloc(c, &B, nv)) || (rc = dalloc(c, &A, nv)) || (rc = dalloc(c, &O, nv)) || (rc = dalloc(c, &U, nv))) {
+    dfree(&B); dfree(&A); dfree(&O); dfree(&U);
     return rc;
   }
   launch_fill_f32(B, c->prior_logodds, nv, c->stream);
-  launch_zero_words(S, nv * 4, c->stream);
-  launch_zero_words(F, nv * 8, c->stream);
   launch_fill_u8(U, 0, nv, c->stream);
   if (old) {
     HIPCHK(c, hipMemcpyAsync(B, c->d_B, old * 4, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(U, c->d_upd, old, hipMemcpyDeviceToDevice, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_S); dfree(&c->d_out); dfree(&c->d_upd); dfree(&c->d_cfix);
-  c->d_B = B; c->d_A = A; c->d_S = S; c->d_out = O; c->d_upd = U; c->d_cfix = F;
+  dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_out); dfree(&c->d_upd);
+  c->d_B = B; c->d_A = A; c->d_out = O; c->d_upd = U;
   c->cap_slots = cap;
-  const size_t nb = pair_blocks(nv);
-  if (nb > c->cap_blocks) {
-    dfree(&c->d_bcnt);
-    dfree(&c->d_boff);
-    if ((rc = dalloc(c, &c->d_bcnt, nb)) || (rc = dalloc(c, &c->d_boff, nb))) return rc;
-    c->cap_blocks = nb;
+  return MRF_OK;
+}
+
+int ensure_sort(mrf_ctx* c, size_t n, int bits) {
+  if (n > c->cap_sort) {
+    dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
+    dfree(&c->d_heads); dfree(&c->d_hscan);
+    c->cap_sort = 0;
+    int rc;
+    if ((rc = dalloc(c, &c->d_keys_in, n)) || (rc = dalloc(c, &c->d_keys_out, n)) || (rc = dalloc(c, &c->d_vals_in, n)) ||
+        (rc = dalloc(c, &c->d_vals_out, n)) || (rc = dalloc(c, &c->d_heads, n)) || (rc = dalloc(c, &c->d_hscan, n)))
+      return rc;
+    c->cap_sort = n;
   }
-  return ensure_scan(c, nb);
+  const size_t tb = segment_sort_temp_bytes((uint32_t)n, bits);
+  if (tb > c->sort_tmp_bytes) {
+    if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
+  dfree(&c->d_apair); dfree(&c->d_Spair);
+  dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
+    c->d_sort_tmp = nullptr;
+    c->sort_tmp_bytes = 0;
+    if (hipMalloc(&c->d_sort_tmp, tb) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, MRF_E_NOMEM, "sort scratch allocation failed");
+    }
+    c->sort_tmp_bytes = tb;
+  }
+  return ensure_scan(c, n);
 }
 
 void free_camera_lists(Camera& cam) {
   dfree(&cam.d_rays);
   dfree(&cam.d_inc);
   dfree(&cam.d_pair_vox);
+  dfree(&cam.d_pair_off);
   dfree(&cam.d_pair_c);
+  dfree(&cam.d_heads);
+  dfree(&cam.d_tile_h0);
   cam.nrays = 0;
   cam.nrec = cam.nupd = 0;
   cam.npairs = 0;
@@ -267,10 +299,19 @@ __global__ void k_count_live(const uint2* __restrict__ inc, uint64_t n, unsigned
   if (threadIdx.x == 0) atomicAdd(out, sh[0]);
 }
 
+int bit_length(uint64_t v) {
+  int b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
 int traverse_camera(mrf_ctx* c, Camera& cam) {
   free_camera_lists(cam);
   const CamGeom cg = make_geom(c, cam);
-  const size_t npix = (size_t)cam.rows * cam.cols;
+  const size_t npix = (size_t)num_tiled_pixels(cg);
   int rc;
   if ((rc = ensure_pixel_scratch(c, npix)) || (rc = ensure_scan(c, npix))) return rc;
   launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, c->stream);
@@ -279,13 +320,13 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   launch_exclusive_scan(c->d_has, c->d_rayidx, npix, c->d_scan_tmp, c->d_total, c->stream);
   const uint64_t nrays = read_total(c);
   HIPCHK(c, hipGetLastError());
-  if (nrec >= 0xFFFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^32 incidence records in one camera");
+  if (nrec >= 0x7FFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^31 incidence records in one camera");
   cam.nrec = nrec;
   cam.nrays = (uint32_t)nrays;
   if ((rc = dalloc(c, &cam.d_inc, nrec)) || (rc = dalloc(c, &cam.d_rays, nrays))) return rc;
   launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_rayidx, cam.d_inc, cam.d_rays,
               c->stream);
-  // live incidences
+  // live incidences (records that are not gap records)
   HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 8, c->stream));
   if (nrec) {
     unsigned g = (unsigned)std::min<uint64_t>((nrec + 255) / 256, 2048);
@@ -296,15 +337,40 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   HIPCHK(c, hipMemcpyAsync(&live, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   cam.nupd = live;
-  // (voxel, C) pairs
-  const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
-  launch_cum_len(cam.d_rays, cam.nrays, cam.d_inc, c->d_cfix, c->stream);
-  launch_build_pairs(c->d_cfix, nvox, c->d_bcnt, c->d_boff, c->d_scan_tmp, c->d_total, nullptr, nullptr, c->stream);
-  const uint64_t np = read_total(c);
+  // (camera, voxel) segments
+  const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
+  const int bits = bit_length(nvox);
+  if ((rc = ensure_sort(c, std::max<uint64_t>(nrec, 1), bits))) return rc;
+  SegmentBuild b;
+  b.rec = cam.d_inc;
+  b.nrec = (uint32_t)nrec;
+  b.gap_key = (uint32_t)nvox;
+  b.key_bits = bits;
+  b.keys_in = c->d_keys_in;
+  b.keys_out = c->d_keys_out;
+  b.vals_in = c->d_vals_in;
+  b.vals_out = c->d_vals_out;
+  b.heads = c->d_heads;
+  b.hscan = c->d_hscan;
+  b.temp = c->d_sort_tmp;
+  b.temp_bytes = c->sort_tmp_bytes;
+  b.scan_tmp = c->d_scan_tmp;
+  b.d_total = c->d_total;
+  HIPCHK(c, build_segments(b, c->stream));
+  const uint64_t np = nrec ? read_total(c) : 0;
   cam.npairs = (uint32_t)np;
-  if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np))) return rc;
-  launch_build_pairs(c->d_cfix, nvox, c->d_bcnt, c->d_boff, c->d_scan_tmp, c->d_total, cam.d_pair_vox,
-                     cam.d_pair_c, c->stream);
+  if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_off, np + 1)) ||
+      (rc = dalloc(c, &cam.d_pair_c, np)))
+    return rc;
+  const size_t nwords = (live + 63) / 64 + 1, ntiles = (live + kTile - 1) / kTile + 1;
+  if ((rc = dalloc(c, &cam.d_heads, nwords)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles))) return rc;
+  if (nrec) {
+    launch_pack_heads(b, (uint32_t)live, cam.d_heads, cam.d_tile_h0, c->stream);
+    finish_segments(b, cam.npairs, (uint32_t)live, cam.d_pair_off, cam.d_pair_vox, cam.d_pair_c, cam.d_heads,
+                    cam.d_tile_h0, c->stream);
+  } else {
+    HIPCHK(c, hipMemsetAsync(cam.d_pair_off, 0, 4, c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipGetLastError());
   return MRF_OK;
@@ -343,7 +409,7 @@ int prepare(mrf_ctx* c) {
     if ((rc = traverse_camera(c, cam))) return rc;
   // 4. message storage: one dense pool-sized array per camera
   const size_t stride = c->slot_lin.size() * (size_t)kBrickVox;
-  const size_t need = std::max<size_t>(1, stride * c->cams.size() * msg_size(c->p.msg_dtype));
+  const size_t need = std::max<size_t>(1, stride * c->cams.size() * sizeof(float));
   if (need > c->M_bytes) {
     if (c->d_M) (void)hipFree(c->d_M);
     c->d_M = nullptr;
@@ -355,6 +421,38 @@ int prepare(mrf_ctx* c) {
     c->M_bytes = need;
   }
   c->M_stride = stride;
+  size_t xneed = 1, pneed = 1, tneed = 1;
+  for (const Camera& cam : c->cams) {
+    xneed = std::max<size_t>(xneed, cam.nupd);
+    pneed = std::max<size_t>(pneed, cam.npairs);
+    tneed = std::max<size_t>(tneed, (cam.nupd + kTile - 1) / kTile);
+  }
+  if (pneed > c->cap_pairs) {
+    dfree(&c->d_apair);
+    dfree(&c->d_Spair);
+    c->cap_pairs = 0;
+    if ((rc = dalloc(c, &c->d_apair, pneed)) || (rc = dalloc(c, &c->d_Spair, pneed))) return rc;
+    c->cap_pairs = pneed;
+  }
+  if (tneed > c->cap_tiles) {
+    dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
+    c->cap_tiles = 0;
+    if ((rc = dalloc(c, &c->d_pf_q, tneed)) || (rc = dalloc(c, &c->d_pl_q, tneed)) ||
+        (rc = dalloc(c, &c->d_pf_v, tneed)) || (rc = dalloc(c, &c->d_pl_v, tneed)) || (rc = dalloc(c, &c->d_mid, tneed)))
+      return rc;
+    c->cap_tiles = tneed;
+  }
+  xneed *= msg_size(c->p.msg_dtype);
+  if (xneed > c->X_bytes) {
+    if (c->d_X) (void)hipFree(c->d_X);
+    c->d_X = nullptr;
+    c->X_bytes = 0;
+    if (hipMalloc(&c->d_X, xneed) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, MRF_E_NOMEM, "segment array allocation failed");
+    }
+    c->X_bytes = xneed;
+  }
   c->geom_dirty = false;
   return MRF_OK;
 }
@@ -493,10 +591,16 @@ void mrf_destroy(mrf_ctx* c) {
     dfree(&cam.d_depth);
   }
   dfree(&c->d_sig); dfree(&c->d_bia); dfree(&c->d_slot); dfree(&c->d_bflags);
-  dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_S); dfree(&c->d_upd); dfree(&c->d_cfix); dfree(&c->d_out);
+  dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
   if (c->d_M) (void)hipFree(c->d_M);
+  if (c->d_X) (void)hipFree(c->d_X);
+  if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
+  dfree(&c->d_apair); dfree(&c->d_Spair);
+  dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
+  dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
+  dfree(&c->d_heads); dfree(&c->d_hscan);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
-  dfree(&c->d_scan_tmp); dfree(&c->d_total); dfree(&c->d_bcnt); dfree(&c->d_boff);
+  dfree(&c->d_scan_tmp); dfree(&c->d_total);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -561,7 +665,7 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   auto t1 = std::chrono::steady_clock::now();
   if (dirty) s.t_geometry_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
-  const size_t esz = msg_size(c->p.msg_dtype);
+  const size_t esz = sizeof(float);  // camera messages M_c are FP32 for every message type
   // reset M_c of the selected cameras (GVDBInference.cpp:128-134)
   for (uint32_t id : sel) launch_zero_words((char*)c->d_M + (size_t)id * c->M_stride * esz, c->M_stride * esz, c->stream);
   launch_marginals(c->d_B, c->d_A, nvox, c->stream);
@@ -582,22 +686,37 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
     HIPCHK(c, hipEventRecord(ev[4 * ncam], c->stream));
     for (size_t ci = 0; ci < ncam; ++ci) {
       Camera& cam = c->cams[ci];
-      MsgLaunch a;
+      CamPass a;
       a.rays = cam.d_rays;
       a.nrays = cam.nrays;
       a.inc = cam.d_inc;
+      a.pair_vox = cam.d_pair_vox;
+      a.pair_off = cam.d_pair_off;
+      a.pair_c = cam.d_pair_c;
+      a.npairs = cam.npairs;
+      a.nlive = (uint32_t)cam.nupd;
+      a.heads = cam.d_heads;
+      a.tile_h0 = cam.d_tile_h0;
+      a.apair = c->d_apair;
+      a.S = c->d_Spair;
+      a.pf_q = c->d_pf_q;
+      a.pl_q = c->d_pl_q;
+      a.pf_v = c->d_pf_v;
+      a.pl_v = c->d_pl_v;
+      a.mid = c->d_mid;
       a.B = c->d_B;
-      a.M = (const char*)c->d_M + ci * c->M_stride * esz;
-      a.S = c->d_S;
+      a.M = (float*)((char*)c->d_M + ci * c->M_stride * esz);
+      a.X = c->d_X;
+      a.upd = c->d_upd;
       a.sig = c->d_sig;
       a.bia = c->d_bia;
       a.lookup_n = c->p.lookup_n;
       a.msg_type = c->p.msg_dtype;
+      launch_expand(a, c->stream);
       HIPCHK(c, hipEventRecord(ev[4 * ci + 0], c->stream));
       launch_ray_messages(a, c->stream);
       HIPCHK(c, hipEventRecord(ev[4 * ci + 1], c->stream));
-      launch_epilogue(cam.d_pair_vox, cam.d_pair_c, cam.npairs, c->d_B, (char*)c->d_M + ci * c->M_stride * esz,
-                      c->p.msg_dtype, c->d_S, c->d_upd, c->stream);
+      launch_epilogue(a, c->stream);
       HIPCHK(c, hipEventRecord(ev[4 * ci + 2], c->stream));
     }
     HIPCHK(c, hipMemsetAsync(c->d_total + 1, 0, 8, c->stream));
@@ -692,7 +811,16 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
   if (n_rays) *n_rays = cam.nrays;
   if (n_records) *n_records = (int64_t)cam.nrec;
   if (rays && cam.nrays) HIPCHK(c, hipMemcpy(rays, cam.d_rays, (size_t)cam.nrays * sizeof(RayRec), hipMemcpyDeviceToHost));
-  if (records && cam.nrec) HIPCHK(c, hipMemcpy(records, cam.d_inc, cam.nrec * sizeof(uint2), hipMemcpyDeviceToHost));
+  if (records && cam.nrec) {
+    uint2* tmp = nullptr;
+    int rc2 = dalloc(c, &tmp, cam.nrec);
+    if (rc2) return rc2;
+    launch_export_vox(cam.d_inc, (uint32_t)cam.nrec, cam.d_pair_off, cam.npairs, cam.d_pair_vox, tmp, c->stream);
+    hipError_t e = hipMemcpyAsync(records, tmp, cam.nrec * sizeof(uint2), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(&tmp);
+    if (e != hipSuccess) return fail(c, MRF_E_HIP, std::string("get_traversal: ") + hipGetErrorString(e));
+  }
   return MRF_OK;
 }
 

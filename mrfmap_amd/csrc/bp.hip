@@ -91,35 +91,87 @@ __device__ __forceinline__ float depth_psi(float s, float d, const Poly& sig, co
   return 0.398942f * __expf(-0.5f * u * u) * inv;
 }
 
-struct VoxTerm {
-  float n;    // clamped P(o = 0) message from the voxel
-  float a;    // l * ln n
+// Kahan-compensated running sum of the per-chunk totals along a ray.
+struct Kahan {
+  float s, c;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  // carry + in-chunk prefix, corrected by the compensation
+  __device__ __forceinline__ float at(float prefix) const { return s + (prefix - c); }
 };
 
-template <int MT>
-__device__ __forceinline__ VoxTerm voxel_term(const float* __restrict__ B, const typename Msg<MT>::T* __restrict__ M,
-                                              uint32_t v, float len, bool live) {
-  VoxTerm t;
-  t.n = 1.0f;
-  t.a = 0.0f;
-  if (live) {
-    const float q = B[v] - Msg<MT>::load(M, v);
-    float nn = 1.0f / (1.0f + __expf(q));
-    nn = fminf(fmaxf(nn, kFEps), 1.0f - kFEps);
-    t.n = nn;
-    t.a = len * __logf(nn);
-  }
-  return t;
+// ---------------------------------------------------------------- segments
+// A camera's incidences are stored pair-major: pair q owns positions
+// [pair_off[q], pair_off[q+1]).  Segment heads are marked in a bitmask and
+// each tile of kTile positions knows how many heads precede it, so the pair
+// of every position follows from one 64-bit word and a popcount.  All
+// segment kernels below work on position tiles (one wavefront per tile), so
+// a pair with hundreds of thousands of incidences (the voxels at a camera's
+// centre, which every ray of that camera crosses) is spread over many
+// wavefronts instead of serialising one.
+
+// Per-lane pair index of positions base + lane (word = head bits of the
+// 64 positions starting at base, before = heads before base).
+__device__ __forceinline__ uint32_t pair_of(uint64_t word, uint32_t before, int lane) {
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  return before + (uint32_t)__popcll(word & upto) - 1u;
 }
 
+// Voxel -> ray message of every pair: ln n = ln clamp(1/(1+exp(B - M_c))),
+// camera c's own previous message excluded from the belief (gvdbkernels.cu:764-773).
+__global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ pair_vox, uint32_t npairs,
+                                                  const float* __restrict__ B, const float* __restrict__ M,
+                                                  float* __restrict__ apair) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npairs) return;
+  const uint32_t v = pair_vox[q];
+  const float qv = B[v] - M[v];
+  float nn = 1.0f / (1.0f + __expf(qv));
+  nn = fminf(fmaxf(nn, kFEps), 1.0f - kFEps);
+  apair[q] = __logf(nn);
+}
+
+// Copy each pair's message into every slot of its segment (one wavefront
+// per tile of kTile positions, coalesced writes).
+template <int MT>
+__global__ __launch_bounds__(256) void k_expand(const uint64_t* __restrict__ heads, const uint32_t* __restrict__ tile_h0,
+                                                uint32_t nlive, const float* __restrict__ apair,
+                                                typename Msg<MT>::T* __restrict__ X) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t j0 = w * kTile;
+  if (j0 >= nlive) return;
+  const uint32_t j1 = min(j0 + kTile, nlive);
+  uint32_t before = tile_h0[w];
+  for (uint32_t base = j0; base < j1; base += 64) {
+    const uint64_t word = heads[base >> 6];
+    const uint32_t j = base + lane;
+    if (j < j1) Msg<MT>::store(X, j, apair[pair_of(word, before, lane)]);
+    before += (uint32_t)__popcll(word);
+  }
+}
+
+// One wavefront per ray.  Pass 1: Lambda_i = sum_{j<=i} l_j ln n_j (the
+// reference's visibility includes voxel i itself, gvdbkernels.cu:781-783),
+// tau_i = V_i (1 - n_i) psi(s_i), T = sum tau (+ V_n if the walk ran out and
+// d lies beyond, gvdbkernels.cu:1196-1208).  Pass 2 recomputes the same terms
+// and writes p_i l_i with mu1 = P_i + V_i psi_i, mu0 = P_i + R_i / n_i
+// (gvdbkernels.cu:936-951) into the incidence's own segment slot.
 template <int MT>
 __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__ rays, uint32_t nrays,
-                                                      const uint2* __restrict__ inc, const float* __restrict__ B,
-                                                      const typename Msg<MT>::T* __restrict__ M,
-                                                      float* __restrict__ S, const float* __restrict__ sigt,
-                                                      const float* __restrict__ biat, int lookup_n) {
+                                                      const uint2* __restrict__ inc,
+                                                      typename Msg<MT>::T* __restrict__ X,
+                                                      const float* __restrict__ sigt, const float* __restrict__ biat,
+                                                      int lookup_n) {
   const int lane = threadIdx.x & 63;
-  const uint32_t r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // XCD-aware order: consecutive blocks (neighbouring pixel tiles) share an XCD's L2
+  const uint32_t per = gridDim.x >> 3;
+  const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  const uint32_t r = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (r >= nrays) return;
   const RayRec rec = rays[r];
   const uint32_t cell = rec.cell_flags & 0x7FFFFFFFu;
@@ -135,152 +187,196 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
   const uint2* __restrict__ list = inc + rec.off;
   const uint32_t n = rec.n;
 
-  // Pass 1: Lambda, tau and T.
-  float cL = 0.0f, cS = 0.0f, T = 0.0f;
+  Kahan kL = {0.f, 0.f}, kS = {0.f, 0.f}, kT = {0.f, 0.f};
   for (uint32_t base = 0; base < n; base += 64) {
     const uint32_t i = base + lane;
-    const bool in = i < n;
-    const uint2 e = in ? list[i] : make_uint2(kGapVox, 0u);
+    const uint2 e = i < n ? list[i] : make_uint2(kGapVox, 0u);
     const float len = __uint_as_float(e.y);
     const bool live = e.x != kGapVox;
-    const VoxTerm vt = voxel_term<MT>(B, M, e.x, len, live);
-    const float L = cL + wave_scan(vt.a);
-    const float sl = cS + wave_scan(len);
-    const float s = rec.z0 + sl * rec.inv_norm;
-    const float tau = live ? __expf(L) * (1.0f - vt.n) * depth_psi(s, d, sig, bia) : 0.0f;
-    T += lane63(wave_scan(tau));
-    cL = lane63(L);
-    cS = lane63(sl);
+    const float lnn = live ? Msg<MT>::load(X, e.x) : 0.0f;
+    const float nn = live ? __expf(lnn) : 1.0f;
+    const float scL = wave_scan(len * lnn);
+    const float scS = wave_scan(len);
+    const float s = rec.z0 + kS.at(scS) * rec.inv_norm;
+    const float tau = live ? __expf(kL.at(scL)) * (1.0f - nn) * depth_psi(s, d, sig, bia) : 0.0f;
+    kT.add(lane63(wave_scan(tau)));
+    kL.add(lane63(scL));
+    kS.add(lane63(scS));
   }
+  float T = kT.s;
   if (rec.cell_flags & kRanOut) {
-    const float sn = rec.z0 + cS * rec.inv_norm;
-    if (d > sn + bia(sn)) T += __expf(cL);
+    const float sn = rec.z0 + kS.s * rec.inv_norm;
+    if (d > sn + bia(sn)) T += __expf(kL.s);
   }
 
-  // Pass 2: outgoing messages p_i and the scatter of p_i * l_i.
-  cL = 0.0f;
-  cS = 0.0f;
-  float cP = 0.0f;
+  kL = {0.f, 0.f};
+  kS = {0.f, 0.f};
+  Kahan kP = {0.f, 0.f};
   for (uint32_t base = 0; base < n; base += 64) {
     const uint32_t i = base + lane;
-    const bool in = i < n;
-    const uint2 e = in ? list[i] : make_uint2(kGapVox, 0u);
+    const uint2 e = i < n ? list[i] : make_uint2(kGapVox, 0u);
     const float len = __uint_as_float(e.y);
     const bool live = e.x != kGapVox;
-    const VoxTerm vt = voxel_term<MT>(B, M, e.x, len, live);
-    const float L = cL + wave_scan(vt.a);
-    const float sl = cS + wave_scan(len);
-    const float s = rec.z0 + sl * rec.inv_norm;
-    const float vpsi = live ? __expf(L) * depth_psi(s, d, sig, bia) : 0.0f;
-    const float tau = vpsi * (1.0f - vt.n);
-    const float Pin = wave_scan(tau);
-    const float P = cP + (Pin - tau);
+    const float lnn = live ? Msg<MT>::load(X, e.x) : 0.0f;
+    const float nn = live ? __expf(lnn) : 1.0f;
+    const float scL = wave_scan(len * lnn);
+    const float scS = wave_scan(len);
+    const float s = rec.z0 + kS.at(scS) * rec.inv_norm;
+    const float vpsi = live ? __expf(kL.at(scL)) * depth_psi(s, d, sig, bia) : 0.0f;
+    const float tau = vpsi * (1.0f - nn);
+    const float scP = wave_scan(tau);
+    const float P = kP.at(scP - tau);
     const float R = (T - P) - tau;
     if (live) {
       const float mu1 = P + vpsi;
-      const float mu0 = P + R / vt.n;
+      const float mu0 = P + R / nn;
       float p = mu1 / (mu1 + mu0 + kMsgNormEps);
       p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
-      unsafeAtomicAdd(S + e.x, p * len);
+      Msg<MT>::store(X, e.x, p * len);
     }
-    cL = lane63(L);
-    cS = lane63(sl);
-    cP += lane63(Pin);
+    kL.add(lane63(scL));
+    kS.add(lane63(scS));
+    kP.add(lane63(scP));
   }
 }
 
-__global__ void k_cum_len(const RayRec* __restrict__ rays, uint32_t nrays, const uint2* __restrict__ inc,
-                          unsigned long long* __restrict__ cfix) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (r >= nrays) return;
-  const RayRec rec = rays[r];
-  for (uint32_t i = lane; i < rec.n; i += 64) {
-    const uint2 e = inc[rec.off + i];
-    if (e.x == kGapVox) continue;
-    const double l = (double)__uint_as_float(e.y) * 4294967296.0;
-    atomicAdd(cfix + e.x, (unsigned long long)__double2ll_rn(l));
-  }
-}
-
-constexpr int kPairThreads = 256;
-constexpr int kPairItems = 8;
-constexpr int kPairPerBlock = kPairThreads * kPairItems;
-
-__device__ uint32_t block_excl_u32(uint32_t v, uint32_t* sh, uint32_t* total) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < kPairThreads; off <<= 1) {
-    uint32_t add = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += add;
-    __syncthreads();
-  }
-  const uint32_t incl = sh[t];
-  *total = sh[kPairThreads - 1];
-  __syncthreads();
-  return incl - v;
-}
-
-__global__ __launch_bounds__(kPairThreads) void k_pair_count(const unsigned long long* __restrict__ cfix,
-                                                              size_t nvox, uint32_t* __restrict__ block_cnt) {
-  __shared__ uint32_t sh[kPairThreads];
-  const size_t base = (size_t)blockIdx.x * kPairPerBlock + (size_t)threadIdx.x * kPairItems;
-  uint32_t c = 0;
-  for (int k = 0; k < kPairItems; ++k) {
-    const size_t v = base + k;
-    if (v < nvox && cfix[v] != 0ull) ++c;
-  }
-  uint32_t tot;
-  block_excl_u32(c, sh, &tot);
-  if (threadIdx.x == 0) block_cnt[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kPairThreads) void k_pair_write(unsigned long long* __restrict__ cfix, size_t nvox,
-                                                              const uint32_t* __restrict__ block_off,
-                                                              uint32_t* __restrict__ pair_vox,
-                                                              float* __restrict__ pair_c) {
-  __shared__ uint32_t sh[kPairThreads];
-  const size_t base = (size_t)blockIdx.x * kPairPerBlock + (size_t)threadIdx.x * kPairItems;
-  unsigned long long vals[kPairItems];
-  uint32_t c = 0;
-  for (int k = 0; k < kPairItems; ++k) {
-    const size_t v = base + k;
-    vals[k] = v < nvox ? cfix[v] : 0ull;
-    if (vals[k] != 0ull) ++c;
-  }
-  uint32_t tot;
-  uint32_t pos = block_off[blockIdx.x] + block_excl_u32(c, sh, &tot);
-  for (int k = 0; k < kPairItems; ++k) {
-    if (vals[k] != 0ull) {
-      pair_vox[pos] = (uint32_t)(base + k);
-      pair_c[pos] = (float)((double)vals[k] * (1.0 / 4294967296.0));
-      ++pos;
-      cfix[base + k] = 0ull;
-    }
-  }
-}
+// Segment sums S_q = sum of p*l over pair q's positions, in a fixed order.
+// One wavefront per tile: a segmented inclusive scan per 64-wide chunk, the
+// open segment carried from chunk to chunk.  Segments that start and end in
+// the tile are written to S directly; the part of a segment that started in
+// an earlier tile goes to pf, the part of one that continues into the next
+// tile to pl, and a tile lying wholly inside one segment to mid.
+constexpr uint32_t kNoPair = 0xFFFFFFFFu;
 
 template <int MT>
-__global__ void k_epilogue(const uint32_t* __restrict__ pair_vox, const float* __restrict__ pair_c,
-                           uint32_t npairs, float* __restrict__ B, typename Msg<MT>::T* __restrict__ M,
-                           float* __restrict__ S, uint8_t* __restrict__ upd) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npairs) return;
-  const uint32_t v = pair_vox[i];
-  const float C = pair_c[i];
-  const float s = S[v];
-  S[v] = 0.0f;
-  if (C > kCumLenMin) {
-    float a = s / C;
-    a = fminf(fmaxf(a, kFEps), 1.0f - kFEps);
-    const float m = __logf(a) - __logf(1.0f - a);
-    const float mo = Msg<MT>::load(M, v);
-    Msg<MT>::store(M, v, m);
-    B[v] = (B[v] + m) - mo;
-    upd[v] = 1;
+__global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ heads, const uint32_t* __restrict__ tile_h0,
+                                                 uint32_t nlive, const typename Msg<MT>::T* __restrict__ X,
+                                                 float* __restrict__ S, uint32_t* __restrict__ pf_q,
+                                                 float* __restrict__ pf_v, uint32_t* __restrict__ pl_q,
+                                                 float* __restrict__ pl_v, float* __restrict__ mid) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t j0 = w * kTile;
+  if (j0 >= nlive) return;
+  const uint32_t j1 = min(j0 + kTile, nlive);
+  uint32_t before = tile_h0[w];
+  const bool open_first = ((heads[j0 >> 6] >> (j0 & 63)) & 1ull) == 0ull;  // tile starts inside a segment
+  float carry = 0.0f;                                                      // open segment's running sum
+  bool carry_from_before = open_first;
+  uint32_t pfq = kNoPair, plq = kNoPair;
+  float pfv = 0.0f, plv = 0.0f, midv = 0.0f;
+  bool is_mid = false;
+  for (uint32_t base = j0; base < j1; base += 64) {
+    const uint64_t word = heads[base >> 6];
+    const uint64_t next_word = (base + 64 < nlive) ? heads[(base >> 6) + 1] : ~0ull;
+    const uint32_t j = base + lane;
+    const bool valid = j < j1;
+    const uint32_t q = pair_of(word, before, lane);
+    int head = (int)((word >> lane) & 1ull);
+    float x = valid ? Msg<MT>::load(X, j) : 0.0f;
+    if (lane == 0 && !head) x += carry;  // continue the open segment
+    if (lane == 0) head = 1;
+#pragma unroll
+    for (int dlt = 1; dlt < 64; dlt <<= 1) {
+      const float y = __shfl_up(x, dlt);
+      const int h = __shfl_up(head, dlt);
+      if (lane >= dlt && !head) {
+        x += y;
+        head |= h;
+      }
+    }
+    // a segment ends at j when j + 1 starts another pair or is past the data
+    const bool nxt_head = lane == 63 ? (next_word & 1ull) != 0ull : ((word >> (lane + 1)) & 1ull) != 0ull;
+    const bool ends = valid && (j + 1 >= nlive || nxt_head);
+    // the chunk's first segment is the continuation of the carried one
+    const uint64_t headbits = word & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const bool in_first = headbits == 0ull;  // no head at or before this lane in this chunk
+    if (ends) {
+      if (in_first && carry_from_before) {
+        pfq = q;  // unique per tile: lane-uniform values written once below
+        pfv = x;
+      } else {
+        S[q] = x;
+      }
+    }
+    // the last valid lane carries its open segment into the next chunk
+    const uint32_t last = min(63u, j1 - 1 - base);
+    const float xl = __shfl(x, (int)last);
+    const int el = __shfl((int)ends, (int)last);
+    const int firstl = __shfl((int)in_first, (int)last);
+    const uint32_t ql = (uint32_t)__shfl((int)q, (int)last);
+    carry_from_before = el ? false : (firstl ? carry_from_before : false);
+    carry = el ? 0.0f : xl;
+    if (base + 64 >= j1 && !el) {  // tile ends inside a segment
+      if (carry_from_before) {
+        is_mid = true;
+        midv = xl;
+      } else {
+        plq = ql;
+        plv = xl;
+      }
+    }
+    before += (uint32_t)__popcll(word);
   }
+  // pf was set by the single lane holding the end of the tile's first segment
+  const unsigned long long pfmask = __ballot(pfq != kNoPair);
+  if (lane == 0) {
+    pf_q[w] = kNoPair;
+    pl_q[w] = plq;
+    pl_v[w] = plv;
+    mid[w] = is_mid ? midv : 0.0f;
+  }
+  if (pfmask) {
+    const int src = __ffsll((long long)pfmask) - 1;
+    const uint32_t q = (uint32_t)__shfl((int)pfq, src);
+    const float v = __shfl(pfv, src);
+    if (lane == 0) {
+      pf_q[w] = q;
+      pf_v[w] = v;
+    }
+  }
+}
+
+// Pairs whose segment crosses tiles: S = pl(head tile) + sum of mid over the
+// tiles in between + pf(tail tile); one wavefront per head tile, the middle
+// tiles reduced 64 at a time in a fixed order.
+__global__ __launch_bounds__(256) void k_seg_span(const uint32_t* __restrict__ pair_off, const uint32_t* __restrict__ pl_q,
+                                                  const float* __restrict__ pl_v, const float* __restrict__ pf_v,
+                                                  const float* __restrict__ mid, uint32_t ntiles,
+                                                  float* __restrict__ S) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= ntiles) return;
+  const uint32_t q = pl_q[w];
+  if (q == kNoPair) return;
+  const uint32_t wt = (pair_off[q + 1] - 1u) / kTile;  // tile holding the segment's last position
+  float acc = 0.0f;
+  for (uint32_t b = w + 1; b < wt; b += 64) {
+    const uint32_t t = b + lane;
+    acc += lane63(wave_scan(t < wt ? mid[t] : 0.0f));
+  }
+  if (lane == 0) S[q] = (pl_v[w] + acc) + pf_v[wt];
+}
+
+// Camera message of each (camera, voxel) pair from its segment sum:
+// a = clamp(S/C), m = ln a - ln(1-a), B += m - M_c, M_c = m, for C > 0.1
+// (compute_gvdb_message_to_factors, gvdbkernels.cu:408, 457-479).
+__global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
+                                                    const float* __restrict__ pair_c, uint32_t npairs,
+                                                    const float* __restrict__ S, float* __restrict__ B,
+                                                    float* __restrict__ M, uint8_t* __restrict__ upd) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npairs) return;
+  const float C = pair_c[q];
+  if (!(C > kCumLenMin)) return;
+  const uint32_t v = pair_vox[q];
+  float a = S[q] / C;
+  a = fminf(fmaxf(a, kFEps), 1.0f - kFEps);
+  const float m = __logf(a) - __logf(1.0f - a);
+  const float mo = M[v];
+  M[v] = m;
+  B[v] = (B[v] + m) - mo;
+  upd[v] = 1;
 }
 
 // marginal occupancy 1 - clamp(1/(1+e^B)) (gvdbkernels.cu:469-476)
@@ -339,60 +435,77 @@ inline unsigned grid_for(size_t n, int threads) { return (unsigned)((n + threads
 
 }  // namespace
 
-void launch_ray_messages(const MsgLaunch& a, hipStream_t s) {
-  if (a.nrays == 0) return;
-  const int waves = 4;
-  dim3 grid((a.nrays + waves - 1) / waves), block(64 * waves);
+void launch_expand(const CamPass& a, hipStream_t s) {
+  if (a.npairs == 0) return;
+  hipLaunchKernelGGL(k_pair_msg, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.npairs, a.B, a.M,
+                     a.apair);
+  const uint32_t ntiles = (a.nlive + kTile - 1) / kTile;
+  dim3 grid(grid_for(ntiles, 4)), block(256);
   switch (a.msg_type) {
     case kMsgF16:
-      hipLaunchKernelGGL(k_ray_messages<kMsgF16>, grid, block, 0, s, a.rays, a.nrays, a.inc, a.B,
-                         (const __half*)a.M, a.S, a.sig, a.bia, a.lookup_n);
+      hipLaunchKernelGGL(k_expand<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, a.apair, (__half*)a.X);
       break;
     case kMsgBF16:
-      hipLaunchKernelGGL(k_ray_messages<kMsgBF16>, grid, block, 0, s, a.rays, a.nrays, a.inc, a.B,
-                         (const uint16_t*)a.M, a.S, a.sig, a.bia, a.lookup_n);
+      hipLaunchKernelGGL(k_expand<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, a.apair,
+                         (uint16_t*)a.X);
       break;
     default:
-      hipLaunchKernelGGL(k_ray_messages<kMsgF32>, grid, block, 0, s, a.rays, a.nrays, a.inc, a.B,
-                         (const float*)a.M, a.S, a.sig, a.bia, a.lookup_n);
+      hipLaunchKernelGGL(k_expand<kMsgF32>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, a.apair, (float*)a.X);
   }
 }
 
-void launch_cum_len(const RayRec* rays, uint32_t nrays, const uint2* inc, unsigned long long* cfix,
-                    hipStream_t s) {
-  if (nrays == 0) return;
-  hipLaunchKernelGGL(k_cum_len, dim3((nrays + 3) / 4), dim3(256), 0, s, rays, nrays, inc, cfix);
-}
-
-size_t pair_blocks(size_t nvox) { return (nvox + kPairPerBlock - 1) / kPairPerBlock; }
-
-void launch_build_pairs(unsigned long long* cfix, size_t nvox, uint32_t* block_cnt, uint32_t* block_off,
-                        uint64_t* scan_tmp, uint64_t* d_total, uint32_t* pair_vox, float* pair_c,
-                        hipStream_t s) {
-  const size_t nb = pair_blocks(nvox);
-  if (nb == 0) return;
-  hipLaunchKernelGGL(k_pair_count, dim3((unsigned)nb), dim3(kPairThreads), 0, s, cfix, nvox, block_cnt);
-  launch_exclusive_scan(block_cnt, block_off, nb, scan_tmp, d_total, s);
-  if (pair_vox != nullptr)
-    hipLaunchKernelGGL(k_pair_write, dim3((unsigned)nb), dim3(kPairThreads), 0, s, cfix, nvox, block_off,
-                       pair_vox, pair_c);
-}
-
-void launch_epilogue(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, float* B, void* M,
-                     int msg_type, float* S, uint8_t* upd, hipStream_t s) {
-  if (npairs == 0) return;
-  dim3 grid(grid_for(npairs, 256)), block(256);
-  switch (msg_type) {
+void launch_ray_messages(const CamPass& a, hipStream_t s) {
+  if (a.nrays == 0) return;
+  const int waves = 4;
+  // grid padded to a multiple of the 8 XCDs for the block remapping
+  const unsigned nb = ((a.nrays + waves - 1) / waves + 7) / 8 * 8;
+  dim3 grid(nb), block(64 * waves);
+  switch (a.msg_type) {
     case kMsgF16:
-      hipLaunchKernelGGL(k_epilogue<kMsgF16>, grid, block, 0, s, pair_vox, pair_c, npairs, B, (__half*)M, S, upd);
+      hipLaunchKernelGGL(k_ray_messages<kMsgF16>, grid, block, 0, s, a.rays, a.nrays, a.inc, (__half*)a.X, a.sig,
+                         a.bia, a.lookup_n);
       break;
     case kMsgBF16:
-      hipLaunchKernelGGL(k_epilogue<kMsgBF16>, grid, block, 0, s, pair_vox, pair_c, npairs, B, (uint16_t*)M, S,
-                         upd);
+      hipLaunchKernelGGL(k_ray_messages<kMsgBF16>, grid, block, 0, s, a.rays, a.nrays, a.inc, (uint16_t*)a.X, a.sig,
+                         a.bia, a.lookup_n);
       break;
     default:
-      hipLaunchKernelGGL(k_epilogue<kMsgF32>, grid, block, 0, s, pair_vox, pair_c, npairs, B, (float*)M, S, upd);
+      hipLaunchKernelGGL(k_ray_messages<kMsgF32>, grid, block, 0, s, a.rays, a.nrays, a.inc, (float*)a.X, a.sig,
+                         a.bia, a.lookup_n);
   }
+}
+
+void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,
+                            uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v, float* mid,
+                            const uint32_t* pair_off, hipStream_t s) {
+  if (nlive == 0) return;
+  const uint32_t ntiles = (nlive + kTile - 1) / kTile;
+  dim3 grid(grid_for(ntiles, 4)), block(256);
+  hipLaunchKernelGGL(k_seg_sum<kMsgF32>, grid, block, 0, s, heads, tile_h0, nlive, X, S, pf_q, pf_v, pl_q, pl_v, mid);
+  hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, pair_off, pl_q, pl_v, pf_v, mid, ntiles, S);
+}
+
+void launch_epilogue(const CamPass& a, hipStream_t s) {
+  if (a.npairs == 0) return;
+  const uint32_t ntiles = (a.nlive + kTile - 1) / kTile;
+  dim3 grid(grid_for(ntiles, 4)), block(256);
+  switch (a.msg_type) {
+    case kMsgF16:
+      hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const __half*)a.X,
+                         a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
+      hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.pair_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+      break;
+    case kMsgBF16:
+      hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const uint16_t*)a.X,
+                         a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
+      hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.pair_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+      break;
+    default:
+      launch_segment_sum_f32(a.heads, a.tile_h0, a.nlive, (const float*)a.X, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v,
+                             a.mid, a.pair_off, s);
+  }
+  hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.npairs,
+                     a.S, a.B, a.M, a.upd);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

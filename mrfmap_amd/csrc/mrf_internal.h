@@ -23,6 +23,7 @@ constexpr double kFixScale = 65536.0;
 constexpr int kBrickLog = 3;           // 8^3 voxels per brick
 constexpr int kBrickVox = 512;
 constexpr uint32_t kGapVox = 0xFFFFFFFFu;  // incidence record that only advances z
+constexpr uint32_t kTile = 512;           // positions per wavefront in the segment kernels (8 x 64)
 
 // Per-ray record of one camera's traversal list.
 struct RayRec {

@@ -24,32 +24,64 @@ size_t scan_tmp_words(size_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint64_t* tmp, uint64_t* d_total,
                            hipStream_t s);
 
-// bp.hip
-struct MsgLaunch {
+int num_tiled_pixels(const CamGeom& c);
+
+// geometry.hip: stable sort of a camera's incidences by voxel into
+// (camera, voxel) segments; records are rewritten to sorted positions.
+struct SegmentBuild {
+  uint2* rec;          // incidence records (voxel -> sorted position in place)
+  uint32_t nrec;
+  uint32_t gap_key;    // key of gap records (> every voxel id)
+  int key_bits;
+  uint32_t *keys_in, *keys_out, *vals_in, *vals_out, *heads, *hscan;
+  void* temp;
+  size_t temp_bytes;
+  uint64_t* scan_tmp;
+  uint64_t* d_total;   // receives the number of pairs
+};
+size_t segment_sort_temp_bytes(uint32_t n, int bits);
+hipError_t build_segments(SegmentBuild& b, hipStream_t s);
+// pair offsets/voxels, summed chord length C per pair (needs the head
+// bitmask of launch_pack_heads), then records rewritten to sorted positions
+void finish_segments(SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t* pair_off, uint32_t* pair_vox,
+                     float* pair_c, const uint64_t* head_bits, const uint32_t* tile_h0, hipStream_t s);
+void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_off, uint32_t npairs,
+                       const uint32_t* pair_vox, uint2* out, hipStream_t s);
+// head bitmask (bit j = position j starts a pair) and heads before each tile
+void launch_pack_heads(const SegmentBuild& b, uint32_t nlive, uint64_t* bits, uint32_t* tile_h0, hipStream_t s);
+
+// bp.hip: one camera pass = expand -> ray messages -> segmented epilogue.
+struct CamPass {
   const RayRec* rays;
   uint32_t nrays;
-  const uint2* inc;
-  const float* B;
-  const void* M;     // message array of this camera (MsgType)
-  float* S;          // ray->voxel accumulator (pool sized)
-  const float* sig;  // sensor-model lookup tables
+  const uint2* inc;        // {sorted position | kGapVox, chord length}
+  const uint32_t* pair_vox;
+  const uint32_t* pair_off;  // npairs + 1
+  const float* pair_c;
+  uint32_t npairs;
+  uint32_t nlive;          // positions (live incidences)
+  const uint64_t* heads;   // head bitmask
+  const uint32_t* tile_h0; // heads before each kTile tile
+  float* apair;            // per-pair scratch: voxel -> ray message
+  float* S;                // per-pair scratch: segment sums
+  uint32_t *pf_q, *pl_q;   // per-tile scratch of the segment sums
+  float *pf_v, *pl_v, *mid;
+  float* B;
+  float* M;                // this camera's messages M_c (FP32, pool sized)
+  void* X;                 // segment array: ln n in, p*l out (MsgType)
+  uint8_t* upd;
+  const float* sig;        // sensor-model lookup tables
   const float* bia;
   int lookup_n;
   int msg_type;
 };
-void launch_ray_messages(const MsgLaunch& a, hipStream_t s);
-
-void launch_cum_len(const RayRec* rays, uint32_t nrays, const uint2* inc, unsigned long long* cfix,
-                    hipStream_t s);
-// Compacts pool voxels with cfix > 0 into (vox, C) pairs in ascending voxel
-// order and clears cfix.  Needs scan scratch for nvox / 2048 blocks.
-void launch_build_pairs(unsigned long long* cfix, size_t nvox, uint32_t* block_cnt, uint32_t* block_off,
-                        uint64_t* scan_tmp, uint64_t* d_total, uint32_t* pair_vox, float* pair_c,
-                        hipStream_t s);
-size_t pair_blocks(size_t nvox);
-
-void launch_epilogue(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, float* B, void* M,
-                     int msg_type, float* S, uint8_t* upd, hipStream_t s);
+void launch_expand(const CamPass& a, hipStream_t s);
+// deterministic segment sums of an FP32 position array into S[pair]
+void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,
+                            uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v, float* mid,
+                            const uint32_t* pair_off, hipStream_t s);
+void launch_ray_messages(const CamPass& a, hipStream_t s);
+void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
 void launch_residual(const float* B, float* Aprev, size_t nvox, unsigned long long* d_max, hipStream_t s);
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t nvox, hipStream_t s);

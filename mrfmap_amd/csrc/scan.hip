@@ -86,7 +86,7 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint64_t
                            hipStream_t s) {
   const size_t nb = (n + kPerBlock - 1) / kPerBlock;
   if (nb == 0) {
-    hipMemsetAsync(d_total, 0, sizeof(uint64_t), s);
+    (void)hipMemsetAsync(d_total, 0, sizeof(uint64_t), s);
     return;
   }
   hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nb), dim3(kThreads), 0, s, in, n, tmp);

@@ -33,6 +33,10 @@ namespace mrf {
 
 namespace {
 
+__host__ __device__ __forceinline__ int tiled_pixels(const CamGeom& c) {
+  return ((c.cols + 7) >> 3) * ((c.rows + 7) >> 3) * 64;
+}
+
 struct Rat {
   int64_t n, d;  // value n / d with d > 0
 };
@@ -250,13 +254,23 @@ struct PixelCtx {
   uint32_t cell;
 };
 
+// Rays are enumerated in 8x8 pixel tiles (tiles row-major, pixels row-major
+// inside a tile) so that rays sharing voxels sit next to each other in the
+// lists and run together on the GPU.
+__device__ __forceinline__ bool tile_pixel(const CamGeom& c, int t, int* x, int* y) {
+  const int ntx = (c.cols + 7) >> 3;
+  const int tile = t >> 6, l = t & 63;
+  *x = (tile % ntx) * 8 + (l & 7);
+  *y = (tile / ntx) * 8 + (l >> 3);
+  return *x < c.cols && *y < c.rows;
+}
+
 __device__ __forceinline__ PixelCtx load_pixel(const CamGeom& c, const GridGeom& g, const float* depth,
-                                               int pix) {
+                                               int t) {
   PixelCtx p;
   p.ok = false;
-  p.x = pix % c.cols;
-  p.y = pix / c.cols;
-  p.d = depth[pix];
+  if (!tile_pixel(c, t, &p.x, &p.y)) return p;
+  p.d = depth[p.y * c.cols + p.x];
   if (!pixel_valid(c, p.x, p.y, p.d)) return p;
   if (!setup_ray(c, g, p.x, p.y, p.rs)) return p;
   p.cell = lookup_offset(c, g.lookup_n, p.x, p.y);
@@ -268,7 +282,7 @@ __global__ void k_mark_rays(CamGeom c, GridGeom g, const float* __restrict__ dep
                             const float* __restrict__ sig, const float* __restrict__ bia,
                             uint8_t* __restrict__ bflags) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= c.rows * c.cols) return;
+  if (pix >= tiled_pixels(c)) return;
   PixelCtx p = load_pixel(c, g, depth, pix);
   if (!p.ok) return;
   walk<kMark>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, nullptr, bflags, nullptr);
@@ -277,7 +291,7 @@ __global__ void k_mark_rays(CamGeom c, GridGeom g, const float* __restrict__ dep
 __global__ void k_mark_endpoints(CamGeom c, GridGeom g, const float* __restrict__ depth,
                                  uint8_t* __restrict__ bflags) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= c.rows * c.cols) return;
+  if (pix >= tiled_pixels(c)) return;
   PixelCtx p = load_pixel(c, g, depth, pix);
   if (!p.ok) return;
   const int M[3] = {g.mx, g.my, g.mz};
@@ -304,7 +318,7 @@ __global__ void k_count(CamGeom c, GridGeom g, const float* __restrict__ depth,
                         const int32_t* __restrict__ slot, uint32_t* __restrict__ counts,
                         uint32_t* __restrict__ has) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= c.rows * c.cols) return;
+  if (pix >= tiled_pixels(c)) return;
   PixelCtx p = load_pixel(c, g, depth, pix);
   uint32_t n = 0;
   if (p.ok) n = walk<kCount>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, nullptr).n;
@@ -318,7 +332,7 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth,
                        const uint32_t* __restrict__ ray_idx, uint2* __restrict__ inc,
                        RayRec* __restrict__ rays) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= c.rows * c.cols) return;
+  if (pix >= tiled_pixels(c)) return;
   PixelCtx p = load_pixel(c, g, depth, pix);
   if (!p.ok) return;
   const uint32_t off = offs[pix];
@@ -336,21 +350,23 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth,
 
 }  // namespace
 
+int num_tiled_pixels(const CamGeom& c) { return tiled_pixels(c); }
+
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
                       const float* bia, uint8_t* bflags, hipStream_t s) {
-  const int n = c.rows * c.cols;
+  const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_mark_rays, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, bflags);
 }
 
 void launch_mark_endpoints(const CamGeom& c, const GridGeom& g, const float* depth, uint8_t* bflags,
                            hipStream_t s) {
-  const int n = c.rows * c.cols;
+  const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_mark_endpoints, dim3((n + 255) / 256), dim3(256), 0, s, c, g, depth, bflags);
 }
 
 void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
                   const float* bia, const int32_t* slot, uint32_t* counts, uint32_t* has, hipStream_t s) {
-  const int n = c.rows * c.cols;
+  const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_count, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, counts,
                      has);
 }
@@ -358,7 +374,7 @@ void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const
 void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
                  const float* bia, const int32_t* slot, const uint32_t* offs, const uint32_t* ray_idx,
                  uint2* inc, RayRec* rays, hipStream_t s) {
-  const int n = c.rows * c.cols;
+  const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_emit, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, offs,
                      ray_idx, inc, rays);
 }

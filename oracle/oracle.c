@@ -495,8 +495,13 @@ int orc_prepare(orc_ctx* c) {
     free_cam_lists(k);
     int64_t cap = 1024;
     k->rays = (uint32_t*)malloc(sizeof(uint32_t) * 6 * (size_t)cap);
-    for (int y = 0; y < k->rows; ++y)
-      for (int x = 0; x < k->cols; ++x) {
+    /* rays are listed in 8x8 pixel tiles (tiles row-major, pixels row-major
+       within a tile), the order the traversal contract fixes */
+    const int ntx = (k->cols + 7) / 8, nty = (k->rows + 7) / 8;
+    for (int tile = 0; tile < ntx * nty; ++tile)
+      for (int l = 0; l < 64; ++l) {
+        const int x = (tile % ntx) * 8 + (l % 8), y = (tile / ntx) * 8 + (l / 8);
+        if (x >= k->cols || y >= k->rows) continue;
         float d = k->depth[(size_t)y * k->cols + x];
         int64_t Oi[3], Di[3];
         double norm;

@@ -57,6 +57,22 @@ def test_traversal_bit_exact_real_shape():
     assert n > 5_000_000
 
 
+def test_bp_marginals_real_shape():
+    """640x480, f=525, 1 cm: two keyframes, every 3rd pixel.  Every ray of a
+    camera crosses the voxels at its centre, so those segments hold tens of
+    thousands of incidences and span many tiles of the segment kernels."""
+    p, frames, _ = synth.make_workload("real", frames=13)
+    frames = [(pose, K, decimate(d, 3)) for (pose, K, d) in frames[:13:6]]
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    st = g.perform_inference(max_iters=3)
+    so = o.perform_inference(3)
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"real-shape max-abs marginal error {err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e}")
+    assert err.max() <= MARG_TOL
+
+
 @pytest.mark.parametrize("activation", [_lib.ACT_FULL_RAY, _lib.ACT_ENDPOINT])
 def test_bp_marginals_small(activation):
     p, frames, _ = synth.make_workload("small", activation=activation)
@@ -105,3 +121,28 @@ def test_bad_camera_id():
     with pytest.raises(_lib.MrfError) as e:
         g.set_pose(5, np.eye(4))
     assert e.value.code == -5
+
+
+def test_bp_bitwise_reproducible():
+    """No atomics in the BP iteration: two runs give identical bits."""
+    p, frames, _ = synth.make_workload("small")
+    outs = []
+    for _ in range(2):
+        g = gpu_for(p, frames)
+        g.perform_inference(max_iters=3)
+        outs.append(g.get_beliefs())
+    np.testing.assert_array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("msg", [_lib.MSG_F16, _lib.MSG_BF16])
+def test_bp_mixed_precision_small(msg):
+    """FP16 / BF16 message storage with FP32 scans: max-abs marginal error vs
+    the FP64 oracle within 1e-3."""
+    p, frames, _ = synth.make_workload("small", msg_dtype=msg)
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.perform_inference(max_iters=5)
+    o.perform_inference(5)
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"msg {msg}: max-abs {err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e}")
+    assert err.max() <= MARG_TOL
