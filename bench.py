@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--msg", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged run (profiling)")
     return ap.parse_args()
 
 
@@ -87,9 +88,14 @@ def main():
 
     # time-to-converged from a fresh map (geometry build included)
     g = build()
-    conv = g.perform_inference(max_iters=200, tol=a.tol)
-    print(f"[bench] converged: {conv['iters']} iters residual {conv['residual']:.2e} "
-          f"total {conv['t_total_ms']:.1f} ms (geometry {conv['t_geometry_ms']:.1f} ms)", file=sys.stderr, flush=True)
+    if a.no_converge:
+        conv = g.perform_inference(max_iters=1)
+        conv["t_total_ms"] = None
+    else:
+        conv = g.perform_inference(max_iters=200, tol=a.tol)
+        print(f"[bench] converged: {conv['iters']} iters residual {conv['residual']:.2e} "
+              f"total {conv['t_total_ms']:.1f} ms (geometry {conv['t_geometry_ms']:.1f} ms)", file=sys.stderr,
+              flush=True)
 
     # throughput: W warmup steps then exactly K timed steps on resident data
     if a.warmup > 0:

@@ -49,10 +49,15 @@ extern "C" {
 #define MRF_ACT_ENDPOINT 0 /* bricks within 1 voxel of a depth endpoint (GVDBMRFMap.cpp:117-135) */
 #define MRF_ACT_FULL_RAY 1 /* every brick a truncated ray crosses */
 
-/* Storage type of the per-camera ray->voxel messages M_c. */
-#define MRF_MSG_F32 0
-#define MRF_MSG_F16 1
-#define MRF_MSG_BF16 2
+/* Storage type of the per-incidence messages of a camera pass: the
+ * voxel->ray log message ln n and the ray->voxel message p*l.  Scans and
+ * accumulation are FP32 (Kahan-compensated along rays) in every mode; the
+ * per-(camera, voxel) messages M_c and the beliefs stay FP32. */
+#define MRF_MSG_F32 0        /* both FP32 */
+#define MRF_MSG_F16 1        /* both FP16 */
+#define MRF_MSG_BF16 2       /* both BF16 */
+#define MRF_MSG_MIXED_F16 3  /* ln n FP32, p*l FP16 */
+#define MRF_MSG_MIXED_BF16 4 /* ln n FP32, p*l BF16 */
 
 typedef struct mrf_ctx mrf_ctx;
 

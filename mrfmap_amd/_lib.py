@@ -15,7 +15,7 @@ ERRORS = {-1: "MRF_E_INVALID", -2: "MRF_E_HIP", -3: "MRF_E_NOMEM", -4: "MRF_E_SH
           -6: "MRF_E_STATE", -7: "MRF_E_LIMIT"}
 ROW_MAJOR, COL_MAJOR = 0, 1
 ACT_ENDPOINT, ACT_FULL_RAY = 0, 1
-MSG_F32, MSG_F16, MSG_BF16 = 0, 1, 2
+MSG_F32, MSG_F16, MSG_BF16, MSG_MIXED_F16, MSG_MIXED_BF16 = 0, 1, 2, 3, 4
 
 # every symbol include/mrfmap_hip.h declares
 EXPORTS = [

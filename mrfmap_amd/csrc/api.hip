@@ -34,10 +34,13 @@ struct Camera {
   float K[9];
   int rows = 0, cols = 0;
   float* d_depth = nullptr;
-  RayRec* d_rays = nullptr;
-  uint32_t nrays = 0;
-  uint2* d_inc = nullptr;
-  uint64_t nrec = 0, nupd = 0;
+  RayRec* d_lanes = nullptr;       // one per lane of every slice (tiled pixel order)
+  uint32_t* d_slice_len = nullptr; // records per lane of each slice
+  uint32_t nslices = 0;
+  uint32_t nrays = 0;              // rays with at least one record
+  uint2* d_inc = nullptr;          // sliced records (padded)
+  uint64_t nrec = 0, nupd = 0;     // padded records, live incidences
+  uint64_t nlist = 0;              // records without the slice padding
   uint32_t* d_pair_vox = nullptr;
   uint32_t* d_pair_off = nullptr;
   float* d_pair_c = nullptr;
@@ -45,8 +48,6 @@ struct Camera {
   uint64_t* d_heads = nullptr;   // head bitmask over sorted positions
   uint32_t* d_tile_h0 = nullptr; // heads before each kTile tile
 };
-
-size_t msg_size(int t) { return t == MRF_MSG_F32 ? 4 : 2; }
 
 }  // namespace
 
@@ -74,8 +75,10 @@ struct mrf_ctx {
   void* d_M = nullptr;
   size_t M_bytes = 0;
   size_t M_stride = 0;    // elements per camera
-  void* d_X = nullptr;    // segment array of the camera being processed
+  void* d_X = nullptr;    // segment array (voxel->ray side) of the camera being processed
   size_t X_bytes = 0;
+  void* d_Xo = nullptr;   // ray->voxel side when its storage type differs
+  size_t Xo_bytes = 0;
   // per-pair / per-tile scratch of a camera pass
   float *d_apair = nullptr, *d_Spair = nullptr;
   size_t cap_pairs = 0;
@@ -273,7 +276,8 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
 }
 
 void free_camera_lists(Camera& cam) {
-  dfree(&cam.d_rays);
+  dfree(&cam.d_lanes);
+  dfree(&cam.d_slice_len);
   dfree(&cam.d_inc);
   dfree(&cam.d_pair_vox);
   dfree(&cam.d_pair_off);
@@ -281,7 +285,8 @@ void free_camera_lists(Camera& cam) {
   dfree(&cam.d_heads);
   dfree(&cam.d_tile_h0);
   cam.nrays = 0;
-  cam.nrec = cam.nupd = 0;
+  cam.nslices = 0;
+  cam.nrec = cam.nupd = cam.nlist = 0;
   cam.npairs = 0;
 }
 
@@ -312,20 +317,29 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   free_camera_lists(cam);
   const CamGeom cg = make_geom(c, cam);
   const size_t npix = (size_t)num_tiled_pixels(cg);
+  const uint32_t ntiles = (uint32_t)(npix / kSlice);
   int rc;
   if ((rc = ensure_pixel_scratch(c, npix)) || (rc = ensure_scan(c, npix))) return rc;
   launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, c->stream);
   launch_exclusive_scan(c->d_counts, c->d_offs, npix, c->d_scan_tmp, c->d_total, c->stream);
-  const uint64_t nrec = read_total(c);
+  const uint64_t nlist = read_total(c);
   launch_exclusive_scan(c->d_has, c->d_rayidx, npix, c->d_scan_tmp, c->d_total, c->stream);
   const uint64_t nrays = read_total(c);
+  // slices: per tile the longest of its 64 rays; d_offs/d_rayidx are reused as
+  // the slice sizes and offsets
+  if ((rc = dalloc(c, &cam.d_slice_len, ntiles))) return rc;
+  launch_slice_len(c->d_counts, ntiles, cam.d_slice_len, c->d_rayidx, c->stream);
+  launch_exclusive_scan(c->d_rayidx, c->d_offs, ntiles, c->d_scan_tmp, c->d_total, c->stream);
+  const uint64_t nrec = read_total(c);
   HIPCHK(c, hipGetLastError());
   if (nrec >= 0x7FFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^31 incidence records in one camera");
   cam.nrec = nrec;
+  cam.nlist = nlist;
   cam.nrays = (uint32_t)nrays;
-  if ((rc = dalloc(c, &cam.d_inc, nrec)) || (rc = dalloc(c, &cam.d_rays, nrays))) return rc;
-  launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_rayidx, cam.d_inc, cam.d_rays,
-              c->stream);
+  cam.nslices = ntiles;
+  if ((rc = dalloc(c, &cam.d_inc, nrec)) || (rc = dalloc(c, &cam.d_lanes, npix))) return rc;
+  launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, cam.d_slice_len, cam.d_inc,
+              cam.d_lanes, c->stream);
   // live incidences (records that are not gap records)
   HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 8, c->stream));
   if (nrec) {
@@ -362,8 +376,8 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_off, np + 1)) ||
       (rc = dalloc(c, &cam.d_pair_c, np)))
     return rc;
-  const size_t nwords = (live + 63) / 64 + 1, ntiles = (live + kTile - 1) / kTile + 1;
-  if ((rc = dalloc(c, &cam.d_heads, nwords)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles))) return rc;
+  const size_t nwords = (live + 63) / 64 + 1, nseg_tiles = (live + kTile - 1) / kTile + 1;
+  if ((rc = dalloc(c, &cam.d_heads, nwords)) || (rc = dalloc(c, &cam.d_tile_h0, nseg_tiles))) return rc;
   if (nrec) {
     launch_pack_heads(b, (uint32_t)live, cam.d_heads, cam.d_tile_h0, c->stream);
     finish_segments(b, cam.npairs, (uint32_t)live, cam.d_pair_off, cam.d_pair_vox, cam.d_pair_c, cam.d_heads,
@@ -442,16 +456,27 @@ int prepare(mrf_ctx* c) {
       return rc;
     c->cap_tiles = tneed;
   }
-  xneed *= msg_size(c->p.msg_dtype);
-  if (xneed > c->X_bytes) {
+  const int ti = msg_in_type(c->p.msg_dtype), to = msg_out_type(c->p.msg_dtype);
+  const size_t xin = xneed * msg_bytes(ti), xout = ti == to ? 0 : xneed * msg_bytes(to);
+  if (xin > c->X_bytes) {
     if (c->d_X) (void)hipFree(c->d_X);
     c->d_X = nullptr;
     c->X_bytes = 0;
-    if (hipMalloc(&c->d_X, xneed) != hipSuccess) {
+    if (hipMalloc(&c->d_X, xin) != hipSuccess) {
       (void)hipGetLastError();
       return fail(c, MRF_E_NOMEM, "segment array allocation failed");
     }
-    c->X_bytes = xneed;
+    c->X_bytes = xin;
+  }
+  if (xout > c->Xo_bytes) {
+    if (c->d_Xo) (void)hipFree(c->d_Xo);
+    c->d_Xo = nullptr;
+    c->Xo_bytes = 0;
+    if (hipMalloc(&c->d_Xo, xout) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, MRF_E_NOMEM, "segment array allocation failed");
+    }
+    c->Xo_bytes = xout;
   }
   c->geom_dirty = false;
   return MRF_OK;
@@ -515,7 +540,7 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
   if (!p || !out) return MRF_E_INVALID;
   *out = nullptr;
   if (!(p->res > 0) || !(p->prior > 0 && p->prior < 1) || (p->lookup_n != 2 && p->lookup_n != 3) ||
-      p->activation < 0 || p->activation > 1 || p->msg_dtype < 0 || p->msg_dtype > 2)
+      p->activation < 0 || p->activation > 1 || p->msg_dtype < 0 || p->msg_dtype > 4)
     return MRF_E_INVALID;
   int cnt = 0;
   if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) {
@@ -594,6 +619,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
   if (c->d_M) (void)hipFree(c->d_M);
   if (c->d_X) (void)hipFree(c->d_X);
+  if (c->d_Xo) (void)hipFree(c->d_Xo);
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
   dfree(&c->d_apair); dfree(&c->d_Spair);
   dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
@@ -673,7 +699,7 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
     s.updates_per_iter += (int64_t)cam.nupd;
     s.pairs_per_iter += cam.npairs;
     s.rays += cam.nrays;
-    s.list_records += (int64_t)cam.nrec;
+    s.list_records += (int64_t)cam.nlist;
   }
   const size_t ncam = c->cams.size();
   std::vector<hipEvent_t> ev(4 * ncam + 2);
@@ -687,8 +713,9 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
     for (size_t ci = 0; ci < ncam; ++ci) {
       Camera& cam = c->cams[ci];
       CamPass a;
-      a.rays = cam.d_rays;
-      a.nrays = cam.nrays;
+      a.lanes = cam.d_lanes;
+      a.slice_len = cam.d_slice_len;
+      a.nslices = cam.nslices;
       a.inc = cam.d_inc;
       a.pair_vox = cam.d_pair_vox;
       a.pair_off = cam.d_pair_off;
@@ -707,6 +734,7 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
       a.B = c->d_B;
       a.M = (float*)((char*)c->d_M + ci * c->M_stride * esz);
       a.X = c->d_X;
+      a.Xo = msg_in_type(c->p.msg_dtype) == msg_out_type(c->p.msg_dtype) ? c->d_X : c->d_Xo;
       a.upd = c->d_upd;
       a.sig = c->d_sig;
       a.bia = c->d_bia;
@@ -809,17 +837,43 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
   if (rc) return rc;
   const Camera& cam = c->cams[id];
   if (n_rays) *n_rays = cam.nrays;
-  if (n_records) *n_records = (int64_t)cam.nrec;
-  if (rays && cam.nrays) HIPCHK(c, hipMemcpy(rays, cam.d_rays, (size_t)cam.nrays * sizeof(RayRec), hipMemcpyDeviceToHost));
-  if (records && cam.nrec) {
+  if (n_records) *n_records = (int64_t)cam.nlist;
+  if (!rays && !records) return MRF_OK;
+  // compact the sliced lists back to per-ray lists in tile order
+  const size_t npix = (size_t)cam.nslices * kSlice;
+  std::vector<RayRec> lanes(npix);
+  std::vector<uint2> sell(cam.nrec);
+  HIPCHK(c, hipMemcpy(lanes.data(), cam.d_lanes, npix * sizeof(RayRec), hipMemcpyDeviceToHost));
+  if (cam.nrec) {
     uint2* tmp = nullptr;
-    int rc2 = dalloc(c, &tmp, cam.nrec);
-    if (rc2) return rc2;
+    if ((rc = dalloc(c, &tmp, cam.nrec))) return rc;
     launch_export_vox(cam.d_inc, (uint32_t)cam.nrec, cam.d_pair_off, cam.npairs, cam.d_pair_vox, tmp, c->stream);
-    hipError_t e = hipMemcpyAsync(records, tmp, cam.nrec * sizeof(uint2), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipMemcpyAsync(sell.data(), tmp, cam.nrec * sizeof(uint2), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(&tmp);
     if (e != hipSuccess) return fail(c, MRF_E_HIP, std::string("get_traversal: ") + hipGetErrorString(e));
+  }
+  uint64_t off = 0, r = 0;
+  for (size_t pix = 0; pix < npix; ++pix) {
+    const RayRec& L = lanes[pix];
+    if (L.n == 0) continue;
+    if (rays) {
+      uint32_t* o = rays + 6 * r;
+      o[0] = (uint32_t)off;
+      o[1] = L.n;
+      std::memcpy(&o[2], &L.z0, 4);
+      std::memcpy(&o[3], &L.inv_norm, 4);
+      std::memcpy(&o[4], &L.depth, 4);
+      o[5] = L.cell_flags;
+    }
+    if (records)
+      for (uint32_t k = 0; k < L.n; ++k) {
+        const uint2 e = sell[L.off + (size_t)k * kSlice];
+        records[2 * (off + k)] = e.x;
+        records[2 * (off + k) + 1] = e.y;
+      }
+    off += L.n;
+    ++r;
   }
   return MRF_OK;
 }

@@ -27,21 +27,26 @@ namespace {
 // ---------------------------------------------------------------- storage
 template <int MT>
 struct Msg;
+// kOut: factor applied to ray->voxel messages p*l before storage (FP16 keeps
+// them out of its subnormal range: p*l >= 1e-4 * l).
 template <>
 struct Msg<kMsgF32> {
   typedef float T;
+  static constexpr float kOut = 1.0f;
   __device__ static float load(const T* p, uint32_t i) { return p[i]; }
   __device__ static void store(T* p, uint32_t i, float v) { p[i] = v; }
 };
 template <>
 struct Msg<kMsgF16> {
   typedef __half T;
+  static constexpr float kOut = 4096.0f;
   __device__ static float load(const T* p, uint32_t i) { return __half2float(p[i]); }
   __device__ static void store(T* p, uint32_t i, float v) { p[i] = __float2half_rn(v); }
 };
 template <>
 struct Msg<kMsgBF16> {
   typedef uint16_t T;
+  static constexpr float kOut = 1.0f;
   __device__ static float load(const T* p, uint32_t i) { return __uint_as_float((uint32_t)p[i] << 16); }
   __device__ static void store(T* p, uint32_t i, float v) {
     uint32_t u = __float_as_uint(v);
@@ -86,7 +91,7 @@ struct Poly {
 // psi(s; d) = N(d; s + bias(s), (3 sigma(s))^2) (depth_potential, gvdbkernels.cu:57-72).
 __device__ __forceinline__ float depth_psi(float s, float d, const Poly& sig, const Poly& bia) {
   const float diff = fabsf(d - (s + bia(s)));
-  const float inv = 1.0f / (3.0f * sig(s));
+  const float inv = __builtin_amdgcn_rcpf(3.0f * sig(s));
   const float u = diff * inv;
   return 0.398942f * __expf(-0.5f * u * u) * inv;
 }
@@ -141,102 +146,136 @@ template <int MT>
 __global__ __launch_bounds__(256) void k_expand(const uint64_t* __restrict__ heads, const uint32_t* __restrict__ tile_h0,
                                                 uint32_t nlive, const float* __restrict__ apair,
                                                 typename Msg<MT>::T* __restrict__ X) {
+  constexpr int C = kTile / 64;
   const int lane = threadIdx.x & 63;
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const uint32_t j0 = w * kTile;
   if (j0 >= nlive) return;
   const uint32_t j1 = min(j0 + kTile, nlive);
   uint32_t before = tile_h0[w];
-  for (uint32_t base = j0; base < j1; base += 64) {
-    const uint64_t word = heads[base >> 6];
+  float v[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const uint32_t base = j0 + 64u * c;
+    const uint64_t word = base < j1 ? heads[base >> 6] : 0ull;
     const uint32_t j = base + lane;
-    if (j < j1) Msg<MT>::store(X, j, apair[pair_of(word, before, lane)]);
+    v[c] = j < j1 ? apair[pair_of(word, before, lane)] : 0.0f;
     before += (uint32_t)__popcll(word);
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const uint32_t j = j0 + 64u * c + lane;
+    if (j < j1) Msg<MT>::store(X, j, v[c]);
   }
 }
 
-// One wavefront per ray.  Pass 1: Lambda_i = sum_{j<=i} l_j ln n_j (the
-// reference's visibility includes voxel i itself, gvdbkernels.cu:781-783),
-// tau_i = V_i (1 - n_i) psi(s_i), T = sum tau (+ V_n if the walk ran out and
-// d lies beyond, gvdbkernels.cu:1196-1208).  Pass 2 recomputes the same terms
-// and writes p_i l_i with mu1 = P_i + V_i psi_i, mu0 = P_i + R_i / n_i
-// (gvdbkernels.cu:936-951) into the incidence's own segment slot.
-template <int MT>
-__global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__ rays, uint32_t nrays,
+// One lane per ray, one wavefront per slice (the 64 rays of an 8x8 pixel
+// tile, records interleaved so that record k of all 64 rays is one coalesced
+// load and the 64 lanes touch neighbouring voxels together).
+//
+// Pass 1: Lambda_k = sum_{j<=k} l_j ln n_j (the reference's visibility
+// includes voxel k itself, gvdbkernels.cu:781-783), vpsi_k = exp(Lambda_k)
+// psi(s_k), tau_k = vpsi_k (1 - n_k), T = sum tau (+ V_n when the walk ran
+// out and d lies beyond, gvdbkernels.cu:1196-1208).  Until the first record
+// with vpsi > 0 every earlier tau is exactly 0, so P = 0, mu1 = 0 and the
+// message is exactly the clamp floor F_EPS: pass 1 writes those directly.
+// Pass 2 restarts from that record with the saved sums and writes
+// p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi, mu0 = P + R / n
+// (gvdbkernels.cu:936-951).  Long sums along the ray are Kahan-compensated.
+struct RayCtx {
+  Poly sig, bia;
+  float d, z0, inv;
+};
+
+template <int MI, int MO>
+__global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__ lanes,
+                                                      const uint32_t* __restrict__ slice_len, uint32_t nslices,
                                                       const uint2* __restrict__ inc,
-                                                      typename Msg<MT>::T* __restrict__ X,
+                                                      const typename Msg<MI>::T* __restrict__ Xi,
+                                                      typename Msg<MO>::T* __restrict__ Xo,
                                                       const float* __restrict__ sigt, const float* __restrict__ biat,
                                                       int lookup_n) {
   const int lane = threadIdx.x & 63;
   // XCD-aware order: consecutive blocks (neighbouring pixel tiles) share an XCD's L2
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const uint32_t r = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (r >= nrays) return;
-  const RayRec rec = rays[r];
+  const uint32_t t = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (t >= nslices) return;
+  const uint32_t len = slice_len[t];
+  if (len == 0) return;
+  const RayRec rec = lanes[t * kSlice + lane];
   const uint32_t cell = rec.cell_flags & 0x7FFFFFFFu;
-  Poly sig, bia;
-  sig.n = bia.n = lookup_n;
-  sig.c0 = sigt[cell];
-  sig.c1 = sigt[cell + 1];
-  sig.c2 = lookup_n == 3 ? sigt[cell + 2] : 0.0f;
-  bia.c0 = biat[cell];
-  bia.c1 = biat[cell + 1];
-  bia.c2 = lookup_n == 3 ? biat[cell + 2] : 0.0f;
-  const float d = rec.depth;
-  const uint2* __restrict__ list = inc + rec.off;
-  const uint32_t n = rec.n;
+  RayCtx rc;
+  rc.sig.n = rc.bia.n = lookup_n;
+  rc.sig.c0 = sigt[cell];
+  rc.sig.c1 = sigt[cell + 1];
+  rc.sig.c2 = lookup_n == 3 ? sigt[cell + 2] : 0.0f;
+  rc.bia.c0 = biat[cell];
+  rc.bia.c1 = biat[cell + 1];
+  rc.bia.c2 = lookup_n == 3 ? biat[cell + 2] : 0.0f;
+  rc.d = rec.depth;
+  rc.z0 = rec.z0;
+  rc.inv = rec.inv_norm;
+  const uint2* __restrict__ rl = inc + rec.off;  // record k at rl[k * kSlice]
 
   Kahan kL = {0.f, 0.f}, kS = {0.f, 0.f}, kT = {0.f, 0.f};
-  for (uint32_t base = 0; base < n; base += 64) {
-    const uint32_t i = base + lane;
-    const uint2 e = i < n ? list[i] : make_uint2(kGapVox, 0u);
-    const float len = __uint_as_float(e.y);
+  Kahan L0 = kL, S0 = kS;
+  uint32_t i0 = len;  // first record with vpsi > 0
+  for (uint32_t k = 0; k < len; ++k) {
+    const uint2 e = rl[(size_t)k * kSlice];
     const bool live = e.x != kGapVox;
-    const float lnn = live ? Msg<MT>::load(X, e.x) : 0.0f;
-    const float nn = live ? __expf(lnn) : 1.0f;
-    const float scL = wave_scan(len * lnn);
-    const float scS = wave_scan(len);
-    const float s = rec.z0 + kS.at(scS) * rec.inv_norm;
-    const float tau = live ? __expf(kL.at(scL)) * (1.0f - nn) * depth_psi(s, d, sig, bia) : 0.0f;
-    kT.add(lane63(wave_scan(tau)));
-    kL.add(lane63(scL));
-    kS.add(lane63(scS));
+    const float lnn = live ? Msg<MI>::load(Xi, e.x) : 0.0f;
+    const float ln = __uint_as_float(e.y);
+    const Kahan Lb = kL, Sb = kS;
+    kL.add(ln * lnn);
+    kS.add(ln);
+    const float s = rc.z0 + kS.at(0.0f) * rc.inv;
+    const float vpsi = live ? __expf(kL.at(0.0f)) * depth_psi(s, rc.d, rc.sig, rc.bia) : 0.0f;
+    if (i0 == len) {
+      if (vpsi > 0.0f) {
+        i0 = k;
+        L0 = Lb;
+        S0 = Sb;
+      } else if (live) {
+        Msg<MO>::store(Xo, e.x, kFEps * ln * Msg<MO>::kOut);
+      }
+    }
+    if (i0 != len) kT.add(vpsi * (1.0f - __expf(lnn)));
   }
-  float T = kT.s;
+  float T = kT.at(0.0f);
   if (rec.cell_flags & kRanOut) {
-    const float sn = rec.z0 + kS.s * rec.inv_norm;
-    if (d > sn + bia(sn)) T += __expf(kL.s);
+    const float sn = rc.z0 + kS.at(0.0f) * rc.inv;
+    if (rc.d > sn + rc.bia(sn)) T += __expf(kL.at(0.0f));
   }
 
-  kL = {0.f, 0.f};
-  kS = {0.f, 0.f};
+  uint32_t kmin = i0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, o));
+  kL = L0;
+  kS = S0;
   Kahan kP = {0.f, 0.f};
-  for (uint32_t base = 0; base < n; base += 64) {
-    const uint32_t i = base + lane;
-    const uint2 e = i < n ? list[i] : make_uint2(kGapVox, 0u);
-    const float len = __uint_as_float(e.y);
+  for (uint32_t k = kmin; k < len; ++k) {
+    if (k < i0) continue;
+    const uint2 e = rl[(size_t)k * kSlice];
     const bool live = e.x != kGapVox;
-    const float lnn = live ? Msg<MT>::load(X, e.x) : 0.0f;
-    const float nn = live ? __expf(lnn) : 1.0f;
-    const float scL = wave_scan(len * lnn);
-    const float scS = wave_scan(len);
-    const float s = rec.z0 + kS.at(scS) * rec.inv_norm;
-    const float vpsi = live ? __expf(kL.at(scL)) * depth_psi(s, d, sig, bia) : 0.0f;
+    const float lnn = live ? Msg<MI>::load(Xi, e.x) : 0.0f;
+    const float ln = __uint_as_float(e.y);
+    kL.add(ln * lnn);
+    kS.add(ln);
+    const float s = rc.z0 + kS.at(0.0f) * rc.inv;
+    const float vpsi = live ? __expf(kL.at(0.0f)) * depth_psi(s, rc.d, rc.sig, rc.bia) : 0.0f;
+    const float nn = __expf(lnn);
     const float tau = vpsi * (1.0f - nn);
-    const float scP = wave_scan(tau);
-    const float P = kP.at(scP - tau);
+    const float P = kP.at(0.0f);
     const float R = (T - P) - tau;
     if (live) {
       const float mu1 = P + vpsi;
-      const float mu0 = P + R / nn;
-      float p = mu1 / (mu1 + mu0 + kMsgNormEps);
+      const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
+      float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
       p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
-      Msg<MT>::store(X, e.x, p * len);
+      Msg<MO>::store(Xo, e.x, p * ln * Msg<MO>::kOut);
     }
-    kL.add(lane63(scL));
-    kS.add(lane63(scS));
-    kP.add(lane63(scP));
+    kP.add(tau);
   }
 }
 
@@ -266,14 +305,24 @@ __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ he
   uint32_t pfq = kNoPair, plq = kNoPair;
   float pfv = 0.0f, plv = 0.0f, midv = 0.0f;
   bool is_mid = false;
-  for (uint32_t base = j0; base < j1; base += 64) {
+  constexpr int C = kTile / 64;
+  float xs[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const uint32_t j = j0 + 64u * c + lane;
+    xs[c] = j < j1 ? Msg<MT>::load(X, j) : 0.0f;
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const uint32_t base = j0 + 64u * c;
+    if (base >= j1) break;
     const uint64_t word = heads[base >> 6];
     const uint64_t next_word = (base + 64 < nlive) ? heads[(base >> 6) + 1] : ~0ull;
     const uint32_t j = base + lane;
     const bool valid = j < j1;
     const uint32_t q = pair_of(word, before, lane);
     int head = (int)((word >> lane) & 1ull);
-    float x = valid ? Msg<MT>::load(X, j) : 0.0f;
+    float x = xs[c];
     if (lane == 0 && !head) x += carry;  // continue the open segment
     if (lane == 0) head = 1;
 #pragma unroll
@@ -363,14 +412,14 @@ __global__ __launch_bounds__(256) void k_seg_span(const uint32_t* __restrict__ p
 // (compute_gvdb_message_to_factors, gvdbkernels.cu:408, 457-479).
 __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
                                                     const float* __restrict__ pair_c, uint32_t npairs,
-                                                    const float* __restrict__ S, float* __restrict__ B,
+                                                    const float* __restrict__ S, float out_scale, float* __restrict__ B,
                                                     float* __restrict__ M, uint8_t* __restrict__ upd) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npairs) return;
   const float C = pair_c[q];
   if (!(C > kCumLenMin)) return;
   const uint32_t v = pair_vox[q];
-  float a = S[q] / C;
+  float a = S[q] / (C * out_scale);
   a = fminf(fmaxf(a, kFEps), 1.0f - kFEps);
   const float m = __logf(a) - __logf(1.0f - a);
   const float mo = M[v];
@@ -441,7 +490,7 @@ void launch_expand(const CamPass& a, hipStream_t s) {
                      a.apair);
   const uint32_t ntiles = (a.nlive + kTile - 1) / kTile;
   dim3 grid(grid_for(ntiles, 4)), block(256);
-  switch (a.msg_type) {
+  switch (msg_in_type(a.msg_type)) {
     case kMsgF16:
       hipLaunchKernelGGL(k_expand<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, a.apair, (__half*)a.X);
       break;
@@ -455,24 +504,22 @@ void launch_expand(const CamPass& a, hipStream_t s) {
 }
 
 void launch_ray_messages(const CamPass& a, hipStream_t s) {
-  if (a.nrays == 0) return;
+  if (a.nslices == 0) return;
   const int waves = 4;
   // grid padded to a multiple of the 8 XCDs for the block remapping
-  const unsigned nb = ((a.nrays + waves - 1) / waves + 7) / 8 * 8;
+  const unsigned nb = ((a.nslices + waves - 1) / waves + 7) / 8 * 8;
   dim3 grid(nb), block(64 * waves);
+#define MRF_RAY(MI, MO)                                                                                        \
+  hipLaunchKernelGGL((k_ray_messages<MI, MO>), grid, block, 0, s, a.lanes, a.slice_len, a.nslices, a.inc,       \
+                     (const typename Msg<MI>::T*)a.X, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n)
   switch (a.msg_type) {
-    case kMsgF16:
-      hipLaunchKernelGGL(k_ray_messages<kMsgF16>, grid, block, 0, s, a.rays, a.nrays, a.inc, (__half*)a.X, a.sig,
-                         a.bia, a.lookup_n);
-      break;
-    case kMsgBF16:
-      hipLaunchKernelGGL(k_ray_messages<kMsgBF16>, grid, block, 0, s, a.rays, a.nrays, a.inc, (uint16_t*)a.X, a.sig,
-                         a.bia, a.lookup_n);
-      break;
-    default:
-      hipLaunchKernelGGL(k_ray_messages<kMsgF32>, grid, block, 0, s, a.rays, a.nrays, a.inc, (float*)a.X, a.sig,
-                         a.bia, a.lookup_n);
+    case kMsgF16: MRF_RAY(kMsgF16, kMsgF16); break;
+    case kMsgBF16: MRF_RAY(kMsgBF16, kMsgBF16); break;
+    case kMsgMixedF16: MRF_RAY(kMsgF32, kMsgF16); break;
+    case kMsgMixedBF16: MRF_RAY(kMsgF32, kMsgBF16); break;
+    default: MRF_RAY(kMsgF32, kMsgF32);
   }
+#undef MRF_RAY
 }
 
 void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,
@@ -489,23 +536,26 @@ void launch_epilogue(const CamPass& a, hipStream_t s) {
   if (a.npairs == 0) return;
   const uint32_t ntiles = (a.nlive + kTile - 1) / kTile;
   dim3 grid(grid_for(ntiles, 4)), block(256);
-  switch (a.msg_type) {
+  float out_scale = 1.0f;
+  switch (msg_out_type(a.msg_type)) {
     case kMsgF16:
-      hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const __half*)a.X,
+      hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const __half*)a.Xo,
                          a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
       hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.pair_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+      out_scale = Msg<kMsgF16>::kOut;
       break;
     case kMsgBF16:
-      hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const uint16_t*)a.X,
+      hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const uint16_t*)a.Xo,
                          a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
       hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.pair_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+      out_scale = Msg<kMsgBF16>::kOut;
       break;
     default:
-      launch_segment_sum_f32(a.heads, a.tile_h0, a.nlive, (const float*)a.X, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v,
+      launch_segment_sum_f32(a.heads, a.tile_h0, a.nlive, (const float*)a.Xo, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v,
                              a.mid, a.pair_off, s);
   }
   hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.npairs,
-                     a.S, a.B, a.M, a.upd);
+                     a.S, out_scale, a.B, a.M, a.upd);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

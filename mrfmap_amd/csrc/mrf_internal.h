@@ -25,9 +25,11 @@ constexpr int kBrickVox = 512;
 constexpr uint32_t kGapVox = 0xFFFFFFFFu;  // incidence record that only advances z
 constexpr uint32_t kTile = 512;           // positions per wavefront in the segment kernels (8 x 64)
 
-// Per-ray record of one camera's traversal list.
+constexpr uint32_t kSlice = 64;           // rays per slice (one 8x8 pixel tile, one per lane)
+
+// Per-ray record of one camera's traversal (one per lane of a slice).
 struct RayRec {
-  uint32_t off;        // first incidence record
+  uint32_t off;        // first incidence record (record k at off + k * kSlice)
   uint32_t n;          // number of incidence records (voxels + gap records)
   float z0;            // planar depth at the entry of the first record (m)
   float inv_norm;      // metres of planar depth per voxel of chord length

@@ -5,7 +5,14 @@
 
 namespace mrf {
 
-enum MsgType { kMsgF32 = 0, kMsgF16 = 1, kMsgBF16 = 2 };
+// Storage of the per-incidence messages: voxel->ray ln n (input side) and
+// ray->voxel p*l (output side).  The mixed types keep ln n in FP32.
+enum MsgType { kMsgF32 = 0, kMsgF16 = 1, kMsgBF16 = 2, kMsgMixedF16 = 3, kMsgMixedBF16 = 4 };
+inline int msg_in_type(int t) { return t == kMsgF16 ? kMsgF16 : (t == kMsgBF16 ? kMsgBF16 : kMsgF32); }
+inline int msg_out_type(int t) {
+  return (t == kMsgF16 || t == kMsgMixedF16) ? kMsgF16 : ((t == kMsgBF16 || t == kMsgMixedBF16) ? kMsgBF16 : kMsgF32);
+}
+inline size_t msg_bytes(int type) { return type == kMsgF32 ? 4 : 2; }
 
 // traverse.hip
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
@@ -14,9 +21,14 @@ void launch_mark_endpoints(const CamGeom& c, const GridGeom& g, const float* dep
                            hipStream_t s);
 void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
                   const float* bia, const int32_t* slot, uint32_t* counts, uint32_t* has, hipStream_t s);
-void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                 const float* bia, const int32_t* slot, const uint32_t* offs, const uint32_t* ray_idx,
-                 uint2* inc, RayRec* rays, hipStream_t s);
+// slice (8x8 pixel tile) lengths: max records over the tile's 64 rays
+void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_len, uint32_t* slice_rec,
+                      hipStream_t s);
+// sliced-ELL emission: record k of lane l of slice t at slice_off[t] + k*64 + l,
+// padded to the slice length with {kGapVox, 0}; lanes[t*64+l] describes the ray
+void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
+                 const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
+                 RayRec* lanes, hipStream_t s);
 
 // scan.hip: exclusive prefix sum of n uint32 values; total is written to
 // *d_total (device, uint64).  tmp must hold scan_tmp_words(n) uint64 words.
@@ -52,9 +64,10 @@ void launch_pack_heads(const SegmentBuild& b, uint32_t nlive, uint64_t* bits, ui
 
 // bp.hip: one camera pass = expand -> ray messages -> segmented epilogue.
 struct CamPass {
-  const RayRec* rays;
-  uint32_t nrays;
-  const uint2* inc;        // {sorted position | kGapVox, chord length}
+  const RayRec* lanes;     // one per lane of every slice
+  const uint32_t* slice_len;
+  uint32_t nslices;
+  const uint2* inc;        // sliced records {sorted position | kGapVox, chord length}
   const uint32_t* pair_vox;
   const uint32_t* pair_off;  // npairs + 1
   const float* pair_c;
@@ -68,7 +81,8 @@ struct CamPass {
   float *pf_v, *pl_v, *mid;
   float* B;
   float* M;                // this camera's messages M_c (FP32, pool sized)
-  void* X;                 // segment array: ln n in, p*l out (MsgType)
+  void* X;                 // segment array of voxel->ray messages ln n (input side type)
+  void* Xo;                // segment array of ray->voxel messages p*l (output side type; may alias X)
   uint8_t* upd;
   const float* sig;        // sensor-model lookup tables
   const float* bia;

@@ -110,6 +110,8 @@ struct WalkOut {
   bool ran_out;
 };
 
+// Records of one ray are written out[0], out[kSlice], out[2 kSlice], ...
+// (record k of the 64 rays of a pixel tile is contiguous: sliced ELL).
 template <int MODE>
 __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, const float* sig,
                         const float* bia, const int32_t* __restrict__ slot,
@@ -191,7 +193,7 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
         if (any && skipped) {
           if (MODE == kEmit) {
             float gl = (float)((zc_d - last_exit) * rs.norm);
-            out[cnt] = make_uint2(kGapVox, __float_as_uint(gl));
+            out[(size_t)cnt * kSlice] = make_uint2(kGapVox, __float_as_uint(gl));
           }
           ++cnt;
           skipped = false;
@@ -204,7 +206,7 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
           const uint32_t local = (uint32_t)((((ix[0] & 7) << 3) | (ix[1] & 7)) << 3 | (ix[2] & 7));
           const uint32_t vox = (uint32_t)bslot * kBrickVox + local;
           float len = (float)((zn_d - zc_d) * rs.norm);
-          out[cnt] = make_uint2(vox, __float_as_uint(len));
+          out[(size_t)cnt * kSlice] = make_uint2(vox, __float_as_uint(len));
         }
         ++cnt;
         last_exit = zn_d;
@@ -326,26 +328,47 @@ __global__ void k_count(CamGeom c, GridGeom g, const float* __restrict__ depth,
   has[pix] = n > 0 ? 1u : 0u;
 }
 
-__global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth,
-                       const float* __restrict__ sig, const float* __restrict__ bia,
-                       const int32_t* __restrict__ slot, const uint32_t* __restrict__ offs,
-                       const uint32_t* __restrict__ ray_idx, uint2* __restrict__ inc,
-                       RayRec* __restrict__ rays) {
+// Records per slice (max over the tile's 64 rays) and the slice's padded size.
+__global__ void k_slice_len(const uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t* __restrict__ slice_len,
+                            uint32_t* __restrict__ slice_rec) {
+  const uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (t >= ntiles) return;
+  uint32_t m = counts[t * kSlice + (threadIdx.x & 63)];
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if ((threadIdx.x & 63) == 0) {
+    slice_len[t] = m;
+    slice_rec[t] = m * kSlice;
+  }
+}
+
+__global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, const float* __restrict__ sig,
+                       const float* __restrict__ bia, const int32_t* __restrict__ slot,
+                       const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ slice_len,
+                       uint2* __restrict__ sell, RayRec* __restrict__ lanes) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= tiled_pixels(c)) return;
+  const uint32_t tile = (uint32_t)pix >> 6, lane = (uint32_t)pix & 63u;
+  uint2* out = sell + slice_off[tile] + lane;
   PixelCtx p = load_pixel(c, g, depth, pix);
-  if (!p.ok) return;
-  const uint32_t off = offs[pix];
-  WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, inc + off);
-  if (wo.n == 0) return;
   RayRec r;
-  r.off = off;
-  r.n = wo.n;
-  r.z0 = wo.z0;
-  r.inv_norm = (float)(1.0 / p.rs.norm);
-  r.depth = p.d;
-  r.cell_flags = p.cell | (wo.ran_out ? kRanOut : 0u);
-  rays[ray_idx[pix]] = r;
+  r.off = slice_off[tile] + lane;
+  r.n = 0;
+  r.z0 = 0.0f;
+  r.inv_norm = 0.0f;
+  r.depth = 0.0f;
+  r.cell_flags = 0;
+  if (p.ok) {
+    WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, out);
+    r.n = wo.n;
+    r.z0 = wo.z0;
+    r.inv_norm = (float)(1.0 / p.rs.norm);
+    r.depth = p.d;
+    r.cell_flags = p.cell | (wo.ran_out ? kRanOut : 0u);
+  }
+  // pad the lane to the slice length with no-op records
+  const uint32_t len = slice_len[tile];
+  for (uint32_t k = r.n; k < len; ++k) out[(size_t)k * kSlice] = make_uint2(kGapVox, 0u);
+  lanes[pix] = r;
 }
 
 }  // namespace
@@ -371,12 +394,18 @@ void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const
                      has);
 }
 
-void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                 const float* bia, const int32_t* slot, const uint32_t* offs, const uint32_t* ray_idx,
-                 uint2* inc, RayRec* rays, hipStream_t s) {
+void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_len, uint32_t* slice_rec,
+                      hipStream_t s) {
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_slice_len, dim3((ntiles + 3) / 4), dim3(256), 0, s, counts, ntiles, slice_len, slice_rec);
+}
+
+void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
+                 const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
+                 RayRec* lanes, hipStream_t s) {
   const int n = tiled_pixels(c);
-  hipLaunchKernelGGL(k_emit, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, offs,
-                     ray_idx, inc, rays);
+  hipLaunchKernelGGL(k_emit, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, slice_off,
+                     slice_len, sell, lanes);
 }
 
 }  // namespace mrf
