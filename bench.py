@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="real")
-    ap.add_argument("--msg", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--msg", default="f32", choices=["f32", "f16", "bf16", "mixed-f16", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged run (profiling)")
@@ -73,8 +73,8 @@ def main():
         sys.exit(2)
     from mrfmap_amd import GVDBInference, _lib, synth
 
-    msg = {"f32": _lib.MSG_F32, "f16": _lib.MSG_F16, "bf16": _lib.MSG_BF16}[a.msg]
-    sbytes = 4 if msg == _lib.MSG_F32 else 2
+    msg = {"f32": _lib.MSG_F32, "f16": _lib.MSG_F16, "bf16": _lib.MSG_BF16, "mixed-f16": _lib.MSG_MIXED_F16,
+           "mixed-bf16": _lib.MSG_MIXED_BF16}[a.msg]
     t0 = time.perf_counter()
     p, frames, _ = synth.make_workload(a.workload, msg_dtype=msg)
     t_gen = time.perf_counter() - t0
@@ -109,7 +109,11 @@ def main():
     value = n_upd * a.steps / dt
     launches = a.steps * ncam
     avg_launch_ms = st["t_ray_kernel_ms"] / launches
-    bytes_per_launch = n_upd / ncam * (2 * sbytes + 4 + 8)
+    # ray kernel: 8 B record + voxel->ray message + ray->voxel message per update
+    in_b = 4 if msg in (_lib.MSG_F32, _lib.MSG_MIXED_F16, _lib.MSG_MIXED_BF16) else 2
+    out_b = 4 if msg == _lib.MSG_F32 else 2
+    alg_b = 8 + in_b + out_b
+    bytes_per_launch = n_upd / ncam * alg_b
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     print(f"[bench] {a.steps} steps in {dt * 1e3:.1f} ms; ray kernel {st['t_ray_kernel_ms']:.1f} ms, "
           f"epilogue {st['t_epilogue_ms']:.1f} ms", file=sys.stderr, flush=True)
@@ -139,6 +143,8 @@ def main():
             "pairs_per_step": st["pairs_per_iter"],
             "rays": st["rays"],
             "list_records": st["list_records"],
+            "ray_schedule_lanes_per_ray_x_loads": list(_lib.RAY_SCHEDULES[st["ray_sched"]]),
+            "autotune_ms": st["t_autotune_ms"],
         },
         "time_to_converged_ms": conv["t_total_ms"],
         "iters_to_converge": conv["iters"],
@@ -153,7 +159,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "avg_launch_ms": avg_launch_ms,
-            "alg_bytes_per_update": 2 * sbytes + 4 + 8,
+            "alg_bytes_per_update": alg_b,
         },
         "cpu_baseline": cpu,
     }

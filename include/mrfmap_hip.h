@@ -94,6 +94,8 @@ typedef struct mrf_stats {
   double t_ray_kernel_ms;      /* summed device time of the ray-message kernel */
   double t_epilogue_ms;        /* summed device time of the epilogue kernel */
   double t_total_ms;           /* wall time of the whole call */
+  int ray_sched;               /* ray-kernel schedule in use (lanes per ray x loads in flight) */
+  double t_autotune_ms;        /* time spent choosing it (first call of a context) */
 } mrf_stats;
 
 /* gvdb_params_t() defaults (GVDBParams.h:32-55). */

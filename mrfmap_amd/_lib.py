@@ -16,6 +16,8 @@ ERRORS = {-1: "MRF_E_INVALID", -2: "MRF_E_HIP", -3: "MRF_E_NOMEM", -4: "MRF_E_SH
 ROW_MAJOR, COL_MAJOR = 0, 1
 ACT_ENDPOINT, ACT_FULL_RAY = 0, 1
 MSG_F32, MSG_F16, MSG_BF16, MSG_MIXED_F16, MSG_MIXED_BF16 = 0, 1, 2, 3, 4
+# ray-kernel schedules (lanes per ray, steps of loads in flight)
+RAY_SCHEDULES = {0: (1, 4), 1: (1, 8), 2: (2, 4), 3: (2, 8), 4: (4, 2), 5: (4, 4)}
 
 # every symbol include/mrfmap_hip.h declares
 EXPORTS = [
@@ -44,7 +46,7 @@ class MrfStats(ctypes.Structure):
         ("updates_per_iter", ctypes.c_int64), ("pairs_per_iter", ctypes.c_int64), ("rays", ctypes.c_int64),
         ("list_records", ctypes.c_int64), ("t_geometry_ms", ctypes.c_double), ("t_iters_ms", ctypes.c_double),
         ("t_ray_kernel_ms", ctypes.c_double), ("t_epilogue_ms", ctypes.c_double),
-        ("t_total_ms", ctypes.c_double),
+        ("t_total_ms", ctypes.c_double), ("ray_sched", ctypes.c_int), ("t_autotune_ms", ctypes.c_double),
     ]
 
     def as_dict(self):

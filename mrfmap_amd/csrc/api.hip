@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -101,6 +102,9 @@ struct mrf_ctx {
 
   std::vector<Camera> cams;
   bool geom_dirty = true;
+  int ray_sched = -1;      // chosen RaySched, -1 until autotuned
+  bool sched_forced = false;
+  double autotune_ms = 0.0;
   std::string err;
 };
 
@@ -509,6 +513,82 @@ int add_camera(mrf_ctx* c, const float pose[16], const float K[9], const float* 
   return MRF_OK;
 }
 
+CamPass cam_pass(mrf_ctx* c, size_t ci) {
+  Camera& cam = c->cams[ci];
+  CamPass a;
+  a.lanes = cam.d_lanes;
+  a.slice_len = cam.d_slice_len;
+  a.nslices = cam.nslices;
+  a.inc = cam.d_inc;
+  a.pair_vox = cam.d_pair_vox;
+  a.pair_off = cam.d_pair_off;
+  a.pair_c = cam.d_pair_c;
+  a.npairs = cam.npairs;
+  a.nlive = (uint32_t)cam.nupd;
+  a.heads = cam.d_heads;
+  a.tile_h0 = cam.d_tile_h0;
+  a.apair = c->d_apair;
+  a.S = c->d_Spair;
+  a.pf_q = c->d_pf_q;
+  a.pl_q = c->d_pl_q;
+  a.pf_v = c->d_pf_v;
+  a.pl_v = c->d_pl_v;
+  a.mid = c->d_mid;
+  a.B = c->d_B;
+  a.M = (float*)((char*)c->d_M + ci * c->M_stride * sizeof(float));
+  a.X = c->d_X;
+  a.Xo = msg_in_type(c->p.msg_dtype) == msg_out_type(c->p.msg_dtype) ? c->d_X : c->d_Xo;
+  a.upd = c->d_upd;
+  a.sig = c->d_sig;
+  a.bia = c->d_bia;
+  a.lookup_n = c->p.lookup_n;
+  a.msg_type = c->p.msg_dtype;
+  a.ray_sched = c->ray_sched < 0 ? kSchedL2U8 : c->ray_sched;
+  return a;
+}
+
+// Pick the ray-kernel schedule (lanes per ray x loads in flight) by timing
+// each candidate twice on the camera with the most incidences; the message
+// segment is re-expanded before every trial because the FP32 pass writes its
+// outputs over its inputs.  Only messages are touched, not beliefs.
+int autotune(mrf_ctx* c) {
+  if (c->cams.empty()) return MRF_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  size_t best_cam = 0;
+  for (size_t i = 1; i < c->cams.size(); ++i)
+    if (c->cams[i].nupd > c->cams[best_cam].nupd) best_cam = i;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  float best = 1e30f;
+  int pick = kSchedL2U8;
+  for (int sched = 0; sched < kNumSched; ++sched) {
+    CamPass a = cam_pass(c, best_cam);
+    a.ray_sched = sched;
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      launch_expand(a, c->stream);
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      launch_ray_messages(a, c->stream);
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+      HIPCHK(c, hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+      tmin = std::min(tmin, ms);
+    }
+    if (tmin < best) {
+      best = tmin;
+      pick = sched;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(c, hipGetLastError());
+  c->ray_sched = pick;
+  c->autotune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MRF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -581,6 +661,13 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
     return MRF_E_HIP;
   }
   c->h_slot.assign((size_t)c->nbricks, -1);
+  if (const char* env = getenv("MRF_RAY_SCHED")) {
+    const int v = atoi(env);
+    if (v >= 0 && v < kNumSched) {
+      c->ray_sched = v;
+      c->sched_forced = true;
+    }
+  }
   int rc;
   if ((rc = dalloc(c, &c->d_slot, (size_t)c->nbricks)) || (rc = dalloc(c, &c->d_bflags, (size_t)c->nbricks)) ||
       (rc = dalloc(c, &c->d_total, 4))) {
@@ -690,6 +777,7 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   if (rc) return rc;
   auto t1 = std::chrono::steady_clock::now();
   if (dirty) s.t_geometry_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (c->ray_sched < 0 && (rc = autotune(c))) return rc;
   const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
   const size_t esz = sizeof(float);  // camera messages M_c are FP32 for every message type
   // reset M_c of the selected cameras (GVDBInference.cpp:128-134)
@@ -711,35 +799,7 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   for (; it < max_iters; ++it) {
     HIPCHK(c, hipEventRecord(ev[4 * ncam], c->stream));
     for (size_t ci = 0; ci < ncam; ++ci) {
-      Camera& cam = c->cams[ci];
-      CamPass a;
-      a.lanes = cam.d_lanes;
-      a.slice_len = cam.d_slice_len;
-      a.nslices = cam.nslices;
-      a.inc = cam.d_inc;
-      a.pair_vox = cam.d_pair_vox;
-      a.pair_off = cam.d_pair_off;
-      a.pair_c = cam.d_pair_c;
-      a.npairs = cam.npairs;
-      a.nlive = (uint32_t)cam.nupd;
-      a.heads = cam.d_heads;
-      a.tile_h0 = cam.d_tile_h0;
-      a.apair = c->d_apair;
-      a.S = c->d_Spair;
-      a.pf_q = c->d_pf_q;
-      a.pl_q = c->d_pl_q;
-      a.pf_v = c->d_pf_v;
-      a.pl_v = c->d_pl_v;
-      a.mid = c->d_mid;
-      a.B = c->d_B;
-      a.M = (float*)((char*)c->d_M + ci * c->M_stride * esz);
-      a.X = c->d_X;
-      a.Xo = msg_in_type(c->p.msg_dtype) == msg_out_type(c->p.msg_dtype) ? c->d_X : c->d_Xo;
-      a.upd = c->d_upd;
-      a.sig = c->d_sig;
-      a.bia = c->d_bia;
-      a.lookup_n = c->p.lookup_n;
-      a.msg_type = c->p.msg_dtype;
+      const CamPass a = cam_pass(c, ci);
       launch_expand(a, c->stream);
       HIPCHK(c, hipEventRecord(ev[4 * ci + 0], c->stream));
       launch_ray_messages(a, c->stream);
@@ -778,6 +838,8 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   s.t_epilogue_ms = tepi;
   s.t_iters_ms = titer;
   s.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  s.ray_sched = c->ray_sched;
+  s.t_autotune_ms = c->autotune_ms;
   if (st) *st = s;
   return MRF_OK;
 }

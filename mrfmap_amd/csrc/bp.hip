@@ -1,19 +1,16 @@
 // Loopy-BP message kernels of the MRFMap inference path.
 //
-//  * k_ray_messages: one wavefront per ray.  Replaces the pre pass
-//    (rayStartComputingMessagesFromBrickPre, src/cuda/gvdbkernels.cu:689-819),
-//    the infinity term (gvdbkernels.cu:1196-1208) and the post pass
+//  * k_pair_msg + k_expand: voxel->ray message ln n of every (camera, voxel)
+//    pair (gvdbkernels.cu:764-773), copied into each incidence slot of the
+//    pair's segment of the camera's voxel-sorted incidence order.
+//  * k_ray_messages: L lanes per ray over sliced-ELL pixel tiles.  Replaces
+//    the pre pass (rayStartComputingMessagesFromBrickPre,
+//    src/cuda/gvdbkernels.cu:689-819), the infinity term
+//    (gvdbkernels.cu:1196-1208) and the post pass
 //    (rayStartComputingMessagesFromBrickPost, gvdbkernels.cu:824-1038) with a
-//    single launch: 64 incidences per step, DPP prefix sums for
-//    Lambda = sum l ln n, for the travelled length and for the exclusive sum
-//    of tau, a wave reduction for T, and a scatter of p*l into the
-//    camera's ray->voxel accumulator.
-//  * k_cum_len / pair build: the per-camera summed chord length C(v)
-//    (rayComputeCumLens, gvdbkernels.cu:519-571) is geometry only, so it is
-//    computed once per traversal with exact 2^-32 fixed-point atomics and
-//    compacted into an ascending (voxel, C) pair list.
-//  * k_epilogue: per (camera, touched voxel) pair: camera message
-//    m = logit(clamp(S/C)), B += m - M_c, M_c = m, S = 0
+//    single launch and writes p*l into each incidence's own slot.
+//  * k_seg_sum / k_seg_span / k_seg_update: per pair, the fixed-order sum S of
+//    its slots, m = logit(clamp(S/C)), B += m - M_c, M_c = m
 //    (compute_gvdb_message_to_factors, gvdbkernels.cu:396-496, live 457-479).
 //  * residual / marginal / readback kernels (gvdbkernels.cu:469-476, 360-373).
 #include <hip/hip_fp16.h>
@@ -84,7 +81,7 @@ struct Poly {
   float c0, c1, c2;
   int n;
   __device__ __forceinline__ float operator()(float z) const {
-    return n == 3 ? (z * z * c2 + z * c1 + c0) : (z * c1 + c0);
+    return n == 3 ? fmaf(z, fmaf(z, c2, c1), c0) : fmaf(z, c1, c0);
   }
 };
 
@@ -169,25 +166,72 @@ __global__ __launch_bounds__(256) void k_expand(const uint64_t* __restrict__ hea
   }
 }
 
-// One lane per ray, one wavefront per slice (the 64 rays of an 8x8 pixel
-// tile, records interleaved so that record k of all 64 rays is one coalesced
-// load and the 64 lanes touch neighbouring voxels together).
-//
-// Pass 1: Lambda_k = sum_{j<=k} l_j ln n_j (the reference's visibility
-// includes voxel k itself, gvdbkernels.cu:781-783), vpsi_k = exp(Lambda_k)
-// psi(s_k), tau_k = vpsi_k (1 - n_k), T = sum tau (+ V_n when the walk ran
-// out and d lies beyond, gvdbkernels.cu:1196-1208).  Until the first record
-// with vpsi > 0 every earlier tau is exactly 0, so P = 0, mu1 = 0 and the
-// message is exactly the clamp floor F_EPS: pass 1 writes those directly.
-// Pass 2 restarts from that record with the saved sums and writes
-// p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi, mu0 = P + R / n
-// (gvdbkernels.cu:936-951).  Long sums along the ray are Kahan-compensated.
+// ---------------------------------------------------------------- ray kernel
+// L lanes per ray (L = 1, 2, 4): lane j of a ray handles records j, j+L,
+// j+2L, ...; each step covers L consecutive records with a prefix sum over
+// the ray's L lanes (DPP quad permutes) and a carried, Kahan-compensated sum
+// from earlier steps.  The 64 rays of a slice (one 8x8 pixel tile) are
+// spread over L wavefronts; record k of all rays of a slice is contiguous.
+template <int CTRL>
+__device__ __forceinline__ float qperm(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int qperm_i(int x) {
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, false);
+}
+
+template <int L>
+struct Group {
+  // inclusive prefix over the L lanes of a ray (j = lane index within the ray)
+  __device__ __forceinline__ static float scan(float x, int j) {
+    if (L == 2) {
+      const float y = qperm<0xA0>(x);  // quad_perm [0,0,2,2]
+      return j == 1 ? x + y : x;
+    } else if (L == 4) {
+      float y = qperm<0x90>(x);        // quad_perm [0,0,1,2]
+      x = j >= 1 ? x + y : x;
+      y = qperm<0x40>(x);              // quad_perm [0,0,0,1]
+      return j >= 2 ? x + y : x;
+    }
+    return x;
+  }
+  // value of the ray's last lane
+  __device__ __forceinline__ static float last(float x) {
+    if (L == 2) return qperm<0xF5>(x);  // quad_perm [1,1,3,3]
+    if (L == 4) return qperm<0xFF>(x);  // quad_perm [3,3,3,3]
+    return x;
+  }
+  __device__ __forceinline__ static int min(int v) {
+    if (L >= 2) v = ::min(v, qperm_i<0xB1>(v));  // quad_perm [1,0,3,2]
+    if (L == 4) v = ::min(v, qperm_i<0x4E>(v));  // quad_perm [2,3,0,1]
+    return v;
+  }
+  __device__ __forceinline__ static float sum(float x) {
+    if (L >= 2) x = x + qperm<0xB1>(x);
+    if (L == 4) x = x + qperm<0x4E>(x);
+    return x;
+  }
+};
+
 struct RayCtx {
   Poly sig, bia;
   float d, z0, inv;
 };
 
-template <int MI, int MO>
+// Pass 1: Lambda_k = sum_{i<=k} l_i ln n_i (the reference's visibility
+// includes voxel k itself, gvdbkernels.cu:781-783), vpsi_k = exp(Lambda_k)
+// psi(s_k), tau_k = vpsi_k (1 - n_k), T = sum tau (+ V_n when the walk ran
+// out and d lies beyond, gvdbkernels.cu:1196-1208).  Until the first record
+// with vpsi > 0 every earlier tau is exactly 0, so P = 0, mu1 = 0 and the
+// message is exactly the clamp floor F_EPS: pass 1 writes those of the
+// steps before the one holding that record directly.
+// Pass 2 restarts at the step holding that record with the saved sums and
+// writes p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi, mu0 = P + R / n
+// (gvdbkernels.cu:936-951).  U steps of loads are issued before their
+// arithmetic.  The file is built without FP contraction, so pass 2
+// recomputes pass 1's terms bit for bit.
+template <int MI, int MO, int L, int U>
 __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__ lanes,
                                                       const uint32_t* __restrict__ slice_len, uint32_t nslices,
                                                       const uint2* __restrict__ inc,
@@ -195,15 +239,17 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
                                                       typename Msg<MO>::T* __restrict__ Xo,
                                                       const float* __restrict__ sigt, const float* __restrict__ biat,
                                                       int lookup_n) {
-  const int lane = threadIdx.x & 63;
+  constexpr uint32_t kRaysPerBlock = 256 / L;
   // XCD-aware order: consecutive blocks (neighbouring pixel tiles) share an XCD's L2
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const uint32_t t = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (t >= nslices) return;
+  const uint32_t ray = lb * kRaysPerBlock + threadIdx.x / L;  // global ray id = slice * 64 + r
+  const int j = (int)(threadIdx.x % L);
+  const uint32_t t = ray >> 6;
+  if (t >= nslices) return;  // uniform per wavefront: a wavefront never spans slices
   const uint32_t len = slice_len[t];
   if (len == 0) return;
-  const RayRec rec = lanes[t * kSlice + lane];
+  const RayRec rec = lanes[ray];
   const uint32_t cell = rec.cell_flags & 0x7FFFFFFFu;
   RayCtx rc;
   rc.sig.n = rc.bia.n = lookup_n;
@@ -217,65 +263,119 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
   rc.z0 = rec.z0;
   rc.inv = rec.inv_norm;
   const uint2* __restrict__ rl = inc + rec.off;  // record k at rl[k * kSlice]
+  const uint32_t nsteps = (len + L - 1) / L;
+
+  auto load_rec = [&](uint32_t step) -> uint2 {
+    const uint32_t k = step * L + j;
+    return k < len ? rl[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+  };
+  auto load_msg = [&](const uint2 e) -> float { return Msg<MI>::load(Xi, e.x != kGapVox ? e.x : 0u); };
 
   Kahan kL = {0.f, 0.f}, kS = {0.f, 0.f}, kT = {0.f, 0.f};
   Kahan L0 = kL, S0 = kS;
-  uint32_t i0 = len;  // first record with vpsi > 0
-  for (uint32_t k = 0; k < len; ++k) {
-    const uint2 e = rl[(size_t)k * kSlice];
+  uint32_t i0 = len;  // first record with vpsi > 0 (same in all lanes of a ray)
+  auto step1 = [&](const uint2 e, const float msg, const uint32_t step) {
+    const uint32_t k = step * L + j;
     const bool live = e.x != kGapVox;
-    const float lnn = live ? Msg<MI>::load(Xi, e.x) : 0.0f;
+    const float lnn = live ? msg : 0.0f;
     const float ln = __uint_as_float(e.y);
-    const Kahan Lb = kL, Sb = kS;
-    kL.add(ln * lnn);
-    kS.add(ln);
-    const float s = rc.z0 + kS.at(0.0f) * rc.inv;
-    const float vpsi = live ? __expf(kL.at(0.0f)) * depth_psi(s, rc.d, rc.sig, rc.bia) : 0.0f;
-    if (i0 == len) {
-      if (vpsi > 0.0f) {
-        i0 = k;
-        L0 = Lb;
-        S0 = Sb;
-      } else if (live) {
-        Msg<MO>::store(Xo, e.x, kFEps * ln * Msg<MO>::kOut);
-      }
+    const float a = Group<L>::scan(ln * lnn, j);
+    const float b = Group<L>::scan(ln, j);
+    const float s = rc.z0 + kS.at(b) * rc.inv;
+    const float ps = depth_psi(s, rc.d, rc.sig, rc.bia);
+    const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
+    const bool front = i0 == len;
+    const int firstj = Group<L>::min(vpsi > 0.0f ? j : L);
+    const bool found = front && firstj < L;
+    if (found) {
+      i0 = step * L + (uint32_t)firstj;
+      L0 = kL;
+      S0 = kS;
     }
-    if (i0 != len) kT.add(vpsi * (1.0f - __expf(lnn)));
+    const bool before = front && k < i0;  // record in front of the likelihood window
+    // only steps wholly in front are written here: pass 2 re-reads the step
+    // holding i0, whose inputs may be stored in place of its outputs
+    if (front && !found && live) Msg<MO>::store(Xo, e.x, kFEps * ln * Msg<MO>::kOut);
+    kT.add(before ? 0.0f : vpsi * (1.0f - __expf(lnn)));
+    kL.add(Group<L>::last(a));
+    kS.add(Group<L>::last(b));
+  };
+  uint32_t st = 0;
+  for (; st + U <= nsteps; st += U) {
+    uint2 e[U];
+    float x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) e[u] = load_rec(st + u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) step1(e[u], x[u], st + u);
   }
-  float T = kT.at(0.0f);
+  for (; st < nsteps; ++st) {
+    const uint2 e = load_rec(st);
+    step1(e, load_msg(e), st);
+  }
+  float T = Group<L>::sum(kT.at(0.0f));
   if (rec.cell_flags & kRanOut) {
     const float sn = rc.z0 + kS.at(0.0f) * rc.inv;
     if (rc.d > sn + rc.bia(sn)) T += __expf(kL.at(0.0f));
   }
 
-  uint32_t kmin = i0;
+  // pass 2 from the step holding i0 (all lanes execute every step; writes
+  // and state updates are predicated so the group permutes stay convergent).
+  // Records of that step in front of i0 have vpsi = 0 and P = 0, so their p
+  // clamps to exactly F_EPS, the value pass 1 gives the records before them.
+  // A ray without a record with vpsi > 0 was written wholly by pass 1.
+  const uint32_t s0 = i0 == len ? nsteps : i0 / L;
+  uint32_t smin = s0;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, o));
+  for (int o = 32; o > 0; o >>= 1) smin = ::min(smin, (uint32_t)__shfl_xor((int)smin, o));
   kL = L0;
   kS = S0;
   Kahan kP = {0.f, 0.f};
-  for (uint32_t k = kmin; k < len; ++k) {
-    if (k < i0) continue;
-    const uint2 e = rl[(size_t)k * kSlice];
+  auto step2 = [&](const uint2 e, const float msg, const uint32_t step) {
+    const bool on = step >= s0;
     const bool live = e.x != kGapVox;
-    const float lnn = live ? Msg<MI>::load(Xi, e.x) : 0.0f;
+    const float lnn = live ? msg : 0.0f;
     const float ln = __uint_as_float(e.y);
-    kL.add(ln * lnn);
-    kS.add(ln);
-    const float s = rc.z0 + kS.at(0.0f) * rc.inv;
-    const float vpsi = live ? __expf(kL.at(0.0f)) * depth_psi(s, rc.d, rc.sig, rc.bia) : 0.0f;
+    const float a = Group<L>::scan(ln * lnn, j);
+    const float b = Group<L>::scan(ln, j);
+    const float s = rc.z0 + kS.at(b) * rc.inv;
+    const float ps = depth_psi(s, rc.d, rc.sig, rc.bia);
+    const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
     const float nn = __expf(lnn);
     const float tau = vpsi * (1.0f - nn);
-    const float P = kP.at(0.0f);
+    const float pin = Group<L>::scan(tau, j);
+    const float P = kP.at(pin - tau);
     const float R = (T - P) - tau;
-    if (live) {
+    if (on && live) {
       const float mu1 = P + vpsi;
       const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
       float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
       p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
       Msg<MO>::store(Xo, e.x, p * ln * Msg<MO>::kOut);
     }
-    kP.add(tau);
+    const float ta = Group<L>::last(a), tb = Group<L>::last(b), tp = Group<L>::last(pin);
+    if (on) {
+      kL.add(ta);
+      kS.add(tb);
+      kP.add(tp);
+    }
+  };
+  st = smin;
+  for (; st + U <= nsteps; st += U) {
+    uint2 e[U];
+    float x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) e[u] = load_rec(st + u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) step2(e[u], x[u], st + u);
+  }
+  for (; st < nsteps; ++st) {
+    const uint2 e = load_rec(st);
+    step2(e, load_msg(e), st);
   }
 }
 
@@ -503,23 +603,35 @@ void launch_expand(const CamPass& a, hipStream_t s) {
   }
 }
 
+template <int MI, int MO, int L, int U>
+void launch_ray_t(const CamPass& a, hipStream_t s) {
+  const uint32_t blocks = (a.nslices * kSlice + (256 / L) - 1) / (256 / L);
+  const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
+  hipLaunchKernelGGL((k_ray_messages<MI, MO, L, U>), dim3(nb), dim3(256), 0, s, a.lanes, a.slice_len, a.nslices,
+                     a.inc, (const typename Msg<MI>::T*)a.X, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
+}
+
+template <int MI, int MO>
+void launch_ray_sched(const CamPass& a, hipStream_t s) {
+  switch (a.ray_sched) {
+    case kSchedL1U4: launch_ray_t<MI, MO, 1, 4>(a, s); break;
+    case kSchedL1U8: launch_ray_t<MI, MO, 1, 8>(a, s); break;
+    case kSchedL2U4: launch_ray_t<MI, MO, 2, 4>(a, s); break;
+    case kSchedL4U2: launch_ray_t<MI, MO, 4, 2>(a, s); break;
+    case kSchedL4U4: launch_ray_t<MI, MO, 4, 4>(a, s); break;
+    default: launch_ray_t<MI, MO, 2, 8>(a, s);
+  }
+}
+
 void launch_ray_messages(const CamPass& a, hipStream_t s) {
   if (a.nslices == 0) return;
-  const int waves = 4;
-  // grid padded to a multiple of the 8 XCDs for the block remapping
-  const unsigned nb = ((a.nslices + waves - 1) / waves + 7) / 8 * 8;
-  dim3 grid(nb), block(64 * waves);
-#define MRF_RAY(MI, MO)                                                                                        \
-  hipLaunchKernelGGL((k_ray_messages<MI, MO>), grid, block, 0, s, a.lanes, a.slice_len, a.nslices, a.inc,       \
-                     (const typename Msg<MI>::T*)a.X, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n)
   switch (a.msg_type) {
-    case kMsgF16: MRF_RAY(kMsgF16, kMsgF16); break;
-    case kMsgBF16: MRF_RAY(kMsgBF16, kMsgBF16); break;
-    case kMsgMixedF16: MRF_RAY(kMsgF32, kMsgF16); break;
-    case kMsgMixedBF16: MRF_RAY(kMsgF32, kMsgBF16); break;
-    default: MRF_RAY(kMsgF32, kMsgF32);
+    case kMsgF16: launch_ray_sched<kMsgF16, kMsgF16>(a, s); break;
+    case kMsgBF16: launch_ray_sched<kMsgBF16, kMsgBF16>(a, s); break;
+    case kMsgMixedF16: launch_ray_sched<kMsgF32, kMsgF16>(a, s); break;
+    case kMsgMixedBF16: launch_ray_sched<kMsgF32, kMsgBF16>(a, s); break;
+    default: launch_ray_sched<kMsgF32, kMsgF32>(a, s);
   }
-#undef MRF_RAY
 }
 
 void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,

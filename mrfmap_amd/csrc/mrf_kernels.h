@@ -14,6 +14,11 @@ inline int msg_out_type(int t) {
 }
 inline size_t msg_bytes(int type) { return type == kMsgF32 ? 4 : 2; }
 
+// Schedules of the ray-message kernel: lanes per ray (L) x steps of loads in
+// flight (U).  Picked per workload by the autotuner in api.hip.
+enum RaySched { kSchedL1U4 = 0, kSchedL1U8 = 1, kSchedL2U4 = 2, kSchedL2U8 = 3, kSchedL4U2 = 4, kSchedL4U4 = 5,
+                kNumSched = 6 };
+
 // traverse.hip
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
                       const float* bia, uint8_t* bflags, hipStream_t s);
@@ -88,6 +93,7 @@ struct CamPass {
   const float* bia;
   int lookup_n;
   int msg_type;
+  int ray_sched;           // RaySched
 };
 void launch_expand(const CamPass& a, hipStream_t s);
 // deterministic segment sums of an FP32 position array into S[pair]

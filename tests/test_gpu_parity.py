@@ -164,3 +164,19 @@ def test_bp_mixed_precision_small(msg):
             assert err.max() <= bound["it1"]
         else:
             assert err.max() <= bound["max5"] and rms <= bound["rms5"]
+
+
+@pytest.mark.parametrize("sched", sorted(_lib.RAY_SCHEDULES))
+def test_every_ray_schedule_matches_oracle(sched, monkeypatch):
+    """Each ray-kernel schedule (lanes per ray x loads in flight) the
+    autotuner may pick gives marginals within 1e-3 of the FP64 oracle."""
+    monkeypatch.setenv("MRF_RAY_SCHED", str(sched))
+    p, frames, _ = synth.make_workload("small")
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    st = g.perform_inference(max_iters=5)
+    o.perform_inference(5)
+    assert st["ray_sched"] == sched
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"schedule {sched} {_lib.RAY_SCHEDULES[sched]}: max-abs {err.max():.3e}")
+    assert err.max() <= MARG_TOL
