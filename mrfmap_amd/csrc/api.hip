@@ -40,6 +40,7 @@ struct Camera {
   uint32_t nslices = 0;
   uint32_t nrays = 0;              // rays with at least one record
   uint2* d_inc = nullptr;          // sliced records (padded)
+  uint32_t* d_inc_pair = nullptr;  // pair index of each live sliced record (0 for gap records)
   uint64_t nrec = 0, nupd = 0;     // padded records, live incidences
   uint64_t nlist = 0;              // records without the slice padding
   uint32_t* d_pair_vox = nullptr;
@@ -76,9 +77,7 @@ struct mrf_ctx {
   void* d_M = nullptr;
   size_t M_bytes = 0;
   size_t M_stride = 0;    // elements per camera
-  void* d_X = nullptr;    // segment array (voxel->ray side) of the camera being processed
-  size_t X_bytes = 0;
-  void* d_Xo = nullptr;   // ray->voxel side when its storage type differs
+  void* d_Xo = nullptr;   // segment array of ray->voxel messages of the camera being processed
   size_t Xo_bytes = 0;
   // per-pair / per-tile scratch of a camera pass
   float *d_apair = nullptr, *d_Spair = nullptr;
@@ -266,8 +265,6 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   const size_t tb = segment_sort_temp_bytes((uint32_t)n, bits);
   if (tb > c->sort_tmp_bytes) {
     if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
-  dfree(&c->d_apair); dfree(&c->d_Spair);
-  dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
     c->d_sort_tmp = nullptr;
     c->sort_tmp_bytes = 0;
     if (hipMalloc(&c->d_sort_tmp, tb) != hipSuccess) {
@@ -283,6 +280,7 @@ void free_camera_lists(Camera& cam) {
   dfree(&cam.d_lanes);
   dfree(&cam.d_slice_len);
   dfree(&cam.d_inc);
+  dfree(&cam.d_inc_pair);
   dfree(&cam.d_pair_vox);
   dfree(&cam.d_pair_off);
   dfree(&cam.d_pair_c);
@@ -341,7 +339,10 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.nlist = nlist;
   cam.nrays = (uint32_t)nrays;
   cam.nslices = ntiles;
-  if ((rc = dalloc(c, &cam.d_inc, nrec)) || (rc = dalloc(c, &cam.d_lanes, npix))) return rc;
+  if ((rc = dalloc(c, &cam.d_inc, nrec)) || (rc = dalloc(c, &cam.d_inc_pair, nrec)) ||
+      (rc = dalloc(c, &cam.d_lanes, npix)))
+    return rc;
+  launch_zero_words(cam.d_inc_pair, nrec * sizeof(uint32_t), c->stream);
   launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, cam.d_slice_len, cam.d_inc,
               cam.d_lanes, c->stream);
   // live incidences (records that are not gap records)
@@ -385,7 +386,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if (nrec) {
     launch_pack_heads(b, (uint32_t)live, cam.d_heads, cam.d_tile_h0, c->stream);
     finish_segments(b, cam.npairs, (uint32_t)live, cam.d_pair_off, cam.d_pair_vox, cam.d_pair_c, cam.d_heads,
-                    cam.d_tile_h0, c->stream);
+                    cam.d_tile_h0, cam.d_inc_pair, c->stream);
   } else {
     HIPCHK(c, hipMemsetAsync(cam.d_pair_off, 0, 4, c->stream));
   }
@@ -460,18 +461,7 @@ int prepare(mrf_ctx* c) {
       return rc;
     c->cap_tiles = tneed;
   }
-  const int ti = msg_in_type(c->p.msg_dtype), to = msg_out_type(c->p.msg_dtype);
-  const size_t xin = xneed * msg_bytes(ti), xout = ti == to ? 0 : xneed * msg_bytes(to);
-  if (xin > c->X_bytes) {
-    if (c->d_X) (void)hipFree(c->d_X);
-    c->d_X = nullptr;
-    c->X_bytes = 0;
-    if (hipMalloc(&c->d_X, xin) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(c, MRF_E_NOMEM, "segment array allocation failed");
-    }
-    c->X_bytes = xin;
-  }
+  const size_t xout = xneed * msg_bytes(msg_out_type(c->p.msg_dtype));
   if (xout > c->Xo_bytes) {
     if (c->d_Xo) (void)hipFree(c->d_Xo);
     c->d_Xo = nullptr;
@@ -520,6 +510,7 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.slice_len = cam.d_slice_len;
   a.nslices = cam.nslices;
   a.inc = cam.d_inc;
+  a.inc_pair = cam.d_inc_pair;
   a.pair_vox = cam.d_pair_vox;
   a.pair_off = cam.d_pair_off;
   a.pair_c = cam.d_pair_c;
@@ -536,8 +527,7 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.mid = c->d_mid;
   a.B = c->d_B;
   a.M = (float*)((char*)c->d_M + ci * c->M_stride * sizeof(float));
-  a.X = c->d_X;
-  a.Xo = msg_in_type(c->p.msg_dtype) == msg_out_type(c->p.msg_dtype) ? c->d_X : c->d_Xo;
+  a.Xo = c->d_Xo;
   a.upd = c->d_upd;
   a.sig = c->d_sig;
   a.bia = c->d_bia;
@@ -548,9 +538,8 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
 }
 
 // Pick the ray-kernel schedule (lanes per ray x loads in flight) by timing
-// each candidate twice on the camera with the most incidences; the message
-// segment is re-expanded before every trial because the FP32 pass writes its
-// outputs over its inputs.  Only messages are touched, not beliefs.
+// each candidate twice on the camera with the most incidences.  Only the
+// pass's message scratch is touched, not beliefs.
 int autotune(mrf_ctx* c) {
   if (c->cams.empty()) return MRF_OK;
   auto t0 = std::chrono::steady_clock::now();
@@ -567,7 +556,7 @@ int autotune(mrf_ctx* c) {
     a.ray_sched = sched;
     float tmin = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {
-      launch_expand(a, c->stream);
+      launch_pair_msgs(a, c->stream);
       HIPCHK(c, hipEventRecord(e0, c->stream));
       launch_ray_messages(a, c->stream);
       HIPCHK(c, hipEventRecord(e1, c->stream));
@@ -705,7 +694,6 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_sig); dfree(&c->d_bia); dfree(&c->d_slot); dfree(&c->d_bflags);
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
   if (c->d_M) (void)hipFree(c->d_M);
-  if (c->d_X) (void)hipFree(c->d_X);
   if (c->d_Xo) (void)hipFree(c->d_Xo);
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
   dfree(&c->d_apair); dfree(&c->d_Spair);
@@ -800,7 +788,7 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
     HIPCHK(c, hipEventRecord(ev[4 * ncam], c->stream));
     for (size_t ci = 0; ci < ncam; ++ci) {
       const CamPass a = cam_pass(c, ci);
-      launch_expand(a, c->stream);
+      launch_pair_msgs(a, c->stream);
       HIPCHK(c, hipEventRecord(ev[4 * ci + 0], c->stream));
       launch_ray_messages(a, c->stream);
       HIPCHK(c, hipEventRecord(ev[4 * ci + 1], c->stream));
@@ -909,7 +897,7 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
   if (cam.nrec) {
     uint2* tmp = nullptr;
     if ((rc = dalloc(c, &tmp, cam.nrec))) return rc;
-    launch_export_vox(cam.d_inc, (uint32_t)cam.nrec, cam.d_pair_off, cam.npairs, cam.d_pair_vox, tmp, c->stream);
+    launch_export_vox(cam.d_inc, cam.d_inc_pair, (uint32_t)cam.nrec, cam.d_pair_vox, tmp, c->stream);
     hipError_t e = hipMemcpyAsync(sell.data(), tmp, cam.nrec * sizeof(uint2), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(&tmp);

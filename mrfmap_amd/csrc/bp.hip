@@ -1,14 +1,15 @@
 // Loopy-BP message kernels of the MRFMap inference path.
 //
-//  * k_pair_msg + k_expand: voxel->ray message ln n of every (camera, voxel)
-//    pair (gvdbkernels.cu:764-773), copied into each incidence slot of the
-//    pair's segment of the camera's voxel-sorted incidence order.
-//  * k_ray_messages: L lanes per ray over sliced-ELL pixel tiles.  Replaces
+//  * k_pair_msg: voxel->ray message ln n of every (camera, voxel) pair
+//    (gvdbkernels.cu:764-773) into a compact per-pair array.
+//  * k_ray_messages: L lanes per ray over sliced-ELL pixel tiles; reads ln n
+//    of each record's pair, writes p*l into the record's slot of the
+//    camera's voxel-sorted segment array.  Replaces
 //    the pre pass (rayStartComputingMessagesFromBrickPre,
 //    src/cuda/gvdbkernels.cu:689-819), the infinity term
 //    (gvdbkernels.cu:1196-1208) and the post pass
 //    (rayStartComputingMessagesFromBrickPost, gvdbkernels.cu:824-1038) with a
-//    single launch and writes p*l into each incidence's own slot.
+//    single launch.
 //  * k_seg_sum / k_seg_span / k_seg_update: per pair, the fixed-order sum S of
 //    its slots, m = logit(clamp(S/C)), B += m - M_c, M_c = m
 //    (compute_gvdb_message_to_factors, gvdbkernels.cu:396-496, live 457-479).
@@ -125,45 +126,17 @@ __device__ __forceinline__ uint32_t pair_of(uint64_t word, uint32_t before, int 
 
 // Voxel -> ray message of every pair: ln n = ln clamp(1/(1+exp(B - M_c))),
 // camera c's own previous message excluded from the belief (gvdbkernels.cu:764-773).
+template <int MT>
 __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ pair_vox, uint32_t npairs,
                                                   const float* __restrict__ B, const float* __restrict__ M,
-                                                  float* __restrict__ apair) {
+                                                  typename Msg<MT>::T* __restrict__ apair) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npairs) return;
   const uint32_t v = pair_vox[q];
   const float qv = B[v] - M[v];
   float nn = 1.0f / (1.0f + __expf(qv));
   nn = fminf(fmaxf(nn, kFEps), 1.0f - kFEps);
-  apair[q] = __logf(nn);
-}
-
-// Copy each pair's message into every slot of its segment (one wavefront
-// per tile of kTile positions, coalesced writes).
-template <int MT>
-__global__ __launch_bounds__(256) void k_expand(const uint64_t* __restrict__ heads, const uint32_t* __restrict__ tile_h0,
-                                                uint32_t nlive, const float* __restrict__ apair,
-                                                typename Msg<MT>::T* __restrict__ X) {
-  constexpr int C = kTile / 64;
-  const int lane = threadIdx.x & 63;
-  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint32_t j0 = w * kTile;
-  if (j0 >= nlive) return;
-  const uint32_t j1 = min(j0 + kTile, nlive);
-  uint32_t before = tile_h0[w];
-  float v[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const uint32_t base = j0 + 64u * c;
-    const uint64_t word = base < j1 ? heads[base >> 6] : 0ull;
-    const uint32_t j = base + lane;
-    v[c] = j < j1 ? apair[pair_of(word, before, lane)] : 0.0f;
-    before += (uint32_t)__popcll(word);
-  }
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const uint32_t j = j0 + 64u * c + lane;
-    if (j < j1) Msg<MT>::store(X, j, v[c]);
-  }
+  Msg<MT>::store(apair, q, __logf(nn));
 }
 
 // ---------------------------------------------------------------- ray kernel
@@ -235,6 +208,7 @@ template <int MI, int MO, int L, int U>
 __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__ lanes,
                                                       const uint32_t* __restrict__ slice_len, uint32_t nslices,
                                                       const uint2* __restrict__ inc,
+                                                      const uint32_t* __restrict__ inc_pair,
                                                       const typename Msg<MI>::T* __restrict__ Xi,
                                                       typename Msg<MO>::T* __restrict__ Xo,
                                                       const float* __restrict__ sigt, const float* __restrict__ biat,
@@ -263,13 +237,21 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
   rc.z0 = rec.z0;
   rc.inv = rec.inv_norm;
   const uint2* __restrict__ rl = inc + rec.off;  // record k at rl[k * kSlice]
+  const uint32_t* __restrict__ rq = inc_pair + rec.off;
   const uint32_t nsteps = (len + L - 1) / L;
 
   auto load_rec = [&](uint32_t step) -> uint2 {
     const uint32_t k = step * L + j;
     return k < len ? rl[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
   };
-  auto load_msg = [&](const uint2 e) -> float { return Msg<MI>::load(Xi, e.x != kGapVox ? e.x : 0u); };
+  auto load_pair = [&](uint32_t step) -> uint32_t {
+    const uint32_t k = step * L + j;
+    return k < len ? rq[(size_t)k * kSlice] : 0u;
+  };
+  // voxel -> ray message of the record's pair (gap records read pair 0)
+  auto load_msg = [&](const uint2 e, const uint32_t q) -> float {
+    return Msg<MI>::load(Xi, e.x != kGapVox ? q : 0u);
+  };
 
   Kahan kL = {0.f, 0.f}, kS = {0.f, 0.f}, kT = {0.f, 0.f};
   Kahan L0 = kL, S0 = kS;
@@ -293,8 +275,7 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
       S0 = kS;
     }
     const bool before = front && k < i0;  // record in front of the likelihood window
-    // only steps wholly in front are written here: pass 2 re-reads the step
-    // holding i0, whose inputs may be stored in place of its outputs
+    // steps wholly in front are written here, the step holding i0 by pass 2
     if (front && !found && live) Msg<MO>::store(Xo, e.x, kFEps * ln * Msg<MO>::kOut);
     kT.add(before ? 0.0f : vpsi * (1.0f - __expf(lnn)));
     kL.add(Group<L>::last(a));
@@ -303,17 +284,21 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
   uint32_t st = 0;
   for (; st + U <= nsteps; st += U) {
     uint2 e[U];
+    uint32_t qv[U];
     float x[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) e[u] = load_rec(st + u);
+    for (int u = 0; u < U; ++u) {
+      e[u] = load_rec(st + u);
+      qv[u] = load_pair(st + u);
+    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u]);
+    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u], qv[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) step1(e[u], x[u], st + u);
   }
   for (; st < nsteps; ++st) {
     const uint2 e = load_rec(st);
-    step1(e, load_msg(e), st);
+    step1(e, load_msg(e, load_pair(st)), st);
   }
   float T = Group<L>::sum(kT.at(0.0f));
   if (rec.cell_flags & kRanOut) {
@@ -365,17 +350,21 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
   st = smin;
   for (; st + U <= nsteps; st += U) {
     uint2 e[U];
+    uint32_t qv[U];
     float x[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) e[u] = load_rec(st + u);
+    for (int u = 0; u < U; ++u) {
+      e[u] = load_rec(st + u);
+      qv[u] = load_pair(st + u);
+    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u]);
+    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u], qv[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) step2(e[u], x[u], st + u);
   }
   for (; st < nsteps; ++st) {
     const uint2 e = load_rec(st);
-    step2(e, load_msg(e), st);
+    step2(e, load_msg(e, load_pair(st)), st);
   }
 }
 
@@ -584,22 +573,19 @@ inline unsigned grid_for(size_t n, int threads) { return (unsigned)((n + threads
 
 }  // namespace
 
-void launch_expand(const CamPass& a, hipStream_t s) {
+void launch_pair_msgs(const CamPass& a, hipStream_t s) {
   if (a.npairs == 0) return;
-  hipLaunchKernelGGL(k_pair_msg, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.npairs, a.B, a.M,
-                     a.apair);
-  const uint32_t ntiles = (a.nlive + kTile - 1) / kTile;
-  dim3 grid(grid_for(ntiles, 4)), block(256);
+  const dim3 grid(grid_for(a.npairs, 256)), block(256);
   switch (msg_in_type(a.msg_type)) {
     case kMsgF16:
-      hipLaunchKernelGGL(k_expand<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, a.apair, (__half*)a.X);
+      hipLaunchKernelGGL(k_pair_msg<kMsgF16>, grid, block, 0, s, a.pair_vox, a.npairs, a.B, a.M, (__half*)a.apair);
       break;
     case kMsgBF16:
-      hipLaunchKernelGGL(k_expand<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, a.apair,
-                         (uint16_t*)a.X);
+      hipLaunchKernelGGL(k_pair_msg<kMsgBF16>, grid, block, 0, s, a.pair_vox, a.npairs, a.B, a.M,
+                         (uint16_t*)a.apair);
       break;
     default:
-      hipLaunchKernelGGL(k_expand<kMsgF32>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, a.apair, (float*)a.X);
+      hipLaunchKernelGGL(k_pair_msg<kMsgF32>, grid, block, 0, s, a.pair_vox, a.npairs, a.B, a.M, (float*)a.apair);
   }
 }
 
@@ -608,7 +594,8 @@ void launch_ray_t(const CamPass& a, hipStream_t s) {
   const uint32_t blocks = (a.nslices * kSlice + (256 / L) - 1) / (256 / L);
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
   hipLaunchKernelGGL((k_ray_messages<MI, MO, L, U>), dim3(nb), dim3(256), 0, s, a.lanes, a.slice_len, a.nslices,
-                     a.inc, (const typename Msg<MI>::T*)a.X, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
+                     a.inc, a.inc_pair, (const typename Msg<MI>::T*)a.apair, (typename Msg<MO>::T*)a.Xo, a.sig,
+                     a.bia, a.lookup_n);
 }
 
 template <int MI, int MO>

@@ -5,7 +5,10 @@
 // (voxel, incidence index)).  Each run of equal voxels is one
 // (camera, voxel) pair with a segment [pair_off[q], pair_off[q+1]) of the
 // sorted order; every incidence record is rewritten to hold its sorted
-// position instead of its voxel.  The BP iteration then needs no atomics:
+// position instead of its voxel, and a parallel array receives its pair
+// index.  The BP iteration reads the voxel->ray message of a record from the
+// compact per-pair array (a few tens of MB per camera, cache resident) and
+// writes the ray->voxel message into the record's slot of the segment array.  The BP iteration then needs no atomics:
 // messages are written into the incidence's own slot of the segment array
 // and each pair sums its segment in a fixed order, so results are
 // bit-reproducible.  The pair's summed chord length C (what
@@ -54,31 +57,27 @@ __global__ void k_sorted_len(const uint32_t* __restrict__ vals, uint32_t nlive, 
   len[j] = __uint_as_float(rec[vals[j]].y);
 }
 
-__global__ void k_set_pos(const uint32_t* __restrict__ vals, uint32_t nlive, uint2* __restrict__ rec) {
+// record -> its sorted position, and its pair: heads before position j plus
+// j's own head, minus one
+__global__ void k_set_pos(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ heads,
+                          const uint32_t* __restrict__ hscan, uint32_t nlive, uint2* __restrict__ rec,
+                          uint32_t* __restrict__ pair) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nlive) return;
-  rec[vals[j]].x = j;
+  const uint32_t i = vals[j];
+  rec[i].x = j;
+  pair[i] = hscan[j] + heads[j] - 1u;
 }
 
 __global__ void k_set_last(uint32_t* p, uint32_t v) { *p = v; }
 
-// record -> voxel (for export): binary search of the record's sorted position
-__global__ void k_export_vox(const uint2* __restrict__ rec, uint32_t n, const uint32_t* __restrict__ pair_off,
-                             uint32_t npairs, const uint32_t* __restrict__ pair_vox, uint2* __restrict__ out) {
+// record -> voxel (for export)
+__global__ void k_export_vox(const uint2* __restrict__ rec, const uint32_t* __restrict__ pair, uint32_t n,
+                             const uint32_t* __restrict__ pair_vox, uint2* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint2 e = rec[i];
-  if (e.x != kGapVox) {
-    uint32_t lo = 0, hi = npairs;  // largest q with pair_off[q] <= pos
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (pair_off[mid] <= e.x)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    e.x = pair_vox[lo];
-  }
+  if (e.x != kGapVox) e.x = pair_vox[pair[i]];
   out[i] = e;
 }
 
@@ -119,7 +118,8 @@ hipError_t build_segments(SegmentBuild& b, hipStream_t s) {
 }
 
 void finish_segments(SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t* pair_off, uint32_t* pair_vox,
-                     float* pair_c, const uint64_t* head_bits, const uint32_t* tile_h0, hipStream_t s) {
+                     float* pair_c, const uint64_t* head_bits, const uint32_t* tile_h0, uint32_t* rec_pair,
+                     hipStream_t s) {
   const uint32_t n = b.nrec;
   if (n == 0) return;
   hipLaunchKernelGGL(k_pairs, dim3(blocks(n, 256)), dim3(256), 0, s, b.keys_out, b.heads, b.hscan, n, pair_off,
@@ -134,7 +134,8 @@ void finish_segments(SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t*
     uint32_t* scratch = b.vals_in;
     launch_segment_sum_f32(head_bits, tile_h0, nlive, len, pair_c, scratch, (float*)(scratch + nt), scratch + 2 * nt,
                            (float*)(scratch + 3 * nt), (float*)(scratch + 4 * nt), pair_off, s);
-    hipLaunchKernelGGL(k_set_pos, dim3(blocks(nlive, 256)), dim3(256), 0, s, b.vals_out, nlive, b.rec);
+    hipLaunchKernelGGL(k_set_pos, dim3(blocks(nlive, 256)), dim3(256), 0, s, b.vals_out, b.heads, b.hscan, nlive,
+                       b.rec, rec_pair);
   }
 }
 
@@ -145,10 +146,10 @@ void launch_pack_heads(const SegmentBuild& b, uint32_t nlive, uint64_t* bits, ui
   hipLaunchKernelGGL(k_tile_h0, dim3(blocks(ntiles, 256)), dim3(256), 0, s, b.hscan, nlive, tile_h0);
 }
 
-void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_off, uint32_t npairs,
-                       const uint32_t* pair_vox, uint2* out, hipStream_t s) {
+void launch_export_vox(const uint2* rec, const uint32_t* rec_pair, uint32_t n, const uint32_t* pair_vox, uint2* out,
+                       hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_export_vox, dim3(blocks(n, 256)), dim3(256), 0, s, rec, n, pair_off, npairs, pair_vox, out);
+  hipLaunchKernelGGL(k_export_vox, dim3(blocks(n, 256)), dim3(256), 0, s, rec, rec_pair, n, pair_vox, out);
 }
 
 }  // namespace mrf

@@ -60,19 +60,24 @@ size_t segment_sort_temp_bytes(uint32_t n, int bits);
 hipError_t build_segments(SegmentBuild& b, hipStream_t s);
 // pair offsets/voxels, summed chord length C per pair (needs the head
 // bitmask of launch_pack_heads), then records rewritten to sorted positions
+// and rec_pair[i] set to the pair of live record i
 void finish_segments(SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t* pair_off, uint32_t* pair_vox,
-                     float* pair_c, const uint64_t* head_bits, const uint32_t* tile_h0, hipStream_t s);
-void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_off, uint32_t npairs,
-                       const uint32_t* pair_vox, uint2* out, hipStream_t s);
+                     float* pair_c, const uint64_t* head_bits, const uint32_t* tile_h0, uint32_t* rec_pair,
+                     hipStream_t s);
+// records with their voxel ids restored (for export)
+void launch_export_vox(const uint2* rec, const uint32_t* rec_pair, uint32_t n, const uint32_t* pair_vox, uint2* out,
+                       hipStream_t s);
 // head bitmask (bit j = position j starts a pair) and heads before each tile
 void launch_pack_heads(const SegmentBuild& b, uint32_t nlive, uint64_t* bits, uint32_t* tile_h0, hipStream_t s);
 
-// bp.hip: one camera pass = expand -> ray messages -> segmented epilogue.
+// bp.hip: one camera pass = per-pair voxel->ray messages -> ray messages ->
+// segmented epilogue.
 struct CamPass {
   const RayRec* lanes;     // one per lane of every slice
   const uint32_t* slice_len;
   uint32_t nslices;
   const uint2* inc;        // sliced records {sorted position | kGapVox, chord length}
+  const uint32_t* inc_pair;  // pair index of each live sliced record
   const uint32_t* pair_vox;
   const uint32_t* pair_off;  // npairs + 1
   const float* pair_c;
@@ -80,14 +85,13 @@ struct CamPass {
   uint32_t nlive;          // positions (live incidences)
   const uint64_t* heads;   // head bitmask
   const uint32_t* tile_h0; // heads before each kTile tile
-  float* apair;            // per-pair scratch: voxel -> ray message
+  void* apair;             // per-pair voxel -> ray message ln n (input side type)
   float* S;                // per-pair scratch: segment sums
   uint32_t *pf_q, *pl_q;   // per-tile scratch of the segment sums
   float *pf_v, *pl_v, *mid;
   float* B;
   float* M;                // this camera's messages M_c (FP32, pool sized)
-  void* X;                 // segment array of voxel->ray messages ln n (input side type)
-  void* Xo;                // segment array of ray->voxel messages p*l (output side type; may alias X)
+  void* Xo;                // segment array of ray->voxel messages p*l (output side type)
   uint8_t* upd;
   const float* sig;        // sensor-model lookup tables
   const float* bia;
@@ -95,7 +99,7 @@ struct CamPass {
   int msg_type;
   int ray_sched;           // RaySched
 };
-void launch_expand(const CamPass& a, hipStream_t s);
+void launch_pair_msgs(const CamPass& a, hipStream_t s);
 // deterministic segment sums of an FP32 position array into S[pair]
 void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,
                             uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v, float* mid,

@@ -180,3 +180,23 @@ def test_every_ray_schedule_matches_oracle(sched, monkeypatch):
     err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
     print(f"schedule {sched} {_lib.RAY_SCHEDULES[sched]}: max-abs {err.max():.3e}")
     assert err.max() <= MARG_TOL
+
+
+def test_cameras_added_between_calls():
+    """Keyframes added after an inference (the viewer's worker loop,
+    Viewer.cpp:56-57): buffers grow with the larger camera set and the
+    result matches an oracle fed the same call sequence."""
+    p, frames, _ = synth.make_workload("small")
+    g = gpu_for(p, frames[:1])
+    o = oracle_for(p, frames[:1])
+    g.perform_inference(max_iters=2)
+    o.perform_inference(2)
+    for pose, K, depth in frames[1:]:
+        g.add_camera_with_depth(pose, depth, K)
+        o.add_camera(pose, depth, K)
+    g.perform_inference(max_iters=2)
+    o.perform_inference(2)
+    assert g.num_bricks() == o.num_bricks()
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas()).max()
+    print(f"cameras added between calls: max-abs {err:.3e}")
+    assert err <= MARG_TOL
