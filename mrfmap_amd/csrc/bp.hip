@@ -369,12 +369,71 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
 }
 
 // Segment sums S_q = sum of p*l over pair q's positions, in a fixed order.
-// One wavefront per tile: a segmented inclusive scan per 64-wide chunk, the
-// open segment carried from chunk to chunk.  Segments that start and end in
-// the tile are written to S directly; the part of a segment that started in
-// an earlier tile goes to pf, the part of one that continues into the next
-// tile to pl, and a tile lying wholly inside one segment to mid.
+// One wavefront per tile of kTile = 512 positions; lane l owns the 8
+// consecutive positions j0 + 8l .. j0 + 8l + 7 (two 128-bit loads for FP32).
+// Each lane sums its runs serially and writes the segments that start and
+// end inside it; the runs that cross lanes are joined by one segmented
+// inclusive scan over the 64 lanes (DPP).  Of the segments that cross tiles,
+// the part that started in an earlier tile goes to pf, the part that
+// continues into the next tile to pl, and a tile lying wholly inside one
+// segment to mid; k_seg_span adds them up.
 constexpr uint32_t kNoPair = 0xFFFFFFFFu;
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dppi(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xF, true);
+}
+
+// (x, f) <- (x_left, f_left) (+) (x, f): add the left value unless a segment
+// starts at or before this lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step(float& x, int& f) {
+  const float y = dpp<CTRL, ROWS>(x);
+  const int g = dppi<CTRL, ROWS>(f);
+  x = f ? x : x + y;
+  f |= g;
+}
+
+__device__ __forceinline__ void wave_seg_scan(float& x, int& f) {
+  seg_step<0x111, 0xF>(x, f);
+  seg_step<0x112, 0xF>(x, f);
+  seg_step<0x114, 0xF>(x, f);
+  seg_step<0x118, 0xF>(x, f);
+  seg_step<0x142, 0xA>(x, f);
+  seg_step<0x143, 0xC>(x, f);
+}
+
+__device__ __forceinline__ int wave_scan_i(int x) {
+  x += dppi<0x111, 0xF>(x);
+  x += dppi<0x112, 0xF>(x);
+  x += dppi<0x114, 0xF>(x);
+  x += dppi<0x118, 0xF>(x);
+  x += dppi<0x142, 0xA>(x);
+  x += dppi<0x143, 0xC>(x);
+  return x;
+}
+
+// 8 consecutive messages from p0 (a multiple of 8), zero past j1
+template <int MT>
+__device__ __forceinline__ void load8(const typename Msg<MT>::T* __restrict__ X, uint32_t p0, uint32_t j1,
+                                      float (&x)[8]) {
+  typedef typename Msg<MT>::T T;
+  if (p0 + 8 <= j1) {
+    constexpr int kVec = (int)(8 * sizeof(T) / sizeof(uint4));
+    union {
+      uint4 v[kVec];
+      T t[8];
+    } u;
+    const uint4* src = reinterpret_cast<const uint4*>(X + p0);
+#pragma unroll
+    for (int k = 0; k < kVec; ++k) u.v[k] = src[k];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = Msg<MT>::load(u.t, (uint32_t)i);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = p0 + i < j1 ? Msg<MT>::load(X, p0 + i) : 0.0f;
+  }
+}
 
 template <int MT>
 __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ heads, const uint32_t* __restrict__ tile_h0,
@@ -385,93 +444,88 @@ __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ he
   const int lane = threadIdx.x & 63;
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const uint32_t j0 = w * kTile;
-  if (j0 >= nlive) return;
+  if (j0 >= nlive) return;  // uniform per wavefront
   const uint32_t j1 = min(j0 + kTile, nlive);
-  uint32_t before = tile_h0[w];
-  const bool open_first = ((heads[j0 >> 6] >> (j0 & 63)) & 1ull) == 0ull;  // tile starts inside a segment
-  float carry = 0.0f;                                                      // open segment's running sum
-  bool carry_from_before = open_first;
-  uint32_t pfq = kNoPair, plq = kNoPair;
-  float pfv = 0.0f, plv = 0.0f, midv = 0.0f;
-  bool is_mid = false;
-  constexpr int C = kTile / 64;
-  float xs[C];
+  const uint32_t p0 = j0 + 8u * (uint32_t)lane;
+  // head bits of this lane's 8 positions (zero past nlive)
+  const uint32_t hb = p0 < j1 ? (uint32_t)(heads[p0 >> 6] >> (p0 & 63)) & 0xFFu : 0u;
+  float x[8];
+  load8<MT>(X, p0, j1, x);
+  const int nh = __popc(hb);
+  const int incl = wave_scan_i(nh);
+  const uint32_t before = tile_h0[w] + (uint32_t)(incl - nh);  // heads before p0
+  // serial runs; segments that start and end in the lane are complete
+  float run = 0.0f, headpart = 0.0f;
+  int cnt = 0;
 #pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const uint32_t j = j0 + 64u * c + lane;
-    xs[c] = j < j1 ? Msg<MT>::load(X, j) : 0.0f;
+  for (int i = 0; i < 8; ++i) {
+    if ((hb >> i) & 1u) {
+      if (cnt)
+        S[before + (uint32_t)cnt - 1u] = run;
+      else
+        headpart = run;
+      ++cnt;
+      run = 0.0f;
+    }
+    run += x[i];
   }
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const uint32_t base = j0 + 64u * c;
-    if (base >= j1) break;
-    const uint64_t word = heads[base >> 6];
-    const uint64_t next_word = (base + 64 < nlive) ? heads[(base >> 6) + 1] : ~0ull;
-    const uint32_t j = base + lane;
-    const bool valid = j < j1;
-    const uint32_t q = pair_of(word, before, lane);
-    int head = (int)((word >> lane) & 1ull);
-    float x = xs[c];
-    if (lane == 0 && !head) x += carry;  // continue the open segment
-    if (lane == 0) head = 1;
-#pragma unroll
-    for (int dlt = 1; dlt < 64; dlt <<= 1) {
-      const float y = __shfl_up(x, dlt);
-      const int h = __shfl_up(head, dlt);
-      if (lane >= dlt && !head) {
-        x += y;
-        head |= h;
-      }
-    }
-    // a segment ends at j when j + 1 starts another pair or is past the data
-    const bool nxt_head = lane == 63 ? (next_word & 1ull) != 0ull : ((word >> (lane + 1)) & 1ull) != 0ull;
-    const bool ends = valid && (j + 1 >= nlive || nxt_head);
-    // the chunk's first segment is the continuation of the carried one
-    const uint64_t headbits = word & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
-    const bool in_first = headbits == 0ull;  // no head at or before this lane in this chunk
-    if (ends) {
-      if (in_first && carry_from_before) {
-        pfq = q;  // unique per tile: lane-uniform values written once below
-        pfv = x;
-      } else {
-        S[q] = x;
-      }
-    }
-    // the last valid lane carries its open segment into the next chunk
-    const uint32_t last = min(63u, j1 - 1 - base);
-    const float xl = __shfl(x, (int)last);
-    const int el = __shfl((int)ends, (int)last);
-    const int firstl = __shfl((int)in_first, (int)last);
-    const uint32_t ql = (uint32_t)__shfl((int)q, (int)last);
-    carry_from_before = el ? false : (firstl ? carry_from_before : false);
-    carry = el ? 0.0f : xl;
-    if (base + 64 >= j1 && !el) {  // tile ends inside a segment
-      if (carry_from_before) {
-        is_mid = true;
-        midv = xl;
-      } else {
-        plq = ql;
-        plv = xl;
-      }
-    }
-    before += (uint32_t)__popcll(word);
-  }
-  // pf was set by the single lane holding the end of the tile's first segment
-  const unsigned long long pfmask = __ballot(pfq != kNoPair);
+  // join the runs across lanes: U = open run at the end of this lane
+  float U = run;
+  int F = cnt > 0;
+  wave_seg_scan(U, F);
+  float in = __shfl_up(U, 1);
+  int fin = __shfl_up(F, 1);
   if (lane == 0) {
-    pf_q[w] = kNoPair;
-    pl_q[w] = plq;
-    pl_v[w] = plv;
-    mid[w] = is_mid ? midv : 0.0f;
+    in = 0.0f;
+    fin = 0;
   }
-  if (pfmask) {
-    const int src = __ffsll((long long)pfmask) - 1;
-    const uint32_t q = (uint32_t)__shfl((int)pfq, src);
-    const float v = __shfl(pfv, src);
-    if (lane == 0) {
-      pf_q[w] = q;
-      pf_v[w] = v;
+  const bool open_first = ((heads[j0 >> 6] >> (j0 & 63)) & 1ull) == 0ull;  // tile starts inside a segment
+  // the segment that ends just before this lane's first head
+  bool pf_here = false;
+  float pfv = 0.0f;
+  if (cnt) {
+    const float v = in + headpart;
+    if (fin)
+      S[before - 1u] = v;
+    else if (open_first) {
+      pf_here = true;  // first head of the tile: the segment began in an earlier tile
+      pfv = v;
     }
+  }
+  // the run open at the end of the tile
+  const float Ut = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(U), 63));
+  const int Ft = __builtin_amdgcn_readlane(F, 63);
+  const int ntile = __builtin_amdgcn_readlane(incl, 63);
+  const uint32_t qt = tile_h0[w] + (uint32_t)ntile - 1u;  // pair of the tile's last head
+  const bool ends_here = j1 >= nlive || (heads[j1 >> 6] & 1ull) != 0ull;
+  const unsigned long long pfmask = __ballot(pf_here);
+  const float pfv0 =
+      pfmask ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pfv), __builtin_ctzll(pfmask))) : 0.0f;
+  if (lane == 0) {
+    uint32_t pq = kNoPair, lq = kNoPair;
+    float pv = 0.0f, lv = 0.0f, mv = 0.0f;
+    if (pfmask) {
+      pq = tile_h0[w] - 1u;
+      pv = pfv0;
+    }
+    if (ends_here) {
+      if (Ft)
+        S[qt] = Ut;
+      else {
+        pq = tile_h0[w] - 1u;  // no head in the tile: the whole tile ends an earlier segment
+        pv = Ut;
+      }
+    } else if (Ft) {
+      lq = qt;
+      lv = Ut;
+    } else {
+      mv = Ut;  // the tile lies inside one segment
+    }
+    pf_q[w] = pq;
+    pf_v[w] = pv;
+    pl_q[w] = lq;
+    pl_v[w] = lv;
+    mid[w] = mv;
   }
 }
 
