@@ -94,8 +94,11 @@ typedef struct mrf_stats {
   double t_ray_kernel_ms;      /* summed device time of the ray-message kernel */
   double t_epilogue_ms;        /* summed device time of the epilogue kernel */
   double t_total_ms;           /* wall time of the whole call */
-  int ray_sched;               /* ray-kernel schedule in use (lanes per ray x loads in flight) */
+  int ray_sched;               /* ray-kernel schedule in use (records in flight: front x window) */
   double t_autotune_ms;        /* time spent choosing it (first call of a context) */
+  int64_t window_updates_per_iter; /* updates whose message depends on the ray; the others lie where
+                                      the depth likelihood is exactly 0 in FP32 and carry the constant
+                                      message F_EPS * l, summed per voxel once per traversal */
 } mrf_stats;
 
 /* gvdb_params_t() defaults (GVDBParams.h:32-55). */

@@ -16,8 +16,8 @@ ERRORS = {-1: "MRF_E_INVALID", -2: "MRF_E_HIP", -3: "MRF_E_NOMEM", -4: "MRF_E_SH
 ROW_MAJOR, COL_MAJOR = 0, 1
 ACT_ENDPOINT, ACT_FULL_RAY = 0, 1
 MSG_F32, MSG_F16, MSG_BF16, MSG_MIXED_F16, MSG_MIXED_BF16 = 0, 1, 2, 3, 4
-# ray-kernel schedules (lanes per ray, steps of loads in flight)
-RAY_SCHEDULES = {0: (1, 4), 1: (1, 8), 2: (2, 4), 3: (2, 8), 4: (4, 2), 5: (4, 4)}
+# ray-kernel schedules (records loaded ahead in the front loop, in the window passes)
+RAY_SCHEDULES = {0: (8, 4), 1: (8, 8), 2: (16, 4), 3: (16, 8), 4: (4, 4), 5: (16, 16)}
 
 # every symbol include/mrfmap_hip.h declares
 EXPORTS = [
@@ -47,6 +47,7 @@ class MrfStats(ctypes.Structure):
         ("list_records", ctypes.c_int64), ("t_geometry_ms", ctypes.c_double), ("t_iters_ms", ctypes.c_double),
         ("t_ray_kernel_ms", ctypes.c_double), ("t_epilogue_ms", ctypes.c_double),
         ("t_total_ms", ctypes.c_double), ("ray_sched", ctypes.c_int), ("t_autotune_ms", ctypes.c_double),
+        ("window_updates_per_iter", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -36,19 +36,23 @@ struct Camera {
   int rows = 0, cols = 0;
   float* d_depth = nullptr;
   RayRec* d_lanes = nullptr;       // one per lane of every slice (tiled pixel order)
-  uint32_t* d_slice_len = nullptr; // records per lane of each slice
+  uint32_t* d_len_f = nullptr;     // front records per lane of each slice
+  uint32_t* d_len_w = nullptr;     // window records per lane of each slice
   uint32_t nslices = 0;
   uint32_t nrays = 0;              // rays with at least one record
-  uint2* d_inc = nullptr;          // sliced records (padded)
-  uint32_t* d_inc_pair = nullptr;  // pair index of each live sliced record (0 for gap records)
-  uint64_t nrec = 0, nupd = 0;     // padded records, live incidences
+  uint2* d_inc = nullptr;          // sliced records {pair | gap, length}: front region, then window region
+  uint64_t nrec = 0, base_w = 0;   // padded records; first record of the window region
+  uint64_t nupd = 0;               // live incidences
+  uint64_t nwin = 0;               // live window incidences (window positions)
   uint64_t nlist = 0;              // records without the slice padding
+  uint32_t* d_win_pos = nullptr;   // window position of each window-region record
   uint32_t* d_pair_vox = nullptr;
-  uint32_t* d_pair_off = nullptr;
-  float* d_pair_c = nullptr;
-  uint32_t npairs = 0;
-  uint64_t* d_heads = nullptr;   // head bitmask over sorted positions
-  uint32_t* d_tile_h0 = nullptr; // heads before each kTile tile
+  float* d_pair_c = nullptr;       // summed chord length per pair
+  float* d_pair_sf = nullptr;      // summed constant front messages per pair
+  uint32_t npairs = 0, nwp = 0;    // pairs; pairs with window incidences (numbered first)
+  uint32_t* d_wseg_off = nullptr;  // nwp + 1 window segment offsets
+  uint64_t* d_heads = nullptr;     // window segment head bitmask
+  uint32_t* d_tile_h0 = nullptr;   // window heads before each kTile tile
 };
 
 }  // namespace
@@ -85,10 +89,15 @@ struct mrf_ctx {
   uint32_t *d_pf_q = nullptr, *d_pl_q = nullptr;
   float *d_pf_v = nullptr, *d_pl_v = nullptr, *d_mid = nullptr;
   size_t cap_tiles = 0;
-  // segment-build scratch
+  // segment-build scratch (per record), per-pair scratch and the emitted lists
   uint32_t *d_keys_in = nullptr, *d_keys_out = nullptr, *d_vals_in = nullptr, *d_vals_out = nullptr;
   uint32_t *d_heads = nullptr, *d_hscan = nullptr;
+  uint32_t *d_x0 = nullptr, *d_x1 = nullptr, *d_x2 = nullptr, *d_x3 = nullptr;
   size_t cap_sort = 0;
+  uint32_t* d_pscr = nullptr;
+  size_t cap_pscr = 0;
+  uint2* d_sell0 = nullptr;
+  size_t cap_sell0 = 0;
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
 
@@ -255,10 +264,15 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   if (n > c->cap_sort) {
     dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
     dfree(&c->d_heads); dfree(&c->d_hscan);
+    dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
     c->cap_sort = 0;
     int rc;
-    if ((rc = dalloc(c, &c->d_keys_in, n)) || (rc = dalloc(c, &c->d_keys_out, n)) || (rc = dalloc(c, &c->d_vals_in, n)) ||
-        (rc = dalloc(c, &c->d_vals_out, n)) || (rc = dalloc(c, &c->d_heads, n)) || (rc = dalloc(c, &c->d_hscan, n)))
+    // the x arrays also hold small per-tile tables: a little slack for tiny inputs
+    const size_t nx = n + 1024;
+    if ((rc = dalloc(c, &c->d_keys_in, nx)) || (rc = dalloc(c, &c->d_keys_out, nx)) ||
+        (rc = dalloc(c, &c->d_vals_in, nx)) || (rc = dalloc(c, &c->d_vals_out, nx)) ||
+        (rc = dalloc(c, &c->d_heads, nx)) || (rc = dalloc(c, &c->d_hscan, nx)) || (rc = dalloc(c, &c->d_x0, nx)) ||
+        (rc = dalloc(c, &c->d_x1, nx)) || (rc = dalloc(c, &c->d_x2, nx)) || (rc = dalloc(c, &c->d_x3, nx)))
       return rc;
     c->cap_sort = n;
   }
@@ -276,20 +290,42 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   return ensure_scan(c, n);
 }
 
+int ensure_pair_scratch(mrf_ctx* c, size_t words) {
+  if (words <= c->cap_pscr) return MRF_OK;
+  dfree(&c->d_pscr);
+  c->cap_pscr = 0;
+  int rc = dalloc(c, &c->d_pscr, words);
+  if (rc) return rc;
+  c->cap_pscr = words;
+  return MRF_OK;
+}
+
+int ensure_sell0(mrf_ctx* c, size_t n) {
+  if (n <= c->cap_sell0 && c->d_sell0) return MRF_OK;
+  dfree(&c->d_sell0);
+  c->cap_sell0 = 0;
+  int rc = dalloc(c, &c->d_sell0, n);
+  if (rc) return rc;
+  c->cap_sell0 = n;
+  return MRF_OK;
+}
+
 void free_camera_lists(Camera& cam) {
   dfree(&cam.d_lanes);
-  dfree(&cam.d_slice_len);
+  dfree(&cam.d_len_f);
+  dfree(&cam.d_len_w);
   dfree(&cam.d_inc);
-  dfree(&cam.d_inc_pair);
+  dfree(&cam.d_win_pos);
   dfree(&cam.d_pair_vox);
-  dfree(&cam.d_pair_off);
   dfree(&cam.d_pair_c);
+  dfree(&cam.d_pair_sf);
+  dfree(&cam.d_wseg_off);
   dfree(&cam.d_heads);
   dfree(&cam.d_tile_h0);
   cam.nrays = 0;
   cam.nslices = 0;
-  cam.nrec = cam.nupd = cam.nlist = 0;
-  cam.npairs = 0;
+  cam.nrec = cam.base_w = cam.nupd = cam.nwin = cam.nlist = 0;
+  cam.npairs = cam.nwp = 0;
 }
 
 __global__ void k_count_live(const uint2* __restrict__ inc, uint64_t n, unsigned long long* out) {
@@ -315,48 +351,64 @@ int bit_length(uint64_t v) {
   return b;
 }
 
+// Traversal lists, pairs and the window segment layout of one camera.
 int traverse_camera(mrf_ctx* c, Camera& cam) {
   free_camera_lists(cam);
   const CamGeom cg = make_geom(c, cam);
   const size_t npix = (size_t)num_tiled_pixels(cg);
   const uint32_t ntiles = (uint32_t)(npix / kSlice);
+  hipStream_t s = c->stream;
   int rc;
   if ((rc = ensure_pixel_scratch(c, npix)) || (rc = ensure_scan(c, npix))) return rc;
-  launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, c->stream);
-  launch_exclusive_scan(c->d_counts, c->d_offs, npix, c->d_scan_tmp, c->d_total, c->stream);
+  // 1. every ray's records, in order, into one sliced region: per tile the
+  //    longest of its 64 rays (d_has), slice sizes (d_rayidx) -> offsets (d_offs)
+  launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, s);
+  launch_exclusive_scan(c->d_counts, c->d_offs, npix, c->d_scan_tmp, c->d_total, s);
   const uint64_t nlist = read_total(c);
-  launch_exclusive_scan(c->d_has, c->d_rayidx, npix, c->d_scan_tmp, c->d_total, c->stream);
+  launch_exclusive_scan(c->d_has, c->d_rayidx, npix, c->d_scan_tmp, c->d_total, s);
   const uint64_t nrays = read_total(c);
-  // slices: per tile the longest of its 64 rays; d_offs/d_rayidx are reused as
-  // the slice sizes and offsets
-  if ((rc = dalloc(c, &cam.d_slice_len, ntiles))) return rc;
-  launch_slice_len(c->d_counts, ntiles, cam.d_slice_len, c->d_rayidx, c->stream);
-  launch_exclusive_scan(c->d_rayidx, c->d_offs, ntiles, c->d_scan_tmp, c->d_total, c->stream);
-  const uint64_t nrec = read_total(c);
+  launch_slice_len(c->d_counts, ntiles, c->d_has, c->d_rayidx, s);
+  launch_exclusive_scan(c->d_rayidx, c->d_offs, ntiles, c->d_scan_tmp, c->d_total, s);
+  const uint64_t nrec0 = read_total(c);
+  HIPCHK(c, hipGetLastError());
+  if (nrec0 >= 0x7FFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^31 incidence records in one camera");
+  if ((rc = ensure_sell0(c, std::max<uint64_t>(nrec0, 1))) || (rc = dalloc(c, &cam.d_lanes, npix))) return rc;
+  launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_has, c->d_sell0, cam.d_lanes, s);
+  // 2. front/window split (counts into d_counts / d_rayidx), then the front
+  //    region (sizes d_has -> offsets d_offs) and the window region (sizes
+  //    d_counts -> offsets d_has) of the final record buffer
+  launch_split(cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_sig, c->d_bia, c->p.lookup_n, c->d_counts, c->d_rayidx,
+               s);
+  if ((rc = dalloc(c, &cam.d_len_f, ntiles)) || (rc = dalloc(c, &cam.d_len_w, ntiles))) return rc;
+  launch_slice_len(c->d_counts, ntiles, cam.d_len_f, c->d_has, s);
+  launch_exclusive_scan(c->d_has, c->d_offs, ntiles, c->d_scan_tmp, c->d_total, s);
+  const uint64_t nrec_f = read_total(c);
+  launch_slice_len(c->d_rayidx, ntiles, cam.d_len_w, c->d_counts, s);
+  launch_exclusive_scan(c->d_counts, c->d_has, ntiles, c->d_scan_tmp, c->d_total, s);
+  const uint64_t nrec_w = read_total(c);
+  const uint64_t nrec = nrec_f + nrec_w;
   HIPCHK(c, hipGetLastError());
   if (nrec >= 0x7FFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^31 incidence records in one camera");
   cam.nrec = nrec;
+  cam.base_w = nrec_f;
   cam.nlist = nlist;
   cam.nrays = (uint32_t)nrays;
   cam.nslices = ntiles;
-  if ((rc = dalloc(c, &cam.d_inc, nrec)) || (rc = dalloc(c, &cam.d_inc_pair, nrec)) ||
-      (rc = dalloc(c, &cam.d_lanes, npix)))
-    return rc;
-  launch_zero_words(cam.d_inc_pair, nrec * sizeof(uint32_t), c->stream);
-  launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, cam.d_slice_len, cam.d_inc,
-              cam.d_lanes, c->stream);
-  // live incidences (records that are not gap records)
-  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 8, c->stream));
+  if ((rc = dalloc(c, &cam.d_inc, nrec))) return rc;
+  launch_reslice(cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
+                 (uint32_t)nrec_f, cam.d_inc, s);
+  // 3. live incidences (records that are not gap records)
+  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 8, s));
   if (nrec) {
     unsigned g = (unsigned)std::min<uint64_t>((nrec + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, c->stream, cam.d_inc, nrec,
-                       (unsigned long long*)(c->d_total + 2));
+    hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, s, cam.d_inc, nrec, (unsigned long long*)(c->d_total + 2));
   }
-  uint64_t live = 0;
-  HIPCHK(c, hipMemcpyAsync(&live, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  cam.nupd = live;
-  // (camera, voxel) segments
+  uint64_t live64 = 0;
+  HIPCHK(c, hipMemcpyAsync(&live64, c->d_total + 2, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  cam.nupd = live64;
+  const uint32_t live = (uint32_t)live64;
+  // 4. (camera, voxel) pairs in voxel order
   const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
   const int bits = bit_length(nvox);
   if ((rc = ensure_sort(c, std::max<uint64_t>(nrec, 1), bits))) return rc;
@@ -375,22 +427,75 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   b.temp_bytes = c->sort_tmp_bytes;
   b.scan_tmp = c->d_scan_tmp;
   b.d_total = c->d_total;
-  HIPCHK(c, build_segments(b, c->stream));
-  const uint64_t np = nrec ? read_total(c) : 0;
-  cam.npairs = (uint32_t)np;
-  if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_off, np + 1)) ||
-      (rc = dalloc(c, &cam.d_pair_c, np)))
+  HIPCHK(c, build_segments(b, s));
+  const uint32_t np = nrec ? (uint32_t)read_total(c) : 0u;
+  cam.npairs = np;
+  if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
+      (rc = dalloc(c, &cam.d_pair_sf, np)))
     return rc;
-  const size_t nwords = (live + 63) / 64 + 1, nseg_tiles = (live + kTile - 1) / kTile + 1;
-  if ((rc = dalloc(c, &cam.d_heads, nwords)) || (rc = dalloc(c, &cam.d_tile_h0, nseg_tiles))) return rc;
-  if (nrec) {
-    launch_pack_heads(b, (uint32_t)live, cam.d_heads, cam.d_tile_h0, c->stream);
-    finish_segments(b, cam.npairs, (uint32_t)live, cam.d_pair_off, cam.d_pair_vox, cam.d_pair_c, cam.d_heads,
-                    cam.d_tile_h0, cam.d_inc_pair, c->stream);
-  } else {
-    HIPCHK(c, hipMemsetAsync(cam.d_pair_off, 0, 4, c->stream));
+  // per-pair scratch in the sorted numbering
+  const size_t P = (size_t)np + 1;
+  if ((rc = ensure_pair_scratch(c, 7 * P))) return rc;
+  uint32_t* off_old = c->d_pscr;
+  uint32_t* vox_old = off_old + P;
+  float* c_old = (float*)(vox_old + P);
+  float* sf_old = c_old + P;
+  uint32_t* mark = (uint32_t*)(sf_old + P);
+  uint32_t* rank = mark + P;
+  uint32_t* perm = rank + P;
+  // 5. per pair: C and the constant front messages (fixed-order segment sums);
+  //    window positions of the window incidences in sorted order
+  float* len = (float*)c->d_keys_in;
+  float* fmsg = (float*)c->d_x0;
+  uint32_t* win = c->d_x1;
+  uint32_t* wpos = c->d_x2;
+  uint64_t nwin = 0;
+  if (live) {
+    launch_pairs(b, np, live, off_old, vox_old, s);
+    launch_sorted_vals(b, live, (uint32_t)nrec_f, len, fmsg, win, s);
+    const size_t nwords = live / 64 + 1, ntl = live / kTile + 1;
+    uint64_t* hbits = (uint64_t*)c->d_x3;
+    uint32_t* th0 = c->d_x3 + 2 * nwords;
+    launch_pack_flags(b.heads, b.hscan, live, hbits, th0, s);
+    uint32_t* tscr = c->d_vals_in;
+    launch_segment_sum_f32(hbits, th0, live, len, c_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
+                           (float*)(tscr + 3 * ntl), (float*)(tscr + 4 * ntl), off_old, s);
+    launch_segment_sum_f32(hbits, th0, live, fmsg, sf_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
+                           (float*)(tscr + 3 * ntl), (float*)(tscr + 4 * ntl), off_old, s);
+    launch_exclusive_scan(win, wpos, live, c->d_scan_tmp, c->d_total, s);
+    nwin = read_total(c);
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  cam.nwin = nwin;
+  // 6. window segments: their pairs (wq), heads (whead) and head counts (hw)
+  uint32_t* wq = c->d_keys_out;
+  uint32_t* whead = c->d_x0;
+  uint32_t* hw = c->d_keys_in;
+  uint32_t nwp = 0;
+  if (nwin) {
+    launch_window_pairs(b, win, wpos, live, wq, s);
+    launch_window_heads(wq, (uint32_t)nwin, whead, s);
+    launch_exclusive_scan(whead, hw, nwin, c->d_scan_tmp, c->d_total, s);
+    nwp = (uint32_t)read_total(c);
+  }
+  cam.nwp = nwp;
+  // 7. renumber the pairs (window pairs first) and point the records at them
+  if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_pos, nrec_w))) return rc;
+  if (np) {
+    launch_zero_words(mark, (size_t)np * 4, s);
+    launch_mark_window_pairs(whead, hw, wq, (uint32_t)nwin, nwp, mark, cam.d_wseg_off, s);
+    launch_exclusive_scan(mark, rank, np, c->d_scan_tmp, c->d_total, s);
+    launch_permute_pairs(mark, rank, np, nwp, vox_old, c_old, sf_old, perm, cam.d_pair_vox, cam.d_pair_c,
+                         cam.d_pair_sf, s);
+  } else {
+    launch_zero_words(cam.d_wseg_off, 4, s);
+  }
+  launch_zero_words(cam.d_win_pos, (size_t)nrec_w * 4, s);
+  launch_set_records(b, win, wpos, perm, live, (uint32_t)nrec_f, cam.d_win_pos, s);
+  // 8. window segment head bitmask and per-tile head counts
+  const size_t nwords_w = nwin / 64 + 1, ntiles_w = nwin / kTile + 1;
+  if ((rc = dalloc(c, &cam.d_heads, nwords_w)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles_w))) return rc;
+  launch_pack_flags(whead, hw, (uint32_t)nwin, cam.d_heads, cam.d_tile_h0, s);
+  HIPCHK(c, hipStreamSynchronize(s));
   HIPCHK(c, hipGetLastError());
   return MRF_OK;
 }
@@ -442,9 +547,9 @@ int prepare(mrf_ctx* c) {
   c->M_stride = stride;
   size_t xneed = 1, pneed = 1, tneed = 1;
   for (const Camera& cam : c->cams) {
-    xneed = std::max<size_t>(xneed, cam.nupd);
+    xneed = std::max<size_t>(xneed, cam.nwin);
     pneed = std::max<size_t>(pneed, cam.npairs);
-    tneed = std::max<size_t>(tneed, (cam.nupd + kTile - 1) / kTile);
+    tneed = std::max<size_t>(tneed, (cam.nwin + kTile - 1) / kTile);
   }
   if (pneed > c->cap_pairs) {
     dfree(&c->d_apair);
@@ -507,15 +612,19 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   Camera& cam = c->cams[ci];
   CamPass a;
   a.lanes = cam.d_lanes;
-  a.slice_len = cam.d_slice_len;
+  a.len_f = cam.d_len_f;
+  a.len_w = cam.d_len_w;
   a.nslices = cam.nslices;
   a.inc = cam.d_inc;
-  a.inc_pair = cam.d_inc_pair;
+  a.win_pos = cam.d_win_pos;
+  a.base_w = (uint32_t)cam.base_w;
   a.pair_vox = cam.d_pair_vox;
-  a.pair_off = cam.d_pair_off;
   a.pair_c = cam.d_pair_c;
+  a.pair_sf = cam.d_pair_sf;
   a.npairs = cam.npairs;
-  a.nlive = (uint32_t)cam.nupd;
+  a.nwp = cam.nwp;
+  a.nwin = (uint32_t)cam.nwin;
+  a.wseg_off = cam.d_wseg_off;
   a.heads = cam.d_heads;
   a.tile_h0 = cam.d_tile_h0;
   a.apair = c->d_apair;
@@ -533,7 +642,7 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.bia = c->d_bia;
   a.lookup_n = c->p.lookup_n;
   a.msg_type = c->p.msg_dtype;
-  a.ray_sched = c->ray_sched < 0 ? kSchedL2U8 : c->ray_sched;
+  a.ray_sched = c->ray_sched < 0 ? kSchedF8W4 : c->ray_sched;
   return a;
 }
 
@@ -550,7 +659,7 @@ int autotune(mrf_ctx* c) {
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   float best = 1e30f;
-  int pick = kSchedL2U8;
+  int pick = kSchedF8W4;
   for (int sched = 0; sched < kNumSched; ++sched) {
     CamPass a = cam_pass(c, best_cam);
     a.ray_sched = sched;
@@ -700,6 +809,8 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
   dfree(&c->d_heads); dfree(&c->d_hscan);
+  dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
+  dfree(&c->d_pscr); dfree(&c->d_sell0);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
   dfree(&c->d_scan_tmp); dfree(&c->d_total);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -773,6 +884,7 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   launch_marginals(c->d_B, c->d_A, nvox, c->stream);
   for (const Camera& cam : c->cams) {
     s.updates_per_iter += (int64_t)cam.nupd;
+    s.window_updates_per_iter += (int64_t)cam.nwin;
     s.pairs_per_iter += cam.npairs;
     s.rays += cam.nrays;
     s.list_records += (int64_t)cam.nlist;
@@ -897,7 +1009,7 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
   if (cam.nrec) {
     uint2* tmp = nullptr;
     if ((rc = dalloc(c, &tmp, cam.nrec))) return rc;
-    launch_export_vox(cam.d_inc, cam.d_inc_pair, (uint32_t)cam.nrec, cam.d_pair_vox, tmp, c->stream);
+    launch_export_vox(cam.d_inc, (uint32_t)cam.nrec, cam.d_pair_vox, tmp, c->stream);
     hipError_t e = hipMemcpyAsync(sell.data(), tmp, cam.nrec * sizeof(uint2), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(&tmp);
@@ -906,23 +1018,24 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
   uint64_t off = 0, r = 0;
   for (size_t pix = 0; pix < npix; ++pix) {
     const RayRec& L = lanes[pix];
-    if (L.n == 0) continue;
+    const uint32_t n = L.nf + L.nw;
+    if (n == 0) continue;
     if (rays) {
       uint32_t* o = rays + 6 * r;
       o[0] = (uint32_t)off;
-      o[1] = L.n;
+      o[1] = n;
       std::memcpy(&o[2], &L.z0, 4);
       std::memcpy(&o[3], &L.inv_norm, 4);
       std::memcpy(&o[4], &L.depth, 4);
       o[5] = L.cell_flags;
     }
     if (records)
-      for (uint32_t k = 0; k < L.n; ++k) {
-        const uint2 e = sell[L.off + (size_t)k * kSlice];
+      for (uint32_t k = 0; k < n; ++k) {
+        const uint2 e = k < L.nf ? sell[L.off_f + (size_t)k * kSlice] : sell[L.off_w + (size_t)(k - L.nf) * kSlice];
         records[2 * (off + k)] = e.x;
         records[2 * (off + k) + 1] = e.y;
       }
-    off += L.n;
+    off += n;
     ++r;
   }
   return MRF_OK;

@@ -140,231 +140,217 @@ __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ p
 }
 
 // ---------------------------------------------------------------- ray kernel
-// L lanes per ray (L = 1, 2, 4): lane j of a ray handles records j, j+L,
-// j+2L, ...; each step covers L consecutive records with a prefix sum over
-// the ray's L lanes (DPP quad permutes) and a carried, Kahan-compensated sum
-// from earlier steps.  The 64 rays of a slice (one 8x8 pixel tile) are
-// spread over L wavefronts; record k of all rays of a slice is contiguous.
-template <int CTRL>
-__device__ __forceinline__ float qperm(float x) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ int qperm_i(int x) {
-  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, false);
+// Per-lane sensor model of a ray: lookup cell coefficients (get_depth_sigma /
+// get_depth_bias, gvdbkernels.cu:46-55).
+__device__ __forceinline__ void load_cell(const float* __restrict__ sigt, const float* __restrict__ biat, int lookup_n,
+                                          uint32_t cell, Poly& sig, Poly& bia) {
+  sig.n = bia.n = lookup_n;
+  sig.c0 = sigt[cell];
+  sig.c1 = sigt[cell + 1];
+  sig.c2 = lookup_n == 3 ? sigt[cell + 2] : 0.0f;
+  bia.c0 = biat[cell];
+  bia.c1 = biat[cell + 1];
+  bia.c2 = lookup_n == 3 ? biat[cell + 2] : 0.0f;
 }
 
-template <int L>
-struct Group {
-  // inclusive prefix over the L lanes of a ray (j = lane index within the ray)
-  __device__ __forceinline__ static float scan(float x, int j) {
-    if (L == 2) {
-      const float y = qperm<0xA0>(x);  // quad_perm [0,0,2,2]
-      return j == 1 ? x + y : x;
-    } else if (L == 4) {
-      float y = qperm<0x90>(x);        // quad_perm [0,0,1,2]
-      x = j >= 1 ? x + y : x;
-      y = qperm<0x40>(x);              // quad_perm [0,0,0,1]
-      return j >= 2 ? x + y : x;
+// Front/window split of every ray of a camera (one thread per ray), with the
+// ray kernel's own FP32 depth sums: the front ends at the first record whose
+// exit depth s has d - (s + bias(s)) <= kFrontU * 3 sigma(s).  The emitted
+// ray holds all its records in the front part (nf = n, nw = 0).
+__global__ __launch_bounds__(256) void k_split(RayRec* __restrict__ lanes, uint32_t npix, const uint2* __restrict__ sell,
+                                               const float* __restrict__ sigt, const float* __restrict__ biat,
+                                               int lookup_n, uint32_t* __restrict__ cnt_f,
+                                               uint32_t* __restrict__ cnt_w) {
+  const uint32_t ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= npix) return;
+  RayRec r = lanes[ray];
+  const uint32_t n = r.nf;
+  uint32_t nf = n;
+  if (n) {
+    Poly sig, bia;
+    load_cell(sigt, biat, lookup_n, r.cell_flags & 0x7FFFFFFFu, sig, bia);
+    const uint2* __restrict__ rl = sell + r.off_f;
+    Kahan kS = {0.f, 0.f};
+    for (uint32_t k = 0; k < n; ++k) {
+      const float ln = __uint_as_float(rl[(size_t)k * kSlice].y);
+      const float s = r.z0 + kS.at(ln) * r.inv_norm;
+      const float sg = sig(s);
+      if (!(sg > 0.0f && r.depth - (s + bia(s)) > kFrontU * (3.0f * sg))) {
+        nf = k;
+        break;
+      }
+      kS.add(ln);
     }
-    return x;
   }
-  // value of the ray's last lane
-  __device__ __forceinline__ static float last(float x) {
-    if (L == 2) return qperm<0xF5>(x);  // quad_perm [1,1,3,3]
-    if (L == 4) return qperm<0xFF>(x);  // quad_perm [3,3,3,3]
-    return x;
-  }
-  __device__ __forceinline__ static int min(int v) {
-    if (L >= 2) v = ::min(v, qperm_i<0xB1>(v));  // quad_perm [1,0,3,2]
-    if (L == 4) v = ::min(v, qperm_i<0x4E>(v));  // quad_perm [2,3,0,1]
-    return v;
-  }
-  __device__ __forceinline__ static float sum(float x) {
-    if (L >= 2) x = x + qperm<0xB1>(x);
-    if (L == 4) x = x + qperm<0x4E>(x);
-    return x;
-  }
-};
+  r.nf = nf;
+  r.nw = n - nf;
+  lanes[ray] = r;
+  cnt_f[ray] = nf;
+  cnt_w[ray] = n - nf;
+}
 
-struct RayCtx {
-  Poly sig, bia;
-  float d, z0, inv;
-};
+// Copy every ray's records into the front and window regions of the final
+// record buffer (sliced: lane l of slice t at region offset + k * 64 + l),
+// padding each lane to its slice's length with gap records.
+__global__ __launch_bounds__(256) void k_reslice(RayRec* __restrict__ lanes, uint32_t npix,
+                                                 const uint2* __restrict__ sell, const uint32_t* __restrict__ off_f,
+                                                 const uint32_t* __restrict__ len_f, const uint32_t* __restrict__ off_w,
+                                                 const uint32_t* __restrict__ len_w, uint32_t base_w,
+                                                 uint2* __restrict__ inc) {
+  const uint32_t ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= npix) return;
+  const uint32_t t = ray >> 6, l = ray & 63u;
+  RayRec r = lanes[ray];
+  const uint2* __restrict__ src = sell + r.off_f;
+  const uint32_t fo = off_f[t] + l, wo = base_w + off_w[t] + l;
+  const uint2 gap = make_uint2(kGapVox, 0u);
+  for (uint32_t k = 0; k < len_f[t]; ++k) inc[fo + (size_t)k * kSlice] = k < r.nf ? src[(size_t)k * kSlice] : gap;
+  for (uint32_t k = 0; k < len_w[t]; ++k)
+    inc[wo + (size_t)k * kSlice] = k < r.nw ? src[(size_t)(r.nf + k) * kSlice] : gap;
+  r.off_f = fo;
+  r.off_w = wo;
+  lanes[ray] = r;
+}
 
-// Pass 1: Lambda_k = sum_{i<=k} l_i ln n_i (the reference's visibility
-// includes voxel k itself, gvdbkernels.cu:781-783), vpsi_k = exp(Lambda_k)
-// psi(s_k), tau_k = vpsi_k (1 - n_k), T = sum tau (+ V_n when the walk ran
-// out and d lies beyond, gvdbkernels.cu:1196-1208).  Until the first record
-// with vpsi > 0 every earlier tau is exactly 0, so P = 0, mu1 = 0 and the
-// message is exactly the clamp floor F_EPS: pass 1 writes those of the
-// steps before the one holding that record directly.
-// Pass 2 restarts at the step holding that record with the saved sums and
-// writes p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi, mu0 = P + R / n
-// (gvdbkernels.cu:936-951).  U steps of loads are issued before their
-// arithmetic.  The file is built without FP contraction, so pass 2
-// recomputes pass 1's terms bit for bit.
-template <int MI, int MO, int L, int U>
+// One thread per ray, the 64 rays of a slice (one 8x8 pixel tile) in one
+// wavefront; record k of all lanes of a slice is contiguous.
+//
+// Front (records k < nf): psi(s) = 0 there, so each record only extends the
+// visibility sum Lambda = sum l ln n (the reference's visibility includes the
+// voxel itself, gvdbkernels.cu:781-783) and the travelled length; its
+// message F_EPS * l is constant and summed per pair once per traversal.
+// Window pass 1: vpsi_k = exp(Lambda_k) psi(s_k), tau_k = vpsi_k (1 - n_k),
+// T = sum tau (+ V_n when the walk ran out and d lies beyond,
+// gvdbkernels.cu:1196-1208).  Window pass 2 restarts at the window with the
+// saved sums and writes p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi,
+// mu0 = P + R / n (gvdbkernels.cu:936-951) times l into the record's slot of
+// the camera's voxel-sorted window segment array.  UF / UW records are
+// loaded before their arithmetic.  The file is built without FP contraction,
+// so pass 2 recomputes pass 1's terms bit for bit.  Rays without window
+// records only have constant messages and are skipped.
+template <int MI, int MO, int UF, int UW>
 __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__ lanes,
-                                                      const uint32_t* __restrict__ slice_len, uint32_t nslices,
+                                                      const uint32_t* __restrict__ len_f,
+                                                      const uint32_t* __restrict__ len_w, uint32_t nslices,
                                                       const uint2* __restrict__ inc,
-                                                      const uint32_t* __restrict__ inc_pair,
+                                                      const uint32_t* __restrict__ win_pos, uint32_t base_w,
                                                       const typename Msg<MI>::T* __restrict__ Xi,
                                                       typename Msg<MO>::T* __restrict__ Xo,
                                                       const float* __restrict__ sigt, const float* __restrict__ biat,
                                                       int lookup_n) {
-  constexpr uint32_t kRaysPerBlock = 256 / L;
   // XCD-aware order: consecutive blocks (neighbouring pixel tiles) share an XCD's L2
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const uint32_t ray = lb * kRaysPerBlock + threadIdx.x / L;  // global ray id = slice * 64 + r
-  const int j = (int)(threadIdx.x % L);
+  const uint32_t ray = lb * 256u + threadIdx.x;  // slice * 64 + lane
   const uint32_t t = ray >> 6;
-  if (t >= nslices) return;  // uniform per wavefront: a wavefront never spans slices
-  const uint32_t len = slice_len[t];
-  if (len == 0) return;
+  if (t >= nslices) return;  // uniform per wavefront
+  const uint32_t lw = len_w[t];
+  if (lw == 0) return;
   const RayRec rec = lanes[ray];
-  const uint32_t cell = rec.cell_flags & 0x7FFFFFFFu;
-  RayCtx rc;
-  rc.sig.n = rc.bia.n = lookup_n;
-  rc.sig.c0 = sigt[cell];
-  rc.sig.c1 = sigt[cell + 1];
-  rc.sig.c2 = lookup_n == 3 ? sigt[cell + 2] : 0.0f;
-  rc.bia.c0 = biat[cell];
-  rc.bia.c1 = biat[cell + 1];
-  rc.bia.c2 = lookup_n == 3 ? biat[cell + 2] : 0.0f;
-  rc.d = rec.depth;
-  rc.z0 = rec.z0;
-  rc.inv = rec.inv_norm;
-  const uint2* __restrict__ rl = inc + rec.off;  // record k at rl[k * kSlice]
-  const uint32_t* __restrict__ rq = inc_pair + rec.off;
-  const uint32_t nsteps = (len + L - 1) / L;
+  const uint32_t nf = rec.nf, nw = rec.nw;
+  if (nw == 0) return;  // no lane-crossing operations below
+  const uint32_t lf = len_f[t];
+  auto msg = [&](const uint2 e) -> float { return e.x != kGapVox ? Msg<MI>::load(Xi, e.x) : 0.0f; };
 
-  auto load_rec = [&](uint32_t step) -> uint2 {
-    const uint32_t k = step * L + j;
-    return k < len ? rl[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
-  };
-  auto load_pair = [&](uint32_t step) -> uint32_t {
-    const uint32_t k = step * L + j;
-    return k < len ? rq[(size_t)k * kSlice] : 0u;
-  };
-  // voxel -> ray message of the record's pair (gap records read pair 0)
-  auto load_msg = [&](const uint2 e, const uint32_t q) -> float {
-    return Msg<MI>::load(Xi, e.x != kGapVox ? q : 0u);
-  };
-
-  Kahan kL = {0.f, 0.f}, kS = {0.f, 0.f}, kT = {0.f, 0.f};
-  Kahan L0 = kL, S0 = kS;
-  uint32_t i0 = len;  // first record with vpsi > 0 (same in all lanes of a ray)
-  auto step1 = [&](const uint2 e, const float msg, const uint32_t step) {
-    const uint32_t k = step * L + j;
-    const bool live = e.x != kGapVox;
-    const float lnn = live ? msg : 0.0f;
-    const float ln = __uint_as_float(e.y);
-    const float a = Group<L>::scan(ln * lnn, j);
-    const float b = Group<L>::scan(ln, j);
-    const float s = rc.z0 + kS.at(b) * rc.inv;
-    const float ps = depth_psi(s, rc.d, rc.sig, rc.bia);
-    const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
-    const bool front = i0 == len;
-    const int firstj = Group<L>::min(vpsi > 0.0f ? j : L);
-    const bool found = front && firstj < L;
-    if (found) {
-      i0 = step * L + (uint32_t)firstj;
-      L0 = kL;
-      S0 = kS;
-    }
-    const bool before = front && k < i0;  // record in front of the likelihood window
-    // steps wholly in front are written here, the step holding i0 by pass 2
-    if (front && !found && live) Msg<MO>::store(Xo, e.x, kFEps * ln * Msg<MO>::kOut);
-    kT.add(before ? 0.0f : vpsi * (1.0f - __expf(lnn)));
-    kL.add(Group<L>::last(a));
-    kS.add(Group<L>::last(b));
-  };
-  uint32_t st = 0;
-  for (; st + U <= nsteps; st += U) {
-    uint2 e[U];
-    uint32_t qv[U];
-    float x[U];
+  Kahan kL = {0.f, 0.f}, kS = {0.f, 0.f};
+  const uint2* __restrict__ rf = inc + rec.off_f;
+  for (uint32_t k0 = 0; k0 < lf; k0 += UF) {
+    uint2 e[UF];
+    float x[UF];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      e[u] = load_rec(st + u);
-      qv[u] = load_pair(st + u);
+    for (int u = 0; u < UF; ++u) {
+      const uint32_t k = k0 + u;
+      e[u] = k < nf ? rf[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u], qv[u]);
+    for (int u = 0; u < UF; ++u) x[u] = msg(e[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) step1(e[u], x[u], st + u);
+    for (int u = 0; u < UF; ++u) {
+      if (k0 + u < nf) {
+        const float ln = __uint_as_float(e[u].y);
+        kL.add(ln * x[u]);
+        kS.add(ln);
+      }
+    }
   }
-  for (; st < nsteps; ++st) {
-    const uint2 e = load_rec(st);
-    step1(e, load_msg(e, load_pair(st)), st);
+  Poly sig, bia;
+  load_cell(sigt, biat, lookup_n, rec.cell_flags & 0x7FFFFFFFu, sig, bia);
+  const float z0 = rec.z0, inv = rec.inv_norm, d = rec.depth;
+  const Kahan L0 = kL, S0 = kS;
+  const uint2* __restrict__ rw = inc + rec.off_w;
+  Kahan kT = {0.f, 0.f};
+  for (uint32_t k0 = 0; k0 < lw; k0 += UW) {
+    uint2 e[UW];
+    float x[UW];
+#pragma unroll
+    for (int u = 0; u < UW; ++u) {
+      const uint32_t k = k0 + u;
+      e[u] = k < nw ? rw[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < UW; ++u) x[u] = msg(e[u]);
+#pragma unroll
+    for (int u = 0; u < UW; ++u) {
+      if (k0 + u < nw) {
+        const bool live = e[u].x != kGapVox;
+        const float lnn = x[u];
+        const float ln = __uint_as_float(e[u].y);
+        const float a = ln * lnn;
+        const float s = z0 + kS.at(ln) * inv;
+        const float ps = depth_psi(s, d, sig, bia);
+        const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
+        kT.add(vpsi * (1.0f - __expf(lnn)));
+        kL.add(a);
+        kS.add(ln);
+      }
+    }
   }
-  float T = Group<L>::sum(kT.at(0.0f));
+  float T = kT.at(0.0f);
   if (rec.cell_flags & kRanOut) {
-    const float sn = rc.z0 + kS.at(0.0f) * rc.inv;
-    if (rc.d > sn + rc.bia(sn)) T += __expf(kL.at(0.0f));
+    const float sn = z0 + kS.at(0.0f) * inv;
+    if (d > sn + bia(sn)) T += __expf(kL.at(0.0f));
   }
 
-  // pass 2 from the step holding i0 (all lanes execute every step; writes
-  // and state updates are predicated so the group permutes stay convergent).
-  // Records of that step in front of i0 have vpsi = 0 and P = 0, so their p
-  // clamps to exactly F_EPS, the value pass 1 gives the records before them.
-  // A ray without a record with vpsi > 0 was written wholly by pass 1.
-  const uint32_t s0 = i0 == len ? nsteps : i0 / L;
-  uint32_t smin = s0;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) smin = ::min(smin, (uint32_t)__shfl_xor((int)smin, o));
   kL = L0;
   kS = S0;
   Kahan kP = {0.f, 0.f};
-  auto step2 = [&](const uint2 e, const float msg, const uint32_t step) {
-    const bool on = step >= s0;
-    const bool live = e.x != kGapVox;
-    const float lnn = live ? msg : 0.0f;
-    const float ln = __uint_as_float(e.y);
-    const float a = Group<L>::scan(ln * lnn, j);
-    const float b = Group<L>::scan(ln, j);
-    const float s = rc.z0 + kS.at(b) * rc.inv;
-    const float ps = depth_psi(s, rc.d, rc.sig, rc.bia);
-    const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
-    const float nn = __expf(lnn);
-    const float tau = vpsi * (1.0f - nn);
-    const float pin = Group<L>::scan(tau, j);
-    const float P = kP.at(pin - tau);
-    const float R = (T - P) - tau;
-    if (on && live) {
-      const float mu1 = P + vpsi;
-      const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
-      float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
-      p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
-      Msg<MO>::store(Xo, e.x, p * ln * Msg<MO>::kOut);
-    }
-    const float ta = Group<L>::last(a), tb = Group<L>::last(b), tp = Group<L>::last(pin);
-    if (on) {
-      kL.add(ta);
-      kS.add(tb);
-      kP.add(tp);
-    }
-  };
-  st = smin;
-  for (; st + U <= nsteps; st += U) {
-    uint2 e[U];
-    uint32_t qv[U];
-    float x[U];
+  const uint32_t* __restrict__ rp = win_pos + (rec.off_w - base_w);
+  for (uint32_t k0 = 0; k0 < lw; k0 += UW) {
+    uint2 e[UW];
+    uint32_t pos[UW];
+    float x[UW];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      e[u] = load_rec(st + u);
-      qv[u] = load_pair(st + u);
+    for (int u = 0; u < UW; ++u) {
+      const uint32_t k = k0 + u;
+      e[u] = k < nw ? rw[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+      pos[u] = k < nw ? rp[(size_t)k * kSlice] : 0u;
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) x[u] = load_msg(e[u], qv[u]);
+    for (int u = 0; u < UW; ++u) x[u] = msg(e[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) step2(e[u], x[u], st + u);
-  }
-  for (; st < nsteps; ++st) {
-    const uint2 e = load_rec(st);
-    step2(e, load_msg(e, load_pair(st)), st);
+    for (int u = 0; u < UW; ++u) {
+      if (k0 + u < nw) {
+        const bool live = e[u].x != kGapVox;
+        const float lnn = x[u];
+        const float ln = __uint_as_float(e[u].y);
+        const float a = ln * lnn;
+        const float s = z0 + kS.at(ln) * inv;
+        const float ps = depth_psi(s, d, sig, bia);
+        const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
+        const float nn = __expf(lnn);
+        const float tau = vpsi * (1.0f - nn);
+        const float P = kP.at(0.0f);
+        const float R = (T - P) - tau;
+        const float mu1 = P + vpsi;
+        const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
+        float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
+        p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
+        if (live) Msg<MO>::store(Xo, pos[u], p * ln * Msg<MO>::kOut);
+        kL.add(a);
+        kS.add(ln);
+        kP.add(tau);
+      }
+    }
   }
 }
 
@@ -550,11 +536,14 @@ __global__ __launch_bounds__(256) void k_seg_span(const uint32_t* __restrict__ p
   if (lane == 0) S[q] = (pl_v[w] + acc) + pf_v[wt];
 }
 
-// Camera message of each (camera, voxel) pair from its segment sum:
+// Camera message of each (camera, voxel) pair from its message sum:
 // a = clamp(S/C), m = ln a - ln(1-a), B += m - M_c, M_c = m, for C > 0.1
 // (compute_gvdb_message_to_factors, gvdbkernels.cu:408, 457-479).
+// S = (sum of the pair's constant front messages) + (its window segment
+// sum); window pairs are numbered first.
 __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
-                                                    const float* __restrict__ pair_c, uint32_t npairs,
+                                                    const float* __restrict__ pair_c,
+                                                    const float* __restrict__ pair_sf, uint32_t npairs, uint32_t nwp,
                                                     const float* __restrict__ S, float out_scale, float* __restrict__ B,
                                                     float* __restrict__ M, uint8_t* __restrict__ upd) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,7 +551,9 @@ __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__
   const float C = pair_c[q];
   if (!(C > kCumLenMin)) return;
   const uint32_t v = pair_vox[q];
-  float a = S[q] / (C * out_scale);
+  float Sq = pair_sf[q];
+  if (q < nwp) Sq += S[q] / out_scale;
+  float a = Sq / C;
   a = fminf(fmaxf(a, kFEps), 1.0f - kFEps);
   const float m = __logf(a) - __logf(1.0f - a);
   const float mo = M[v];
@@ -643,29 +634,29 @@ void launch_pair_msgs(const CamPass& a, hipStream_t s) {
   }
 }
 
-template <int MI, int MO, int L, int U>
+template <int MI, int MO, int UF, int UW>
 void launch_ray_t(const CamPass& a, hipStream_t s) {
-  const uint32_t blocks = (a.nslices * kSlice + (256 / L) - 1) / (256 / L);
+  const uint32_t blocks = (a.nslices * kSlice + 255) / 256;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
-  hipLaunchKernelGGL((k_ray_messages<MI, MO, L, U>), dim3(nb), dim3(256), 0, s, a.lanes, a.slice_len, a.nslices,
-                     a.inc, a.inc_pair, (const typename Msg<MI>::T*)a.apair, (typename Msg<MO>::T*)a.Xo, a.sig,
-                     a.bia, a.lookup_n);
+  hipLaunchKernelGGL((k_ray_messages<MI, MO, UF, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_f, a.len_w,
+                     a.nslices, a.inc, a.win_pos, a.base_w, (const typename Msg<MI>::T*)a.apair,
+                     (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
 }
 
 template <int MI, int MO>
 void launch_ray_sched(const CamPass& a, hipStream_t s) {
   switch (a.ray_sched) {
-    case kSchedL1U4: launch_ray_t<MI, MO, 1, 4>(a, s); break;
-    case kSchedL1U8: launch_ray_t<MI, MO, 1, 8>(a, s); break;
-    case kSchedL2U4: launch_ray_t<MI, MO, 2, 4>(a, s); break;
-    case kSchedL4U2: launch_ray_t<MI, MO, 4, 2>(a, s); break;
-    case kSchedL4U4: launch_ray_t<MI, MO, 4, 4>(a, s); break;
-    default: launch_ray_t<MI, MO, 2, 8>(a, s);
+    case kSchedF8W8: launch_ray_t<MI, MO, 8, 8>(a, s); break;
+    case kSchedF16W4: launch_ray_t<MI, MO, 16, 4>(a, s); break;
+    case kSchedF16W8: launch_ray_t<MI, MO, 16, 8>(a, s); break;
+    case kSchedF4W4: launch_ray_t<MI, MO, 4, 4>(a, s); break;
+    case kSchedF16W16: launch_ray_t<MI, MO, 16, 16>(a, s); break;
+    default: launch_ray_t<MI, MO, 8, 4>(a, s);
   }
 }
 
 void launch_ray_messages(const CamPass& a, hipStream_t s) {
-  if (a.nslices == 0) return;
+  if (a.nslices == 0 || a.nwin == 0) return;
   switch (a.msg_type) {
     case kMsgF16: launch_ray_sched<kMsgF16, kMsgF16>(a, s); break;
     case kMsgBF16: launch_ray_sched<kMsgBF16, kMsgBF16>(a, s); break;
@@ -687,28 +678,44 @@ void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint
 
 void launch_epilogue(const CamPass& a, hipStream_t s) {
   if (a.npairs == 0) return;
-  const uint32_t ntiles = (a.nlive + kTile - 1) / kTile;
-  dim3 grid(grid_for(ntiles, 4)), block(256);
   float out_scale = 1.0f;
-  switch (msg_out_type(a.msg_type)) {
-    case kMsgF16:
-      hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const __half*)a.Xo,
-                         a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
-      hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.pair_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
-      out_scale = Msg<kMsgF16>::kOut;
-      break;
-    case kMsgBF16:
-      hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nlive, (const uint16_t*)a.Xo,
-                         a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
-      hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.pair_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
-      out_scale = Msg<kMsgBF16>::kOut;
-      break;
-    default:
-      launch_segment_sum_f32(a.heads, a.tile_h0, a.nlive, (const float*)a.Xo, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v,
-                             a.mid, a.pair_off, s);
+  if (a.nwin) {
+    const uint32_t ntiles = (a.nwin + kTile - 1) / kTile;
+    dim3 grid(grid_for(ntiles, 4)), block(256);
+    switch (msg_out_type(a.msg_type)) {
+      case kMsgF16:
+        hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const __half*)a.Xo,
+                           a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
+        hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.wseg_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+        out_scale = Msg<kMsgF16>::kOut;
+        break;
+      case kMsgBF16:
+        hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const uint16_t*)a.Xo,
+                           a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
+        hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.wseg_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+        out_scale = Msg<kMsgBF16>::kOut;
+        break;
+      default:
+        launch_segment_sum_f32(a.heads, a.tile_h0, a.nwin, (const float*)a.Xo, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v,
+                               a.mid, a.wseg_off, s);
+    }
   }
-  hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.npairs,
-                     a.S, out_scale, a.B, a.M, a.upd);
+  hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf,
+                     a.npairs, a.nwp, a.S, out_scale, a.B, a.M, a.upd);
+}
+
+void launch_split(RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia, int lookup_n,
+                  uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s) {
+  if (npix == 0) return;
+  hipLaunchKernelGGL(k_split, dim3(grid_for(npix, 256)), dim3(256), 0, s, lanes, npix, sell, sig, bia, lookup_n, cnt_f,
+                     cnt_w);
+}
+
+void launch_reslice(RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f, const uint32_t* len_f,
+                    const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc, hipStream_t s) {
+  if (npix == 0) return;
+  hipLaunchKernelGGL(k_reslice, dim3(grid_for(npix, 256)), dim3(256), 0, s, lanes, npix, sell, off_f, len_f, off_w,
+                     len_w, base_w, inc);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

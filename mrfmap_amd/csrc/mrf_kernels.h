@@ -14,9 +14,10 @@ inline int msg_out_type(int t) {
 }
 inline size_t msg_bytes(int type) { return type == kMsgF32 ? 4 : 2; }
 
-// Schedules of the ray-message kernel: lanes per ray (L) x steps of loads in
-// flight (U).  Picked per workload by the autotuner in api.hip.
-enum RaySched { kSchedL1U4 = 0, kSchedL1U8 = 1, kSchedL2U4 = 2, kSchedL2U8 = 3, kSchedL4U2 = 4, kSchedL4U4 = 5,
+// Schedules of the ray-message kernel: records loaded ahead of their
+// arithmetic in the front loop (F) and in the window passes (W).  Picked per
+// workload by the autotuner in api.hip.
+enum RaySched { kSchedF8W4 = 0, kSchedF8W8 = 1, kSchedF16W4 = 2, kSchedF16W8 = 3, kSchedF4W4 = 4, kSchedF16W16 = 5,
                 kNumSched = 6 };
 
 // traverse.hip
@@ -44,9 +45,9 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint64_t
 int num_tiled_pixels(const CamGeom& c);
 
 // geometry.hip: stable sort of a camera's incidences by voxel into
-// (camera, voxel) segments; records are rewritten to sorted positions.
+// (camera, voxel) pairs, and the window segment layout of the BP iteration.
 struct SegmentBuild {
-  uint2* rec;          // incidence records (voxel -> sorted position in place)
+  uint2* rec;          // incidence records {voxel | kGapVox, length}
   uint32_t nrec;
   uint32_t gap_key;    // key of gap records (> every voxel id)
   int key_bits;
@@ -57,41 +58,60 @@ struct SegmentBuild {
   uint64_t* d_total;   // receives the number of pairs
 };
 size_t segment_sort_temp_bytes(uint32_t n, int bits);
+// sort + pair heads; the number of pairs lands in *d_total
 hipError_t build_segments(SegmentBuild& b, hipStream_t s);
-// pair offsets/voxels, summed chord length C per pair (needs the head
-// bitmask of launch_pack_heads), then records rewritten to sorted positions
-// and rec_pair[i] set to the pair of live record i
-void finish_segments(SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t* pair_off, uint32_t* pair_vox,
-                     float* pair_c, const uint64_t* head_bits, const uint32_t* tile_h0, uint32_t* rec_pair,
-                     hipStream_t s);
-// records with their voxel ids restored (for export)
-void launch_export_vox(const uint2* rec, const uint32_t* rec_pair, uint32_t n, const uint32_t* pair_vox, uint2* out,
+// pair offsets and voxels in sorted order
+void launch_pairs(const SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t* pair_off, uint32_t* pair_vox,
+                  hipStream_t s);
+// per sorted position: chord length, constant front message F_EPS * l (0 for
+// window incidences, the records at index >= base_w) and the window flag
+void launch_sorted_vals(const SegmentBuild& b, uint32_t nlive, uint32_t base_w, float* len, float* fmsg,
+                        uint32_t* win, hipStream_t s);
+// flag bitmask (bit j = flags[j] != 0) and, per kTile tile, fscan at its start
+void launch_pack_flags(const uint32_t* flags, const uint32_t* fscan, uint32_t n, uint64_t* bits, uint32_t* tile_h0,
                        hipStream_t s);
-// head bitmask (bit j = position j starts a pair) and heads before each tile
-void launch_pack_heads(const SegmentBuild& b, uint32_t nlive, uint64_t* bits, uint32_t* tile_h0, hipStream_t s);
+// window compaction: wq[wpos[j]] = pair (sorted numbering) of window position j
+void launch_window_pairs(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, uint32_t nlive,
+                         uint32_t* wq, hipStream_t s);
+void launch_window_heads(const uint32_t* wq, uint32_t nwin, uint32_t* whead, hipStream_t s);
+void launch_mark_window_pairs(const uint32_t* whead, const uint32_t* hw, const uint32_t* wq, uint32_t nwin,
+                              uint32_t nwp, uint32_t* mark, uint32_t* wseg_off, hipStream_t s);
+// renumber pairs: those with window incidences first, each group in voxel order
+void launch_permute_pairs(const uint32_t* mark, const uint32_t* rank, uint32_t npairs, uint32_t nwp,
+                          const uint32_t* vox_old, const float* c_old, const float* sf_old, uint32_t* perm,
+                          uint32_t* pair_vox, float* pair_c, float* pair_sf, hipStream_t s);
+// records get their pair (new numbering), window records their window position
+void launch_set_records(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, const uint32_t* perm,
+                        uint32_t nlive, uint32_t base_w, uint32_t* win_pos, hipStream_t s);
+// records with their voxel ids restored (for export)
+void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_vox, uint2* out, hipStream_t s);
 
 // bp.hip: one camera pass = per-pair voxel->ray messages -> ray messages ->
-// segmented epilogue.
+// window segment sums -> per-pair update.
 struct CamPass {
   const RayRec* lanes;     // one per lane of every slice
-  const uint32_t* slice_len;
+  const uint32_t* len_f;   // front records per lane of each slice
+  const uint32_t* len_w;   // window records per lane of each slice
   uint32_t nslices;
-  const uint2* inc;        // sliced records {sorted position | kGapVox, chord length}
-  const uint32_t* inc_pair;  // pair index of each live sliced record
+  const uint2* inc;        // sliced records {pair | kGapVox, chord length}: front region, then window region
+  const uint32_t* win_pos; // window position of each window record (index - base_w)
+  uint32_t base_w;         // first record of the window region
   const uint32_t* pair_vox;
-  const uint32_t* pair_off;  // npairs + 1
-  const float* pair_c;
+  const float* pair_c;     // summed chord length C per pair
+  const float* pair_sf;    // summed constant front messages per pair
   uint32_t npairs;
-  uint32_t nlive;          // positions (live incidences)
-  const uint64_t* heads;   // head bitmask
-  const uint32_t* tile_h0; // heads before each kTile tile
+  uint32_t nwp;            // pairs with window incidences (numbered first)
+  uint32_t nwin;           // window positions (live window incidences)
+  const uint32_t* wseg_off;  // nwp + 1 window segment offsets
+  const uint64_t* heads;   // window segment head bitmask
+  const uint32_t* tile_h0; // window heads before each kTile tile
   void* apair;             // per-pair voxel -> ray message ln n (input side type)
-  float* S;                // per-pair scratch: segment sums
+  float* S;                // per window pair: window segment sum
   uint32_t *pf_q, *pl_q;   // per-tile scratch of the segment sums
   float *pf_v, *pl_v, *mid;
   float* B;
   float* M;                // this camera's messages M_c (FP32, pool sized)
-  void* Xo;                // segment array of ray->voxel messages p*l (output side type)
+  void* Xo;                // window segment array of ray->voxel messages p*l (output side type)
   uint8_t* upd;
   const float* sig;        // sensor-model lookup tables
   const float* bia;
@@ -99,6 +119,13 @@ struct CamPass {
   int msg_type;
   int ray_sched;           // RaySched
 };
+// front/window split of an emitted camera (records of ray r at r.off_f, all
+// in the front): sets nf/nw and per-pixel counts
+void launch_split(RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia, int lookup_n,
+                  uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s);
+// copy into the final front/window regions; window region starts at base_w
+void launch_reslice(RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f, const uint32_t* len_f,
+                    const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc, hipStream_t s);
 void launch_pair_msgs(const CamPass& a, hipStream_t s);
 // deterministic segment sums of an FP32 position array into S[pair]
 void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,

@@ -351,15 +351,17 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
   uint2* out = sell + slice_off[tile] + lane;
   PixelCtx p = load_pixel(c, g, depth, pix);
   RayRec r;
-  r.off = slice_off[tile] + lane;
-  r.n = 0;
+  r.off_f = slice_off[tile] + lane;  // all records in the front part until the split
+  r.nf = 0;
+  r.off_w = 0;
+  r.nw = 0;
   r.z0 = 0.0f;
   r.inv_norm = 0.0f;
   r.depth = 0.0f;
   r.cell_flags = 0;
   if (p.ok) {
     WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, out);
-    r.n = wo.n;
+    r.nf = wo.n;
     r.z0 = wo.z0;
     r.inv_norm = (float)(1.0 / p.rs.norm);
     r.depth = p.d;
@@ -367,7 +369,7 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
   }
   // pad the lane to the slice length with no-op records
   const uint32_t len = slice_len[tile];
-  for (uint32_t k = r.n; k < len; ++k) out[(size_t)k * kSlice] = make_uint2(kGapVox, 0u);
+  for (uint32_t k = r.nf; k < len; ++k) out[(size_t)k * kSlice] = make_uint2(kGapVox, 0u);
   lanes[pix] = r;
 }
 
