@@ -65,6 +65,21 @@ def cpu_baseline(wl):
                       f"({st['updates_per_iter']} updates, {dt:.2f} s)"}
 
 
+def measured_traffic(a):
+    """HBM bytes per ray-kernel launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated on a
+    1 GiB copy), or None when it does not cover this configuration."""
+    if a.workload != "real" or a.msg != "f32":
+        return None, None
+    path = os.path.join(ROOT, "profiles", "k_ray_messages_real_f32.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["bytes"], os.path.relpath(path, ROOT)
+    except (OSError, ValueError, KeyError):
+        return None, None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -109,13 +124,16 @@ def main():
     value = n_upd * a.steps / dt
     launches = a.steps * ncam
     avg_launch_ms = st["t_ray_kernel_ms"] / launches
-    # ray kernel per update: 8 B record + 4 B pair index + the pair's voxel->ray
-    # message + the ray->voxel message written into the record's segment slot
+    # ray kernel, algorithmic bytes: every incidence reads its 8 B record and
+    # its pair's voxel->ray message; a window incidence also reads its 4 B
+    # segment position and writes its ray->voxel message
     in_b = 4 if msg in (_lib.MSG_F32, _lib.MSG_MIXED_F16, _lib.MSG_MIXED_BF16) else 2
     out_b = 4 if msg == _lib.MSG_F32 else 2
-    alg_b = 8 + 4 + in_b + out_b
-    bytes_per_launch = n_upd / ncam * alg_b
+    n_win = st["window_updates_per_iter"]
+    front_b, window_b = 8 + in_b, 8 + in_b + 4 + out_b
+    bytes_per_launch = ((n_upd - n_win) * front_b + n_win * window_b) / ncam
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(a)
     print(f"[bench] {a.steps} steps in {dt * 1e3:.1f} ms; ray kernel {st['t_ray_kernel_ms']:.1f} ms, "
           f"epilogue {st['t_epilogue_ms']:.1f} ms", file=sys.stderr, flush=True)
 
@@ -141,6 +159,7 @@ def main():
             "workload": f"{a.workload}: {ncam} keyframes {p.cols}x{p.rows} f={float(p.K[0][0])}, "
                         f"{p.res * 100:.0f} cm voxels, {list(map(float, p.dims))} m room, full-ray activation",
             "updates_per_step": n_upd,
+            "window_updates_per_step": n_win,
             "pairs_per_step": st["pairs_per_iter"],
             "rays": st["rays"],
             "list_records": st["list_records"],
@@ -158,9 +177,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "avg_launch_ms": avg_launch_ms,
-            "alg_bytes_per_update": alg_b,
+            "alg_bytes_per_launch": bytes_per_launch,
+            "alg_bytes_per_front_update": front_b,
+            "alg_bytes_per_window_update": window_b,
         },
         "cpu_baseline": cpu,
     }

@@ -35,6 +35,7 @@ struct Camera {
   float K[9];
   int rows = 0, cols = 0;
   float* d_depth = nullptr;
+  FrontRec* d_front = nullptr;     // one per lane of every slice (tiled pixel order)
   RayRec* d_lanes = nullptr;       // one per lane of every slice (tiled pixel order)
   uint32_t* d_len_f = nullptr;     // front records per lane of each slice
   uint32_t* d_len_w = nullptr;     // window records per lane of each slice
@@ -45,7 +46,7 @@ struct Camera {
   uint64_t nupd = 0;               // live incidences
   uint64_t nwin = 0;               // live window incidences (window positions)
   uint64_t nlist = 0;              // records without the slice padding
-  uint32_t* d_win_pos = nullptr;   // window position of each window-region record
+  uint32_t* d_win_idx = nullptr;   // per window position (voxel order): the record's slot in the window region
   uint32_t* d_pair_vox = nullptr;
   float* d_pair_c = nullptr;       // summed chord length per pair
   float* d_pair_sf = nullptr;      // summed constant front messages per pair
@@ -53,6 +54,9 @@ struct Camera {
   uint32_t* d_wseg_off = nullptr;  // nwp + 1 window segment offsets
   uint64_t* d_heads = nullptr;     // window segment head bitmask
   uint32_t* d_tile_h0 = nullptr;   // window heads before each kTile tile
+  uint32_t* d_chunk_off = nullptr; // nslices + 1: first front chunk of each slice
+  uint32_t* d_chunk_tile = nullptr;
+  uint32_t nchunks = 0;
 };
 
 }  // namespace
@@ -86,6 +90,8 @@ struct mrf_ctx {
   // per-pair / per-tile scratch of a camera pass
   float *d_apair = nullptr, *d_Spair = nullptr;
   size_t cap_pairs = 0;
+  float* d_partial = nullptr;  // front partial sums (64 per chunk)
+  size_t cap_partial = 0;
   uint32_t *d_pf_q = nullptr, *d_pl_q = nullptr;
   float *d_pf_v = nullptr, *d_pl_v = nullptr, *d_mid = nullptr;
   size_t cap_tiles = 0;
@@ -311,17 +317,21 @@ int ensure_sell0(mrf_ctx* c, size_t n) {
 }
 
 void free_camera_lists(Camera& cam) {
+  dfree(&cam.d_front);
   dfree(&cam.d_lanes);
   dfree(&cam.d_len_f);
   dfree(&cam.d_len_w);
   dfree(&cam.d_inc);
-  dfree(&cam.d_win_pos);
+  dfree(&cam.d_win_idx);
   dfree(&cam.d_pair_vox);
   dfree(&cam.d_pair_c);
   dfree(&cam.d_pair_sf);
   dfree(&cam.d_wseg_off);
   dfree(&cam.d_heads);
   dfree(&cam.d_tile_h0);
+  dfree(&cam.d_chunk_off);
+  dfree(&cam.d_chunk_tile);
+  cam.nchunks = 0;
   cam.nrays = 0;
   cam.nslices = 0;
   cam.nrec = cam.base_w = cam.nupd = cam.nwin = cam.nlist = 0;
@@ -372,13 +382,16 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   const uint64_t nrec0 = read_total(c);
   HIPCHK(c, hipGetLastError());
   if (nrec0 >= 0x7FFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^31 incidence records in one camera");
-  if ((rc = ensure_sell0(c, std::max<uint64_t>(nrec0, 1))) || (rc = dalloc(c, &cam.d_lanes, npix))) return rc;
-  launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_has, c->d_sell0, cam.d_lanes, s);
+  if ((rc = ensure_sell0(c, std::max<uint64_t>(nrec0, 1))) || (rc = dalloc(c, &cam.d_front, npix)) ||
+      (rc = dalloc(c, &cam.d_lanes, npix)))
+    return rc;
+  launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_has, c->d_sell0, cam.d_front,
+              cam.d_lanes, s);
   // 2. front/window split (counts into d_counts / d_rayidx), then the front
   //    region (sizes d_has -> offsets d_offs) and the window region (sizes
   //    d_counts -> offsets d_has) of the final record buffer
-  launch_split(cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_sig, c->d_bia, c->p.lookup_n, c->d_counts, c->d_rayidx,
-               s);
+  launch_split(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_sig, c->d_bia, c->p.lookup_n, c->d_counts,
+               c->d_rayidx, s);
   if ((rc = dalloc(c, &cam.d_len_f, ntiles)) || (rc = dalloc(c, &cam.d_len_w, ntiles))) return rc;
   launch_slice_len(c->d_counts, ntiles, cam.d_len_f, c->d_has, s);
   launch_exclusive_scan(c->d_has, c->d_offs, ntiles, c->d_scan_tmp, c->d_total, s);
@@ -395,8 +408,15 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.nrays = (uint32_t)nrays;
   cam.nslices = ntiles;
   if ((rc = dalloc(c, &cam.d_inc, nrec))) return rc;
-  launch_reslice(cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
+  launch_reslice(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
                  (uint32_t)nrec_f, cam.d_inc, s);
+  // front chunks: ceil(len_f / kChunk) per slice that has window records
+  if ((rc = dalloc(c, &cam.d_chunk_off, (size_t)ntiles + 1))) return rc;
+  launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
+  launch_exclusive_scan(c->d_counts, cam.d_chunk_off, (size_t)ntiles + 1, c->d_scan_tmp, c->d_total, s);
+  cam.nchunks = (uint32_t)read_total(c);
+  if ((rc = dalloc(c, &cam.d_chunk_tile, cam.nchunks))) return rc;
+  launch_chunk_tiles(cam.d_chunk_off, ntiles, cam.d_chunk_tile, s);
   // 3. live incidences (records that are not gap records)
   HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 8, s));
   if (nrec) {
@@ -458,9 +478,9 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     uint32_t* th0 = c->d_x3 + 2 * nwords;
     launch_pack_flags(b.heads, b.hscan, live, hbits, th0, s);
     uint32_t* tscr = c->d_vals_in;
-    launch_segment_sum_f32(hbits, th0, live, len, c_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
+    launch_segment_sum_f32(hbits, th0, live, len, nullptr, c_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
                            (float*)(tscr + 3 * ntl), (float*)(tscr + 4 * ntl), off_old, s);
-    launch_segment_sum_f32(hbits, th0, live, fmsg, sf_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
+    launch_segment_sum_f32(hbits, th0, live, fmsg, nullptr, sf_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
                            (float*)(tscr + 3 * ntl), (float*)(tscr + 4 * ntl), off_old, s);
     launch_exclusive_scan(win, wpos, live, c->d_scan_tmp, c->d_total, s);
     nwin = read_total(c);
@@ -479,7 +499,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   }
   cam.nwp = nwp;
   // 7. renumber the pairs (window pairs first) and point the records at them
-  if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_pos, nrec_w))) return rc;
+  if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_idx, nwin))) return rc;
   if (np) {
     launch_zero_words(mark, (size_t)np * 4, s);
     launch_mark_window_pairs(whead, hw, wq, (uint32_t)nwin, nwp, mark, cam.d_wseg_off, s);
@@ -489,8 +509,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   } else {
     launch_zero_words(cam.d_wseg_off, 4, s);
   }
-  launch_zero_words(cam.d_win_pos, (size_t)nrec_w * 4, s);
-  launch_set_records(b, win, wpos, perm, live, (uint32_t)nrec_f, cam.d_win_pos, s);
+  launch_set_records(b, win, wpos, perm, live, (uint32_t)nrec_f, cam.d_win_idx, s);
   // 8. window segment head bitmask and per-tile head counts
   const size_t nwords_w = nwin / 64 + 1, ntiles_w = nwin / kTile + 1;
   if ((rc = dalloc(c, &cam.d_heads, nwords_w)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles_w))) return rc;
@@ -545,9 +564,10 @@ int prepare(mrf_ctx* c) {
     c->M_bytes = need;
   }
   c->M_stride = stride;
-  size_t xneed = 1, pneed = 1, tneed = 1;
+  size_t xneed = 1, pneed = 1, tneed = 1, cneed = 1;
   for (const Camera& cam : c->cams) {
-    xneed = std::max<size_t>(xneed, cam.nwin);
+    cneed = std::max<size_t>(cneed, (size_t)cam.nchunks * kSlice);
+    xneed = std::max<size_t>(xneed, cam.nrec - cam.base_w);  // window region, padding included
     pneed = std::max<size_t>(pneed, cam.npairs);
     tneed = std::max<size_t>(tneed, (cam.nwin + kTile - 1) / kTile);
   }
@@ -557,6 +577,12 @@ int prepare(mrf_ctx* c) {
     c->cap_pairs = 0;
     if ((rc = dalloc(c, &c->d_apair, pneed)) || (rc = dalloc(c, &c->d_Spair, pneed))) return rc;
     c->cap_pairs = pneed;
+  }
+  if (cneed > c->cap_partial) {
+    dfree(&c->d_partial);
+    c->cap_partial = 0;
+    if ((rc = dalloc(c, &c->d_partial, cneed))) return rc;
+    c->cap_partial = cneed;
   }
   if (tneed > c->cap_tiles) {
     dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
@@ -611,13 +637,18 @@ int add_camera(mrf_ctx* c, const float pose[16], const float K[9], const float* 
 CamPass cam_pass(mrf_ctx* c, size_t ci) {
   Camera& cam = c->cams[ci];
   CamPass a;
+  a.front = cam.d_front;
   a.lanes = cam.d_lanes;
   a.len_f = cam.d_len_f;
   a.len_w = cam.d_len_w;
   a.nslices = cam.nslices;
   a.inc = cam.d_inc;
-  a.win_pos = cam.d_win_pos;
+  a.win_idx = cam.d_win_idx;
   a.base_w = (uint32_t)cam.base_w;
+  a.chunk_off = cam.d_chunk_off;
+  a.chunk_tile = cam.d_chunk_tile;
+  a.nchunks = cam.nchunks;
+  a.partial = c->d_partial;
   a.pair_vox = cam.d_pair_vox;
   a.pair_c = cam.d_pair_c;
   a.pair_sf = cam.d_pair_sf;
@@ -805,7 +836,7 @@ void mrf_destroy(mrf_ctx* c) {
   if (c->d_M) (void)hipFree(c->d_M);
   if (c->d_Xo) (void)hipFree(c->d_Xo);
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
-  dfree(&c->d_apair); dfree(&c->d_Spair);
+  dfree(&c->d_apair); dfree(&c->d_Spair); dfree(&c->d_partial);
   dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
   dfree(&c->d_heads); dfree(&c->d_hscan);
@@ -1003,8 +1034,10 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
   if (!rays && !records) return MRF_OK;
   // compact the sliced lists back to per-ray lists in tile order
   const size_t npix = (size_t)cam.nslices * kSlice;
+  std::vector<FrontRec> front(npix);
   std::vector<RayRec> lanes(npix);
   std::vector<uint2> sell(cam.nrec);
+  HIPCHK(c, hipMemcpy(front.data(), cam.d_front, npix * sizeof(FrontRec), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(lanes.data(), cam.d_lanes, npix * sizeof(RayRec), hipMemcpyDeviceToHost));
   if (cam.nrec) {
     uint2* tmp = nullptr;
@@ -1017,8 +1050,9 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
   }
   uint64_t off = 0, r = 0;
   for (size_t pix = 0; pix < npix; ++pix) {
+    const FrontRec& F = front[pix];
     const RayRec& L = lanes[pix];
-    const uint32_t n = L.nf + L.nw;
+    const uint32_t n = F.nf + L.nw;
     if (n == 0) continue;
     if (rays) {
       uint32_t* o = rays + 6 * r;
@@ -1031,7 +1065,7 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
     }
     if (records)
       for (uint32_t k = 0; k < n; ++k) {
-        const uint2 e = k < L.nf ? sell[L.off_f + (size_t)k * kSlice] : sell[L.off_w + (size_t)(k - L.nf) * kSlice];
+        const uint2 e = k < F.nf ? sell[F.off + (size_t)k * kSlice] : sell[L.off_w + (size_t)(k - F.nf) * kSlice];
         records[2 * (off + k)] = e.x;
         records[2 * (off + k) + 1] = e.y;
       }

@@ -156,21 +156,23 @@ __device__ __forceinline__ void load_cell(const float* __restrict__ sigt, const 
 // Front/window split of every ray of a camera (one thread per ray), with the
 // ray kernel's own FP32 depth sums: the front ends at the first record whose
 // exit depth s has d - (s + bias(s)) <= kFrontU * 3 sigma(s).  The emitted
-// ray holds all its records in the front part (nf = n, nw = 0).
-__global__ __launch_bounds__(256) void k_split(RayRec* __restrict__ lanes, uint32_t npix, const uint2* __restrict__ sell,
-                                               const float* __restrict__ sigt, const float* __restrict__ biat,
-                                               int lookup_n, uint32_t* __restrict__ cnt_f,
-                                               uint32_t* __restrict__ cnt_w) {
+// ray holds all its records in the front part.  The Kahan state of the
+// front's length sum is kept: the window continues it.
+__global__ __launch_bounds__(256) void k_split(FrontRec* __restrict__ front, RayRec* __restrict__ lanes, uint32_t npix,
+                                               const uint2* __restrict__ sell, const float* __restrict__ sigt,
+                                               const float* __restrict__ biat, int lookup_n,
+                                               uint32_t* __restrict__ cnt_f, uint32_t* __restrict__ cnt_w) {
   const uint32_t ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= npix) return;
+  FrontRec f = front[ray];
   RayRec r = lanes[ray];
-  const uint32_t n = r.nf;
+  const uint32_t n = f.nf;
   uint32_t nf = n;
+  Kahan kS = {0.f, 0.f};
   if (n) {
     Poly sig, bia;
     load_cell(sigt, biat, lookup_n, r.cell_flags & 0x7FFFFFFFu, sig, bia);
-    const uint2* __restrict__ rl = sell + r.off_f;
-    Kahan kS = {0.f, 0.f};
+    const uint2* __restrict__ rl = sell + f.off;
     for (uint32_t k = 0; k < n; ++k) {
       const float ln = __uint_as_float(rl[(size_t)k * kSlice].y);
       const float s = r.z0 + kS.at(ln) * r.inv_norm;
@@ -182,8 +184,11 @@ __global__ __launch_bounds__(256) void k_split(RayRec* __restrict__ lanes, uint3
       kS.add(ln);
     }
   }
-  r.nf = nf;
+  f.nf = nf;
   r.nw = n - nf;
+  r.ks = kS.s;
+  r.kc = kS.c;
+  front[ray] = f;
   lanes[ray] = r;
   cnt_f[ray] = nf;
   cnt_w[ray] = n - nf;
@@ -192,87 +197,127 @@ __global__ __launch_bounds__(256) void k_split(RayRec* __restrict__ lanes, uint3
 // Copy every ray's records into the front and window regions of the final
 // record buffer (sliced: lane l of slice t at region offset + k * 64 + l),
 // padding each lane to its slice's length with gap records.
-__global__ __launch_bounds__(256) void k_reslice(RayRec* __restrict__ lanes, uint32_t npix,
-                                                 const uint2* __restrict__ sell, const uint32_t* __restrict__ off_f,
-                                                 const uint32_t* __restrict__ len_f, const uint32_t* __restrict__ off_w,
-                                                 const uint32_t* __restrict__ len_w, uint32_t base_w,
-                                                 uint2* __restrict__ inc) {
+__global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, RayRec* __restrict__ lanes,
+                                                 uint32_t npix, const uint2* __restrict__ sell,
+                                                 const uint32_t* __restrict__ off_f, const uint32_t* __restrict__ len_f,
+                                                 const uint32_t* __restrict__ off_w, const uint32_t* __restrict__ len_w,
+                                                 uint32_t base_w, uint2* __restrict__ inc) {
   const uint32_t ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= npix) return;
   const uint32_t t = ray >> 6, l = ray & 63u;
+  FrontRec f = front[ray];
   RayRec r = lanes[ray];
-  const uint2* __restrict__ src = sell + r.off_f;
+  const uint2* __restrict__ src = sell + f.off;
   const uint32_t fo = off_f[t] + l, wo = base_w + off_w[t] + l;
   const uint2 gap = make_uint2(kGapVox, 0u);
-  for (uint32_t k = 0; k < len_f[t]; ++k) inc[fo + (size_t)k * kSlice] = k < r.nf ? src[(size_t)k * kSlice] : gap;
+  for (uint32_t k = 0; k < len_f[t]; ++k) inc[fo + (size_t)k * kSlice] = k < f.nf ? src[(size_t)k * kSlice] : gap;
   for (uint32_t k = 0; k < len_w[t]; ++k)
-    inc[wo + (size_t)k * kSlice] = k < r.nw ? src[(size_t)(r.nf + k) * kSlice] : gap;
-  r.off_f = fo;
+    inc[wo + (size_t)k * kSlice] = k < r.nw ? src[(size_t)(f.nf + k) * kSlice] : gap;
+  f.off = fo;
   r.off_w = wo;
+  front[ray] = f;
   lanes[ray] = r;
 }
 
-// One thread per ray, the 64 rays of a slice (one 8x8 pixel tile) in one
-// wavefront; record k of all lanes of a slice is contiguous.
-//
-// Front (records k < nf): psi(s) = 0 there, so each record only extends the
-// visibility sum Lambda = sum l ln n (the reference's visibility includes the
-// voxel itself, gvdbkernels.cu:781-783) and the travelled length; its
-// message F_EPS * l is constant and summed per pair once per traversal.
-// Window pass 1: vpsi_k = exp(Lambda_k) psi(s_k), tau_k = vpsi_k (1 - n_k),
-// T = sum tau (+ V_n when the walk ran out and d lies beyond,
-// gvdbkernels.cu:1196-1208).  Window pass 2 restarts at the window with the
-// saved sums and writes p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi,
-// mu0 = P + R / n (gvdbkernels.cu:936-951) times l into the record's slot of
-// the camera's voxel-sorted window segment array.  UF / UW records are
-// loaded before their arithmetic.  The file is built without FP contraction,
-// so pass 2 recomputes pass 1's terms bit for bit.  Rays without window
-// records only have constant messages and are skipped.
-template <int MI, int MO, int UF, int UW>
-__global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__ lanes,
-                                                      const uint32_t* __restrict__ len_f,
-                                                      const uint32_t* __restrict__ len_w, uint32_t nslices,
-                                                      const uint2* __restrict__ inc,
-                                                      const uint32_t* __restrict__ win_pos, uint32_t base_w,
-                                                      const typename Msg<MI>::T* __restrict__ Xi,
-                                                      typename Msg<MO>::T* __restrict__ Xo,
-                                                      const float* __restrict__ sigt, const float* __restrict__ biat,
-                                                      int lookup_n) {
+// Front chunks: a slice with window records needs ceil(len_f / kChunk) of
+// them, a slice without none.
+// (cnt[nslices] = 0, so that an exclusive scan of nslices + 1 counts ends
+// with the total)
+__global__ void k_chunk_counts(const uint32_t* __restrict__ len_f, const uint32_t* __restrict__ len_w,
+                               uint32_t nslices, uint32_t* __restrict__ cnt) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nslices) return;
+  cnt[t] = (t < nslices && len_w[t]) ? (len_f[t] + kChunk - 1) / kChunk : 0u;
+}
+
+__global__ void k_chunk_tiles(const uint32_t* __restrict__ chunk_off, uint32_t nslices,
+                              uint32_t* __restrict__ chunk_tile) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslices) return;
+  for (uint32_t w = chunk_off[t]; w < chunk_off[t + 1]; ++w) chunk_tile[w] = t;
+}
+
+// Front sums.  Front records have psi(s) = 0, so all they contribute to a ray
+// is the visibility sum Lambda = sum l ln n (the reference's visibility
+// includes the voxel itself, gvdbkernels.cu:781-783); their messages are
+// the constant F_EPS * l, summed per pair once per traversal, and their
+// length sum is geometric (RayRec::ks/kc).  One wavefront sums records
+// [c * kChunk, (c + 1) * kChunk) of the 64 rays of a slice, so the work
+// splits into near-equal pieces whatever the ray lengths; each lane writes
+// its Kahan-compensated partial sum.  Rays without window records are
+// skipped.
+template <int MI, int U>
+__global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__ front, const RayRec* __restrict__ lanes,
+                                                    const uint32_t* __restrict__ len_f,
+                                                    const uint32_t* __restrict__ chunk_tile,
+                                                    const uint32_t* __restrict__ chunk_off, uint32_t nchunks,
+                                                    const uint2* __restrict__ inc,
+                                                    const typename Msg<MI>::T* __restrict__ Xi,
+                                                    float* __restrict__ partial) {
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nchunks) return;  // uniform per wavefront
+  const uint32_t l = threadIdx.x & 63u;
+  const uint32_t t = chunk_tile[w];
+  const uint32_t k0 = (w - chunk_off[t]) * kChunk;
+  const uint32_t kmax = min(k0 + kChunk, len_f[t]);
+  const uint32_t ray = t * kSlice + l;
+  const FrontRec f = front[ray];
+  const uint32_t k1 = lanes[ray].nw ? min(k0 + kChunk, f.nf) : 0u;
+  const uint2* __restrict__ rf = inc + f.off;
+  Kahan acc = {0.f, 0.f};
+  for (uint32_t kb = k0; kb < kmax; kb += U) {
+    uint2 e[U];
+    float x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t k = kb + u;
+      e[u] = k < k1 ? rf[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = e[u].x != kGapVox ? Msg<MI>::load(Xi, e[u].x) : 0.0f;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (kb + u < k1) acc.add(__uint_as_float(e[u].y) * x[u]);
+  }
+  partial[(size_t)w * kSlice + l] = acc.at(0.0f);
+}
+
+// Window of every ray: one thread per ray, the 64 rays of a slice in one
+// wavefront; record k of all lanes of a slice is contiguous.  Lambda starts
+// from the ray's front partial sums, the travelled length from the front's
+// Kahan state.  Pass 1: vpsi_k = exp(Lambda_k) psi(s_k), tau_k = vpsi_k
+// (1 - n_k), T = sum tau (+ V_n when the walk ran out and d lies beyond,
+// gvdbkernels.cu:1196-1208).  Pass 2 restarts at the window with the saved
+// sums and writes p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi,
+// mu0 = P + R / n (gvdbkernels.cu:936-951) times l into the record's own
+// slot of the window region (coalesced; the segment sums gather them in
+// voxel order).  UW records are loaded before their arithmetic.  The file is built without FP contraction, so
+// pass 2 recomputes pass 1's terms bit for bit.
+template <int MI, int MO, int UW>
+__global__ __launch_bounds__(256) void k_ray_window(const RayRec* __restrict__ lanes, const uint32_t* __restrict__ len_w,
+                                                    uint32_t nslices, const uint32_t* __restrict__ chunk_off,
+                                                    const float* __restrict__ partial, const uint2* __restrict__ inc,
+                                                    uint32_t base_w,
+                                                    const typename Msg<MI>::T* __restrict__ Xi,
+                                                    typename Msg<MO>::T* __restrict__ Xo,
+                                                    const float* __restrict__ sigt, const float* __restrict__ biat,
+                                                    int lookup_n) {
   // XCD-aware order: consecutive blocks (neighbouring pixel tiles) share an XCD's L2
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   const uint32_t ray = lb * 256u + threadIdx.x;  // slice * 64 + lane
-  const uint32_t t = ray >> 6;
-  if (t >= nslices) return;  // uniform per wavefront
+  const uint32_t t = ray >> 6, l = ray & 63u;
+  if (t >= nslices) return;
   const uint32_t lw = len_w[t];
   if (lw == 0) return;
   const RayRec rec = lanes[ray];
-  const uint32_t nf = rec.nf, nw = rec.nw;
+  const uint32_t nw = rec.nw;
   if (nw == 0) return;  // no lane-crossing operations below
-  const uint32_t lf = len_f[t];
   auto msg = [&](const uint2 e) -> float { return e.x != kGapVox ? Msg<MI>::load(Xi, e.x) : 0.0f; };
 
-  Kahan kL = {0.f, 0.f}, kS = {0.f, 0.f};
-  const uint2* __restrict__ rf = inc + rec.off_f;
-  for (uint32_t k0 = 0; k0 < lf; k0 += UF) {
-    uint2 e[UF];
-    float x[UF];
-#pragma unroll
-    for (int u = 0; u < UF; ++u) {
-      const uint32_t k = k0 + u;
-      e[u] = k < nf ? rf[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
-    }
-#pragma unroll
-    for (int u = 0; u < UF; ++u) x[u] = msg(e[u]);
-#pragma unroll
-    for (int u = 0; u < UF; ++u) {
-      if (k0 + u < nf) {
-        const float ln = __uint_as_float(e[u].y);
-        kL.add(ln * x[u]);
-        kS.add(ln);
-      }
-    }
-  }
+  Kahan kL = {0.f, 0.f};
+  for (uint32_t c = chunk_off[t]; c < chunk_off[t + 1]; ++c) kL.add(partial[(size_t)c * kSlice + l]);
+  Kahan kS = {rec.ks, rec.kc};
   Poly sig, bia;
   load_cell(sigt, biat, lookup_n, rec.cell_flags & 0x7FFFFFFFu, sig, bia);
   const float z0 = rec.z0, inv = rec.inv_norm, d = rec.depth;
@@ -314,16 +359,14 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
   kL = L0;
   kS = S0;
   Kahan kP = {0.f, 0.f};
-  const uint32_t* __restrict__ rp = win_pos + (rec.off_w - base_w);
+  typename Msg<MO>::T* __restrict__ out = Xo + (rec.off_w - base_w);
   for (uint32_t k0 = 0; k0 < lw; k0 += UW) {
     uint2 e[UW];
-    uint32_t pos[UW];
     float x[UW];
 #pragma unroll
     for (int u = 0; u < UW; ++u) {
       const uint32_t k = k0 + u;
       e[u] = k < nw ? rw[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
-      pos[u] = k < nw ? rp[(size_t)k * kSlice] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < UW; ++u) x[u] = msg(e[u]);
@@ -345,7 +388,7 @@ __global__ __launch_bounds__(256) void k_ray_messages(const RayRec* __restrict__
         const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
         float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
         p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
-        if (live) Msg<MO>::store(Xo, pos[u], p * ln * Msg<MO>::kOut);
+        if (live) Msg<MO>::store(out, (k0 + u) * kSlice, p * ln * Msg<MO>::kOut);
         kL.add(a);
         kS.add(ln);
         kP.add(tau);
@@ -399,11 +442,25 @@ __device__ __forceinline__ int wave_scan_i(int x) {
   return x;
 }
 
-// 8 consecutive messages from p0 (a multiple of 8), zero past j1
+// 8 consecutive messages from p0 (a multiple of 8), zero past j1; with an
+// index array, message j is X[idx[j]]
 template <int MT>
-__device__ __forceinline__ void load8(const typename Msg<MT>::T* __restrict__ X, uint32_t p0, uint32_t j1,
-                                      float (&x)[8]) {
+__device__ __forceinline__ void load8(const typename Msg<MT>::T* __restrict__ X, const uint32_t* __restrict__ idx,
+                                      uint32_t p0, uint32_t j1, float (&x)[8]) {
   typedef typename Msg<MT>::T T;
+  if (idx) {
+    if (p0 + 8 <= j1) {
+      const uint4 a = *reinterpret_cast<const uint4*>(idx + p0);
+      const uint4 b = *reinterpret_cast<const uint4*>(idx + p0 + 4);
+      const uint32_t ix[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = Msg<MT>::load(X, ix[i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = p0 + i < j1 ? Msg<MT>::load(X, idx[p0 + i]) : 0.0f;
+    }
+    return;
+  }
   if (p0 + 8 <= j1) {
     constexpr int kVec = (int)(8 * sizeof(T) / sizeof(uint4));
     union {
@@ -424,6 +481,7 @@ __device__ __forceinline__ void load8(const typename Msg<MT>::T* __restrict__ X,
 template <int MT>
 __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ heads, const uint32_t* __restrict__ tile_h0,
                                                  uint32_t nlive, const typename Msg<MT>::T* __restrict__ X,
+                                                 const uint32_t* __restrict__ idx,
                                                  float* __restrict__ S, uint32_t* __restrict__ pf_q,
                                                  float* __restrict__ pf_v, uint32_t* __restrict__ pl_q,
                                                  float* __restrict__ pl_v, float* __restrict__ mid) {
@@ -436,7 +494,7 @@ __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ he
   // head bits of this lane's 8 positions (zero past nlive)
   const uint32_t hb = p0 < j1 ? (uint32_t)(heads[p0 >> 6] >> (p0 & 63)) & 0xFFu : 0u;
   float x[8];
-  load8<MT>(X, p0, j1, x);
+  load8<MT>(X, idx, p0, j1, x);
   const int nh = __popc(hb);
   const int incl = wave_scan_i(nh);
   const uint32_t before = tile_h0[w] + (uint32_t)(incl - nh);  // heads before p0
@@ -636,11 +694,14 @@ void launch_pair_msgs(const CamPass& a, hipStream_t s) {
 
 template <int MI, int MO, int UF, int UW>
 void launch_ray_t(const CamPass& a, hipStream_t s) {
+  const typename Msg<MI>::T* Xi = (const typename Msg<MI>::T*)a.apair;
+  if (a.nchunks)
+    hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.front, a.lanes, a.len_f,
+                       a.chunk_tile, a.chunk_off, a.nchunks, a.inc, Xi, a.partial);
   const uint32_t blocks = (a.nslices * kSlice + 255) / 256;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
-  hipLaunchKernelGGL((k_ray_messages<MI, MO, UF, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_f, a.len_w,
-                     a.nslices, a.inc, a.win_pos, a.base_w, (const typename Msg<MI>::T*)a.apair,
-                     (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
+  hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
+                     a.partial, a.inc, a.base_w, Xi, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
 }
 
 template <int MI, int MO>
@@ -666,13 +727,15 @@ void launch_ray_messages(const CamPass& a, hipStream_t s) {
   }
 }
 
-void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,
+void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X,
+                            const uint32_t* idx, float* S,
                             uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v, float* mid,
                             const uint32_t* pair_off, hipStream_t s) {
   if (nlive == 0) return;
   const uint32_t ntiles = (nlive + kTile - 1) / kTile;
   dim3 grid(grid_for(ntiles, 4)), block(256);
-  hipLaunchKernelGGL(k_seg_sum<kMsgF32>, grid, block, 0, s, heads, tile_h0, nlive, X, S, pf_q, pf_v, pl_q, pl_v, mid);
+  hipLaunchKernelGGL(k_seg_sum<kMsgF32>, grid, block, 0, s, heads, tile_h0, nlive, X, idx, S, pf_q, pf_v, pl_q, pl_v,
+                     mid);
   hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, pair_off, pl_q, pl_v, pf_v, mid, ntiles, S);
 }
 
@@ -685,37 +748,47 @@ void launch_epilogue(const CamPass& a, hipStream_t s) {
     switch (msg_out_type(a.msg_type)) {
       case kMsgF16:
         hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const __half*)a.Xo,
-                           a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
+                           a.win_idx, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
         hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.wseg_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
         out_scale = Msg<kMsgF16>::kOut;
         break;
       case kMsgBF16:
         hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const uint16_t*)a.Xo,
-                           a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
+                           a.win_idx, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
         hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.wseg_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
         out_scale = Msg<kMsgBF16>::kOut;
         break;
       default:
-        launch_segment_sum_f32(a.heads, a.tile_h0, a.nwin, (const float*)a.Xo, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v,
-                               a.mid, a.wseg_off, s);
+        launch_segment_sum_f32(a.heads, a.tile_h0, a.nwin, (const float*)a.Xo, a.win_idx, a.S, a.pf_q, a.pf_v, a.pl_q,
+                               a.pl_v, a.mid, a.wseg_off, s);
     }
   }
   hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf,
                      a.npairs, a.nwp, a.S, out_scale, a.B, a.M, a.upd);
 }
 
-void launch_split(RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia, int lookup_n,
-                  uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s) {
+void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
+                  int lookup_n, uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s) {
   if (npix == 0) return;
-  hipLaunchKernelGGL(k_split, dim3(grid_for(npix, 256)), dim3(256), 0, s, lanes, npix, sell, sig, bia, lookup_n, cnt_f,
-                     cnt_w);
+  hipLaunchKernelGGL(k_split, dim3(grid_for(npix, 256)), dim3(256), 0, s, front, lanes, npix, sell, sig, bia, lookup_n,
+                     cnt_f, cnt_w);
 }
 
-void launch_reslice(RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f, const uint32_t* len_f,
-                    const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc, hipStream_t s) {
+void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
+                    const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
+                    hipStream_t s) {
   if (npix == 0) return;
-  hipLaunchKernelGGL(k_reslice, dim3(grid_for(npix, 256)), dim3(256), 0, s, lanes, npix, sell, off_f, len_f, off_w,
-                     len_w, base_w, inc);
+  hipLaunchKernelGGL(k_reslice, dim3(grid_for(npix, 256)), dim3(256), 0, s, front, lanes, npix, sell, off_f, len_f,
+                     off_w, len_w, base_w, inc);
+}
+
+void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t nslices, uint32_t* cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_chunk_counts, dim3(grid_for(nslices + 1, 256)), dim3(256), 0, s, len_f, len_w, nslices, cnt);
+}
+
+void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s) {
+  if (nslices == 0) return;
+  hipLaunchKernelGGL(k_chunk_tiles, dim3(grid_for(nslices, 256)), dim3(256), 0, s, chunk_off, nslices, chunk_tile);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

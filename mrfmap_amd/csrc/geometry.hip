@@ -9,10 +9,10 @@
 // src/cuda/gvdbkernels.cu:519-571, 1276-1302) and the sum of the constant
 // messages F_EPS * l of its front incidences (those where the depth
 // likelihood is exactly 0).  The window incidences, in sorted order, form
-// the window segment array of the BP iteration: the ray kernel writes each
-// window message into the incidence's own slot and each pair sums its
-// segment in a fixed order, so the iteration needs no atomics and is
-// bit-reproducible.  Pairs are renumbered with the pairs that have window
+// the window segments of the BP iteration: the ray kernel writes each window
+// message into the incidence's own slot of the window region and each pair
+// sums its segment, gathered through win_idx, in a fixed order, so the
+// iteration needs no atomics and is bit-reproducible.  Pairs are renumbered with the pairs that have window
 // incidences first (both groups in voxel order), so that window segment m
 // belongs to pair m, and every record is rewritten to hold its pair.
 #include <hipcub/hipcub.hpp>
@@ -117,12 +117,12 @@ __global__ void k_permute_pairs(const uint32_t* __restrict__ mark, const uint32_
 __global__ void k_set_records(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ heads,
                               const uint32_t* __restrict__ hscan, const uint32_t* __restrict__ win,
                               const uint32_t* __restrict__ wpos, const uint32_t* __restrict__ perm, uint32_t nlive,
-                              uint32_t base_w, uint2* __restrict__ rec, uint32_t* __restrict__ win_pos) {
+                              uint32_t base_w, uint2* __restrict__ rec, uint32_t* __restrict__ win_idx) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nlive) return;
   const uint32_t i = vals[j];
   rec[i].x = perm[hscan[j] + heads[j] - 1u];
-  if (win[j]) win_pos[i - base_w] = wpos[j];
+  if (win[j]) win_idx[wpos[j]] = i - base_w;
 }
 
 // record -> voxel (for export)
@@ -210,10 +210,10 @@ void launch_permute_pairs(const uint32_t* mark, const uint32_t* rank, uint32_t n
 }
 
 void launch_set_records(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, const uint32_t* perm,
-                        uint32_t nlive, uint32_t base_w, uint32_t* win_pos, hipStream_t s) {
+                        uint32_t nlive, uint32_t base_w, uint32_t* win_idx, hipStream_t s) {
   if (nlive == 0) return;
   hipLaunchKernelGGL(k_set_records, dim3(blocks(nlive, 256)), dim3(256), 0, s, b.vals_out, b.heads, b.hscan, win, wpos,
-                     perm, nlive, base_w, b.rec, win_pos);
+                     perm, nlive, base_w, b.rec, win_idx);
 }
 
 void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_vox, uint2* out, hipStream_t s) {

@@ -27,21 +27,25 @@ constexpr uint32_t kTile = 512;           // positions per wavefront in the segm
 
 constexpr uint32_t kSlice = 64;           // rays per slice (one 8x8 pixel tile, one per lane)
 
-// Per-ray record of one camera's traversal (one per lane of a slice).  A
+// Per-ray records of one camera's traversal (one per lane of a slice).  A
 // ray's records are split into a front part, where the depth likelihood
 // psi(s) is exactly 0 in FP32, and a window part; each part is a sliced-ELL
-// region of its own: front record k at off_f + k * kSlice, window record k
-// (record nf + k of the ray) at off_w + k * kSlice.
-struct RayRec {
-  uint32_t off_f;      // first front record
+// region of its own: front record k at FrontRec::off + k * kSlice, window
+// record k (record nf + k of the ray) at RayRec::off_w + k * kSlice.
+struct FrontRec {
+  uint32_t off;        // first front record
   uint32_t nf;         // front records (voxels + gap records)
+};
+struct RayRec {
   uint32_t off_w;      // first window record
   uint32_t nw;         // window records
   float z0;            // planar depth at the entry of the first record (m)
   float inv_norm;      // metres of planar depth per voxel of chord length
   float depth;         // measured depth d (m)
   uint32_t cell_flags; // sensor-model lookup offset (bits 0..30) | kRanOut
+  float ks, kc;        // Kahan sum of the front chord lengths (sum, compensation)
 };
+constexpr uint32_t kChunk = 64;  // front records per lane summed by one wavefront of the front kernel
 
 // A record belongs to the front while d - (s + bias(s)) > kFrontU * 3 sigma(s)
 // at its exit depth s: then psi(s) = 0.398942 exp(-u^2/2) / (3 sigma) has

@@ -34,7 +34,7 @@ void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_l
 // padded to the slice length with {kGapVox, 0}; lanes[t*64+l] describes the ray
 void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
                  const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
-                 RayRec* lanes, hipStream_t s);
+                 FrontRec* front, RayRec* lanes, hipStream_t s);
 
 // scan.hip: exclusive prefix sum of n uint32 values; total is written to
 // *d_total (device, uint64).  tmp must hold scan_tmp_words(n) uint64 words.
@@ -80,22 +80,28 @@ void launch_mark_window_pairs(const uint32_t* whead, const uint32_t* hw, const u
 void launch_permute_pairs(const uint32_t* mark, const uint32_t* rank, uint32_t npairs, uint32_t nwp,
                           const uint32_t* vox_old, const float* c_old, const float* sf_old, uint32_t* perm,
                           uint32_t* pair_vox, float* pair_c, float* pair_sf, hipStream_t s);
-// records get their pair (new numbering), window records their window position
+// records get their pair (new numbering); win_idx[window position] = the
+// window record's slot (index - base_w)
 void launch_set_records(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, const uint32_t* perm,
-                        uint32_t nlive, uint32_t base_w, uint32_t* win_pos, hipStream_t s);
+                        uint32_t nlive, uint32_t base_w, uint32_t* win_idx, hipStream_t s);
 // records with their voxel ids restored (for export)
 void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_vox, uint2* out, hipStream_t s);
 
 // bp.hip: one camera pass = per-pair voxel->ray messages -> ray messages ->
 // window segment sums -> per-pair update.
 struct CamPass {
+  const FrontRec* front;   // one per lane of every slice
   const RayRec* lanes;     // one per lane of every slice
   const uint32_t* len_f;   // front records per lane of each slice
   const uint32_t* len_w;   // window records per lane of each slice
   uint32_t nslices;
   const uint2* inc;        // sliced records {pair | kGapVox, chord length}: front region, then window region
-  const uint32_t* win_pos; // window position of each window record (index - base_w)
+  const uint32_t* win_idx; // per window position (voxel order): its record's slot in the window region
   uint32_t base_w;         // first record of the window region
+  const uint32_t* chunk_off;   // nslices + 1: first front chunk of each slice
+  const uint32_t* chunk_tile;  // slice of each front chunk
+  uint32_t nchunks;
+  float* partial;          // per chunk and lane: front visibility partial sum
   const uint32_t* pair_vox;
   const float* pair_c;     // summed chord length C per pair
   const float* pair_sf;    // summed constant front messages per pair
@@ -111,7 +117,7 @@ struct CamPass {
   float *pf_v, *pl_v, *mid;
   float* B;
   float* M;                // this camera's messages M_c (FP32, pool sized)
-  void* Xo;                // window segment array of ray->voxel messages p*l (output side type)
+  void* Xo;                // ray->voxel messages p*l in window-region order (output side type)
   uint8_t* upd;
   const float* sig;        // sensor-model lookup tables
   const float* bia;
@@ -119,18 +125,24 @@ struct CamPass {
   int msg_type;
   int ray_sched;           // RaySched
 };
-// front/window split of an emitted camera (records of ray r at r.off_f, all
-// in the front): sets nf/nw and per-pixel counts
-void launch_split(RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia, int lookup_n,
-                  uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s);
+// front/window split of an emitted camera (all records of a ray in its
+// front part): sets nf, nw, the front length sums and per-pixel counts
+void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
+                  int lookup_n, uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s);
 // copy into the final front/window regions; window region starts at base_w
-void launch_reslice(RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f, const uint32_t* len_f,
-                    const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc, hipStream_t s);
+void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
+                    const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
+                    hipStream_t s);
+// front chunks per slice (cnt has nslices + 1 entries, the last 0), and the
+// slice of every chunk from their offsets
+void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t nslices, uint32_t* cnt, hipStream_t s);
+void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s);
 void launch_pair_msgs(const CamPass& a, hipStream_t s);
-// deterministic segment sums of an FP32 position array into S[pair]
-void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X, float* S,
-                            uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v, float* mid,
-                            const uint32_t* pair_off, hipStream_t s);
+// deterministic segment sums of an FP32 position array into S[pair]; with
+// idx, position j holds X[idx[j]]
+void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X,
+                            const uint32_t* idx, float* S, uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v,
+                            float* mid, const uint32_t* pair_off, hipStream_t s);
 void launch_ray_messages(const CamPass& a, hipStream_t s);
 void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);

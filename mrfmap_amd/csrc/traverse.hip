@@ -344,24 +344,27 @@ __global__ void k_slice_len(const uint32_t* __restrict__ counts, uint32_t ntiles
 __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, const float* __restrict__ sig,
                        const float* __restrict__ bia, const int32_t* __restrict__ slot,
                        const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ slice_len,
-                       uint2* __restrict__ sell, RayRec* __restrict__ lanes) {
+                       uint2* __restrict__ sell, FrontRec* __restrict__ front, RayRec* __restrict__ lanes) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= tiled_pixels(c)) return;
   const uint32_t tile = (uint32_t)pix >> 6, lane = (uint32_t)pix & 63u;
   uint2* out = sell + slice_off[tile] + lane;
   PixelCtx p = load_pixel(c, g, depth, pix);
+  FrontRec f;
+  f.off = slice_off[tile] + lane;  // all records in the front part until the split
+  f.nf = 0;
   RayRec r;
-  r.off_f = slice_off[tile] + lane;  // all records in the front part until the split
-  r.nf = 0;
   r.off_w = 0;
   r.nw = 0;
+  r.ks = 0.0f;
+  r.kc = 0.0f;
   r.z0 = 0.0f;
   r.inv_norm = 0.0f;
   r.depth = 0.0f;
   r.cell_flags = 0;
   if (p.ok) {
     WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, out);
-    r.nf = wo.n;
+    f.nf = wo.n;
     r.z0 = wo.z0;
     r.inv_norm = (float)(1.0 / p.rs.norm);
     r.depth = p.d;
@@ -369,7 +372,8 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
   }
   // pad the lane to the slice length with no-op records
   const uint32_t len = slice_len[tile];
-  for (uint32_t k = r.nf; k < len; ++k) out[(size_t)k * kSlice] = make_uint2(kGapVox, 0u);
+  for (uint32_t k = f.nf; k < len; ++k) out[(size_t)k * kSlice] = make_uint2(kGapVox, 0u);
+  front[pix] = f;
   lanes[pix] = r;
 }
 
@@ -404,10 +408,10 @@ void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_l
 
 void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
                  const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
-                 RayRec* lanes, hipStream_t s) {
+                 FrontRec* front, RayRec* lanes, hipStream_t s) {
   const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_emit, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, slice_off,
-                     slice_len, sell, lanes);
+                     slice_len, sell, front, lanes);
 }
 
 }  // namespace mrf

@@ -1,0 +1,149 @@
+"""Profiling helpers for the BP path on MI355X (rocprofv3 output -> summaries).
+
+    python -m mrfmap_amd.profiling copy
+        One plain device-to-device copy of 1 GiB (torch), the reference
+        workload for calibrating the FETCH_SIZE / WRITE_SIZE counters: an
+        ideal copy fetches and writes exactly 1 GiB.
+    python -m mrfmap_amd.profiling summarize DIR [--calib CALDIR] [--kernel NAME]
+        Markdown summary of rocprofv3 runs kept under DIR: trace/ (from
+        --kernel-trace --stats), fetch/ and write/ (from --pmc FETCH_SIZE and
+        --pmc WRITE_SIZE, csv output).  With --calib, CALDIR/fetch and
+        CALDIR/write hold the same counters for the copy above; the counters
+        of every kernel are then scaled by 1 GiB / (counter of the copy).
+        Also prints one JSON line with the calibrated HBM bytes per launch of
+        NAME (default k_ray_messages).
+"""
+import argparse
+import collections
+import csv
+import json
+import os
+import re
+import sys
+
+GIB = 1 << 30
+
+
+def short_name(name):
+    """Kernel name without namespaces and argument lists; template arguments kept."""
+    n = name.replace("mrf::(anonymous namespace)::", "").replace("(anonymous namespace)::", "")
+    n = n.replace("void ", "")
+    n = re.sub(r"\(.*$", "", n)
+    n = n.replace("rocprim::ROCPRIM_400200_NS::detail::", "rocprim::")
+    return n[:70]
+
+
+def _read_csv(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find(d, suffix):
+    for root, _, files in os.walk(d):
+        for fn in files:
+            if fn.endswith(suffix):
+                return os.path.join(root, fn)
+    return None
+
+
+def counters(d):
+    """{kernel short name: [values]} of the single counter collected under d."""
+    path = _find(d, "counter_collection.csv")
+    if path is None:
+        return {}, None
+    out = collections.defaultdict(list)
+    cname = None
+    for r in _read_csv(path):
+        out[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        cname = r["Counter_Name"]
+    return out, cname
+
+
+def kernel_stats(d):
+    path = _find(d, "kernel_stats.csv")
+    return _read_csv(path) if path else []
+
+
+def calibration(caldir):
+    """Scale factors (bytes per counted KiB) for FETCH_SIZE and WRITE_SIZE from the 1 GiB copy."""
+    fac = {}
+    for kind in ("fetch", "write"):
+        c, _ = counters(os.path.join(caldir, kind))
+        vals = [v for k, vs in c.items() if "copy" in k.lower() or "elementwise" in k.lower() for v in vs]
+        if not vals:
+            vals = [max(v for vs in c.values() for v in vs)] if c else []
+        if vals:
+            fac[kind] = GIB / max(vals)
+    return fac
+
+
+def summarize(d, caldir=None, kernel="k_ray_messages", top=16):
+    lines = []
+    stats = kernel_stats(os.path.join(d, "trace"))
+    if stats:
+        lines.append("| kernel | calls | avg ms | total ms | % |")
+        lines.append("|---|---:|---:|---:|---:|")
+        for r in stats[:top]:
+            lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
+                         f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.1f} |")
+    fac = calibration(caldir) if caldir else {}
+    fetch, _ = counters(os.path.join(d, "fetch"))
+    write, _ = counters(os.path.join(d, "write"))
+    traffic = None
+    if fetch or write:
+        lines.append("")
+        if fac:
+            lines.append(f"Calibration (1 GiB copy): FETCH_SIZE x {fac.get('fetch', 1) / 1024:.3f}, "
+                         f"WRITE_SIZE x {fac.get('write', 1) / 1024:.3f} (counted KiB -> KiB)")
+            lines.append("")
+        lines.append("| kernel | launches | FETCH_SIZE KiB/launch | WRITE_SIZE KiB/launch | calibrated GB/launch |")
+        lines.append("|---|---:|---:|---:|---:|")
+        names = sorted(set(fetch) | set(write), key=lambda k: -(sum(fetch.get(k, [0])) / max(1, len(fetch.get(k, [1])))))
+        for k in names[:top]:
+            f = fetch.get(k, [])
+            w = write.get(k, [])
+            fa = sum(f) / len(f) if f else 0.0
+            wa = sum(w) / len(w) if w else 0.0
+            gb = (fa * fac.get("fetch", 1024) + wa * fac.get("write", 1024)) / 1e9
+            lines.append(f"| {k} | {max(len(f), len(w))} | {fa:.4g} | {wa:.4g} | {gb:.3f} |")
+            if kernel in k and (traffic is None or len(f) > traffic["launches"]):
+                traffic = {"kernel": k, "launches": max(len(f), len(w)), "fetch_kib": fa, "write_kib": wa,
+                           "fetch_factor": fac.get("fetch", 1024) / 1024, "write_factor": fac.get("write", 1024) / 1024,
+                           "bytes": fa * fac.get("fetch", 1024) + wa * fac.get("write", 1024)}
+    return "\n".join(lines), traffic
+
+
+def copy_once():
+    import torch
+
+    n = GIB // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied {n * 4} bytes")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    sub.add_parser("copy")
+    s = sub.add_parser("summarize")
+    s.add_argument("dir")
+    s.add_argument("--calib")
+    s.add_argument("--kernel", default="k_ray_messages")
+    a = ap.parse_args(argv)
+    if a.cmd == "copy":
+        copy_once()
+        return 0
+    text, traffic = summarize(a.dir, a.calib, a.kernel)
+    print(text)
+    if traffic:
+        print()
+        print(json.dumps(traffic))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
