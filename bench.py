@@ -71,7 +71,7 @@ def measured_traffic(a):
     1 GiB copy), or None when it does not cover this configuration."""
     if a.workload != "real" or a.msg != "f32":
         return None, None
-    path = os.path.join(ROOT, "profiles", "k_ray_messages_real_f32.json")
+    path = os.path.join(ROOT, "profiles", "ray_kernels_real_13kf_f32_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -163,7 +163,7 @@ def main():
             "pairs_per_step": st["pairs_per_iter"],
             "rays": st["rays"],
             "list_records": st["list_records"],
-            "ray_schedule_lanes_per_ray_x_loads": list(_lib.RAY_SCHEDULES[st["ray_sched"]]),
+            "ray_schedule_records_ahead_front_x_window": list(_lib.RAY_SCHEDULES[st["ray_sched"]]),
             "autotune_ms": st["t_autotune_ms"],
         },
         "time_to_converged_ms": conv["t_total_ms"],
@@ -171,7 +171,7 @@ def main():
         "converged_residual": conv["residual"],
         "geometry_ms": conv["t_geometry_ms"],
         "roofline": {
-            "kernel": "k_ray_messages",
+            "kernel": "k_front_sums + k_ray_window (one camera pass)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -4,14 +4,15 @@
         One plain device-to-device copy of 1 GiB (torch), the reference
         workload for calibrating the FETCH_SIZE / WRITE_SIZE counters: an
         ideal copy fetches and writes exactly 1 GiB.
-    python -m mrfmap_amd.profiling summarize DIR [--calib CALDIR] [--kernel NAME]
+    python -m mrfmap_amd.profiling summarize DIR [--calib CALDIR] [--kernels A,B]
         Markdown summary of rocprofv3 runs kept under DIR: trace/ (from
         --kernel-trace --stats), fetch/ and write/ (from --pmc FETCH_SIZE and
         --pmc WRITE_SIZE, csv output).  With --calib, CALDIR/fetch and
         CALDIR/write hold the same counters for the copy above; the counters
         of every kernel are then scaled by 1 GiB / (counter of the copy).
         Also prints one JSON line with the calibrated HBM bytes per launch of
-        NAME (default k_ray_messages).
+        the kernel families in --kernels (default: the front-sum and window
+        kernels of one camera pass), summed.
 """
 import argparse
 import collections
@@ -77,8 +78,9 @@ def calibration(caldir):
     return fac
 
 
-def summarize(d, caldir=None, kernel="k_ray_messages", top=16):
+def summarize(d, caldir=None, kernels=("k_front_sums", "k_ray_window"), top=16):
     lines = []
+    parts = []
     stats = kernel_stats(os.path.join(d, "trace"))
     if stats:
         lines.append("| kernel | calls | avg ms | total ms | % |")
@@ -106,10 +108,22 @@ def summarize(d, caldir=None, kernel="k_ray_messages", top=16):
             wa = sum(w) / len(w) if w else 0.0
             gb = (fa * fac.get("fetch", 1024) + wa * fac.get("write", 1024)) / 1e9
             lines.append(f"| {k} | {max(len(f), len(w))} | {fa:.4g} | {wa:.4g} | {gb:.3f} |")
-            if kernel in k and (traffic is None or len(f) > traffic["launches"]):
-                traffic = {"kernel": k, "launches": max(len(f), len(w)), "fetch_kib": fa, "write_kib": wa,
-                           "fetch_factor": fac.get("fetch", 1024) / 1024, "write_factor": fac.get("write", 1024) / 1024,
-                           "bytes": fa * fac.get("fetch", 1024) + wa * fac.get("write", 1024)}
+        # per kernel family (the variant launched most often: the schedule in use)
+        for pat in kernels:
+            best = None
+            for k in names:
+                f = fetch.get(k, [])
+                w = write.get(k, [])
+                if pat in k and (best is None or len(f) > best["launches"]):
+                    fa = sum(f) / len(f) if f else 0.0
+                    wa = sum(w) / len(w) if w else 0.0
+                    best = {"kernel": k, "launches": len(f), "fetch_kib": fa, "write_kib": wa,
+                            "bytes": fa * fac.get("fetch", 1024) + wa * fac.get("write", 1024)}
+            if best:
+                parts.append(best)
+    if parts:
+        traffic = {"kernels": parts, "fetch_factor": fac.get("fetch", 1024) / 1024,
+                   "write_factor": fac.get("write", 1024) / 1024, "bytes": sum(p["bytes"] for p in parts)}
     return "\n".join(lines), traffic
 
 
@@ -132,12 +146,13 @@ def main(argv=None):
     s = sub.add_parser("summarize")
     s.add_argument("dir")
     s.add_argument("--calib")
-    s.add_argument("--kernel", default="k_ray_messages")
+    s.add_argument("--kernels", default="k_front_sums,k_ray_window",
+                   help="comma-separated kernel families whose per-launch bytes are summed")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_once()
         return 0
-    text, traffic = summarize(a.dir, a.calib, a.kernel)
+    text, traffic = summarize(a.dir, a.calib, tuple(a.kernels.split(",")))
     print(text)
     if traffic:
         print()

@@ -101,3 +101,43 @@ def test_recorded_frame_fixture():
         assert T[2, 3] == pytest.approx(2.0)
         R = T[:3, :3]
         np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-6)
+
+
+CPP_CALLER = r"""
+#include <cstdio>
+#include "mrfmap_hip.hpp"
+int main() {
+  mrfmap_hip::Params p;
+  p.res = 0.04f;
+  std::printf("defaults res=%.2f prior=%.2f max_iters=%d lookup_n=%d\n", p.res, p.prior, p.max_iters, p.lookup_n);
+  try {
+    mrfmap_hip::GVDBInference inf(p);
+    std::printf("created cameras=%u bricks=%lld\n", inf.num_cameras(), (long long)inf.num_bricks());
+  } catch (const mrfmap_hip::Error& e) {
+    std::printf("error %d\n", e.code());
+  }
+  return 0;
+}
+"""
+
+
+def test_cpp_wrapper_compiles_links_and_reports(tmp_path):
+    """include/mrfmap_hip.hpp (the GVDBInference-named C++ surface) builds
+    with the host compiler against libmrfmap_hip.so; without a device the
+    constructor throws MRF_E_HIP, with one it creates an empty context."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "caller.cpp"
+    src.write_text(CPP_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L", libdir, "-lmrfmap_hip", f"-Wl,-rpath,{libdir}"], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=120).stdout
+    assert "defaults res=0.04 prior=" in out
+    if _has_gpu():
+        assert "created cameras=0 bricks=0" in out
+    else:
+        assert "error -2" in out
