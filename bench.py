@@ -40,18 +40,9 @@ def parse():
 def cpu_baseline(wl):
     """FP64 scalar oracle on keyframes 0 and 6 of the same workload, every 4th
     pixel row; one BP iteration, timed on one host core."""
-    import numpy as np
-
     import oracle
-    from mrfmap_amd import synth
 
-    p, frames, _ = synth.make_workload(wl, frames=13)
-    sub = []
-    for k in (0, 6):
-        pose, K, d = frames[k]
-        dd = np.zeros_like(d)
-        dd[::4] = d[::4]
-        sub.append((pose, K, dd))
+    p, sub = _sample(wl)
     o = oracle.Oracle(p.K, p.rows, p.cols, p.res, p.prior, p.dims, p.const_sigmasq_poly, p.const_bias_poly,
                       lookup_n=p.lookup_n, activation=p.activation)
     for pose, K, d in sub:
@@ -63,6 +54,51 @@ def cpu_baseline(wl):
     return {"value": st["updates_per_iter"] / dt, "unit": "ray-msg updates/s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle BP, 1 iteration over keyframes 0 and 6 of '{wl}' with every 4th pixel row "
                       f"({st['updates_per_iter']} updates, {dt:.2f} s)"}
+
+
+def _sample(wl, keyframes=(0, 6), row_step=4):
+    """Some keyframes of the workload with every row_step-th pixel row kept."""
+    import numpy as np
+
+    from mrfmap_amd import synth
+
+    p, frames, _ = synth.make_workload(wl, frames=13)
+    sub = []
+    for k in keyframes:
+        pose, K, d = frames[k]
+        dd = np.zeros_like(d)
+        dd[::row_step] = d[::row_step]
+        sub.append((pose, K, dd))
+    return p, sub
+
+
+def cpu_comparator(wl):
+    """The reference's CPU comparator path, GVDBOctomapWrapper insertion
+    (GVDBOctomapWrapper.cpp:31-43): an OctoMap-equivalent restatement (OcTree
+    at the map resolution, hit 0.7 / miss 0.4, clamping 0.0001 / 0.9999,
+    bounding box = the map) inserting each keyframe's point cloud from its
+    camera centre, one host core, on keyframe 0 with every 8th pixel row."""
+    import numpy as np
+
+    import oracle
+
+    p, sub = _sample(wl, keyframes=(0,), row_step=8)
+    tree = oracle.OctomapEq(p.res, p.dims, p.occ_thresh)
+    clouds = []
+    for pose, K, d in sub:
+        v, u = np.nonzero(d > 0)
+        z = d[v, u].astype(np.float64)
+        cam = np.stack([(u - K[0, 2]) / K[0, 0] * z, (v - K[1, 2]) / K[1, 1] * z, z], axis=1)
+        world = cam @ pose[:3, :3].astype(np.float64).T + pose[:3, 3].astype(np.float64)
+        clouds.append((world.astype(np.float32), pose[:3, 3].astype(np.float64)))
+    t0 = time.perf_counter()
+    for pts, origin in clouds:
+        tree.insert_point_cloud(pts, origin)
+    dt = time.perf_counter() - t0
+    n = sum(len(c[0]) for c in clouds)
+    return {"value": n / dt, "unit": "rays inserted/s", "cores": 1, "kind": "port",
+            "sample": f"OctoMap-equivalent insertion of keyframe 0 of '{wl}' with every 8th pixel row "
+                      f"({n} rays, {dt:.2f} s)"}
 
 
 def measured_traffic(a):
@@ -137,10 +173,12 @@ def main():
     print(f"[bench] {a.steps} steps in {dt * 1e3:.1f} ms; ray kernel {st['t_ray_kernel_ms']:.1f} ms, "
           f"epilogue {st['t_epilogue_ms']:.1f} ms", file=sys.stderr, flush=True)
 
-    cpu = None
+    cpu = comp = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a.workload)
         print(f"[bench] cpu baseline {cpu['value']:.3e} updates/s", file=sys.stderr, flush=True)
+        comp = cpu_comparator(a.workload)
+        print(f"[bench] cpu comparator {comp['value']:.3e} rays/s", file=sys.stderr, flush=True)
 
     out = {
         "metric": "ray-msg updates/s; time-to-converged map (\u22641e-3 vs FP64), warm vs plain BP",
@@ -185,6 +223,7 @@ def main():
             "alg_bytes_per_window_update": window_b,
         },
         "cpu_baseline": cpu,
+        "cpu_comparator": comp,
     }
     print(json.dumps(out))
 

@@ -865,6 +865,22 @@ int mrf_add_camera_with_depth_k(mrf_ctx* c, const float pose[16], const float K[
 int mrf_set_pose(mrf_ctx* c, uint32_t id, const float pose[16], int layout) {
   if (!c || !pose) return MRF_E_INVALID;
   if (id >= c->cams.size()) return fail(c, MRF_E_RANGE, "set_pose: camera id out of range");
+  // remove the camera's rays: voxels whose summed chord length over the other
+  // cameras' traversals is below 0.05 return to the prior
+  // (GVDBInference.cpp:66-82, reset_empty_voxels, gvdbkernels.cu:498-517);
+  // d_A serves as the per-voxel sum (every inference recomputes it first)
+  const size_t nv = c->slot_lin.size() * (size_t)kBrickVox;
+  if (nv && c->d_A) {
+    launch_fill_f32(c->d_A, 0.0f, nv, c->stream);
+    for (size_t ci = 0; ci < c->cams.size(); ++ci)
+      if (ci != id) launch_cum_add(c->cams[ci].d_pair_vox, c->cams[ci].d_pair_c, c->cams[ci].npairs, c->d_A, c->stream);
+    float* Mid = nullptr;
+    if (c->d_M && c->M_stride == nv && (id + 1) * nv * sizeof(float) <= c->M_bytes)
+      Mid = (float*)((char*)c->d_M + id * c->M_stride * sizeof(float));
+    launch_reset_empty(c->d_A, nv, c->prior_logodds, c->d_B, c->d_upd, Mid, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+  }
   Camera& cam = c->cams[id];
   if (layout == MRF_COL_MAJOR) {
     for (int i = 0; i < 4; ++i)

@@ -620,6 +620,25 @@ __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__
   upd[v] = 1;
 }
 
+// Summed chord length per voxel over one camera's pairs (a camera's pairs
+// are distinct voxels, so cameras launched one after another need no atomics).
+__global__ void k_cum_add(const uint32_t* __restrict__ pair_vox, const float* __restrict__ pair_c, uint32_t npairs,
+                          float* __restrict__ cum) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < npairs) cum[pair_vox[q]] += pair_c[q];
+}
+
+// Voxels that no remaining ray reaches go back to the prior, read 0 and lose
+// the moved camera's message (reset_empty_voxels, gvdbkernels.cu:498-517).
+__global__ void k_reset_empty(const float* __restrict__ cum, size_t n, float prior_logodds, float* __restrict__ B,
+                              uint8_t* __restrict__ upd, float* __restrict__ M) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n || !(cum[v] < 0.05f)) return;
+  B[v] = prior_logodds;
+  upd[v] = 0;
+  if (M) M[v] = 0.0f;
+}
+
 // marginal occupancy 1 - clamp(1/(1+e^B)) (gvdbkernels.cu:469-476)
 __device__ __forceinline__ float marginal(float b) {
   float neg = 1.0f / (1.0f + expf(b));
@@ -806,6 +825,17 @@ void launch_residual(const float* B, float* A, size_t n, unsigned long long* d_m
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_alphas, dim3(grid_for(n, 256)), dim3(256), 0, s, B, upd, out, n);
+}
+
+void launch_cum_add(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, float* cum, hipStream_t s) {
+  if (npairs == 0) return;
+  hipLaunchKernelGGL(k_cum_add, dim3(grid_for(npairs, 256)), dim3(256), 0, s, pair_vox, pair_c, npairs, cum);
+}
+
+void launch_reset_empty(const float* cum, size_t n, float prior_logodds, float* B, uint8_t* upd, float* M,
+                        hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_reset_empty, dim3(grid_for(n, 256)), dim3(256), 0, s, cum, n, prior_logodds, B, upd, M);
 }
 
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s) {

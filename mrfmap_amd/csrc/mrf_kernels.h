@@ -148,6 +148,11 @@ void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
 void launch_residual(const float* B, float* Aprev, size_t nvox, unsigned long long* d_max, hipStream_t s);
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t nvox, hipStream_t s);
+// set_pose: summed chord lengths of a camera's pairs into cum; then voxels
+// with cum < 0.05 back to the prior (M may be null)
+void launch_cum_add(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, float* cum, hipStream_t s);
+void launch_reset_empty(const float* cum, size_t n, float prior_logodds, float* B, uint8_t* upd, float* M,
+                        hipStream_t s);
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void launch_zero_words(void* p, size_t bytes, hipStream_t s);

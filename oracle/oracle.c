@@ -377,8 +377,34 @@ int orc_add_camera(orc_ctx* c, const float pose[16], const float K[9], const flo
   return c->ncams - 1;
 }
 
+/* set_pose (GVDBInference.cpp:66-82): the camera's rays are removed from the
+ * summed chord lengths and every voxel that no remaining ray reaches (summed
+ * length of the other cameras' traversals < 0.05) returns to the prior with
+ * its marginal reading 0 and the camera's message cleared
+ * (reset_empty_voxels, gvdbkernels.cu:498-517); then the pose changes and
+ * the camera is traversed again at the next inference.  The reference reads
+ * the lengths from its single cum-len channel, which holds only the last
+ * camera pass's lengths at that point; the other cameras' summed lengths are
+ * what its comment ("voxels that have no ray passing through them") asks for. */
 int orc_set_pose(orc_ctx* c, int cam, const float pose[16]) {
   if (!c || cam < 0 || cam >= c->ncams) return -5;
+  const int64_t nv = c->nslots * 512;
+  if (nv > 0) {
+    double* cum = (double*)calloc((size_t)nv, sizeof(double));
+    for (int ci = 0; ci < c->ncams; ++ci) {
+      if (ci == cam) continue;
+      const ocam* k = &c->cams[ci];
+      for (int64_t q = 0; q < k->npairs; ++q) cum[k->pair_vox[q]] += k->pair_c[q];
+    }
+    double* M = c->cams[cam].M;
+    for (int64_t v = 0; v < nv; ++v)
+      if (cum[v] < 0.05) {
+        c->B[v] = c->prior_lo;
+        c->upd[v] = 0;
+        if (M) M[v] = 0.0;
+      }
+    free(cum);
+  }
   set_pose(&c->cams[cam], pose);
   c->dirty = 1;
   return 0;

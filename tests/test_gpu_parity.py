@@ -200,3 +200,28 @@ def test_cameras_added_between_calls():
     err = np.abs(g.get_alphas().astype(np.float64) - o.alphas()).max()
     print(f"cameras added between calls: max-abs {err:.3e}")
     assert err <= MARG_TOL
+
+
+def test_set_pose_then_inference_matches_oracle():
+    """set_pose (GVDBInference.cpp:66-82) then perform_inference, the viewer's
+    keyframe-pose update (Viewer.cpp:961-966): voxels no other camera reaches
+    return to the prior at once, the moved camera is traversed again."""
+    p, frames, _ = synth.make_workload("small")
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.perform_inference(max_iters=2)
+    o.perform_inference(2)
+    moved = frames[1][0].copy()
+    moved[0, 3] += 0.05
+    moved[1, 3] -= 0.03
+    g.set_pose(1, moved)
+    o.set_pose(1, moved)
+    assert g.num_bricks() == o.num_bricks()
+    err0 = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    np.testing.assert_array_equal(g.get_alphas() == 0, o.alphas() == 0)
+    g.perform_inference(max_iters=2)
+    o.perform_inference(2)
+    _compare_traversal(g, o, len(frames))
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"set_pose: after reset max-abs {err0.max():.3e}, after inference {err.max():.3e}")
+    assert err0.max() <= MARG_TOL and err.max() <= MARG_TOL

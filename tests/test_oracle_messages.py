@@ -155,3 +155,48 @@ def test_gap_records_do_not_attenuate():
     p2, T2 = oracle.ray_messages(nin2, [1, 5, 1, 1], s2, 1.0, [0.02, 0, 0], live=[1, 0, 1, 1], ran_out=False)
     assert T2 == pytest.approx(T, rel=1e-14)
     np.testing.assert_allclose(p2[[0, 2, 3]], p, rtol=1e-14)
+
+
+def test_set_pose_resets_voxels_no_other_camera_reaches():
+    """set_pose (GVDBInference.cpp:66-82): after the moved camera's rays are
+    removed, voxels whose summed chord length over the other cameras is below
+    0.05 return to the prior and read 0 (reset_empty_voxels,
+    gvdbkernels.cu:498-517); voxels the other cameras still reach keep their
+    beliefs."""
+    from mrfmap_amd import synth
+    from helpers import oracle_for
+
+    p, frames, _ = synth.make_workload("small", frames=2)
+    o = oracle_for(p, frames)
+    o.perform_inference(2)
+    B0 = o.beliefs().ravel().copy()
+    prior = np.float64(np.log(np.float32(p.prior)) - np.log(np.float32(1 - p.prior)))
+    # voxels the second camera reaches, from its traversal
+    _, rec = o.traversal(1)
+    vox = rec[:, 0][rec[:, 0] != 0xFFFFFFFF]
+    length = rec[:, 1][rec[:, 0] != 0xFFFFFFFF].view(np.float32).astype(np.float64)
+    cum = np.bincount(vox, weights=length, minlength=B0.size)
+    moved = frames[0][0].copy()
+    moved[0, 3] += 0.1
+    o.set_pose(0, moved)
+    # the reset is immediate; reading re-traverses the moved camera, whose new
+    # bricks are appended after the existing ones
+    B1 = o.beliefs().ravel()[:B0.size]
+    reset = cum < 0.05
+    np.testing.assert_array_equal(B1[reset], np.full(reset.sum(), prior))
+    np.testing.assert_array_equal(B1[~reset], B0[~reset])
+    assert reset.sum() > 0 and (~reset).sum() > 0
+    a = o.alphas().ravel()[:B0.size]
+    assert (a[reset] == 0).all()
+
+
+def test_set_pose_single_camera_clears_the_map():
+    from mrfmap_amd import synth
+    from helpers import oracle_for
+
+    p, frames, _ = synth.make_workload("small", frames=1)
+    o = oracle_for(p, frames)
+    o.perform_inference(2)
+    assert (o.alphas() != 0).any()
+    o.set_pose(0, frames[0][0])
+    assert (o.alphas() == 0).all()
