@@ -142,6 +142,36 @@ int mrf_get_alphas(mrf_ctx* ctx, float* out, int64_t rows, int64_t cols);
  * counterpart (the reference keeps B in a half2 atlas channel). */
 int mrf_get_beliefs(mrf_ctx* ctx, float* out, int64_t rows, int64_t cols);
 
+/* Map evaluation from a camera: every pixel's ray is walked through the
+ * current marginals to the end of the map.  Replaces
+ * GVDBInference::compute_likelihood, get_likelihood_image[_at_pose],
+ * get_expected_depth_image and get_accuracy_image_at_pose
+ * (GVDBInference.cpp:180-232, GVDBInference.h:86-102), and the simple
+ * accuracy of GVDBMapLikelihoodEstimator; kernels gvdbkernels.cu:135-354,
+ * 1066-1134, 1246-1274, 1332-1363, reductions gvdbhelper.cu:10-63.  Images
+ * are rows x cols float, row-major, NULL to skip:
+ *   loglik          log p(z | map) per pixel, 0 where the depth is invalid;
+ *   expected_depth  sum of ray weights x voxel depth / (1 - vis_inf + F_EPS);
+ *   accuracy        1 / 0 when the ray's weight peak lies within / beyond
+ *                   1 sigma of the measurement, -1 invalid or no peak;
+ *   simple_accuracy first voxel with marginal >= acc_thresh within one voxel
+ *                   size of the measurement 1, elsewhere 2, none before it 0,
+ *                   invalid -1. */
+typedef struct mrf_eval {
+  double log_likelihood;      /* sum of the per-pixel log-likelihoods (sum_likelihood) */
+  double accuracy;            /* #accuracy 1 / (#pixels - #accuracy -1) (compute_accuracy) */
+  double simple_accuracy;     /* (#simple 1 + #simple -1) / #pixels (compute_tp_fp_tn_fn) */
+  int64_t accuracy_counts[3]; /* pixels with accuracy -1, 0, 1 */
+  int64_t simple_counts[4];   /* pixels with simple accuracy -1 (tn), 0 (fn), 1 (tp), 2 (fp) */
+} mrf_eval;
+int mrf_evaluate_camera(mrf_ctx* ctx, uint32_t cam_id, float* loglik, float* expected_depth, float* accuracy,
+                        float* simple_accuracy, mrf_eval* out);
+/* Same from any pose and depth image (K NULL: the params' intrinsics; depth
+ * NULL: only expected_depth is meaningful). */
+int mrf_evaluate_at_pose(mrf_ctx* ctx, const float pose[16], const float K[9], const float* depth, int rows,
+                         int cols, int layout, float* loglik, float* expected_depth, float* accuracy,
+                         float* simple_accuracy, mrf_eval* out);
+
 /* Traversal lists of one camera (for parity tests and tools).  No
  * counterpart: GVDB recomputed the DDA in every pass.  Ray records are
  * 6 x uint32 {off, n, z0 bits, inv_norm bits, depth bits, cell | ran_out<<31};

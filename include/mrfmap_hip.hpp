@@ -122,6 +122,28 @@ class GVDBInference {
     get_alphas(a.data(), n, 512);
     return a;
   }
+  /* compute_likelihood(id) / get_likelihood_image(id, img) (GVDBInference.cpp:180-193):
+   * sum of the per-pixel log-likelihoods; img (rows x cols, row-major) may be null */
+  float compute_likelihood(uint32_t id, float* img = nullptr) {
+    mrf_eval e;
+    check(mrf_evaluate_camera(ctx_, id, img, nullptr, nullptr, nullptr, &e), ctx_, "compute_likelihood");
+    return (float)e.log_likelihood;
+  }
+  /* get_expected_depth_image(id, img) (GVDBInference.cpp:226-231) */
+  void get_expected_depth_image(uint32_t id, float* img) {
+    check(mrf_evaluate_camera(ctx_, id, nullptr, img, nullptr, nullptr, nullptr), ctx_, "get_expected_depth_image");
+  }
+  /* get_likelihood_image_at_pose / get_accuracy_image_at_pose: every image and
+   * reduction of one evaluation from any pose (K null: the params' intrinsics) */
+  mrf_eval evaluate_at_pose(const float* pose, const float* depth, int rows, int cols, float* loglik,
+                            float* expected_depth, float* accuracy, float* simple_accuracy, const float* K = nullptr,
+                            int layout = MRF_COL_MAJOR) {
+    mrf_eval e;
+    check(mrf_evaluate_at_pose(ctx_, pose, K, depth, rows, cols, layout, loglik, expected_depth, accuracy,
+                               simple_accuracy, &e),
+          ctx_, "evaluate_at_pose");
+    return e;
+  }
   mrf_ctx* handle() { return ctx_; }
 
  private:

@@ -24,7 +24,7 @@ EXPORTS = [
     "mrf_params_default", "mrf_create", "mrf_destroy", "mrf_add_camera_with_depth",
     "mrf_add_camera_with_depth_k", "mrf_set_pose", "mrf_num_cameras", "mrf_perform_inference",
     "mrf_num_bricks", "mrf_get_occupied_bricks", "mrf_get_alphas", "mrf_get_beliefs", "mrf_prepare",
-    "mrf_get_traversal", "mrf_last_error",
+    "mrf_get_traversal", "mrf_last_error", "mrf_evaluate_camera", "mrf_evaluate_at_pose",
 ]
 
 
@@ -52,6 +52,18 @@ class MrfStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MrfEval(ctypes.Structure):
+    _fields_ = [
+        ("log_likelihood", ctypes.c_double), ("accuracy", ctypes.c_double), ("simple_accuracy", ctypes.c_double),
+        ("accuracy_counts", ctypes.c_int64 * 3), ("simple_counts", ctypes.c_int64 * 4),
+    ]
+
+    def as_dict(self):
+        return {"log_likelihood": self.log_likelihood, "accuracy": self.accuracy,
+                "simple_accuracy": self.simple_accuracy, "accuracy_counts": list(self.accuracy_counts),
+                "simple_counts": list(self.simple_counts)}
 
 
 class MrfError(RuntimeError):
@@ -91,6 +103,9 @@ def load():
     L.mrf_get_beliefs.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64]
     L.mrf_prepare.argtypes = [P]
     L.mrf_get_traversal.argtypes = [P, ctypes.c_uint32, i64p, i64p, P, P]
+    L.mrf_evaluate_camera.argtypes = [P, ctypes.c_uint32, P, P, P, P, ctypes.POINTER(MrfEval)]
+    L.mrf_evaluate_at_pose.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P,
+                                       ctypes.POINTER(MrfEval)]
     L.mrf_last_error.argtypes = [P]
     L.mrf_last_error.restype = ctypes.c_char_p
     _lib = L

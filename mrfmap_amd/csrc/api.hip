@@ -104,6 +104,12 @@ struct mrf_ctx {
   size_t cap_pscr = 0;
   uint2* d_sell0 = nullptr;
   size_t cap_sell0 = 0;
+  // map evaluation scratch: 4 images + depth, FP64 block sums, class counts
+  float* d_eval = nullptr;
+  size_t cap_eval = 0;
+  double* d_eval_part = nullptr;
+  size_t cap_eval_part = 0;
+  unsigned long long* d_eval_counts = nullptr;
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
 
@@ -718,6 +724,61 @@ int autotune(mrf_ctx* c) {
   return MRF_OK;
 }
 
+// Evaluate the current map from camera geometry cg (depth already on the
+// device, or null), images copied back to the non-null outputs.
+int evaluate(mrf_ctx* c, const CamGeom& cg, const float* h_depth, const float* d_depth, float* out_ll, float* out_ed,
+             float* out_acc, float* out_sacc, mrf_eval* out) {
+  int rc;
+  if ((rc = prepare(c))) return rc;
+  if (!c->d_B) return fail(c, MRF_E_STATE, "evaluate: the map is empty (no keyframe added)");
+  const size_t n = (size_t)cg.rows * cg.cols;
+  const size_t need = 5 * n;
+  if (need > c->cap_eval) {
+    dfree(&c->d_eval);
+    c->cap_eval = 0;
+    if ((rc = dalloc(c, &c->d_eval, need))) return rc;
+    c->cap_eval = need;
+  }
+  const unsigned nb = eval_blocks(cg);
+  if (nb > c->cap_eval_part) {
+    dfree(&c->d_eval_part);
+    c->cap_eval_part = 0;
+    if ((rc = dalloc(c, &c->d_eval_part, nb))) return rc;
+    c->cap_eval_part = nb;
+  }
+  if (!c->d_eval_counts && (rc = dalloc(c, &c->d_eval_counts, 8))) return rc;
+  float* img[4] = {c->d_eval, c->d_eval + n, c->d_eval + 2 * n, c->d_eval + 3 * n};
+  const float* dep = d_depth;
+  if (h_depth) {
+    float* up = c->d_eval + 4 * n;
+    HIPCHK(c, hipMemcpyAsync(up, h_depth, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    dep = up;
+  }
+  launch_zero_words(c->d_eval_counts, 8 * sizeof(unsigned long long), c->stream);
+  launch_eval(cg, c->g, dep, c->d_sig, c->d_bia, c->d_slot, c->d_B, c->d_upd, c->p.acc_thresh, img[0], img[1], img[2],
+              img[3], c->d_eval_part, c->d_eval_counts, c->stream);
+  HIPCHK(c, hipGetLastError());
+  float* outs[4] = {out_ll, out_ed, out_acc, out_sacc};
+  for (int k = 0; k < 4; ++k)
+    if (outs[k]) HIPCHK(c, hipMemcpyAsync(outs[k], img[k], n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> part(nb);
+  unsigned long long cnt[8];
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->d_eval_part, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cnt, c->d_eval_counts, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out) {
+    double ll = 0.0;
+    for (double v : part) ll += v;
+    out->log_likelihood = ll;
+    for (int k = 0; k < 3; ++k) out->accuracy_counts[k] = (int64_t)cnt[k];
+    for (int k = 0; k < 4; ++k) out->simple_counts[k] = (int64_t)cnt[3 + k];
+    const double den = (double)n - (double)cnt[0];
+    out->accuracy = den > 0 ? (double)cnt[2] / den : 0.0;
+    out->simple_accuracy = n ? ((double)cnt[5] + (double)cnt[3]) / (double)n : 0.0;
+  }
+  return MRF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -842,6 +903,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_heads); dfree(&c->d_hscan);
   dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
   dfree(&c->d_pscr); dfree(&c->d_sell0);
+  dfree(&c->d_eval); dfree(&c->d_eval_part); dfree(&c->d_eval_counts);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
   dfree(&c->d_scan_tmp); dfree(&c->d_total);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1089,6 +1151,37 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
     ++r;
   }
   return MRF_OK;
+}
+
+int mrf_evaluate_camera(mrf_ctx* c, uint32_t id, float* ll, float* ed, float* acc, float* sacc, mrf_eval* out) {
+  if (!c) return MRF_E_INVALID;
+  if (id >= c->cams.size()) return fail(c, MRF_E_RANGE, "evaluate_camera: camera id out of range");
+  const CamGeom cg = make_geom(c, c->cams[id]);
+  return evaluate(c, cg, nullptr, c->cams[id].d_depth, ll, ed, acc, sacc, out);
+}
+
+int mrf_evaluate_at_pose(mrf_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols,
+                         int layout, float* ll, float* ed, float* acc, float* sacc, mrf_eval* out) {
+  if (!c || !pose || rows <= 0 || cols <= 0 || (layout != MRF_ROW_MAJOR && layout != MRF_COL_MAJOR))
+    return fail(c, MRF_E_INVALID, "evaluate_at_pose: invalid argument");
+  const float* k = K ? K : c->p.K;
+  if (k[0] <= 0 || k[4] <= 0) return fail(c, MRF_E_INVALID, "evaluate_at_pose: focal lengths must be positive");
+  int rc = ensure_tables(c, rows, cols);
+  if (rc) return rc;
+  Camera cam;
+  if (layout == MRF_COL_MAJOR) {
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) cam.pose[i * 4 + j] = pose[j * 4 + i];
+  } else {
+    std::memcpy(cam.pose, pose, sizeof(cam.pose));
+  }
+  std::memcpy(cam.K, k, sizeof(cam.K));
+  cam.rows = rows;
+  cam.cols = cols;
+  std::vector<float> d;
+  if (depth) to_row_major(depth, rows, cols, layout, d);
+  const CamGeom cg = make_geom(c, cam);
+  return evaluate(c, cg, depth ? d.data() : nullptr, nullptr, ll, ed, acc, sacc, out);
 }
 
 const char* mrf_last_error(mrf_ctx* c) { return c ? c->err.c_str() : "null context"; }

@@ -36,6 +36,16 @@ void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const 
                  const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
                  FrontRec* front, RayRec* lanes, hipStream_t s);
 
+// map evaluation of one camera's pixels (log-likelihood, expected depth,
+// accuracy and simple accuracy images in row-major pixel order, any may be
+// null); part gets eval_blocks(c) FP64 partial sums of the log-likelihoods,
+// counts (7, zeroed by the caller) the accuracy classes -1, 0, 1 and the
+// simple accuracy classes -1, 0, 1, 2
+unsigned eval_blocks(const CamGeom& c);
+void launch_eval(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
+                 const int32_t* slot, const float* B, const uint8_t* upd, float acc_thresh, float* ll, float* ed,
+                 float* acc, float* sacc, double* part, unsigned long long* counts, hipStream_t s);
+
 // scan.hip: exclusive prefix sum of n uint32 values; total is written to
 // *d_total (device, uint64).  tmp must hold scan_tmp_words(n) uint64 words.
 size_t scan_tmp_words(size_t n);

@@ -102,7 +102,7 @@ __device__ __forceinline__ uint32_t lookup_offset(const CamGeom& c, int lookup_n
   return (uint32_t)((v / kLookupPx) * (c.lookup_u * lookup_n) + (x / kLookupPx) * lookup_n);
 }
 
-enum WalkMode { kMark = 0, kCount = 1, kEmit = 2 };
+enum WalkMode { kMark = 0, kCount = 1, kEmit = 2, kVisit = 3 };
 
 struct WalkOut {
   uint32_t n;
@@ -112,10 +112,14 @@ struct WalkOut {
 
 // Records of one ray are written out[0], out[kSlice], out[2 kSlice], ...
 // (record k of the 64 rays of a pixel tile is contiguous: sliced ELL).
-template <int MODE>
+// kVisit walks to the end of the grid without truncation and hands every
+// voxel of an active brick to vis->voxel(id, ix, chord, z entry, z exit) and
+// every active brick's exit depth to vis->brick_end(z); a true return from
+// either ends the walk.
+template <int MODE, class V = int>
 __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, const float* sig,
                         const float* bia, const int32_t* __restrict__ slot,
-                        uint8_t* __restrict__ bflags, uint2* __restrict__ out) {
+                        uint8_t* __restrict__ bflags, uint2* __restrict__ out, V* vis = nullptr) {
   WalkOut wo = {0u, 0.0f, true};
   const int N[3] = {g.nx, g.ny, g.nz};
   Rat zin = {0, 1}, zout = {0, 1};
@@ -202,11 +206,15 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
           wo.z0 = (float)zc_d;
           any = true;
         }
-        if (MODE == kEmit) {
+        if constexpr (MODE == kEmit || MODE == kVisit) {
           const uint32_t local = (uint32_t)((((ix[0] & 7) << 3) | (ix[1] & 7)) << 3 | (ix[2] & 7));
           const uint32_t vox = (uint32_t)bslot * kBrickVox + local;
           float len = (float)((zn_d - zc_d) * rs.norm);
-          out[(size_t)cnt * kSlice] = make_uint2(vox, __float_as_uint(len));
+          if constexpr (MODE == kEmit) {
+            out[(size_t)cnt * kSlice] = make_uint2(vox, __float_as_uint(len));
+          } else {
+            if (vis->voxel(vox, ix, len, zc_d, zn_d)) break;
+          }
         }
         ++cnt;
         last_exit = zn_d;
@@ -227,13 +235,17 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
                               ((nx[1] >> kBrickLog) != (ix[1] >> kBrickLog)) ||
                               ((nx[2] >> kBrickLog) != (ix[2] >> kBrickLog));
     if (brick_change && active) {
-      const float* sc = sig;
-      const float* bc = bia;
-      const double lhs = zn_d + poly_eval(bc, lookup_n, zn_d);
-      const double rhs = depth + 3.0 * poly_eval(sc, lookup_n, zn_d);
-      if (lhs >= rhs) {
-        wo.ran_out = false;
-        break;
+      if constexpr (MODE == kVisit) {
+        if (vis->brick_end(zn_d)) break;
+      } else {
+        const float* sc = sig;
+        const float* bc = bia;
+        const double lhs = zn_d + poly_eval(bc, lookup_n, zn_d);
+        const double rhs = depth + 3.0 * poly_eval(sc, lookup_n, zn_d);
+        if (lhs >= rhs) {
+          wo.ran_out = false;
+          break;
+        }
       }
     }
     if (leaving) break;
@@ -377,9 +389,201 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
   lanes[pix] = r;
 }
 
+// ---------------------------------------------------------------- map evaluation
+// One pixel ray walked through the current map to the end of the grid (no
+// truncation), every voxel of an active brick read as its marginal
+// occupancy p (0 where never updated, gvdbkernels.cu:469-476).  Per voxel:
+// alpha = -ln(1 - p), Lambda += -alpha l, vis = e^Lambda, weight
+// vis_prev - vis (Crispell's visibility model, gvdbkernels.cu:230-236).
+//  * log-likelihood (rayLikelihoodBrick + render_likelihood_kernel,
+//    gvdbkernels.cu:183-293, 1246-1274): sum of weight x p(z | voxel), with
+//    p(z | voxel) = 0 below Z_MIN, (1 - LIKE_EPS) (psi(s) + psi(s_prev)) / 2
+//    within 3 sigma of the measured depth, LIKE_EPS elsewhere; plus
+//    vis_inf x (d >= Z_MAX ? 1 - LIKE_EPS : LIKE_EPS); the pixel's value is
+//    its log; invalid depth gives 0.
+//  * expected depth (rayExpectedDepthBrick + render_expected_depth_kernel,
+//    gvdbkernels.cu:295-354, 1332-1363): sum of weight x the planar depth of
+//    the voxel centre, over 1 - vis_inf + F_EPS; every pixel.
+//  * accuracy (render_accuracy_kernel, gvdbkernels.cu:1066-1104): at the
+//    voxel with the largest weight per unit length, 1 if its depth is within
+//    1 sigma of the measurement else 0; -1 for an invalid pixel, a ray that
+//    met nothing, or one that kept vis >= 0.9 and ended short of the
+//    measurement (0 if it ended beyond it).
+//  * simple accuracy (raySimpleAccuracyBrick + render_simple_accuracy_kernel,
+//    gvdbkernels.cu:135-181, 1106-1134): the first voxel with
+//    p >= acc_thresh gives 1 (within one voxel size of the measured depth) or
+//    2; an active brick ending at or beyond the measured depth first gives 0;
+//    otherwise -1.
+// The prefix depth s is the planar depth at the voxel's exit; at an active
+// brick's end the previous depth carried into the next brick is s + bias(s)
+// (the reference passes it on in norm.x).
+constexpr float kZMin = 0.1f;
+constexpr float kLikeEps = 1e-3f;
+
+__device__ __forceinline__ float poly_f(const float* c, int n, float s) {
+  return n == 3 ? (s * s) * c[2] + s * c[1] + c[0] : s * c[1] + c[0];
+}
+
+__device__ __forceinline__ float psi_f(float s, float d, const float* sc, const float* bc, int n) {
+  const float sg3 = 3.0f * poly_f(sc, n, s);
+  const float u = (d - s - poly_f(bc, n, s)) / sg3;
+  return 0.398942f * expf(-0.5f * u * u) / sg3;
+}
+
+struct EvalVisitor {
+  const float* B;
+  const uint8_t* upd;
+  const float* sc;
+  const float* bc;
+  int ln;
+  float d, res, acc_thresh;
+  bool valid;
+  double R2[3], t[3], off[3];  // camera z axis in world, centre, world->voxel offset
+  float prev_vis, lam, sum, prev_s, norm_x, peak, esum, sacc;
+  int peak_ok;
+  bool first, sdone;
+
+  __device__ float marginal_of(uint32_t v) const {
+    if (!upd[v]) return 0.0f;
+    float neg = 1.0f / (1.0f + expf(B[v]));
+    neg = fminf(fmaxf(neg, kFEps), 1.0f - kFEps);
+    return 1.0f - neg;
+  }
+  __device__ bool voxel(uint32_t v, const int ix[3], float len, double z0, double z1) {
+    const float p = marginal_of(v);
+    const float s = (float)z1;
+    if (first) {
+      prev_s = (float)z0;
+      first = false;
+    }
+    const float alpha = -logf(1.0f - p);
+    lam += -alpha * len;
+    const float vis = expf(lam);
+    const float w = (prev_vis - vis) / len;
+    if (valid) {
+      const float pdm = s + poly_f(bc, ln, s) - d;
+      const float sg = poly_f(sc, ln, s);
+      float prob;
+      if (s < kZMin)
+        prob = 0.0f;
+      else if (fabsf(pdm) <= 3.0f * sg)
+        prob = (1.0f - kLikeEps) * 0.5f * (psi_f(s, d, sc, bc, ln) + psi_f(prev_s, d, sc, bc, ln));
+      else
+        prob = kLikeEps;
+      if (w > peak) {
+        peak = w;
+        peak_ok = fabsf(pdm) <= sg ? 1 : 0;
+      }
+      sum += (prev_vis - vis) * prob;
+      norm_x = pdm + d;
+      if (!sdone && p >= acc_thresh) {
+        sacc = fabsf(s - d) <= res ? 1.0f : 2.0f;
+        sdone = true;
+      }
+    }
+    double zc = 0.0;
+    for (int a = 0; a < 3; ++a) zc += R2[a] * ((((double)ix[a] + 0.5) - off[a]) * res - t[a]);
+    esum += (prev_vis - vis) * (float)zc;
+    prev_vis = vis;
+    prev_s = s;
+    return false;
+  }
+  __device__ bool brick_end(double z) {
+    if (valid) {
+      prev_s = norm_x;
+      if (!sdone && (float)z >= d) {
+        sacc = 0.0f;
+        sdone = true;
+      }
+    }
+    return false;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_eval(CamGeom c, GridGeom g, const float* __restrict__ depth,
+                                              const float* __restrict__ sigt, const float* __restrict__ biat,
+                                              const int32_t* __restrict__ slot, const float* __restrict__ B,
+                                              const uint8_t* __restrict__ upd, float acc_thresh,
+                                              float* __restrict__ out_ll, float* __restrict__ out_ed,
+                                              float* __restrict__ out_acc, float* __restrict__ out_sacc,
+                                              double* __restrict__ part, unsigned long long* __restrict__ counts) {
+  __shared__ double sh[256];
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  int x = 0, y = 0;
+  const bool in = pix < tiled_pixels(c) && tile_pixel(c, pix, &x, &y);
+  float ll = 0.0f, ed = 0.0f, acc = -1.0f, sacc = -1.0f;
+  if (in) {
+    const float d = depth ? depth[y * c.cols + x] : 0.0f;
+    EvalVisitor v;
+    v.B = B;
+    v.upd = upd;
+    const uint32_t cell = lookup_offset(c, g.lookup_n, x, y);
+    v.sc = sigt + cell;
+    v.bc = biat + cell;
+    v.ln = g.lookup_n;
+    v.d = d;
+    v.res = (float)g.res;
+    v.acc_thresh = acc_thresh;
+    v.valid = depth != nullptr && pixel_valid(c, x, y, d);
+    for (int a = 0; a < 3; ++a) {
+      v.R2[a] = c.R[a * 3 + 2];
+      v.t[a] = c.t[a];
+      v.off[a] = g.off[a];
+    }
+    v.prev_vis = 1.0f;
+    v.lam = v.sum = v.prev_s = v.norm_x = v.peak = v.esum = 0.0f;
+    v.sacc = -1.0f;
+    v.peak_ok = 0;
+    v.first = true;
+    v.sdone = false;
+    RaySetup rs;
+    if (setup_ray(c, g, x, y, rs)) walk<kVisit>(g, rs, 0.0, nullptr, nullptr, slot, nullptr, nullptr, &v);
+    ed = v.esum / (1.0f - v.prev_vis + kFEps);
+    if (v.valid) {
+      ll = logf(v.sum + v.prev_vis * (d >= kZMax ? 1.0f - kLikeEps : kLikeEps));
+      if (v.peak > 0.0f && v.prev_vis >= 0.9f)
+        acc = v.norm_x <= d ? -1.0f : 0.0f;
+      else if (v.peak > 0.0f)
+        acc = (float)v.peak_ok;
+      sacc = v.sacc;
+    }
+    const int o = y * c.cols + x;
+    if (out_ll) out_ll[o] = ll;
+    if (out_ed) out_ed[o] = ed;
+    if (out_acc) out_acc[o] = acc;
+    if (out_sacc) out_sacc[o] = sacc;
+  }
+  // class counts: accuracy -1, 0, 1 then simple accuracy -1, 0, 1, 2 (pixels of the image only)
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const float want = k < 3 ? (float)(k - 1) : (float)(k - 4);
+    const float val = k < 3 ? acc : sacc;
+    const unsigned long long m = __ballot(in && val == want);
+    if (lane == 0 && m) atomicAdd(counts + k, (unsigned long long)__popcll(m));
+  }
+  // fixed-order FP64 block sum of the log-likelihoods
+  sh[threadIdx.x] = (double)ll;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 }  // namespace
 
 int num_tiled_pixels(const CamGeom& c) { return tiled_pixels(c); }
+
+unsigned eval_blocks(const CamGeom& c) { return (unsigned)((tiled_pixels(c) + 255) / 256); }
+
+void launch_eval(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
+                 const int32_t* slot, const float* B, const uint8_t* upd, float acc_thresh, float* ll, float* ed,
+                 float* acc, float* sacc, double* part, unsigned long long* counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_eval, dim3(eval_blocks(c)), dim3(256), 0, s, c, g, depth, sig, bia, slot, B, upd, acc_thresh, ll,
+                     ed, acc, sacc, part, counts);
+}
 
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
                       const float* bia, uint8_t* bflags, hipStream_t s) {

@@ -46,6 +46,7 @@ class GVDBInference:
         self._h = ctypes.c_void_p()
         _lib.check(self._L.mrf_create(ctypes.byref(self._cp), ctypes.byref(self._h)))
         self.last_stats = None
+        self._cam_shapes = {}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -71,6 +72,7 @@ class GVDBInference:
             rc = self._L.mrf_add_camera_with_depth_k(self._h, _ptr(pose), _ptr(Kc), _ptr(depth), depth.shape[0],
                                                      depth.shape[1], _lib.ROW_MAJOR, ctypes.byref(cid))
         _lib.check(rc, self._h)
+        self._cam_shapes[cid.value] = depth.shape
         return cid.value
 
     def add_camera(self, pose, image):
@@ -142,3 +144,64 @@ class GVDBInference:
         recs = np.zeros((nc.value, 2), dtype=np.uint32)
         _lib.check(self._L.mrf_get_traversal(self._h, int(cam_id), None, None, _ptr(rays), _ptr(recs)), self._h)
         return rays, recs
+
+    # ------------------------------------------------------------ map evaluation
+    def _cam_shape(self, cam_id):
+        return self._cam_shapes.get(int(cam_id), (self.params.rows, self.params.cols))
+
+    def evaluate(self, cam_id=None, pose=None, depth=None, K=None, shape=None):
+        """Per-pixel log-likelihood, expected depth, accuracy and simple
+        accuracy images of the current map seen from camera ``cam_id`` or from
+        ``pose`` with ``depth`` (row-major), and the image-level reductions."""
+        if cam_id is not None:
+            rows, cols = shape or self._cam_shape(cam_id)
+        else:
+            rows, cols = (np.asarray(depth).shape if depth is not None else (shape or (self.params.rows, self.params.cols)))
+        imgs = [np.zeros((rows, cols), dtype=np.float32) for _ in range(4)]
+        ev = _lib.MrfEval()
+        if cam_id is not None:
+            rc = self._L.mrf_evaluate_camera(self._h, int(cam_id), *[_ptr(i) for i in imgs], ctypes.byref(ev))
+        else:
+            pose = np.ascontiguousarray(np.asarray(pose, dtype=np.float32).reshape(4, 4))
+            d = None if depth is None else np.ascontiguousarray(np.asarray(depth, dtype=np.float32))
+            k = None if K is None else np.ascontiguousarray(np.asarray(K, dtype=np.float32).reshape(3, 3))
+            rc = self._L.mrf_evaluate_at_pose(self._h, _ptr(pose), None if k is None else _ptr(k),
+                                              None if d is None else _ptr(d), rows, cols, _lib.ROW_MAJOR,
+                                              *[_ptr(i) for i in imgs], ctypes.byref(ev))
+        _lib.check(rc, self._h)
+        out = ev.as_dict()
+        out.update(loglik=imgs[0], expected_depth=imgs[1], accuracy_image=imgs[2], simple_accuracy_image=imgs[3])
+        return out
+
+    def compute_likelihood(self, cam_id):
+        """Sum of the per-pixel log-likelihoods of camera ``cam_id``'s depth
+        image under the current map (GVDBInference.cpp:180-186)."""
+        return self.evaluate(cam_id)["log_likelihood"]
+
+    def get_likelihood_image(self, cam_id, out):
+        r = self.evaluate(cam_id, shape=out.shape)
+        out[...] = r["loglik"]
+        return r["log_likelihood"]
+
+    def get_likelihood_image_at_pose(self, pose, image, out):
+        depth = image.depth if isinstance(image, GVDBImage) else image
+        r = self.evaluate(pose=pose, depth=depth)
+        out[...] = r["loglik"]
+        return r["log_likelihood"]
+
+    def get_expected_depth_image(self, cam_id, out):
+        out[...] = self.evaluate(cam_id, shape=out.shape)["expected_depth"]
+
+    def get_accuracy_image_at_pose(self, pose, image, out):
+        depth = image.depth if isinstance(image, GVDBImage) else image
+        r = self.evaluate(pose=pose, depth=depth)
+        out[...] = r["accuracy_image"]
+        return r["accuracy"]
+
+    def get_simple_accuracy_at_pose(self, pose, image):
+        """(accuracy, tp, fp, tn, fn) of the first-voxel-above-acc_thresh test
+        (compute_tp_fp_tn_fn, gvdbhelper.cu:40-63)."""
+        depth = image.depth if isinstance(image, GVDBImage) else image
+        r = self.evaluate(pose=pose, depth=depth)
+        tn, fn, tp, fp = r["simple_counts"]
+        return r["simple_accuracy"], tp, fp, tn, fn

@@ -66,6 +66,8 @@ def lib():
         L.orc_destroy.argtypes = [P]
         L.orc_add_camera.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int]
         L.orc_set_pose.argtypes = [P, ctypes.c_int, P]
+        L.orc_set_beliefs.argtypes = [P, P, P]
+        L.orc_evaluate.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P, P, P]
         L.orc_prepare.argtypes = [P]
         L.orc_num_bricks.argtypes = [P]
         L.orc_num_bricks.restype = ctypes.c_int64
@@ -149,6 +151,33 @@ class Oracle:
 
     def prepare(self):
         lib().orc_prepare(self._h)
+
+    def set_beliefs(self, B, upd):
+        """Test hook: beliefs (num_bricks x 512 log-odds) and update flags."""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        upd = np.ascontiguousarray(upd, dtype=np.uint8)
+        if lib().orc_set_beliefs(self._h, _ptr(B), _ptr(upd)) != 0:
+            raise RuntimeError("orc_set_beliefs failed")
+
+    def evaluate(self, pose, depth, K=None, acc_thresh=0.5, shape=None):
+        """Map evaluation images and their reductions (sum of log-likelihoods,
+        accuracy = #1 / (#pixels - #-1), simple accuracy class counts)."""
+        pose = np.ascontiguousarray(pose, dtype=np.float32).reshape(16)
+        Kc = self.K if K is None else np.ascontiguousarray(K, dtype=np.float32).reshape(9)
+        d = None if depth is None else np.ascontiguousarray(depth, dtype=np.float32)
+        rows, cols = d.shape if d is not None else shape
+        imgs = [np.zeros((rows, cols), dtype=np.float64) for _ in range(4)]
+        if lib().orc_evaluate(self._h, _ptr(pose), _ptr(Kc), None if d is None else _ptr(d), rows, cols,
+                              float(acc_thresh), *[_ptr(i) for i in imgs]) != 0:
+            raise RuntimeError("orc_evaluate failed")
+        ll, ed, acc, sacc = imgs
+        n = rows * cols
+        acc_counts = [int((acc == k).sum()) for k in (-1, 0, 1)]
+        simple_counts = [int((sacc == k).sum()) for k in (-1, 0, 1, 2)]
+        return {"loglik": ll, "expected_depth": ed, "accuracy_image": acc, "simple_accuracy_image": sacc,
+                "log_likelihood": float(ll.sum()), "accuracy_counts": acc_counts, "simple_counts": simple_counts,
+                "accuracy": acc_counts[2] / (n - acc_counts[0]) if n > acc_counts[0] else 0.0,
+                "simple_accuracy": (simple_counts[2] + simple_counts[0]) / n}
 
     def num_bricks(self):
         return int(lib().orc_num_bricks(self._h))

@@ -812,6 +812,131 @@ int orc_perform_inference(orc_ctx* c, const int* ids, int n_ids, int max_iters, 
   return it;
 }
 
+/* Map evaluation from a camera, FP64 (the device path is FP32): every pixel
+ * ray walked through the current marginals to the end of the grid.
+ * Restates rayLikelihoodBrick / render_likelihood_kernel
+ * (gvdbkernels.cu:183-293, 1246-1274), rayExpectedDepthBrick /
+ * render_expected_depth_kernel (:295-354, 1332-1363), render_accuracy_kernel
+ * (:1066-1104) and raySimpleAccuracyBrick / render_simple_accuracy_kernel
+ * (:135-181, 1106-1134).  Images rows x cols, row-major, any may be NULL. */
+#define ORC_Z_MIN 0.1
+#define ORC_Z_MAX 10.0
+#define ORC_LIKE_EPS 1e-3
+
+static double psi_d(double s, double d, const double sig[3], const double bia[3], int ln) {
+  double sg3 = 3.0 * poly(sig, ln, s);
+  double u = (d - s - poly(bia, ln, s)) / sg3;
+  return 0.398942 * exp(-0.5 * u * u) / sg3;
+}
+
+int orc_evaluate(orc_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols,
+                 double acc_thresh, double* ll, double* ed, double* acc, double* sacc) {
+  if (!c || !pose || !K || rows <= 0 || cols <= 0 || orc_prepare(c)) return -1;
+  ocam k;
+  memset(&k, 0, sizeof(k));
+  set_pose(&k, pose);
+  k.fx = K[0];
+  k.fy = K[4];
+  k.cx = K[2];
+  k.cy = K[5];
+  k.rows = rows;
+  k.cols = cols;
+  k.lookup_u = (cols + LOOKUP_PX - 1) / LOOKUP_PX;
+  const int ln = c->p.lookup_n;
+  brickmap bm = {c->B3[0], c->B3[1], c->B3[2], c->slot, NULL};
+  segv sv = {0};
+  for (int y = 0; y < rows; ++y)
+    for (int x = 0; x < cols; ++x) {
+      const size_t o = (size_t)y * cols + x;
+      const double d = depth ? (double)depth[o] : 0.0;
+      const int valid = depth && pixel_ok(c, &k, x, y, (float)d);
+      double prev_vis = 1.0, lam = 0.0, sum = 0.0, prev_s = 0.0, norm_x = 0.0, peak = 0.0, esum = 0.0;
+      double sa = -1.0;
+      int peak_ok = 0, first = 1, sdone = 0;
+      int64_t Oi[3], Di[3];
+      double norm;
+      if (setup_ray(c, &k, x, y, Oi, Di, &norm)) {
+        ray_segments(Oi, Di, c->N, &sv);
+        for (int j = 0; j < sv.n; ++j) {
+          const seg_t* e = &sv.s[j];
+          const int lin = brick_lin(&bm, e->v);
+          if (c->slot[lin] < 0) continue;
+          const double z0 = rdbl(e->z0), z1 = rdbl(e->z1);
+          const double len = (double)(float)((z1 - z0) * norm);
+          const uint32_t local = (uint32_t)(((e->v[0] & 7) * 8 + (e->v[1] & 7)) * 8 + (e->v[2] & 7));
+          const int64_t v = (int64_t)c->slot[lin] * 512 + local;
+          const double p = c->upd[v] ? marginal(c->B[v]) : 0.0;
+          const double s = z1;
+          if (first) {
+            prev_s = z0;
+            first = 0;
+          }
+          lam += log(1.0 - p) * len;
+          const double vis = exp(lam);
+          const double w = (prev_vis - vis) / len;
+          if (valid) {
+            const double pdm = s + poly(c->bia, ln, s) - d;
+            const double sg = poly(c->sig, ln, s);
+            double prob;
+            if (s < ORC_Z_MIN)
+              prob = 0.0;
+            else if (fabs(pdm) <= 3.0 * sg)
+              prob = (1.0 - ORC_LIKE_EPS) * 0.5 * (psi_d(s, d, c->sig, c->bia, ln) + psi_d(prev_s, d, c->sig, c->bia, ln));
+            else
+              prob = ORC_LIKE_EPS;
+            if (w > peak) {
+              peak = w;
+              peak_ok = fabs(pdm) <= sg;
+            }
+            sum += (prev_vis - vis) * prob;
+            norm_x = pdm + d;
+            if (!sdone && p >= acc_thresh) {
+              sa = fabs(s - d) <= c->res ? 1.0 : 2.0;
+              sdone = 1;
+            }
+          }
+          double zc = 0.0;
+          for (int a = 0; a < 3; ++a) zc += k.R[a * 3 + 2] * ((((double)e->v[a] + 0.5) - c->off[a]) * c->res - k.t[a]);
+          esum += (prev_vis - vis) * zc;
+          prev_vis = vis;
+          prev_s = s;
+          const int brick_exit = (j + 1 == sv.n) || brick_lin(&bm, sv.s[j + 1].v) != lin;
+          if (brick_exit && valid) {
+            prev_s = norm_x;
+            if (!sdone && s >= d) {
+              sa = 0.0;
+              sdone = 1;
+            }
+          }
+        }
+      }
+      if (ed) ed[o] = esum / (1.0 - prev_vis + F_EPS);
+      double l = 0.0, a = -1.0;
+      if (valid) {
+        l = log(sum + prev_vis * (d >= ORC_Z_MAX ? 1.0 - ORC_LIKE_EPS : ORC_LIKE_EPS));
+        if (peak > 0.0 && prev_vis >= 0.9)
+          a = norm_x <= d ? -1.0 : 0.0;
+        else if (peak > 0.0)
+          a = peak_ok;
+      } else {
+        sa = -1.0;
+      }
+      if (ll) ll[o] = l;
+      if (acc) acc[o] = a;
+      if (sacc) sacc[o] = sa;
+    }
+  free(sv.s);
+  return 0;
+}
+
+/* Test hook: overwrite the beliefs and update flags of the pool (nslots x 512). */
+int orc_set_beliefs(orc_ctx* c, const double* B, const uint8_t* upd) {
+  if (!c || orc_prepare(c)) return -1;
+  memcpy(c->B, B, sizeof(double) * (size_t)(c->nslots * 512));
+  memcpy(c->upd, upd, (size_t)(c->nslots * 512));
+  return 0;
+}
+
 int orc_get_alphas(orc_ctx* c, double* out) {
   if (!c || orc_prepare(c)) return -1;
   for (int64_t v = 0; v < c->nslots * 512; ++v) out[v] = c->upd[v] ? marginal(c->B[v]) : 0.0;

@@ -48,6 +48,11 @@ void orc_destroy(orc_ctx* c);
 /* pose row-major 4x4 camera-to-world; K row-major 3x3; depth row-major */
 int orc_add_camera(orc_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols);
 int orc_set_pose(orc_ctx* c, int cam, const float pose[16]);
+/* test hook: beliefs and update flags of the whole pool */
+int orc_set_beliefs(orc_ctx* c, const double* B, const uint8_t* upd);
+/* map evaluation images (log-likelihood, expected depth, accuracy, simple accuracy) */
+int orc_evaluate(orc_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols,
+                 double acc_thresh, double* ll, double* ed, double* acc, double* sacc);
 /* activation + slot assignment + traversal of every camera */
 int orc_prepare(orc_ctx* c);
 int64_t orc_num_bricks(orc_ctx* c);

@@ -225,3 +225,37 @@ def test_set_pose_then_inference_matches_oracle():
     err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
     print(f"set_pose: after reset max-abs {err0.max():.3e}, after inference {err.max():.3e}")
     assert err0.max() <= MARG_TOL and err.max() <= MARG_TOL
+
+
+@pytest.mark.parametrize("shift", [0.0, 0.08])
+def test_map_evaluation_matches_oracle(shift):
+    """Log-likelihood, expected depth, accuracy and simple accuracy images of
+    the map after 3 BP iterations, seen from keyframe 0 (shift 0: the
+    camera itself) or 8 cm beside it, against the FP64 oracle.  The map
+    itself differs by <= 1e-4 (FP32 BP), so the continuous images agree to
+    a few 1e-3 and the threshold classes flip for at most a few pixels."""
+    p, frames, _ = synth.make_workload("small")
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.perform_inference(max_iters=3)
+    o.perform_inference(3)
+    pose, K, depth = frames[0]
+    if shift:
+        pose = pose.copy()
+        pose[0, 3] += shift
+        rg = g.evaluate(pose=pose, depth=depth, K=K)
+    else:
+        rg = g.evaluate(0)
+    ro = o.evaluate(pose, depth, K, acc_thresh=p.acc_thresh)
+    dll = np.abs(rg["loglik"].astype(np.float64) - ro["loglik"])
+    ded = np.abs(rg["expected_depth"].astype(np.float64) - ro["expected_depth"])
+    flips_acc = float(np.mean(rg["accuracy_image"] != ro["accuracy_image"]))
+    flips_s = float(np.mean(rg["simple_accuracy_image"] != ro["simple_accuracy_image"]))
+    print(f"shift {shift}: loglik max {dll.max():.2e} p99.9 {np.percentile(dll, 99.9):.2e}; expected depth max "
+          f"{ded.max():.2e}; class flips {flips_acc:.2e} / {flips_s:.2e}; sums {rg['log_likelihood']:.6g} vs "
+          f"{ro['log_likelihood']:.6g}")
+    assert np.percentile(dll, 99.9) <= 1e-2 and np.median(dll) <= 1e-4
+    assert np.percentile(ded, 99.9) <= 2e-3
+    assert flips_acc <= 2e-3 and flips_s <= 2e-3
+    assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-4 * abs(ro["log_likelihood"]) + 1.0
+    assert abs(rg["accuracy"] - ro["accuracy"]) <= 2e-3
