@@ -172,6 +172,18 @@ int mrf_evaluate_at_pose(mrf_ctx* ctx, const float pose[16], const float K[9], c
                          int cols, int layout, float* loglik, float* expected_depth, float* accuracy,
                          float* simple_accuracy, mrf_eval* out);
 
+/* Loads an external occupancy map (GVDBMapLikelihoodEstimator::load_map,
+ * GVDBMapLikelihoodEstimator.cpp:168-240, fed by
+ * GVDBOctomapWrapper::push_to_gvdb_volume, GVDBOctomapWrapper.cpp:45-74):
+ * points n x 3 row-major in voxel units (world / res + (dims_x/2, dims_y/2, 0)),
+ * occupancy probabilities n.  Bricks within one voxel of each point inside
+ * the map box are activated; the voxel holding a point gets belief
+ * logit(occupancy) (the last point of a voxel wins) and reads back as that
+ * marginal, clamped to [F_EPS, 1 - F_EPS]. */
+int mrf_load_map(mrf_ctx* ctx, const float* points, const float* occupancies, int64_t n);
+/* Thresholds of the map evaluation (GVDBMapLikelihoodEstimator::set_acc_thresh). */
+int mrf_set_eval_thresholds(mrf_ctx* ctx, float occ_thresh, float acc_thresh);
+
 /* Traversal lists of one camera (for parity tests and tools).  No
  * counterpart: GVDB recomputed the DDA in every pass.  Ray records are
  * 6 x uint32 {off, n, z0 bits, inv_norm bits, depth bits, cell | ran_out<<31};

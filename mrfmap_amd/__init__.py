@@ -6,6 +6,6 @@ include/mrfmap_hip.h; HIP kernels for gfx950 in mrfmap_amd/csrc/).
 """
 from . import _lib  # noqa: F401
 from .params import gvdb_params, global_params  # noqa: F401
-from .inference import GVDBImage, GVDBInference  # noqa: F401
+from .inference import GVDBImage, GVDBInference, GVDBMapLikelihoodEstimator  # noqa: F401
 
-__all__ = ["gvdb_params", "global_params", "GVDBImage", "GVDBInference"]
+__all__ = ["gvdb_params", "global_params", "GVDBImage", "GVDBInference", "GVDBMapLikelihoodEstimator"]

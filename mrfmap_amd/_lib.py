@@ -24,7 +24,8 @@ EXPORTS = [
     "mrf_params_default", "mrf_create", "mrf_destroy", "mrf_add_camera_with_depth",
     "mrf_add_camera_with_depth_k", "mrf_set_pose", "mrf_num_cameras", "mrf_perform_inference",
     "mrf_num_bricks", "mrf_get_occupied_bricks", "mrf_get_alphas", "mrf_get_beliefs", "mrf_prepare",
-    "mrf_get_traversal", "mrf_last_error", "mrf_evaluate_camera", "mrf_evaluate_at_pose",
+    "mrf_get_traversal", "mrf_last_error", "mrf_evaluate_camera", "mrf_evaluate_at_pose", "mrf_load_map",
+    "mrf_set_eval_thresholds",
 ]
 
 
@@ -106,6 +107,8 @@ def load():
     L.mrf_evaluate_camera.argtypes = [P, ctypes.c_uint32, P, P, P, P, ctypes.POINTER(MrfEval)]
     L.mrf_evaluate_at_pose.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P,
                                        ctypes.POINTER(MrfEval)]
+    L.mrf_load_map.argtypes = [P, P, P, ctypes.c_int64]
+    L.mrf_set_eval_thresholds.argtypes = [P, ctypes.c_float, ctypes.c_float]
     L.mrf_last_error.argtypes = [P]
     L.mrf_last_error.restype = ctypes.c_char_p
     _lib = L

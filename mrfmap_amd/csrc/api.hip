@@ -1184,6 +1184,93 @@ int mrf_evaluate_at_pose(mrf_ctx* c, const float pose[16], const float K[9], con
   return evaluate(c, cg, depth ? d.data() : nullptr, nullptr, ll, ed, acc, sacc, out);
 }
 
+int mrf_load_map(mrf_ctx* c, const float* points, const float* occupancies, int64_t n) {
+  if (!c || n < 0 || (n > 0 && (!points || !occupancies))) return fail(c, MRF_E_INVALID, "load_map: invalid argument");
+  int rc;
+  if ((rc = prepare(c))) return rc;
+  const GridGeom& g = c->g;
+  const int M[3] = {g.mx, g.my, g.mz}, N[3] = {g.nx, g.ny, g.nz};
+  // 1. bricks within one voxel of every point in the map box
+  std::vector<uint8_t> flags((size_t)c->nbricks, 0);
+  std::vector<int64_t> cell((size_t)n, -1);  // voxel of each point as x, y, z packed, -1 outside
+  for (int64_t i = 0; i < n; ++i) {
+    int lo[3], hi[3], v[3];
+    bool ok = true;
+    for (int a = 0; a < 3 && ok; ++a) {
+      const double P = (double)points[3 * i + a];
+      if (!(P >= 0.0 && P < (double)M[a])) {
+        ok = false;
+        break;
+      }
+      v[a] = (int)std::floor(P);
+      lo[a] = std::max(0, (int)std::floor(P - 1.0)) >> kBrickLog;
+      hi[a] = std::min(N[a] - 1, (int)std::floor(P + 1.0)) >> kBrickLog;
+    }
+    if (!ok) continue;
+    for (int x = lo[0]; x <= hi[0]; ++x)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int z = lo[2]; z <= hi[2]; ++z) flags[((size_t)x * g.by + y) * g.bz + z] = 1;
+    cell[i] = ((int64_t)v[0] * N[1] + v[1]) * N[2] + v[2];
+  }
+  bool grew = false;
+  for (int lin = 0; lin < c->nbricks; ++lin)
+    if (flags[lin] && c->h_slot[lin] < 0) {
+      c->h_slot[lin] = (int32_t)c->slot_lin.size();
+      c->slot_lin.push_back(lin);
+      grew = true;
+    }
+  if ((rc = ensure_pool(c, c->slot_lin.size()))) return rc;
+  if (grew) {
+    HIPCHK(c, hipMemcpyAsync(c->d_slot, c->h_slot.data(), c->h_slot.size() * 4, hipMemcpyHostToDevice, c->stream));
+    c->geom_dirty = true;  // the cameras' rays now cross more active bricks
+  }
+  // 2. beliefs: logit of the occupancy (clamped like every marginal); the
+  //    last point in a voxel wins (set_channel_kernel, gvdbkernels.cu:1040-1064)
+  std::vector<std::pair<uint32_t, float>> vals;
+  vals.reserve((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (cell[i] < 0) continue;
+    const int64_t cz = cell[i] % N[2], cy = (cell[i] / N[2]) % N[1], cx = cell[i] / ((int64_t)N[1] * N[2]);
+    const int lin = (int)(((cx >> kBrickLog) * g.by + (cy >> kBrickLog)) * g.bz + (cz >> kBrickLog));
+    const uint32_t local = (uint32_t)((((cx & 7) << 3) | (cy & 7)) << 3 | (cz & 7));
+    const float o = std::min(std::max(occupancies[i], kFEps), 1.0f - kFEps);
+    vals.emplace_back((uint32_t)c->h_slot[lin] * kBrickVox + local, logf(o) - logf(1.0f - o));
+  }
+  std::stable_sort(vals.begin(), vals.end(),
+                   [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) { return a.first < b.first; });
+  std::vector<uint32_t> vox;
+  std::vector<float> lo;
+  for (size_t i = 0; i < vals.size(); ++i)
+    if (i + 1 == vals.size() || vals[i + 1].first != vals[i].first) {
+      vox.push_back(vals[i].first);
+      lo.push_back(vals[i].second);
+    }
+  if (!vox.empty()) {
+    uint32_t* d_vox = nullptr;
+    float* d_lo = nullptr;
+    if ((rc = dalloc(c, &d_vox, vox.size())) || (rc = dalloc(c, &d_lo, lo.size()))) {
+      dfree(&d_vox);
+      return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_vox, vox.data(), vox.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_lo, lo.data(), lo.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) launch_set_voxels(d_vox, d_lo, (uint32_t)vox.size(), c->d_B, c->d_upd, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    dfree(&d_vox);
+    dfree(&d_lo);
+    if (e != hipSuccess) return fail(c, MRF_E_HIP, std::string("load_map: ") + hipGetErrorString(e));
+  }
+  return MRF_OK;
+}
+
+int mrf_set_eval_thresholds(mrf_ctx* c, float occ_thresh, float acc_thresh) {
+  if (!c) return MRF_E_INVALID;
+  c->p.occ_thresh = occ_thresh;
+  c->p.acc_thresh = acc_thresh;
+  return MRF_OK;
+}
+
 const char* mrf_last_error(mrf_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 }  // extern "C"

@@ -620,6 +620,16 @@ __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__
   upd[v] = 1;
 }
 
+// Beliefs of an external map (distinct voxels): B = the given log-odds,
+// marked as updated so that the marginal reads back.
+__global__ void k_set_voxels(const uint32_t* __restrict__ vox, const float* __restrict__ logodds, uint32_t n,
+                             float* __restrict__ B, uint8_t* __restrict__ upd) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  B[vox[i]] = logodds[i];
+  upd[vox[i]] = 1;
+}
+
 // Summed chord length per voxel over one camera's pairs (a camera's pairs
 // are distinct voxels, so cameras launched one after another need no atomics).
 __global__ void k_cum_add(const uint32_t* __restrict__ pair_vox, const float* __restrict__ pair_c, uint32_t npairs,
@@ -825,6 +835,11 @@ void launch_residual(const float* B, float* A, size_t n, unsigned long long* d_m
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_alphas, dim3(grid_for(n, 256)), dim3(256), 0, s, B, upd, out, n);
+}
+
+void launch_set_voxels(const uint32_t* vox, const float* logodds, uint32_t n, float* B, uint8_t* upd, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_set_voxels, dim3(grid_for(n, 256)), dim3(256), 0, s, vox, logodds, n, B, upd);
 }
 
 void launch_cum_add(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, float* cum, hipStream_t s) {

@@ -158,6 +158,8 @@ void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
 void launch_residual(const float* B, float* Aprev, size_t nvox, unsigned long long* d_max, hipStream_t s);
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t nvox, hipStream_t s);
+// load_map: B[vox[i]] = logodds[i], upd = 1 (distinct voxels)
+void launch_set_voxels(const uint32_t* vox, const float* logodds, uint32_t n, float* B, uint8_t* upd, hipStream_t s);
 // set_pose: summed chord lengths of a camera's pairs into cum; then voxels
 // with cum < 0.05 back to the prior (M may be null)
 void launch_cum_add(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, float* cum, hipStream_t s);

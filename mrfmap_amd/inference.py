@@ -205,3 +205,39 @@ class GVDBInference:
         r = self.evaluate(pose=pose, depth=depth)
         tn, fn, tp, fp = r["simple_counts"]
         return r["simple_accuracy"], tp, fp, tn, fn
+
+    # ------------------------------------------------------------ external maps
+    def load_map(self, points, occupancies):
+        """Load an occupancy map: points (n x 3, voxel units) and their
+        occupancy probabilities (GVDBMapLikelihoodEstimator::load_map)."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        occ = np.ascontiguousarray(np.asarray(occupancies, dtype=np.float32).reshape(-1))
+        if len(occ) != len(pts):
+            raise ValueError("one occupancy per point")
+        _lib.check(self._L.mrf_load_map(self._h, _ptr(pts), _ptr(occ), len(pts)), self._h)
+        return True
+
+    def set_acc_thresh(self, thresh):
+        self.params.acc_thresh = float(thresh)
+        _lib.check(self._L.mrf_set_eval_thresholds(self._h, float(self.params.occ_thresh), float(thresh)), self._h)
+
+
+class GVDBMapLikelihoodEstimator(GVDBInference):
+    """Scores an occupancy map against depth keyframes
+    (include/mrfmap/GVDBMapLikelihoodEstimator.h, GVDBPyModules.cpp:85-105):
+    ``load_map`` an external map (e.g. an OctoMap's occupied leaves), then
+    the likelihood / accuracy images at any pose."""
+
+    def __init__(self, params=None, use_gl=False):
+        super().__init__(True, use_gl, params=params)
+
+    def activate_camera(self, cam_id):
+        return int(cam_id)
+
+    def compute_likelihood(self, cam_id):
+        return super().compute_likelihood(cam_id)
+
+    def get_rendered_image_at_pose(self, pose, image, out):
+        """Expected depth image of the loaded map from ``pose``."""
+        depth = image.depth if isinstance(image, GVDBImage) else image
+        out[...] = self.evaluate(pose=pose, depth=depth)["expected_depth"]

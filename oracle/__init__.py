@@ -67,6 +67,7 @@ def lib():
         L.orc_add_camera.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int]
         L.orc_set_pose.argtypes = [P, ctypes.c_int, P]
         L.orc_set_beliefs.argtypes = [P, P, P]
+        L.orc_load_map.argtypes = [P, P, P, ctypes.c_int64]
         L.orc_evaluate.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P, P, P]
         L.orc_prepare.argtypes = [P]
         L.orc_num_bricks.argtypes = [P]
@@ -151,6 +152,12 @@ class Oracle:
 
     def prepare(self):
         lib().orc_prepare(self._h)
+
+    def load_map(self, points, occupancies):
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        occ = np.ascontiguousarray(occupancies, dtype=np.float32).reshape(-1)
+        if lib().orc_load_map(self._h, _ptr(pts), _ptr(occ), len(pts)) != 0:
+            raise RuntimeError("orc_load_map failed")
 
     def set_beliefs(self, B, upd):
         """Test hook: beliefs (num_bricks x 512 log-odds) and update flags."""

@@ -929,6 +929,61 @@ int orc_evaluate(orc_ctx* c, const float pose[16], const float K[9], const float
   return 0;
 }
 
+/* load_map (GVDBMapLikelihoodEstimator.cpp:168-240, set_channel_kernel
+ * gvdbkernels.cu:1040-1064): bricks within one voxel of every point inside
+ * the map box become active (new slots in ascending linear brick index);
+ * the voxel holding a point gets belief logit(clamp(occupancy)), the last
+ * point of a voxel winning. */
+int orc_load_map(orc_ctx* c, const float* pts, const float* occ, int64_t n) {
+  if (!c || n < 0 || orc_prepare(c)) return -1;
+  uint8_t* mark = (uint8_t*)calloc((size_t)c->nbricks, 1);
+  for (int64_t i = 0; i < n; ++i) {
+    int lo[3], hi[3], ok = 1;
+    for (int a = 0; a < 3 && ok; ++a) {
+      double P = (double)pts[3 * i + a];
+      if (!(P >= 0.0 && P < (double)c->Mx[a])) {
+        ok = 0;
+        break;
+      }
+      int l = (int)floor(P - 1.0), h = (int)floor(P + 1.0);
+      if (l < 0) l = 0;
+      if (h > c->N[a] - 1) h = c->N[a] - 1;
+      lo[a] = l >> 3;
+      hi[a] = h >> 3;
+    }
+    if (!ok) continue;
+    for (int x = lo[0]; x <= hi[0]; ++x)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int z = lo[2]; z <= hi[2]; ++z) mark[(x * c->B3[1] + y) * c->B3[2] + z] = 1;
+  }
+  int grew = 0;
+  for (int lin = 0; lin < c->nbricks; ++lin)
+    if (mark[lin] && c->slot[lin] < 0) {
+      c->slot[lin] = (int32_t)c->nslots;
+      c->slot_lin[c->nslots++] = lin;
+      grew = 1;
+    }
+  free(mark);
+  ensure_pool(c);
+  for (int64_t i = 0; i < n; ++i) {
+    int v[3], ok = 1;
+    for (int a = 0; a < 3 && ok; ++a) {
+      double P = (double)pts[3 * i + a];
+      if (!(P >= 0.0 && P < (double)c->Mx[a])) ok = 0;
+      v[a] = (int)floor(P);
+    }
+    if (!ok) continue;
+    const int lin = ((v[0] >> 3) * c->B3[1] + (v[1] >> 3)) * c->B3[2] + (v[2] >> 3);
+    const int64_t vox = (int64_t)c->slot[lin] * 512 + ((v[0] & 7) * 8 + (v[1] & 7)) * 8 + (v[2] & 7);
+    double o = (double)occ[i];
+    o = o < F_EPS ? F_EPS : (o > 1.0 - F_EPS ? 1.0 - F_EPS : o);
+    c->B[vox] = log(o) - log(1.0 - o);
+    c->upd[vox] = 1;
+  }
+  if (grew) c->dirty = 1;
+  return 0;
+}
+
 /* Test hook: overwrite the beliefs and update flags of the pool (nslots x 512). */
 int orc_set_beliefs(orc_ctx* c, const double* B, const uint8_t* upd) {
   if (!c || orc_prepare(c)) return -1;

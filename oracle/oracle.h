@@ -48,6 +48,8 @@ void orc_destroy(orc_ctx* c);
 /* pose row-major 4x4 camera-to-world; K row-major 3x3; depth row-major */
 int orc_add_camera(orc_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols);
 int orc_set_pose(orc_ctx* c, int cam, const float pose[16]);
+/* external occupancy map: points n x 3 in voxel units, occupancy n */
+int orc_load_map(orc_ctx* c, const float* pts, const float* occ, int64_t n);
 /* test hook: beliefs and update flags of the whole pool */
 int orc_set_beliefs(orc_ctx* c, const double* B, const uint8_t* upd);
 /* map evaluation images (log-likelihood, expected depth, accuracy, simple accuracy) */

@@ -9,6 +9,7 @@ import pytest
 
 from mrfmap_amd import _lib, synth
 
+import oracle
 from helpers import decimate, gpu_for, oracle_for
 
 pytestmark = pytest.mark.gpu
@@ -259,3 +260,35 @@ def test_map_evaluation_matches_oracle(shift):
     assert flips_acc <= 2e-3 and flips_s <= 2e-3
     assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-4 * abs(ro["log_likelihood"]) + 1.0
     assert abs(rg["accuracy"] - ro["accuracy"]) <= 2e-3
+
+
+def test_loaded_map_evaluation_matches_oracle():
+    """An external map (the synthetic room's ground-truth occupancy at the
+    voxel centres, 0.95 inside objects and walls) loaded with load_map
+    (GVDBMapLikelihoodEstimator.cpp:168-240) and scored from keyframe 1."""
+    from mrfmap_amd import GVDBMapLikelihoodEstimator
+
+    p, frames, scene = synth.make_workload("small")
+    n = [int(round(d / p.res)) for d in p.dims]
+    ii, jj, kk = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    vox = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(np.float32) + 0.5
+    world = (vox - np.array([n[0] / 2, n[1] / 2, 0], np.float32)) * np.float32(p.res)
+    occ = scene.occupancy(world)
+    pts = vox[occ > 0.5]
+    vals = np.full(len(pts), 0.95, np.float32)
+    est = GVDBMapLikelihoodEstimator(params=p)
+    o = oracle.Oracle(p.K, p.rows, p.cols, p.res, p.prior, p.dims, p.const_sigmasq_poly, p.const_bias_poly,
+                      lookup_n=p.lookup_n, activation=p.activation)
+    est.load_map(pts, vals)
+    o.load_map(pts, vals)
+    assert est.num_bricks() == o.num_bricks()
+    pose, K, depth = frames[1]
+    rg = est.evaluate(pose=pose, depth=depth, K=K)
+    ro = o.evaluate(pose, depth, K, acc_thresh=p.acc_thresh)
+    dll = np.abs(rg["loglik"].astype(np.float64) - ro["loglik"])
+    ded = np.abs(rg["expected_depth"].astype(np.float64) - ro["expected_depth"])
+    flips = float(np.mean(rg["simple_accuracy_image"] != ro["simple_accuracy_image"]))
+    print(f"loaded map: {len(pts)} points, loglik max {dll.max():.2e}, expected depth max {ded.max():.2e}, "
+          f"simple flips {flips:.2e}, accuracy {rg['accuracy']:.4f} vs {ro['accuracy']:.4f}")
+    assert np.percentile(dll, 99.9) <= 1e-2 and ded.max() <= 2e-3 and flips <= 2e-3
+    assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-4 * abs(ro["log_likelihood"]) + 1.0

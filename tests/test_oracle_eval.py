@@ -112,3 +112,22 @@ def test_evaluation_of_a_converged_map_prefers_the_true_pose():
     valid = depth > 0
     err = np.abs(r0["expected_depth"][valid] - depth[valid])
     assert np.median(err) < 0.2
+
+
+def test_load_map_single_point_closed_form():
+    """load_map (GVDBMapLikelihoodEstimator.cpp:168-240): one point of
+    occupancy 0.9999 in voxel (8, 8, 7) on the axis ray gives the closed-form
+    expected depth of an occupied voxel; a point outside the map box is
+    ignored; bricks within one voxel of the point become active."""
+    K = np.array([[10, 0, 2], [0, 10, 2], [0, 0, 1]], dtype=np.float32)
+    o = oracle.Oracle(K, 5, 5, RES, 0.1, [1.6, 1.6, 1.6], [SIG, 0, 0], [0, 0, 0], lookup_n=3, activation=1)
+    o.load_map(np.array([[8.5, 8.5, 7.5], [-3.0, 2.0, 2.0]], np.float32), np.array([0.9999, 0.7], np.float32))
+    # bricks (1, 1, 0) plus the neighbours of voxel 8.5 +- 1 -> x, y bricks 0..1, z brick 0..1
+    assert o.num_bricks() == 8
+    pose = np.eye(4, dtype=np.float32)
+    pose[:3, 3] = [0.05, 0.05, 0.01]
+    dep = np.full((5, 5), D, dtype=np.float32)
+    r = o.evaluate(pose, dep, K)
+    zc = 7.5 * RES32 - TZ32
+    vis = 1.0 - float(np.float32(0.9999))
+    assert math.isclose(r["expected_depth"][2, 2], (1 - vis) * zc / (1 - vis + F_EPS), rel_tol=1e-6)
