@@ -50,6 +50,7 @@ struct Camera {
   uint32_t* d_pair_vox = nullptr;
   float* d_pair_c = nullptr;       // summed chord length per pair
   float* d_pair_sf = nullptr;      // summed constant front messages per pair
+  float* d_M = nullptr;            // camera messages M_c per pair (FP32)
   uint32_t npairs = 0, nwp = 0;    // pairs; pairs with window incidences (numbered first)
   uint32_t* d_wseg_off = nullptr;  // nwp + 1 window segment offsets
   uint64_t* d_heads = nullptr;     // window segment head bitmask
@@ -82,9 +83,6 @@ struct mrf_ctx {
   float* d_A = nullptr;   // marginal at the end of the previous iteration
   uint8_t* d_upd = nullptr;
   float* d_out = nullptr; // readback staging
-  void* d_M = nullptr;
-  size_t M_bytes = 0;
-  size_t M_stride = 0;    // elements per camera
   void* d_Xo = nullptr;   // segment array of ray->voxel messages of the camera being processed
   size_t Xo_bytes = 0;
   // per-pair / per-tile scratch of a camera pass
@@ -332,6 +330,7 @@ void free_camera_lists(Camera& cam) {
   dfree(&cam.d_pair_vox);
   dfree(&cam.d_pair_c);
   dfree(&cam.d_pair_sf);
+  dfree(&cam.d_M);
   dfree(&cam.d_wseg_off);
   dfree(&cam.d_heads);
   dfree(&cam.d_tile_h0);
@@ -369,6 +368,14 @@ int bit_length(uint64_t v) {
 
 // Traversal lists, pairs and the window segment layout of one camera.
 int traverse_camera(mrf_ctx* c, Camera& cam) {
+  // the camera's messages follow its voxels through the re-traversal
+  // (d_A, pool sized, is free until the next inference recomputes it)
+  const size_t nvox_pool = c->slot_lin.size() * (size_t)kBrickVox;
+  const bool keep_msgs = cam.d_M && cam.npairs && c->d_A && nvox_pool;
+  if (keep_msgs) {
+    launch_fill_f32(c->d_A, 0.0f, nvox_pool, c->stream);
+    launch_scatter_pairs(cam.d_pair_vox, cam.d_M, cam.npairs, c->d_A, c->stream);
+  }
   free_camera_lists(cam);
   const CamGeom cg = make_geom(c, cam);
   const size_t npix = (size_t)num_tiled_pixels(cg);
@@ -457,7 +464,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   const uint32_t np = nrec ? (uint32_t)read_total(c) : 0u;
   cam.npairs = np;
   if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
-      (rc = dalloc(c, &cam.d_pair_sf, np)))
+      (rc = dalloc(c, &cam.d_pair_sf, np)) || (rc = dalloc(c, &cam.d_M, np)))
     return rc;
   // per-pair scratch in the sorted numbering
   const size_t P = (size_t)np + 1;
@@ -520,6 +527,10 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   const size_t nwords_w = nwin / 64 + 1, ntiles_w = nwin / kTile + 1;
   if ((rc = dalloc(c, &cam.d_heads, nwords_w)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles_w))) return rc;
   launch_pack_flags(whead, hw, (uint32_t)nwin, cam.d_heads, cam.d_tile_h0, s);
+  if (keep_msgs)
+    launch_gather_pairs(cam.d_pair_vox, c->d_A, np, cam.d_M, s);
+  else
+    launch_zero_words(cam.d_M, (size_t)np * sizeof(float), s);
   HIPCHK(c, hipStreamSynchronize(s));
   HIPCHK(c, hipGetLastError());
   return MRF_OK;
@@ -556,20 +567,7 @@ int prepare(mrf_ctx* c) {
   // 3. traversal lists and pairs of every camera
   for (Camera& cam : c->cams)
     if ((rc = traverse_camera(c, cam))) return rc;
-  // 4. message storage: one dense pool-sized array per camera
-  const size_t stride = c->slot_lin.size() * (size_t)kBrickVox;
-  const size_t need = std::max<size_t>(1, stride * c->cams.size() * sizeof(float));
-  if (need > c->M_bytes) {
-    if (c->d_M) (void)hipFree(c->d_M);
-    c->d_M = nullptr;
-    c->M_bytes = 0;
-    if (hipMalloc(&c->d_M, need) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(c, MRF_E_NOMEM, "message storage allocation failed");
-    }
-    c->M_bytes = need;
-  }
-  c->M_stride = stride;
+  // 4. per-pass scratch sized for the largest camera
   size_t xneed = 1, pneed = 1, tneed = 1, cneed = 1;
   for (const Camera& cam : c->cams) {
     cneed = std::max<size_t>(cneed, (size_t)cam.nchunks * kSlice);
@@ -672,7 +670,7 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.pl_v = c->d_pl_v;
   a.mid = c->d_mid;
   a.B = c->d_B;
-  a.M = (float*)((char*)c->d_M + ci * c->M_stride * sizeof(float));
+  a.M = cam.d_M;
   a.Xo = c->d_Xo;
   a.upd = c->d_upd;
   a.sig = c->d_sig;
@@ -894,7 +892,6 @@ void mrf_destroy(mrf_ctx* c) {
   }
   dfree(&c->d_sig); dfree(&c->d_bia); dfree(&c->d_slot); dfree(&c->d_bflags);
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
-  if (c->d_M) (void)hipFree(c->d_M);
   if (c->d_Xo) (void)hipFree(c->d_Xo);
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
   dfree(&c->d_apair); dfree(&c->d_Spair); dfree(&c->d_partial);
@@ -936,10 +933,9 @@ int mrf_set_pose(mrf_ctx* c, uint32_t id, const float pose[16], int layout) {
     launch_fill_f32(c->d_A, 0.0f, nv, c->stream);
     for (size_t ci = 0; ci < c->cams.size(); ++ci)
       if (ci != id) launch_cum_add(c->cams[ci].d_pair_vox, c->cams[ci].d_pair_c, c->cams[ci].npairs, c->d_A, c->stream);
-    float* Mid = nullptr;
-    if (c->d_M && c->M_stride == nv && (id + 1) * nv * sizeof(float) <= c->M_bytes)
-      Mid = (float*)((char*)c->d_M + id * c->M_stride * sizeof(float));
-    launch_reset_empty(c->d_A, nv, c->prior_logodds, c->d_B, c->d_upd, Mid, c->stream);
+    launch_reset_empty(c->d_A, nv, c->prior_logodds, c->d_B, c->d_upd, c->stream);
+    const Camera& mc = c->cams[id];
+    if (mc.d_M) launch_reset_pair_msgs(mc.d_pair_vox, mc.npairs, c->d_A, mc.d_M, c->stream);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
   }
@@ -987,9 +983,8 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   if (dirty) s.t_geometry_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (c->ray_sched < 0 && (rc = autotune(c))) return rc;
   const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
-  const size_t esz = sizeof(float);  // camera messages M_c are FP32 for every message type
   // reset M_c of the selected cameras (GVDBInference.cpp:128-134)
-  for (uint32_t id : sel) launch_zero_words((char*)c->d_M + (size_t)id * c->M_stride * esz, c->M_stride * esz, c->stream);
+  for (uint32_t id : sel) launch_zero_words(c->cams[id].d_M, (size_t)c->cams[id].npairs * sizeof(float), c->stream);
   launch_marginals(c->d_B, c->d_A, nvox, c->stream);
   for (const Camera& cam : c->cams) {
     s.updates_per_iter += (int64_t)cam.nupd;

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ p
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npairs) return;
   const uint32_t v = pair_vox[q];
-  const float qv = B[v] - M[v];
+  const float qv = B[v] - M[q];
   float nn = 1.0f / (1.0f + __expf(qv));
   nn = fminf(fmaxf(nn, kFEps), 1.0f - kFEps);
   Msg<MT>::store(apair, q, __logf(nn));
@@ -614,8 +614,8 @@ __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__
   float a = Sq / C;
   a = fminf(fmaxf(a, kFEps), 1.0f - kFEps);
   const float m = __logf(a) - __logf(1.0f - a);
-  const float mo = M[v];
-  M[v] = m;
+  const float mo = M[q];
+  M[q] = m;
   B[v] = (B[v] + m) - mo;
   upd[v] = 1;
 }
@@ -638,15 +638,34 @@ __global__ void k_cum_add(const uint32_t* __restrict__ pair_vox, const float* __
   if (q < npairs) cum[pair_vox[q]] += pair_c[q];
 }
 
-// Voxels that no remaining ray reaches go back to the prior, read 0 and lose
-// the moved camera's message (reset_empty_voxels, gvdbkernels.cu:498-517).
+// Voxels that no remaining ray reaches go back to the prior and read 0
+// (reset_empty_voxels, gvdbkernels.cu:498-517) ...
 __global__ void k_reset_empty(const float* __restrict__ cum, size_t n, float prior_logodds, float* __restrict__ B,
-                              uint8_t* __restrict__ upd, float* __restrict__ M) {
+                              uint8_t* __restrict__ upd) {
   const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n || !(cum[v] < 0.05f)) return;
   B[v] = prior_logodds;
   upd[v] = 0;
-  if (M) M[v] = 0.0f;
+}
+
+// ... and lose the moved camera's message.
+__global__ void k_reset_pair_msgs(const uint32_t* __restrict__ pair_vox, uint32_t npairs, const float* __restrict__ cum,
+                                  float* __restrict__ M) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < npairs && cum[pair_vox[q]] < 0.05f) M[q] = 0.0f;
+}
+
+// Camera messages across a re-traversal: old pairs -> voxels -> new pairs.
+__global__ void k_scatter_pairs(const uint32_t* __restrict__ pair_vox, const float* __restrict__ M, uint32_t npairs,
+                                float* __restrict__ dense) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < npairs) dense[pair_vox[q]] = M[q];
+}
+
+__global__ void k_gather_pairs(const uint32_t* __restrict__ pair_vox, const float* __restrict__ dense, uint32_t npairs,
+                               float* __restrict__ M) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < npairs) M[q] = dense[pair_vox[q]];
 }
 
 // marginal occupancy 1 - clamp(1/(1+e^B)) (gvdbkernels.cu:469-476)
@@ -847,10 +866,24 @@ void launch_cum_add(const uint32_t* pair_vox, const float* pair_c, uint32_t npai
   hipLaunchKernelGGL(k_cum_add, dim3(grid_for(npairs, 256)), dim3(256), 0, s, pair_vox, pair_c, npairs, cum);
 }
 
-void launch_reset_empty(const float* cum, size_t n, float prior_logodds, float* B, uint8_t* upd, float* M,
-                        hipStream_t s) {
+void launch_reset_empty(const float* cum, size_t n, float prior_logodds, float* B, uint8_t* upd, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_reset_empty, dim3(grid_for(n, 256)), dim3(256), 0, s, cum, n, prior_logodds, B, upd, M);
+  hipLaunchKernelGGL(k_reset_empty, dim3(grid_for(n, 256)), dim3(256), 0, s, cum, n, prior_logodds, B, upd);
+}
+
+void launch_reset_pair_msgs(const uint32_t* pair_vox, uint32_t npairs, const float* cum, float* M, hipStream_t s) {
+  if (npairs == 0) return;
+  hipLaunchKernelGGL(k_reset_pair_msgs, dim3(grid_for(npairs, 256)), dim3(256), 0, s, pair_vox, npairs, cum, M);
+}
+
+void launch_scatter_pairs(const uint32_t* pair_vox, const float* M, uint32_t npairs, float* dense, hipStream_t s) {
+  if (npairs == 0) return;
+  hipLaunchKernelGGL(k_scatter_pairs, dim3(grid_for(npairs, 256)), dim3(256), 0, s, pair_vox, M, npairs, dense);
+}
+
+void launch_gather_pairs(const uint32_t* pair_vox, const float* dense, uint32_t npairs, float* M, hipStream_t s) {
+  if (npairs == 0) return;
+  hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for(npairs, 256)), dim3(256), 0, s, pair_vox, dense, npairs, M);
 }
 
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s) {

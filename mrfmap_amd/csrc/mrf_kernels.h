@@ -126,7 +126,7 @@ struct CamPass {
   uint32_t *pf_q, *pl_q;   // per-tile scratch of the segment sums
   float *pf_v, *pl_v, *mid;
   float* B;
-  float* M;                // this camera's messages M_c (FP32, pool sized)
+  float* M;                // this camera's messages M_c (FP32, one per pair)
   void* Xo;                // ray->voxel messages p*l in window-region order (output side type)
   uint8_t* upd;
   const float* sig;        // sensor-model lookup tables
@@ -161,10 +161,14 @@ void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t nvox, 
 // load_map: B[vox[i]] = logodds[i], upd = 1 (distinct voxels)
 void launch_set_voxels(const uint32_t* vox, const float* logodds, uint32_t n, float* B, uint8_t* upd, hipStream_t s);
 // set_pose: summed chord lengths of a camera's pairs into cum; then voxels
-// with cum < 0.05 back to the prior (M may be null)
+// with cum < 0.05 back to the prior, and the moved camera's messages there
+// cleared
 void launch_cum_add(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, float* cum, hipStream_t s);
-void launch_reset_empty(const float* cum, size_t n, float prior_logodds, float* B, uint8_t* upd, float* M,
-                        hipStream_t s);
+void launch_reset_empty(const float* cum, size_t n, float prior_logodds, float* B, uint8_t* upd, hipStream_t s);
+void launch_reset_pair_msgs(const uint32_t* pair_vox, uint32_t npairs, const float* cum, float* M, hipStream_t s);
+// camera messages per pair <-> per voxel (across a re-traversal)
+void launch_scatter_pairs(const uint32_t* pair_vox, const float* M, uint32_t npairs, float* dense, hipStream_t s);
+void launch_gather_pairs(const uint32_t* pair_vox, const float* dense, uint32_t npairs, float* M, hipStream_t s);
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void launch_zero_words(void* p, size_t bytes, hipStream_t s);
