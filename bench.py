@@ -34,6 +34,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged run (profiling)")
+    ap.add_argument("--launch", default="timed", choices=["timed", "graph"],
+                    help="launch mode of the timed steps: kernels one by one with per-kernel HIP events "
+                         "(the roofline's kernel durations come from these steps), or graph replays; "
+                         "the other mode is timed afterwards for comparison")
     return ap.parse_args()
 
 
@@ -149,17 +153,27 @@ def main():
               flush=True)
 
     # throughput: W warmup steps then exactly K timed steps on resident data
+    modes = {"timed": _lib.LAUNCH_TIMED, "graph": _lib.LAUNCH_GRAPH}
+    g.set_launch_mode(modes[a.launch])
     if a.warmup > 0:
         g.perform_inference(max_iters=a.warmup)
     t1 = time.perf_counter()
     st = g.perform_inference(max_iters=a.steps)
     dt = time.perf_counter() - t1
     assert st["iters"] == a.steps
+    # the same steps in the other launch mode, for comparison
+    other = "graph" if a.launch == "timed" else "timed"
+    g.set_launch_mode(modes[other])
+    g.perform_inference(max_iters=max(1, a.warmup))
+    t2 = time.perf_counter()
+    st_other = g.perform_inference(max_iters=a.steps)
+    dt_other = time.perf_counter() - t2
+    timed = st if a.launch == "timed" else st_other
     n_upd = st["updates_per_iter"]
     ncam = len(frames)
     value = n_upd * a.steps / dt
     launches = a.steps * ncam
-    avg_launch_ms = st["t_ray_kernel_ms"] / launches
+    avg_launch_ms = timed["t_ray_kernel_ms"] / launches
     # ray kernel, algorithmic bytes: every incidence reads its 8 B record and
     # its pair's voxel->ray message; a window incidence also reads its 4 B
     # segment position and writes its ray->voxel message
@@ -170,8 +184,9 @@ def main():
     bytes_per_launch = ((n_upd - n_win) * front_b + n_win * window_b) / ncam
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(a)
-    print(f"[bench] {a.steps} steps in {dt * 1e3:.1f} ms; ray kernel {st['t_ray_kernel_ms']:.1f} ms, "
-          f"epilogue {st['t_epilogue_ms']:.1f} ms", file=sys.stderr, flush=True)
+    print(f"[bench] {a.steps} steps ({a.launch}) in {dt * 1e3:.1f} ms, ({other}) {dt_other * 1e3:.1f} ms; "
+          f"ray kernels {timed['t_ray_kernel_ms']:.1f} ms, epilogue {timed['t_epilogue_ms']:.1f} ms",
+          file=sys.stderr, flush=True)
 
     cpu = comp = None
     if not a.no_cpu_baseline:
@@ -203,6 +218,8 @@ def main():
             "list_records": st["list_records"],
             "ray_schedule_records_ahead_front_x_window": list(_lib.RAY_SCHEDULES[st["ray_sched"]]),
             "autotune_ms": st["t_autotune_ms"],
+            "launch": a.launch,
+            f"ms_per_step_{other}": dt_other * 1e3 / a.steps,
         },
         "time_to_converged_ms": conv["t_total_ms"],
         "iters_to_converge": conv["iters"],
@@ -218,6 +235,8 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_src,
             "avg_launch_ms": avg_launch_ms,
+            "duration_source": "HIP events on the library stream around each camera's ray kernels, "
+                               f"{'timed steps' if a.launch == 'timed' else 'comparison steps'}",
             "alg_bytes_per_launch": bytes_per_launch,
             "alg_bytes_per_front_update": front_b,
             "alg_bytes_per_window_update": window_b,

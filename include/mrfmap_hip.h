@@ -45,6 +45,11 @@ extern "C" {
 #define MRF_ROW_MAJOR 0
 #define MRF_COL_MAJOR 1
 
+/* How perform_inference launches a BP iteration. */
+#define MRF_LAUNCH_GRAPH 0 /* replay of a captured HIP graph (the default; eager if capture fails) */
+#define MRF_LAUNCH_TIMED 1 /* kernels launched one by one with HIP events around each camera's
+                              ray and epilogue kernels (per-kernel timings in mrf_stats) */
+
 /* Which bricks join the MRF. */
 #define MRF_ACT_ENDPOINT 0 /* bricks within 1 voxel of a depth endpoint (GVDBMRFMap.cpp:117-135) */
 #define MRF_ACT_FULL_RAY 1 /* every brick a truncated ray crosses */
@@ -91,14 +96,17 @@ typedef struct mrf_stats {
   int64_t list_records;        /* incidence records incl. gap records */
   double t_geometry_ms;        /* activation + traversal + pair build (0 if cached) */
   double t_iters_ms;           /* wall time of all iterations (device events) */
-  double t_ray_kernel_ms;      /* summed device time of the ray-message kernel */
-  double t_epilogue_ms;        /* summed device time of the epilogue kernel */
+  double t_ray_kernel_ms;      /* summed device time of the ray-message kernels; -1 unless the
+                                  launch mode is MRF_LAUNCH_TIMED */
+  double t_epilogue_ms;        /* summed device time of the epilogue kernels; same */
   double t_total_ms;           /* wall time of the whole call */
   int ray_sched;               /* ray-kernel schedule in use (records in flight: front x window) */
   double t_autotune_ms;        /* time spent choosing it (first call of a context) */
   int64_t window_updates_per_iter; /* updates whose message depends on the ray; the others lie where
                                       the depth likelihood is exactly 0 in FP32 and carry the constant
                                       message F_EPS * l, summed per voxel once per traversal */
+  int graph;                   /* 1: iterations replayed from a captured HIP graph, 0: launched one by one
+                                  (MRF_LAUNCH_TIMED, MRF_NO_GRAPH=1, or capture unavailable) */
 } mrf_stats;
 
 /* gvdb_params_t() defaults (GVDBParams.h:32-55). */
@@ -183,6 +191,10 @@ int mrf_evaluate_at_pose(mrf_ctx* ctx, const float pose[16], const float K[9], c
 int mrf_load_map(mrf_ctx* ctx, const float* points, const float* occupancies, int64_t n);
 /* Thresholds of the map evaluation (GVDBMapLikelihoodEstimator::set_acc_thresh). */
 int mrf_set_eval_thresholds(mrf_ctx* ctx, float occ_thresh, float acc_thresh);
+/* MRF_LAUNCH_GRAPH or MRF_LAUNCH_TIMED (no counterpart: GVDB launched its
+ * kernels one by one).  The environment variable MRF_NO_GRAPH=1 makes
+ * MRF_LAUNCH_TIMED the default of new contexts.  Results are bit-identical. */
+int mrf_set_launch_mode(mrf_ctx* ctx, int mode);
 
 /* Traversal lists of one camera (for parity tests and tools).  No
  * counterpart: GVDB recomputed the DDA in every pass.  Ray records are

@@ -17,6 +17,7 @@ ROW_MAJOR, COL_MAJOR = 0, 1
 ACT_ENDPOINT, ACT_FULL_RAY = 0, 1
 MSG_F32, MSG_F16, MSG_BF16, MSG_MIXED_F16, MSG_MIXED_BF16 = 0, 1, 2, 3, 4
 # ray-kernel schedules (records loaded ahead in the front loop, in the window passes)
+LAUNCH_GRAPH, LAUNCH_TIMED = 0, 1
 RAY_SCHEDULES = {0: (8, 4), 1: (8, 8), 2: (16, 4), 3: (16, 8), 4: (4, 4), 5: (16, 16)}
 
 # every symbol include/mrfmap_hip.h declares
@@ -26,6 +27,7 @@ EXPORTS = [
     "mrf_num_bricks", "mrf_get_occupied_bricks", "mrf_get_alphas", "mrf_get_beliefs", "mrf_prepare",
     "mrf_get_traversal", "mrf_last_error", "mrf_evaluate_camera", "mrf_evaluate_at_pose", "mrf_load_map",
     "mrf_set_eval_thresholds",
+    "mrf_set_launch_mode",
 ]
 
 
@@ -48,7 +50,7 @@ class MrfStats(ctypes.Structure):
         ("list_records", ctypes.c_int64), ("t_geometry_ms", ctypes.c_double), ("t_iters_ms", ctypes.c_double),
         ("t_ray_kernel_ms", ctypes.c_double), ("t_epilogue_ms", ctypes.c_double),
         ("t_total_ms", ctypes.c_double), ("ray_sched", ctypes.c_int), ("t_autotune_ms", ctypes.c_double),
-        ("window_updates_per_iter", ctypes.c_int64),
+        ("window_updates_per_iter", ctypes.c_int64), ("graph", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -109,6 +111,7 @@ def load():
                                        ctypes.POINTER(MrfEval)]
     L.mrf_load_map.argtypes = [P, P, P, ctypes.c_int64]
     L.mrf_set_eval_thresholds.argtypes = [P, ctypes.c_float, ctypes.c_float]
+    L.mrf_set_launch_mode.argtypes = [P, ctypes.c_int]
     L.mrf_last_error.argtypes = [P]
     L.mrf_last_error.restype = ctypes.c_char_p
     _lib = L

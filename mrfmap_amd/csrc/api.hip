@@ -121,6 +121,12 @@ struct mrf_ctx {
   std::vector<Camera> cams;
   bool geom_dirty = true;
   int ray_sched = -1;      // chosen RaySched, -1 until autotuned
+  // captured BP iteration: valid while no buffer it uses is reallocated
+  uint64_t epoch = 0;      // bumped whenever geometry or pool buffers change
+  hipGraphExec_t gexec = nullptr;
+  uint64_t gexec_key = 0;
+  int launch_mode = MRF_LAUNCH_GRAPH;
+  bool graph_ok = true;    // false once a capture failed
   bool sched_forced = false;
   double autotune_ms = 0.0;
   std::string err;
@@ -130,6 +136,7 @@ namespace {
 
 int fail(mrf_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
+  (void)hipGetLastError();  // reported here; must not resurface in a later call's launch check
   return code;
 }
 
@@ -207,6 +214,7 @@ int ensure_tables(mrf_ctx* c, int rows, int cols) {
     }
   dfree(&c->d_sig);
   dfree(&c->d_bia);
+  ++c->epoch;  // a captured iteration holds the old table pointers
   int rc;
   if ((rc = dalloc(c, &c->d_sig, c->sig_tab.size())) || (rc = dalloc(c, &c->d_bia, c->bia_tab.size()))) return rc;
   HIPCHK(c, hipMemcpy(c->d_sig, c->sig_tab.data(), c->sig_tab.size() * 4, hipMemcpyHostToDevice));
@@ -267,6 +275,7 @@ int ensure_pool(mrf_ctx* c, size_t nslots) {
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_out); dfree(&c->d_upd);
   c->d_B = B; c->d_A = A; c->d_out = O; c->d_upd = U;
   c->cap_slots = cap;
+  ++c->epoch;
   return MRF_OK;
 }
 
@@ -538,6 +547,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
 
 int prepare(mrf_ctx* c) {
   if (!c->geom_dirty) return MRF_OK;
+  ++c->epoch;  // camera lists and per-pass scratch are rebuilt below
   int rc;
   // 1. activation
   HIPCHK(c, hipMemsetAsync(c->d_bflags, 0, (size_t)c->nbricks, c->stream));
@@ -777,6 +787,58 @@ int evaluate(mrf_ctx* c, const CamGeom& cg, const float* h_depth, const float* d
   return MRF_OK;
 }
 
+// One BP iteration on c->stream: every camera's pass (Gauss-Seidel), then
+// the FP64 residual into d_total[1].  ev (3 per camera), when given, times
+// each camera's ray kernels and epilogue; a captured sequence records no
+// events (the runtime cannot time events recorded by graph nodes).
+hipError_t enqueue_iteration(mrf_ctx* c, const hipEvent_t* ev) {
+  const size_t ncam = c->cams.size();
+  const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
+  hipError_t e;
+  for (size_t ci = 0; ci < ncam; ++ci) {
+    const CamPass a = cam_pass(c, ci);
+    launch_pair_msgs(a, c->stream);
+    if (ev && (e = hipEventRecord(ev[3 * ci + 0], c->stream)) != hipSuccess) return e;
+    launch_ray_messages(a, c->stream);
+    if (ev && (e = hipEventRecord(ev[3 * ci + 1], c->stream)) != hipSuccess) return e;
+    launch_epilogue(a, c->stream);
+    if (ev && (e = hipEventRecord(ev[3 * ci + 2], c->stream)) != hipSuccess) return e;
+  }
+  launch_zero_words(c->d_total + 1, 8, c->stream);  // a kernel, not a memset: the sequence may be captured
+  launch_residual(c->d_B, c->d_A, nvox, (unsigned long long*)(c->d_total + 1), c->stream);
+  return hipGetLastError();
+}
+
+void drop_graph(mrf_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+}
+
+// Capture the iteration into a graph (once per buffer epoch and schedule).
+// Any failure switches the context to eager launches for good.
+void ensure_graph(mrf_ctx* c) {
+  const uint64_t key = (c->epoch << 8) ^ (uint64_t)(c->ray_sched + 1) ^ ((uint64_t)c->cams.size() << 40);
+  if (!c->graph_ok || (c->gexec && c->gexec_key == key)) return;
+  drop_graph(c);
+  hipGraph_t g = nullptr;
+  bool ok = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    const hipError_t e1 = enqueue_iteration(c, nullptr);
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    ok = e1 == hipSuccess && e2 == hipSuccess && g;
+  }
+  if (ok) ok = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0) == hipSuccess;
+  if (g) (void)hipGraphDestroy(g);
+  if (!ok) {
+    (void)hipGetLastError();
+    c->gexec = nullptr;
+    c->graph_ok = false;
+    drop_graph(c);
+    return;
+  }
+  c->gexec_key = key;
+}
+
 }  // namespace
 
 extern "C" {
@@ -849,6 +911,8 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
     return MRF_E_HIP;
   }
   c->h_slot.assign((size_t)c->nbricks, -1);
+  if (const char* env = getenv("MRF_NO_GRAPH"))
+    if (env[0] && env[0] != '0') c->launch_mode = MRF_LAUNCH_TIMED;
   if (const char* env = getenv("MRF_RAY_SCHED")) {
     const int v = atoi(env);
     if (v >= 0 && v < kNumSched) {
@@ -901,6 +965,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
   dfree(&c->d_pscr); dfree(&c->d_sell0);
   dfree(&c->d_eval); dfree(&c->d_eval_part); dfree(&c->d_eval_counts);
+  drop_graph(c);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
   dfree(&c->d_scan_tmp); dfree(&c->d_total);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -994,39 +1059,45 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
     s.list_records += (int64_t)cam.nlist;
   }
   const size_t ncam = c->cams.size();
-  std::vector<hipEvent_t> ev(4 * ncam + 2);
+  // the iteration is replayed from a captured graph unless per-kernel timing is asked for
+  if (c->launch_mode == MRF_LAUNCH_GRAPH) ensure_graph(c);
+  const bool graph = c->launch_mode == MRF_LAUNCH_GRAPH && c->gexec != nullptr;
+  // events: [0, 1] around the iteration, then 3 per camera unless graph-launched
+  std::vector<hipEvent_t> ev(graph ? 2 : 2 + 3 * ncam, nullptr);
+  struct Events {
+    std::vector<hipEvent_t>& v;
+    ~Events() {
+      for (auto& e : v)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } ev_guard{ev};
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
   double tray = 0, tepi = 0, titer = 0;
   int it = 0;
   double resid = 0;
   bool conv = false;
   for (; it < max_iters; ++it) {
-    HIPCHK(c, hipEventRecord(ev[4 * ncam], c->stream));
-    for (size_t ci = 0; ci < ncam; ++ci) {
-      const CamPass a = cam_pass(c, ci);
-      launch_pair_msgs(a, c->stream);
-      HIPCHK(c, hipEventRecord(ev[4 * ci + 0], c->stream));
-      launch_ray_messages(a, c->stream);
-      HIPCHK(c, hipEventRecord(ev[4 * ci + 1], c->stream));
-      launch_epilogue(a, c->stream);
-      HIPCHK(c, hipEventRecord(ev[4 * ci + 2], c->stream));
-    }
-    HIPCHK(c, hipMemsetAsync(c->d_total + 1, 0, 8, c->stream));
-    launch_residual(c->d_B, c->d_A, nvox, (unsigned long long*)(c->d_total + 1), c->stream);
-    HIPCHK(c, hipEventRecord(ev[4 * ncam + 1], c->stream));
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    if (graph)
+      HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    else
+      HIPCHK(c, enqueue_iteration(c, ev.data() + 2));
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
     uint64_t bits = 0;
     HIPCHK(c, hipMemcpyAsync(&bits, c->d_total + 1, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
     std::memcpy(&resid, &bits, 8);
     float ms = 0;
-    for (size_t ci = 0; ci < ncam; ++ci) {
-      HIPCHK(c, hipEventElapsedTime(&ms, ev[4 * ci + 0], ev[4 * ci + 1]));
-      tray += ms;
-      HIPCHK(c, hipEventElapsedTime(&ms, ev[4 * ci + 1], ev[4 * ci + 2]));
-      tepi += ms;
+    if (!graph) {
+      for (size_t ci = 0; ci < ncam; ++ci) {
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[2 + 3 * ci + 0], ev[2 + 3 * ci + 1]));
+        tray += ms;
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[2 + 3 * ci + 1], ev[2 + 3 * ci + 2]));
+        tepi += ms;
+      }
     }
-    HIPCHK(c, hipEventElapsedTime(&ms, ev[4 * ncam], ev[4 * ncam + 1]));
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
     titer += ms;
     if (tol > 0 && resid < tol) {
       conv = true;
@@ -1034,12 +1105,12 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
       break;
     }
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
   s.iters = it;
+  s.graph = graph ? 1 : 0;
   s.converged = conv ? 1 : 0;
   s.residual = resid;
-  s.t_ray_kernel_ms = tray;
-  s.t_epilogue_ms = tepi;
+  s.t_ray_kernel_ms = graph ? -1.0 : tray;
+  s.t_epilogue_ms = graph ? -1.0 : tepi;
   s.t_iters_ms = titer;
   s.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   s.ray_sched = c->ray_sched;
@@ -1256,6 +1327,13 @@ int mrf_load_map(mrf_ctx* c, const float* points, const float* occupancies, int6
     dfree(&d_lo);
     if (e != hipSuccess) return fail(c, MRF_E_HIP, std::string("load_map: ") + hipGetErrorString(e));
   }
+  return MRF_OK;
+}
+
+int mrf_set_launch_mode(mrf_ctx* c, int mode) {
+  if (!c) return MRF_E_INVALID;
+  if (mode != MRF_LAUNCH_GRAPH && mode != MRF_LAUNCH_TIMED) return fail(c, MRF_E_INVALID, "unknown launch mode");
+  c->launch_mode = mode;
   return MRF_OK;
 }
 

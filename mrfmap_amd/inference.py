@@ -217,6 +217,12 @@ class GVDBInference:
         _lib.check(self._L.mrf_load_map(self._h, _ptr(pts), _ptr(occ), len(pts)), self._h)
         return True
 
+    def set_launch_mode(self, mode):
+        """_lib.LAUNCH_GRAPH (iterations replayed from a captured HIP graph, the
+        default) or _lib.LAUNCH_TIMED (kernels launched one by one with
+        per-kernel device timings in the returned statistics)."""
+        _lib.check(self._L.mrf_set_launch_mode(self._h, int(mode)), self._h)
+
     def set_acc_thresh(self, thresh):
         self.params.acc_thresh = float(thresh)
         _lib.check(self._L.mrf_set_eval_thresholds(self._h, float(self.params.occ_thresh), float(thresh)), self._h)

@@ -292,3 +292,36 @@ def test_loaded_map_evaluation_matches_oracle():
           f"simple flips {flips:.2e}, accuracy {rg['accuracy']:.4f} vs {ro['accuracy']:.4f}")
     assert np.percentile(dll, 99.9) <= 1e-2 and ded.max() <= 2e-3 and flips <= 2e-3
     assert abs(rg["log_likelihood"] - ro["log_likelihood"]) <= 1e-4 * abs(ro["log_likelihood"]) + 1.0
+
+
+def test_graph_replays_match_eager_launches(monkeypatch):
+    """The BP iteration replayed from a captured HIP graph gives the same bits
+    as the same kernels launched one by one, call after call (each call's
+    message reset, marginal refresh and residual readback run between the
+    replays), also after a keyframe is added (a new capture).  Eager launches
+    are selected both through the API and through MRF_NO_GRAPH=1."""
+    p, frames, _ = synth.make_workload("small")
+    runs = {}
+    for mode in ("graph", "timed", "env"):
+        if mode == "env":
+            monkeypatch.setenv("MRF_NO_GRAPH", "1")
+        g = gpu_for(p, frames[:3])
+        if mode == "timed":
+            g.set_launch_mode(_lib.LAUNCH_TIMED)
+        want_graph = 1 if mode == "graph" else 0
+        outs = []
+        for k in range(3):
+            st = g.perform_inference(max_iters=2 + k)
+            assert st["graph"] == want_graph
+            assert (st["t_ray_kernel_ms"] > 0) == (not want_graph)
+            assert st["t_iters_ms"] > 0
+            outs.append(g.get_beliefs().copy())
+        g.add_camera_with_depth(frames[3][0], frames[3][2], frames[3][1])
+        st = g.perform_inference(max_iters=3)
+        assert st["graph"] == want_graph
+        outs.append(g.get_beliefs().copy())
+        runs[mode] = outs
+        monkeypatch.delenv("MRF_NO_GRAPH", raising=False)
+    for mode in ("timed", "env"):
+        for a, b in zip(runs["graph"], runs[mode]):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
