@@ -107,6 +107,10 @@ typedef struct mrf_stats {
                                       message F_EPS * l, summed per voxel once per traversal */
   int graph;                   /* 1: iterations replayed from a captured HIP graph, 0: launched one by one
                                   (MRF_LAUNCH_TIMED, MRF_NO_GRAPH=1, or capture unavailable) */
+  int warm;                    /* learned warm start: 0 not used, 1 applied, 2 applied and undone
+                                  because the residual grew (plain BP from the first iteration on) */
+  int64_t warm_voxels;         /* voxels the warm start predicted */
+  double t_warm_ms;            /* features + MLP + warm-start update, wall time */
 } mrf_stats;
 
 /* gvdb_params_t() defaults (GVDBParams.h:32-55). */
@@ -191,6 +195,59 @@ int mrf_evaluate_at_pose(mrf_ctx* ctx, const float pose[16], const float K[9], c
 int mrf_load_map(mrf_ctx* ctx, const float* points, const float* occupancies, int64_t n);
 /* Thresholds of the map evaluation (GVDBMapLikelihoodEstimator::set_acc_thresh). */
 int mrf_set_eval_thresholds(mrf_ctx* ctx, float occ_thresh, float acc_thresh);
+/* ---- Learned warm start (no counterpart in the reference, whose BP always
+ * starts from the prior with all camera messages 0, GVDBInference.cpp:128-134).
+ *
+ * After the first BP iteration every voxel with C_c > 0.1 for some camera
+ * gets MRF_MLP_FEATURES features: 0 log1p(depth endpoints in the voxel),
+ * 1 log1p(summed chord length of all rays through it), 2 and 3 the mean and
+ * the max over cameras of the camera messages M_c / 9.2102404, 4 the
+ * Euclidean distance (voxels) to the nearest voxel holding an endpoint,
+ * capped at 4, / 4, and 5..31 the beliefs of the 3x3x3 block around it
+ * (x slowest) / 9.2102404 clamped to [-3, 3], the prior outside active bricks.
+ * A 32-64-64-1 GELU MLP (BF16 matrix cores, FP32 accumulation) predicts the
+ * converged belief log-odds; the belief moves there and every camera
+ * message of the voxel by the same share of the change, so
+ * B = logit(prior) + sum_c M_c keeps holding.  MLP parameters are
+ * MRF_MLP_PARAMS floats: W1 64x32, b1 64, W2 64x64, b2 64, w3 64, b3 1,
+ * matrices row-major. */
+#define MRF_MLP_FEATURES 32
+#define MRF_MLP_PARAMS 6337
+
+typedef struct mrf_train_opts {
+  int epochs;      /* passes over the samples */
+  int batch;       /* samples per Adam step */
+  float lr;        /* Adam step size */
+  float beta1, beta2, eps;
+  float lr_decay;  /* lr multiplied by this after every epoch */
+  uint64_t seed;   /* sample order */
+} mrf_train_opts;
+
+void mrf_train_opts_default(mrf_train_opts* o);
+/* perform_inference with the warm start applied after the first iteration;
+ * with fallback != 0 the warm start is undone (beliefs and messages return
+ * to their state after the first iteration, plain BP continues) when a
+ * residual exceeds the first iteration's (second iteration) or twice the
+ * smallest warm residual so far (later iterations). */
+int mrf_perform_inference_warm(mrf_ctx* ctx, const float* params, int max_iters, double tol, int fallback,
+                               mrf_stats* out);
+/* Features of the current state (call after perform_inference): *n rows of
+ * MRF_MLP_FEATURES floats (row-major) and their pool indices
+ * (slot * 512 + voxel, the order of mrf_get_alphas), in pool order.  With X
+ * and vox null only *n is set. */
+int mrf_features(mrf_ctx* ctx, float* X, uint32_t* vox, int64_t cap, int64_t* n);
+/* (pool index, predicted log-odds) of the last warm start or feature call. */
+int mrf_warm_predictions(mrf_ctx* ctx, uint32_t* vox, float* pred, int64_t cap, int64_t* n);
+/* The MLP on the GPU for host buffers: y[n] = f(X[n x 32]); gradient of
+ * 1/(2n) sum (y - t)^2 into grad[MRF_MLP_PARAMS]; Adam training of params
+ * in place (epoch_loss[epochs]: mean loss over each epoch, may be null).
+ * Deterministic: the same inputs give the same bits. */
+int mrf_mlp_forward(mrf_ctx* ctx, const float* params, const float* X, int64_t n, float* y);
+int mrf_mlp_grad(mrf_ctx* ctx, const float* params, const float* X, const float* t, int64_t n, float* grad,
+                 double* loss);
+int mrf_mlp_train(mrf_ctx* ctx, float* params, const float* X, const float* t, int64_t n, const mrf_train_opts* o,
+                  double* epoch_loss);
+
 /* MRF_LAUNCH_GRAPH or MRF_LAUNCH_TIMED (no counterpart: GVDB launched its
  * kernels one by one).  The environment variable MRF_NO_GRAPH=1 makes
  * MRF_LAUNCH_TIMED the default of new contexts.  Results are bit-identical. */

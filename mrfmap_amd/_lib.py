@@ -27,8 +27,10 @@ EXPORTS = [
     "mrf_num_bricks", "mrf_get_occupied_bricks", "mrf_get_alphas", "mrf_get_beliefs", "mrf_prepare",
     "mrf_get_traversal", "mrf_last_error", "mrf_evaluate_camera", "mrf_evaluate_at_pose", "mrf_load_map",
     "mrf_set_eval_thresholds",
-    "mrf_set_launch_mode",
+    "mrf_set_launch_mode", "mrf_train_opts_default", "mrf_perform_inference_warm", "mrf_features",
+    "mrf_warm_predictions", "mrf_mlp_forward", "mrf_mlp_grad", "mrf_mlp_train",
 ]
+MLP_FEATURES, MLP_PARAMS = 32, 6337
 
 
 class MrfParams(ctypes.Structure):
@@ -50,11 +52,19 @@ class MrfStats(ctypes.Structure):
         ("list_records", ctypes.c_int64), ("t_geometry_ms", ctypes.c_double), ("t_iters_ms", ctypes.c_double),
         ("t_ray_kernel_ms", ctypes.c_double), ("t_epilogue_ms", ctypes.c_double),
         ("t_total_ms", ctypes.c_double), ("ray_sched", ctypes.c_int), ("t_autotune_ms", ctypes.c_double),
-        ("window_updates_per_iter", ctypes.c_int64), ("graph", ctypes.c_int),
+        ("window_updates_per_iter", ctypes.c_int64), ("graph", ctypes.c_int), ("warm", ctypes.c_int),
+        ("warm_voxels", ctypes.c_int64), ("t_warm_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MrfTrainOpts(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int), ("batch", ctypes.c_int), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("lr_decay", ctypes.c_float), ("seed", ctypes.c_uint64),
+    ]
 
 
 class MrfEval(ctypes.Structure):
@@ -112,6 +122,15 @@ def load():
     L.mrf_load_map.argtypes = [P, P, P, ctypes.c_int64]
     L.mrf_set_eval_thresholds.argtypes = [P, ctypes.c_float, ctypes.c_float]
     L.mrf_set_launch_mode.argtypes = [P, ctypes.c_int]
+    L.mrf_train_opts_default.argtypes = [ctypes.POINTER(MrfTrainOpts)]
+    L.mrf_train_opts_default.restype = None
+    L.mrf_perform_inference_warm.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                             ctypes.POINTER(MrfStats)]
+    L.mrf_features.argtypes = [P, P, P, ctypes.c_int64, i64p]
+    L.mrf_warm_predictions.argtypes = [P, P, P, ctypes.c_int64, i64p]
+    L.mrf_mlp_forward.argtypes = [P, P, P, ctypes.c_int64, P]
+    L.mrf_mlp_grad.argtypes = [P, P, P, P, ctypes.c_int64, P, ctypes.POINTER(ctypes.c_double)]
+    L.mrf_mlp_train.argtypes = [P, P, P, P, ctypes.c_int64, ctypes.POINTER(MrfTrainOpts), P]
     L.mrf_last_error.argtypes = [P]
     L.mrf_last_error.restype = ctypes.c_char_p
     _lib = L

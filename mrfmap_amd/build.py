@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("MRF_OFFLOAD_ARCH", "gfx950")
-HIP_SOURCES = ["api.hip", "traverse.hip", "bp.hip", "scan.hip", "geometry.hip"]
+HIP_SOURCES = ["api.hip", "traverse.hip", "bp.hip", "scan.hip", "geometry.hip", "warm.hip"]
 # traverse.hip: every double operation is one IEEE operation (bit-exact lists);
 # bp.hip: the ray kernel's two passes recompute the same terms bit for bit
 NO_CONTRACT = {"traverse.hip", "bp.hip"}

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <random>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -129,6 +130,25 @@ struct mrf_ctx {
   bool graph_ok = true;    // false once a capture failed
   bool sched_forced = false;
   double autotune_ms = 0.0;
+  // learned warm start: per-voxel feature accumulators, feature rows,
+  // predictions, the fallback snapshot, MLP parameters and training scratch
+  float *d_wpass = nullptr, *d_wmsum = nullptr, *d_wmmax = nullptr, *d_wdelta = nullptr;
+  uint32_t *d_wnc = nullptr, *d_whits = nullptr, *d_wflag = nullptr, *d_wrow = nullptr;
+  size_t cap_wvox = 0;
+  int32_t* d_slot_lin = nullptr;
+  size_t cap_slot_lin = 0;
+  float* d_wX = nullptr;
+  uint32_t* d_wvox = nullptr;
+  float* d_wpred = nullptr;
+  size_t cap_wrows = 0;
+  uint32_t wrows = 0;  // rows of the last feature extraction
+  float* d_mlp = nullptr;
+  float *d_snapB = nullptr, *d_snapM = nullptr;
+  size_t cap_snapB = 0, cap_snapM = 0;
+  float *d_tX = nullptr, *d_tT = nullptr, *d_tG = nullptr, *d_tM = nullptr, *d_tV = nullptr, *d_tPart = nullptr;
+  uint32_t* d_tSel = nullptr;
+  double *d_tLpart = nullptr, *d_tLoss = nullptr;
+  size_t cap_trows = 0, cap_tpart = 0;
   std::string err;
 };
 
@@ -839,6 +859,125 @@ void ensure_graph(mrf_ctx* c) {
   c->gexec_key = key;
 }
 
+// ---- learned warm start
+int ensure_mlp(mrf_ctx* c) { return c->d_mlp ? MRF_OK : dalloc(c, &c->d_mlp, (size_t)kMlpParams); }
+
+// Warm-start features of every touched voxel of the current state (after an
+// iteration) into d_wX / d_wvox, in pool order; *n_out rows.
+int features_device(mrf_ctx* c, uint32_t* n_out) {
+  const size_t nslots = c->slot_lin.size(), nvox = nslots * (size_t)kBrickVox;
+  if (nvox > 0xFFFFFFFFull) return fail(c, MRF_E_LIMIT, "features: more than 2^32 voxels in the pool");
+  int rc;
+  if (nvox > c->cap_wvox) {
+    dfree(&c->d_wpass); dfree(&c->d_wmsum); dfree(&c->d_wmmax); dfree(&c->d_wdelta);
+    dfree(&c->d_wnc); dfree(&c->d_whits); dfree(&c->d_wflag); dfree(&c->d_wrow);
+    c->cap_wvox = 0;
+    if ((rc = dalloc(c, &c->d_wpass, nvox)) || (rc = dalloc(c, &c->d_wmsum, nvox)) ||
+        (rc = dalloc(c, &c->d_wmmax, nvox)) || (rc = dalloc(c, &c->d_wdelta, nvox)) ||
+        (rc = dalloc(c, &c->d_wnc, nvox)) || (rc = dalloc(c, &c->d_whits, nvox)) ||
+        (rc = dalloc(c, &c->d_wflag, nvox)) || (rc = dalloc(c, &c->d_wrow, nvox)))
+      return rc;
+    c->cap_wvox = nvox;
+  }
+  if (nslots > c->cap_slot_lin) {
+    dfree(&c->d_slot_lin);
+    c->cap_slot_lin = 0;
+    if ((rc = dalloc(c, &c->d_slot_lin, nslots))) return rc;
+    c->cap_slot_lin = nslots;
+  }
+  if ((rc = ensure_scan(c, nvox))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_slot_lin, c->slot_lin.data(), nslots * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream));
+  launch_zero_words(c->d_wpass, nvox * 4, c->stream);
+  launch_zero_words(c->d_wmsum, nvox * 4, c->stream);
+  launch_zero_words(c->d_wnc, nvox * 4, c->stream);
+  launch_zero_words(c->d_whits, nvox * 4, c->stream);
+  for (const Camera& cam : c->cams) {
+    launch_feat_hits(make_geom(c, cam), c->g, cam.d_depth, c->d_slot, c->d_whits, c->stream);
+    launch_feat_pairs(cam.d_pair_vox, cam.d_pair_c, cam.d_M, cam.npairs, c->d_wpass, c->d_wmsum, c->d_wmmax,
+                      c->d_wnc, c->stream);
+  }
+  launch_feat_flags(c->d_wnc, nvox, c->d_wflag, c->stream);
+  launch_exclusive_scan(c->d_wflag, c->d_wrow, nvox, c->d_scan_tmp, c->d_total, c->stream);
+  const uint64_t n = read_total(c);
+  if (n > c->cap_wrows) {
+    dfree(&c->d_wX); dfree(&c->d_wvox); dfree(&c->d_wpred);
+    c->cap_wrows = 0;
+    if ((rc = dalloc(c, &c->d_wX, (size_t)n * kMlpIn)) || (rc = dalloc(c, &c->d_wvox, (size_t)n)) ||
+        (rc = dalloc(c, &c->d_wpred, (size_t)n)))
+      return rc;
+    c->cap_wrows = n;
+  }
+  launch_features(c->g, c->d_slot, c->d_slot_lin, (uint32_t)nslots, c->d_whits, c->d_wpass, c->d_wmsum, c->d_wmmax,
+                  c->d_wnc, c->d_B, c->prior_logodds, c->d_wrow, c->d_wX, c->d_wvox, c->stream);
+  HIPCHK(c, hipGetLastError());
+  c->wrows = (uint32_t)n;
+  *n_out = (uint32_t)n;
+  return MRF_OK;
+}
+
+// Copies of the beliefs and every camera's messages (to = true: take the
+// snapshot, false: restore it).
+int snapshot(mrf_ctx* c, bool take) {
+  const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
+  size_t npairs = 0;
+  for (const Camera& cam : c->cams) npairs += cam.npairs;
+  int rc;
+  if (take && nvox > c->cap_snapB) {
+    dfree(&c->d_snapB);
+    c->cap_snapB = 0;
+    if ((rc = dalloc(c, &c->d_snapB, nvox))) return rc;
+    c->cap_snapB = nvox;
+  }
+  if (take && npairs > c->cap_snapM) {
+    dfree(&c->d_snapM);
+    c->cap_snapM = 0;
+    if ((rc = dalloc(c, &c->d_snapM, npairs ? npairs : 1))) return rc;
+    c->cap_snapM = npairs;
+  }
+  const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+  if (take)
+    HIPCHK(c, hipMemcpyAsync(c->d_snapB, c->d_B, nvox * 4, k, c->stream));
+  else
+    HIPCHK(c, hipMemcpyAsync(c->d_B, c->d_snapB, nvox * 4, k, c->stream));
+  size_t off = 0;
+  for (Camera& cam : c->cams) {
+    if (cam.npairs) {
+      if (take)
+        HIPCHK(c, hipMemcpyAsync(c->d_snapM + off, cam.d_M, (size_t)cam.npairs * 4, k, c->stream));
+      else
+        HIPCHK(c, hipMemcpyAsync(cam.d_M, c->d_snapM + off, (size_t)cam.npairs * 4, k, c->stream));
+    }
+    off += cam.npairs;
+  }
+  return MRF_OK;
+}
+
+// Training/check buffers for n samples.
+int ensure_train(mrf_ctx* c, size_t n, uint32_t batch) {
+  int rc;
+  if (n > c->cap_trows) {
+    dfree(&c->d_tX); dfree(&c->d_tT); dfree(&c->d_tSel);
+    c->cap_trows = 0;
+    if ((rc = dalloc(c, &c->d_tX, n * kMlpIn)) || (rc = dalloc(c, &c->d_tT, n)) || (rc = dalloc(c, &c->d_tSel, n)))
+      return rc;
+    c->cap_trows = n;
+  }
+  if (!c->d_tG) {
+    if ((rc = dalloc(c, &c->d_tG, (size_t)kMlpParams)) || (rc = dalloc(c, &c->d_tM, (size_t)kMlpParams)) ||
+        (rc = dalloc(c, &c->d_tV, (size_t)kMlpParams)) || (rc = dalloc(c, &c->d_tLoss, (size_t)1)))
+      return rc;
+  }
+  const size_t nw = mlp_grad_waves(batch);
+  if (nw > c->cap_tpart) {
+    dfree(&c->d_tPart); dfree(&c->d_tLpart);
+    c->cap_tpart = 0;
+    if ((rc = dalloc(c, &c->d_tPart, nw * kMlpParams)) || (rc = dalloc(c, &c->d_tLpart, nw))) return rc;
+    c->cap_tpart = nw;
+  }
+  return ensure_mlp(c);
+}
+
 }  // namespace
 
 extern "C" {
@@ -965,6 +1104,12 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
   dfree(&c->d_pscr); dfree(&c->d_sell0);
   dfree(&c->d_eval); dfree(&c->d_eval_part); dfree(&c->d_eval_counts);
+  dfree(&c->d_wpass); dfree(&c->d_wmsum); dfree(&c->d_wmmax); dfree(&c->d_wdelta);
+  dfree(&c->d_wnc); dfree(&c->d_whits); dfree(&c->d_wflag); dfree(&c->d_wrow); dfree(&c->d_slot_lin);
+  dfree(&c->d_wX); dfree(&c->d_wvox); dfree(&c->d_wpred); dfree(&c->d_mlp);
+  dfree(&c->d_snapB); dfree(&c->d_snapM);
+  dfree(&c->d_tX); dfree(&c->d_tT); dfree(&c->d_tG); dfree(&c->d_tM); dfree(&c->d_tV); dfree(&c->d_tPart);
+  dfree(&c->d_tSel); dfree(&c->d_tLpart); dfree(&c->d_tLoss);
   drop_graph(c);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
   dfree(&c->d_scan_tmp); dfree(&c->d_total);
@@ -1026,7 +1171,11 @@ int mrf_prepare(mrf_ctx* c) {
   return prepare(c);
 }
 
-int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_iters, double tol, mrf_stats* st) {
+// One perform_inference call; with warm != nullptr the learned warm start
+// is applied after the first iteration (and undone if the residual grows,
+// when fallback is set).
+static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_iters, double tol, const float* warm,
+                         int fallback, mrf_stats* st) {
   if (!c) return MRF_E_INVALID;
   auto t0 = std::chrono::steady_clock::now();
   mrf_stats s;
@@ -1076,6 +1225,12 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   int it = 0;
   double resid = 0;
   bool conv = false;
+  if (warm) {
+    if ((rc = ensure_mlp(c))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_mlp, warm, kMlpParams * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  }
+  int warm_state = 0;  // 0: none, 1: applied, 2: applied and undone
+  double r1 = 0, best = 0;
   for (; it < max_iters; ++it) {
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
     if (graph)
@@ -1099,7 +1254,38 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
     }
     HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
     titer += ms;
-    if (tol > 0 && resid < tol) {
+    bool check = true;
+    if (warm && it == 0) {
+      // features of the first-iteration state, MLP, warm-start update
+      const auto w0 = std::chrono::steady_clock::now();
+      uint32_t n = 0;
+      if ((rc = features_device(c, &n))) return rc;
+      launch_mlp_fwd(c->d_mlp, c->d_wX, n, c->d_wpred, c->stream);
+      if ((rc = snapshot(c, true))) return rc;
+      launch_warm_apply(c->d_wvox, c->d_wpred, n, c->d_wnc, c->d_B, c->d_wdelta, c->stream);
+      for (const Camera& cam : c->cams)
+        launch_warm_pairs(cam.d_pair_vox, cam.d_pair_c, cam.npairs, c->d_wdelta, cam.d_M, c->stream);
+      launch_marginals(c->d_B, c->d_A, nvox, c->stream);
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipGetLastError());
+      s.t_warm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+      s.warm_voxels = n;
+      warm_state = 1;
+      r1 = best = resid;
+      check = false;
+    } else if (warm_state == 1 && fallback && resid > (it == 1 ? r1 : 2.0 * best)) {
+      // the residual grows: back to the plain state after the first iteration
+      if ((rc = snapshot(c, false))) return rc;
+      launch_marginals(c->d_B, c->d_A, nvox, c->stream);
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipGetLastError());
+      warm_state = 2;
+      resid = r1;
+      check = false;
+    } else if (warm_state == 1 && resid < best) {
+      best = resid;
+    }
+    if (check && tol > 0 && resid < tol) {
       conv = true;
       ++it;
       break;
@@ -1115,7 +1301,147 @@ int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_it
   s.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   s.ray_sched = c->ray_sched;
   s.t_autotune_ms = c->autotune_ms;
+  s.warm = warm_state;
   if (st) *st = s;
+  return MRF_OK;
+}
+
+int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_iters, double tol, mrf_stats* st) {
+  return run_inference(c, ids, n_ids, max_iters, tol, nullptr, 0, st);
+}
+
+int mrf_perform_inference_warm(mrf_ctx* c, const float* params, int max_iters, double tol, int fallback,
+                               mrf_stats* st) {
+  if (!c) return MRF_E_INVALID;
+  if (!params) return fail(c, MRF_E_INVALID, "perform_inference_warm: no MLP parameters");
+  return run_inference(c, nullptr, 0, max_iters, tol, params, fallback, st);
+}
+
+int mrf_features(mrf_ctx* c, float* X, uint32_t* vox, int64_t cap, int64_t* n) {
+  if (!c || !n) return MRF_E_INVALID;
+  if (c->geom_dirty) return fail(c, MRF_E_STATE, "features: run perform_inference first");
+  uint32_t rows = 0;
+  int rc = features_device(c, &rows);
+  if (rc) return rc;
+  *n = rows;
+  if (!X && !vox) return MRF_OK;
+  if (cap < (int64_t)rows) return fail(c, MRF_E_SHAPE, "features: buffers hold fewer rows than there are voxels");
+  if (X) HIPCHK(c, hipMemcpyAsync(X, c->d_wX, (size_t)rows * kMlpIn * 4, hipMemcpyDeviceToHost, c->stream));
+  if (vox) HIPCHK(c, hipMemcpyAsync(vox, c->d_wvox, (size_t)rows * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRF_OK;
+}
+
+int mrf_warm_predictions(mrf_ctx* c, uint32_t* vox, float* pred, int64_t cap, int64_t* n) {
+  if (!c || !n) return MRF_E_INVALID;
+  *n = c->wrows;
+  if (!vox && !pred) return MRF_OK;
+  if (cap < (int64_t)c->wrows) return fail(c, MRF_E_SHAPE, "warm_predictions: buffers too small");
+  if (c->wrows) {
+    if (vox) HIPCHK(c, hipMemcpyAsync(vox, c->d_wvox, (size_t)c->wrows * 4, hipMemcpyDeviceToHost, c->stream));
+    if (pred) HIPCHK(c, hipMemcpyAsync(pred, c->d_wpred, (size_t)c->wrows * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRF_OK;
+}
+
+int mrf_mlp_forward(mrf_ctx* c, const float* params, const float* X, int64_t n, float* y) {
+  if (!c || !params || (n > 0 && (!X || !y)) || n < 0) return MRF_E_INVALID;
+  if (n > 0xFFFFFFFFll) return fail(c, MRF_E_LIMIT, "mlp_forward: more than 2^32 samples");
+  int rc = ensure_train(c, (size_t)(n ? n : 1), 1);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_mlp, params, kMlpParams * 4, hipMemcpyHostToDevice, c->stream));
+  if (n == 0) return MRF_OK;
+  HIPCHK(c, hipMemcpyAsync(c->d_tX, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, c->stream));
+  launch_mlp_fwd(c->d_mlp, c->d_tX, (uint32_t)n, c->d_tT, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(y, c->d_tT, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRF_OK;
+}
+
+int mrf_mlp_grad(mrf_ctx* c, const float* params, const float* X, const float* t, int64_t n, float* grad,
+                 double* loss) {
+  if (!c || !params || !X || !t || !grad || n <= 0) return MRF_E_INVALID;
+  if (n > 0xFFFFFFFFll) return fail(c, MRF_E_LIMIT, "mlp_grad: more than 2^32 samples");
+  int rc = ensure_train(c, (size_t)n, (uint32_t)n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_mlp, params, kMlpParams * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tX, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tT, t, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  launch_zero_words(c->d_tLoss, 8, c->stream);
+  launch_mlp_grad(c->d_mlp, c->d_tX, c->d_tT, nullptr, 0, (uint32_t)n, 1.0f / (float)n, c->d_tPart, c->d_tLpart,
+                  c->stream);
+  launch_mlp_reduce(c->d_tPart, c->d_tLpart, mlp_grad_waves((uint32_t)n), c->d_tG, c->d_tLoss, 1.0 / (double)n,
+                    c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(grad, c->d_tG, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+  double l = 0;
+  HIPCHK(c, hipMemcpyAsync(&l, c->d_tLoss, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss) *loss = l;
+  return MRF_OK;
+}
+
+void mrf_train_opts_default(mrf_train_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->epochs = 20;
+  o->batch = 8192;
+  o->lr = 1e-3f;
+  o->beta1 = 0.9f;
+  o->beta2 = 0.999f;
+  o->eps = 1e-8f;
+  o->lr_decay = 1.0f;
+  o->seed = 1;
+}
+
+int mrf_mlp_train(mrf_ctx* c, float* params, const float* X, const float* t, int64_t n, const mrf_train_opts* o,
+                  double* epoch_loss) {
+  if (!c || !params || !X || !t || !o || n <= 0 || o->epochs < 0 || o->batch <= 0) return MRF_E_INVALID;
+  if (n > 0xFFFFFFFFll) return fail(c, MRF_E_LIMIT, "mlp_train: more than 2^32 samples");
+  const uint32_t batch = (uint32_t)std::min<int64_t>(o->batch, n);
+  int rc = ensure_train(c, (size_t)n, batch);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_mlp, params, kMlpParams * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tX, X, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tT, t, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  launch_zero_words(c->d_tM, kMlpParams * 4, c->stream);
+  launch_zero_words(c->d_tV, kMlpParams * 4, c->stream);
+  std::vector<uint32_t> sel((size_t)n);
+  std::mt19937_64 rng(o->seed);
+  float lr = o->lr;
+  double b1t = 1.0, b2t = 1.0;
+  for (int e = 0; e < o->epochs; ++e) {
+    for (size_t i = 0; i < sel.size(); ++i) sel[i] = (uint32_t)i;
+    for (size_t i = sel.size() - 1; i > 0; --i) {  // Fisher-Yates
+      const size_t j = (size_t)(rng() % (uint64_t)(i + 1));
+      std::swap(sel[i], sel[j]);
+    }
+    // the previous epoch's kernels may still read d_tSel
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_tSel, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, c->stream));
+    launch_zero_words(c->d_tLoss, 8, c->stream);
+    for (uint64_t off = 0; off < (uint64_t)n; off += batch) {
+      const uint32_t cnt = (uint32_t)std::min<uint64_t>(batch, (uint64_t)n - off);
+      launch_mlp_grad(c->d_mlp, c->d_tX, c->d_tT, c->d_tSel, (uint32_t)off, cnt, 1.0f / (float)cnt, c->d_tPart,
+                      c->d_tLpart, c->stream);
+      launch_mlp_reduce(c->d_tPart, c->d_tLpart, mlp_grad_waves(cnt), c->d_tG, c->d_tLoss, 1.0 / (double)n,
+                        c->stream);
+      b1t *= o->beta1;
+      b2t *= o->beta2;
+      launch_adam(c->d_mlp, c->d_tG, c->d_tM, c->d_tV, lr, o->beta1, o->beta2, o->eps, (float)(1.0 - b1t),
+                  (float)(1.0 - b2t), c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    double l = 0;
+    HIPCHK(c, hipMemcpyAsync(&l, c->d_tLoss, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (epoch_loss) epoch_loss[e] = l;
+    lr *= o->lr_decay;
+  }
+  HIPCHK(c, hipMemcpyAsync(params, c->d_mlp, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MRF_OK;
 }
 

@@ -173,4 +173,34 @@ void launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void launch_zero_words(void* p, size_t bytes, hipStream_t s);
 
+// ---- learned warm start (warm.hip): features, MLP, warm-start update
+constexpr int kMlpIn = 32, kMlpHid = 64;
+constexpr int kMlpW1 = 0;                          // [64][32]
+constexpr int kMlpB1 = kMlpW1 + kMlpHid * kMlpIn;  // [64]
+constexpr int kMlpW2 = kMlpB1 + kMlpHid;           // [64][64]
+constexpr int kMlpB2 = kMlpW2 + kMlpHid * kMlpHid; // [64]
+constexpr int kMlpW3 = kMlpB2 + kMlpHid;           // [64]
+constexpr int kMlpB3 = kMlpW3 + kMlpHid;           // [1]
+constexpr int kMlpParams = kMlpB3 + 1;             // 6337
+uint32_t mlp_grad_waves(uint32_t cnt);
+void launch_mlp_fwd(const float* P, const float* X, uint32_t n, float* y, hipStream_t s);
+void launch_mlp_grad(const float* P, const float* X, const float* T, const uint32_t* sel, uint32_t off,
+                     uint32_t cnt, float scale, float* partial, double* lpart, hipStream_t s);
+void launch_mlp_reduce(const float* partial, const double* lpart, uint32_t nw, float* grad, double* loss,
+                       double scale, hipStream_t s);
+void launch_adam(float* P, const float* G, float* m, float* v, float lr, float b1, float b2, float eps, float c1,
+                 float c2, hipStream_t s);
+void launch_feat_hits(const CamGeom& c, const GridGeom& g, const float* depth, const int32_t* slot, uint32_t* hits,
+                      hipStream_t s);
+void launch_feat_pairs(const uint32_t* pair_vox, const float* pair_c, const float* M, uint32_t npairs, float* pass,
+                       float* msum, float* mmax, uint32_t* nc, hipStream_t s);
+void launch_feat_flags(const uint32_t* nc, size_t n, uint32_t* flag, hipStream_t s);
+void launch_features(const GridGeom& g, const int32_t* slot, const int32_t* slot_lin, uint32_t nslots,
+                     const uint32_t* hits, const float* pass, const float* msum, const float* mmax, const uint32_t* nc,
+                     const float* B, float prior, const uint32_t* row, float* X, uint32_t* vox, hipStream_t s);
+void launch_warm_apply(const uint32_t* vox, const float* pred, uint32_t n, const uint32_t* nc, float* B,
+                       float* delta, hipStream_t s);
+void launch_warm_pairs(const uint32_t* pair_vox, const float* pair_c, uint32_t npairs, const float* delta, float* M,
+                       hipStream_t s);
+
 }  // namespace mrf

@@ -327,6 +327,29 @@ __global__ void k_mark_endpoints(CamGeom c, GridGeom g, const float* __restrict_
       for (int k = lo[2]; k <= hi[2]; ++k) bflags[(i * g.by + j) * g.bz + k] = 1;
 }
 
+// Depth endpoints per voxel (a feature of the learned warm start): the voxel
+// holding the endpoint of every valid pixel, with the endpoint formula and
+// map-box gate of k_mark_endpoints; endpoints in inactive bricks are dropped.
+__global__ void k_feat_hits(CamGeom c, GridGeom g, const float* __restrict__ depth,
+                            const int32_t* __restrict__ slot, uint32_t* __restrict__ hits) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= tiled_pixels(c)) return;
+  PixelCtx p = load_pixel(c, g, depth, pix);
+  if (!p.ok) return;
+  const int M[3] = {g.mx, g.my, g.mz};
+  int v[3];
+  for (int a = 0; a < 3; ++a) {
+    double P = (double)p.d * (double)p.rs.Di[a];
+    P = (double)p.rs.Oi[a] + P;
+    P = P / kFixScale;
+    if (!(P >= 0.0 && P < (double)M[a])) return;
+    v[a] = (int)floor(P);
+  }
+  const int sl = slot[((v[0] >> kBrickLog) * g.by + (v[1] >> kBrickLog)) * g.bz + (v[2] >> kBrickLog)];
+  if (sl < 0) return;
+  atomicAdd(&hits[(size_t)sl * kBrickVox + ((v[0] & 7) * 64 + (v[1] & 7) * 8 + (v[2] & 7))], 1u);
+}
+
 __global__ void k_count(CamGeom c, GridGeom g, const float* __restrict__ depth,
                         const float* __restrict__ sig, const float* __restrict__ bia,
                         const int32_t* __restrict__ slot, uint32_t* __restrict__ counts,
@@ -589,6 +612,12 @@ void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, c
                       const float* bia, uint8_t* bflags, hipStream_t s) {
   const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_mark_rays, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, bflags);
+}
+
+void launch_feat_hits(const CamGeom& c, const GridGeom& g, const float* depth, const int32_t* slot, uint32_t* hits,
+                      hipStream_t s) {
+  const int n = tiled_pixels(c);
+  hipLaunchKernelGGL(k_feat_hits, dim3((n + 255) / 256), dim3(256), 0, s, c, g, depth, slot, hits);
 }
 
 void launch_mark_endpoints(const CamGeom& c, const GridGeom& g, const float* depth, uint8_t* bflags,

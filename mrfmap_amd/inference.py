@@ -103,6 +103,75 @@ class GVDBInference:
         self.last_stats = st.as_dict()
         return self.last_stats
 
+    def perform_inference_warm(self, params, max_iters=0, tol=0.0, fallback=True):
+        """perform_inference with the learned warm start after the first
+        iteration (``params``: MLP_PARAMS floats, see mrfmap_amd.warmstart);
+        with ``fallback`` the warm start is undone if the residual grows."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (_lib.MLP_PARAMS,):
+            raise ValueError(f"params must hold {_lib.MLP_PARAMS} floats")
+        st = _lib.MrfStats()
+        rc = self._L.mrf_perform_inference_warm(self._h, _ptr(p), int(max_iters), float(tol), int(bool(fallback)),
+                                                ctypes.byref(st))
+        _lib.check(rc, self._h)
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def features(self):
+        """(X n x 32 float32, pool index n uint32) of the warm-start features of
+        the current state (call after perform_inference)."""
+        n = ctypes.c_int64()
+        _lib.check(self._L.mrf_features(self._h, None, None, 0, ctypes.byref(n)), self._h)
+        X = np.zeros((max(n.value, 1), _lib.MLP_FEATURES), dtype=np.float32)
+        vox = np.zeros(max(n.value, 1), dtype=np.uint32)
+        _lib.check(self._L.mrf_features(self._h, _ptr(X), _ptr(vox), n.value, ctypes.byref(n)), self._h)
+        return X[:n.value], vox[:n.value]
+
+    def warm_predictions(self):
+        """(pool index, predicted log-odds) of the last warm start."""
+        n = ctypes.c_int64()
+        _lib.check(self._L.mrf_warm_predictions(self._h, None, None, 0, ctypes.byref(n)), self._h)
+        vox = np.zeros(max(n.value, 1), dtype=np.uint32)
+        pred = np.zeros(max(n.value, 1), dtype=np.float32)
+        _lib.check(self._L.mrf_warm_predictions(self._h, _ptr(vox), _ptr(pred), n.value, ctypes.byref(n)), self._h)
+        return vox[:n.value], pred[:n.value]
+
+    # ------------------------------------------------------------ warm-start MLP on this context's GPU
+    def mlp_forward(self, params, X):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, _lib.MLP_FEATURES)
+        y = np.zeros(len(X), dtype=np.float32)
+        _lib.check(self._L.mrf_mlp_forward(self._h, _ptr(p), _ptr(X), len(X), _ptr(y)), self._h)
+        return y
+
+    def mlp_grad(self, params, X, t):
+        """(loss, gradient) of 1/(2n) sum (y - t)^2."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, _lib.MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(_lib.MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _lib.check(self._L.mrf_mlp_grad(self._h, _ptr(p), _ptr(X), _ptr(t), len(X), _ptr(g), ctypes.byref(loss)),
+                   self._h)
+        return loss.value, g
+
+    def mlp_train(self, params, X, t, **opts):
+        """Adam training on the GPU; returns (params, per-epoch mean loss).
+        opts: epochs, batch, lr, beta1, beta2, eps, lr_decay, seed."""
+        o = _lib.MrfTrainOpts()
+        self._L.mrf_train_opts_default(ctypes.byref(o))
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown training option {k}")
+            setattr(o, k, v)
+        p = np.array(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, _lib.MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        losses = np.zeros(max(o.epochs, 1), dtype=np.float64)
+        _lib.check(self._L.mrf_mlp_train(self._h, _ptr(p), _ptr(X), _ptr(t), len(X), ctypes.byref(o),
+                                         _ptr(losses)), self._h)
+        return p, losses[:o.epochs]
+
     def prepare(self):
         _lib.check(self._L.mrf_prepare(self._h), self._h)
 

@@ -77,6 +77,9 @@ def lib():
                                         ctypes.POINTER(ctypes.c_int64), P, P]
         L.orc_perform_inference.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p,
                                             ctypes.POINTER(ctypes.c_int64)]
+        L.orc_continue_inference.argtypes = [P, ctypes.c_int, ctypes.c_double, f64p]
+        L.orc_features.argtypes = [P, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.orc_warm_start.argtypes = [P, P, P, ctypes.c_int64]
         L.orc_get_alphas.argtypes = [P, P]
         L.orc_get_beliefs.argtypes = [P, P]
         L.orc_ray_messages.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_double, P, P, ctypes.c_int,
@@ -215,6 +218,31 @@ class Oracle:
         if it < 0:
             raise RuntimeError(f"orc_perform_inference failed ({it})")
         return {"iters": it, "residual": res.value, "updates_per_iter": upd.value}
+
+    def continue_inference(self, max_iters, tol=0.0):
+        """Iterations without the per-call message reset."""
+        res = ctypes.c_double()
+        it = lib().orc_continue_inference(self._h, int(max_iters), float(tol), ctypes.byref(res))
+        if it < 0:
+            raise RuntimeError(f"orc_continue_inference failed ({it})")
+        return {"iters": it, "residual": res.value}
+
+    def features(self):
+        """(X n x 32 float32, pool index n) of the warm-start features."""
+        n = ctypes.c_int64()
+        lib().orc_features(self._h, None, None, 0, ctypes.byref(n))
+        X = np.zeros((max(n.value, 1), 32), dtype=np.float32)
+        vox = np.zeros(max(n.value, 1), dtype=np.uint32)
+        rc = lib().orc_features(self._h, _ptr(X), _ptr(vox), n.value, ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"orc_features failed ({rc})")
+        return X[:n.value], vox[:n.value]
+
+    def warm_start(self, vox, pred):
+        vox = np.ascontiguousarray(vox, dtype=np.uint32)
+        pred = _f64(pred)
+        if lib().orc_warm_start(self._h, _ptr(vox), _ptr(pred), len(vox)) != 0:
+            raise RuntimeError("orc_warm_start failed")
 
     def alphas(self):
         out = np.zeros((self.num_bricks(), 512), dtype=np.float64)

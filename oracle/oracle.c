@@ -729,11 +729,12 @@ static double marginal(double b) {
 
 static double clampd(double x) { return x < F_EPS ? F_EPS : (x > 1.0 - F_EPS ? 1.0 - F_EPS : x); }
 
+static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, int64_t* updates_per_iter);
+
 int orc_perform_inference(orc_ctx* c, const int* ids, int n_ids, int max_iters, double tol, double* residual,
                           int64_t* updates_per_iter) {
   if (!c) return -1;
   if (orc_prepare(c)) return -1;
-  const int ln = c->p.lookup_n;
   const int64_t nv = c->nslots * 512;
   if (ids) {
     for (int i = 0; i < n_ids; ++i) {
@@ -743,6 +744,19 @@ int orc_perform_inference(orc_ctx* c, const int* ids, int n_ids, int max_iters, 
   } else {
     for (int i = 0; i < c->ncams; ++i) memset(c->cams[i].M, 0, sizeof(double) * (size_t)nv);
   }
+  return bp_iterate(c, max_iters, tol, residual, updates_per_iter);
+}
+
+int orc_continue_inference(orc_ctx* c, int max_iters, double tol, double* residual) {
+  if (!c) return -1;
+  if (orc_prepare(c)) return -1;
+  return bp_iterate(c, max_iters, tol, residual, NULL);
+}
+
+/* BP iterations from the current beliefs and messages (no reset). */
+static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, int64_t* updates_per_iter) {
+  const int ln = c->p.lookup_n;
+  const int64_t nv = c->nslots * 512;
   int64_t upd = 0;
   for (int ci = 0; ci < c->ncams; ++ci)
     for (int64_t i = 0; i < c->cams[ci].rb.n; ++i) upd += c->cams[ci].rb.rec[2 * i] != GAP;
@@ -810,6 +824,143 @@ int orc_perform_inference(orc_ctx* c, const int* ids, int n_ids, int max_iters, 
   free(live);
   if (residual) *residual = res;
   return it;
+}
+
+/* ------------------------------------------------------------ warm start
+ * Features of the learned warm start, as mrfmap_amd/csrc/warm.hip defines
+ * them (no counterpart in the reference), from the current state in FP64:
+ * one row of 32 per voxel with C_c > 0.1 for some camera, in pool order. */
+#define ORC_LF 9.2102404
+
+static int64_t pool_index(const orc_ctx* c, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= c->N[0] || y >= c->N[1] || z >= c->N[2]) return -1;
+  const int s = c->slot[((x >> 3) * c->B3[1] + (y >> 3)) * c->B3[2] + (z >> 3)];
+  if (s < 0) return -1;
+  return (int64_t)s * 512 + ((x & 7) * 64 + (y & 7) * 8 + (z & 7));
+}
+
+int orc_features(orc_ctx* c, float* X, uint32_t* vox, int64_t cap, int64_t* n_out) {
+  if (!c || orc_prepare(c)) return -1;
+  const int64_t nv = c->nslots * 512;
+  double* pass = (double*)calloc((size_t)nv, sizeof(double));
+  double* msum = (double*)calloc((size_t)nv, sizeof(double));
+  double* mmax = (double*)calloc((size_t)nv, sizeof(double));
+  uint32_t* nc = (uint32_t*)calloc((size_t)nv, sizeof(uint32_t));
+  uint32_t* hits = (uint32_t*)calloc((size_t)nv, sizeof(uint32_t));
+  for (int ci = 0; ci < c->ncams; ++ci) {
+    ocam* k = &c->cams[ci];
+    for (int64_t q = 0; q < k->npairs; ++q) {
+      const uint32_t v = k->pair_vox[q];
+      const double C = k->pair_c[q];
+      pass[v] += C;
+      if (C > 0.1) {
+        const double m = k->M[v];
+        msum[v] += m;
+        mmax[v] = nc[v] ? (m > mmax[v] ? m : mmax[v]) : m;
+        nc[v] += 1;
+      }
+    }
+    /* depth endpoints: the activation's endpoint formula and map-box gate */
+    for (int y = 0; y < k->rows; ++y)
+      for (int x = 0; x < k->cols; ++x) {
+        float d = k->depth[(size_t)y * k->cols + x];
+        int64_t Oi[3], Di[3];
+        double norm;
+        if (!pixel_ok(c, k, x, y, d) || !setup_ray(c, k, x, y, Oi, Di, &norm)) continue;
+        int v3[3], ok = 1;
+        for (int a = 0; a < 3; ++a) {
+          double P = (double)d * (double)Di[a];
+          P = (double)Oi[a] + P;
+          P = P / FIXS;
+          if (!(P >= 0.0 && P < (double)c->Mx[a])) {
+            ok = 0;
+            break;
+          }
+          v3[a] = (int)floor(P);
+        }
+        if (!ok) continue;
+        const int64_t pv = pool_index(c, v3[0], v3[1], v3[2]);
+        if (pv >= 0) hits[pv] += 1;
+      }
+  }
+  int64_t n = 0;
+  for (int64_t v = 0; v < nv; ++v) {
+    if (nc[v] == 0) continue;
+    if (n < cap) {
+      const int lin = c->slot_lin[v / 512], t = (int)(v % 512);
+      const int x = (lin / (c->B3[1] * c->B3[2])) * 8 + (t >> 6);
+      const int y = ((lin / c->B3[2]) % c->B3[1]) * 8 + ((t >> 3) & 7);
+      const int z = (lin % c->B3[2]) * 8 + (t & 7);
+      float* r = X + n * 32;
+      r[0] = (float)log1p((double)hits[v]);
+      r[1] = (float)log1p(pass[v]);
+      r[2] = (float)((msum[v] / nc[v]) / ORC_LF);
+      r[3] = (float)(mmax[v] / ORC_LF);
+      int best = 16;
+      for (int a = -3; a <= 3; ++a)
+        for (int b = -3; b <= 3; ++b)
+          for (int e = -3; e <= 3; ++e) {
+            const int d2 = a * a + b * b + e * e;
+            if (d2 >= best) continue;
+            const int64_t u = pool_index(c, x + a, y + b, z + e);
+            if (u >= 0 && hits[u] > 0) best = d2;
+          }
+      r[4] = (float)(sqrt((double)best) / 4.0);
+      int f = 5;
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          for (int e = -1; e <= 1; ++e) {
+            const int64_t u = pool_index(c, x + a, y + b, z + e);
+            double q = (u >= 0 ? c->B[u] : c->prior_lo) / ORC_LF;
+            r[f++] = (float)(q < -3.0 ? -3.0 : (q > 3.0 ? 3.0 : q));
+          }
+      vox[n] = (uint32_t)v;
+    }
+    ++n;
+  }
+  free(pass);
+  free(msum);
+  free(mmax);
+  free(nc);
+  free(hits);
+  *n_out = n;
+  return n <= cap ? 0 : -7;
+}
+
+/* Warm start from predicted log-odds pred[i] of voxel vox[i]: the belief
+ * moves to the prediction (kept where prediction and belief are saturated on
+ * the same side and the belief is stronger), and every camera message with
+ * C_c > 0.1 moves by the same share (B' - B) / n_c, so B = L0 + sum_c M_c
+ * keeps holding; the marginals of the next residual start from B'. */
+int orc_warm_start(orc_ctx* c, const uint32_t* vox, const double* pred, int64_t n) {
+  if (!c || orc_prepare(c)) return -1;
+  const int64_t nv = c->nslots * 512;
+  uint32_t* nc = (uint32_t*)calloc((size_t)nv, sizeof(uint32_t));
+  double* delta = (double*)calloc((size_t)nv, sizeof(double));
+  for (int ci = 0; ci < c->ncams; ++ci)
+    for (int64_t q = 0; q < c->cams[ci].npairs; ++q)
+      if (c->cams[ci].pair_c[q] > 0.1) nc[c->cams[ci].pair_vox[q]] += 1;
+  for (int64_t i = 0; i < n; ++i) {
+    const uint32_t v = vox[i];
+    if ((int64_t)v >= nv || nc[v] == 0) {
+      free(nc);
+      free(delta);
+      return -1;
+    }
+    const double b = c->B[v], p = pred[i];
+    double bn = p;
+    if ((p <= -ORC_LF && b <= p) || (p >= ORC_LF && b >= p)) bn = b;
+    delta[v] = (bn - b) / nc[v];
+    c->B[v] = bn;
+  }
+  for (int ci = 0; ci < c->ncams; ++ci) {
+    ocam* k = &c->cams[ci];
+    for (int64_t q = 0; q < k->npairs; ++q)
+      if (k->pair_c[q] > 0.1) k->M[k->pair_vox[q]] += delta[k->pair_vox[q]];
+  }
+  free(nc);
+  free(delta);
+  return 0;
 }
 
 /* Map evaluation from a camera, FP64 (the device path is FP32): every pixel

@@ -66,6 +66,13 @@ int orc_get_traversal(orc_ctx* c, int cam, int64_t* n_rays, int64_t* n_records, 
  * iterations run; *residual gets the last max |delta marginal|. */
 int orc_perform_inference(orc_ctx* c, const int* ids, int n_ids, int max_iters, double tol, double* residual,
                           int64_t* updates_per_iter);
+/* Iterations from the current beliefs and messages, without the per-call
+ * message reset (the warm-started continuation). */
+int orc_continue_inference(orc_ctx* c, int max_iters, double tol, double* residual);
+/* Warm-start features (32 per touched voxel, pool order) and the warm-start
+ * update from given predictions; mrfmap_amd/csrc/warm.hip defines both. */
+int orc_features(orc_ctx* c, float* X, uint32_t* vox, int64_t cap, int64_t* n);
+int orc_warm_start(orc_ctx* c, const uint32_t* vox, const double* pred, int64_t n);
 int orc_get_alphas(orc_ctx* c, double* out /* nbricks x 512 */);
 int orc_get_beliefs(orc_ctx* c, double* out);
 

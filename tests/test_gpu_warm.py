@@ -1,0 +1,148 @@
+"""GPU tests of the learned warm start: features, the BF16-MFMA MLP (forward,
+gradient, training) and the warm-started BP, each against the oracle
+(oracle/oracle.c, oracle/mlp.py) on the same inputs.
+
+Tolerances: features are FP32 on the device and FP64 in the oracle (1e-4
+on the normalised values; endpoint counts and distances exact).  The MLP
+oracle rounds to BF16 where the device does, so forward outputs agree to
+FP32 accumulation order (1e-4 relative to the output scale) and gradients
+to 2e-3 of each block's largest entry (an FP64 value can round to the other
+BF16 neighbour than its FP32 counterpart).  The warm-started BP is checked
+with the device's own predictions fed to the oracle: marginals within the
+BP parity bound of the plain path.
+"""
+import numpy as np
+import pytest
+
+from mrfmap_amd import _lib, synth
+from oracle import mlp as omlp
+
+from helpers import gpu_for, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+MARG_TOL = 2e-4
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = omlp.init_params(seed)
+    p[omlp.B1] = rng.normal(0, 0.3, omlp.HID)
+    p[omlp.B2] = rng.normal(0, 0.3, omlp.HID)
+    p[omlp.B3] = 0.3
+    return p.astype(np.float32)
+
+
+def test_features_match_oracle():
+    p, frames, _ = synth.make_workload("small")
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.perform_inference(max_iters=1)
+    o.perform_inference(1)
+    Xg, vg = g.features()
+    Xo, vo = o.features()
+    print(f"features: gpu {len(vg)} rows, oracle {len(vo)} rows")
+    common, ig, io = np.intersect1d(vg, vo, assume_unique=True, return_indices=True)
+    # the touched set differs only where an FP32 and an FP64 chord sum sit on either side of 0.1
+    assert len(common) >= 0.999 * max(len(vg), len(vo))
+    np.testing.assert_array_equal(np.sort(vg), vg)  # pool order
+    A, Bo = Xg[ig].astype(np.float64), Xo[io].astype(np.float64)
+    np.testing.assert_array_equal(A[:, 0], Bo[:, 0])  # endpoint counts
+    np.testing.assert_allclose(A[:, 4], Bo[:, 4], rtol=0, atol=1e-6)  # distance
+    err = np.abs(A - Bo).max(axis=0)
+    print("max-abs feature error per column", np.array2string(err, precision=1))
+    assert err.max() <= 1e-4
+
+
+@pytest.mark.parametrize("n", [1, 37, 1000])
+def test_mlp_forward_matches_oracle(n):
+    p, frames, _ = synth.make_workload("small", frames=1)
+    g = gpu_for(p, frames)
+    params = _random_params(5)
+    X = np.random.default_rng(n).normal(0, 1, (n, 32)).astype(np.float32)
+    y = g.mlp_forward(params, X)
+    yo, _ = omlp.forward(params, X, bf16_ops=True)
+    scale = max(1.0, np.abs(yo).max())
+    print(f"n={n} max-abs {np.abs(y - yo).max():.2e} (scale {scale:.2f})")
+    assert np.abs(y - yo).max() <= 1e-4 * scale
+
+
+@pytest.mark.parametrize("n", [7, 100, 4099])
+def test_mlp_gradient_matches_oracle(n):
+    p, frames, _ = synth.make_workload("small", frames=1)
+    g = gpu_for(p, frames)
+    params = _random_params(11)
+    rng = np.random.default_rng(100 + n)
+    X = rng.normal(0, 1, (n, 32)).astype(np.float32)
+    t = rng.normal(0, 2, n).astype(np.float32)
+    loss, grad = g.mlp_grad(params, X, t)
+    lo, go = omlp.loss_and_grad(params, X, t, bf16_ops=True)
+    assert abs(loss - lo) <= 1e-5 * max(1.0, lo)
+    for name, sl in (("W1", omlp.W1), ("b1", omlp.B1), ("W2", omlp.W2), ("b2", omlp.B2), ("w3", omlp.W3),
+                     ("b3", omlp.B3)):
+        d = np.abs(grad[sl] - go[sl]).max()
+        ref = np.abs(go[sl]).max()
+        print(f"n={n} {name}: max-abs {d:.2e} of {ref:.2e}")
+        assert d <= 2e-3 * ref + 1e-9, name
+    # bitwise reproducible (no atomics)
+    loss2, grad2 = g.mlp_grad(params, X, t)
+    assert loss2 == loss
+    np.testing.assert_array_equal(grad2.view(np.uint32), grad.view(np.uint32))
+
+
+def test_mlp_training_learns_a_teacher():
+    """A random teacher network labels Gaussian inputs; Adam on the device
+    brings a fresh student's loss well below its start, reproducibly."""
+    p, frames, _ = synth.make_workload("small", frames=1)
+    g = gpu_for(p, frames)
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (20000, 32)).astype(np.float32)
+    t = omlp.forward(_random_params(21), X, bf16_ops=False)[0].astype(np.float32)
+    p0 = omlp.init_params(3)
+    l0 = g.mlp_grad(p0, X, t)[0]
+    p1, losses = g.mlp_train(p0, X, t, epochs=15, batch=1024, lr=2e-3)
+    l1 = g.mlp_grad(p1, X, t)[0]
+    print(f"loss {l0:.4f} -> {l1:.4f}; epochs {np.array2string(losses, precision=4)}")
+    assert l1 < 0.1 * l0
+    assert losses[-1] < losses[0]
+    p2, _ = g.mlp_train(p0, X, t, epochs=15, batch=1024, lr=2e-3)
+    np.testing.assert_array_equal(p1.view(np.uint32), p2.view(np.uint32))
+
+
+def test_warm_start_matches_oracle():
+    """Warm-started BP on the device against the oracle run from the same
+    predictions: one plain iteration, the warm-start update, three more."""
+    p, frames, _ = synth.make_workload("small")
+    g = gpu_for(p, frames)
+    params = _random_params(8)
+    params[omlp.B3] = -2.0
+    st = g.perform_inference_warm(params, max_iters=4, fallback=False)
+    assert st["iters"] == 4 and st["warm"] == 1 and st["warm_voxels"] > 0
+    vox, pred = g.warm_predictions()
+    assert len(vox) == st["warm_voxels"]
+    o = oracle_for(p, frames)
+    o.perform_inference(1)
+    _, vo = o.features()
+    common = np.intersect1d(vox, vo, assume_unique=True)
+    assert len(common) >= 0.999 * len(vo)
+    o.warm_start(common, pred[np.searchsorted(vox, common)].astype(np.float64))
+    o.continue_inference(3)
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"warm-started BP: max-abs marginal error {err.max():.3e}")
+    assert err.max() <= MARG_TOL
+
+
+def test_warm_start_falls_back_when_the_residual_grows():
+    """A network that calls everything occupied makes the second iteration's
+    residual exceed the first's; the warm start is undone and the rest is
+    plain BP from the first iteration: the same bits as plain BP with one
+    iteration less (the undone iteration)."""
+    p, frames, _ = synth.make_workload("small")
+    bad = np.zeros(_lib.MLP_PARAMS, np.float32)
+    bad[omlp.B3] = 9.0
+    g = gpu_for(p, frames)
+    st = g.perform_inference_warm(bad, max_iters=6, fallback=True)
+    assert st["warm"] == 2 and st["iters"] == 6
+    ref = gpu_for(p, frames)
+    ref.perform_inference(max_iters=5)
+    np.testing.assert_array_equal(g.get_beliefs().view(np.uint32), ref.get_beliefs().view(np.uint32))
