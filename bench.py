@@ -7,10 +7,15 @@ brick activation, FP32 messages.  A step is one BP iteration: every camera's
 ray-message pass + epilogue + the FP64 residual, with all keyframes, lists
 and map state resident in HBM.
 
-Prints one JSON line: updates/s over the timed steps, the time-to-converged
-(max marginal change < 1e-4, from "keyframes uploaded", traversal included),
-the roofline of the ray-message kernel and the CPU baseline (the FP64 oracle
-on a bounded sample of the same workload, one host core).
+Prints one JSON line: updates/s over the timed steps; the time-to-converged
+map -- from "keyframes uploaded" (traversal included) to a map within 1e-3
+max-abs marginal error of plain FP32 BP iterated to a residual below 3e-5
+(the FP64 map is out of the CPU oracle's reach at this size; FP32 and FP64
+fixed points agree to ~1e-5 where the oracle can run) -- for plain FP32 BP,
+for the learned warm start (mrfmap_amd/data/warm_mlp.npz) with FP32
+messages and with FP16 ray->voxel messages; the roofline of the ray-message
+kernels and the CPU baseline (the FP64 oracle on a bounded sample of the
+same workload, one host core).
 """
 import argparse
 import json
@@ -32,7 +37,7 @@ def parse():
     ap.add_argument("--workload", default="real")
     ap.add_argument("--msg", default="f32", choices=["f32", "f16", "bf16", "mixed-f16", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--tol", type=float, default=3e-5, help="residual of the reference run")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged run (profiling)")
     ap.add_argument("--launch", default="timed", choices=["timed", "graph"],
                     help="launch mode of the timed steps: kernels one by one with per-kernel HIP events "
@@ -105,6 +110,35 @@ def cpu_comparator(wl):
                       f"({n} rays, {dt:.2f} s)"}
 
 
+def time_to_converged(p, frames, ref):
+    """Plain vs warm-started BP to a map within 1e-3 of ``ref`` (fresh contexts)."""
+    from mrfmap_amd import _lib
+    from mrfmap_amd import warmstart as W
+
+    params, meta = W.load_weights()
+    keep = ("ms", "geometry_ms", "iters", "error", "reached", "warm", "warm_ms")
+    legs = {}
+    legs["plain_f32"] = W.time_to_error(p, frames, ref, 1e-3, max_iters=300)
+    legs["warm_f32"] = W.time_to_error(p, frames, ref, 1e-3, params=params, warm_after=meta["warm_after"],
+                                       max_iters=300)
+    msg0 = p.msg_dtype
+    try:
+        p.msg_dtype = _lib.MSG_MIXED_F16
+        legs["warm_mixed_f16"] = W.time_to_error(p, frames, ref, 1e-3, params=params,
+                                                 warm_after=meta["warm_after"], max_iters=300)
+    finally:
+        p.msg_dtype = msg0
+    out = {k: {f: v[f] for f in keep} for k, v in legs.items()}
+    for k, v in out.items():
+        print(f"[bench] time-to-1e-3 {k}: {v['iters']} iters {v['ms']:.1f} ms reached={v['reached']} "
+              f"error {v['error']:.1e}", file=sys.stderr, flush=True)
+    best = min((v["ms"] for k, v in out.items() if k != "plain_f32" and v["reached"]), default=None)
+    out["speedup_warm_vs_plain"] = (out["plain_f32"]["ms"] / best) if best else None
+    out["warm_network"] = {"file": "mrfmap_amd/data/warm_mlp.npz", "warm_after": meta["warm_after"],
+                           "training_seeds": meta["seeds"], "samples": meta["samples"]}
+    return out
+
+
 def measured_traffic(a):
     """HBM bytes per ray-kernel launch from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated on a
@@ -141,16 +175,20 @@ def main():
             g.add_camera_with_depth(pose, depth, K)
         return g
 
-    # time-to-converged from a fresh map (geometry build included)
+    # reference map: plain BP from a fresh map to a residual below tol; its
+    # context then serves the timed steps
     g = build()
+    ttc = None
     if a.no_converge:
         conv = g.perform_inference(max_iters=1)
         conv["t_total_ms"] = None
     else:
-        conv = g.perform_inference(max_iters=200, tol=a.tol)
-        print(f"[bench] converged: {conv['iters']} iters residual {conv['residual']:.2e} "
+        conv = g.perform_inference(max_iters=400, tol=a.tol)
+        print(f"[bench] reference: {conv['iters']} iters residual {conv['residual']:.2e} "
               f"total {conv['t_total_ms']:.1f} ms (geometry {conv['t_geometry_ms']:.1f} ms)", file=sys.stderr,
               flush=True)
+        if msg == _lib.MSG_F32:
+            ttc = time_to_converged(p, frames, g.get_alphas())
 
     # throughput: W warmup steps then exactly K timed steps on resident data
     modes = {"timed": _lib.LAUNCH_TIMED, "graph": _lib.LAUNCH_GRAPH}
@@ -221,9 +259,9 @@ def main():
             "launch": a.launch,
             f"ms_per_step_{other}": dt_other * 1e3 / a.steps,
         },
-        "time_to_converged_ms": conv["t_total_ms"],
-        "iters_to_converge": conv["iters"],
-        "converged_residual": conv["residual"],
+        "time_to_converged": ttc,
+        "reference_run": {"iters": conv["iters"], "residual": conv["residual"], "tol": a.tol,
+                          "ms": conv["t_total_ms"]},
         "geometry_ms": conv["t_geometry_ms"],
         "roofline": {
             "kernel": "k_front_sums + k_ray_window (one camera pass)",

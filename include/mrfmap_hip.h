@@ -111,6 +111,8 @@ typedef struct mrf_stats {
                                   because the residual grew (plain BP from the first iteration on) */
   int64_t warm_voxels;         /* voxels the warm start predicted */
   double t_warm_ms;            /* features + MLP + warm-start update, wall time */
+  double ref_error;            /* max-abs marginal difference to the mrf_set_reference map after the
+                                  last iteration, -1 without a reference */
 } mrf_stats;
 
 /* gvdb_params_t() defaults (GVDBParams.h:32-55). */
@@ -198,7 +200,7 @@ int mrf_set_eval_thresholds(mrf_ctx* ctx, float occ_thresh, float acc_thresh);
 /* ---- Learned warm start (no counterpart in the reference, whose BP always
  * starts from the prior with all camera messages 0, GVDBInference.cpp:128-134).
  *
- * After the first BP iteration every voxel with C_c > 0.1 for some camera
+ * After the first BP iterations every voxel with C_c > 0.1 for some camera
  * gets MRF_MLP_FEATURES features: 0 log1p(depth endpoints in the voxel),
  * 1 log1p(summed chord length of all rays through it), 2 and 3 the mean and
  * the max over cameras of the camera messages M_c / 9.2102404, 4 the
@@ -224,13 +226,14 @@ typedef struct mrf_train_opts {
 } mrf_train_opts;
 
 void mrf_train_opts_default(mrf_train_opts* o);
-/* perform_inference with the warm start applied after the first iteration;
- * with fallback != 0 the warm start is undone (beliefs and messages return
- * to their state after the first iteration, plain BP continues) when a
- * residual exceeds the first iteration's (second iteration) or twice the
- * smallest warm residual so far (later iterations). */
-int mrf_perform_inference_warm(mrf_ctx* ctx, const float* params, int max_iters, double tol, int fallback,
-                               mrf_stats* out);
+/* perform_inference with the warm start applied after warm_after plain
+ * iterations (features of that state; the network must have been trained on
+ * features taken after as many iterations).  With growth > 0 the warm start
+ * is undone -- beliefs and messages return to their state before it and
+ * plain BP continues -- when, from the second warm iteration on, a residual
+ * exceeds both growth x the previous iteration's residual and 0.05. */
+int mrf_perform_inference_warm(mrf_ctx* ctx, const float* params, int warm_after, int max_iters, double tol,
+                               double growth, mrf_stats* out);
 /* Features of the current state (call after perform_inference): *n rows of
  * MRF_MLP_FEATURES floats (row-major) and their pool indices
  * (slot * 512 + voxel, the order of mrf_get_alphas), in pool order.  With X
@@ -247,6 +250,19 @@ int mrf_mlp_grad(mrf_ctx* ctx, const float* params, const float* X, const float*
                  double* loss);
 int mrf_mlp_train(mrf_ctx* ctx, float* params, const float* X, const float* t, int64_t n, const mrf_train_opts* o,
                   double* epoch_loss);
+
+/* Reference marginals (rows = num_bricks, cols = 512, the layout of
+ * mrf_get_alphas; call after mrf_prepare) for measuring convergence to a
+ * known map, e.g. an FP64 or tightly converged solution: after every
+ * iteration the GPU reduces max |alpha - ref| into mrf_stats.ref_error, and
+ * with stop_at > 0 perform_inference returns (converged = 1) once it is
+ * <= stop_at.  rows = 0 removes the reference.  No counterpart in the
+ * reference. */
+int mrf_set_reference(mrf_ctx* ctx, const float* alphas, int64_t rows, int64_t cols, double stop_at);
+/* Per-iteration residual and reference error (-1 without a reference) of
+ * the last perform_inference call: *n iterations, the first min(cap, *n)
+ * copied (either array may be null). */
+int mrf_get_trace(mrf_ctx* ctx, double* resid, double* err, int cap, int* n);
 
 /* MRF_LAUNCH_GRAPH or MRF_LAUNCH_TIMED (no counterpart: GVDB launched its
  * kernels one by one).  The environment variable MRF_NO_GRAPH=1 makes

@@ -28,7 +28,7 @@ EXPORTS = [
     "mrf_get_traversal", "mrf_last_error", "mrf_evaluate_camera", "mrf_evaluate_at_pose", "mrf_load_map",
     "mrf_set_eval_thresholds",
     "mrf_set_launch_mode", "mrf_train_opts_default", "mrf_perform_inference_warm", "mrf_features",
-    "mrf_warm_predictions", "mrf_mlp_forward", "mrf_mlp_grad", "mrf_mlp_train",
+    "mrf_warm_predictions", "mrf_mlp_forward", "mrf_mlp_grad", "mrf_mlp_train", "mrf_set_reference", "mrf_get_trace",
 ]
 MLP_FEATURES, MLP_PARAMS = 32, 6337
 
@@ -53,7 +53,7 @@ class MrfStats(ctypes.Structure):
         ("t_ray_kernel_ms", ctypes.c_double), ("t_epilogue_ms", ctypes.c_double),
         ("t_total_ms", ctypes.c_double), ("ray_sched", ctypes.c_int), ("t_autotune_ms", ctypes.c_double),
         ("window_updates_per_iter", ctypes.c_int64), ("graph", ctypes.c_int), ("warm", ctypes.c_int),
-        ("warm_voxels", ctypes.c_int64), ("t_warm_ms", ctypes.c_double),
+        ("warm_voxels", ctypes.c_int64), ("t_warm_ms", ctypes.c_double), ("ref_error", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -124,9 +124,11 @@ def load():
     L.mrf_set_launch_mode.argtypes = [P, ctypes.c_int]
     L.mrf_train_opts_default.argtypes = [ctypes.POINTER(MrfTrainOpts)]
     L.mrf_train_opts_default.restype = None
-    L.mrf_perform_inference_warm.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+    L.mrf_perform_inference_warm.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.POINTER(MrfStats)]
     L.mrf_features.argtypes = [P, P, P, ctypes.c_int64, i64p]
+    L.mrf_set_reference.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64, ctypes.c_double]
+    L.mrf_get_trace.argtypes = [P, P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mrf_warm_predictions.argtypes = [P, P, P, ctypes.c_int64, i64p]
     L.mrf_mlp_forward.argtypes = [P, P, P, ctypes.c_int64, P]
     L.mrf_mlp_grad.argtypes = [P, P, P, P, ctypes.c_int64, P, ctypes.POINTER(ctypes.c_double)]

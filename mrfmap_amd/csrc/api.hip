@@ -142,6 +142,12 @@ struct mrf_ctx {
   float* d_wpred = nullptr;
   size_t cap_wrows = 0;
   uint32_t wrows = 0;  // rows of the last feature extraction
+  // reference marginals for convergence measurements (mrf_set_reference)
+  float* d_ref = nullptr;
+  size_t ref_n = 0;
+  double ref_stop = 0.0;
+  unsigned long long* d_rerr = nullptr;
+  std::vector<double> trace_resid, trace_err;  // per iteration of the last call
   float* d_mlp = nullptr;
   float *d_snapB = nullptr, *d_snapM = nullptr;
   size_t cap_snapB = 0, cap_snapM = 0;
@@ -1107,7 +1113,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_wpass); dfree(&c->d_wmsum); dfree(&c->d_wmmax); dfree(&c->d_wdelta);
   dfree(&c->d_wnc); dfree(&c->d_whits); dfree(&c->d_wflag); dfree(&c->d_wrow); dfree(&c->d_slot_lin);
   dfree(&c->d_wX); dfree(&c->d_wvox); dfree(&c->d_wpred); dfree(&c->d_mlp);
-  dfree(&c->d_snapB); dfree(&c->d_snapM);
+  dfree(&c->d_snapB); dfree(&c->d_snapM); dfree(&c->d_ref); dfree(&c->d_rerr);
   dfree(&c->d_tX); dfree(&c->d_tT); dfree(&c->d_tG); dfree(&c->d_tM); dfree(&c->d_tV); dfree(&c->d_tPart);
   dfree(&c->d_tSel); dfree(&c->d_tLpart); dfree(&c->d_tLoss);
   drop_graph(c);
@@ -1172,10 +1178,10 @@ int mrf_prepare(mrf_ctx* c) {
 }
 
 // One perform_inference call; with warm != nullptr the learned warm start
-// is applied after the first iteration (and undone if the residual grows,
-// when fallback is set).
+// is applied after warm_after iterations and undone when a later residual
+// exceeds growth x the previous one (growth > 0) and 0.05.
 static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_iters, double tol, const float* warm,
-                         int fallback, mrf_stats* st) {
+                         int warm_after, double growth, mrf_stats* st) {
   if (!c) return MRF_E_INVALID;
   auto t0 = std::chrono::steady_clock::now();
   mrf_stats s;
@@ -1230,7 +1236,12 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
     HIPCHK(c, hipMemcpyAsync(c->d_mlp, warm, kMlpParams * sizeof(float), hipMemcpyHostToDevice, c->stream));
   }
   int warm_state = 0;  // 0: none, 1: applied, 2: applied and undone
-  double r1 = 0, best = 0;
+  double r_plain = 0, r_prev = 0;
+  if (c->ref_n && c->ref_n != nvox)
+    return fail(c, MRF_E_STATE, "perform_inference: the map changed since mrf_set_reference");
+  double rerr = -1.0;
+  c->trace_resid.clear();
+  c->trace_err.clear();
   for (; it < max_iters; ++it) {
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
     if (graph)
@@ -1238,11 +1249,18 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
     else
       HIPCHK(c, enqueue_iteration(c, ev.data() + 2));
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    uint64_t bits = 0;
+    uint64_t bits = 0, ebits = 0;
+    if (c->ref_n) {
+      launch_ref_error(c->d_A, c->d_upd, c->d_ref, nvox, c->d_rerr, c->stream);
+      HIPCHK(c, hipMemcpyAsync(&ebits, c->d_rerr, 8, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipMemcpyAsync(&bits, c->d_total + 1, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
     std::memcpy(&resid, &bits, 8);
+    if (c->ref_n) std::memcpy(&rerr, &ebits, 8);
+    c->trace_resid.push_back(resid);
+    c->trace_err.push_back(rerr);
     float ms = 0;
     if (!graph) {
       for (size_t ci = 0; ci < ncam; ++ci) {
@@ -1255,7 +1273,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
     HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
     titer += ms;
     bool check = true;
-    if (warm && it == 0) {
+    if (warm && it == warm_after - 1) {
       // features of the first-iteration state, MLP, warm-start update
       const auto w0 = std::chrono::steady_clock::now();
       uint32_t n = 0;
@@ -1271,21 +1289,20 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
       s.t_warm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
       s.warm_voxels = n;
       warm_state = 1;
-      r1 = best = resid;
+      r_plain = resid;
       check = false;
-    } else if (warm_state == 1 && fallback && resid > (it == 1 ? r1 : 2.0 * best)) {
-      // the residual grows: back to the plain state after the first iteration
+    } else if (warm_state == 1 && it > warm_after && growth > 0 && resid > growth * r_prev && resid > 0.05) {
+      // the residual grows: back to the plain state before the warm start
       if ((rc = snapshot(c, false))) return rc;
       launch_marginals(c->d_B, c->d_A, nvox, c->stream);
       HIPCHK(c, hipStreamSynchronize(c->stream));
       HIPCHK(c, hipGetLastError());
       warm_state = 2;
-      resid = r1;
+      resid = r_plain;
       check = false;
-    } else if (warm_state == 1 && resid < best) {
-      best = resid;
     }
-    if (check && tol > 0 && resid < tol) {
+    r_prev = resid;
+    if (check && ((tol > 0 && resid < tol) || (c->ref_n && c->ref_stop > 0 && rerr <= c->ref_stop))) {
       conv = true;
       ++it;
       break;
@@ -1302,19 +1319,21 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   s.ray_sched = c->ray_sched;
   s.t_autotune_ms = c->autotune_ms;
   s.warm = warm_state;
+  s.ref_error = rerr;
   if (st) *st = s;
   return MRF_OK;
 }
 
 int mrf_perform_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_iters, double tol, mrf_stats* st) {
-  return run_inference(c, ids, n_ids, max_iters, tol, nullptr, 0, st);
+  return run_inference(c, ids, n_ids, max_iters, tol, nullptr, 0, 0.0, st);
 }
 
-int mrf_perform_inference_warm(mrf_ctx* c, const float* params, int max_iters, double tol, int fallback,
-                               mrf_stats* st) {
+int mrf_perform_inference_warm(mrf_ctx* c, const float* params, int warm_after, int max_iters, double tol,
+                               double growth, mrf_stats* st) {
   if (!c) return MRF_E_INVALID;
   if (!params) return fail(c, MRF_E_INVALID, "perform_inference_warm: no MLP parameters");
-  return run_inference(c, nullptr, 0, max_iters, tol, params, fallback, st);
+  if (warm_after < 1) return fail(c, MRF_E_INVALID, "perform_inference_warm: warm_after must be >= 1");
+  return run_inference(c, nullptr, 0, max_iters, tol, params, warm_after, growth, st);
 }
 
 int mrf_features(mrf_ctx* c, float* X, uint32_t* vox, int64_t cap, int64_t* n) {
@@ -1342,6 +1361,42 @@ int mrf_warm_predictions(mrf_ctx* c, uint32_t* vox, float* pred, int64_t cap, in
     if (pred) HIPCHK(c, hipMemcpyAsync(pred, c->d_wpred, (size_t)c->wrows * 4, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MRF_OK;
+}
+
+int mrf_get_trace(mrf_ctx* c, double* resid, double* err, int cap, int* n) {
+  if (!c || !n) return MRF_E_INVALID;
+  *n = (int)c->trace_resid.size();
+  const int m = std::min(cap, *n);
+  for (int i = 0; i < m; ++i) {
+    if (resid) resid[i] = c->trace_resid[i];
+    if (err) err[i] = c->trace_err[i];
+  }
+  return MRF_OK;
+}
+
+int mrf_set_reference(mrf_ctx* c, const float* alphas, int64_t rows, int64_t cols, double stop_at) {
+  if (!c) return MRF_E_INVALID;
+  if (rows == 0) {
+    c->ref_n = 0;
+    c->ref_stop = 0.0;
+    return MRF_OK;
+  }
+  if (!alphas || cols != kBrickVox) return fail(c, MRF_E_SHAPE, "set_reference: reference must be bricks x 512");
+  if (c->geom_dirty) return fail(c, MRF_E_STATE, "set_reference: call mrf_prepare first");
+  const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
+  if ((size_t)rows * kBrickVox != nvox) return fail(c, MRF_E_SHAPE, "set_reference: rows != num_bricks");
+  int rc;
+  if (!c->d_rerr && (rc = dalloc(c, &c->d_rerr, (size_t)1))) return rc;
+  if (nvox > c->ref_n || !c->d_ref) {
+    dfree(&c->d_ref);
+    c->ref_n = 0;
+    if ((rc = dalloc(c, &c->d_ref, nvox))) return rc;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_ref, alphas, nvox * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ref_n = nvox;
+  c->ref_stop = stop_at;
   return MRF_OK;
 }
 

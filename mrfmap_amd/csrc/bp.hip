@@ -701,6 +701,27 @@ __global__ __launch_bounds__(256) void k_residual(const float* __restrict__ B, f
   if (threadIdx.x == 0) atomicMax(d_max, (unsigned long long)__double_as_longlong(sh[0]));
 }
 
+// max |alpha(v) - ref(v)| with alpha the readback marginal (A, or 0 where
+// never updated); A holds the marginals of the current beliefs.
+__global__ __launch_bounds__(256) void k_ref_error(const float* __restrict__ A, const uint8_t* __restrict__ upd,
+                                                   const float* __restrict__ ref, size_t n,
+                                                   unsigned long long* __restrict__ d_max) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float a = upd[i] ? A[i] : 0.0f;
+    const double dlt = fabs((double)a - (double)ref[i]);
+    m = dlt > m ? dlt : m;
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(d_max, (unsigned long long)__double_as_longlong(sh[0]));
+}
+
 __global__ void k_alphas(const float* __restrict__ B, const uint8_t* __restrict__ upd, float* __restrict__ out,
                          size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -849,6 +870,15 @@ void launch_residual(const float* B, float* A, size_t n, unsigned long long* d_m
   unsigned g = grid_for(n, 256);
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, B, A, n, d_max);
+}
+
+void launch_ref_error(const float* A, const uint8_t* upd, const float* ref, size_t n, unsigned long long* d_max,
+                      hipStream_t s) {
+  launch_zero_words(d_max, 8, s);
+  if (n == 0) return;
+  size_t g = grid_for(n, 256);
+  if (g > 2048) g = 2048;
+  hipLaunchKernelGGL(k_ref_error, dim3(g), dim3(256), 0, s, A, upd, ref, n, d_max);
 }
 
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t n, hipStream_t s) {

@@ -157,6 +157,8 @@ void launch_ray_messages(const CamPass& a, hipStream_t s);
 void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
 void launch_residual(const float* B, float* Aprev, size_t nvox, unsigned long long* d_max, hipStream_t s);
+void launch_ref_error(const float* A, const uint8_t* upd, const float* ref, size_t n, unsigned long long* d_max,
+                      hipStream_t s);
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t nvox, hipStream_t s);
 // load_map: B[vox[i]] = logodds[i], upd = 1 (distinct voxels)
 void launch_set_voxels(const uint32_t* vox, const float* logodds, uint32_t n, float* B, uint8_t* upd, hipStream_t s);

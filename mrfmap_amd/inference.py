@@ -103,19 +103,43 @@ class GVDBInference:
         self.last_stats = st.as_dict()
         return self.last_stats
 
-    def perform_inference_warm(self, params, max_iters=0, tol=0.0, fallback=True):
-        """perform_inference with the learned warm start after the first
-        iteration (``params``: MLP_PARAMS floats, see mrfmap_amd.warmstart);
-        with ``fallback`` the warm start is undone if the residual grows."""
+    def perform_inference_warm(self, params, max_iters=0, tol=0.0, growth=2.0, warm_after=1):
+        """perform_inference with the learned warm start after ``warm_after``
+        plain iterations (``params``: MLP_PARAMS floats trained on features
+        taken after as many iterations, see mrfmap_amd.warmstart); the warm
+        start is undone when a residual exceeds ``growth`` x the previous one
+        and 0.05 (``growth`` = 0: never)."""
         p = np.ascontiguousarray(params, dtype=np.float32)
         if p.shape != (_lib.MLP_PARAMS,):
             raise ValueError(f"params must hold {_lib.MLP_PARAMS} floats")
         st = _lib.MrfStats()
-        rc = self._L.mrf_perform_inference_warm(self._h, _ptr(p), int(max_iters), float(tol), int(bool(fallback)),
-                                                ctypes.byref(st))
+        rc = self._L.mrf_perform_inference_warm(self._h, _ptr(p), int(warm_after), int(max_iters), float(tol),
+                                                float(growth), ctypes.byref(st))
         _lib.check(rc, self._h)
         self.last_stats = st.as_dict()
         return self.last_stats
+
+    def set_reference(self, alphas, stop_at=0.0):
+        """Reference marginals (num_bricks x 512) for convergence measurements:
+        every later perform_inference reports max |alpha - ref| after its last
+        iteration in ``ref_error`` and, with ``stop_at`` > 0, stops once it is
+        <= stop_at.  ``None`` removes it."""
+        if alphas is None:
+            _lib.check(self._L.mrf_set_reference(self._h, None, 0, 0, 0.0), self._h)
+            return
+        a = np.ascontiguousarray(alphas, dtype=np.float32)
+        if a.ndim != 2:
+            raise ValueError("reference must be num_bricks x 512")
+        _lib.check(self._L.mrf_set_reference(self._h, _ptr(a), a.shape[0], a.shape[1], float(stop_at)), self._h)
+
+    def get_trace(self):
+        """(residual, reference error) per iteration of the last call."""
+        n = ctypes.c_int()
+        _lib.check(self._L.mrf_get_trace(self._h, None, None, 0, ctypes.byref(n)), self._h)
+        r = np.zeros(max(n.value, 1))
+        e = np.zeros(max(n.value, 1))
+        _lib.check(self._L.mrf_get_trace(self._h, _ptr(r), _ptr(e), n.value, ctypes.byref(n)), self._h)
+        return r[:n.value], e[:n.value]
 
     def features(self):
         """(X n x 32 float32, pool index n uint32) of the warm-start features of
