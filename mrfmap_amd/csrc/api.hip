@@ -43,6 +43,7 @@ struct Camera {
   uint32_t nslices = 0;
   uint32_t nrays = 0;              // rays with at least one record
   uint2* d_inc = nullptr;          // sliced records {pair | gap, length}: front region, then window region
+  uint32_t* d_walkn = nullptr;     // full-ray activation: records per tiled pixel, then has-records flags
   uint64_t nrec = 0, base_w = 0;   // padded records; first record of the window region
   uint64_t nupd = 0;               // live incidences
   uint64_t nwin = 0;               // live window incidences (window positions)
@@ -99,8 +100,10 @@ struct mrf_ctx {
   uint32_t *d_heads = nullptr, *d_hscan = nullptr;
   uint32_t *d_x0 = nullptr, *d_x1 = nullptr, *d_x2 = nullptr, *d_x3 = nullptr;
   size_t cap_sort = 0;
-  uint32_t* d_pscr = nullptr;
-  size_t cap_pscr = 0;
+  uint32_t* d_vflag = nullptr;  // 4 pool-sized tables: touched, window-touched and their scans
+  size_t cap_vflag = 0;
+  unsigned long long* d_acc = nullptr;  // 2 per pair: fixed-point chord-length sums
+  size_t cap_acc = 0;
   uint2* d_sell0 = nullptr;
   size_t cap_sell0 = 0;
   // map evaluation scratch: 4 images + depth, FP64 block sums, class counts
@@ -118,6 +121,7 @@ struct mrf_ctx {
   uint64_t* d_scan_tmp = nullptr;
   size_t cap_scan = 0;
   uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
+  uint32_t* d_all_pairs = nullptr;  // 1: the next epilogues refresh every pair (see CamPass::all_pairs)
 
   std::vector<Camera> cams;
   bool geom_dirty = true;
@@ -321,7 +325,7 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
       return rc;
     c->cap_sort = n;
   }
-  const size_t tb = segment_sort_temp_bytes((uint32_t)n, bits);
+  const size_t tb = window_sort_temp_bytes((uint32_t)n, bits);
   if (tb > c->sort_tmp_bytes) {
     if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
     c->d_sort_tmp = nullptr;
@@ -335,13 +339,23 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   return ensure_scan(c, n);
 }
 
-int ensure_pair_scratch(mrf_ctx* c, size_t words) {
-  if (words <= c->cap_pscr) return MRF_OK;
-  dfree(&c->d_pscr);
-  c->cap_pscr = 0;
-  int rc = dalloc(c, &c->d_pscr, words);
+int ensure_vflags(mrf_ctx* c, size_t nvox) {
+  if (nvox <= c->cap_vflag && c->d_vflag) return MRF_OK;
+  dfree(&c->d_vflag);
+  c->cap_vflag = 0;
+  int rc = dalloc(c, &c->d_vflag, 4 * nvox);
   if (rc) return rc;
-  c->cap_pscr = words;
+  c->cap_vflag = nvox;
+  return ensure_scan(c, nvox);
+}
+
+int ensure_acc(mrf_ctx* c, size_t np) {
+  if (np <= c->cap_acc && c->d_acc) return MRF_OK;
+  dfree(&c->d_acc);
+  c->cap_acc = 0;
+  int rc = dalloc(c, &c->d_acc, 2 * np);
+  if (rc) return rc;
+  c->cap_acc = np;
   return MRF_OK;
 }
 
@@ -420,7 +434,13 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if ((rc = ensure_pixel_scratch(c, npix)) || (rc = ensure_scan(c, npix))) return rc;
   // 1. every ray's records, in order, into one sliced region: per tile the
   //    longest of its 64 rays (d_has), slice sizes (d_rayidx) -> offsets (d_offs)
-  launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, s);
+  if (cam.d_walkn) {
+    // counted by the activation walk (full-ray activation)
+    HIPCHK(c, hipMemcpyAsync(c->d_counts, cam.d_walkn, npix * 4, hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->d_has, cam.d_walkn + npix, npix * 4, hipMemcpyDeviceToDevice, s));
+  } else {
+    launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, s);
+  }
   launch_exclusive_scan(c->d_counts, c->d_offs, npix, c->d_scan_tmp, c->d_total, s);
   const uint64_t nlist = read_total(c);
   launch_exclusive_scan(c->d_has, c->d_rayidx, npix, c->d_scan_tmp, c->d_total, s);
@@ -435,6 +455,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     return rc;
   launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_has, c->d_sell0, cam.d_front,
               cam.d_lanes, s);
+  dfree(&cam.d_walkn);
   // 2. front/window split (counts into d_counts / d_rayidx), then the front
   //    region (sizes d_has -> offsets d_offs) and the window region (sizes
   //    d_counts -> offsets d_has) of the final record buffer
@@ -465,99 +486,61 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.nchunks = (uint32_t)read_total(c);
   if ((rc = dalloc(c, &cam.d_chunk_tile, cam.nchunks))) return rc;
   launch_chunk_tiles(cam.d_chunk_off, ntiles, cam.d_chunk_tile, s);
-  // 3. live incidences (records that are not gap records)
-  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 8, s));
+  // 3. live incidences (records that are not gap records): all, window region
+  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 16, s));
   if (nrec) {
     unsigned g = (unsigned)std::min<uint64_t>((nrec + 255) / 256, 2048);
     hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, s, cam.d_inc, nrec, (unsigned long long*)(c->d_total + 2));
   }
-  uint64_t live64 = 0;
-  HIPCHK(c, hipMemcpyAsync(&live64, c->d_total + 2, 8, hipMemcpyDeviceToHost, s));
+  if (nrec_w) {
+    unsigned g = (unsigned)std::min<uint64_t>((nrec_w + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, s, cam.d_inc + nrec_f, nrec_w,
+                       (unsigned long long*)(c->d_total + 3));
+  }
+  uint64_t counts2[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 16, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
-  cam.nupd = live64;
-  const uint32_t live = (uint32_t)live64;
-  // 4. (camera, voxel) pairs in voxel order
-  const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
-  const int bits = bit_length(nvox);
-  if ((rc = ensure_sort(c, std::max<uint64_t>(nrec, 1), bits))) return rc;
-  SegmentBuild b;
-  b.rec = cam.d_inc;
-  b.nrec = (uint32_t)nrec;
-  b.gap_key = (uint32_t)nvox;
-  b.key_bits = bits;
-  b.keys_in = c->d_keys_in;
-  b.keys_out = c->d_keys_out;
-  b.vals_in = c->d_vals_in;
-  b.vals_out = c->d_vals_out;
-  b.heads = c->d_heads;
-  b.hscan = c->d_hscan;
-  b.temp = c->d_sort_tmp;
-  b.temp_bytes = c->sort_tmp_bytes;
-  b.scan_tmp = c->d_scan_tmp;
-  b.d_total = c->d_total;
-  HIPCHK(c, build_segments(b, s));
-  const uint32_t np = nrec ? (uint32_t)read_total(c) : 0u;
-  cam.npairs = np;
-  if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
-      (rc = dalloc(c, &cam.d_pair_sf, np)) || (rc = dalloc(c, &cam.d_M, np)))
-    return rc;
-  // per-pair scratch in the sorted numbering
-  const size_t P = (size_t)np + 1;
-  if ((rc = ensure_pair_scratch(c, 7 * P))) return rc;
-  uint32_t* off_old = c->d_pscr;
-  uint32_t* vox_old = off_old + P;
-  float* c_old = (float*)(vox_old + P);
-  float* sf_old = c_old + P;
-  uint32_t* mark = (uint32_t*)(sf_old + P);
-  uint32_t* rank = mark + P;
-  uint32_t* perm = rank + P;
-  // 5. per pair: C and the constant front messages (fixed-order segment sums);
-  //    window positions of the window incidences in sorted order
-  float* len = (float*)c->d_keys_in;
-  float* fmsg = (float*)c->d_x0;
-  uint32_t* win = c->d_x1;
-  uint32_t* wpos = c->d_x2;
-  uint64_t nwin = 0;
-  if (live) {
-    launch_pairs(b, np, live, off_old, vox_old, s);
-    launch_sorted_vals(b, live, (uint32_t)nrec_f, len, fmsg, win, s);
-    const size_t nwords = live / 64 + 1, ntl = live / kTile + 1;
-    uint64_t* hbits = (uint64_t*)c->d_x3;
-    uint32_t* th0 = c->d_x3 + 2 * nwords;
-    launch_pack_flags(b.heads, b.hscan, live, hbits, th0, s);
-    uint32_t* tscr = c->d_vals_in;
-    launch_segment_sum_f32(hbits, th0, live, len, nullptr, c_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
-                           (float*)(tscr + 3 * ntl), (float*)(tscr + 4 * ntl), off_old, s);
-    launch_segment_sum_f32(hbits, th0, live, fmsg, nullptr, sf_old, tscr, (float*)(tscr + ntl), tscr + 2 * ntl,
-                           (float*)(tscr + 3 * ntl), (float*)(tscr + 4 * ntl), off_old, s);
-    launch_exclusive_scan(win, wpos, live, c->d_scan_tmp, c->d_total, s);
-    nwin = read_total(c);
-  }
+  cam.nupd = counts2[0];
+  const uint64_t nwin = counts2[1];
   cam.nwin = nwin;
-  // 6. window segments: their pairs (wq), heads (whead) and head counts (hw)
-  uint32_t* wq = c->d_keys_out;
-  uint32_t* whead = c->d_x0;
-  uint32_t* hw = c->d_keys_in;
-  uint32_t nwp = 0;
-  if (nwin) {
-    launch_window_pairs(b, win, wpos, live, wq, s);
-    launch_window_heads(wq, (uint32_t)nwin, whead, s);
-    launch_exclusive_scan(whead, hw, nwin, c->d_scan_tmp, c->d_total, s);
-    nwp = (uint32_t)read_total(c);
-  }
+  // 4. (camera, voxel) pairs: window pairs first, each group in voxel order
+  const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
+  if ((rc = ensure_vflags(c, std::max<uint64_t>(nvox, 1)))) return rc;
+  uint32_t* any = c->d_vflag;
+  uint32_t* wany = any + nvox;
+  uint32_t* pid = wany + nvox;  // rank of any, then the pair id
+  uint32_t* rank_w = pid + nvox;
+  launch_zero_words(any, (size_t)nvox * 8, s);  // any and wany
+  launch_vox_flags(cam.d_inc, nrec, nrec_f, any, wany, s);
+  launch_exclusive_scan(any, pid, nvox, c->d_scan_tmp, c->d_total, s);
+  const uint32_t np = (uint32_t)read_total(c);
+  launch_exclusive_scan(wany, rank_w, nvox, c->d_scan_tmp, c->d_total, s);
+  const uint32_t nwp = (uint32_t)read_total(c);
+  cam.npairs = np;
   cam.nwp = nwp;
-  // 7. renumber the pairs (window pairs first) and point the records at them
+  if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
+      (rc = dalloc(c, &cam.d_pair_sf, np)) || (rc = dalloc(c, &cam.d_M, np)) ||
+      (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
+    return rc;
+  launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
+  // 5. per pair: C and the constant front messages
+  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + np, np, cam.d_pair_c, cam.d_pair_sf, s);
+  // 6. window segments: the window records stably sorted by pair
+  const int wbits = std::max(1, bit_length(nwp));
+  if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits))) return rc;
   if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_idx, nwin))) return rc;
-  if (np) {
-    launch_zero_words(mark, (size_t)np * 4, s);
-    launch_mark_window_pairs(whead, hw, wq, (uint32_t)nwin, nwp, mark, cam.d_wseg_off, s);
-    launch_exclusive_scan(mark, rank, np, c->d_scan_tmp, c->d_total, s);
-    launch_permute_pairs(mark, rank, np, nwp, vox_old, c_old, sf_old, perm, cam.d_pair_vox, cam.d_pair_c,
-                         cam.d_pair_sf, s);
-  } else {
-    launch_zero_words(cam.d_wseg_off, 4, s);
+  HIPCHK(c, sort_window(cam.d_inc + nrec_f, (uint32_t)nrec_w, pid, nwp, wbits, c->d_keys_in, c->d_keys_out,
+                        c->d_vals_in, c->d_vals_out, c->d_sort_tmp, c->sort_tmp_bytes, s));
+  uint32_t* whead = c->d_x0;
+  uint32_t* hw = c->d_x1;
+  if (nwin) {
+    HIPCHK(c, hipMemcpyAsync(cam.d_win_idx, c->d_vals_out, nwin * 4, hipMemcpyDeviceToDevice, s));
+    launch_window_heads(c->d_keys_out, (uint32_t)nwin, whead, s);
+    launch_exclusive_scan(whead, hw, nwin, c->d_scan_tmp, c->d_total, s);
   }
-  launch_set_records(b, win, wpos, perm, live, (uint32_t)nrec_f, cam.d_win_idx, s);
+  launch_window_offsets(c->d_keys_out, whead, (uint32_t)nwin, nwp, cam.d_wseg_off, s);
+  // 7. records point at their pairs
+  launch_set_pairs(cam.d_inc, nrec, pid, s);
   // 8. window segment head bitmask and per-tile head counts
   const size_t nwords_w = nwin / 64 + 1, ntiles_w = nwin / kTile + 1;
   if ((rc = dalloc(c, &cam.d_heads, nwords_w)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles_w))) return rc;
@@ -579,9 +562,13 @@ int prepare(mrf_ctx* c) {
   HIPCHK(c, hipMemsetAsync(c->d_bflags, 0, (size_t)c->nbricks, c->stream));
   for (Camera& cam : c->cams) {
     const CamGeom cg = make_geom(c, cam);
-    if (c->p.activation == MRF_ACT_FULL_RAY)
-      launch_mark_rays(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_bflags, c->stream);
-    else
+    if (c->p.activation == MRF_ACT_FULL_RAY) {
+      const size_t npix = (size_t)num_tiled_pixels(cg);
+      dfree(&cam.d_walkn);
+      if ((rc = dalloc(c, &cam.d_walkn, 2 * npix))) return rc;
+      launch_mark_rays(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_bflags, cam.d_walkn, cam.d_walkn + npix,
+                       c->stream);
+    } else
       launch_mark_endpoints(cg, c->g, cam.d_depth, c->d_bflags, c->stream);
   }
   std::vector<uint8_t> flags((size_t)c->nbricks);
@@ -709,6 +696,7 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.M = cam.d_M;
   a.Xo = c->d_Xo;
   a.upd = c->d_upd;
+  a.all_pairs = c->d_all_pairs;
   a.sig = c->d_sig;
   a.bia = c->d_bia;
   a.lookup_n = c->p.lookup_n;
@@ -1067,7 +1055,7 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
   }
   int rc;
   if ((rc = dalloc(c, &c->d_slot, (size_t)c->nbricks)) || (rc = dalloc(c, &c->d_bflags, (size_t)c->nbricks)) ||
-      (rc = dalloc(c, &c->d_total, 4))) {
+      (rc = dalloc(c, &c->d_total, 4)) || (rc = dalloc(c, &c->d_all_pairs, 1))) {
     mrf_destroy(c);
     return rc;
   }
@@ -1098,6 +1086,7 @@ void mrf_destroy(mrf_ctx* c) {
   for (Camera& cam : c->cams) {
     free_camera_lists(cam);
     dfree(&cam.d_depth);
+    dfree(&cam.d_walkn);
   }
   dfree(&c->d_sig); dfree(&c->d_bia); dfree(&c->d_slot); dfree(&c->d_bflags);
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
@@ -1108,7 +1097,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
   dfree(&c->d_heads); dfree(&c->d_hscan);
   dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
-  dfree(&c->d_pscr); dfree(&c->d_sell0);
+  dfree(&c->d_vflag); dfree(&c->d_acc); dfree(&c->d_sell0);
   dfree(&c->d_eval); dfree(&c->d_eval_part); dfree(&c->d_eval_counts);
   dfree(&c->d_wpass); dfree(&c->d_wmsum); dfree(&c->d_wmmax); dfree(&c->d_wdelta);
   dfree(&c->d_wnc); dfree(&c->d_whits); dfree(&c->d_wflag); dfree(&c->d_wrow); dfree(&c->d_slot_lin);
@@ -1118,7 +1107,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_tSel); dfree(&c->d_tLpart); dfree(&c->d_tLoss);
   drop_graph(c);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
-  dfree(&c->d_scan_tmp); dfree(&c->d_total);
+  dfree(&c->d_scan_tmp); dfree(&c->d_total); dfree(&c->d_all_pairs);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1203,8 +1192,10 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   if (dirty) s.t_geometry_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   if (c->ray_sched < 0 && (rc = autotune(c))) return rc;
   const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
-  // reset M_c of the selected cameras (GVDBInference.cpp:128-134)
+  // reset M_c of the selected cameras (GVDBInference.cpp:128-134); the
+  // first iteration refreshes every pair's message
   for (uint32_t id : sel) launch_zero_words(c->cams[id].d_M, (size_t)c->cams[id].npairs * sizeof(float), c->stream);
+  launch_fill_u32(c->d_all_pairs, 1u, 1, c->stream);
   launch_marginals(c->d_B, c->d_A, nvox, c->stream);
   for (const Camera& cam : c->cams) {
     s.updates_per_iter += (int64_t)cam.nupd;
@@ -1249,6 +1240,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
     else
       HIPCHK(c, enqueue_iteration(c, ev.data() + 2));
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    launch_fill_u32(c->d_all_pairs, 0u, 1, c->stream);  // front-only messages are in M now
     uint64_t bits = 0, ebits = 0;
     if (c->ref_n) {
       launch_ref_error(c->d_A, c->d_upd, c->d_ref, nvox, c->d_rerr, c->stream);
@@ -1284,6 +1276,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
       for (const Camera& cam : c->cams)
         launch_warm_pairs(cam.d_pair_vox, cam.d_pair_c, cam.npairs, c->d_wdelta, cam.d_M, c->stream);
       launch_marginals(c->d_B, c->d_A, nvox, c->stream);
+      launch_fill_u32(c->d_all_pairs, 1u, 1, c->stream);  // every message was moved
       HIPCHK(c, hipStreamSynchronize(c->stream));
       HIPCHK(c, hipGetLastError());
       s.t_warm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
@@ -1293,6 +1286,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
       check = false;
     } else if (warm_state == 1 && it > warm_after && growth > 0 && resid > growth * r_prev && resid > 0.05) {
       // the residual grows: back to the plain state before the warm start
+      // (the restored front-only messages are their constants already)
       if ((rc = snapshot(c, false))) return rc;
       launch_marginals(c->d_B, c->d_A, nvox, c->stream);
       HIPCHK(c, hipStreamSynchronize(c->stream));

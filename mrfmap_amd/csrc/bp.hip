@@ -603,9 +603,13 @@ __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__
                                                     const float* __restrict__ pair_c,
                                                     const float* __restrict__ pair_sf, uint32_t npairs, uint32_t nwp,
                                                     const float* __restrict__ S, float out_scale, float* __restrict__ B,
-                                                    float* __restrict__ M, uint8_t* __restrict__ upd) {
+                                                    float* __restrict__ M, uint8_t* __restrict__ upd,
+                                                    const uint32_t* __restrict__ all_pairs) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npairs) return;
+  // a front-only pair's m is the same every iteration: once it is in M the
+  // update B = (B + m) - M would add nothing but FP32 rounding
+  if (q >= nwp && *all_pairs == 0u) return;
   const float C = pair_c[q];
   if (!(C > kCumLenMin)) return;
   const uint32_t v = pair_vox[q];
@@ -732,6 +736,11 @@ __global__ void k_fill_f32(float* p, float v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ void k_fill_u8(uint8_t* p, uint8_t v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -833,7 +842,7 @@ void launch_epilogue(const CamPass& a, hipStream_t s) {
     }
   }
   hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf,
-                     a.npairs, a.nwp, a.S, out_scale, a.B, a.M, a.upd);
+                     a.npairs, a.nwp, a.S, out_scale, a.B, a.M, a.upd, a.all_pairs);
 }
 
 void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
@@ -919,6 +928,11 @@ void launch_gather_pairs(const uint32_t* pair_vox, const float* dense, uint32_t 
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
 void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s) {

@@ -1,20 +1,24 @@
-// Per-camera pair and segment layout of the incidence lists (built once per
-// traversal, not per BP iteration).
+// Per-camera pair and window-segment layout of the incidence lists (built
+// once per traversal, not per BP iteration).
 //
-// The camera's incidences are stably sorted by voxel (hipCUB radix sort of
-// (voxel, incidence index)); each run of equal voxels is one (camera, voxel)
-// pair.  Per pair, two deterministic segment sums over the sorted order give
-// the summed chord length C (what render_cumulative_length_kernel
-// accumulates with float atomics every camera pass,
-// src/cuda/gvdbkernels.cu:519-571, 1276-1302) and the sum of the constant
-// messages F_EPS * l of its front incidences (those where the depth
-// likelihood is exactly 0).  The window incidences, in sorted order, form
-// the window segments of the BP iteration: the ray kernel writes each window
-// message into the incidence's own slot of the window region and each pair
-// sums its segment, gathered through win_idx, in a fixed order, so the
-// iteration needs no atomics and is bit-reproducible.  Pairs are renumbered with the pairs that have window
-// incidences first (both groups in voxel order), so that window segment m
-// belongs to pair m, and every record is rewritten to hold its pair.
+// Pairs.  Every voxel the camera's live records touch is one (camera, voxel)
+// pair.  Two voxel flags over the brick pool -- touched by any record, by a
+// window record -- and their exclusive scans number the pairs: the pairs
+// with window records first, then the others, each group in voxel order, so
+// that window segment m belongs to pair m.
+//
+// Per pair, the summed chord length C (what render_cumulative_length_kernel
+// accumulates with float atomics in every camera pass,
+// src/cuda/gvdbkernels.cu:519-571, 1276-1302) and the chord length of its
+// front records (whose messages are the constant F_EPS * l) are summed in
+// 2^-32 fixed point with 64-bit integer atomics: the sums are exact, so they
+// do not depend on the order in which records arrive.
+//
+// Window segments.  The window records are stably radix-sorted by pair, so
+// each pair's records stay in record order; the BP iteration sums each
+// segment in that order, gathered through win_idx (no atomics,
+// bit-reproducible).  Finally every live record is rewritten to hold its
+// pair instead of its voxel.
 #include <hipcub/hipcub.hpp>
 
 #include "mrf_kernels.h"
@@ -23,46 +27,126 @@ namespace mrf {
 
 namespace {
 
-__global__ void k_keys(const uint2* __restrict__ rec, uint32_t n, uint32_t gap_key, uint32_t* __restrict__ keys,
-                       uint32_t* __restrict__ vals) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t v = rec[i].x;
-  keys[i] = v == kGapVox ? gap_key : v;
-  vals[i] = i;
+constexpr double kFix32 = 4294967296.0;  // 2^32
+
+// any[v] = 1 for every voxel of a live record, wany[v] = 1 for a window record
+__global__ void k_vox_flags(const uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w, uint32_t* __restrict__ any,
+                            uint32_t* __restrict__ wany) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = rec[i].x;
+    if (v == kGapVox) continue;
+    any[v] = 1u;
+    if (i >= base_w) wany[v] = 1u;
+  }
 }
 
-__global__ void k_heads(const uint32_t* __restrict__ keys, uint32_t n, uint32_t gap_key,
-                        uint32_t* __restrict__ heads) {
+// pid[v] (written over rank_all) and pair_vox[pid] of every touched voxel
+__global__ void k_pair_table(const uint32_t* __restrict__ any, const uint32_t* __restrict__ wany,
+                             uint32_t* __restrict__ rank_all, const uint32_t* __restrict__ rank_w, uint64_t nvox,
+                             uint32_t nwp, uint32_t* __restrict__ pair_vox) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvox || !any[v]) return;
+  const uint32_t rw = rank_w[v];
+  const uint32_t p = wany[v] ? rw : nwp + (rank_all[v] - rw);
+  rank_all[v] = p;
+  pair_vox[p] = (uint32_t)v;
+}
+
+// A wave takes 64 consecutive records -- one step of the 64 rays of a pixel
+// tile, whose voxels repeat across neighbouring rays -- sorts them by pair
+// across its lanes (bitonic network), sums each run of equal pairs
+// (segmented scan) and issues one atomic per run instead of one per record.
+__global__ void __launch_bounds__(256) k_pair_lengths(const uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
+                                                      const uint32_t* __restrict__ pid,
+                                                      unsigned long long* __restrict__ acc_c,
+                                                      unsigned long long* __restrict__ acc_f) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = wave * 64; base < nrec; base += nwaves * 64) {
+    const uint64_t i = base + lane;
+    uint32_t key = 0xFFFFFFFFu;
+    unsigned long long qc = 0, qf = 0;
+    if (i < nrec) {
+      const uint2 e = rec[i];
+      if (e.x != kGapVox) {
+        key = pid[e.x];
+        qc = (unsigned long long)((double)__uint_as_float(e.y) * kFix32);
+        qf = i < base_w ? qc : 0ull;
+      }
+    }
+    // bitonic sort of (key, qc, qf) across the 64 lanes, ascending key
+    for (int k = 2; k <= 64; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const uint32_t ok = __shfl_xor(key, j, 64);
+        const unsigned long long oc = __shfl_xor(qc, j, 64), of = __shfl_xor(qf, j, 64);
+        const bool up = (lane & k) == 0;        // this block of k lanes sorts ascending
+        const bool lower = (lane & j) == 0;     // this lane keeps the smaller of the pair when ascending
+        const bool take = (lower == up) ? (ok < key) : (ok > key);
+        if (take) {
+          key = ok;
+          qc = oc;
+          qf = of;
+        }
+      }
+    // segmented inclusive scan over runs of equal keys
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    bool head = lane == 0 || prev != key;
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned long long oc = __shfl_up(qc, d, 64), of = __shfl_up(qf, d, 64);
+      const bool oh = __shfl_up((int)head, d, 64) != 0;
+      if (lane >= d && !head) {
+        qc += oc;
+        qf += of;
+        head = oh;
+      }
+    }
+    const uint32_t next = __shfl_down(key, 1, 64);
+    const bool tail = lane == 63 || next != key;
+    if (tail && key != 0xFFFFFFFFu) {
+      atomicAdd(&acc_c[key], qc);
+      if (qf) atomicAdd(&acc_f[key], qf);
+    }
+  }
+}
+
+__global__ void k_pair_finish(const unsigned long long* __restrict__ acc_c, const unsigned long long* __restrict__ acc_f,
+                              uint32_t np, float* __restrict__ pair_c, float* __restrict__ pair_sf) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  pair_c[p] = (float)((double)acc_c[p] / kFix32);
+  pair_sf[p] = (float)((double)kFEps * ((double)acc_f[p] / kFix32));
+}
+
+// sort keys of the window region: the record's pair, nwp (sorts last) for
+// gap and padding records; values: the record's index in the region
+__global__ void k_window_keys(const uint2* __restrict__ recw, uint32_t n, const uint32_t* __restrict__ pid,
+                              uint32_t sentinel, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const uint32_t k = keys[j];
-  heads[j] = (k != gap_key && (j == 0 || keys[j - 1] != k)) ? 1u : 0u;
+  const uint32_t v = recw[j].x;
+  keys[j] = v == kGapVox ? sentinel : pid[v];
+  vals[j] = j;
 }
 
-__global__ void k_pairs(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ heads,
-                        const uint32_t* __restrict__ hscan, uint32_t n, uint32_t* __restrict__ pair_off,
-                        uint32_t* __restrict__ pair_vox) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || !heads[j]) return;
-  const uint32_t q = hscan[j];
-  pair_off[q] = j;
-  pair_vox[q] = keys[j];
+__global__ void k_window_heads(const uint32_t* __restrict__ keys, uint32_t nwin, uint32_t* __restrict__ whead) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nwin) return;
+  whead[m] = (m == 0 || keys[m - 1] != keys[m]) ? 1u : 0u;
 }
 
-__global__ void k_set_last(uint32_t* p, uint32_t v) { *p = v; }
+__global__ void k_window_offsets(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ whead, uint32_t nwin,
+                                 uint32_t nwp, uint32_t* __restrict__ wseg_off) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nwin && whead[m]) wseg_off[keys[m]] = m;
+  if (m == 0) wseg_off[nwp] = nwin;
+}
 
-__global__ void k_sorted_vals(const uint32_t* __restrict__ vals, uint32_t nlive, const uint2* __restrict__ rec,
-                              uint32_t base_w, float* __restrict__ len, float* __restrict__ fmsg,
-                              uint32_t* __restrict__ win) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nlive) return;
-  const uint32_t i = vals[j];
-  const float l = __uint_as_float(rec[i].y);
-  const bool w = i >= base_w;
-  len[j] = l;
-  fmsg[j] = w ? 0.0f : kFEps * l;
-  win[j] = w ? 1u : 0u;
+__global__ void k_set_pairs(uint2* __restrict__ rec, uint64_t nrec, const uint32_t* __restrict__ pid) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = rec[i].x;
+    if (v != kGapVox) rec[i].x = pid[v];
+  }
 }
 
 __global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t n, uint64_t* __restrict__ bits) {
@@ -77,54 +161,6 @@ __global__ void k_tile_h0(const uint32_t* __restrict__ fscan, uint32_t n, uint32
   if (w * kTile < n) tile_h0[w] = fscan[w * kTile];
 }
 
-__global__ void k_window_pairs(const uint32_t* __restrict__ win, const uint32_t* __restrict__ wpos,
-                               const uint32_t* __restrict__ heads, const uint32_t* __restrict__ hscan, uint32_t nlive,
-                               uint32_t* __restrict__ wq) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nlive || !win[j]) return;
-  wq[wpos[j]] = hscan[j] + heads[j] - 1u;
-}
-
-__global__ void k_window_heads(const uint32_t* __restrict__ wq, uint32_t nwin, uint32_t* __restrict__ whead) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nwin) return;
-  whead[m] = (m == 0 || wq[m - 1] != wq[m]) ? 1u : 0u;
-}
-
-__global__ void k_mark_window_pairs(const uint32_t* __restrict__ whead, const uint32_t* __restrict__ hw,
-                                    const uint32_t* __restrict__ wq, uint32_t nwin, uint32_t* __restrict__ mark,
-                                    uint32_t* __restrict__ wseg_off) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nwin || !whead[m]) return;
-  mark[wq[m]] = 1u;
-  wseg_off[hw[m]] = m;
-}
-
-__global__ void k_permute_pairs(const uint32_t* __restrict__ mark, const uint32_t* __restrict__ rank, uint32_t npairs,
-                                uint32_t nwp, const uint32_t* __restrict__ vox_old, const float* __restrict__ c_old,
-                                const float* __restrict__ sf_old, uint32_t* __restrict__ perm,
-                                uint32_t* __restrict__ pair_vox, float* __restrict__ pair_c,
-                                float* __restrict__ pair_sf) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npairs) return;
-  const uint32_t p = mark[q] ? rank[q] : nwp + (q - rank[q]);
-  perm[q] = p;
-  pair_vox[p] = vox_old[q];
-  pair_c[p] = c_old[q];
-  pair_sf[p] = sf_old[q];
-}
-
-__global__ void k_set_records(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ heads,
-                              const uint32_t* __restrict__ hscan, const uint32_t* __restrict__ win,
-                              const uint32_t* __restrict__ wpos, const uint32_t* __restrict__ perm, uint32_t nlive,
-                              uint32_t base_w, uint2* __restrict__ rec, uint32_t* __restrict__ win_idx) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nlive) return;
-  const uint32_t i = vals[j];
-  rec[i].x = perm[hscan[j] + heads[j] - 1u];
-  if (win[j]) win_idx[wpos[j]] = i - base_w;
-}
-
 // record -> voxel (for export)
 __global__ void k_export_vox(const uint2* __restrict__ rec, uint32_t n, const uint32_t* __restrict__ pair_vox,
                              uint2* __restrict__ out) {
@@ -136,42 +172,66 @@ __global__ void k_export_vox(const uint2* __restrict__ rec, uint32_t n, const ui
 }
 
 inline unsigned blocks(size_t n, int t) { return (unsigned)((n + t - 1) / t); }
+inline unsigned stride_blocks(uint64_t n) {
+  const uint64_t b = (n + 255) / 256;
+  return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
+}
 
 }  // namespace
 
-size_t segment_sort_temp_bytes(uint32_t n, int bits) {
+size_t window_sort_temp_bytes(uint32_t n, int bits) {
   size_t bytes = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, bits);
   return bytes;
 }
 
-hipError_t build_segments(SegmentBuild& b, hipStream_t s) {
-  const uint32_t n = b.nrec;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_keys, dim3(blocks(n, 256)), dim3(256), 0, s, b.rec, n, b.gap_key, b.keys_in, b.vals_in);
-  size_t tb = b.temp_bytes;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(b.temp, tb, b.keys_in, b.keys_out, b.vals_in, b.vals_out,
-                                                    (int)n, 0, b.key_bits, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_heads, dim3(blocks(n, 256)), dim3(256), 0, s, b.keys_out, n, b.gap_key, b.heads);
-  launch_exclusive_scan(b.heads, b.hscan, n, b.scan_tmp, b.d_total, s);
-  return hipGetLastError();
+void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t* any, uint32_t* wany,
+                      hipStream_t s) {
+  if (nrec == 0) return;
+  hipLaunchKernelGGL(k_vox_flags, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, any, wany);
 }
 
-void launch_pairs(const SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t* pair_off, uint32_t* pair_vox,
-                  hipStream_t s) {
-  if (b.nrec == 0) return;
-  hipLaunchKernelGGL(k_pairs, dim3(blocks(b.nrec, 256)), dim3(256), 0, s, b.keys_out, b.heads, b.hscan, b.nrec,
-                     pair_off, pair_vox);
-  hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, pair_off + npairs, nlive);
+void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
+                       uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s) {
+  if (nvox == 0) return;
+  hipLaunchKernelGGL(k_pair_table, dim3(blocks(nvox, 256)), dim3(256), 0, s, any, wany, rank_all, rank_w, nvox, nwp,
+                     pair_vox);
 }
 
-void launch_sorted_vals(const SegmentBuild& b, uint32_t nlive, uint32_t base_w, float* len, float* fmsg,
-                        uint32_t* win, hipStream_t s) {
-  if (nlive == 0) return;
-  hipLaunchKernelGGL(k_sorted_vals, dim3(blocks(nlive, 256)), dim3(256), 0, s, b.vals_out, nlive, b.rec, base_w, len,
-                     fmsg, win);
+void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid,
+                         unsigned long long* acc_c, unsigned long long* acc_f, uint32_t np, float* pair_c,
+                         float* pair_sf, hipStream_t s) {
+  if (np == 0) return;
+  launch_zero_words(acc_c, (size_t)np * 8, s);
+  launch_zero_words(acc_f, (size_t)np * 8, s);
+  hipLaunchKernelGGL(k_pair_lengths, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_c, acc_f);
+  hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_c, acc_f, np, pair_c, pair_sf);
+}
+
+hipError_t sort_window(const uint2* recw, uint32_t nwr, const uint32_t* pid, uint32_t nwp, int bits, uint32_t* keys_in,
+                       uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
+                       hipStream_t s) {
+  if (nwr == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_window_keys, dim3(blocks(nwr, 256)), dim3(256), 0, s, recw, nwr, pid, nwp, keys_in, vals_in);
+  size_t tb = temp_bytes;
+  return hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (int)nwr, 0, bits, s);
+}
+
+void launch_window_heads(const uint32_t* keys, uint32_t nwin, uint32_t* whead, hipStream_t s) {
+  if (nwin == 0) return;
+  hipLaunchKernelGGL(k_window_heads, dim3(blocks(nwin, 256)), dim3(256), 0, s, keys, nwin, whead);
+}
+
+void launch_window_offsets(const uint32_t* keys, const uint32_t* whead, uint32_t nwin, uint32_t nwp,
+                           uint32_t* wseg_off, hipStream_t s) {
+  hipLaunchKernelGGL(k_window_offsets, dim3(blocks(nwin ? nwin : 1, 256)), dim3(256), 0, s, keys, whead, nwin, nwp,
+                     wseg_off);
+}
+
+void launch_set_pairs(uint2* rec, uint64_t nrec, const uint32_t* pid, hipStream_t s) {
+  if (nrec == 0) return;
+  hipLaunchKernelGGL(k_set_pairs, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, pid);
 }
 
 void launch_pack_flags(const uint32_t* flags, const uint32_t* fscan, uint32_t n, uint64_t* bits, uint32_t* tile_h0,
@@ -180,40 +240,6 @@ void launch_pack_flags(const uint32_t* flags, const uint32_t* fscan, uint32_t n,
   hipLaunchKernelGGL(k_pack_flags, dim3(blocks(n, 256)), dim3(256), 0, s, flags, n, bits);
   const uint32_t ntiles = (n + kTile - 1) / kTile;
   hipLaunchKernelGGL(k_tile_h0, dim3(blocks(ntiles, 256)), dim3(256), 0, s, fscan, n, tile_h0);
-}
-
-void launch_window_pairs(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, uint32_t nlive,
-                         uint32_t* wq, hipStream_t s) {
-  if (nlive == 0) return;
-  hipLaunchKernelGGL(k_window_pairs, dim3(blocks(nlive, 256)), dim3(256), 0, s, win, wpos, b.heads, b.hscan, nlive,
-                     wq);
-}
-
-void launch_window_heads(const uint32_t* wq, uint32_t nwin, uint32_t* whead, hipStream_t s) {
-  if (nwin == 0) return;
-  hipLaunchKernelGGL(k_window_heads, dim3(blocks(nwin, 256)), dim3(256), 0, s, wq, nwin, whead);
-}
-
-void launch_mark_window_pairs(const uint32_t* whead, const uint32_t* hw, const uint32_t* wq, uint32_t nwin,
-                              uint32_t nwp, uint32_t* mark, uint32_t* wseg_off, hipStream_t s) {
-  if (nwin) hipLaunchKernelGGL(k_mark_window_pairs, dim3(blocks(nwin, 256)), dim3(256), 0, s, whead, hw, wq, nwin,
-                               mark, wseg_off);
-  hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, wseg_off + nwp, nwin);
-}
-
-void launch_permute_pairs(const uint32_t* mark, const uint32_t* rank, uint32_t npairs, uint32_t nwp,
-                          const uint32_t* vox_old, const float* c_old, const float* sf_old, uint32_t* perm,
-                          uint32_t* pair_vox, float* pair_c, float* pair_sf, hipStream_t s) {
-  if (npairs == 0) return;
-  hipLaunchKernelGGL(k_permute_pairs, dim3(blocks(npairs, 256)), dim3(256), 0, s, mark, rank, npairs, nwp, vox_old,
-                     c_old, sf_old, perm, pair_vox, pair_c, pair_sf);
-}
-
-void launch_set_records(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, const uint32_t* perm,
-                        uint32_t nlive, uint32_t base_w, uint32_t* win_idx, hipStream_t s) {
-  if (nlive == 0) return;
-  hipLaunchKernelGGL(k_set_records, dim3(blocks(nlive, 256)), dim3(256), 0, s, b.vals_out, b.heads, b.hscan, win, wpos,
-                     perm, nlive, base_w, b.rec, win_idx);
 }
 
 void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_vox, uint2* out, hipStream_t s) {

@@ -22,7 +22,7 @@ enum RaySched { kSchedF8W4 = 0, kSchedF8W8 = 1, kSchedF16W4 = 2, kSchedF16W8 = 3
 
 // traverse.hip
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                      const float* bia, uint8_t* bflags, hipStream_t s);
+                      const float* bia, uint8_t* bflags, uint32_t* counts, uint32_t* has, hipStream_t s);
 void launch_mark_endpoints(const CamGeom& c, const GridGeom& g, const float* depth, uint8_t* bflags,
                            hipStream_t s);
 void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
@@ -54,46 +54,36 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint64_t
 
 int num_tiled_pixels(const CamGeom& c);
 
-// geometry.hip: stable sort of a camera's incidences by voxel into
-// (camera, voxel) pairs, and the window segment layout of the BP iteration.
-struct SegmentBuild {
-  uint2* rec;          // incidence records {voxel | kGapVox, length}
-  uint32_t nrec;
-  uint32_t gap_key;    // key of gap records (> every voxel id)
-  int key_bits;
-  uint32_t *keys_in, *keys_out, *vals_in, *vals_out, *heads, *hscan;
-  void* temp;
-  size_t temp_bytes;
-  uint64_t* scan_tmp;
-  uint64_t* d_total;   // receives the number of pairs
-};
-size_t segment_sort_temp_bytes(uint32_t n, int bits);
-// sort + pair heads; the number of pairs lands in *d_total
-hipError_t build_segments(SegmentBuild& b, hipStream_t s);
-// pair offsets and voxels in sorted order
-void launch_pairs(const SegmentBuild& b, uint32_t npairs, uint32_t nlive, uint32_t* pair_off, uint32_t* pair_vox,
-                  hipStream_t s);
-// per sorted position: chord length, constant front message F_EPS * l (0 for
-// window incidences, the records at index >= base_w) and the window flag
-void launch_sorted_vals(const SegmentBuild& b, uint32_t nlive, uint32_t base_w, float* len, float* fmsg,
-                        uint32_t* win, hipStream_t s);
+// geometry.hip: (camera, voxel) pairs of a camera's incidences and the
+// window segment layout of the BP iteration.
+// any / wany: pool-sized flags of voxels touched by a live / window record
+void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t* any, uint32_t* wany,
+                      hipStream_t s);
+// from the exclusive scans rank_all (of any) and rank_w (of wany): pair id
+// of every touched voxel (window pairs first, each group in voxel order),
+// written over rank_all, and pair_vox[pair]
+void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
+                       uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s);
+// per pair: C (summed chord length) and F_EPS x the chord length of its
+// front records (index < base_w), exact fixed-point sums
+void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid,
+                         unsigned long long* acc_c, unsigned long long* acc_f, uint32_t np, float* pair_c,
+                         float* pair_sf, hipStream_t s);
+size_t window_sort_temp_bytes(uint32_t n, int bits);
+// window region records stably sorted by pair (gap / padding records last):
+// keys_out = pairs, vals_out = record index in the region
+hipError_t sort_window(const uint2* recw, uint32_t nwr, const uint32_t* pid, uint32_t nwp, int bits, uint32_t* keys_in,
+                       uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
+                       hipStream_t s);
+void launch_window_heads(const uint32_t* keys, uint32_t nwin, uint32_t* whead, hipStream_t s);
+// wseg_off[pair] = first window position of the pair, wseg_off[nwp] = nwin
+void launch_window_offsets(const uint32_t* keys, const uint32_t* whead, uint32_t nwin, uint32_t nwp,
+                           uint32_t* wseg_off, hipStream_t s);
+// live records: voxel -> pair
+void launch_set_pairs(uint2* rec, uint64_t nrec, const uint32_t* pid, hipStream_t s);
 // flag bitmask (bit j = flags[j] != 0) and, per kTile tile, fscan at its start
 void launch_pack_flags(const uint32_t* flags, const uint32_t* fscan, uint32_t n, uint64_t* bits, uint32_t* tile_h0,
                        hipStream_t s);
-// window compaction: wq[wpos[j]] = pair (sorted numbering) of window position j
-void launch_window_pairs(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, uint32_t nlive,
-                         uint32_t* wq, hipStream_t s);
-void launch_window_heads(const uint32_t* wq, uint32_t nwin, uint32_t* whead, hipStream_t s);
-void launch_mark_window_pairs(const uint32_t* whead, const uint32_t* hw, const uint32_t* wq, uint32_t nwin,
-                              uint32_t nwp, uint32_t* mark, uint32_t* wseg_off, hipStream_t s);
-// renumber pairs: those with window incidences first, each group in voxel order
-void launch_permute_pairs(const uint32_t* mark, const uint32_t* rank, uint32_t npairs, uint32_t nwp,
-                          const uint32_t* vox_old, const float* c_old, const float* sf_old, uint32_t* perm,
-                          uint32_t* pair_vox, float* pair_c, float* pair_sf, hipStream_t s);
-// records get their pair (new numbering); win_idx[window position] = the
-// window record's slot (index - base_w)
-void launch_set_records(const SegmentBuild& b, const uint32_t* win, const uint32_t* wpos, const uint32_t* perm,
-                        uint32_t nlive, uint32_t base_w, uint32_t* win_idx, hipStream_t s);
 // records with their voxel ids restored (for export)
 void launch_export_vox(const uint2* rec, uint32_t n, const uint32_t* pair_vox, uint2* out, hipStream_t s);
 
@@ -129,6 +119,12 @@ struct CamPass {
   float* M;                // this camera's messages M_c (FP32, one per pair)
   void* Xo;                // ray->voxel messages p*l in window-region order (output side type)
   uint8_t* upd;
+  // device flag: nonzero while every pair's message must be refreshed (the
+  // first iteration of a call, or after messages were moved); otherwise only
+  // the window pairs are updated (a front-only pair's message is the constant
+  // logit(clamp(sf / C)) and already in M; its update would only add FP32
+  // rounding to B)
+  const uint32_t* all_pairs;
   const float* sig;        // sensor-model lookup tables
   const float* bia;
   int lookup_n;
@@ -173,6 +169,7 @@ void launch_scatter_pairs(const uint32_t* pair_vox, const float* M, uint32_t npa
 void launch_gather_pairs(const uint32_t* pair_vox, const float* dense, uint32_t npairs, float* M, hipStream_t s);
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_zero_words(void* p, size_t bytes, hipStream_t s);
 
 // ---- learned warm start (warm.hip): features, MLP, warm-start update

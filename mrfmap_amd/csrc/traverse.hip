@@ -190,9 +190,14 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
       if (best < 0 || rat_lt(t[a], t[best])) best = a;
     }
     const Rat zn = t[best];
-    const double zn_d = (double)zn.n / (double)zn.d;
+    // per-voxel depths only where records or visits need them; the marking and
+    // counting walks convert the exit depth at brick exits alone
+    constexpr bool kDepths = MODE == kEmit || MODE == kVisit;
+    double zn_d = 0.0;
+    if constexpr (kDepths) zn_d = (double)zn.n / (double)zn.d;
     const bool active = bslot >= 0;
     if (active) {
+      if (MODE == kMark) ++cnt;  // every brick counts as active: no gaps, the count walk's n
       if (MODE != kMark) {
         if (any && skipped) {
           if (MODE == kEmit) {
@@ -240,8 +245,9 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
       } else {
         const float* sc = sig;
         const float* bc = bia;
-        const double lhs = zn_d + poly_eval(bc, lookup_n, zn_d);
-        const double rhs = depth + 3.0 * poly_eval(sc, lookup_n, zn_d);
+        const double zb = kDepths ? zn_d : (double)zn.n / (double)zn.d;
+        const double lhs = zb + poly_eval(bc, lookup_n, zb);
+        const double rhs = depth + 3.0 * poly_eval(sc, lookup_n, zb);
         if (lhs >= rhs) {
           wo.ran_out = false;
           break;
@@ -292,14 +298,24 @@ __device__ __forceinline__ PixelCtx load_pixel(const CamGeom& c, const GridGeom&
   return p;
 }
 
+// Full-ray activation: marks every brick a ray crosses up to its truncation.
+// A ray's own walk activates every brick before and at its stopping brick,
+// so its later list walk sees exactly these bricks, with no gaps: the
+// number of voxels stepped here is that walk's record count, written to
+// counts / has (per tiled pixel) when they are given.
 __global__ void k_mark_rays(CamGeom c, GridGeom g, const float* __restrict__ depth,
                             const float* __restrict__ sig, const float* __restrict__ bia,
-                            uint8_t* __restrict__ bflags) {
+                            uint8_t* __restrict__ bflags, uint32_t* __restrict__ counts,
+                            uint32_t* __restrict__ has) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= tiled_pixels(c)) return;
   PixelCtx p = load_pixel(c, g, depth, pix);
-  if (!p.ok) return;
-  walk<kMark>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, nullptr, bflags, nullptr);
+  uint32_t n = 0;
+  if (p.ok) n = walk<kMark>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, nullptr, bflags, nullptr).n;
+  if (counts) {
+    counts[pix] = n;
+    has[pix] = n > 0 ? 1u : 0u;
+  }
 }
 
 __global__ void k_mark_endpoints(CamGeom c, GridGeom g, const float* __restrict__ depth,
@@ -609,9 +625,10 @@ void launch_eval(const CamGeom& c, const GridGeom& g, const float* depth, const 
 }
 
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                      const float* bia, uint8_t* bflags, hipStream_t s) {
+                      const float* bia, uint8_t* bflags, uint32_t* counts, uint32_t* has, hipStream_t s) {
   const int n = tiled_pixels(c);
-  hipLaunchKernelGGL(k_mark_rays, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, bflags);
+  hipLaunchKernelGGL(k_mark_rays, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, bflags, counts,
+                     has);
 }
 
 void launch_feat_hits(const CamGeom& c, const GridGeom& g, const float* depth, const int32_t* slot, uint32_t* hits,
