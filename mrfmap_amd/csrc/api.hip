@@ -44,6 +44,9 @@ struct Camera {
   uint32_t nrays = 0;              // rays with at least one record
   uint2* d_inc = nullptr;          // sliced records {pair | gap, length}: front region, then window region
   uint32_t* d_walkn = nullptr;     // full-ray activation: records per tiled pixel, then has-records flags
+  uint8_t* d_lnn = nullptr;        // per pair: voxel -> ray message ln n (input-side message type)
+  uint32_t* d_dyn = nullptr;       // front-only pairs at voxels where some camera has window records
+  uint32_t ndyn = 0;
   uint64_t nrec = 0, base_w = 0;   // padded records; first record of the window region
   uint64_t nupd = 0;               // live incidences
   uint64_t nwin = 0;               // live window incidences (window positions)
@@ -88,7 +91,7 @@ struct mrf_ctx {
   void* d_Xo = nullptr;   // segment array of ray->voxel messages of the camera being processed
   size_t Xo_bytes = 0;
   // per-pair / per-tile scratch of a camera pass
-  float *d_apair = nullptr, *d_Spair = nullptr;
+  float* d_Spair = nullptr;
   size_t cap_pairs = 0;
   float* d_partial = nullptr;  // front partial sums (64 per chunk)
   size_t cap_partial = 0;
@@ -121,7 +124,7 @@ struct mrf_ctx {
   uint64_t* d_scan_tmp = nullptr;
   size_t cap_scan = 0;
   uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
-  uint32_t* d_all_pairs = nullptr;  // 1: the next epilogues refresh every pair (see CamPass::all_pairs)
+  uint32_t* d_all_pairs = nullptr;  // [0] CamPass::all_pairs, [1] CamPass::msg_all
 
   std::vector<Camera> cams;
   bool geom_dirty = true;
@@ -370,6 +373,9 @@ int ensure_sell0(mrf_ctx* c, size_t n) {
 }
 
 void free_camera_lists(Camera& cam) {
+  dfree(&cam.d_lnn);
+  dfree(&cam.d_dyn);
+  cam.ndyn = 0;
   dfree(&cam.d_front);
   dfree(&cam.d_lanes);
   dfree(&cam.d_len_f);
@@ -520,6 +526,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.nwp = nwp;
   if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
       (rc = dalloc(c, &cam.d_pair_sf, np)) || (rc = dalloc(c, &cam.d_M, np)) ||
+      (rc = dalloc(c, &cam.d_lnn, (size_t)np * msg_bytes(msg_in_type(c->p.msg_dtype)))) ||
       (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
     return rc;
   launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
@@ -590,6 +597,32 @@ int prepare(mrf_ctx* c) {
   // 3. traversal lists and pairs of every camera
   for (Camera& cam : c->cams)
     if ((rc = traverse_camera(c, cam))) return rc;
+  // 3b. voxels where some camera has window records; per camera, its
+  //     front-only pairs there (their ln n keeps changing after the first
+  //     iterations of a call, the others' stops)
+  {
+    const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
+    if ((rc = ensure_vflags(c, std::max<size_t>(nvox, 1)))) return rc;
+    uint32_t* W = c->d_vflag;
+    uint32_t* flags = W + nvox;
+    uint32_t* pos = flags + nvox;
+    launch_zero_words(W, nvox * 4, c->stream);
+    for (const Camera& cam : c->cams) launch_mark_window_voxels(cam.d_pair_vox, cam.nwp, W, c->stream);
+    for (Camera& cam : c->cams) {
+      const uint32_t nrest = cam.npairs - cam.nwp;
+      uint64_t ndyn = 0;
+      if (nrest) {
+        launch_dyn_flags(cam.d_pair_vox + cam.nwp, nrest, W, flags, c->stream);
+        launch_exclusive_scan(flags, pos, nrest, c->d_scan_tmp, c->d_total, c->stream);
+        ndyn = read_total(c);
+      }
+      dfree(&cam.d_dyn);
+      if ((rc = dalloc(c, &cam.d_dyn, std::max<uint64_t>(ndyn, 1)))) return rc;
+      cam.ndyn = (uint32_t)ndyn;
+      launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+  }
   // 4. per-pass scratch sized for the largest camera
   size_t xneed = 1, pneed = 1, tneed = 1, cneed = 1;
   for (const Camera& cam : c->cams) {
@@ -599,10 +632,9 @@ int prepare(mrf_ctx* c) {
     tneed = std::max<size_t>(tneed, (cam.nwin + kTile - 1) / kTile);
   }
   if (pneed > c->cap_pairs) {
-    dfree(&c->d_apair);
     dfree(&c->d_Spair);
     c->cap_pairs = 0;
-    if ((rc = dalloc(c, &c->d_apair, pneed)) || (rc = dalloc(c, &c->d_Spair, pneed))) return rc;
+    if ((rc = dalloc(c, &c->d_Spair, pneed))) return rc;
     c->cap_pairs = pneed;
   }
   if (cneed > c->cap_partial) {
@@ -685,7 +717,10 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.wseg_off = cam.d_wseg_off;
   a.heads = cam.d_heads;
   a.tile_h0 = cam.d_tile_h0;
-  a.apair = c->d_apair;
+  a.apair = cam.d_lnn;
+  a.dyn = cam.d_dyn;
+  a.ndyn = cam.ndyn;
+  a.msg_all = c->d_all_pairs + 1;
   a.S = c->d_Spair;
   a.pf_q = c->d_pf_q;
   a.pl_q = c->d_pl_q;
@@ -1055,9 +1090,16 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
   }
   int rc;
   if ((rc = dalloc(c, &c->d_slot, (size_t)c->nbricks)) || (rc = dalloc(c, &c->d_bflags, (size_t)c->nbricks)) ||
-      (rc = dalloc(c, &c->d_total, 4)) || (rc = dalloc(c, &c->d_all_pairs, 1))) {
+      (rc = dalloc(c, &c->d_total, 4)) || (rc = dalloc(c, &c->d_all_pairs, 2))) {
     mrf_destroy(c);
     return rc;
+  }
+  {
+    const uint32_t flags[2] = {1u, 2u};  // defined before the autotuner's first pair pass
+    if (hipMemcpy(c->d_all_pairs, flags, sizeof(flags), hipMemcpyHostToDevice) != hipSuccess) {
+      mrf_destroy(c);
+      return MRF_E_HIP;
+    }
   }
   if (p->depth_sigma_lookup && p->depth_bias_lookup) {
     const size_t cells = (size_t)((p->rows + kLookupPx - 1) / kLookupPx) * ((p->cols + kLookupPx - 1) / kLookupPx);
@@ -1092,7 +1134,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
   if (c->d_Xo) (void)hipFree(c->d_Xo);
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
-  dfree(&c->d_apair); dfree(&c->d_Spair); dfree(&c->d_partial);
+  dfree(&c->d_Spair); dfree(&c->d_partial);
   dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
   dfree(&c->d_heads); dfree(&c->d_hscan);
@@ -1195,7 +1237,8 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   // reset M_c of the selected cameras (GVDBInference.cpp:128-134); the
   // first iteration refreshes every pair's message
   for (uint32_t id : sel) launch_zero_words(c->cams[id].d_M, (size_t)c->cams[id].npairs * sizeof(float), c->stream);
-  launch_fill_u32(c->d_all_pairs, 1u, 1, c->stream);
+  launch_fill_u32(c->d_all_pairs, 1u, 1, c->stream);      // every pair's update
+  launch_fill_u32(c->d_all_pairs + 1, 2u, 1, c->stream);  // every pair's ln n, two iterations
   launch_marginals(c->d_B, c->d_A, nvox, c->stream);
   for (const Camera& cam : c->cams) {
     s.updates_per_iter += (int64_t)cam.nupd;
@@ -1241,6 +1284,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
       HIPCHK(c, enqueue_iteration(c, ev.data() + 2));
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
     launch_fill_u32(c->d_all_pairs, 0u, 1, c->stream);  // front-only messages are in M now
+    launch_countdown(c->d_all_pairs + 1, c->stream);
     uint64_t bits = 0, ebits = 0;
     if (c->ref_n) {
       launch_ref_error(c->d_A, c->d_upd, c->d_ref, nvox, c->d_rerr, c->stream);
@@ -1277,6 +1321,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
         launch_warm_pairs(cam.d_pair_vox, cam.d_pair_c, cam.npairs, c->d_wdelta, cam.d_M, c->stream);
       launch_marginals(c->d_B, c->d_A, nvox, c->stream);
       launch_fill_u32(c->d_all_pairs, 1u, 1, c->stream);  // every message was moved
+      launch_fill_u32(c->d_all_pairs + 1, 2u, 1, c->stream);
       HIPCHK(c, hipStreamSynchronize(c->stream));
       HIPCHK(c, hipGetLastError());
       s.t_warm_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
@@ -1286,9 +1331,11 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
       check = false;
     } else if (warm_state == 1 && it > warm_after && growth > 0 && resid > growth * r_prev && resid > 0.05) {
       // the residual grows: back to the plain state before the warm start
-      // (the restored front-only messages are their constants already)
+      // (the restored front-only messages are their constants already; the
+      // stored ln n of the warm iterations are recomputed once)
       if ((rc = snapshot(c, false))) return rc;
       launch_marginals(c->d_B, c->d_A, nvox, c->stream);
+      launch_fill_u32(c->d_all_pairs + 1, 1u, 1, c->stream);
       HIPCHK(c, hipStreamSynchronize(c->stream));
       HIPCHK(c, hipGetLastError());
       warm_state = 2;

@@ -127,16 +127,43 @@ __device__ __forceinline__ uint32_t pair_of(uint64_t word, uint32_t before, int 
 // Voxel -> ray message of every pair: ln n = ln clamp(1/(1+exp(B - M_c))),
 // camera c's own previous message excluded from the belief (gvdbkernels.cu:764-773).
 template <int MT>
-__global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ pair_vox, uint32_t npairs,
-                                                  const float* __restrict__ B, const float* __restrict__ M,
-                                                  typename Msg<MT>::T* __restrict__ apair) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npairs) return;
+__device__ __forceinline__ void pair_msg_one(uint32_t q, const uint32_t* __restrict__ pair_vox,
+                                             const float* __restrict__ B, const float* __restrict__ M,
+                                             typename Msg<MT>::T* __restrict__ apair) {
   const uint32_t v = pair_vox[q];
   const float qv = B[v] - M[q];
   float nn = 1.0f / (1.0f + __expf(qv));
   nn = fminf(fmaxf(nn, kFEps), 1.0f - kFEps);
   Msg<MT>::store(apair, q, __logf(nn));
+}
+
+// the window pairs [0, nwp): their beliefs change every iteration
+template <int MT>
+__global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ pair_vox, uint32_t nwp,
+                                                  const float* __restrict__ B, const float* __restrict__ M,
+                                                  typename Msg<MT>::T* __restrict__ apair) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nwp) pair_msg_one<MT>(q, pair_vox, B, M, apair);
+}
+
+// the front-only pairs: all of them while *msg_all (the first two iterations
+// of a call, or after messages were moved), else only those at voxels where
+// some camera has window records (dyn): elsewhere the belief and M_c have
+// stopped changing and the stored ln n is the value a recomputation gives.
+template <int MT>
+__global__ __launch_bounds__(256) void k_pair_msg_rest(const uint32_t* __restrict__ pair_vox, uint32_t npairs,
+                                                       uint32_t nwp, const uint32_t* __restrict__ dyn, uint32_t ndyn,
+                                                       const float* __restrict__ B, const float* __restrict__ M,
+                                                       typename Msg<MT>::T* __restrict__ apair,
+                                                       const uint32_t* __restrict__ msg_all) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  if (*msg_all) {
+    for (uint32_t q = nwp + blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += stride)
+      pair_msg_one<MT>(q, pair_vox, B, M, apair);
+  } else {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ndyn; k += stride)
+      pair_msg_one<MT>(dyn[k], pair_vox, B, M, apair);
+  }
 }
 
 // ---------------------------------------------------------------- ray kernel
@@ -599,17 +626,12 @@ __global__ __launch_bounds__(256) void k_seg_span(const uint32_t* __restrict__ p
 // (compute_gvdb_message_to_factors, gvdbkernels.cu:408, 457-479).
 // S = (sum of the pair's constant front messages) + (its window segment
 // sum); window pairs are numbered first.
-__global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
-                                                    const float* __restrict__ pair_c,
-                                                    const float* __restrict__ pair_sf, uint32_t npairs, uint32_t nwp,
-                                                    const float* __restrict__ S, float out_scale, float* __restrict__ B,
-                                                    float* __restrict__ M, uint8_t* __restrict__ upd,
-                                                    const uint32_t* __restrict__ all_pairs) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npairs) return;
-  // a front-only pair's m is the same every iteration: once it is in M the
-  // update B = (B + m) - M would add nothing but FP32 rounding
-  if (q >= nwp && *all_pairs == 0u) return;
+// m = logit(clamp(S / C)), B += m - M_c, M_c = m for pair q
+__device__ __forceinline__ void seg_update_one(uint32_t q, const uint32_t* __restrict__ pair_vox,
+                                               const float* __restrict__ pair_c, const float* __restrict__ pair_sf,
+                                               uint32_t nwp, const float* __restrict__ S, float out_scale,
+                                               float* __restrict__ B, float* __restrict__ M,
+                                               uint8_t* __restrict__ upd) {
   const float C = pair_c[q];
   if (!(C > kCumLenMin)) return;
   const uint32_t v = pair_vox[q];
@@ -622,6 +644,31 @@ __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__
   M[q] = m;
   B[v] = (B[v] + m) - mo;
   upd[v] = 1;
+}
+
+// the window pairs [0, nwp)
+__global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
+                                                    const float* __restrict__ pair_c,
+                                                    const float* __restrict__ pair_sf, uint32_t nwp,
+                                                    const float* __restrict__ S, float out_scale, float* __restrict__ B,
+                                                    float* __restrict__ M, uint8_t* __restrict__ upd) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nwp) seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, S, out_scale, B, M, upd);
+}
+
+// the front-only pairs [nwp, npairs), only while *all_pairs: a front-only
+// pair's m is the same every iteration, and once it is in M the update
+// B = (B + m) - M would add nothing but FP32 rounding.  A small grid that
+// strides over the pairs, so that an idle launch costs a few blocks.
+__global__ __launch_bounds__(256) void k_seg_update_front(const uint32_t* __restrict__ pair_vox,
+                                                          const float* __restrict__ pair_c,
+                                                          const float* __restrict__ pair_sf, uint32_t npairs,
+                                                          uint32_t nwp, float* __restrict__ B, float* __restrict__ M,
+                                                          uint8_t* __restrict__ upd,
+                                                          const uint32_t* __restrict__ all_pairs) {
+  if (*all_pairs == 0u) return;
+  for (uint32_t q = nwp + blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += gridDim.x * blockDim.x)
+    seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, nullptr, 1.0f, B, M, upd);
 }
 
 // Beliefs of an external map (distinct voxels): B = the given log-odds,
@@ -736,6 +783,28 @@ __global__ void k_fill_f32(float* p, float v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_countdown(uint32_t* p) {
+  if (threadIdx.x == 0 && *p) *p -= 1u;
+}
+
+// window-touched voxels: W[v] = 1 for the voxel of every window pair
+__global__ void k_mark_window_voxels(const uint32_t* __restrict__ pair_vox, uint32_t nwp, uint32_t* __restrict__ W) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nwp) W[pair_vox[q]] = 1u;
+}
+
+__global__ void k_dyn_flags(const uint32_t* __restrict__ pair_vox, uint32_t n, const uint32_t* __restrict__ W,
+                            uint32_t* __restrict__ flags) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) flags[k] = W[pair_vox[k]];
+}
+
+__global__ void k_dyn_compact(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
+                              uint32_t q0, uint32_t* __restrict__ dyn) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && flags[k]) dyn[pos[k]] = q0 + k;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -754,19 +823,30 @@ inline unsigned grid_for(size_t n, int threads) { return (unsigned)((n + threads
 
 }  // namespace
 
+template <int MT>
+void launch_pair_msgs_t(const CamPass& a, hipStream_t s) {
+  typedef typename Msg<MT>::T T;
+  if (a.nwp)
+    hipLaunchKernelGGL(k_pair_msg<MT>, dim3(grid_for(a.nwp, 256)), dim3(256), 0, s, a.pair_vox, a.nwp, a.B, a.M,
+                       (T*)a.apair);
+  if (a.npairs > a.nwp) {
+    const unsigned g = std::min<unsigned>(grid_for(a.npairs - a.nwp, 256), 4096u);
+    hipLaunchKernelGGL(k_pair_msg_rest<MT>, dim3(g), dim3(256), 0, s, a.pair_vox, a.npairs, a.nwp, a.dyn, a.ndyn,
+                       a.B, a.M, (T*)a.apair, a.msg_all);
+  }
+}
+
 void launch_pair_msgs(const CamPass& a, hipStream_t s) {
   if (a.npairs == 0) return;
-  const dim3 grid(grid_for(a.npairs, 256)), block(256);
   switch (msg_in_type(a.msg_type)) {
     case kMsgF16:
-      hipLaunchKernelGGL(k_pair_msg<kMsgF16>, grid, block, 0, s, a.pair_vox, a.npairs, a.B, a.M, (__half*)a.apair);
+      launch_pair_msgs_t<kMsgF16>(a, s);
       break;
     case kMsgBF16:
-      hipLaunchKernelGGL(k_pair_msg<kMsgBF16>, grid, block, 0, s, a.pair_vox, a.npairs, a.B, a.M,
-                         (uint16_t*)a.apair);
+      launch_pair_msgs_t<kMsgBF16>(a, s);
       break;
     default:
-      hipLaunchKernelGGL(k_pair_msg<kMsgF32>, grid, block, 0, s, a.pair_vox, a.npairs, a.B, a.M, (float*)a.apair);
+      launch_pair_msgs_t<kMsgF32>(a, s);
   }
 }
 
@@ -841,8 +921,14 @@ void launch_epilogue(const CamPass& a, hipStream_t s) {
                                a.pl_v, a.mid, a.wseg_off, s);
     }
   }
-  hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.npairs, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf,
-                     a.npairs, a.nwp, a.S, out_scale, a.B, a.M, a.upd, a.all_pairs);
+  if (a.nwp)
+    hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.nwp, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf,
+                       a.nwp, a.S, out_scale, a.B, a.M, a.upd);
+  if (a.npairs > a.nwp) {
+    const unsigned g = std::min<unsigned>(grid_for(a.npairs - a.nwp, 256), 4096u);
+    hipLaunchKernelGGL(k_seg_update_front, dim3(g), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf, a.npairs, a.nwp,
+                       a.B, a.M, a.upd, a.all_pairs);
+  }
 }
 
 void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
@@ -928,6 +1014,21 @@ void launch_gather_pairs(const uint32_t* pair_vox, const float* dense, uint32_t 
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+void launch_countdown(uint32_t* p, hipStream_t s) { hipLaunchKernelGGL(k_countdown, dim3(1), dim3(64), 0, s, p); }
+
+void launch_mark_window_voxels(const uint32_t* pair_vox, uint32_t nwp, uint32_t* W, hipStream_t s) {
+  if (nwp) hipLaunchKernelGGL(k_mark_window_voxels, dim3(grid_for(nwp, 256)), dim3(256), 0, s, pair_vox, nwp, W);
+}
+
+void launch_dyn_flags(const uint32_t* pair_vox, uint32_t n, const uint32_t* W, uint32_t* flags, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_dyn_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, pair_vox, n, W, flags);
+}
+
+void launch_dyn_compact(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t q0, uint32_t* dyn,
+                        hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_dyn_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, pos, n, q0, dyn);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {

@@ -111,7 +111,10 @@ struct CamPass {
   const uint32_t* wseg_off;  // nwp + 1 window segment offsets
   const uint64_t* heads;   // window segment head bitmask
   const uint32_t* tile_h0; // window heads before each kTile tile
-  void* apair;             // per-pair voxel -> ray message ln n (input side type)
+  void* apair;             // per-pair voxel -> ray message ln n (input side type), kept per camera
+  const uint32_t* dyn;     // front-only pairs at voxels where some camera has window records
+  uint32_t ndyn;
+  const uint32_t* msg_all; // device countdown: nonzero while every pair's ln n must be recomputed
   float* S;                // per window pair: window segment sum
   uint32_t *pf_q, *pl_q;   // per-tile scratch of the segment sums
   float *pf_v, *pl_v, *mid;
@@ -170,6 +173,11 @@ void launch_gather_pairs(const uint32_t* pair_vox, const float* dense, uint32_t 
 void launch_fill_f32(float* p, float v, size_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint8_t v, size_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_countdown(uint32_t* p, hipStream_t s);  // *p -= 1 unless 0
+void launch_mark_window_voxels(const uint32_t* pair_vox, uint32_t nwp, uint32_t* W, hipStream_t s);
+void launch_dyn_flags(const uint32_t* pair_vox, uint32_t n, const uint32_t* W, uint32_t* flags, hipStream_t s);
+void launch_dyn_compact(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t q0, uint32_t* dyn,
+                        hipStream_t s);
 void launch_zero_words(void* p, size_t bytes, hipStream_t s);
 
 // ---- learned warm start (warm.hip): features, MLP, warm-start update

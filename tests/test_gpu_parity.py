@@ -45,6 +45,31 @@ def test_traversal_bit_exact_small(activation):
     assert n > 1_000_000
 
 
+def test_uneven_keyframes_match_oracle():
+    """The batched generator at reduced size: 6 keyframes drawn from
+    320x240, 640x480 and 848x480 (each with its own intrinsics; 848x480 with
+    the 24/20-pixel border mask, GVDBImage.cpp:35-40) in the 10x10x3 m scene,
+    8 cm voxels, every 4th pixel row and column: traversal lists bit-exact
+    per camera and marginals after 3 iterations."""
+    p, frames, _ = synth.make_workload("batched", frames=6, res=0.08)
+    shapes = {f[2].shape for f in frames}
+    assert len(shapes) >= 2, shapes
+    frames = [(pose, K, decimate(depth, 4)) for pose, K, depth in frames]
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    n = _compare_traversal(g, o, len(frames))
+    assert n > 100_000
+    st = g.perform_inference(max_iters=3)
+    so = o.perform_inference(3)
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"uneven keyframes {sorted(shapes)}: {n} records, max-abs marginal error {err.max():.3e}")
+    assert err.max() <= MARG_TOL
+
+
 def test_traversal_bit_exact_real_shape():
     """640x480, f=525, 1 cm voxels: two keyframes, every 3rd pixel."""
     p, frames, _ = synth.make_workload("real", frames=13)
