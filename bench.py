@@ -157,7 +157,8 @@ def measured_traffic(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU split is described in DESIGN.md",
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build measures one GPU's share; "
+              "keyframes would shard across GPUs with one belief all-reduce per iteration, which is not built",
               file=sys.stderr)
         sys.exit(2)
     from mrfmap_amd import GVDBInference, _lib, synth

@@ -88,8 +88,8 @@ def test_params_missing_node_raises(tmp_path):
 
 
 def test_recorded_frame_fixture():
-    """data/depth_k.bin is noise-free planar depth; the bottom row sees the
-    floor at constant z and the top rows are NaN."""
+    """data/depth_k.bin of the reference is noise-free planar depth: the bottom
+    row sees the floor at constant z (1.9159 m) and the top rows are NaN."""
     z = np.load(os.path.join(GOLD, "coffin_frames.npz"))
     depth, pose = z["depth"], z["body_pose"]
     assert depth.shape == (2, 480, 640)
