@@ -7,7 +7,7 @@ messages land far inside it).
 import numpy as np
 import pytest
 
-from mrfmap_amd import _lib, synth
+from mrfmap_amd import GVDBInference, _lib, synth
 
 import oracle
 from helpers import decimate, gpu_for, oracle_for
@@ -350,3 +350,60 @@ def test_graph_replays_match_eager_launches(monkeypatch):
     for mode in ("timed", "env"):
         for a, b in zip(runs["graph"], runs[mode]):
             np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _edge_frames():
+    """A normal keyframe, two without valid depth (NaN, 0), a 1x1 image and a
+    75x101 image whose 8x8 tiles overhang the border, each with its own
+    intrinsics."""
+    p, frames, scene = synth.make_workload("small")
+    pose, K, depth = frames[0]
+    K2 = synth.intrinsics(120, 160).copy()
+    K2[0, 2], K2[1, 2] = 50.0, 37.0
+    K1 = np.array([[60, 0, 0.0], [0, 60, 0.0], [0, 0, 1]], np.float32)
+    fr = [frames[0], (frames[1][0], K, np.full_like(depth, np.nan)), (frames[2][0], K, np.zeros_like(depth)),
+          (frames[3][0], K1, np.full((1, 1), 2.0, np.float32)),
+          (frames[1][0], K2, scene.render(frames[1][0], K2, 75, 101))]
+    return p, fr
+
+
+def test_edge_case_keyframes_match_oracle():
+    """Empty keyframes, a 1x1 image and ragged tiles: traversal bit-exact per
+    camera (the empty ones have no rays), marginals after 3 iterations."""
+    p, fr = _edge_frames()
+    g = gpu_for(p, fr)
+    o = oracle_for(p, fr)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    for cam in (1, 2):
+        rays, recs = g.get_traversal(cam)
+        assert len(rays) == 0 and len(recs) == 0
+    n = _compare_traversal(g, o, len(fr))
+    assert n > 2_000_000
+    st = g.perform_inference(max_iters=3)
+    so = o.perform_inference(3)
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"edge-case keyframes: max-abs marginal error {err.max():.3e}")
+    assert err.max() <= MARG_TOL
+
+
+def test_empty_map_and_no_cameras():
+    """Only keyframes without valid depth: no bricks, empty readback, no
+    features, a warm start over no voxels; a context without keyframes runs
+    its iterations on an empty map."""
+    p, fr = _edge_frames()
+    g = gpu_for(p, fr[1:3])
+    st = g.perform_inference(max_iters=2)
+    assert st["iters"] == 2 and st["updates_per_iter"] == 0 and st["residual"] == 0.0
+    assert g.num_bricks() == 0
+    assert g.get_alphas().shape == (0, 512)
+    assert g.get_occupied_bricks().shape == (0, 3)
+    X, vox = g.features()
+    assert X.shape == (0, 32) and vox.shape == (0,)
+    st = g.perform_inference_warm(np.zeros(_lib.MLP_PARAMS, np.float32), max_iters=3)
+    assert st["iters"] == 3 and st["warm_voxels"] == 0
+    e = GVDBInference(params=p)
+    st = e.perform_inference(max_iters=2)
+    assert st["iters"] == 2 and st["updates_per_iter"] == 0
