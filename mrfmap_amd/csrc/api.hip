@@ -105,7 +105,7 @@ struct mrf_ctx {
   size_t cap_sort = 0;
   uint32_t* d_vflag = nullptr;  // 4 pool-sized tables: touched, window-touched and their scans
   size_t cap_vflag = 0;
-  unsigned long long* d_acc = nullptr;  // 2 per pair: fixed-point chord-length sums
+  uint32_t* d_acc = nullptr;  // 4 words per pair: two 64-bit fixed-point chord-length sums
   size_t cap_acc = 0;
   uint2* d_sell0 = nullptr;
   size_t cap_sell0 = 0;
@@ -123,6 +123,8 @@ struct mrf_ctx {
   size_t cap_pix = 0;
   uint64_t* d_scan_tmp = nullptr;
   size_t cap_scan = 0;
+  void* d_fscan_tmp = nullptr;  // flag-scan scratch
+  size_t fscan_bytes = 0;
   uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
   uint32_t* d_all_pairs = nullptr;  // [0] CamPass::all_pairs, [1] CamPass::msg_all
 
@@ -342,6 +344,29 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   return ensure_scan(c, n);
 }
 
+// Exclusive scan of 0/1 flags; the total lands in d_total[0].
+int flag_scan(mrf_ctx* c, const uint32_t* in, uint32_t* out, size_t n) {
+  if (n >= 0x7FFFFFFFull) {  // beyond the single-pass scan's item count: the general scan
+    int rc = ensure_scan(c, n);
+    if (rc) return rc;
+    launch_exclusive_scan(in, out, n, c->d_scan_tmp, c->d_total, c->stream);
+    return MRF_OK;
+  }
+  const size_t tb = flag_scan_temp_bytes(n);
+  if (tb > c->fscan_bytes || !c->d_fscan_tmp) {
+    if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp);
+    c->d_fscan_tmp = nullptr;
+    c->fscan_bytes = 0;
+    if (hipMalloc(&c->d_fscan_tmp, std::max<size_t>(tb, 256)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, MRF_E_NOMEM, "flag-scan scratch allocation failed");
+    }
+    c->fscan_bytes = std::max<size_t>(tb, 256);
+  }
+  HIPCHK(c, launch_flag_scan(in, out, n, c->d_fscan_tmp, c->fscan_bytes, c->d_total, c->stream));
+  return MRF_OK;
+}
+
 int ensure_vflags(mrf_ctx* c, size_t nvox) {
   if (nvox <= c->cap_vflag && c->d_vflag) return MRF_OK;
   dfree(&c->d_vflag);
@@ -356,7 +381,7 @@ int ensure_acc(mrf_ctx* c, size_t np) {
   if (np <= c->cap_acc && c->d_acc) return MRF_OK;
   dfree(&c->d_acc);
   c->cap_acc = 0;
-  int rc = dalloc(c, &c->d_acc, 2 * np);
+  int rc = dalloc(c, &c->d_acc, 4 * np);
   if (rc) return rc;
   c->cap_acc = np;
   return MRF_OK;
@@ -518,9 +543,9 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   uint32_t* rank_w = pid + nvox;
   launch_zero_words(any, (size_t)nvox * 8, s);  // any and wany
   launch_vox_flags(cam.d_inc, nrec, nrec_f, any, wany, s);
-  launch_exclusive_scan(any, pid, nvox, c->d_scan_tmp, c->d_total, s);
+  if ((rc = flag_scan(c, any, pid, nvox))) return rc;
   const uint32_t np = (uint32_t)read_total(c);
-  launch_exclusive_scan(wany, rank_w, nvox, c->d_scan_tmp, c->d_total, s);
+  if ((rc = flag_scan(c, wany, rank_w, nvox))) return rc;
   const uint32_t nwp = (uint32_t)read_total(c);
   cam.npairs = np;
   cam.nwp = nwp;
@@ -531,7 +556,8 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     return rc;
   launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
   // 5. per pair: C and the constant front messages
-  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + np, np, cam.d_pair_c, cam.d_pair_sf, s);
+  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, np, cam.d_pair_c,
+                      cam.d_pair_sf, s);
   // 6. window segments: the window records stably sorted by pair
   const int wbits = std::max(1, bit_length(nwp));
   if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits))) return rc;
@@ -543,7 +569,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if (nwin) {
     HIPCHK(c, hipMemcpyAsync(cam.d_win_idx, c->d_vals_out, nwin * 4, hipMemcpyDeviceToDevice, s));
     launch_window_heads(c->d_keys_out, (uint32_t)nwin, whead, s);
-    launch_exclusive_scan(whead, hw, nwin, c->d_scan_tmp, c->d_total, s);
+    if ((rc = flag_scan(c, whead, hw, nwin))) return rc;
   }
   launch_window_offsets(c->d_keys_out, whead, (uint32_t)nwin, nwp, cam.d_wseg_off, s);
   // 7. records point at their pairs
@@ -613,7 +639,7 @@ int prepare(mrf_ctx* c) {
       uint64_t ndyn = 0;
       if (nrest) {
         launch_dyn_flags(cam.d_pair_vox + cam.nwp, nrest, W, flags, c->stream);
-        launch_exclusive_scan(flags, pos, nrest, c->d_scan_tmp, c->d_total, c->stream);
+        if ((rc = flag_scan(c, flags, pos, nrest))) return rc;
         ndyn = read_total(c);
       }
       dfree(&cam.d_dyn);
@@ -927,7 +953,7 @@ int features_device(mrf_ctx* c, uint32_t* n_out) {
                       c->d_wnc, c->stream);
   }
   launch_feat_flags(c->d_wnc, nvox, c->d_wflag, c->stream);
-  launch_exclusive_scan(c->d_wflag, c->d_wrow, nvox, c->d_scan_tmp, c->d_total, c->stream);
+  if ((rc = flag_scan(c, c->d_wflag, c->d_wrow, nvox))) return rc;
   const uint64_t n = read_total(c);
   if (n > c->cap_wrows) {
     dfree(&c->d_wX); dfree(&c->d_wvox); dfree(&c->d_wpred);
@@ -1149,7 +1175,8 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_tSel); dfree(&c->d_tLpart); dfree(&c->d_tLoss);
   drop_graph(c);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
-  dfree(&c->d_scan_tmp); dfree(&c->d_total); dfree(&c->d_all_pairs);
+  dfree(&c->d_scan_tmp); dfree(&c->d_total);
+  if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp); dfree(&c->d_all_pairs);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

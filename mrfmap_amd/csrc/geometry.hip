@@ -11,8 +11,10 @@
 // accumulates with float atomics in every camera pass,
 // src/cuda/gvdbkernels.cu:519-571, 1276-1302) and the chord length of its
 // front records (whose messages are the constant F_EPS * l) are summed in
-// 2^-32 fixed point with 64-bit integer atomics: the sums are exact, so they
-// do not depend on the order in which records arrive.
+// 2^-24 fixed point as 64-bit values kept in two 32-bit words: a 32-bit
+// atomic add to the low word, plus one to the high word when it wraps.  The
+// sums are exact, so they do not depend on the order in which records
+// arrive.
 //
 // Window segments.  The window records are stably radix-sorted by pair, so
 // each pair's records stay in record order; the BP iteration sums each
@@ -27,7 +29,13 @@ namespace mrf {
 
 namespace {
 
-constexpr double kFix32 = 4294967296.0;  // 2^32
+constexpr double kFixLen = 16777216.0;  // 2^24: chord lengths (< 2) in fixed point, a 64-record sum < 2^31
+
+// exact add of q to the 64-bit value {lo = p[0], hi = p[1]}
+__device__ __forceinline__ void add_split(uint32_t* p, uint32_t q) {
+  const uint32_t old = atomicAdd(&p[0], q);
+  if (old + q < old) atomicAdd(&p[1], 1u);
+}
 
 // any[v] = 1 for every voxel of a live record, wany[v] = 1 for a window record
 __global__ void k_vox_flags(const uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w, uint32_t* __restrict__ any,
@@ -57,29 +65,28 @@ __global__ void k_pair_table(const uint32_t* __restrict__ any, const uint32_t* _
 // across its lanes (bitonic network), sums each run of equal pairs
 // (segmented scan) and issues one atomic per run instead of one per record.
 __global__ void __launch_bounds__(256) k_pair_lengths(const uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
-                                                      const uint32_t* __restrict__ pid,
-                                                      unsigned long long* __restrict__ acc_c,
-                                                      unsigned long long* __restrict__ acc_f) {
+                                                      const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_c,
+                                                      uint32_t* __restrict__ acc_f) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t base = wave * 64; base < nrec; base += nwaves * 64) {
     const uint64_t i = base + lane;
     uint32_t key = 0xFFFFFFFFu;
-    unsigned long long qc = 0, qf = 0;
+    uint32_t qc = 0, qf = 0;
     if (i < nrec) {
       const uint2 e = rec[i];
       if (e.x != kGapVox) {
         key = pid[e.x];
-        qc = (unsigned long long)((double)__uint_as_float(e.y) * kFix32);
-        qf = i < base_w ? qc : 0ull;
+        qc = (uint32_t)((double)__uint_as_float(e.y) * kFixLen);
+        qf = i < base_w ? qc : 0u;
       }
     }
     // bitonic sort of (key, qc, qf) across the 64 lanes, ascending key
     for (int k = 2; k <= 64; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
         const uint32_t ok = __shfl_xor(key, j, 64);
-        const unsigned long long oc = __shfl_xor(qc, j, 64), of = __shfl_xor(qf, j, 64);
+        const uint32_t oc = __shfl_xor(qc, j, 64), of = __shfl_xor(qf, j, 64);
         const bool up = (lane & k) == 0;        // this block of k lanes sorts ascending
         const bool lower = (lane & j) == 0;     // this lane keeps the smaller of the pair when ascending
         const bool take = (lower == up) ? (ok < key) : (ok > key);
@@ -93,7 +100,7 @@ __global__ void __launch_bounds__(256) k_pair_lengths(const uint2* __restrict__ 
     const uint32_t prev = __shfl_up(key, 1, 64);
     bool head = lane == 0 || prev != key;
     for (int d = 1; d < 64; d <<= 1) {
-      const unsigned long long oc = __shfl_up(qc, d, 64), of = __shfl_up(qf, d, 64);
+      const uint32_t oc = __shfl_up(qc, d, 64), of = __shfl_up(qf, d, 64);
       const bool oh = __shfl_up((int)head, d, 64) != 0;
       if (lane >= d && !head) {
         qc += oc;
@@ -104,18 +111,20 @@ __global__ void __launch_bounds__(256) k_pair_lengths(const uint2* __restrict__ 
     const uint32_t next = __shfl_down(key, 1, 64);
     const bool tail = lane == 63 || next != key;
     if (tail && key != 0xFFFFFFFFu) {
-      atomicAdd(&acc_c[key], qc);
-      if (qf) atomicAdd(&acc_f[key], qf);
+      add_split(acc_c + 2 * (size_t)key, qc);
+      if (qf) add_split(acc_f + 2 * (size_t)key, qf);
     }
   }
 }
 
-__global__ void k_pair_finish(const unsigned long long* __restrict__ acc_c, const unsigned long long* __restrict__ acc_f,
-                              uint32_t np, float* __restrict__ pair_c, float* __restrict__ pair_sf) {
+__global__ void k_pair_finish(const uint32_t* __restrict__ acc_c, const uint32_t* __restrict__ acc_f, uint32_t np,
+                              float* __restrict__ pair_c, float* __restrict__ pair_sf) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  pair_c[p] = (float)((double)acc_c[p] / kFix32);
-  pair_sf[p] = (float)((double)kFEps * ((double)acc_f[p] / kFix32));
+  const double c = (double)acc_c[2 * p + 1] * 4294967296.0 + (double)acc_c[2 * p];
+  const double f = (double)acc_f[2 * p + 1] * 4294967296.0 + (double)acc_f[2 * p];
+  pair_c[p] = (float)(c / kFixLen);
+  pair_sf[p] = (float)((double)kFEps * (f / kFixLen));
 }
 
 // sort keys of the window region: the record's pair, nwp (sorts last) for
@@ -199,9 +208,8 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
                      pair_vox);
 }
 
-void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid,
-                         unsigned long long* acc_c, unsigned long long* acc_f, uint32_t np, float* pair_c,
-                         float* pair_sf, hipStream_t s) {
+void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
+                         uint32_t* acc_f, uint32_t np, float* pair_c, float* pair_sf, hipStream_t s) {
   if (np == 0) return;
   launch_zero_words(acc_c, (size_t)np * 8, s);
   launch_zero_words(acc_f, (size_t)np * 8, s);

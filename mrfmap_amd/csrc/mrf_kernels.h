@@ -48,6 +48,11 @@ void launch_eval(const CamGeom& c, const GridGeom& g, const float* depth, const 
 
 // scan.hip: exclusive prefix sum of n uint32 values; total is written to
 // *d_total (device, uint64).  tmp must hold scan_tmp_words(n) uint64 words.
+// exclusive scan of 0/1 flags (n < 2^31), total into *d_total; temp holds
+// flag_scan_temp_bytes(n) bytes
+size_t flag_scan_temp_bytes(size_t n);
+hipError_t launch_flag_scan(const uint32_t* in, uint32_t* out, size_t n, void* temp, size_t temp_bytes,
+                            uint64_t* d_total, hipStream_t s);
 size_t scan_tmp_words(size_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint64_t* tmp, uint64_t* d_total,
                            hipStream_t s);
@@ -65,10 +70,10 @@ void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t
 void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
                        uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s);
 // per pair: C (summed chord length) and F_EPS x the chord length of its
-// front records (index < base_w), exact fixed-point sums
-void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid,
-                         unsigned long long* acc_c, unsigned long long* acc_f, uint32_t np, float* pair_c,
-                         float* pair_sf, hipStream_t s);
+// front records (index < base_w), exact fixed-point sums; acc_c / acc_f hold
+// 2 words per pair
+void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
+                         uint32_t* acc_f, uint32_t np, float* pair_c, float* pair_sf, hipStream_t s);
 size_t window_sort_temp_bytes(uint32_t n, int bits);
 // window region records stably sorted by pair (gap / padding records last):
 // keys_out = pairs, vals_out = record index in the region

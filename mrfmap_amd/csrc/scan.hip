@@ -1,7 +1,14 @@
-// Device-wide exclusive prefix sum of uint32 values (three launches: block
-// totals, one-block scan of the totals, block-local scan + offset).  Used to
-// turn per-ray incidence counts into list offsets and to compact rays and
-// voxel pairs; deterministic by construction.
+// Device-wide exclusive prefix sums of uint32 values, deterministic by
+// construction.
+//  * launch_exclusive_scan: three launches (block totals, one-block scan of
+//    the totals, block-local scan + offset) with a 64-bit total, for counts
+//    whose sum may exceed 32 bits (list offsets; the callers check limits).
+//  * launch_flag_scan: 0/1 flags, whose sum is at most n, so a 32-bit scan is
+//    exact: hipCUB's single-pass scan and a one-thread total; used for the
+//    brick-pool-sized flag arrays, where the three-launch scan's one-block
+//    middle step dominates.
+#include <hipcub/hipcub.hpp>
+
 #include "mrf_kernels.h"
 
 namespace mrf {
@@ -78,7 +85,31 @@ __global__ __launch_bounds__(kThreads) void k_apply(const uint32_t* __restrict__
   }
 }
 
+__global__ void k_flag_total(const uint32_t* __restrict__ in, const uint32_t* __restrict__ out, size_t n,
+                             uint64_t* __restrict__ d_total) {
+  if (threadIdx.x == 0) *d_total = (uint64_t)out[n - 1] + in[n - 1];
+}
+
 }  // namespace
+
+size_t flag_scan_temp_bytes(size_t n) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n);
+  return bytes;
+}
+
+hipError_t launch_flag_scan(const uint32_t* in, uint32_t* out, size_t n, void* temp, size_t temp_bytes,
+                            uint64_t* d_total, hipStream_t s) {
+  if (n == 0) {
+    launch_zero_words(d_total, sizeof(uint64_t), s);
+    return hipGetLastError();
+  }
+  size_t tb = temp_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, tb, in, out, (int)n, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_flag_total, dim3(1), dim3(64), 0, s, in, out, n, d_total);
+  return hipGetLastError();
+}
 
 size_t scan_tmp_words(size_t n) { return (n + kPerBlock - 1) / kPerBlock + 1; }
 
