@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=3e-5, help="residual of the reference run")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged run (profiling)")
+    ap.add_argument("--legs", default="plain,warm,warm-f16",
+                    help="time-to-converged runs: plain (FP32 BP), warm (learned warm start, FP32), "
+                         "warm-f16 (learned warm start, FP16 ray->voxel messages)")
     ap.add_argument("--launch", default="timed", choices=["timed", "graph"],
                     help="launch mode of the timed steps: kernels one by one with per-kernel HIP events "
                          "(the roofline's kernel durations come from these steps), or graph replays; "
@@ -110,30 +113,34 @@ def cpu_comparator(wl):
                       f"({n} rays, {dt:.2f} s)"}
 
 
-def time_to_converged(p, frames, ref):
-    """Plain vs warm-started BP to a map within 1e-3 of ``ref`` (fresh contexts)."""
+def time_to_converged(p, frames, ref, which):
+    """Plain vs warm-started BP to a map within 1e-3 of ``ref`` (fresh
+    contexts, one at a time); ``which`` lists the legs to run."""
     from mrfmap_amd import _lib
     from mrfmap_amd import warmstart as W
 
     params, meta = W.load_weights()
     keep = ("ms", "geometry_ms", "iters", "error", "reached", "warm", "warm_ms")
     legs = {}
-    legs["plain_f32"] = W.time_to_error(p, frames, ref, 1e-3, max_iters=300)
-    legs["warm_f32"] = W.time_to_error(p, frames, ref, 1e-3, params=params, warm_after=meta["warm_after"],
-                                       max_iters=300)
-    msg0 = p.msg_dtype
-    try:
-        p.msg_dtype = _lib.MSG_MIXED_F16
-        legs["warm_mixed_f16"] = W.time_to_error(p, frames, ref, 1e-3, params=params,
-                                                 warm_after=meta["warm_after"], max_iters=300)
-    finally:
-        p.msg_dtype = msg0
+    if "plain" in which:
+        legs["plain_f32"] = W.time_to_error(p, frames, ref, 1e-3, max_iters=300)
+    if "warm" in which:
+        legs["warm_f32"] = W.time_to_error(p, frames, ref, 1e-3, params=params, warm_after=meta["warm_after"],
+                                           max_iters=300)
+    if "warm-f16" in which:
+        msg0 = p.msg_dtype
+        try:
+            p.msg_dtype = _lib.MSG_MIXED_F16
+            legs["warm_mixed_f16"] = W.time_to_error(p, frames, ref, 1e-3, params=params,
+                                                     warm_after=meta["warm_after"], max_iters=300)
+        finally:
+            p.msg_dtype = msg0
     out = {k: {f: v[f] for f in keep} for k, v in legs.items()}
     for k, v in out.items():
         print(f"[bench] time-to-1e-3 {k}: {v['iters']} iters {v['ms']:.1f} ms reached={v['reached']} "
               f"error {v['error']:.1e}", file=sys.stderr, flush=True)
     best = min((v["ms"] for k, v in out.items() if k != "plain_f32" and v["reached"]), default=None)
-    out["speedup_warm_vs_plain"] = (out["plain_f32"]["ms"] / best) if best else None
+    out["speedup_warm_vs_plain"] = (out["plain_f32"]["ms"] / best) if (best and "plain_f32" in out) else None
     out["warm_network"] = {"file": "mrfmap_amd/data/warm_mlp.npz", "warm_after": meta["warm_after"],
                            "training_seeds": meta["seeds"], "samples": meta["samples"]}
     return out
@@ -177,7 +184,8 @@ def main():
         return g
 
     # reference map: plain BP from a fresh map to a residual below tol; its
-    # context then serves the timed steps
+    # context then serves the timed steps and is closed before the
+    # time-to-converged runs (a 64-keyframe context fills half the HBM)
     g = build()
     ttc = None
     if a.no_converge:
@@ -188,8 +196,7 @@ def main():
         print(f"[bench] reference: {conv['iters']} iters residual {conv['residual']:.2e} "
               f"total {conv['t_total_ms']:.1f} ms (geometry {conv['t_geometry_ms']:.1f} ms)", file=sys.stderr,
               flush=True)
-        if msg == _lib.MSG_F32:
-            ttc = time_to_converged(p, frames, g.get_alphas())
+    ref = g.get_alphas() if (not a.no_converge and msg == _lib.MSG_F32) else None
 
     # throughput: W warmup steps then exactly K timed steps on resident data
     modes = {"timed": _lib.LAUNCH_TIMED, "graph": _lib.LAUNCH_GRAPH}
@@ -208,8 +215,16 @@ def main():
     st_other = g.perform_inference(max_iters=a.steps)
     dt_other = time.perf_counter() - t2
     timed = st if a.launch == "timed" else st_other
+    g.close()  # one context at a time: the convergence runs below build their own
+    if ref is not None:
+        ttc = time_to_converged(p, frames, ref, a.legs.split(","))
     n_upd = st["updates_per_iter"]
     ncam = len(frames)
+    counts = {}
+    for _, K, depth in frames:
+        key = f"{depth.shape[1]}x{depth.shape[0]} f={float(K[0][0])}"
+        counts[key] = counts.get(key, 0) + 1
+    shapes = ", ".join(f"{n} x {k}" for k, n in sorted(counts.items()))
     value = n_upd * a.steps / dt
     launches = a.steps * ncam
     avg_launch_ms = timed["t_ray_kernel_ms"] / launches
@@ -248,7 +263,7 @@ def main():
         "dtype": a.msg,
         "data": "synthetic",
         "config": {
-            "workload": f"{a.workload}: {ncam} keyframes {p.cols}x{p.rows} f={float(p.K[0][0])}, "
+            "workload": f"{a.workload}: {ncam} keyframes ({shapes}), "
                         f"{p.res * 100:.0f} cm voxels, {list(map(float, p.dims))} m room, full-ray activation",
             "updates_per_step": n_upd,
             "window_updates_per_step": n_win,
