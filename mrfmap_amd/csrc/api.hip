@@ -100,8 +100,7 @@ struct mrf_ctx {
   size_t cap_tiles = 0;
   // segment-build scratch (per record), per-pair scratch and the emitted lists
   uint32_t *d_keys_in = nullptr, *d_keys_out = nullptr, *d_vals_in = nullptr, *d_vals_out = nullptr;
-  uint32_t *d_heads = nullptr, *d_hscan = nullptr;
-  uint32_t *d_x0 = nullptr, *d_x1 = nullptr, *d_x2 = nullptr, *d_x3 = nullptr;
+  uint32_t *d_x0 = nullptr, *d_x1 = nullptr;  // window head flags and their scan
   size_t cap_sort = 0;
   uint32_t* d_vflag = nullptr;  // 4 pool-sized tables: touched, window-touched and their scans
   size_t cap_vflag = 0;
@@ -316,17 +315,15 @@ int ensure_pool(mrf_ctx* c, size_t nslots) {
 
 int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   if (n > c->cap_sort) {
+    n += n / 4;  // headroom: the next camera is likely a little larger, and a free waits for the device
     dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
-    dfree(&c->d_heads); dfree(&c->d_hscan);
-    dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
+    dfree(&c->d_x0); dfree(&c->d_x1);
     c->cap_sort = 0;
     int rc;
-    // the x arrays also hold small per-tile tables: a little slack for tiny inputs
     const size_t nx = n + 1024;
     if ((rc = dalloc(c, &c->d_keys_in, nx)) || (rc = dalloc(c, &c->d_keys_out, nx)) ||
         (rc = dalloc(c, &c->d_vals_in, nx)) || (rc = dalloc(c, &c->d_vals_out, nx)) ||
-        (rc = dalloc(c, &c->d_heads, nx)) || (rc = dalloc(c, &c->d_hscan, nx)) || (rc = dalloc(c, &c->d_x0, nx)) ||
-        (rc = dalloc(c, &c->d_x1, nx)) || (rc = dalloc(c, &c->d_x2, nx)) || (rc = dalloc(c, &c->d_x3, nx)))
+        (rc = dalloc(c, &c->d_x0, nx)) || (rc = dalloc(c, &c->d_x1, nx)))
       return rc;
     c->cap_sort = n;
   }
@@ -379,6 +376,7 @@ int ensure_vflags(mrf_ctx* c, size_t nvox) {
 
 int ensure_acc(mrf_ctx* c, size_t np) {
   if (np <= c->cap_acc && c->d_acc) return MRF_OK;
+  np += np / 4;
   dfree(&c->d_acc);
   c->cap_acc = 0;
   int rc = dalloc(c, &c->d_acc, 4 * np);
@@ -389,6 +387,7 @@ int ensure_acc(mrf_ctx* c, size_t np) {
 
 int ensure_sell0(mrf_ctx* c, size_t n) {
   if (n <= c->cap_sell0 && c->d_sell0) return MRF_OK;
+  n += n / 4;
   dfree(&c->d_sell0);
   c->cap_sell0 = 0;
   int rc = dalloc(c, &c->d_sell0, n);
@@ -486,7 +485,6 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     return rc;
   launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_has, c->d_sell0, cam.d_front,
               cam.d_lanes, s);
-  dfree(&cam.d_walkn);
   // 2. front/window split (counts into d_counts / d_rayidx), then the front
   //    region (sizes d_has -> offsets d_offs) and the window region (sizes
   //    d_counts -> offsets d_has) of the final record buffer
@@ -584,6 +582,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     launch_zero_words(cam.d_M, (size_t)np * sizeof(float), s);
   HIPCHK(c, hipStreamSynchronize(s));
   HIPCHK(c, hipGetLastError());
+  dfree(&cam.d_walkn);  // after the sync: a free waits for the device
   return MRF_OK;
 }
 
@@ -1163,8 +1162,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_Spair); dfree(&c->d_partial);
   dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
-  dfree(&c->d_heads); dfree(&c->d_hscan);
-  dfree(&c->d_x0); dfree(&c->d_x1); dfree(&c->d_x2); dfree(&c->d_x3);
+  dfree(&c->d_x0); dfree(&c->d_x1);
   dfree(&c->d_vflag); dfree(&c->d_acc); dfree(&c->d_sell0);
   dfree(&c->d_eval); dfree(&c->d_eval_part); dfree(&c->d_eval_counts);
   dfree(&c->d_wpass); dfree(&c->d_wmsum); dfree(&c->d_wmmax); dfree(&c->d_wdelta);
