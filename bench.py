@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=3e-5, help="residual of the reference run")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged run (profiling)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the steps in the other launch mode (profiling: with --warmup 0 and --no-converge "
+                         "the ray kernels of the trace are exactly those of the timed steps)")
     ap.add_argument("--legs", default="plain,warm,warm-f16",
                     help="time-to-converged runs: plain (FP32 BP), warm (learned warm start, FP32), "
                          "warm-f16 (learned warm start, FP16 ray->voxel messages)")
@@ -189,8 +192,10 @@ def main():
     g = build()
     ttc = None
     if a.no_converge:
-        conv = g.perform_inference(max_iters=1)
-        conv["t_total_ms"] = None
+        t0 = time.perf_counter()
+        g.prepare()
+        conv = {"iters": 0, "residual": None, "t_total_ms": None,
+                "t_geometry_ms": (time.perf_counter() - t0) * 1e3}
     else:
         conv = g.perform_inference(max_iters=400, tol=a.tol)
         print(f"[bench] reference: {conv['iters']} iters residual {conv['residual']:.2e} "
@@ -209,11 +214,15 @@ def main():
     assert st["iters"] == a.steps
     # the same steps in the other launch mode, for comparison
     other = "graph" if a.launch == "timed" else "timed"
-    g.set_launch_mode(modes[other])
-    g.perform_inference(max_iters=max(1, a.warmup))
-    t2 = time.perf_counter()
-    st_other = g.perform_inference(max_iters=a.steps)
-    dt_other = time.perf_counter() - t2
+    dt_other = None
+    if not a.no_compare:
+        g.set_launch_mode(modes[other])
+        g.perform_inference(max_iters=max(1, a.warmup))
+        t2 = time.perf_counter()
+        st_other = g.perform_inference(max_iters=a.steps)
+        dt_other = time.perf_counter() - t2
+    elif a.launch != "timed":
+        sys.exit("--no-compare needs --launch timed: the kernel durations come from the timed mode")
     timed = st if a.launch == "timed" else st_other
     g.close()  # one context at a time: the convergence runs below build their own
     if ref is not None:
@@ -230,15 +239,20 @@ def main():
     avg_launch_ms = timed["t_ray_kernel_ms"] / launches
     # ray kernel, algorithmic bytes: every incidence reads its 8 B record and
     # its pair's voxel->ray message; a window incidence also reads its 4 B
-    # segment position and writes its ray->voxel message
+    # segment position and writes its ray->voxel message.  Front records of
+    # chunks whose sums stop changing are read in the first two iterations of
+    # a call only; the average over the K timed launches of a camera counts them so.
     in_b = 4 if msg in (_lib.MSG_F32, _lib.MSG_MIXED_F16, _lib.MSG_MIXED_BF16) else 2
     out_b = 4 if msg == _lib.MSG_F32 else 2
     n_win = st["window_updates_per_iter"]
+    n_static = st["static_front_updates"]
     front_b, window_b = 8 + in_b, 8 + in_b + 4 + out_b
-    bytes_per_launch = ((n_upd - n_win) * front_b + n_win * window_b) / ncam
+    front_reads = (n_upd - n_win - n_static) + n_static * min(a.steps, 2) / a.steps
+    bytes_per_launch = (front_reads * front_b + n_win * window_b) / ncam
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(a)
-    print(f"[bench] {a.steps} steps ({a.launch}) in {dt * 1e3:.1f} ms, ({other}) {dt_other * 1e3:.1f} ms; "
+    other_s = f"{dt_other * 1e3:.1f} ms" if dt_other is not None else "not run"
+    print(f"[bench] {a.steps} steps ({a.launch}) in {dt * 1e3:.1f} ms, ({other}) {other_s}; "
           f"ray kernels {timed['t_ray_kernel_ms']:.1f} ms, epilogue {timed['t_epilogue_ms']:.1f} ms",
           file=sys.stderr, flush=True)
 
@@ -267,13 +281,14 @@ def main():
                         f"{p.res * 100:.0f} cm voxels, {list(map(float, p.dims))} m room, full-ray activation",
             "updates_per_step": n_upd,
             "window_updates_per_step": n_win,
+            "static_front_updates_per_step": n_static,
             "pairs_per_step": st["pairs_per_iter"],
             "rays": st["rays"],
             "list_records": st["list_records"],
             "ray_schedule_records_ahead_front_x_window": list(_lib.RAY_SCHEDULES[st["ray_sched"]]),
             "autotune_ms": st["t_autotune_ms"],
             "launch": a.launch,
-            f"ms_per_step_{other}": dt_other * 1e3 / a.steps,
+            f"ms_per_step_{other}": dt_other * 1e3 / a.steps if dt_other is not None else None,
         },
         "time_to_converged": ttc,
         "reference_run": {"iters": conv["iters"], "residual": conv["residual"], "tol": a.tol,

@@ -47,6 +47,9 @@ struct Camera {
   uint8_t* d_lnn = nullptr;        // per pair: voxel -> ray message ln n (input-side message type)
   uint32_t* d_dyn = nullptr;       // front-only pairs at voxels where some camera has window records
   uint32_t ndyn = 0;
+  float* d_partial = nullptr;      // front partial sums (64 per chunk), kept between iterations
+  uint8_t* d_chunk_static = nullptr;  // per front chunk: partial sums stop changing (see k_chunk_static)
+  uint64_t nstatic = 0;            // live records in such chunks
   uint64_t nrec = 0, base_w = 0;   // padded records; first record of the window region
   uint64_t nupd = 0;               // live incidences
   uint64_t nwin = 0;               // live window incidences (window positions)
@@ -93,8 +96,6 @@ struct mrf_ctx {
   // per-pair / per-tile scratch of a camera pass
   float* d_Spair = nullptr;
   size_t cap_pairs = 0;
-  float* d_partial = nullptr;  // front partial sums (64 per chunk)
-  size_t cap_partial = 0;
   uint32_t *d_pf_q = nullptr, *d_pl_q = nullptr;
   float *d_pf_v = nullptr, *d_pl_v = nullptr, *d_mid = nullptr;
   size_t cap_tiles = 0;
@@ -397,6 +398,8 @@ int ensure_sell0(mrf_ctx* c, size_t n) {
 }
 
 void free_camera_lists(Camera& cam) {
+  dfree(&cam.d_partial);
+  dfree(&cam.d_chunk_static);
   dfree(&cam.d_lnn);
   dfree(&cam.d_dyn);
   cam.ndyn = 0;
@@ -513,7 +516,9 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
   launch_exclusive_scan(c->d_counts, cam.d_chunk_off, (size_t)ntiles + 1, c->d_scan_tmp, c->d_total, s);
   cam.nchunks = (uint32_t)read_total(c);
-  if ((rc = dalloc(c, &cam.d_chunk_tile, cam.nchunks))) return rc;
+  if ((rc = dalloc(c, &cam.d_chunk_tile, cam.nchunks)) || (rc = dalloc(c, &cam.d_partial, (size_t)cam.nchunks * kSlice)) ||
+      (rc = dalloc(c, &cam.d_chunk_static, cam.nchunks)))
+    return rc;
   launch_chunk_tiles(cam.d_chunk_off, ntiles, cam.d_chunk_tile, s);
   // 3. live incidences (records that are not gap records): all, window region
   HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 16, s));
@@ -586,6 +591,8 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   return MRF_OK;
 }
 
+CamPass cam_pass(mrf_ctx* c, size_t ci);
+
 int prepare(mrf_ctx* c) {
   if (!c->geom_dirty) return MRF_OK;
   ++c->epoch;  // camera lists and per-pass scratch are rebuilt below
@@ -646,12 +653,20 @@ int prepare(mrf_ctx* c) {
       cam.ndyn = (uint32_t)ndyn;
       launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
     }
+    for (size_t ci = 0; ci < c->cams.size(); ++ci) {
+      launch_zero_words(c->d_total + 2, 8, c->stream);
+      launch_chunk_static(cam_pass(c, ci), W, c->cams[ci].d_chunk_static, (unsigned long long*)(c->d_total + 2),
+                          c->stream);
+      uint64_t n = 0;
+      HIPCHK(c, hipMemcpyAsync(&n, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      c->cams[ci].nstatic = n;
+    }
     HIPCHK(c, hipGetLastError());
   }
   // 4. per-pass scratch sized for the largest camera
-  size_t xneed = 1, pneed = 1, tneed = 1, cneed = 1;
+  size_t xneed = 1, pneed = 1, tneed = 1;
   for (const Camera& cam : c->cams) {
-    cneed = std::max<size_t>(cneed, (size_t)cam.nchunks * kSlice);
     xneed = std::max<size_t>(xneed, cam.nrec - cam.base_w);  // window region, padding included
     pneed = std::max<size_t>(pneed, cam.npairs);
     tneed = std::max<size_t>(tneed, (cam.nwin + kTile - 1) / kTile);
@@ -661,12 +676,6 @@ int prepare(mrf_ctx* c) {
     c->cap_pairs = 0;
     if ((rc = dalloc(c, &c->d_Spair, pneed))) return rc;
     c->cap_pairs = pneed;
-  }
-  if (cneed > c->cap_partial) {
-    dfree(&c->d_partial);
-    c->cap_partial = 0;
-    if ((rc = dalloc(c, &c->d_partial, cneed))) return rc;
-    c->cap_partial = cneed;
   }
   if (tneed > c->cap_tiles) {
     dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
@@ -732,7 +741,8 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.chunk_off = cam.d_chunk_off;
   a.chunk_tile = cam.d_chunk_tile;
   a.nchunks = cam.nchunks;
-  a.partial = c->d_partial;
+  a.partial = cam.d_partial;
+  a.chunk_static = cam.d_chunk_static;
   a.pair_vox = cam.d_pair_vox;
   a.pair_c = cam.d_pair_c;
   a.pair_sf = cam.d_pair_sf;
@@ -1159,7 +1169,7 @@ void mrf_destroy(mrf_ctx* c) {
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
   if (c->d_Xo) (void)hipFree(c->d_Xo);
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
-  dfree(&c->d_Spair); dfree(&c->d_partial);
+  dfree(&c->d_Spair);
   dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
   dfree(&c->d_x0); dfree(&c->d_x1);
@@ -1268,6 +1278,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   for (const Camera& cam : c->cams) {
     s.updates_per_iter += (int64_t)cam.nupd;
     s.window_updates_per_iter += (int64_t)cam.nwin;
+    s.static_front_updates += (int64_t)cam.nstatic;
     s.pairs_per_iter += cam.npairs;
     s.rays += cam.nrays;
     s.list_records += (int64_t)cam.nlist;

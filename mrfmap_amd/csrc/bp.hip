@@ -280,9 +280,13 @@ __global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__
                                                     const uint32_t* __restrict__ chunk_off, uint32_t nchunks,
                                                     const uint2* __restrict__ inc,
                                                     const typename Msg<MI>::T* __restrict__ Xi,
-                                                    float* __restrict__ partial) {
+                                                    float* __restrict__ partial,
+                                                    const uint8_t* __restrict__ chunk_static,
+                                                    const uint32_t* __restrict__ msg_all) {
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nchunks) return;  // uniform per wavefront
+  // a chunk whose ln n have all stopped changing keeps its partial sums
+  if (chunk_static[w] && *msg_all == 0u) return;
   const uint32_t l = threadIdx.x & 63u;
   const uint32_t t = chunk_tile[w];
   const uint32_t k0 = (w - chunk_off[t]) * kChunk;
@@ -307,6 +311,45 @@ __global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__
       if (kb + u < k1) acc.add(__uint_as_float(e[u].y) * x[u]);
   }
   partial[(size_t)w * kSlice + l] = acc.at(0.0f);
+}
+
+// Per front chunk (the layout of k_front_sums): 1 when every record of every
+// lane belongs to a front-only pair at a voxel that no camera reaches with
+// window records (W), i.e. when its ln n stop changing after the first two
+// iterations of a call; its partial sums can then be kept.
+__global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict__ front,
+                                                      const RayRec* __restrict__ lanes,
+                                                      const uint32_t* __restrict__ len_f,
+                                                      const uint32_t* __restrict__ chunk_tile,
+                                                      const uint32_t* __restrict__ chunk_off, uint32_t nchunks,
+                                                      const uint2* __restrict__ inc,
+                                                      const uint32_t* __restrict__ pair_vox, uint32_t nwp,
+                                                      const uint32_t* __restrict__ W,
+                                                      uint8_t* __restrict__ chunk_static,
+                                                      unsigned long long* __restrict__ nstatic) {
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nchunks) return;  // uniform per wavefront
+  const uint32_t l = threadIdx.x & 63u;
+  const uint32_t t = chunk_tile[w];
+  const uint32_t k0 = (w - chunk_off[t]) * kChunk;
+  const uint32_t ray = t * kSlice + l;
+  const FrontRec f = front[ray];
+  const uint32_t k1 = lanes[ray].nw ? min(k0 + kChunk, f.nf) : 0u;
+  const uint2* __restrict__ rf = inc + f.off;
+  bool dyn = false;
+  uint32_t live = 0;
+  for (uint32_t k = k0; k < k1; ++k) {
+    const uint32_t q = rf[(size_t)k * kSlice].x;
+    if (q == kGapVox) continue;
+    ++live;
+    if (q < nwp || W[pair_vox[q]] != 0u) dyn = true;
+  }
+  const unsigned long long any = __ballot(dyn);
+  if (!any) {  // the records whose sums are kept: counted once per chunk
+    for (int m = 32; m > 0; m >>= 1) live += __shfl_xor(live, m, 64);
+    if (l == 0) atomicAdd(nstatic, (unsigned long long)live);
+  }
+  if (l == 0) chunk_static[w] = any ? 0 : 1;
 }
 
 // Window of every ray: one thread per ray, the 64 rays of a slice in one
@@ -855,7 +898,7 @@ void launch_ray_t(const CamPass& a, hipStream_t s) {
   const typename Msg<MI>::T* Xi = (const typename Msg<MI>::T*)a.apair;
   if (a.nchunks)
     hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.front, a.lanes, a.len_f,
-                       a.chunk_tile, a.chunk_off, a.nchunks, a.inc, Xi, a.partial);
+                       a.chunk_tile, a.chunk_off, a.nchunks, a.inc, Xi, a.partial, a.chunk_static, a.msg_all);
   const uint32_t blocks = (a.nslices * kSlice + 255) / 256;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
   hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
@@ -953,6 +996,13 @@ void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t 
 void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s) {
   if (nslices == 0) return;
   hipLaunchKernelGGL(k_chunk_tiles, dim3(grid_for(nslices, 256)), dim3(256), 0, s, chunk_off, nslices, chunk_tile);
+}
+
+void launch_chunk_static(const CamPass& a, const uint32_t* W, uint8_t* chunk_static, unsigned long long* nstatic,
+                         hipStream_t s) {
+  if (a.nchunks == 0) return;
+  hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.front, a.lanes, a.len_f,
+                     a.chunk_tile, a.chunk_off, a.nchunks, a.inc, a.pair_vox, a.nwp, W, chunk_static, nstatic);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

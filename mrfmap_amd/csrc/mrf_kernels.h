@@ -120,6 +120,7 @@ struct CamPass {
   const uint32_t* dyn;     // front-only pairs at voxels where some camera has window records
   uint32_t ndyn;
   const uint32_t* msg_all; // device countdown: nonzero while every pair's ln n must be recomputed
+  const uint8_t* chunk_static;  // per front chunk: its ln n stop changing with the non-window ones
   float* S;                // per window pair: window segment sum
   uint32_t *pf_q, *pl_q;   // per-tile scratch of the segment sums
   float *pf_v, *pl_v, *mid;
@@ -158,6 +159,10 @@ void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint
                             const uint32_t* idx, float* S, uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v,
                             float* mid, const uint32_t* pair_off, hipStream_t s);
 void launch_ray_messages(const CamPass& a, hipStream_t s);
+// per front chunk of the camera: 1 if all its pairs are front-only pairs at voxels outside W; the
+// records of such chunks are added to *nstatic
+void launch_chunk_static(const CamPass& a, const uint32_t* W, uint8_t* chunk_static, unsigned long long* nstatic,
+                         hipStream_t s);
 void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
 void launch_residual(const float* B, float* Aprev, size_t nvox, unsigned long long* d_max, hipStream_t s);
