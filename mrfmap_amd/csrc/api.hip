@@ -554,7 +554,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.nwp = nwp;
   if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
       (rc = dalloc(c, &cam.d_pair_sf, np)) || (rc = dalloc(c, &cam.d_M, np)) ||
-      (rc = dalloc(c, &cam.d_lnn, (size_t)np * msg_bytes(msg_in_type(c->p.msg_dtype)))) ||
+      (rc = dalloc(c, &cam.d_lnn, (size_t)std::max<uint32_t>(np, 1) * msg_bytes(msg_in_type(c->p.msg_dtype)))) ||
       (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
     return rc;
   launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);

@@ -78,12 +78,11 @@ __device__ __forceinline__ float lane63(float x) {
 // ---------------------------------------------------------------- sensor model
 // sigma(z), bias(z): p0 + p1 z (+ p2 z^2) per 20x20-pixel lookup cell
 // (get_depth_sigma / get_depth_bias, gvdbkernels.cu:46-55).
+// A linear cell has c2 = 0, for which fmaf(z, fmaf(z, 0, c1), c0) is
+// fmaf(z, c1, c0) exactly.
 struct Poly {
   float c0, c1, c2;
-  int n;
-  __device__ __forceinline__ float operator()(float z) const {
-    return n == 3 ? fmaf(z, fmaf(z, c2, c1), c0) : fmaf(z, c1, c0);
-  }
+  __device__ __forceinline__ float operator()(float z) const { return fmaf(z, fmaf(z, c2, c1), c0); }
 };
 
 // psi(s; d) = N(d; s + bias(s), (3 sigma(s))^2) (depth_potential, gvdbkernels.cu:57-72).
@@ -171,7 +170,6 @@ __global__ __launch_bounds__(256) void k_pair_msg_rest(const uint32_t* __restric
 // get_depth_bias, gvdbkernels.cu:46-55).
 __device__ __forceinline__ void load_cell(const float* __restrict__ sigt, const float* __restrict__ biat, int lookup_n,
                                           uint32_t cell, Poly& sig, Poly& bia) {
-  sig.n = bia.n = lookup_n;
   sig.c0 = sigt[cell];
   sig.c1 = sigt[cell + 1];
   sig.c2 = lookup_n == 3 ? sigt[cell + 2] : 0.0f;
@@ -352,6 +350,51 @@ __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict
   if (l == 0) chunk_static[w] = any ? 0 : 1;
 }
 
+// ---------------------------------------------------------------- buffer access
+// Raw buffer loads and stores through a wave-uniform resource (base address,
+// size in bytes): the hardware's range check makes an access at or past the
+// size read 0 or be dropped, and constant offsets fold into the instruction.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kRsrcWord3);
+}
+
+template <int MT>
+struct MsgB;
+template <>
+struct MsgB<kMsgF32> {
+  static constexpr uint32_t kShift = 2;
+  __device__ static float load(Rsrc r, uint32_t off) {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+  }
+  __device__ static void store(Rsrc r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+  }
+};
+template <>
+struct MsgB<kMsgF16> {
+  static constexpr uint32_t kShift = 1;
+  __device__ static float load(Rsrc r, uint32_t off) {
+    return __half2float(__ushort_as_half((unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)));
+  }
+  __device__ static void store(Rsrc r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, (int)off, 0, 0);
+  }
+};
+template <>
+struct MsgB<kMsgBF16> {
+  static constexpr uint32_t kShift = 1;
+  __device__ static float load(Rsrc r, uint32_t off) {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0) << 16);
+  }
+  __device__ static void store(Rsrc r, uint32_t off, float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(u >> 16), r, (int)off, 0, 0);
+  }
+};
+
 // Window of every ray: one thread per ray, the 64 rays of a slice in one
 // wavefront; record k of all lanes of a slice is contiguous.  Lambda starts
 // from the ray's front partial sums, the travelled length from the front's
@@ -361,17 +404,114 @@ __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict
 // sums and writes p = mu1 / (mu1 + mu0 + 1e-7), mu1 = P + vpsi,
 // mu0 = P + R / n (gvdbkernels.cu:936-951) times l into the record's own
 // slot of the window region (coalesced; the segment sums gather them in
-// voxel order).  UW records are loaded before their arithmetic.  The file is built without FP contraction, so
-// pass 2 recomputes pass 1's terms bit for bit.
+// voxel order).  The file is built without FP contraction, so pass 2
+// recomputes pass 1's terms bit for bit.
+//
+// Records are processed in batches of UW, software-pipelined: while a batch
+// is computed, the message gathers of the next batch and the record loads of
+// the one after are in flight (two batches of lengths and messages and one
+// of records in registers, used in turn, so nothing is copied).  Every
+// access goes through a buffer resource: the records and the message slots
+// of the slice's window region, and the pair messages.  Loads past the
+// region read {0, 0} (pair 0, length 0); a gap record's pair index is past
+// the message array, so its message reads 0, and since every pair's
+// ln n <= ln(1 - F_EPS) < 0, a record is live exactly when its message is
+// nonzero.  All lanes run to the slice's longest ray (uniform loop control);
+// slots past a lane's last record are computed and discarded by selects,
+// their stores land in the slice's padding or past the region (dropped),
+// and no kernel reads those slots.
+template <int MI, int UW>
+__device__ __forceinline__ void win_load(Rsrc rr, uint32_t lane_off, uint32_t k0, uint2 (&e)[UW]) {
+#pragma unroll
+  for (int u = 0; u < UW; ++u) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rr, (int)(lane_off + (k0 + (uint32_t)u) * (kSlice * 8u)), 0, 0);
+    e[u] = make_uint2((uint32_t)v[0], (uint32_t)v[1]);
+  }
+}
+// gathers of the batch's messages; the lengths move to their own registers
+template <int MI, int UW>
+__device__ __forceinline__ void win_issue(Rsrc rx, const uint2 (&e)[UW], float (&ln)[UW], float (&x)[UW]) {
+#pragma unroll
+  for (int u = 0; u < UW; ++u) {
+    x[u] = MsgB<MI>::load(rx, e[u].x << MsgB<MI>::kShift);
+    ln[u] = __uint_as_float(e[u].y);
+  }
+}
+
+// Kahan add that keeps the state unchanged when !p
+__device__ __forceinline__ void kahan_add_if(Kahan& k, bool p, float x) {
+  const float y = x - k.c;
+  const float t = k.s + y;
+  const float c = (t - k.s) - y;
+  k.s = p ? t : k.s;
+  k.c = p ? c : k.c;
+}
+
+struct WinRay {
+  float z0, inv, d;
+  Poly sig, bia;
+};
+
+// terms of one record shared by both passes: a = l ln n, vpsi, n
+__device__ __forceinline__ void win_terms(const WinRay& r, const Kahan& kL, const Kahan& kS, float ln, float lnn,
+                                          float& a, float& vpsi, float& nn) {
+  a = ln * lnn;
+  const float s = r.z0 + kS.at(ln) * r.inv;
+  const float ps = depth_psi(s, r.d, r.sig, r.bia);
+  // exp(.) ps is finite, so the product with 0 is exactly 0 (no branch)
+  vpsi = __expf(kL.at(a)) * ps * (lnn != 0.0f ? 1.0f : 0.0f);
+  nn = __expf(lnn);
+}
+
+template <int UW>
+__device__ __forceinline__ void win_pass1(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
+                                          const float (&x)[UW], Kahan& kL, Kahan& kS, Kahan& kT) {
+#pragma unroll
+  for (int u = 0; u < UW; ++u) {
+    const bool valid = k0 + u < nw;
+    float a, vpsi, nn;
+    win_terms(r, kL, kS, ln[u], x[u], a, vpsi, nn);
+    kahan_add_if(kT, valid, vpsi * (1.0f - nn));
+    kahan_add_if(kL, valid, a);
+    kahan_add_if(kS, valid, ln[u]);
+  }
+}
+
+template <int MO, int UW>
+__device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
+                                          const float (&x)[UW], float T, Rsrc ro, uint32_t lane_off, Kahan& kL,
+                                          Kahan& kS, Kahan& kP) {
+  typedef typename Msg<MO>::T TO;
+#pragma unroll
+  for (int u = 0; u < UW; ++u) {
+    const bool valid = k0 + u < nw;
+    float a, vpsi, nn;
+    win_terms(r, kL, kS, ln[u], x[u], a, vpsi, nn);
+    const float tau = vpsi * (1.0f - nn);
+    const float P = kP.at(0.0f);
+    const float R = (T - P) - tau;
+    const float mu1 = P + vpsi;
+    const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
+    float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
+    p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
+    MsgB<MO>::store(ro, lane_off + (k0 + (uint32_t)u) * (kSlice * (uint32_t)sizeof(TO)), p * ln[u] * Msg<MO>::kOut);
+    kahan_add_if(kL, valid, a);
+    kahan_add_if(kS, valid, ln[u]);
+    kahan_add_if(kP, valid, tau);
+  }
+}
+
 template <int MI, int MO, int UW>
 __global__ __launch_bounds__(256) void k_ray_window(const RayRec* __restrict__ lanes, const uint32_t* __restrict__ len_w,
                                                     uint32_t nslices, const uint32_t* __restrict__ chunk_off,
                                                     const float* __restrict__ partial, const uint2* __restrict__ inc,
                                                     uint32_t base_w,
-                                                    const typename Msg<MI>::T* __restrict__ Xi,
+                                                    const typename Msg<MI>::T* __restrict__ Xi, uint32_t npairs,
                                                     typename Msg<MO>::T* __restrict__ Xo,
                                                     const float* __restrict__ sigt, const float* __restrict__ biat,
                                                     int lookup_n) {
+  typedef typename Msg<MI>::T TI;
+  typedef typename Msg<MO>::T TO;
   // XCD-aware order: consecutive blocks (neighbouring pixel tiles) share an XCD's L2
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
@@ -383,87 +523,63 @@ __global__ __launch_bounds__(256) void k_ray_window(const RayRec* __restrict__ l
   const RayRec rec = lanes[ray];
   const uint32_t nw = rec.nw;
   if (nw == 0) return;  // no lane-crossing operations below
-  auto msg = [&](const uint2 e) -> float { return e.x != kGapVox ? Msg<MI>::load(Xi, e.x) : 0.0f; };
 
   Kahan kL = {0.f, 0.f};
   for (uint32_t c = chunk_off[t]; c < chunk_off[t + 1]; ++c) kL.add(partial[(size_t)c * kSlice + l]);
   Kahan kS = {rec.ks, rec.kc};
-  Poly sig, bia;
-  load_cell(sigt, biat, lookup_n, rec.cell_flags & 0x7FFFFFFFu, sig, bia);
-  const float z0 = rec.z0, inv = rec.inv_norm, d = rec.depth;
+  WinRay r;
+  load_cell(sigt, biat, lookup_n, rec.cell_flags & 0x7FFFFFFFu, r.sig, r.bia);
+  r.z0 = rec.z0;
+  r.inv = rec.inv_norm;
+  r.d = rec.depth;
   const Kahan L0 = kL, S0 = kS;
-  const uint2* __restrict__ rw = inc + rec.off_w;
+  // the slice's window region (record k of lane l at sw + k * 64 + l), wave-uniform
+  const uint32_t sw = __builtin_amdgcn_readfirstlane(rec.off_w - l);
+  const uint32_t lws = __builtin_amdgcn_readfirstlane(lw);
+  const Rsrc rr = make_rsrc(inc + sw, lws * kSlice * 8u);
+  const Rsrc rx = make_rsrc(Xi, npairs * (uint32_t)sizeof(TI));
+  const Rsrc ro = make_rsrc(Xo + (sw - base_w), lws * kSlice * (uint32_t)sizeof(TO));
+  const uint32_t lr = l * 8u, lo = l * (uint32_t)sizeof(TO);
+
+  float lA[UW], xA[UW], lB[UW], xB[UW];
+  uint2 e[UW];
+  // pass 1
   Kahan kT = {0.f, 0.f};
-  for (uint32_t k0 = 0; k0 < lw; k0 += UW) {
-    uint2 e[UW];
-    float x[UW];
-#pragma unroll
-    for (int u = 0; u < UW; ++u) {
-      const uint32_t k = k0 + u;
-      e[u] = k < nw ? rw[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
-    }
-#pragma unroll
-    for (int u = 0; u < UW; ++u) x[u] = msg(e[u]);
-#pragma unroll
-    for (int u = 0; u < UW; ++u) {
-      if (k0 + u < nw) {
-        const bool live = e[u].x != kGapVox;
-        const float lnn = x[u];
-        const float ln = __uint_as_float(e[u].y);
-        const float a = ln * lnn;
-        const float s = z0 + kS.at(ln) * inv;
-        const float ps = depth_psi(s, d, sig, bia);
-        const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
-        kT.add(vpsi * (1.0f - __expf(lnn)));
-        kL.add(a);
-        kS.add(ln);
-      }
-    }
+  win_load<MI, UW>(rr, lr, 0, e);
+  win_issue<MI, UW>(rx, e, lA, xA);
+  win_load<MI, UW>(rr, lr, UW, e);
+  for (uint32_t k0 = 0;; k0 += 2 * UW) {  // bounded by the slice's longest ray: uniform control flow
+    win_issue<MI, UW>(rx, e, lB, xB);
+    win_load<MI, UW>(rr, lr, k0 + 2 * UW, e);
+    win_pass1<UW>(r, k0, nw, lA, xA, kL, kS, kT);
+    if (k0 + UW >= lws) break;
+    win_issue<MI, UW>(rx, e, lA, xA);
+    win_load<MI, UW>(rr, lr, k0 + 3 * UW, e);
+    win_pass1<UW>(r, k0 + UW, nw, lB, xB, kL, kS, kT);
+    if (k0 + 2 * UW >= lws) break;
   }
   float T = kT.at(0.0f);
   if (rec.cell_flags & kRanOut) {
-    const float sn = z0 + kS.at(0.0f) * inv;
-    if (d > sn + bia(sn)) T += __expf(kL.at(0.0f));
+    const float sn = r.z0 + kS.at(0.0f) * r.inv;
+    if (r.d > sn + r.bia(sn)) T += __expf(kL.at(0.0f));
   }
 
+  // pass 2
   kL = L0;
   kS = S0;
   Kahan kP = {0.f, 0.f};
-  typename Msg<MO>::T* __restrict__ out = Xo + (rec.off_w - base_w);
-  for (uint32_t k0 = 0; k0 < lw; k0 += UW) {
-    uint2 e[UW];
-    float x[UW];
-#pragma unroll
-    for (int u = 0; u < UW; ++u) {
-      const uint32_t k = k0 + u;
-      e[u] = k < nw ? rw[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
-    }
-#pragma unroll
-    for (int u = 0; u < UW; ++u) x[u] = msg(e[u]);
-#pragma unroll
-    for (int u = 0; u < UW; ++u) {
-      if (k0 + u < nw) {
-        const bool live = e[u].x != kGapVox;
-        const float lnn = x[u];
-        const float ln = __uint_as_float(e[u].y);
-        const float a = ln * lnn;
-        const float s = z0 + kS.at(ln) * inv;
-        const float ps = depth_psi(s, d, sig, bia);
-        const float vpsi = live ? __expf(kL.at(a)) * ps : 0.0f;
-        const float nn = __expf(lnn);
-        const float tau = vpsi * (1.0f - nn);
-        const float P = kP.at(0.0f);
-        const float R = (T - P) - tau;
-        const float mu1 = P + vpsi;
-        const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
-        float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
-        p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
-        if (live) Msg<MO>::store(out, (k0 + u) * kSlice, p * ln * Msg<MO>::kOut);
-        kL.add(a);
-        kS.add(ln);
-        kP.add(tau);
-      }
-    }
+  win_load<MI, UW>(rr, lr, 0, e);
+  win_issue<MI, UW>(rx, e, lA, xA);
+  win_load<MI, UW>(rr, lr, UW, e);
+  for (uint32_t k0 = 0;; k0 += 2 * UW) {
+    win_issue<MI, UW>(rx, e, lB, xB);
+    win_load<MI, UW>(rr, lr, k0 + 2 * UW, e);
+    win_pass2<MO, UW>(r, k0, nw, lA, xA, T, ro, lo, kL, kS, kP);
+    if (k0 + UW >= lws) break;
+    win_issue<MI, UW>(rx, e, lA, xA);
+    win_load<MI, UW>(rr, lr, k0 + 3 * UW, e);
+    win_pass2<MO, UW>(r, k0 + UW, nw, lB, xB, T, ro, lo, kL, kS, kP);
+    if (k0 + 2 * UW >= lws) break;
   }
 }
 
@@ -674,7 +790,7 @@ __device__ __forceinline__ void seg_update_one(uint32_t q, const uint32_t* __res
                                                const float* __restrict__ pair_c, const float* __restrict__ pair_sf,
                                                uint32_t nwp, const float* __restrict__ S, float out_scale,
                                                float* __restrict__ B, float* __restrict__ M,
-                                               uint8_t* __restrict__ upd) {
+                                               uint8_t* __restrict__ upd, bool mark) {
   const float C = pair_c[q];
   if (!(C > kCumLenMin)) return;
   const uint32_t v = pair_vox[q];
@@ -686,17 +802,21 @@ __device__ __forceinline__ void seg_update_one(uint32_t q, const uint32_t* __res
   const float mo = M[q];
   M[q] = m;
   B[v] = (B[v] + m) - mo;
-  upd[v] = 1;
+  if (mark) upd[v] = 1;
 }
 
-// the window pairs [0, nwp)
+// the window pairs [0, nwp).  The update flags (a voxel's belief has been
+// set by a message) are written in the first iteration of a call only
+// (*all_pairs): every later iteration updates the same voxels.
 __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
                                                     const float* __restrict__ pair_c,
                                                     const float* __restrict__ pair_sf, uint32_t nwp,
                                                     const float* __restrict__ S, float out_scale, float* __restrict__ B,
-                                                    float* __restrict__ M, uint8_t* __restrict__ upd) {
+                                                    float* __restrict__ M, uint8_t* __restrict__ upd,
+                                                    const uint32_t* __restrict__ all_pairs) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < nwp) seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, S, out_scale, B, M, upd);
+  const bool mark = *all_pairs != 0u;
+  if (q < nwp) seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, S, out_scale, B, M, upd, mark);
 }
 
 // the front-only pairs [nwp, npairs), only while *all_pairs: a front-only
@@ -711,7 +831,7 @@ __global__ __launch_bounds__(256) void k_seg_update_front(const uint32_t* __rest
                                                           const uint32_t* __restrict__ all_pairs) {
   if (*all_pairs == 0u) return;
   for (uint32_t q = nwp + blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += gridDim.x * blockDim.x)
-    seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, nullptr, 1.0f, B, M, upd);
+    seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, nullptr, 1.0f, B, M, upd, true);
 }
 
 // Beliefs of an external map (distinct voxels): B = the given log-odds,
@@ -902,7 +1022,7 @@ void launch_ray_t(const CamPass& a, hipStream_t s) {
   const uint32_t blocks = (a.nslices * kSlice + 255) / 256;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
   hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
-                     a.partial, a.inc, a.base_w, Xi, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
+                     a.partial, a.inc, a.base_w, Xi, a.npairs, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
 }
 
 template <int MI, int MO>
@@ -966,7 +1086,7 @@ void launch_epilogue(const CamPass& a, hipStream_t s) {
   }
   if (a.nwp)
     hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.nwp, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf,
-                       a.nwp, a.S, out_scale, a.B, a.M, a.upd);
+                       a.nwp, a.S, out_scale, a.B, a.M, a.upd, a.all_pairs);
   if (a.npairs > a.nwp) {
     const unsigned g = std::min<unsigned>(grid_for(a.npairs - a.nwp, 256), 4096u);
     hipLaunchKernelGGL(k_seg_update_front, dim3(g), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf, a.npairs, a.nwp,
