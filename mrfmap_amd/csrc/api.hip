@@ -558,14 +558,14 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
       (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
     return rc;
   launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
-  // 5. per pair: C and the constant front messages
+  // 5. per pair: C and the constant front messages; records point at their pairs from here on
   launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, np, cam.d_pair_c,
                       cam.d_pair_sf, s);
   // 6. window segments: the window records stably sorted by pair
   const int wbits = std::max(1, bit_length(nwp));
   if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits))) return rc;
   if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_idx, nwin))) return rc;
-  HIPCHK(c, sort_window(cam.d_inc + nrec_f, (uint32_t)nrec_w, pid, nwp, wbits, c->d_keys_in, c->d_keys_out,
+  HIPCHK(c, sort_window(cam.d_inc + nrec_f, (uint32_t)nrec_w, nwp, wbits, c->d_keys_in, c->d_keys_out,
                         c->d_vals_in, c->d_vals_out, c->d_sort_tmp, c->sort_tmp_bytes, s));
   uint32_t* whead = c->d_x0;
   uint32_t* hw = c->d_x1;
@@ -575,9 +575,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     if ((rc = flag_scan(c, whead, hw, nwin))) return rc;
   }
   launch_window_offsets(c->d_keys_out, whead, (uint32_t)nwin, nwp, cam.d_wseg_off, s);
-  // 7. records point at their pairs
-  launch_set_pairs(cam.d_inc, nrec, pid, s);
-  // 8. window segment head bitmask and per-tile head counts
+  // 7. window segment head bitmask and per-tile head counts
   const size_t nwords_w = nwin / 64 + 1, ntiles_w = nwin / kTile + 1;
   if ((rc = dalloc(c, &cam.d_heads, nwords_w)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles_w))) return rc;
   launch_pack_flags(whead, hw, (uint32_t)nwin, cam.d_heads, cam.d_tile_h0, s);

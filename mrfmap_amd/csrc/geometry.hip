@@ -19,8 +19,8 @@
 // Window segments.  The window records are stably radix-sorted by pair, so
 // each pair's records stay in record order; the BP iteration sums each
 // segment in that order, gathered through win_idx (no atomics,
-// bit-reproducible).  Finally every live record is rewritten to hold its
-// pair instead of its voxel.
+// bit-reproducible).  The pass that sums C also rewrites every live record
+// to hold its pair instead of its voxel.
 #include <hipcub/hipcub.hpp>
 
 #include "mrf_kernels.h"
@@ -60,59 +60,95 @@ __global__ void k_pair_table(const uint32_t* __restrict__ any, const uint32_t* _
   pair_vox[p] = (uint32_t)v;
 }
 
+// value of lane ^ J; J < 32 stays inside the 32-lane half (ds_swizzle, bitmask mode)
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+  if constexpr (J < 32)
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (J << 10) | 0x1F);
+  else
+    return (uint32_t)__shfl_xor((int)v, J, 64);
+}
+
+// one bitonic step: lanes l and l ^ J order (key, val) ascending inside
+// blocks of K lanes whose index bit K is 0, descending otherwise
+template <int K, int J>
+__device__ __forceinline__ void bitonic_step(int lane, uint32_t& key, uint32_t& val) {
+  const uint32_t ok = xor_lane<J>(key), ov = xor_lane<J>(val);
+  const bool up = (lane & K) == 0;
+  const bool lower = (lane & J) == 0;
+  const bool take = (lower == up) ? (ok < key) : (ok > key);
+  key = take ? ok : key;
+  val = take ? ov : val;
+}
+
+template <int K, int J>
+__device__ __forceinline__ void bitonic_merge(int lane, uint32_t& key, uint32_t& val) {
+  bitonic_step<K, J>(lane, key, val);
+  if constexpr (J > 1) bitonic_merge<K, J / 2>(lane, key, val);
+}
+
+template <int K>
+__device__ __forceinline__ void bitonic_sort(int lane, uint32_t& key, uint32_t& val) {
+  if constexpr (K > 2) bitonic_sort<K / 2>(lane, key, val);
+  bitonic_merge<K, K / 2>(lane, key, val);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, true);
+}
+
+// (x, f) <- (x_left, f_left) (+) (x, f): add the left value unless a run
+// starts at or before this lane
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step_u(uint32_t& x, uint32_t& f) {
+  const uint32_t y = dpp_u<CTRL, ROWS>(x);
+  const uint32_t g = dpp_u<CTRL, ROWS>(f);
+  x = f ? x : x + y;
+  f |= g;
+}
+
+// Inclusive segmented sum over the 64 lanes (all active): row_shr 1/2/4/8
+// inside each row of 16, then row_bcast 15 and 31.
+__device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t f) {
+  seg_step_u<0x111, 0xF>(x, f);
+  seg_step_u<0x112, 0xF>(x, f);
+  seg_step_u<0x114, 0xF>(x, f);
+  seg_step_u<0x118, 0xF>(x, f);
+  seg_step_u<0x142, 0xA>(x, f);
+  seg_step_u<0x143, 0xC>(x, f);
+}
+
 // A wave takes 64 consecutive records -- one step of the 64 rays of a pixel
-// tile, whose voxels repeat across neighbouring rays -- sorts them by pair
-// across its lanes (bitonic network), sums each run of equal pairs
+// tile, whose voxels repeat across neighbouring rays -- rewrites each live
+// record's voxel to its pair, sorts the 64 (pair, fixed-point length) by
+// pair across its lanes (bitonic network), sums each run of equal pairs
 // (segmented scan) and issues one atomic per run instead of one per record.
-__global__ void __launch_bounds__(256) k_pair_lengths(const uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
+// Regions are whole rows of 64 records, so a row is either all front (its
+// lengths also go to the front sum) or all window.
+__global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
                                                       const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_c,
                                                       uint32_t* __restrict__ acc_f) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t base = wave * 64; base < nrec; base += nwaves * 64) {
+  for (uint64_t base = wave * 64; base < nrec; base += nwaves * 64) {  // nrec is a multiple of 64
     const uint64_t i = base + lane;
-    uint32_t key = 0xFFFFFFFFu;
-    uint32_t qc = 0, qf = 0;
-    if (i < nrec) {
-      const uint2 e = rec[i];
-      if (e.x != kGapVox) {
-        key = pid[e.x];
-        qc = (uint32_t)((double)__uint_as_float(e.y) * kFixLen);
-        qf = i < base_w ? qc : 0u;
-      }
+    uint32_t key = 0xFFFFFFFFu, q = 0;
+    const uint2 e = rec[i];
+    if (e.x != kGapVox) {
+      key = pid[e.x];
+      rec[i].x = key;
+      q = (uint32_t)((double)__uint_as_float(e.y) * kFixLen);
     }
-    // bitonic sort of (key, qc, qf) across the 64 lanes, ascending key
-    for (int k = 2; k <= 64; k <<= 1)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const uint32_t ok = __shfl_xor(key, j, 64);
-        const uint32_t oc = __shfl_xor(qc, j, 64), of = __shfl_xor(qf, j, 64);
-        const bool up = (lane & k) == 0;        // this block of k lanes sorts ascending
-        const bool lower = (lane & j) == 0;     // this lane keeps the smaller of the pair when ascending
-        const bool take = (lower == up) ? (ok < key) : (ok > key);
-        if (take) {
-          key = ok;
-          qc = oc;
-          qf = of;
-        }
-      }
-    // segmented inclusive scan over runs of equal keys
-    const uint32_t prev = __shfl_up(key, 1, 64);
-    bool head = lane == 0 || prev != key;
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t oc = __shfl_up(qc, d, 64), of = __shfl_up(qf, d, 64);
-      const bool oh = __shfl_up((int)head, d, 64) != 0;
-      if (lane >= d && !head) {
-        qc += oc;
-        qf += of;
-        head = oh;
-      }
-    }
-    const uint32_t next = __shfl_down(key, 1, 64);
+    bitonic_sort<64>(lane, key, q);
+    const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
+    const uint32_t next = (uint32_t)__shfl_down((int)key, 1, 64);
+    wave_seg_sum_u(q, lane == 0 || prev != key ? 1u : 0u);
     const bool tail = lane == 63 || next != key;
     if (tail && key != 0xFFFFFFFFu) {
-      add_split(acc_c + 2 * (size_t)key, qc);
-      if (qf) add_split(acc_f + 2 * (size_t)key, qf);
+      add_split(acc_c + 2 * (size_t)key, q);
+      if (base < base_w) add_split(acc_f + 2 * (size_t)key, q);
     }
   }
 }
@@ -129,12 +165,12 @@ __global__ void k_pair_finish(const uint32_t* __restrict__ acc_c, const uint32_t
 
 // sort keys of the window region: the record's pair, nwp (sorts last) for
 // gap and padding records; values: the record's index in the region
-__global__ void k_window_keys(const uint2* __restrict__ recw, uint32_t n, const uint32_t* __restrict__ pid,
-                              uint32_t sentinel, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void k_window_keys(const uint2* __restrict__ recw, uint32_t n, uint32_t sentinel,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const uint32_t v = recw[j].x;
-  keys[j] = v == kGapVox ? sentinel : pid[v];
+  const uint32_t q = recw[j].x;
+  keys[j] = q == kGapVox ? sentinel : q;
   vals[j] = j;
 }
 
@@ -149,13 +185,6 @@ __global__ void k_window_offsets(const uint32_t* __restrict__ keys, const uint32
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m < nwin && whead[m]) wseg_off[keys[m]] = m;
   if (m == 0) wseg_off[nwp] = nwin;
-}
-
-__global__ void k_set_pairs(uint2* __restrict__ rec, uint64_t nrec, const uint32_t* __restrict__ pid) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = rec[i].x;
-    if (v != kGapVox) rec[i].x = pid[v];
-  }
 }
 
 __global__ void k_pack_flags(const uint32_t* __restrict__ flags, uint32_t n, uint64_t* __restrict__ bits) {
@@ -208,7 +237,7 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
                      pair_vox);
 }
 
-void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
+void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
                          uint32_t* acc_f, uint32_t np, float* pair_c, float* pair_sf, hipStream_t s) {
   if (np == 0) return;
   launch_zero_words(acc_c, (size_t)np * 8, s);
@@ -217,11 +246,11 @@ void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const
   hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_c, acc_f, np, pair_c, pair_sf);
 }
 
-hipError_t sort_window(const uint2* recw, uint32_t nwr, const uint32_t* pid, uint32_t nwp, int bits, uint32_t* keys_in,
+hipError_t sort_window(const uint2* recw, uint32_t nwr, uint32_t nwp, int bits, uint32_t* keys_in,
                        uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
                        hipStream_t s) {
   if (nwr == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_window_keys, dim3(blocks(nwr, 256)), dim3(256), 0, s, recw, nwr, pid, nwp, keys_in, vals_in);
+  hipLaunchKernelGGL(k_window_keys, dim3(blocks(nwr, 256)), dim3(256), 0, s, recw, nwr, nwp, keys_in, vals_in);
   size_t tb = temp_bytes;
   return hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (int)nwr, 0, bits, s);
 }
@@ -235,11 +264,6 @@ void launch_window_offsets(const uint32_t* keys, const uint32_t* whead, uint32_t
                            uint32_t* wseg_off, hipStream_t s) {
   hipLaunchKernelGGL(k_window_offsets, dim3(blocks(nwin ? nwin : 1, 256)), dim3(256), 0, s, keys, whead, nwin, nwp,
                      wseg_off);
-}
-
-void launch_set_pairs(uint2* rec, uint64_t nrec, const uint32_t* pid, hipStream_t s) {
-  if (nrec == 0) return;
-  hipLaunchKernelGGL(k_set_pairs, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, pid);
 }
 
 void launch_pack_flags(const uint32_t* flags, const uint32_t* fscan, uint32_t n, uint64_t* bits, uint32_t* tile_h0,

@@ -71,21 +71,19 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
                        uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s);
 // per pair: C (summed chord length) and F_EPS x the chord length of its
 // front records (index < base_w), exact fixed-point sums; acc_c / acc_f hold
-// 2 words per pair
-void launch_pair_lengths(const uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
+// 2 words per pair.  Rewrites every live record's voxel to its pair (pid).
+void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
                          uint32_t* acc_f, uint32_t np, float* pair_c, float* pair_sf, hipStream_t s);
 size_t window_sort_temp_bytes(uint32_t n, int bits);
-// window region records stably sorted by pair (gap / padding records last):
-// keys_out = pairs, vals_out = record index in the region
-hipError_t sort_window(const uint2* recw, uint32_t nwr, const uint32_t* pid, uint32_t nwp, int bits, uint32_t* keys_in,
+// window region records (holding pairs) stably sorted by pair (gap /
+// padding records last): keys_out = pairs, vals_out = record index in the region
+hipError_t sort_window(const uint2* recw, uint32_t nwr, uint32_t nwp, int bits, uint32_t* keys_in,
                        uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
                        hipStream_t s);
 void launch_window_heads(const uint32_t* keys, uint32_t nwin, uint32_t* whead, hipStream_t s);
 // wseg_off[pair] = first window position of the pair, wseg_off[nwp] = nwin
 void launch_window_offsets(const uint32_t* keys, const uint32_t* whead, uint32_t nwin, uint32_t nwp,
                            uint32_t* wseg_off, hipStream_t s);
-// live records: voxel -> pair
-void launch_set_pairs(uint2* rec, uint64_t nrec, const uint32_t* pid, hipStream_t s);
 // flag bitmask (bit j = flags[j] != 0) and, per kTile tile, fscan at its start
 void launch_pack_flags(const uint32_t* flags, const uint32_t* fscan, uint32_t n, uint64_t* bits, uint32_t* tile_h0,
                        hipStream_t s);
