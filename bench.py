@@ -45,10 +45,10 @@ def parse():
     ap.add_argument("--legs", default="plain,warm,warm-f16",
                     help="time-to-converged runs: plain (FP32 BP), warm (learned warm start, FP32), "
                          "warm-f16 (learned warm start, FP16 ray->voxel messages)")
-    ap.add_argument("--launch", default="timed", choices=["timed", "graph"],
-                    help="launch mode of the timed steps: kernels one by one with per-kernel HIP events "
-                         "(the roofline's kernel durations come from these steps), or graph replays; "
-                         "the other mode is timed afterwards for comparison")
+    ap.add_argument("--launch", default="graph", choices=["timed", "graph"],
+                    help="launch mode of the timed steps that give value: graph replays (the library's "
+                         "default) or kernels one by one with per-kernel HIP events; the other mode is timed "
+                         "afterwards, and the roofline's kernel durations come from the one-by-one steps")
     return ap.parse_args()
 
 
