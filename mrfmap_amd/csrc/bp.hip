@@ -136,32 +136,23 @@ __device__ __forceinline__ void pair_msg_one(uint32_t q, const uint32_t* __restr
   Msg<MT>::store(apair, q, __logf(nn));
 }
 
-// the window pairs [0, nwp): their beliefs change every iteration
+// The window pairs [0, nwp) change every iteration; of the front-only pairs,
+// all while *msg_all (the first iterations of a call), afterwards only the
+// list dyn (front-only pairs at voxels that some camera reaches with window
+// records).  One launch: the window pairs one per thread, the others
+// strided over the grid.
 template <int MT>
 __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ pair_vox, uint32_t nwp,
+                                                  uint32_t npairs, const uint32_t* __restrict__ dyn, uint32_t ndyn,
                                                   const float* __restrict__ B, const float* __restrict__ M,
-                                                  typename Msg<MT>::T* __restrict__ apair) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < nwp) pair_msg_one<MT>(q, pair_vox, B, M, apair);
-}
-
-// the front-only pairs: all of them while *msg_all (the first two iterations
-// of a call, or after messages were moved), else only those at voxels where
-// some camera has window records (dyn): elsewhere the belief and M_c have
-// stopped changing and the stored ln n is the value a recomputation gives.
-template <int MT>
-__global__ __launch_bounds__(256) void k_pair_msg_rest(const uint32_t* __restrict__ pair_vox, uint32_t npairs,
-                                                       uint32_t nwp, const uint32_t* __restrict__ dyn, uint32_t ndyn,
-                                                       const float* __restrict__ B, const float* __restrict__ M,
-                                                       typename Msg<MT>::T* __restrict__ apair,
-                                                       const uint32_t* __restrict__ msg_all) {
-  const uint32_t stride = gridDim.x * blockDim.x;
+                                                  typename Msg<MT>::T* __restrict__ apair,
+                                                  const uint32_t* __restrict__ msg_all) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (uint32_t q = tid; q < nwp; q += stride) pair_msg_one<MT>(q, pair_vox, B, M, apair);
   if (*msg_all) {
-    for (uint32_t q = nwp + blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += stride)
-      pair_msg_one<MT>(q, pair_vox, B, M, apair);
+    for (uint32_t q = nwp + tid; q < npairs; q += stride) pair_msg_one<MT>(q, pair_vox, B, M, apair);
   } else {
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ndyn; k += stride)
-      pair_msg_one<MT>(dyn[k], pair_vox, B, M, apair);
+    for (uint32_t k = tid; k < ndyn; k += stride) pair_msg_one<MT>(dyn[k], pair_vox, B, M, apair);
   }
 }
 
@@ -805,33 +796,26 @@ __device__ __forceinline__ void seg_update_one(uint32_t q, const uint32_t* __res
   if (mark) upd[v] = 1;
 }
 
-// the window pairs [0, nwp).  The update flags (a voxel's belief has been
-// set by a message) are written in the first iteration of a call only
-// (*all_pairs): every later iteration updates the same voxels.
+// The window pairs [0, nwp), one per thread, and while *all_pairs (the
+// first iteration of a call) the front-only pairs [nwp, npairs), strided
+// over the grid: a front-only pair's m is the same every iteration, and once
+// it is in M the update B = (B + m) - M would add nothing but FP32 rounding.
+// The update flags (a voxel's belief has been set by a message) are written
+// in that first iteration only: every later iteration updates the same
+// voxels.  A camera has one pair per voxel, so no two threads share a B[v].
 __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
                                                     const float* __restrict__ pair_c,
-                                                    const float* __restrict__ pair_sf, uint32_t nwp,
+                                                    const float* __restrict__ pair_sf, uint32_t nwp, uint32_t npairs,
                                                     const float* __restrict__ S, float out_scale, float* __restrict__ B,
                                                     float* __restrict__ M, uint8_t* __restrict__ upd,
                                                     const uint32_t* __restrict__ all_pairs) {
-  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool mark = *all_pairs != 0u;
-  if (q < nwp) seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, S, out_scale, B, M, upd, mark);
-}
-
-// the front-only pairs [nwp, npairs), only while *all_pairs: a front-only
-// pair's m is the same every iteration, and once it is in M the update
-// B = (B + m) - M would add nothing but FP32 rounding.  A small grid that
-// strides over the pairs, so that an idle launch costs a few blocks.
-__global__ __launch_bounds__(256) void k_seg_update_front(const uint32_t* __restrict__ pair_vox,
-                                                          const float* __restrict__ pair_c,
-                                                          const float* __restrict__ pair_sf, uint32_t npairs,
-                                                          uint32_t nwp, float* __restrict__ B, float* __restrict__ M,
-                                                          uint8_t* __restrict__ upd,
-                                                          const uint32_t* __restrict__ all_pairs) {
-  if (*all_pairs == 0u) return;
-  for (uint32_t q = nwp + blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += gridDim.x * blockDim.x)
-    seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, nullptr, 1.0f, B, M, upd, true);
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  const bool first = *all_pairs != 0u;
+  for (uint32_t q = tid; q < nwp; q += stride)
+    seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, S, out_scale, B, M, upd, first);
+  if (first)
+    for (uint32_t q = nwp + tid; q < npairs; q += stride)
+      seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, nullptr, 1.0f, B, M, upd, true);
 }
 
 // Beliefs of an external map (distinct voxels): B = the given log-odds,
@@ -983,20 +967,19 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 }
 
 inline unsigned grid_for(size_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+// grid of the per-pair kernels: a thread per window pair, at most 4096
+// blocks strided over the front-only pairs
+inline unsigned pair_grid(const CamPass& a) {
+  return std::max(1u, std::max(grid_for(a.nwp, 256), std::min<unsigned>(grid_for(a.npairs - a.nwp, 256), 4096u)));
+}
 
 }  // namespace
 
 template <int MT>
 void launch_pair_msgs_t(const CamPass& a, hipStream_t s) {
   typedef typename Msg<MT>::T T;
-  if (a.nwp)
-    hipLaunchKernelGGL(k_pair_msg<MT>, dim3(grid_for(a.nwp, 256)), dim3(256), 0, s, a.pair_vox, a.nwp, a.B, a.M,
-                       (T*)a.apair);
-  if (a.npairs > a.nwp) {
-    const unsigned g = std::min<unsigned>(grid_for(a.npairs - a.nwp, 256), 4096u);
-    hipLaunchKernelGGL(k_pair_msg_rest<MT>, dim3(g), dim3(256), 0, s, a.pair_vox, a.npairs, a.nwp, a.dyn, a.ndyn,
-                       a.B, a.M, (T*)a.apair, a.msg_all);
-  }
+  hipLaunchKernelGGL(k_pair_msg<MT>, dim3(pair_grid(a)), dim3(256), 0, s, a.pair_vox, a.nwp, a.npairs, a.dyn, a.ndyn,
+                     a.B, a.M, (T*)a.apair, a.msg_all);
 }
 
 void launch_pair_msgs(const CamPass& a, hipStream_t s) {
@@ -1084,14 +1067,8 @@ void launch_epilogue(const CamPass& a, hipStream_t s) {
                                a.pl_v, a.mid, a.wseg_off, s);
     }
   }
-  if (a.nwp)
-    hipLaunchKernelGGL(k_seg_update, dim3(grid_for(a.nwp, 256)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf,
-                       a.nwp, a.S, out_scale, a.B, a.M, a.upd, a.all_pairs);
-  if (a.npairs > a.nwp) {
-    const unsigned g = std::min<unsigned>(grid_for(a.npairs - a.nwp, 256), 4096u);
-    hipLaunchKernelGGL(k_seg_update_front, dim3(g), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf, a.npairs, a.nwp,
-                       a.B, a.M, a.upd, a.all_pairs);
-  }
+  hipLaunchKernelGGL(k_seg_update, dim3(pair_grid(a)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf, a.nwp,
+                     a.npairs, a.S, out_scale, a.B, a.M, a.upd, a.all_pairs);
 }
 
 void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
