@@ -96,7 +96,6 @@ struct mrf_ctx {
   // per-pair / per-tile scratch of a camera pass
   float* d_Spair = nullptr;
   size_t cap_pairs = 0;
-  uint32_t *d_pf_q = nullptr, *d_pl_q = nullptr;
   float *d_pf_v = nullptr, *d_pl_v = nullptr, *d_mid = nullptr;
   size_t cap_tiles = 0;
   // segment-build scratch (per record), per-pair scratch and the emitted lists
@@ -676,10 +675,9 @@ int prepare(mrf_ctx* c) {
     c->cap_pairs = pneed;
   }
   if (tneed > c->cap_tiles) {
-    dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
+    dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
     c->cap_tiles = 0;
-    if ((rc = dalloc(c, &c->d_pf_q, tneed)) || (rc = dalloc(c, &c->d_pl_q, tneed)) ||
-        (rc = dalloc(c, &c->d_pf_v, tneed)) || (rc = dalloc(c, &c->d_pl_v, tneed)) || (rc = dalloc(c, &c->d_mid, tneed)))
+    if ((rc = dalloc(c, &c->d_pf_v, tneed)) || (rc = dalloc(c, &c->d_pl_v, tneed)) || (rc = dalloc(c, &c->d_mid, tneed)))
       return rc;
     c->cap_tiles = tneed;
   }
@@ -755,8 +753,6 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.ndyn = cam.ndyn;
   a.msg_all = c->d_all_pairs + 1;
   a.S = c->d_Spair;
-  a.pf_q = c->d_pf_q;
-  a.pl_q = c->d_pl_q;
   a.pf_v = c->d_pf_v;
   a.pl_v = c->d_pl_v;
   a.mid = c->d_mid;
@@ -1168,7 +1164,7 @@ void mrf_destroy(mrf_ctx* c) {
   if (c->d_Xo) (void)hipFree(c->d_Xo);
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
   dfree(&c->d_Spair);
-  dfree(&c->d_pf_q); dfree(&c->d_pl_q); dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
+  dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
   dfree(&c->d_x0); dfree(&c->d_x1);
   dfree(&c->d_vflag); dfree(&c->d_acc); dfree(&c->d_sell0);

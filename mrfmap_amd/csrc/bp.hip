@@ -10,7 +10,7 @@
 //    (gvdbkernels.cu:1196-1208) and the post pass
 //    (rayStartComputingMessagesFromBrickPost, gvdbkernels.cu:824-1038) with a
 //    single launch.
-//  * k_seg_sum / k_seg_span / k_seg_update: per pair, the fixed-order sum S of
+//  * k_seg_sum / k_seg_update: per pair, the fixed-order sum S of
 //    its slots, m = logit(clamp(S/C)), B += m - M_c, M_c = m
 //    (compute_gvdb_message_to_factors, gvdbkernels.cu:396-496, live 457-479).
 //  * residual / marginal / readback kernels (gvdbkernels.cu:469-476, 360-373).
@@ -582,8 +582,7 @@ __global__ __launch_bounds__(256) void k_ray_window(const RayRec* __restrict__ l
 // inclusive scan over the 64 lanes (DPP).  Of the segments that cross tiles,
 // the part that started in an earlier tile goes to pf, the part that
 // continues into the next tile to pl, and a tile lying wholly inside one
-// segment to mid; k_seg_span adds them up.
-constexpr uint32_t kNoPair = 0xFFFFFFFFu;
+// segment to mid; k_seg_update joins them.
 
 template <int CTRL, int ROWS>
 __device__ __forceinline__ int dppi(int x) {
@@ -659,8 +658,8 @@ template <int MT>
 __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ heads, const uint32_t* __restrict__ tile_h0,
                                                  uint32_t nlive, const typename Msg<MT>::T* __restrict__ X,
                                                  const uint32_t* __restrict__ idx,
-                                                 float* __restrict__ S, uint32_t* __restrict__ pf_q,
-                                                 float* __restrict__ pf_v, uint32_t* __restrict__ pl_q,
+                                                 float* __restrict__ S,
+                                                 float* __restrict__ pf_v,
                                                  float* __restrict__ pl_v, float* __restrict__ mid) {
   const int lane = threadIdx.x & 63;
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -723,53 +722,23 @@ __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ he
   const float pfv0 =
       pfmask ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pfv), __builtin_ctzll(pfmask))) : 0.0f;
   if (lane == 0) {
-    uint32_t pq = kNoPair, lq = kNoPair;
-    float pv = 0.0f, lv = 0.0f, mv = 0.0f;
-    if (pfmask) {
-      pq = tile_h0[w] - 1u;
-      pv = pfv0;
-    }
+    float pv = pfmask ? pfv0 : 0.0f, lv = 0.0f, mv = 0.0f;
     if (ends_here) {
       if (Ft)
         S[qt] = Ut;
-      else {
-        pq = tile_h0[w] - 1u;  // no head in the tile: the whole tile ends an earlier segment
-        pv = Ut;
-      }
+      else
+        pv = Ut;  // no head in the tile: the whole tile ends an earlier segment
     } else if (Ft) {
-      lq = qt;
-      lv = Ut;
+      lv = Ut;  // the tile's last segment continues into the next tile
     } else {
       mv = Ut;  // the tile lies inside one segment
     }
-    pf_q[w] = pq;
     pf_v[w] = pv;
-    pl_q[w] = lq;
     pl_v[w] = lv;
     mid[w] = mv;
   }
 }
 
-// Pairs whose segment crosses tiles: S = pl(head tile) + sum of mid over the
-// tiles in between + pf(tail tile); one wavefront per head tile, the middle
-// tiles reduced 64 at a time in a fixed order.
-__global__ __launch_bounds__(256) void k_seg_span(const uint32_t* __restrict__ pair_off, const uint32_t* __restrict__ pl_q,
-                                                  const float* __restrict__ pl_v, const float* __restrict__ pf_v,
-                                                  const float* __restrict__ mid, uint32_t ntiles,
-                                                  float* __restrict__ S) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= ntiles) return;
-  const uint32_t q = pl_q[w];
-  if (q == kNoPair) return;
-  const uint32_t wt = (pair_off[q + 1] - 1u) / kTile;  // tile holding the segment's last position
-  float acc = 0.0f;
-  for (uint32_t b = w + 1; b < wt; b += 64) {
-    const uint32_t t = b + lane;
-    acc += lane63(wave_scan(t < wt ? mid[t] : 0.0f));
-  }
-  if (lane == 0) S[q] = (pl_v[w] + acc) + pf_v[wt];
-}
 
 // Camera message of each (camera, voxel) pair from its message sum:
 // a = clamp(S/C), m = ln a - ln(1-a), B += m - M_c, M_c = m, for C > 0.1
@@ -777,16 +746,28 @@ __global__ __launch_bounds__(256) void k_seg_span(const uint32_t* __restrict__ p
 // S = (sum of the pair's constant front messages) + (its window segment
 // sum); window pairs are numbered first.
 // m = logit(clamp(S / C)), B += m - M_c, M_c = m for pair q
+// Window segment sum of window pair q.  A segment inside one tile of the
+// segment kernel has its sum in S; one that crosses tiles is joined from
+// its parts: the head tile's pl, the tiles inside it (mid), the tail tile's pf.
+__device__ __forceinline__ float window_sum(uint32_t q, const uint32_t* __restrict__ wseg_off,
+                                            const float* __restrict__ S, const float* __restrict__ pl_v,
+                                            const float* __restrict__ pf_v, const float* __restrict__ mid) {
+  const uint32_t w = wseg_off[q] / kTile, wt = (wseg_off[q + 1] - 1u) / kTile;
+  if (w == wt) return S[q];
+  float acc = 0.0f;
+  for (uint32_t t = w + 1; t < wt; ++t) acc += mid[t];
+  return (pl_v[w] + acc) + pf_v[wt];
+}
+
+// Sw: the window segment sum (window pairs) or 0 (front-only pairs)
 __device__ __forceinline__ void seg_update_one(uint32_t q, const uint32_t* __restrict__ pair_vox,
                                                const float* __restrict__ pair_c, const float* __restrict__ pair_sf,
-                                               uint32_t nwp, const float* __restrict__ S, float out_scale,
-                                               float* __restrict__ B, float* __restrict__ M,
+                                               float Sw, float* __restrict__ B, float* __restrict__ M,
                                                uint8_t* __restrict__ upd, bool mark) {
   const float C = pair_c[q];
   if (!(C > kCumLenMin)) return;
   const uint32_t v = pair_vox[q];
-  float Sq = pair_sf[q];
-  if (q < nwp) Sq += S[q] / out_scale;
+  const float Sq = pair_sf[q] + Sw;
   float a = Sq / C;
   a = fminf(fmaxf(a, kFEps), 1.0f - kFEps);
   const float m = __logf(a) - __logf(1.0f - a);
@@ -806,16 +787,18 @@ __device__ __forceinline__ void seg_update_one(uint32_t q, const uint32_t* __res
 __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__ pair_vox,
                                                     const float* __restrict__ pair_c,
                                                     const float* __restrict__ pair_sf, uint32_t nwp, uint32_t npairs,
-                                                    const float* __restrict__ S, float out_scale, float* __restrict__ B,
-                                                    float* __restrict__ M, uint8_t* __restrict__ upd,
-                                                    const uint32_t* __restrict__ all_pairs) {
+                                                    const uint32_t* __restrict__ wseg_off, const float* __restrict__ S,
+                                                    const float* __restrict__ pl_v, const float* __restrict__ pf_v,
+                                                    const float* __restrict__ mid, float out_scale,
+                                                    float* __restrict__ B, float* __restrict__ M,
+                                                    uint8_t* __restrict__ upd, const uint32_t* __restrict__ all_pairs) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const bool first = *all_pairs != 0u;
   for (uint32_t q = tid; q < nwp; q += stride)
-    seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, S, out_scale, B, M, upd, first);
+    seg_update_one(q, pair_vox, pair_c, pair_sf, window_sum(q, wseg_off, S, pl_v, pf_v, mid) / out_scale, B, M, upd,
+                   first);
   if (first)
-    for (uint32_t q = nwp + tid; q < npairs; q += stride)
-      seg_update_one(q, pair_vox, pair_c, pair_sf, nwp, nullptr, 1.0f, B, M, upd, true);
+    for (uint32_t q = nwp + tid; q < npairs; q += stride) seg_update_one(q, pair_vox, pair_c, pair_sf, 0.0f, B, M, upd, true);
 }
 
 // Beliefs of an external map (distinct voxels): B = the given log-odds,
@@ -1031,17 +1014,6 @@ void launch_ray_messages(const CamPass& a, hipStream_t s) {
   }
 }
 
-void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X,
-                            const uint32_t* idx, float* S,
-                            uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v, float* mid,
-                            const uint32_t* pair_off, hipStream_t s) {
-  if (nlive == 0) return;
-  const uint32_t ntiles = (nlive + kTile - 1) / kTile;
-  dim3 grid(grid_for(ntiles, 4)), block(256);
-  hipLaunchKernelGGL(k_seg_sum<kMsgF32>, grid, block, 0, s, heads, tile_h0, nlive, X, idx, S, pf_q, pf_v, pl_q, pl_v,
-                     mid);
-  hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, pair_off, pl_q, pl_v, pf_v, mid, ntiles, S);
-}
 
 void launch_epilogue(const CamPass& a, hipStream_t s) {
   if (a.npairs == 0) return;
@@ -1052,23 +1024,22 @@ void launch_epilogue(const CamPass& a, hipStream_t s) {
     switch (msg_out_type(a.msg_type)) {
       case kMsgF16:
         hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const __half*)a.Xo,
-                           a.win_idx, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
-        hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.wseg_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+                           a.win_idx, a.S, a.pf_v, a.pl_v, a.mid);
         out_scale = Msg<kMsgF16>::kOut;
         break;
       case kMsgBF16:
         hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const uint16_t*)a.Xo,
-                           a.win_idx, a.S, a.pf_q, a.pf_v, a.pl_q, a.pl_v, a.mid);
-        hipLaunchKernelGGL(k_seg_span, grid, block, 0, s, a.wseg_off, a.pl_q, a.pl_v, a.pf_v, a.mid, ntiles, a.S);
+                           a.win_idx, a.S, a.pf_v, a.pl_v, a.mid);
         out_scale = Msg<kMsgBF16>::kOut;
         break;
       default:
-        launch_segment_sum_f32(a.heads, a.tile_h0, a.nwin, (const float*)a.Xo, a.win_idx, a.S, a.pf_q, a.pf_v, a.pl_q,
-                               a.pl_v, a.mid, a.wseg_off, s);
+        hipLaunchKernelGGL(k_seg_sum<kMsgF32>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const float*)a.Xo,
+                           a.win_idx, a.S, a.pf_v, a.pl_v, a.mid);
     }
   }
+  // the segments that cross tiles are joined by k_seg_update
   hipLaunchKernelGGL(k_seg_update, dim3(pair_grid(a)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf, a.nwp,
-                     a.npairs, a.S, out_scale, a.B, a.M, a.upd, a.all_pairs);
+                     a.npairs, a.wseg_off, a.S, a.pl_v, a.pf_v, a.mid, out_scale, a.B, a.M, a.upd, a.all_pairs);
 }
 
 void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,

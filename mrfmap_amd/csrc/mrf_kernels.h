@@ -120,8 +120,7 @@ struct CamPass {
   const uint32_t* msg_all; // device countdown: nonzero while every pair's ln n must be recomputed
   const uint8_t* chunk_static;  // per front chunk: its ln n stop changing with the non-window ones
   float* S;                // per window pair: window segment sum
-  uint32_t *pf_q, *pl_q;   // per-tile scratch of the segment sums
-  float *pf_v, *pl_v, *mid;
+  float *pf_v, *pl_v, *mid;  // per-tile parts of the segments that cross tiles
   float* B;
   float* M;                // this camera's messages M_c (FP32, one per pair)
   void* Xo;                // ray->voxel messages p*l in window-region order (output side type)
@@ -151,11 +150,6 @@ void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* 
 void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t nslices, uint32_t* cnt, hipStream_t s);
 void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s);
 void launch_pair_msgs(const CamPass& a, hipStream_t s);
-// deterministic segment sums of an FP32 position array into S[pair]; with
-// idx, position j holds X[idx[j]]
-void launch_segment_sum_f32(const uint64_t* heads, const uint32_t* tile_h0, uint32_t nlive, const float* X,
-                            const uint32_t* idx, float* S, uint32_t* pf_q, float* pf_v, uint32_t* pl_q, float* pl_v,
-                            float* mid, const uint32_t* pair_off, hipStream_t s);
 void launch_ray_messages(const CamPass& a, hipStream_t s);
 // per front chunk of the camera: 1 if all its pairs are front-only pairs at voxels outside W; the
 // records of such chunks are added to *nstatic
