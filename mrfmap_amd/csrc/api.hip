@@ -48,7 +48,8 @@ struct Camera {
   uint32_t* d_dyn = nullptr;       // front-only pairs at voxels where some camera has window records
   uint32_t ndyn = 0;
   float* d_partial = nullptr;      // front partial sums (64 per chunk), kept between iterations
-  uint8_t* d_chunk_static = nullptr;  // per front chunk: partial sums stop changing (see k_chunk_static)
+  uint32_t* d_dyn_chunks = nullptr;   // front chunks whose partial sums keep changing (see k_chunk_static)
+  uint32_t ndyn_chunks = 0;
   uint64_t nstatic = 0;            // live records in such chunks
   uint64_t nrec = 0, base_w = 0;   // padded records; first record of the window region
   uint64_t nupd = 0;               // live incidences
@@ -398,7 +399,7 @@ int ensure_sell0(mrf_ctx* c, size_t n) {
 
 void free_camera_lists(Camera& cam) {
   dfree(&cam.d_partial);
-  dfree(&cam.d_chunk_static);
+  dfree(&cam.d_dyn_chunks);
   dfree(&cam.d_lnn);
   dfree(&cam.d_dyn);
   cam.ndyn = 0;
@@ -515,8 +516,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
   launch_exclusive_scan(c->d_counts, cam.d_chunk_off, (size_t)ntiles + 1, c->d_scan_tmp, c->d_total, s);
   cam.nchunks = (uint32_t)read_total(c);
-  if ((rc = dalloc(c, &cam.d_chunk_tile, cam.nchunks)) || (rc = dalloc(c, &cam.d_partial, (size_t)cam.nchunks * kSlice)) ||
-      (rc = dalloc(c, &cam.d_chunk_static, cam.nchunks)))
+  if ((rc = dalloc(c, &cam.d_chunk_tile, cam.nchunks)) || (rc = dalloc(c, &cam.d_partial, (size_t)cam.nchunks * kSlice)))
     return rc;
   launch_chunk_tiles(cam.d_chunk_off, ntiles, cam.d_chunk_tile, s);
   // 3. live incidences (records that are not gap records): all, window region
@@ -631,10 +631,12 @@ int prepare(mrf_ctx* c) {
   //     iterations of a call, the others' stops)
   {
     const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
-    if ((rc = ensure_vflags(c, std::max<size_t>(nvox, 1)))) return rc;
+    size_t nflag = std::max<size_t>(nvox, 1);  // flags of the front-only pairs, then of the chunks
+    for (const Camera& cam : c->cams) nflag = std::max<size_t>(nflag, cam.nchunks);
+    if ((rc = ensure_vflags(c, nflag))) return rc;
     uint32_t* W = c->d_vflag;
-    uint32_t* flags = W + nvox;
-    uint32_t* pos = flags + nvox;
+    uint32_t* flags = W + nflag;
+    uint32_t* pos = flags + nflag;
     launch_zero_words(W, nvox * 4, c->stream);
     for (const Camera& cam : c->cams) launch_mark_window_voxels(cam.d_pair_vox, cam.nwp, W, c->stream);
     for (Camera& cam : c->cams) {
@@ -650,14 +652,24 @@ int prepare(mrf_ctx* c) {
       cam.ndyn = (uint32_t)ndyn;
       launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
     }
+    // per camera, the front chunks whose ln n keep changing, as a list
     for (size_t ci = 0; ci < c->cams.size(); ++ci) {
+      Camera& cam = c->cams[ci];
       launch_zero_words(c->d_total + 2, 8, c->stream);
-      launch_chunk_static(cam_pass(c, ci), W, c->cams[ci].d_chunk_static, (unsigned long long*)(c->d_total + 2),
-                          c->stream);
+      launch_chunk_static(cam_pass(c, ci), W, flags, (unsigned long long*)(c->d_total + 2), c->stream);
       uint64_t n = 0;
       HIPCHK(c, hipMemcpyAsync(&n, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      c->cams[ci].nstatic = n;
+      cam.nstatic = n;
+      uint64_t ndc = 0;
+      if (cam.nchunks) {
+        if ((rc = flag_scan(c, flags, pos, cam.nchunks))) return rc;
+        ndc = read_total(c);
+      }
+      dfree(&cam.d_dyn_chunks);
+      if ((rc = dalloc(c, &cam.d_dyn_chunks, std::max<uint64_t>(ndc, 1)))) return rc;
+      cam.ndyn_chunks = (uint32_t)ndc;
+      launch_dyn_compact(flags, pos, cam.nchunks, 0, cam.d_dyn_chunks, c->stream);
     }
     HIPCHK(c, hipGetLastError());
   }
@@ -738,7 +750,8 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.chunk_tile = cam.d_chunk_tile;
   a.nchunks = cam.nchunks;
   a.partial = cam.d_partial;
-  a.chunk_static = cam.d_chunk_static;
+  a.dyn_chunks = cam.d_dyn_chunks;
+  a.ndyn_chunks = cam.ndyn_chunks;
   a.pair_vox = cam.d_pair_vox;
   a.pair_c = cam.d_pair_c;
   a.pair_sf = cam.d_pair_sf;

@@ -257,11 +257,11 @@ __global__ void k_chunk_tiles(const uint32_t* __restrict__ chunk_off, uint32_t n
 // is the visibility sum Lambda = sum l ln n (the reference's visibility
 // includes the voxel itself, gvdbkernels.cu:781-783); their messages are
 // the constant F_EPS * l, summed per pair once per traversal, and their
-// length sum is geometric (RayRec::ks/kc).  One wavefront sums records
-// [c * kChunk, (c + 1) * kChunk) of the 64 rays of a slice, so the work
-// splits into near-equal pieces whatever the ray lengths; each lane writes
-// its Kahan-compensated partial sum.  Rays without window records are
-// skipped.
+// length sum is geometric (RayRec::ks/kc).  Chunk c of a slice is records
+// [c * kChunk, (c + 1) * kChunk) of its 64 rays, one wavefront at a time,
+// so the work splits into near-equal pieces whatever the ray lengths; each
+// lane writes its Kahan-compensated partial sum.  Rays without window
+// records are skipped.
 template <int MI, int U>
 __global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__ front, const RayRec* __restrict__ lanes,
                                                     const uint32_t* __restrict__ len_f,
@@ -270,42 +270,47 @@ __global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__
                                                     const uint2* __restrict__ inc,
                                                     const typename Msg<MI>::T* __restrict__ Xi,
                                                     float* __restrict__ partial,
-                                                    const uint8_t* __restrict__ chunk_static,
+                                                    const uint32_t* __restrict__ dyn_chunks, uint32_t ndyn_chunks,
                                                     const uint32_t* __restrict__ msg_all) {
-  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= nchunks) return;  // uniform per wavefront
-  // a chunk whose ln n have all stopped changing keeps its partial sums
-  if (chunk_static[w] && *msg_all == 0u) return;
+  // every chunk while the ln n of all pairs are recomputed, afterwards only
+  // the chunks with a pair whose ln n still changes (the others keep their
+  // partial sums); waves stride over the chunk list
+  const bool all = *msg_all != 0u;
+  const uint32_t n = all ? nchunks : ndyn_chunks;
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
   const uint32_t l = threadIdx.x & 63u;
-  const uint32_t t = chunk_tile[w];
-  const uint32_t k0 = (w - chunk_off[t]) * kChunk;
-  const uint32_t kmax = min(k0 + kChunk, len_f[t]);
-  const uint32_t ray = t * kSlice + l;
-  const FrontRec f = front[ray];
-  const uint32_t k1 = lanes[ray].nw ? min(k0 + kChunk, f.nf) : 0u;
-  const uint2* __restrict__ rf = inc + f.off;
-  Kahan acc = {0.f, 0.f};
-  for (uint32_t kb = k0; kb < kmax; kb += U) {
-    uint2 e[U];
-    float x[U];
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += nwaves) {  // uniform per wave
+    const uint32_t w = all ? i : dyn_chunks[i];
+    const uint32_t t = chunk_tile[w];
+    const uint32_t k0 = (w - chunk_off[t]) * kChunk;
+    const uint32_t kmax = min(k0 + kChunk, len_f[t]);
+    const uint32_t ray = t * kSlice + l;
+    const FrontRec f = front[ray];
+    const uint32_t k1 = lanes[ray].nw ? min(k0 + kChunk, f.nf) : 0u;
+    const uint2* __restrict__ rf = inc + f.off;
+    Kahan acc = {0.f, 0.f};
+    for (uint32_t kb = k0; kb < kmax; kb += U) {
+      uint2 e[U];
+      float x[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t k = kb + u;
-      e[u] = k < k1 ? rf[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+      for (int u = 0; u < U; ++u) {
+        const uint32_t k = kb + u;
+        e[u] = k < k1 ? rf[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = e[u].x != kGapVox ? Msg<MI>::load(Xi, e[u].x) : 0.0f;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (kb + u < k1) acc.add(__uint_as_float(e[u].y) * x[u]);
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) x[u] = e[u].x != kGapVox ? Msg<MI>::load(Xi, e[u].x) : 0.0f;
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (kb + u < k1) acc.add(__uint_as_float(e[u].y) * x[u]);
+    partial[(size_t)w * kSlice + l] = acc.at(0.0f);
   }
-  partial[(size_t)w * kSlice + l] = acc.at(0.0f);
 }
 
-// Per front chunk (the layout of k_front_sums): 1 when every record of every
+// Per front chunk (the layout of k_front_sums): 0 when every record of every
 // lane belongs to a front-only pair at a voxel that no camera reaches with
 // window records (W), i.e. when its ln n stop changing after the first two
-// iterations of a call; its partial sums can then be kept.
+// iterations of a call and its partial sums can be kept; 1 otherwise.
 __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict__ front,
                                                       const RayRec* __restrict__ lanes,
                                                       const uint32_t* __restrict__ len_f,
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict
                                                       const uint2* __restrict__ inc,
                                                       const uint32_t* __restrict__ pair_vox, uint32_t nwp,
                                                       const uint32_t* __restrict__ W,
-                                                      uint8_t* __restrict__ chunk_static,
+                                                      uint32_t* __restrict__ dyn_flag,
                                                       unsigned long long* __restrict__ nstatic) {
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nchunks) return;  // uniform per wavefront
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict
     for (int m = 32; m > 0; m >>= 1) live += __shfl_xor(live, m, 64);
     if (l == 0) atomicAdd(nstatic, (unsigned long long)live);
   }
-  if (l == 0) chunk_static[w] = any ? 0 : 1;
+  if (l == 0) dyn_flag[w] = any ? 1u : 0u;
 }
 
 // ---------------------------------------------------------------- buffer access
@@ -515,8 +520,17 @@ __global__ __launch_bounds__(256) void k_ray_window(const RayRec* __restrict__ l
   const uint32_t nw = rec.nw;
   if (nw == 0) return;  // no lane-crossing operations below
 
+  // the front's visibility sum: the partial sums of the slice's front chunks, loaded 8 at a time
   Kahan kL = {0.f, 0.f};
-  for (uint32_t c = chunk_off[t]; c < chunk_off[t + 1]; ++c) kL.add(partial[(size_t)c * kSlice + l]);
+  const uint32_t c1 = chunk_off[t + 1];
+  for (uint32_t c0 = chunk_off[t]; c0 < c1; c0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = c0 + i < c1 ? partial[(size_t)(c0 + i) * kSlice + l] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (c0 + i < c1) kL.add(v[i]);
+  }
   Kahan kS = {rec.ks, rec.kc};
   WinRay r;
   load_cell(sigt, biat, lookup_n, rec.cell_flags & 0x7FFFFFFFu, r.sig, r.bia);
@@ -950,6 +964,8 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 }
 
 inline unsigned grid_for(size_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+// blocks of the front kernel: 8 waves on each SIMD of the 256 CUs, striding over the chunks
+constexpr unsigned kFrontBlocks = 2048;
 // grid of the per-pair kernels: a thread per window pair, at most 4096
 // blocks strided over the front-only pairs
 inline unsigned pair_grid(const CamPass& a) {
@@ -983,8 +999,9 @@ template <int MI, int MO, int UF, int UW>
 void launch_ray_t(const CamPass& a, hipStream_t s) {
   const typename Msg<MI>::T* Xi = (const typename Msg<MI>::T*)a.apair;
   if (a.nchunks)
-    hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.front, a.lanes, a.len_f,
-                       a.chunk_tile, a.chunk_off, a.nchunks, a.inc, Xi, a.partial, a.chunk_static, a.msg_all);
+    hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(std::min(grid_for(a.nchunks, 4), kFrontBlocks)), dim3(256), 0, s,
+                       a.front, a.lanes, a.len_f, a.chunk_tile, a.chunk_off, a.nchunks, a.inc, Xi, a.partial,
+                       a.dyn_chunks, a.ndyn_chunks, a.msg_all);
   const uint32_t blocks = (a.nslices * kSlice + 255) / 256;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
   hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
@@ -1066,11 +1083,11 @@ void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* c
   hipLaunchKernelGGL(k_chunk_tiles, dim3(grid_for(nslices, 256)), dim3(256), 0, s, chunk_off, nslices, chunk_tile);
 }
 
-void launch_chunk_static(const CamPass& a, const uint32_t* W, uint8_t* chunk_static, unsigned long long* nstatic,
+void launch_chunk_static(const CamPass& a, const uint32_t* W, uint32_t* dyn_flag, unsigned long long* nstatic,
                          hipStream_t s) {
   if (a.nchunks == 0) return;
   hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.front, a.lanes, a.len_f,
-                     a.chunk_tile, a.chunk_off, a.nchunks, a.inc, a.pair_vox, a.nwp, W, chunk_static, nstatic);
+                     a.chunk_tile, a.chunk_off, a.nchunks, a.inc, a.pair_vox, a.nwp, W, dyn_flag, nstatic);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

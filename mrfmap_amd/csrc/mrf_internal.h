@@ -45,7 +45,7 @@ struct RayRec {
   uint32_t cell_flags; // sensor-model lookup offset (bits 0..30) | kRanOut
   float ks, kc;        // Kahan sum of the front chord lengths (sum, compensation)
 };
-constexpr uint32_t kChunk = 64;  // front records per lane summed by one wavefront of the front kernel
+constexpr uint32_t kChunk = 16;  // front records per lane in one chunk of the front kernel
 
 // A record belongs to the front while d - (s + bias(s)) > kFrontU * 3 sigma(s)
 // at its exit depth s: then psi(s) = 0.398942 exp(-u^2/2) / (3 sigma) has

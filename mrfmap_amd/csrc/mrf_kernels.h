@@ -118,7 +118,8 @@ struct CamPass {
   const uint32_t* dyn;     // front-only pairs at voxels where some camera has window records
   uint32_t ndyn;
   const uint32_t* msg_all; // device countdown: nonzero while every pair's ln n must be recomputed
-  const uint8_t* chunk_static;  // per front chunk: its ln n stop changing with the non-window ones
+  const uint32_t* dyn_chunks;   // front chunks whose ln n keep changing (the others keep their partial sums)
+  uint32_t ndyn_chunks;
   float* S;                // per window pair: window segment sum
   float *pf_v, *pl_v, *mid;  // per-tile parts of the segments that cross tiles
   float* B;
@@ -151,9 +152,9 @@ void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t 
 void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s);
 void launch_pair_msgs(const CamPass& a, hipStream_t s);
 void launch_ray_messages(const CamPass& a, hipStream_t s);
-// per front chunk of the camera: 1 if all its pairs are front-only pairs at voxels outside W; the
-// records of such chunks are added to *nstatic
-void launch_chunk_static(const CamPass& a, const uint32_t* W, uint8_t* chunk_static, unsigned long long* nstatic,
+// per front chunk of the camera: 0 if all its pairs are front-only pairs at voxels outside W
+// (static; its records are added to *nstatic), 1 otherwise
+void launch_chunk_static(const CamPass& a, const uint32_t* W, uint32_t* dyn_flag, unsigned long long* nstatic,
                          hipStream_t s);
 void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
