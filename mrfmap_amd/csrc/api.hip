@@ -425,19 +425,6 @@ void free_camera_lists(Camera& cam) {
   cam.npairs = cam.nwp = 0;
 }
 
-__global__ void k_count_live(const uint2* __restrict__ inc, uint64_t n, unsigned long long* out) {
-  __shared__ unsigned long long sh[256];
-  unsigned long long c = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    c += inc[i].x != kGapVox;
-  sh[threadIdx.x] = c;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(out, sh[0]);
-}
 
 int bit_length(uint64_t v) {
   int b = 0;
@@ -520,22 +507,6 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     return rc;
   launch_chunk_tiles(cam.d_chunk_off, ntiles, cam.d_chunk_tile, s);
   // 3. live incidences (records that are not gap records): all, window region
-  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 16, s));
-  if (nrec) {
-    unsigned g = (unsigned)std::min<uint64_t>((nrec + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, s, cam.d_inc, nrec, (unsigned long long*)(c->d_total + 2));
-  }
-  if (nrec_w) {
-    unsigned g = (unsigned)std::min<uint64_t>((nrec_w + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_count_live, dim3(g), dim3(256), 0, s, cam.d_inc + nrec_f, nrec_w,
-                       (unsigned long long*)(c->d_total + 3));
-  }
-  uint64_t counts2[2] = {0, 0};
-  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 16, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
-  cam.nupd = counts2[0];
-  const uint64_t nwin = counts2[1];
-  cam.nwin = nwin;
   // 4. (camera, voxel) pairs: window pairs first, each group in voxel order
   const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
   if ((rc = ensure_vflags(c, std::max<uint64_t>(nvox, 1)))) return rc;
@@ -544,9 +515,15 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   uint32_t* pid = wany + nvox;  // rank of any, then the pair id
   uint32_t* rank_w = pid + nvox;
   launch_zero_words(any, (size_t)nvox * 8, s);  // any and wany
-  launch_vox_flags(cam.d_inc, nrec, nrec_f, any, wany, s);
+  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 16, s));
+  launch_vox_flags(cam.d_inc, nrec, nrec_f, any, wany, (unsigned long long*)(c->d_total + 2), s);
+  uint64_t counts2[2] = {0, 0};  // live records, live window records
+  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 16, hipMemcpyDeviceToHost, s));
   if ((rc = flag_scan(c, any, pid, nvox))) return rc;
-  const uint32_t np = (uint32_t)read_total(c);
+  const uint32_t np = (uint32_t)read_total(c);  // synchronises: counts2 is in
+  cam.nupd = counts2[0];
+  const uint64_t nwin = counts2[1];
+  cam.nwin = nwin;
   if ((rc = flag_scan(c, wany, rank_w, nvox))) return rc;
   const uint32_t nwp = (uint32_t)read_total(c);
   cam.npairs = np;
@@ -635,11 +612,13 @@ int prepare(mrf_ctx* c) {
     for (const Camera& cam : c->cams) nflag = std::max<size_t>(nflag, cam.nchunks);
     if ((rc = ensure_vflags(c, nflag))) return rc;
     uint32_t* W = c->d_vflag;
-    uint32_t* flags = W + nflag;
+    uint32_t* flags = W + nflag;   // per front-only pair: at a voxel of W
     uint32_t* pos = flags + nflag;
+    uint32_t* cflags = pos + nflag;  // per front chunk: some ln n of it keeps changing
     launch_zero_words(W, nvox * 4, c->stream);
     for (const Camera& cam : c->cams) launch_mark_window_voxels(cam.d_pair_vox, cam.nwp, W, c->stream);
-    for (Camera& cam : c->cams) {
+    for (size_t ci = 0; ci < c->cams.size(); ++ci) {
+      Camera& cam = c->cams[ci];
       const uint32_t nrest = cam.npairs - cam.nwp;
       uint64_t ndyn = 0;
       if (nrest) {
@@ -651,25 +630,22 @@ int prepare(mrf_ctx* c) {
       if ((rc = dalloc(c, &cam.d_dyn, std::max<uint64_t>(ndyn, 1)))) return rc;
       cam.ndyn = (uint32_t)ndyn;
       launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
-    }
-    // per camera, the front chunks whose ln n keep changing, as a list
-    for (size_t ci = 0; ci < c->cams.size(); ++ci) {
-      Camera& cam = c->cams[ci];
+      // the front chunks whose ln n keep changing, as a list
       launch_zero_words(c->d_total + 2, 8, c->stream);
-      launch_chunk_static(cam_pass(c, ci), W, flags, (unsigned long long*)(c->d_total + 2), c->stream);
+      launch_chunk_static(cam_pass(c, ci), flags, cflags, (unsigned long long*)(c->d_total + 2), c->stream);
       uint64_t n = 0;
       HIPCHK(c, hipMemcpyAsync(&n, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
       cam.nstatic = n;
       uint64_t ndc = 0;
       if (cam.nchunks) {
-        if ((rc = flag_scan(c, flags, pos, cam.nchunks))) return rc;
+        if ((rc = flag_scan(c, cflags, pos, cam.nchunks))) return rc;
         ndc = read_total(c);
       }
       dfree(&cam.d_dyn_chunks);
       if ((rc = dalloc(c, &cam.d_dyn_chunks, std::max<uint64_t>(ndc, 1)))) return rc;
       cam.ndyn_chunks = (uint32_t)ndc;
-      launch_dyn_compact(flags, pos, cam.nchunks, 0, cam.d_dyn_chunks, c->stream);
+      launch_dyn_compact(cflags, pos, cam.nchunks, 0, cam.d_dyn_chunks, c->stream);
     }
     HIPCHK(c, hipGetLastError());
   }

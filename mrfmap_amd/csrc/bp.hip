@@ -309,16 +309,16 @@ __global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__
 
 // Per front chunk (the layout of k_front_sums): 0 when every record of every
 // lane belongs to a front-only pair at a voxel that no camera reaches with
-// window records (W), i.e. when its ln n stop changing after the first two
-// iterations of a call and its partial sums can be kept; 1 otherwise.
+// window records (pflag[q - nwp] == 0), i.e. when its ln n stop changing
+// after the first two iterations of a call and its partial sums can be
+// kept; 1 otherwise.
 __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict__ front,
                                                       const RayRec* __restrict__ lanes,
                                                       const uint32_t* __restrict__ len_f,
                                                       const uint32_t* __restrict__ chunk_tile,
                                                       const uint32_t* __restrict__ chunk_off, uint32_t nchunks,
                                                       const uint2* __restrict__ inc,
-                                                      const uint32_t* __restrict__ pair_vox, uint32_t nwp,
-                                                      const uint32_t* __restrict__ W,
+                                                      uint32_t nwp, const uint32_t* __restrict__ pflag,
                                                       uint32_t* __restrict__ dyn_flag,
                                                       unsigned long long* __restrict__ nstatic) {
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict
     const uint32_t q = rf[(size_t)k * kSlice].x;
     if (q == kGapVox) continue;
     ++live;
-    if (q < nwp || W[pair_vox[q]] != 0u) dyn = true;
+    if (q < nwp || pflag[q - nwp] != 0u) dyn = true;
   }
   const unsigned long long any = __ballot(dyn);
   if (!any) {  // the records whose sums are kept: counted once per chunk
@@ -1083,11 +1083,11 @@ void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* c
   hipLaunchKernelGGL(k_chunk_tiles, dim3(grid_for(nslices, 256)), dim3(256), 0, s, chunk_off, nslices, chunk_tile);
 }
 
-void launch_chunk_static(const CamPass& a, const uint32_t* W, uint32_t* dyn_flag, unsigned long long* nstatic,
+void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, unsigned long long* nstatic,
                          hipStream_t s) {
   if (a.nchunks == 0) return;
   hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.front, a.lanes, a.len_f,
-                     a.chunk_tile, a.chunk_off, a.nchunks, a.inc, a.pair_vox, a.nwp, W, dyn_flag, nstatic);
+                     a.chunk_tile, a.chunk_off, a.nchunks, a.inc, a.nwp, pflag, dyn_flag, nstatic);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

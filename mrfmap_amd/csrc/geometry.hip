@@ -37,14 +37,26 @@ __device__ __forceinline__ void add_split(uint32_t* p, uint32_t q) {
   if (old + q < old) atomicAdd(&p[1], 1u);
 }
 
-// any[v] = 1 for every voxel of a live record, wany[v] = 1 for a window record
+// any[v] = 1 for every voxel of a live record, wany[v] = 1 for a window
+// record; counts[0] += live records, counts[1] += live window records
 __global__ void k_vox_flags(const uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w, uint32_t* __restrict__ any,
-                            uint32_t* __restrict__ wany) {
+                            uint32_t* __restrict__ wany, unsigned long long* __restrict__ counts) {
+  // nrec is a multiple of 64 and the rows of 64 lie wholly in one region, so
+  // the loop is uniform per wave and the counts are wave-wide popcounts
+  uint64_t nl = 0, nw = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = rec[i].x;
-    if (v == kGapVox) continue;
+    const bool live = v != kGapVox;
+    const uint64_t n = (uint64_t)__popcll(__ballot(live));
+    nl += n;
+    if (i >= base_w) nw += n;
+    if (!live) continue;
     any[v] = 1u;
     if (i >= base_w) wany[v] = 1u;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (nl) atomicAdd(&counts[0], (unsigned long long)nl);
+    if (nw) atomicAdd(&counts[1], (unsigned long long)nw);
   }
 }
 
@@ -225,9 +237,9 @@ size_t window_sort_temp_bytes(uint32_t n, int bits) {
 }
 
 void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t* any, uint32_t* wany,
-                      hipStream_t s) {
+                      unsigned long long* counts, hipStream_t s) {
   if (nrec == 0) return;
-  hipLaunchKernelGGL(k_vox_flags, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, any, wany);
+  hipLaunchKernelGGL(k_vox_flags, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, any, wany, counts);
 }
 
 void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,

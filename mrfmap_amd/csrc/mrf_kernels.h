@@ -61,9 +61,10 @@ int num_tiled_pixels(const CamGeom& c);
 
 // geometry.hip: (camera, voxel) pairs of a camera's incidences and the
 // window segment layout of the BP iteration.
-// any / wany: pool-sized flags of voxels touched by a live / window record
+// any / wany: pool-sized flags of voxels touched by a live / window record;
+// counts[0] / counts[1] += the live / live window records
 void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t* any, uint32_t* wany,
-                      hipStream_t s);
+                      unsigned long long* counts, hipStream_t s);
 // from the exclusive scans rank_all (of any) and rank_w (of wany): pair id
 // of every touched voxel (window pairs first, each group in voxel order),
 // written over rank_all, and pair_vox[pair]
@@ -152,9 +153,10 @@ void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t 
 void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s);
 void launch_pair_msgs(const CamPass& a, hipStream_t s);
 void launch_ray_messages(const CamPass& a, hipStream_t s);
-// per front chunk of the camera: 0 if all its pairs are front-only pairs at voxels outside W
-// (static; its records are added to *nstatic), 1 otherwise
-void launch_chunk_static(const CamPass& a, const uint32_t* W, uint32_t* dyn_flag, unsigned long long* nstatic,
+// per front chunk of the camera: 0 if all its pairs are front-only pairs whose pflag (front-only
+// pair q at pflag[q - nwp]: at a voxel of W) is 0 (static; its records are added to *nstatic), 1
+// otherwise
+void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, unsigned long long* nstatic,
                          hipStream_t s);
 void launch_epilogue(const CamPass& a, hipStream_t s);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
