@@ -520,7 +520,10 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   uint64_t counts2[2] = {0, 0};  // live records, live window records
   HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 16, hipMemcpyDeviceToHost, s));
   if ((rc = flag_scan(c, any, pid, nvox))) return rc;
-  const uint32_t np = (uint32_t)read_total(c);  // synchronises: counts2 is in
+  const uint64_t np64 = read_total(c);  // synchronises: counts2 is in
+  // the ray kernels address a camera's per-pair messages with 32-bit byte offsets
+  if (np64 >= (1ull << 30)) return fail(c, MRF_E_LIMIT, "more than 2^30 (camera, voxel) pairs in one camera");
+  const uint32_t np = (uint32_t)np64;
   cam.nupd = counts2[0];
   const uint64_t nwin = counts2[1];
   cam.nwin = nwin;
