@@ -48,8 +48,9 @@ struct Camera {
   uint32_t* d_dyn = nullptr;       // front-only pairs at voxels where some camera has window records
   uint32_t ndyn = 0;
   float* d_partial = nullptr;      // front partial sums (64 per chunk), kept between iterations
-  uint32_t* d_dyn_chunks = nullptr;   // front chunks whose partial sums keep changing (see k_chunk_static)
+  uint4* d_dyn_desc = nullptr;        // descriptors of the front chunks whose partial sums keep changing
   uint32_t ndyn_chunks = 0;
+  uint2* d_fspan = nullptr;           // per lane of every slice: first front record, front records to sum
   uint64_t nstatic = 0;            // live records in such chunks
   uint64_t nrec = 0, base_w = 0;   // padded records; first record of the window region
   uint64_t nupd = 0;               // live incidences
@@ -65,7 +66,7 @@ struct Camera {
   uint64_t* d_heads = nullptr;     // window segment head bitmask
   uint32_t* d_tile_h0 = nullptr;   // window heads before each kTile tile
   uint32_t* d_chunk_off = nullptr; // nslices + 1: first front chunk of each slice
-  uint32_t* d_chunk_tile = nullptr;
+  uint4* d_chunk_desc = nullptr;
   uint32_t nchunks = 0;
 };
 
@@ -399,7 +400,8 @@ int ensure_sell0(mrf_ctx* c, size_t n) {
 
 void free_camera_lists(Camera& cam) {
   dfree(&cam.d_partial);
-  dfree(&cam.d_dyn_chunks);
+  dfree(&cam.d_dyn_desc);
+  dfree(&cam.d_fspan);
   dfree(&cam.d_lnn);
   dfree(&cam.d_dyn);
   cam.ndyn = 0;
@@ -417,7 +419,7 @@ void free_camera_lists(Camera& cam) {
   dfree(&cam.d_heads);
   dfree(&cam.d_tile_h0);
   dfree(&cam.d_chunk_off);
-  dfree(&cam.d_chunk_tile);
+  dfree(&cam.d_chunk_desc);
   cam.nchunks = 0;
   cam.nrays = 0;
   cam.nslices = 0;
@@ -503,9 +505,11 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
   launch_exclusive_scan(c->d_counts, cam.d_chunk_off, (size_t)ntiles + 1, c->d_scan_tmp, c->d_total, s);
   cam.nchunks = (uint32_t)read_total(c);
-  if ((rc = dalloc(c, &cam.d_chunk_tile, cam.nchunks)) || (rc = dalloc(c, &cam.d_partial, (size_t)cam.nchunks * kSlice)))
+  if ((rc = dalloc(c, &cam.d_chunk_desc, cam.nchunks)) || (rc = dalloc(c, &cam.d_partial, (size_t)cam.nchunks * kSlice)) ||
+      (rc = dalloc(c, &cam.d_fspan, (size_t)ntiles * kSlice)))
     return rc;
-  launch_chunk_tiles(cam.d_chunk_off, ntiles, cam.d_chunk_tile, s);
+  launch_chunk_desc(cam.d_chunk_off, cam.d_len_f, ntiles, cam.d_chunk_desc, s);
+  launch_front_span(cam.d_front, cam.d_lanes, ntiles * kSlice, cam.d_fspan, s);
   // 3. live incidences (records that are not gap records): all, window region
   // 4. (camera, voxel) pairs: window pairs first, each group in voxel order
   const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
@@ -645,10 +649,11 @@ int prepare(mrf_ctx* c) {
         if ((rc = flag_scan(c, cflags, pos, cam.nchunks))) return rc;
         ndc = read_total(c);
       }
-      dfree(&cam.d_dyn_chunks);
-      if ((rc = dalloc(c, &cam.d_dyn_chunks, std::max<uint64_t>(ndc, 1)))) return rc;
+      dfree(&cam.d_dyn_desc);
+      if ((rc = dalloc(c, &cam.d_dyn_desc, std::max<uint64_t>(ndc, 1)))) return rc;
       cam.ndyn_chunks = (uint32_t)ndc;
-      launch_dyn_compact(cflags, pos, cam.nchunks, 0, cam.d_dyn_chunks, c->stream);
+      launch_dyn_compact(cflags, pos, cam.nchunks, 0, flags, c->stream);  // chunk ids, over the used pair flags
+      launch_gather_desc(cam.d_chunk_desc, flags, (uint32_t)ndc, cam.d_dyn_desc, c->stream);
     }
     HIPCHK(c, hipGetLastError());
   }
@@ -726,10 +731,11 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.win_idx = cam.d_win_idx;
   a.base_w = (uint32_t)cam.base_w;
   a.chunk_off = cam.d_chunk_off;
-  a.chunk_tile = cam.d_chunk_tile;
+  a.chunk_desc = cam.d_chunk_desc;
+  a.fspan = cam.d_fspan;
   a.nchunks = cam.nchunks;
   a.partial = cam.d_partial;
-  a.dyn_chunks = cam.d_dyn_chunks;
+  a.dyn_desc = cam.d_dyn_desc;
   a.ndyn_chunks = cam.ndyn_chunks;
   a.pair_vox = cam.d_pair_vox;
   a.pair_c = cam.d_pair_c;

@@ -246,11 +246,26 @@ __global__ void k_chunk_counts(const uint32_t* __restrict__ len_f, const uint32_
   cnt[t] = (t < nslices && len_w[t]) ? (len_f[t] + kChunk - 1) / kChunk : 0u;
 }
 
-__global__ void k_chunk_tiles(const uint32_t* __restrict__ chunk_off, uint32_t nslices,
-                              uint32_t* __restrict__ chunk_tile) {
+// Chunk descriptors {chunk, slice, first row, end row} and, per lane of every
+// slice, its front span {first record, records to sum: nf, 0 without window}.
+__global__ void k_chunk_desc(const uint32_t* __restrict__ chunk_off, const uint32_t* __restrict__ len_f,
+                             uint32_t nslices, uint4* __restrict__ desc) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nslices) return;
-  for (uint32_t w = chunk_off[t]; w < chunk_off[t + 1]; ++w) chunk_tile[w] = t;
+  for (uint32_t w = chunk_off[t]; w < chunk_off[t + 1]; ++w) {
+    const uint32_t k0 = (w - chunk_off[t]) * kChunk;
+    desc[w] = make_uint4(w, t, k0, min(k0 + kChunk, len_f[t]));
+  }
+}
+__global__ void k_front_span(const FrontRec* __restrict__ front, const RayRec* __restrict__ lanes, uint32_t n,
+                             uint2* __restrict__ span) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) span[r] = make_uint2(front[r].off, lanes[r].nw ? front[r].nf : 0u);
+}
+__global__ void k_gather_desc(const uint4* __restrict__ desc, const uint32_t* __restrict__ idx, uint32_t n,
+                              uint4* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = desc[idx[j]];
 }
 
 // Front sums.  Front records have psi(s) = 0, so all they contribute to a ray
@@ -263,33 +278,30 @@ __global__ void k_chunk_tiles(const uint32_t* __restrict__ chunk_off, uint32_t n
 // lane writes its Kahan-compensated partial sum.  Rays without window
 // records are skipped.
 template <int MI, int U>
-__global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__ front, const RayRec* __restrict__ lanes,
-                                                    const uint32_t* __restrict__ len_f,
-                                                    const uint32_t* __restrict__ chunk_tile,
-                                                    const uint32_t* __restrict__ chunk_off, uint32_t nchunks,
-                                                    const uint2* __restrict__ inc,
+__global__ __launch_bounds__(256) void k_front_sums(const uint2* __restrict__ fspan, const uint4* __restrict__ desc,
+                                                    uint32_t nchunks, const uint4* __restrict__ dyn_desc,
+                                                    uint32_t ndyn_chunks, const uint2* __restrict__ inc,
                                                     const typename Msg<MI>::T* __restrict__ Xi,
                                                     float* __restrict__ partial,
-                                                    const uint32_t* __restrict__ dyn_chunks, uint32_t ndyn_chunks,
                                                     const uint32_t* __restrict__ msg_all) {
   // every chunk while the ln n of all pairs are recomputed, afterwards only
   // the chunks with a pair whose ln n still changes (the others keep their
-  // partial sums); waves stride over the chunk list
+  // partial sums): a wave per entry of the descriptor list (one wave-uniform
+  // load per chunk), the waves past the list's end exit at once.  The grid
+  // covers every chunk; the loop strides for a smaller one.
   const bool all = *msg_all != 0u;
+  const uint4* __restrict__ list = all ? desc : dyn_desc;
   const uint32_t n = all ? nchunks : ndyn_chunks;
   const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
   const uint32_t l = threadIdx.x & 63u;
-  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += nwaves) {  // uniform per wave
-    const uint32_t w = all ? i : dyn_chunks[i];
-    const uint32_t t = chunk_tile[w];
-    const uint32_t k0 = (w - chunk_off[t]) * kChunk;
-    const uint32_t kmax = min(k0 + kChunk, len_f[t]);
-    const uint32_t ray = t * kSlice + l;
-    const FrontRec f = front[ray];
-    const uint32_t k1 = lanes[ray].nw ? min(k0 + kChunk, f.nf) : 0u;
-    const uint2* __restrict__ rf = inc + f.off;
+  for (uint32_t i = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); i < n;
+       i += nwaves) {
+    const uint4 d = list[i];  // {chunk, slice, first row, end row}
+    const uint2 fs = fspan[d.y * kSlice + l];
+    const uint32_t k1 = min(d.z + kChunk, fs.y);
+    const uint2* __restrict__ rf = inc + fs.x;
     Kahan acc = {0.f, 0.f};
-    for (uint32_t kb = k0; kb < kmax; kb += U) {
+    for (uint32_t kb = d.z; kb < d.w; kb += U) {
       uint2 e[U];
       float x[U];
 #pragma unroll
@@ -303,7 +315,7 @@ __global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__
       for (int u = 0; u < U; ++u)
         if (kb + u < k1) acc.add(__uint_as_float(e[u].y) * x[u]);
     }
-    partial[(size_t)w * kSlice + l] = acc.at(0.0f);
+    partial[(size_t)d.x * kSlice + l] = acc.at(0.0f);
   }
 }
 
@@ -312,11 +324,8 @@ __global__ __launch_bounds__(256) void k_front_sums(const FrontRec* __restrict__
 // window records (pflag[q - nwp] == 0), i.e. when its ln n stop changing
 // after the first two iterations of a call and its partial sums can be
 // kept; 1 otherwise.
-__global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict__ front,
-                                                      const RayRec* __restrict__ lanes,
-                                                      const uint32_t* __restrict__ len_f,
-                                                      const uint32_t* __restrict__ chunk_tile,
-                                                      const uint32_t* __restrict__ chunk_off, uint32_t nchunks,
+__global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ fspan,
+                                                      const uint4* __restrict__ desc, uint32_t nchunks,
                                                       const uint2* __restrict__ inc,
                                                       uint32_t nwp, const uint32_t* __restrict__ pflag,
                                                       uint32_t* __restrict__ dyn_flag,
@@ -324,12 +333,10 @@ __global__ __launch_bounds__(256) void k_chunk_static(const FrontRec* __restrict
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nchunks) return;  // uniform per wavefront
   const uint32_t l = threadIdx.x & 63u;
-  const uint32_t t = chunk_tile[w];
-  const uint32_t k0 = (w - chunk_off[t]) * kChunk;
-  const uint32_t ray = t * kSlice + l;
-  const FrontRec f = front[ray];
-  const uint32_t k1 = lanes[ray].nw ? min(k0 + kChunk, f.nf) : 0u;
-  const uint2* __restrict__ rf = inc + f.off;
+  const uint4 d = desc[w];
+  const uint2 fs = fspan[d.y * kSlice + l];
+  const uint32_t k0 = d.z, k1 = min(k0 + kChunk, fs.y);
+  const uint2* __restrict__ rf = inc + fs.x;
   bool dyn = false;
   uint32_t live = 0;
   for (uint32_t k = k0; k < k1; ++k) {
@@ -964,8 +971,6 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 }
 
 inline unsigned grid_for(size_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
-// blocks of the front kernel: 8 waves on each SIMD of the 256 CUs, striding over the chunks
-constexpr unsigned kFrontBlocks = 2048;
 // grid of the per-pair kernels: a thread per window pair, at most 4096
 // blocks strided over the front-only pairs
 inline unsigned pair_grid(const CamPass& a) {
@@ -999,9 +1004,8 @@ template <int MI, int MO, int UF, int UW>
 void launch_ray_t(const CamPass& a, hipStream_t s) {
   const typename Msg<MI>::T* Xi = (const typename Msg<MI>::T*)a.apair;
   if (a.nchunks)
-    hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(std::min(grid_for(a.nchunks, 4), kFrontBlocks)), dim3(256), 0, s,
-                       a.front, a.lanes, a.len_f, a.chunk_tile, a.chunk_off, a.nchunks, a.inc, Xi, a.partial,
-                       a.dyn_chunks, a.ndyn_chunks, a.msg_all);
+    hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s,
+                       a.fspan, a.chunk_desc, a.nchunks, a.dyn_desc, a.ndyn_chunks, a.inc, Xi, a.partial, a.msg_all);
   const uint32_t blocks = (a.nslices * kSlice + 255) / 256;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
   hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
@@ -1078,16 +1082,22 @@ void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t 
   hipLaunchKernelGGL(k_chunk_counts, dim3(grid_for(nslices + 1, 256)), dim3(256), 0, s, len_f, len_w, nslices, cnt);
 }
 
-void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s) {
+void launch_chunk_desc(const uint32_t* chunk_off, const uint32_t* len_f, uint32_t nslices, uint4* desc, hipStream_t s) {
   if (nslices == 0) return;
-  hipLaunchKernelGGL(k_chunk_tiles, dim3(grid_for(nslices, 256)), dim3(256), 0, s, chunk_off, nslices, chunk_tile);
+  hipLaunchKernelGGL(k_chunk_desc, dim3(grid_for(nslices, 256)), dim3(256), 0, s, chunk_off, len_f, nslices, desc);
+}
+void launch_front_span(const FrontRec* front, const RayRec* lanes, uint32_t n, uint2* span, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_front_span, dim3(grid_for(n, 256)), dim3(256), 0, s, front, lanes, n, span);
+}
+void launch_gather_desc(const uint4* desc, const uint32_t* idx, uint32_t n, uint4* out, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_gather_desc, dim3(grid_for(n, 256)), dim3(256), 0, s, desc, idx, n, out);
 }
 
 void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, unsigned long long* nstatic,
                          hipStream_t s) {
   if (a.nchunks == 0) return;
-  hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.front, a.lanes, a.len_f,
-                     a.chunk_tile, a.chunk_off, a.nchunks, a.inc, a.nwp, pflag, dyn_flag, nstatic);
+  hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.fspan, a.chunk_desc, a.nchunks,
+                     a.inc, a.nwp, pflag, dyn_flag, nstatic);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

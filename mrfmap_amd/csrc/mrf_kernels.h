@@ -103,7 +103,8 @@ struct CamPass {
   const uint32_t* win_idx; // per window position (voxel order): its record's slot in the window region
   uint32_t base_w;         // first record of the window region
   const uint32_t* chunk_off;   // nslices + 1: first front chunk of each slice
-  const uint32_t* chunk_tile;  // slice of each front chunk
+  const uint4* chunk_desc;     // per front chunk: {chunk, slice, first row, end row}
+  const uint2* fspan;          // per lane of every slice: {first front record, front records to sum}
   uint32_t nchunks;
   float* partial;          // per chunk and lane: front visibility partial sum
   const uint32_t* pair_vox;
@@ -119,7 +120,7 @@ struct CamPass {
   const uint32_t* dyn;     // front-only pairs at voxels where some camera has window records
   uint32_t ndyn;
   const uint32_t* msg_all; // device countdown: nonzero while every pair's ln n must be recomputed
-  const uint32_t* dyn_chunks;   // front chunks whose ln n keep changing (the others keep their partial sums)
+  const uint4* dyn_desc;        // descriptors of the front chunks whose ln n keep changing
   uint32_t ndyn_chunks;
   float* S;                // per window pair: window segment sum
   float *pf_v, *pl_v, *mid;  // per-tile parts of the segments that cross tiles
@@ -147,10 +148,13 @@ void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* se
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
                     const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
                     hipStream_t s);
-// front chunks per slice (cnt has nslices + 1 entries, the last 0), and the
-// slice of every chunk from their offsets
+// front chunks per slice (cnt has nslices + 1 entries, the last 0), the
+// descriptor of every chunk from their offsets, and the per-lane front spans
 void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t nslices, uint32_t* cnt, hipStream_t s);
-void launch_chunk_tiles(const uint32_t* chunk_off, uint32_t nslices, uint32_t* chunk_tile, hipStream_t s);
+void launch_chunk_desc(const uint32_t* chunk_off, const uint32_t* len_f, uint32_t nslices, uint4* desc, hipStream_t s);
+void launch_front_span(const FrontRec* front, const RayRec* lanes, uint32_t n, uint2* span, hipStream_t s);
+// out[j] = desc[idx[j]]
+void launch_gather_desc(const uint4* desc, const uint32_t* idx, uint32_t n, uint4* out, hipStream_t s);
 void launch_pair_msgs(const CamPass& a, hipStream_t s);
 void launch_ray_messages(const CamPass& a, hipStream_t s);
 // per front chunk of the camera: 0 if all its pairs are front-only pairs whose pflag (front-only
