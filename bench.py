@@ -282,6 +282,8 @@ def main():
             "updates_per_step": n_upd,
             "window_updates_per_step": n_win,
             "static_front_updates_per_step": n_static,
+            "front_slots": st["front_slots"],
+            "window_slots": st["window_slots"],
             "pairs_per_step": st["pairs_per_iter"],
             "rays": st["rays"],
             "list_records": st["list_records"],

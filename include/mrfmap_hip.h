@@ -117,6 +117,9 @@ typedef struct mrf_stats {
                                    iterations of a call (every pair of their chunk is a front-only pair
                                    at a voxel no camera reaches with window records); from the third
                                    iteration on their stored sums are used instead of re-reading them */
+  int64_t front_slots;         /* record slots of the sliced front regions (each slice padded to its
+                                  longest ray), summed over the cameras */
+  int64_t window_slots;        /* the same for the window regions */
 } mrf_stats;
 
 /* gvdb_params_t() defaults (GVDBParams.h:32-55). */

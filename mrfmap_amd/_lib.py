@@ -55,6 +55,7 @@ class MrfStats(ctypes.Structure):
         ("window_updates_per_iter", ctypes.c_int64), ("graph", ctypes.c_int), ("warm", ctypes.c_int),
         ("warm_voxels", ctypes.c_int64), ("t_warm_ms", ctypes.c_double), ("ref_error", ctypes.c_double),
         ("static_front_updates", ctypes.c_int64),
+        ("front_slots", ctypes.c_int64), ("window_slots", ctypes.c_int64),
     ]
 
     def as_dict(self):

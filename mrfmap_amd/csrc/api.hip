@@ -1271,6 +1271,8 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
     s.updates_per_iter += (int64_t)cam.nupd;
     s.window_updates_per_iter += (int64_t)cam.nwin;
     s.static_front_updates += (int64_t)cam.nstatic;
+    s.front_slots += (int64_t)cam.base_w;
+    s.window_slots += (int64_t)(cam.nrec - cam.base_w);
     s.pairs_per_iter += cam.npairs;
     s.rays += cam.nrays;
     s.list_records += (int64_t)cam.nlist;

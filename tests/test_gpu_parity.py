@@ -96,6 +96,9 @@ def test_bp_marginals_real_shape():
     assert st["updates_per_iter"] == so["updates_per_iter"]
     # the third iteration uses the kept sums of the static front chunks
     assert 0 < st["static_front_updates"] <= st["updates_per_iter"] - st["window_updates_per_iter"]
+    # padded slice regions hold every record
+    assert st["window_slots"] >= st["window_updates_per_iter"]
+    assert st["front_slots"] + st["window_slots"] >= st["list_records"]
     err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
     print(f"real-shape max-abs marginal error {err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e}")
     assert err.max() <= MARG_TOL
