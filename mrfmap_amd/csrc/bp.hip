@@ -505,7 +505,7 @@ __device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t
 }
 
 template <int MI, int MO, int UW>
-__global__ __launch_bounds__(256) void k_ray_window(const RayRec* __restrict__ lanes, const uint32_t* __restrict__ len_w,
+__global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ lanes, const uint32_t* __restrict__ len_w,
                                                     uint32_t nslices, const uint32_t* __restrict__ chunk_off,
                                                     const float* __restrict__ partial, const uint2* __restrict__ inc,
                                                     uint32_t base_w,
@@ -515,10 +515,12 @@ __global__ __launch_bounds__(256) void k_ray_window(const RayRec* __restrict__ l
                                                     int lookup_n) {
   typedef typename Msg<MI>::T TI;
   typedef typename Msg<MO>::T TO;
-  // XCD-aware order: consecutive blocks (neighbouring pixel tiles) share an XCD's L2
+  // one wave (one slice) per block, so that a finished wave frees its slot at
+  // once; XCD-aware order: consecutive blocks (neighbouring pixel tiles) share
+  // an XCD's L2
   const uint32_t per = gridDim.x >> 3;
   const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  const uint32_t ray = lb * 256u + threadIdx.x;  // slice * 64 + lane
+  const uint32_t ray = lb * 64u + threadIdx.x;  // slice * 64 + lane
   const uint32_t t = ray >> 6, l = ray & 63u;
   if (t >= nslices) return;
   const uint32_t lw = len_w[t];
@@ -1006,9 +1008,9 @@ void launch_ray_t(const CamPass& a, hipStream_t s) {
   if (a.nchunks)
     hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s,
                        a.fspan, a.chunk_desc, a.nchunks, a.dyn_desc, a.ndyn_chunks, a.inc, Xi, a.partial, a.msg_all);
-  const uint32_t blocks = (a.nslices * kSlice + 255) / 256;
+  const uint32_t blocks = (a.nslices * kSlice + 63) / 64;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
-  hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(256), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
+  hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(64), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
                      a.partial, a.inc, a.base_w, Xi, a.npairs, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
 }
 
