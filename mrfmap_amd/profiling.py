@@ -4,6 +4,10 @@
         One plain device-to-device copy of 1 GiB (torch), the reference
         workload for calibrating the FETCH_SIZE / WRITE_SIZE counters: an
         ideal copy fetches and writes exactly 1 GiB.
+    python -m mrfmap_amd.profiling stream [--gib N] [--reps R]
+        Achievable HBM bandwidth of plain torch kernels on N GiB buffers:
+        copy and add (read and write N GiB), fill (writes N GiB) and sum
+        (reads N GiB), best of R timed repetitions (HIP events), in GB/s.
     python -m mrfmap_amd.profiling summarize DIR [--calib CALDIR] [--kernels A,B]
         Markdown summary of rocprofv3 runs kept under DIR: trace/ (from
         --kernel-trace --stats), fetch/ and write/ (from --pmc FETCH_SIZE and
@@ -139,10 +143,37 @@ def copy_once():
     print(f"copied {n * 4} bytes")
 
 
+def stream(gib=4, reps=10):
+    import torch
+
+    n = gib * GIB // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    out = {}
+    cases = (("copy", lambda: dst.copy_(src), 2 * n * 4), ("add", lambda: torch.add(src, 1.0, out=dst), 2 * n * 4),
+             ("fill", lambda: dst.fill_(2.0), n * 4), ("sum", lambda: src.sum(), n * 4))
+    for name, fn, nbytes in cases:
+        fn()
+        best = None
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        out[name] = {"ms": best, "GB/s": nbytes / (best * 1e-3) / 1e9}
+    print(json.dumps({"buffer_gib": gib, "reps": reps, **out}))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     sub.add_parser("copy")
+    st = sub.add_parser("stream")
+    st.add_argument("--gib", type=int, default=4)
+    st.add_argument("--reps", type=int, default=10)
     s = sub.add_parser("summarize")
     s.add_argument("dir")
     s.add_argument("--calib")
@@ -151,6 +182,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_once()
+        return 0
+    if a.cmd == "stream":
+        stream(a.gib, a.reps)
         return 0
     text, traffic = summarize(a.dir, a.calib, tuple(a.kernels.split(",")))
     print(text)
