@@ -423,7 +423,8 @@ struct MsgB<kMsgBF16> {
 // slots past a lane's last record are computed and discarded by selects,
 // their stores land in the slice's padding or past the region (dropped),
 // and no kernel reads those slots.
-template <int MI, int UW>
+// UW records of a lane from row k0 on (record k at byte lane_off + k * 64 * 8)
+template <int UW>
 __device__ __forceinline__ void win_load(Rsrc rr, uint32_t lane_off, uint32_t k0, uint2 (&e)[UW]) {
 #pragma unroll
   for (int u = 0; u < UW; ++u) {
@@ -559,16 +560,16 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   uint2 e[UW];
   // pass 1
   Kahan kT = {0.f, 0.f};
-  win_load<MI, UW>(rr, lr, 0, e);
+  win_load<UW>(rr, lr, 0, e);
   win_issue<MI, UW>(rx, e, lA, xA);
-  win_load<MI, UW>(rr, lr, UW, e);
+  win_load<UW>(rr, lr, UW, e);
   for (uint32_t k0 = 0;; k0 += 2 * UW) {  // bounded by the slice's longest ray: uniform control flow
     win_issue<MI, UW>(rx, e, lB, xB);
-    win_load<MI, UW>(rr, lr, k0 + 2 * UW, e);
+    win_load<UW>(rr, lr, k0 + 2 * UW, e);
     win_pass1<UW>(r, k0, nw, lA, xA, kL, kS, kT);
     if (k0 + UW >= lws) break;
     win_issue<MI, UW>(rx, e, lA, xA);
-    win_load<MI, UW>(rr, lr, k0 + 3 * UW, e);
+    win_load<UW>(rr, lr, k0 + 3 * UW, e);
     win_pass1<UW>(r, k0 + UW, nw, lB, xB, kL, kS, kT);
     if (k0 + 2 * UW >= lws) break;
   }
@@ -582,16 +583,16 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   kL = L0;
   kS = S0;
   Kahan kP = {0.f, 0.f};
-  win_load<MI, UW>(rr, lr, 0, e);
+  win_load<UW>(rr, lr, 0, e);
   win_issue<MI, UW>(rx, e, lA, xA);
-  win_load<MI, UW>(rr, lr, UW, e);
+  win_load<UW>(rr, lr, UW, e);
   for (uint32_t k0 = 0;; k0 += 2 * UW) {
     win_issue<MI, UW>(rx, e, lB, xB);
-    win_load<MI, UW>(rr, lr, k0 + 2 * UW, e);
+    win_load<UW>(rr, lr, k0 + 2 * UW, e);
     win_pass2<MO, UW>(r, k0, nw, lA, xA, T, ro, lo, kL, kS, kP);
     if (k0 + UW >= lws) break;
     win_issue<MI, UW>(rx, e, lA, xA);
-    win_load<MI, UW>(rr, lr, k0 + 3 * UW, e);
+    win_load<UW>(rr, lr, k0 + 3 * UW, e);
     win_pass2<MO, UW>(r, k0 + UW, nw, lB, xB, T, ro, lo, kL, kS, kP);
     if (k0 + 2 * UW >= lws) break;
   }
