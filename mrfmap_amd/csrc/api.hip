@@ -1657,7 +1657,7 @@ int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_recor
       std::memcpy(&o[2], &L.z0, 4);
       std::memcpy(&o[3], &L.inv_norm, 4);
       std::memcpy(&o[4], &L.depth, 4);
-      o[5] = L.cell_flags;
+      o[5] = L.cell_flags & ~kInfTerm;  // lookup offset | kRanOut, as the oracle lists them
     }
     if (records)
       for (uint32_t k = 0; k < n; ++k) {

@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_split(FrontRec* __restrict__ front, Ray
   Kahan kS = {0.f, 0.f};
   if (n) {
     Poly sig, bia;
-    load_cell(sigt, biat, lookup_n, r.cell_flags & 0x7FFFFFFFu, sig, bia);
+    load_cell(sigt, biat, lookup_n, r.cell_flags & kCellMask, sig, bia);
     const uint2* __restrict__ rl = sell + f.off;
     for (uint32_t k = 0; k < n; ++k) {
       const float ln = __uint_as_float(rl[(size_t)k * kSlice].y);
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   }
   Kahan kS = {rec.ks, rec.kc};
   WinRay r;
-  load_cell(sigt, biat, lookup_n, rec.cell_flags & 0x7FFFFFFFu, r.sig, r.bia);
+  load_cell(sigt, biat, lookup_n, rec.cell_flags & kCellMask, r.sig, r.bia);
   r.z0 = rec.z0;
   r.inv = rec.inv_norm;
   r.d = rec.depth;
@@ -574,10 +574,7 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
     if (k0 + 2 * UW >= lws) break;
   }
   float T = kT.at(0.0f);
-  if (rec.cell_flags & kRanOut) {
-    const float sn = r.z0 + kS.at(0.0f) * r.inv;
-    if (r.d > sn + r.bia(sn)) T += __expf(kL.at(0.0f));
-  }
+  if (rec.cell_flags & kInfTerm) T += __expf(kL.at(0.0f));  // decided at traversal
 
   // pass 2
   kL = L0;

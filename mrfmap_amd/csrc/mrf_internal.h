@@ -42,7 +42,7 @@ struct RayRec {
   float z0;            // planar depth at the entry of the first record (m)
   float inv_norm;      // metres of planar depth per voxel of chord length
   float depth;         // measured depth d (m)
-  uint32_t cell_flags; // sensor-model lookup offset (bits 0..30) | kRanOut
+  uint32_t cell_flags; // sensor-model lookup offset (bits 0..29) | kInfTerm | kRanOut
   float ks, kc;        // Kahan sum of the front chord lengths (sum, compensation)
 };
 constexpr uint32_t kChunk = 16;  // front records per lane in one chunk of the front kernel
@@ -54,6 +54,10 @@ constexpr uint32_t kChunk = 16;  // front records per lane in one chunk of the f
 // clamp floor F_EPS * l).
 constexpr float kFrontU = 15.0f;
 constexpr uint32_t kRanOut = 0x80000000u;  // ray left the active space before truncation
+// the ray ran out and its depth lies beyond the exit: the infinity term applies
+// (gvdbkernels.cu:1196-1208), decided in FP64 from the exact exit depth
+constexpr uint32_t kInfTerm = 0x40000000u;
+constexpr uint32_t kCellMask = 0x3FFFFFFFu;
 
 // Camera geometry in the form the kernels consume (all double so that the
 // host oracle and the device agree bit for bit on the fixed-point setup).

@@ -108,6 +108,7 @@ struct WalkOut {
   uint32_t n;
   float z0;
   bool ran_out;
+  double last_exit;  // exact exit depth of the last voxel record (kEmit, kVisit)
 };
 
 // Records of one ray are written out[0], out[kSlice], out[2 kSlice], ...
@@ -263,6 +264,7 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
     if (brick_change) bslot = brick_lookup();
   }
   wo.n = cnt;
+  wo.last_exit = last_exit;
   return wo;
 }
 
@@ -419,7 +421,11 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
     r.z0 = wo.z0;
     r.inv_norm = (float)(1.0 / p.rs.norm);
     r.depth = p.d;
-    r.cell_flags = p.cell | (wo.ran_out ? kRanOut : 0u);
+    // the infinity term's test d > s_n + bias(s_n) at the exact exit depth of
+    // the last record, in FP64 as the traversal itself: an FP32 running sum
+    // of chord lengths can land on either side when d is within an ulp
+    const bool inf = wo.ran_out && (double)p.d > wo.last_exit + poly_eval(bia + p.cell, g.lookup_n, wo.last_exit);
+    r.cell_flags = p.cell | (wo.ran_out ? kRanOut : 0u) | (inf ? kInfTerm : 0u);
   }
   // pad the lane to the slice length with no-op records
   const uint32_t len = slice_len[tile];

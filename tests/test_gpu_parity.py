@@ -123,6 +123,23 @@ def test_bp_marginals_small(activation):
     assert abs(st["residual"] - so["residual"]) < 1e-3
 
 
+@pytest.mark.parametrize("seed", [3, 10, 17])
+def test_infinity_term_decided_as_oracle(seed):
+    """Rays that ran out take the infinity term only when d > s_n + bias(s_n)
+    (gvdbkernels.cu:1196-1208). Seed 10 holds a ray whose depth lies 1.3e-7
+    beyond s_n: an FP32 s_n puts it on the other side and its voxels' first
+    marginals move by ~5e-2. After one iteration every marginal must agree
+    with the FP64 oracle to 1e-4 (FP32 messages land near 1e-6)."""
+    p, frames, _ = synth.make_workload("small", seed=seed)
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.perform_inference(max_iters=1)
+    o.perform_inference(1)
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"seed {seed}: first-iteration max-abs {err.max():.3e}")
+    assert err.max() <= 1e-4
+
+
 def test_bp_repeated_calls_keep_beliefs():
     """Beliefs persist across perform_inference calls, messages are reset
     (GVDBInference.cpp:128-134)."""
