@@ -106,7 +106,7 @@ struct mrf_ctx {
   size_t cap_sort = 0;
   uint32_t* d_vflag = nullptr;  // 4 pool-sized tables: touched, window-touched and their scans
   size_t cap_vflag = 0;
-  uint32_t* d_acc = nullptr;  // 4 words per pair: two 64-bit fixed-point chord-length sums
+  uint32_t* d_acc = nullptr;  // 6 words per pair: three 64-bit fixed-point chord-length sums
   size_t cap_acc = 0;
   uint2* d_sell0 = nullptr;
   size_t cap_sell0 = 0;
@@ -381,7 +381,7 @@ int ensure_acc(mrf_ctx* c, size_t np) {
   np += np / 4;
   dfree(&c->d_acc);
   c->cap_acc = 0;
-  int rc = dalloc(c, &c->d_acc, 4 * np);
+  int rc = dalloc(c, &c->d_acc, 6 * np);
   if (rc) return rc;
   c->cap_acc = np;
   return MRF_OK;
@@ -542,8 +542,8 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     return rc;
   launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
   // 5. per pair: C and the constant front messages; records point at their pairs from here on
-  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, np, cam.d_pair_c,
-                      cam.d_pair_sf, s);
+  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, c->d_acc + 4 * (size_t)np,
+                      np, cam.d_pair_c, cam.d_pair_sf, s);
   // 6. window segments: the window records stably sorted by pair
   const int wbits = std::max(1, bit_length(nwp));
   if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits))) return rc;

@@ -14,7 +14,10 @@
 // 2^-24 fixed point as 64-bit values kept in two 32-bit words: a 32-bit
 // atomic add to the low word, plus one to the high word when it wraps.  The
 // sums are exact, so they do not depend on the order in which records
-// arrive.
+// arrive.  The part of each length below 2^-24 is summed the same way in
+// units of 2^-48, so C is exact for every chord of at least 2^-25 and the
+// epilogue gate C > 0.1 falls as it does on the FP64 sum
+// (gvdbkernels.cu:1276-1302), however close to 0.1 C lies.
 //
 // Window segments.  The window records are stably radix-sorted by pair, so
 // each pair's records stay in record order; the BP iteration sums each
@@ -84,25 +87,27 @@ __device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
 // one bitonic step: lanes l and l ^ J order (key, val) ascending inside
 // blocks of K lanes whose index bit K is 0, descending otherwise
 template <int K, int J>
-__device__ __forceinline__ void bitonic_step(int lane, uint32_t& key, uint32_t& val) {
-  const uint32_t ok = xor_lane<J>(key), ov = xor_lane<J>(val);
+__device__ __forceinline__ void bitonic_step(int lane, uint32_t& key, uint32_t& val, uint32_t& val2) {
+  const uint32_t ok = xor_lane<J>(key), ov = xor_lane<J>(val), ov2 = xor_lane<J>(val2);
   const bool up = (lane & K) == 0;
   const bool lower = (lane & J) == 0;
   const bool take = (lower == up) ? (ok < key) : (ok > key);
   key = take ? ok : key;
   val = take ? ov : val;
+  val2 = take ? ov2 : val2;
 }
 
 template <int K, int J>
-__device__ __forceinline__ void bitonic_merge(int lane, uint32_t& key, uint32_t& val) {
-  bitonic_step<K, J>(lane, key, val);
-  if constexpr (J > 1) bitonic_merge<K, J / 2>(lane, key, val);
+__device__ __forceinline__ void bitonic_merge(int lane, uint32_t& key, uint32_t& val, uint32_t& val2) {
+  bitonic_step<K, J>(lane, key, val, val2);
+  if constexpr (J > 1) bitonic_merge<K, J / 2>(lane, key, val, val2);
 }
 
+// sorts key ascending over the K lanes, val and val2 travel with it
 template <int K>
-__device__ __forceinline__ void bitonic_sort(int lane, uint32_t& key, uint32_t& val) {
-  if constexpr (K > 2) bitonic_sort<K / 2>(lane, key, val);
-  bitonic_merge<K, K / 2>(lane, key, val);
+__device__ __forceinline__ void bitonic_sort(int lane, uint32_t& key, uint32_t& val, uint32_t& val2) {
+  if constexpr (K > 2) bitonic_sort<K / 2>(lane, key, val, val2);
+  bitonic_merge<K, K / 2>(lane, key, val, val2);
 }
 
 template <int CTRL, int ROWS>
@@ -113,22 +118,24 @@ __device__ __forceinline__ uint32_t dpp_u(uint32_t x) {
 // (x, f) <- (x_left, f_left) (+) (x, f): add the left value unless a run
 // starts at or before this lane
 template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step_u(uint32_t& x, uint32_t& f) {
+__device__ __forceinline__ void seg_step_u(uint32_t& x, uint32_t& x2, uint32_t& f) {
   const uint32_t y = dpp_u<CTRL, ROWS>(x);
+  const uint32_t y2 = dpp_u<CTRL, ROWS>(x2);
   const uint32_t g = dpp_u<CTRL, ROWS>(f);
   x = f ? x : x + y;
+  x2 = f ? x2 : x2 + y2;
   f |= g;
 }
 
 // Inclusive segmented sum over the 64 lanes (all active): row_shr 1/2/4/8
 // inside each row of 16, then row_bcast 15 and 31.
-__device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t f) {
-  seg_step_u<0x111, 0xF>(x, f);
-  seg_step_u<0x112, 0xF>(x, f);
-  seg_step_u<0x114, 0xF>(x, f);
-  seg_step_u<0x118, 0xF>(x, f);
-  seg_step_u<0x142, 0xA>(x, f);
-  seg_step_u<0x143, 0xC>(x, f);
+__device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t& x2, uint32_t f) {
+  seg_step_u<0x111, 0xF>(x, x2, f);
+  seg_step_u<0x112, 0xF>(x, x2, f);
+  seg_step_u<0x114, 0xF>(x, x2, f);
+  seg_step_u<0x118, 0xF>(x, x2, f);
+  seg_step_u<0x142, 0xA>(x, x2, f);
+  seg_step_u<0x143, 0xC>(x, x2, f);
 }
 
 // A wave takes 64 consecutive records -- one step of the 64 rays of a pixel
@@ -140,38 +147,54 @@ __device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t f) {
 // lengths also go to the front sum) or all window.
 __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
                                                       const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_c,
-                                                      uint32_t* __restrict__ acc_f) {
+                                                      uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t base = wave * 64; base < nrec; base += nwaves * 64) {  // nrec is a multiple of 64
     const uint64_t i = base + lane;
-    uint32_t key = 0xFFFFFFFFu, q = 0;
+    uint32_t key = 0xFFFFFFFFu, q = 0, r = 0;
     const uint2 e = rec[i];
     if (e.x != kGapVox) {
       key = pid[e.x];
       rec[i].x = key;
-      q = (uint32_t)((double)__uint_as_float(e.y) * kFixLen);
+      // l 2^24 is exact in double; q its integer part (< 2^25), r the rest
+      // in units of 2^-48 (< 2^24)
+      const double x = (double)__uint_as_float(e.y) * kFixLen;
+      q = (uint32_t)x;
+      r = (uint32_t)((x - (double)q) * kFixLen);
     }
-    bitonic_sort<64>(lane, key, q);
+    bitonic_sort<64>(lane, key, q, r);
     const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
     const uint32_t next = (uint32_t)__shfl_down((int)key, 1, 64);
-    wave_seg_sum_u(q, lane == 0 || prev != key ? 1u : 0u);
+    wave_seg_sum_u(q, r, lane == 0 || prev != key ? 1u : 0u);
     const bool tail = lane == 63 || next != key;
     if (tail && key != 0xFFFFFFFFu) {
       add_split(acc_c + 2 * (size_t)key, q);
+      if (r) add_split(acc_r + 2 * (size_t)key, r);
       if (base < base_w) add_split(acc_f + 2 * (size_t)key, q);
     }
   }
 }
 
-__global__ void k_pair_finish(const uint32_t* __restrict__ acc_c, const uint32_t* __restrict__ acc_f, uint32_t np,
-                              float* __restrict__ pair_c, float* __restrict__ pair_sf) {
+// C in FP32 with the gate of the exact sum: where rounding to float would
+// move C across kCumLenMin, C takes the nearest float on the exact sum's side
+__global__ void k_pair_finish(const uint32_t* __restrict__ acc_c, const uint32_t* __restrict__ acc_f,
+                              const uint32_t* __restrict__ acc_r, uint32_t np, float* __restrict__ pair_c,
+                              float* __restrict__ pair_sf) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const double c = (double)acc_c[2 * p + 1] * 4294967296.0 + (double)acc_c[2 * p];
+  const double cr = (double)acc_r[2 * p + 1] * 4294967296.0 + (double)acc_r[2 * p];
   const double f = (double)acc_f[2 * p + 1] * 4294967296.0 + (double)acc_f[2 * p];
-  pair_c[p] = (float)(c / kFixLen);
+  const double cd = c / kFixLen + cr / (kFixLen * kFixLen);
+  float cf = (float)cd;
+  if (cd > (double)kCumLenGate) {
+    if (!(cf > kCumLenMin)) cf = __uint_as_float(__float_as_uint(kCumLenMin) + 1u);
+  } else if (cf > kCumLenMin) {
+    cf = kCumLenMin;
+  }
+  pair_c[p] = cf;
   pair_sf[p] = (float)((double)kFEps * (f / kFixLen));
 }
 
@@ -250,12 +273,16 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
 }
 
 void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
-                         uint32_t* acc_f, uint32_t np, float* pair_c, float* pair_sf, hipStream_t s) {
+                         uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
+                         hipStream_t s) {
   if (np == 0) return;
   launch_zero_words(acc_c, (size_t)np * 8, s);
   launch_zero_words(acc_f, (size_t)np * 8, s);
-  hipLaunchKernelGGL(k_pair_lengths, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_c, acc_f);
-  hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_c, acc_f, np, pair_c, pair_sf);
+  launch_zero_words(acc_r, (size_t)np * 8, s);
+  hipLaunchKernelGGL(k_pair_lengths, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_c, acc_f,
+                     acc_r);
+  hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_c, acc_f, acc_r, np, pair_c,
+                     pair_sf);
 }
 
 hipError_t sort_window(const uint2* recw, uint32_t nwr, uint32_t nwp, int bits, uint32_t* keys_in,
