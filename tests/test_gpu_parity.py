@@ -104,6 +104,31 @@ def test_bp_marginals_real_shape():
     assert err.max() <= MARG_TOL
 
 
+@pytest.mark.timeout(600)
+def test_full_resolution_keyframe_matches_oracle():
+    """One keyframe of the 13-keyframe workload at full size (640x480,
+    f=525, 1 cm; about 150 M incidences): the per-camera unit of the
+    benchmark at its real shape.  Traversal bit-exact; marginals after three
+    iterations (the third reads the kept sums of the static front chunks)
+    within 1e-4 of the FP64 oracle.  The oracle takes about 45 s."""
+    p, frames, _ = synth.make_workload("real", frames=13)
+    frames = frames[:1]
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    n = _compare_traversal(g, o, 1)
+    assert n > 100_000_000
+    st = g.perform_inference(max_iters=3)
+    so = o.perform_inference(3)
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    assert st["static_front_updates"] > 0
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"full-resolution keyframe: {n} records, max-abs marginal error {err.max():.3e}")
+    assert err.max() <= 1e-4
+
+
 @pytest.mark.parametrize("activation", [_lib.ACT_FULL_RAY, _lib.ACT_ENDPOINT])
 def test_bp_marginals_small(activation):
     p, frames, _ = synth.make_workload("small", activation=activation)
