@@ -162,6 +162,13 @@ int mrf_get_alphas(mrf_ctx* ctx, float* out, int64_t rows, int64_t cols);
 /* Belief log-odds B per voxel, same shape/order as mrf_get_alphas.  No
  * counterpart (the reference keeps B in a half2 atlas channel). */
 int mrf_get_beliefs(mrf_ctx* ctx, float* out, int64_t rows, int64_t cols);
+/* Camera cam_id's summed chord length C per voxel, same shape/order as
+ * mrf_get_alphas (0 where its rays do not pass): the reference's CUM_LENS
+ * channel (GVDBCommon.cuh:37-44) after that camera's cumulative-length pass
+ * (gvdbkernels.cu:519-571, 1276-1302), read back as get_channel_kernel does
+ * (gvdbkernels.cu:360-373).  Exact sums rounded to float; C > 0.1 holds
+ * exactly where it holds for the exact sum.  MRF_E_RANGE for a bad id. */
+int mrf_get_cum_lengths(mrf_ctx* ctx, uint32_t cam_id, float* out, int64_t rows, int64_t cols);
 
 /* Map evaluation from a camera: every pixel's ray is walked through the
  * current marginals to the end of the map.  Replaces

@@ -24,7 +24,8 @@ RAY_SCHEDULES = {0: (8, 4), 1: (8, 8), 2: (16, 4), 3: (16, 8), 4: (4, 4), 5: (16
 EXPORTS = [
     "mrf_params_default", "mrf_create", "mrf_destroy", "mrf_add_camera_with_depth",
     "mrf_add_camera_with_depth_k", "mrf_set_pose", "mrf_num_cameras", "mrf_perform_inference",
-    "mrf_num_bricks", "mrf_get_occupied_bricks", "mrf_get_alphas", "mrf_get_beliefs", "mrf_prepare",
+    "mrf_num_bricks", "mrf_get_occupied_bricks", "mrf_get_alphas", "mrf_get_beliefs", "mrf_get_cum_lengths",
+    "mrf_prepare",
     "mrf_get_traversal", "mrf_last_error", "mrf_evaluate_camera", "mrf_evaluate_at_pose", "mrf_load_map",
     "mrf_set_eval_thresholds",
     "mrf_set_launch_mode", "mrf_train_opts_default", "mrf_perform_inference_warm", "mrf_features",
@@ -116,6 +117,7 @@ def load():
     L.mrf_get_occupied_bricks.argtypes = [P, P, ctypes.c_int64]
     L.mrf_get_alphas.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64]
     L.mrf_get_beliefs.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64]
+    L.mrf_get_cum_lengths.argtypes = [P, ctypes.c_uint32, P, ctypes.c_int64, ctypes.c_int64]
     L.mrf_prepare.argtypes = [P]
     L.mrf_get_traversal.argtypes = [P, ctypes.c_uint32, i64p, i64p, P, P]
     L.mrf_evaluate_camera.argtypes = [P, ctypes.c_uint32, P, P, P, P, ctypes.POINTER(MrfEval)]

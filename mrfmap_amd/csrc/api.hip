@@ -1618,6 +1618,24 @@ int mrf_get_beliefs(mrf_ctx* c, float* out, int64_t rows, int64_t cols) {
   return read_pool(c, out, rows, cols, false);
 }
 
+int mrf_get_cum_lengths(mrf_ctx* c, uint32_t id, float* out, int64_t rows, int64_t cols) {
+  if (!c || (!out && rows)) return MRF_E_INVALID;
+  if (id >= c->cams.size()) return fail(c, MRF_E_RANGE, "get_cum_lengths: camera id out of range");
+  int rc = prepare(c);
+  if (rc) return rc;
+  if (rows != (int64_t)c->slot_lin.size() || cols != kBrickVox)
+    return fail(c, MRF_E_SHAPE, "get_cum_lengths: the array must be num_bricks x 512");
+  const size_t n = (size_t)rows * kBrickVox;
+  if (n == 0) return MRF_OK;
+  const Camera& cam = c->cams[id];
+  launch_fill_f32(c->d_out, 0.0f, n, c->stream);
+  if (cam.npairs) launch_scatter_pairs(cam.d_pair_vox, cam.d_pair_c, cam.npairs, c->d_out, c->stream);
+  HIPCHK(c, hipMemcpyAsync(out, c->d_out, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipGetLastError());
+  return MRF_OK;
+}
+
 int mrf_get_traversal(mrf_ctx* c, uint32_t id, int64_t* n_rays, int64_t* n_records, uint32_t* rays,
                       uint32_t* records) {
   if (!c) return MRF_E_INVALID;

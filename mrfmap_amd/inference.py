@@ -229,6 +229,13 @@ class GVDBInference:
         _lib.check(self._L.mrf_get_beliefs(self._h, _ptr(out), out.shape[0], out.shape[1]), self._h)
         return out
 
+    def get_cum_lengths(self, cam_id):
+        """Camera cam_id's summed chord length C per voxel (num_bricks x 512, 0 where its rays do not pass)."""
+        out = np.zeros((self.num_bricks(), 512), dtype=np.float32)
+        _lib.check(self._L.mrf_get_cum_lengths(self._h, int(cam_id), _ptr(out), out.shape[0], out.shape[1]),
+                   self._h)
+        return out
+
     def get_traversal(self, cam_id):
         nr, nc = ctypes.c_int64(), ctypes.c_int64()
         _lib.check(self._L.mrf_get_traversal(self._h, int(cam_id), ctypes.byref(nr), ctypes.byref(nc), None, None),

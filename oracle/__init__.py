@@ -82,6 +82,7 @@ def lib():
         L.orc_warm_start.argtypes = [P, P, P, ctypes.c_int64]
         L.orc_get_alphas.argtypes = [P, P]
         L.orc_get_beliefs.argtypes = [P, P]
+        L.orc_get_cum_lengths.argtypes = [P, ctypes.c_int, P]
         L.orc_ray_messages.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_double, P, P, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
         L.orc_ray_messages.restype = ctypes.c_double
@@ -252,6 +253,13 @@ class Oracle:
     def beliefs(self):
         out = np.zeros((self.num_bricks(), 512), dtype=np.float64)
         lib().orc_get_beliefs(self._h, _ptr(out))
+        return out
+
+    def cum_lengths(self, cam):
+        """Camera cam's FP64 summed chord length C per voxel (0 where its rays do not pass)."""
+        out = np.zeros((self.num_bricks(), 512), dtype=np.float64)
+        if lib().orc_get_cum_lengths(self._h, int(cam), _ptr(out)) != 0:
+            raise RuntimeError("orc_get_cum_lengths failed")
         return out
 
 

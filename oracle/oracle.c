@@ -565,7 +565,10 @@ int orc_prepare(orc_ctx* c) {
         }
         touched[ntouched++] = vox;
       }
-      csum[vox] += k->rb.len[i];
+      /* C sums the chord lengths the records carry, rounded to float as the
+       * traversal contract defines them (render_cumulative_length_kernel adds
+       * float lengths, gvdbkernels.cu:555-557); the messages keep FP64 lengths */
+      csum[vox] += (double)(float)k->rb.len[i];
     }
     qsort(touched, (size_t)ntouched, sizeof(uint32_t), cmp_u32);
     k->npairs = ntouched;
@@ -1152,6 +1155,17 @@ int orc_get_alphas(orc_ctx* c, double* out) {
 int orc_get_beliefs(orc_ctx* c, double* out) {
   if (!c || orc_prepare(c)) return -1;
   memcpy(out, c->B, sizeof(double) * (size_t)(c->nslots * 512));
+  return 0;
+}
+
+/* gvdbkernels.cu:519-571, 1276-1302: the CUM_LENS channel after camera
+ * cam's cumulative-length pass, the FP64 sum of its chord lengths per voxel */
+int orc_get_cum_lengths(orc_ctx* c, int cam, double* out) {
+  if (!c || orc_prepare(c)) return -1;
+  if (cam < 0 || cam >= c->ncams) return -5;
+  memset(out, 0, sizeof(double) * (size_t)(c->nslots * 512));
+  const ocam* k = &c->cams[cam];
+  for (int64_t q = 0; q < k->npairs; ++q) out[k->pair_vox[q]] = k->pair_c[q];
   return 0;
 }
 

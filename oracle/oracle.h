@@ -75,6 +75,9 @@ int orc_features(orc_ctx* c, float* X, uint32_t* vox, int64_t cap, int64_t* n);
 int orc_warm_start(orc_ctx* c, const uint32_t* vox, const double* pred, int64_t n);
 int orc_get_alphas(orc_ctx* c, double* out /* nbricks x 512 */);
 int orc_get_beliefs(orc_ctx* c, double* out);
+/* camera cam's summed chord length C per voxel (nbricks x 512; 0 where its
+ * rays do not pass) */
+int orc_get_cum_lengths(orc_ctx* c, int cam, double* out);
 
 /* Messages of one ray (the per-ray factor of the MRF).
  *   n[i]   : incoming P(o_i = 0) (already clamped by the caller)

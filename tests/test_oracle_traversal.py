@@ -143,3 +143,23 @@ def test_truncation_ran_out():
     vox, ln, zx, ran_out = oracle.walk_ray([0.5, 4.5, 4.5], [100.0, 0.0, 0.0], N, depth=5.0, trunc=True,
                                            sigma_poly=[0.05, 0, 0])
     assert ran_out and len(vox) == 64
+
+
+def test_cum_lengths_are_the_traversal_chord_sums():
+    """orc_get_cum_lengths (the CUM_LENS channel, gvdbkernels.cu:1276-1302)
+    equals, voxel by voxel, the sum of the chord lengths of the camera's
+    exported traversal records."""
+    from helpers import oracle_for
+    from mrfmap_amd import synth
+    p, frames, _ = synth.make_workload("small", frames=2)
+    o = oracle_for(p, frames)
+    o.prepare()
+    for c in range(len(frames)):
+        _, recs = o.traversal(c)
+        live = recs[:, 0] != 0xFFFFFFFF
+        lens = recs[live, 1].view(np.float32).astype(np.float64)
+        cum = o.cum_lengths(c).reshape(-1)
+        expect = np.bincount(recs[live, 0].astype(np.int64), weights=lens, minlength=cum.size)
+        assert cum.size == expect.size
+        np.testing.assert_allclose(cum, expect, rtol=1e-12, atol=0)
+        assert (cum > 0).sum() > 1000
