@@ -122,6 +122,14 @@ class GVDBInference {
     get_alphas(a.data(), n, 512);
     return a;
   }
+  /* camera id's CUM_LENS channel (GVDBCommon.cuh:37-44): its summed chord
+   * length per voxel, num_bricks x 512, 0 where its rays do not pass */
+  std::vector<float> get_cum_lengths(uint32_t id) {
+    const int64_t n = num_bricks();
+    std::vector<float> a((size_t)n * 512);
+    check(mrf_get_cum_lengths(ctx_, id, a.data(), n, 512), ctx_, "get_cum_lengths");
+    return a;
+  }
   /* compute_likelihood(id) / get_likelihood_image(id, img) (GVDBInference.cpp:180-193):
    * sum of the per-pixel log-likelihoods; img (rows x cols, row-major) may be null */
   float compute_likelihood(uint32_t id, float* img = nullptr) {
