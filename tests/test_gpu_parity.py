@@ -4,6 +4,8 @@ Traversal lists must match bit for bit; marginals must agree with the FP64
 BP within 1e-3 max-abs (the tolerance of the mixed-precision target; FP32
 messages land far inside it).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -162,6 +164,12 @@ def test_cum_lengths_match_oracle():
     with pytest.raises(_lib.MrfError) as e:
         g.get_cum_lengths(len(frames))
     assert e.value.code == -5
+    # through the C ABI with a wrong shape: MRF_E_SHAPE, nothing written
+    n = g.num_bricks()
+    bad = np.full((n + 1, 512), 7.0, dtype=np.float32)
+    rc = g._L.mrf_get_cum_lengths(g._h, 0, bad.ctypes.data_as(ctypes.c_void_p), n + 1, 512)
+    assert rc == -4
+    assert (bad == 7.0).all()
 
 
 @pytest.mark.parametrize("activation", [_lib.ACT_FULL_RAY, _lib.ACT_ENDPOINT])
