@@ -54,53 +54,87 @@ class OrcParams(ctypes.Structure):
     ]
 
 
+def _bind(L):
+    """Declare the C signatures of liboracle (or of a variant built from a
+    modified copy of its sources, see load_variant)."""
+    P = ctypes.c_void_p
+    f64p = ctypes.POINTER(ctypes.c_double)
+    L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(P)]
+    L.orc_destroy.argtypes = [P]
+    L.orc_add_camera.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int]
+    L.orc_set_pose.argtypes = [P, ctypes.c_int, P]
+    L.orc_set_beliefs.argtypes = [P, P, P]
+    L.orc_load_map.argtypes = [P, P, P, ctypes.c_int64]
+    L.orc_evaluate.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P, P, P]
+    L.orc_prepare.argtypes = [P]
+    L.orc_num_bricks.argtypes = [P]
+    L.orc_num_bricks.restype = ctypes.c_int64
+    L.orc_brick_coords.argtypes = [P, P]
+    L.orc_get_traversal.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_int64), P, P]
+    L.orc_perform_inference.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p,
+                                        ctypes.POINTER(ctypes.c_int64)]
+    L.orc_continue_inference.argtypes = [P, ctypes.c_int, ctypes.c_double, f64p]
+    L.orc_features.argtypes = [P, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    L.orc_warm_start.argtypes = [P, P, P, ctypes.c_int64]
+    L.orc_get_alphas.argtypes = [P, P]
+    L.orc_get_beliefs.argtypes = [P, P]
+    L.orc_get_cum_lengths.argtypes = [P, ctypes.c_int, P]
+    L.orc_ray_messages.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_double, P, P, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
+    L.orc_ray_messages.restype = ctypes.c_double
+    L.orc_ray_messages_bruteforce.argtypes = [ctypes.c_int, P, P, ctypes.c_double, P]
+    L.orc_depth_psi.argtypes = [ctypes.c_double, ctypes.c_double, P, P, ctypes.c_int]
+    L.orc_depth_psi.restype = ctypes.c_double
+    L.orc_walk_ray.argtypes = [P, P, P, ctypes.c_double, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int,
+                               P, P, P, ctypes.POINTER(ctypes.c_int)]
+    L.oct_create.argtypes = [ctypes.c_double, P, P, ctypes.c_double, ctypes.POINTER(P)]
+    L.oct_destroy.argtypes = [P]
+    L.oct_insert_point_cloud.argtypes = [P, P, ctypes.c_int64, P]
+    L.oct_insert_point_cloud.restype = ctypes.c_int64
+    L.oct_num_nodes.argtypes = [P]
+    L.oct_num_nodes.restype = ctypes.c_int64
+    L.oct_logodds_at.argtypes = [P, P]
+    L.oct_logodds_at.restype = ctypes.c_double
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        P = ctypes.c_void_p
-        f64p = ctypes.POINTER(ctypes.c_double)
-        L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(P)]
-        L.orc_destroy.argtypes = [P]
-        L.orc_add_camera.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int]
-        L.orc_set_pose.argtypes = [P, ctypes.c_int, P]
-        L.orc_set_beliefs.argtypes = [P, P, P]
-        L.orc_load_map.argtypes = [P, P, P, ctypes.c_int64]
-        L.orc_evaluate.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P, P, P]
-        L.orc_prepare.argtypes = [P]
-        L.orc_num_bricks.argtypes = [P]
-        L.orc_num_bricks.restype = ctypes.c_int64
-        L.orc_brick_coords.argtypes = [P, P]
-        L.orc_get_traversal.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
-                                        ctypes.POINTER(ctypes.c_int64), P, P]
-        L.orc_perform_inference.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p,
-                                            ctypes.POINTER(ctypes.c_int64)]
-        L.orc_continue_inference.argtypes = [P, ctypes.c_int, ctypes.c_double, f64p]
-        L.orc_features.argtypes = [P, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
-        L.orc_warm_start.argtypes = [P, P, P, ctypes.c_int64]
-        L.orc_get_alphas.argtypes = [P, P]
-        L.orc_get_beliefs.argtypes = [P, P]
-        L.orc_get_cum_lengths.argtypes = [P, ctypes.c_int, P]
-        L.orc_ray_messages.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_double, P, P, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
-        L.orc_ray_messages.restype = ctypes.c_double
-        L.orc_ray_messages_bruteforce.argtypes = [ctypes.c_int, P, P, ctypes.c_double, P]
-        L.orc_depth_psi.argtypes = [ctypes.c_double, ctypes.c_double, P, P, ctypes.c_int]
-        L.orc_depth_psi.restype = ctypes.c_double
-        L.orc_walk_ray.argtypes = [P, P, P, ctypes.c_double, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int,
-                                   P, P, P, ctypes.POINTER(ctypes.c_int)]
-        L.oct_create.argtypes = [ctypes.c_double, P, P, ctypes.c_double, ctypes.POINTER(P)]
-        L.oct_destroy.argtypes = [P]
-        L.oct_insert_point_cloud.argtypes = [P, P, ctypes.c_int64, P]
-        L.oct_insert_point_cloud.restype = ctypes.c_int64
-        L.oct_num_nodes.argtypes = [P]
-        L.oct_num_nodes.restype = ctypes.c_int64
-        L.oct_logodds_at.argtypes = [P, P]
-        L.oct_logodds_at.restype = ctypes.c_double
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIB))
     return _lib
+
+
+def load_variant(so_path):
+    """A liboracle built elsewhere (tests build planted-bug variants of
+    oracle.c to show that a check catches them)."""
+    return _bind(ctypes.CDLL(so_path))
+
+
+def build_variant(replacements, out_dir, name="variant"):
+    """Compile a copy of the oracle with textual replacements [(old, new)]
+    (each `old` must occur exactly once in oracle.c) into out_dir and load
+    it.  Tests use this to plant a mistake in the oracle, on the CPU, and
+    show that a check fails on it."""
+    src = open(os.path.join(HERE, "oracle.c")).read()
+    for old, new in replacements:
+        n = src.count(old)
+        if n != 1:
+            raise ValueError(f"replacement target occurs {n} times: {old!r}")
+        src = src.replace(old, new)
+    c_path = os.path.join(out_dir, name + ".c")
+    so_path = os.path.join(out_dir, name + ".so")
+    with open(c_path, "w") as f:
+        f.write(src)
+    cmd = ["gcc", "-O1", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-I", HERE,
+           c_path, os.path.join(HERE, "octomap_eq.c"), "-o", so_path, "-lm"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("variant build failed:\n" + r.stdout)
+    return load_variant(so_path)
 
 
 def _ptr(a):
@@ -115,7 +149,7 @@ class Oracle:
     """FP64 scalar BP over the same keyframes the GPU path sees."""
 
     def __init__(self, K, rows, cols, res, prior, dims, sigma_poly=None, bias_poly=None, lookup_n=3,
-                 activation=1, mask_848_border=1):
+                 activation=1, mask_848_border=1, library=None):
         p = OrcParams()
         p.K[:] = [float(v) for v in np.asarray(K, dtype=np.float32).reshape(9)]
         p.rows, p.cols = int(rows), int(cols)
@@ -128,8 +162,9 @@ class Oracle:
         p.bias_poly[:] = [float(v) for v in list(bp) + [0.0] * (3 - len(bp))]
         p.activation = int(activation)
         p.mask_848_border = int(mask_848_border)
+        self._L = library if library is not None else lib()
         self._h = ctypes.c_void_p()
-        rc = lib().orc_create(ctypes.byref(p), ctypes.byref(self._h))
+        rc = self._L.orc_create(ctypes.byref(p), ctypes.byref(self._h))
         if rc != 0:
             raise ValueError(f"orc_create failed ({rc})")
         self.K = np.asarray(K, dtype=np.float32).reshape(9)
@@ -137,37 +172,37 @@ class Oracle:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().orc_destroy(h)
+            self._L.orc_destroy(h)
             self._h = None
 
     def add_camera(self, pose, depth, K=None):
         pose = np.ascontiguousarray(pose, dtype=np.float32).reshape(16)
         depth = np.ascontiguousarray(depth, dtype=np.float32)
         Kc = self.K if K is None else np.ascontiguousarray(K, dtype=np.float32).reshape(9)
-        rc = lib().orc_add_camera(self._h, _ptr(pose), _ptr(Kc), _ptr(depth), depth.shape[0], depth.shape[1])
+        rc = self._L.orc_add_camera(self._h, _ptr(pose), _ptr(Kc), _ptr(depth), depth.shape[0], depth.shape[1])
         if rc < 0:
             raise ValueError(f"orc_add_camera failed ({rc})")
         return rc
 
     def set_pose(self, cam, pose):
         pose = np.ascontiguousarray(pose, dtype=np.float32).reshape(16)
-        if lib().orc_set_pose(self._h, int(cam), _ptr(pose)) != 0:
+        if self._L.orc_set_pose(self._h, int(cam), _ptr(pose)) != 0:
             raise IndexError(cam)
 
     def prepare(self):
-        lib().orc_prepare(self._h)
+        self._L.orc_prepare(self._h)
 
     def load_map(self, points, occupancies):
         pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
         occ = np.ascontiguousarray(occupancies, dtype=np.float32).reshape(-1)
-        if lib().orc_load_map(self._h, _ptr(pts), _ptr(occ), len(pts)) != 0:
+        if self._L.orc_load_map(self._h, _ptr(pts), _ptr(occ), len(pts)) != 0:
             raise RuntimeError("orc_load_map failed")
 
     def set_beliefs(self, B, upd):
         """Test hook: beliefs (num_bricks x 512 log-odds) and update flags."""
         B = np.ascontiguousarray(B, dtype=np.float64)
         upd = np.ascontiguousarray(upd, dtype=np.uint8)
-        if lib().orc_set_beliefs(self._h, _ptr(B), _ptr(upd)) != 0:
+        if self._L.orc_set_beliefs(self._h, _ptr(B), _ptr(upd)) != 0:
             raise RuntimeError("orc_set_beliefs failed")
 
     def evaluate(self, pose, depth, K=None, acc_thresh=0.5, shape=None):
@@ -178,7 +213,7 @@ class Oracle:
         d = None if depth is None else np.ascontiguousarray(depth, dtype=np.float32)
         rows, cols = d.shape if d is not None else shape
         imgs = [np.zeros((rows, cols), dtype=np.float64) for _ in range(4)]
-        if lib().orc_evaluate(self._h, _ptr(pose), _ptr(Kc), None if d is None else _ptr(d), rows, cols,
+        if self._L.orc_evaluate(self._h, _ptr(pose), _ptr(Kc), None if d is None else _ptr(d), rows, cols,
                               float(acc_thresh), *[_ptr(i) for i in imgs]) != 0:
             raise RuntimeError("orc_evaluate failed")
         ll, ed, acc, sacc = imgs
@@ -191,19 +226,19 @@ class Oracle:
                 "simple_accuracy": (simple_counts[2] + simple_counts[0]) / n}
 
     def num_bricks(self):
-        return int(lib().orc_num_bricks(self._h))
+        return int(self._L.orc_num_bricks(self._h))
 
     def brick_coords(self):
         out = np.zeros((self.num_bricks(), 3), dtype=np.int32)
-        lib().orc_brick_coords(self._h, _ptr(out))
+        self._L.orc_brick_coords(self._h, _ptr(out))
         return out
 
     def traversal(self, cam):
         nr, nc = ctypes.c_int64(), ctypes.c_int64()
-        lib().orc_get_traversal(self._h, int(cam), ctypes.byref(nr), ctypes.byref(nc), None, None)
+        self._L.orc_get_traversal(self._h, int(cam), ctypes.byref(nr), ctypes.byref(nc), None, None)
         rays = np.zeros((nr.value, 6), dtype=np.uint32)
         recs = np.zeros((nc.value, 2), dtype=np.uint32)
-        lib().orc_get_traversal(self._h, int(cam), None, None, _ptr(rays), _ptr(recs))
+        self._L.orc_get_traversal(self._h, int(cam), None, None, _ptr(rays), _ptr(recs))
         return rays, recs
 
     def perform_inference(self, max_iters, tol=0.0, ids=None):
@@ -211,10 +246,10 @@ class Oracle:
         upd = ctypes.c_int64()
         if ids is not None:
             ida = np.ascontiguousarray(ids, dtype=np.int32)
-            it = lib().orc_perform_inference(self._h, _ptr(ida), len(ida), int(max_iters), float(tol),
+            it = self._L.orc_perform_inference(self._h, _ptr(ida), len(ida), int(max_iters), float(tol),
                                              ctypes.byref(res), ctypes.byref(upd))
         else:
-            it = lib().orc_perform_inference(self._h, None, 0, int(max_iters), float(tol), ctypes.byref(res),
+            it = self._L.orc_perform_inference(self._h, None, 0, int(max_iters), float(tol), ctypes.byref(res),
                                              ctypes.byref(upd))
         if it < 0:
             raise RuntimeError(f"orc_perform_inference failed ({it})")
@@ -223,7 +258,7 @@ class Oracle:
     def continue_inference(self, max_iters, tol=0.0):
         """Iterations without the per-call message reset."""
         res = ctypes.c_double()
-        it = lib().orc_continue_inference(self._h, int(max_iters), float(tol), ctypes.byref(res))
+        it = self._L.orc_continue_inference(self._h, int(max_iters), float(tol), ctypes.byref(res))
         if it < 0:
             raise RuntimeError(f"orc_continue_inference failed ({it})")
         return {"iters": it, "residual": res.value}
@@ -231,10 +266,10 @@ class Oracle:
     def features(self):
         """(X n x 32 float32, pool index n) of the warm-start features."""
         n = ctypes.c_int64()
-        lib().orc_features(self._h, None, None, 0, ctypes.byref(n))
+        self._L.orc_features(self._h, None, None, 0, ctypes.byref(n))
         X = np.zeros((max(n.value, 1), 32), dtype=np.float32)
         vox = np.zeros(max(n.value, 1), dtype=np.uint32)
-        rc = lib().orc_features(self._h, _ptr(X), _ptr(vox), n.value, ctypes.byref(n))
+        rc = self._L.orc_features(self._h, _ptr(X), _ptr(vox), n.value, ctypes.byref(n))
         if rc != 0:
             raise RuntimeError(f"orc_features failed ({rc})")
         return X[:n.value], vox[:n.value]
@@ -242,23 +277,23 @@ class Oracle:
     def warm_start(self, vox, pred):
         vox = np.ascontiguousarray(vox, dtype=np.uint32)
         pred = _f64(pred)
-        if lib().orc_warm_start(self._h, _ptr(vox), _ptr(pred), len(vox)) != 0:
+        if self._L.orc_warm_start(self._h, _ptr(vox), _ptr(pred), len(vox)) != 0:
             raise RuntimeError("orc_warm_start failed")
 
     def alphas(self):
         out = np.zeros((self.num_bricks(), 512), dtype=np.float64)
-        lib().orc_get_alphas(self._h, _ptr(out))
+        self._L.orc_get_alphas(self._h, _ptr(out))
         return out
 
     def beliefs(self):
         out = np.zeros((self.num_bricks(), 512), dtype=np.float64)
-        lib().orc_get_beliefs(self._h, _ptr(out))
+        self._L.orc_get_beliefs(self._h, _ptr(out))
         return out
 
     def cum_lengths(self, cam):
         """Camera cam's FP64 summed chord length C per voxel (0 where its rays do not pass)."""
         out = np.zeros((self.num_bricks(), 512), dtype=np.float64)
-        if lib().orc_get_cum_lengths(self._h, int(cam), _ptr(out)) != 0:
+        if self._L.orc_get_cum_lengths(self._h, int(cam), _ptr(out)) != 0:
             raise RuntimeError("orc_get_cum_lengths failed")
         return out
 

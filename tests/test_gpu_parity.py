@@ -521,3 +521,46 @@ def test_large_scene_limit_cycle_matches_oracle():
     err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
     print(f"limit cycle: max-abs marginal error {err.max():.3e}")
     assert err.max() <= MARG_TOL
+
+
+class _GpuScene:
+    """A scene of bp_loop_cases on the device, through the C ABI."""
+
+    def __init__(self, cams):
+        from mrfmap_amd import gvdb_params
+        from bp_loop_cases import DIMS, K_UNIT, PRIOR, RES, SIG, camera
+
+        p = gvdb_params()
+        p.res, p.prior, p.dims = RES, PRIOR, np.asarray(DIMS, np.float32)
+        p.rows = p.cols = 1
+        p.K = K_UNIT.reshape(3, 3).copy()
+        p.lookup_n, p.const_sigmasq_poly, p.const_bias_poly = 3, [SIG, 0.0, 0.0], [0.0, 0.0, 0.0]
+        p.activation = _lib.ACT_FULL_RAY
+        self.g = GVDBInference(params=p)
+        for x0, d in cams:
+            self.g.add_camera_with_depth(*camera(x0, d))
+
+    def perform_inference(self, iters):
+        return self.g.perform_inference(max_iters=iters)
+
+    def beliefs(self):
+        return self.g.get_beliefs()
+
+    def alphas(self):
+        return self.g.get_alphas()
+
+    def num_bricks(self):
+        return self.g.num_bricks()
+
+
+@pytest.mark.parametrize("name", ["one_camera_one_iteration", "cum_length_gate", "gauss_seidel",
+                                  "fixed_point_of_one_camera", "second_call_resets_messages"])
+def test_bp_loop_by_hand_on_gpu(name):
+    """The hand-computed BP-loop cases (camera message logit(clamp(S/C)),
+    B += m - M_c, the C > 0.1 gate, Gauss-Seidel order, the per-call reset)
+    on the device: beliefs within FP32 rounding of the hand values."""
+    from bp_loop_cases import CHECKS
+
+    err = CHECKS[name](_GpuScene)
+    print(f"{name}: max-abs error {err:.3e}")
+    assert err < 2e-5
