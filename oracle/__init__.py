@@ -51,6 +51,7 @@ class OrcParams(ctypes.Structure):
         ("res", ctypes.c_float), ("prior", ctypes.c_float), ("dims", ctypes.c_float * 3),
         ("lookup_n", ctypes.c_int), ("sigma_poly", ctypes.c_float * 3), ("bias_poly", ctypes.c_float * 3),
         ("activation", ctypes.c_int), ("mask_848_border", ctypes.c_int),
+        ("sigma_lookup", ctypes.c_void_p), ("bias_lookup", ctypes.c_void_p),
     ]
 
 
@@ -149,7 +150,10 @@ class Oracle:
     """FP64 scalar BP over the same keyframes the GPU path sees."""
 
     def __init__(self, K, rows, cols, res, prior, dims, sigma_poly=None, bias_poly=None, lookup_n=3,
-                 activation=1, mask_848_border=1, library=None):
+                 activation=1, mask_848_border=1, library=None, sigma_lookup=None, bias_lookup=None):
+        """sigma_lookup / bias_lookup: optional per-cell tables of
+        ceil(rows/20) * ceil(cols/20) * lookup_n coefficients (then every
+        keyframe must be rows x cols)."""
         p = OrcParams()
         p.K[:] = [float(v) for v in np.asarray(K, dtype=np.float32).reshape(9)]
         p.rows, p.cols = int(rows), int(cols)
@@ -162,6 +166,13 @@ class Oracle:
         p.bias_poly[:] = [float(v) for v in list(bp) + [0.0] * (3 - len(bp))]
         p.activation = int(activation)
         p.mask_848_border = int(mask_848_border)
+        if sigma_lookup is not None and bias_lookup is not None:
+            tab_sig = np.ascontiguousarray(sigma_lookup, dtype=np.float32).ravel()
+            tab_bia = np.ascontiguousarray(bias_lookup, dtype=np.float32).ravel()
+            need = -(-int(rows) // 20) * -(-int(cols) // 20) * int(lookup_n)
+            if tab_sig.size < need or tab_bia.size < need:
+                raise ValueError(f"lookup tables need {need} coefficients")
+            p.sigma_lookup, p.bias_lookup = tab_sig.ctypes.data, tab_bia.ctypes.data
         self._L = library if library is not None else lib()
         self._h = ctypes.c_void_p()
         rc = self._L.orc_create(ctypes.byref(p), ctypes.byref(self._h))

@@ -273,6 +273,7 @@ struct orc_ctx {
   int N[3], Mx[3], B3[3], nbricks;
   double res, off[3], prior_lo;
   double sig[3], bia[3];
+  float *sig_tab, *bia_tab; /* per-cell tables (NULL: sig/bia in every cell) */
   int32_t* slot;
   int32_t* slot_lin;
   int64_t nslots;
@@ -303,6 +304,19 @@ int orc_create(const orc_params* p, orc_ctx** out) {
   c->off[0] = 0.5 * c->Mx[0];
   c->off[1] = 0.5 * c->Mx[1];
   c->off[2] = 0.0;
+  if (p->sigma_lookup && p->bias_lookup) {
+    if (p->rows <= 0 || p->cols <= 0) {
+      free(c);
+      return -1;
+    }
+    const size_t n = (size_t)((p->rows + LOOKUP_PX - 1) / LOOKUP_PX) * ((p->cols + LOOKUP_PX - 1) / LOOKUP_PX) *
+                     (size_t)p->lookup_n;
+    c->sig_tab = (float*)malloc(sizeof(float) * n);
+    c->bia_tab = (float*)malloc(sizeof(float) * n);
+    memcpy(c->sig_tab, p->sigma_lookup, sizeof(float) * n);
+    memcpy(c->bia_tab, p->bias_lookup, sizeof(float) * n);
+  }
+  c->p.sigma_lookup = c->p.bias_lookup = NULL;
   c->nbricks = c->B3[0] * c->B3[1] * c->B3[2];
   c->slot = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->nbricks);
   for (int i = 0; i < c->nbricks; ++i) c->slot[i] = -1;
@@ -338,6 +352,8 @@ void orc_destroy(orc_ctx* c) {
     free(c->cams[i].M);
   }
   free(c->cams);
+  free(c->sig_tab);
+  free(c->bia_tab);
   free(c->slot);
   free(c->slot_lin);
   free(c->B);
@@ -356,6 +372,7 @@ static void set_pose(ocam* k, const float pose[16]) {
 
 int orc_add_camera(orc_ctx* c, const float pose[16], const float K[9], const float* depth, int rows, int cols) {
   if (!c || !pose || !K || !depth || rows <= 0 || cols <= 0) return -1;
+  if (c->sig_tab && (rows != c->p.rows || cols != c->p.cols)) return -1; /* tables are per params image */
   if (c->ncams == c->capcams) {
     c->capcams = c->capcams ? 2 * c->capcams : 16;
     c->cams = (ocam*)realloc(c->cams, sizeof(ocam) * (size_t)c->capcams);
@@ -418,6 +435,27 @@ static int pixel_ok(const orc_ctx* c, const ocam* k, int x, int y, float d) {
   return 1;
 }
 
+/* offset of pixel (x, y)'s sensor-model cell: 20 x 20 pixel cells of the
+ * vertically flipped image, lookup_n coefficients each (gvdbkernels.cu:46-55) */
+static uint32_t cell_offset(const ocam* k, int ln, int x, int y) {
+  const int v = k->rows - 1 - y;
+  return (uint32_t)((v / LOOKUP_PX) * (k->lookup_u * ln) + (x / LOOKUP_PX) * ln);
+}
+
+/* sigma and bias polynomials of the cell at `cell` (the constant ones without tables) */
+static void cell_polys(const orc_ctx* c, uint32_t cell, double sig[3], double bia[3]) {
+  const int ln = c->p.lookup_n;
+  for (int j = 0; j < 3; ++j) {
+    if (!c->sig_tab) {
+      sig[j] = c->sig[j];
+      bia[j] = c->bia[j];
+    } else {
+      sig[j] = j < ln ? (double)c->sig_tab[cell + (uint32_t)j] : 0.0;
+      bia[j] = j < ln ? (double)c->bia_tab[cell + (uint32_t)j] : 0.0;
+    }
+  }
+}
+
 static int setup_ray(const orc_ctx* c, const ocam* k, int x, int y, int64_t Oi[3], int64_t Di[3], double* norm) {
   double ux = ((double)x - k->cx) / k->fx;
   double uy = ((double)y - k->cy) / k->fy;
@@ -476,10 +514,12 @@ int orc_prepare(orc_ctx* c) {
         double norm;
         if (!pixel_ok(c, k, x, y, d) || !setup_ray(c, k, x, y, Oi, Di, &norm)) continue;
         if (c->p.activation == 1) {
+          double sg[3], bi[3];
+          cell_polys(c, cell_offset(k, ln, x, y), sg, bi);
           ray_segments(Oi, Di, c->N, &sv);
           float z0;
           int ro;
-          walk_filtered(&sv, &bm, norm, (double)d, c->sig, c->bia, ln, NULL, &z0, &ro);
+          walk_filtered(&sv, &bm, norm, (double)d, sg, bi, ln, NULL, &z0, &ro);
         } else {
           int lo[3], hi[3], ok = 1;
           for (int a = 0; a < 3 && ok; ++a) {
@@ -532,19 +572,20 @@ int orc_prepare(orc_ctx* c) {
         int64_t Oi[3], Di[3];
         double norm;
         if (!pixel_ok(c, k, x, y, d) || !setup_ray(c, k, x, y, Oi, Di, &norm)) continue;
+        const uint32_t cell = cell_offset(k, ln, x, y);
+        double sg[3], bi[3];
+        cell_polys(c, cell, sg, bi);
         ray_segments(Oi, Di, c->N, &sv);
         int64_t off = k->rb.n;
         float z0;
         int ro;
-        int64_t n = walk_filtered(&sv, &bm, norm, (double)d, c->sig, c->bia, ln, &k->rb, &z0, &ro);
+        int64_t n = walk_filtered(&sv, &bm, norm, (double)d, sg, bi, ln, &k->rb, &z0, &ro);
         if (n == 0) continue;
         if (k->nrays == cap) {
           cap *= 2;
           k->rays = (uint32_t*)realloc(k->rays, sizeof(uint32_t) * 6 * (size_t)cap);
         }
         uint32_t* r = &k->rays[6 * k->nrays++];
-        int v = k->rows - 1 - y;
-        uint32_t cell = (uint32_t)((v / LOOKUP_PX) * (k->lookup_u * ln) + (x / LOOKUP_PX) * ln);
         float inv = (float)(1.0 / norm);
         r[0] = (uint32_t)off;
         r[1] = (uint32_t)n;
@@ -790,8 +831,9 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
           live[i] = vox != GAP;
           nin[i] = live[i] ? clampd(1.0 / (1.0 + exp(c->B[vox] - k->M[vox]))) : 1.0;
         }
-        orc_ray_messages(n, nin, k->rb.len + off, k->rb.zex + off, live, (double)d, c->sig, c->bia, ln, ro, 1.0,
-                         0, pout);
+        double sg[3], bi[3];
+        cell_polys(c, rr[5] & ~RAN_OUT, sg, bi);
+        orc_ray_messages(n, nin, k->rb.len + off, k->rb.zex + off, live, (double)d, sg, bi, ln, ro, 1.0, 0, pout);
         for (int i = 0; i < n; ++i)
           if (live[i]) S[k->rb.rec[2 * (off + i)]] += pout[i] * k->rb.len[off + i];
       }
@@ -1009,6 +1051,8 @@ int orc_evaluate(orc_ctx* c, const float pose[16], const float K[9], const float
       int peak_ok = 0, first = 1, sdone = 0;
       int64_t Oi[3], Di[3];
       double norm;
+      double csig[3], cbia[3];
+      cell_polys(c, cell_offset(&k, ln, x, y), csig, cbia);
       if (setup_ray(c, &k, x, y, Oi, Di, &norm)) {
         ray_segments(Oi, Di, c->N, &sv);
         for (int j = 0; j < sv.n; ++j) {
@@ -1029,13 +1073,13 @@ int orc_evaluate(orc_ctx* c, const float pose[16], const float K[9], const float
           const double vis = exp(lam);
           const double w = (prev_vis - vis) / len;
           if (valid) {
-            const double pdm = s + poly(c->bia, ln, s) - d;
-            const double sg = poly(c->sig, ln, s);
+            const double pdm = s + poly(cbia, ln, s) - d;
+            const double sg = poly(csig, ln, s);
             double prob;
             if (s < ORC_Z_MIN)
               prob = 0.0;
             else if (fabs(pdm) <= 3.0 * sg)
-              prob = (1.0 - ORC_LIKE_EPS) * 0.5 * (psi_d(s, d, c->sig, c->bia, ln) + psi_d(prev_s, d, c->sig, c->bia, ln));
+              prob = (1.0 - ORC_LIKE_EPS) * 0.5 * (psi_d(s, d, csig, cbia, ln) + psi_d(prev_s, d, csig, cbia, ln));
             else
               prob = ORC_LIKE_EPS;
             if (w > peak) {

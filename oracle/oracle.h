@@ -36,9 +36,14 @@ typedef struct orc_params {
   float res, prior;
   float dims[3];
   int lookup_n;
-  float sigma_poly[3], bias_poly[3]; /* constant polynomials for every cell */
+  float sigma_poly[3], bias_poly[3]; /* polynomials of every cell when the tables are NULL */
   int activation;                    /* 0 endpoint, 1 full ray */
   int mask_848_border;
+  /* optional per-cell tables, ceil(rows/20) x ceil(cols/20) cells of lookup_n
+   * coefficients (GVDBParams.h:93-104, 134-162), indexed in the vertically
+   * flipped image (gvdbkernels.cu:46-55); every keyframe must then be rows x cols */
+  const float* sigma_lookup;
+  const float* bias_lookup;
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;

@@ -13,9 +13,12 @@ def decimate(depth, step):
 
 
 def oracle_for(p, frames, activation=None):
+    tables = {}
+    if p.use_polys and p.depth_sigma_lookup is not None and p.depth_bias_lookup is not None:
+        tables = {"sigma_lookup": p.depth_sigma_lookup, "bias_lookup": p.depth_bias_lookup}
     o = oracle.Oracle(p.K, p.rows, p.cols, p.res, p.prior, p.dims, p.const_sigmasq_poly, p.const_bias_poly,
                       lookup_n=p.lookup_n, activation=p.activation if activation is None else activation,
-                      mask_848_border=int(p.mask_848_border))
+                      mask_848_border=int(p.mask_848_border), **tables)
     for pose, K, depth in frames:
         o.add_camera(pose, depth, K)
     return o

@@ -564,3 +564,34 @@ def test_bp_loop_by_hand_on_gpu(name):
     err = CHECKS[name](_GpuScene)
     print(f"{name}: max-abs error {err:.3e}")
     assert err < 2e-5
+
+
+@pytest.mark.parametrize("ln", [2, 3])
+def test_per_cell_sensor_model_matches_oracle(ln):
+    """use_polys tables (GVDBParams.h:134-162) on the small workload: every
+    20x20-pixel cell has its own sigma and bias polynomial, looked up in the
+    vertically flipped image (gvdbkernels.cu:46-55).  lookup_n = 3 scales the
+    quadratic sigma and bias polynomials of coffin_world_640.yaml:33-35 per
+    cell; lookup_n = 2 a linear pair.  Traversal (truncation depends on the
+    cell's polynomials) bit-exact, marginals after 5 iterations within 1e-3."""
+    p, frames, _ = synth.make_workload("small")
+    rng = np.random.default_rng(40 + ln)
+    cells = p.lookup_cells()
+    base_sig = [0.01, -0.00095238, 0.00857143] if ln == 3 else [0.02, 0.012]
+    base_bia = [0.02375, -0.0425, 0.04833333] if ln == 3 else [0.005, -0.002]
+    p.use_polys, p.lookup_n = True, ln
+    p.depth_sigma_lookup = np.outer(rng.uniform(0.7, 1.6, cells), base_sig).astype(np.float32).ravel()
+    p.depth_bias_lookup = np.outer(rng.uniform(0.0, 1.5, cells), base_bia).astype(np.float32).ravel()
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    n = _compare_traversal(g, o, len(frames))
+    assert n > 500_000
+    st = g.perform_inference(max_iters=5)
+    so = o.perform_inference(5)
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"per-cell sensor model, lookup_n={ln}: {n} records, max-abs marginal error {err.max():.3e}")
+    assert err.max() <= MARG_TOL
