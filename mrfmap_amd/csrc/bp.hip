@@ -762,7 +762,7 @@ __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ he
 
 
 // Camera message of each (camera, voxel) pair from its message sum:
-// a = clamp(S/C), m = ln a - ln(1-a), B += m - M_c, M_c = m, for C > 0.1
+// a = clamp(S/C), m = ln a - ln(1-a), B += m - M_c, M_c = m, for C > 0.1f
 // (compute_gvdb_message_to_factors, gvdbkernels.cu:408, 457-479).
 // S = (sum of the pair's constant front messages) + (its window segment
 // sum); window pairs are numbered first.

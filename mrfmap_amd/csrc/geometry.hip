@@ -16,8 +16,8 @@
 // sums are exact, so they do not depend on the order in which records
 // arrive.  The part of each length below 2^-24 is summed the same way in
 // units of 2^-48, so C is exact for every chord of at least 2^-25 and the
-// epilogue gate C > 0.1 falls as it does on the FP64 sum
-// (gvdbkernels.cu:1276-1302), however close to 0.1 C lies.
+// epilogue gate C > 0.1f (gvdbkernels.cu:408) falls as it does on the exact
+// sum, however close to 0.1f C lies.
 //
 // Window segments.  The window records are stably radix-sorted by pair, so
 // each pair's records stay in record order; the BP iteration sums each
@@ -177,8 +177,9 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
   }
 }
 
-// C in FP32 with the gate of the exact sum: where rounding to float would
-// move C across kCumLenMin, C takes the nearest float on the exact sum's side
+// C in FP32 with the gate of the exact sum (C > 0.1f, gvdbkernels.cu:408):
+// where rounding to float would move C onto kCumLenMin from above, C takes
+// the next float up, so that the float gate falls as on the exact sum
 __global__ void k_pair_finish(const uint32_t* __restrict__ acc_c, const uint32_t* __restrict__ acc_f,
                               const uint32_t* __restrict__ acc_r, uint32_t np, float* __restrict__ pair_c,
                               float* __restrict__ pair_sf) {

@@ -12,8 +12,10 @@ constexpr float kFEps = 1e-4f;         // F_EPS: probability clamp
 constexpr float kZMax = 10.0f;         // Z_MAX
 constexpr int kLookupPx = 20;          // LOOKUP_PX: sensor-model lookup cell (pixels)
 constexpr int kLookupMax = 3096;       // LOOKUP_NUM
-constexpr float kCumLenMin = 0.1f;     // epilogue gate on a camera's summed chord length: C > kCumLenMin
-constexpr double kCumLenGate = 0.1;    // the same gate on the exact sum (gvdbkernels.cu:1276-1302)
+// epilogue gate on a camera's summed chord length, C > 0.1f in float
+// (gvdbkernels.cu:408), and the same gate applied to the exact sum
+constexpr float kCumLenMin = 0.1f;
+constexpr double kCumLenGate = (double)kCumLenMin;
 constexpr float kMsgNormEps = 1e-7f;   // 0.001 * F_EPS in the message normaliser
 
 // Fixed-point scale of the integer DDA: origins and directions are kept in

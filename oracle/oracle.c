@@ -22,6 +22,9 @@
 #define GAP 0xFFFFFFFFu
 #define LOOKUP_PX 20
 #define RAN_OUT 0x80000000u
+/* the epilogue's gate on a camera's summed chord length, C > 0.1f in float
+ * (gvdbkernels.cu:408), applied to the FP64 sum */
+#define CUM_LEN_GATE ((double)0.1f)
 
 typedef __int128 i128;
 typedef struct {
@@ -842,7 +845,7 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
         const double C = k->pair_c[q];
         const double s = S[v];
         S[v] = 0.0;
-        if (C > 0.1) {
+        if (C > CUM_LEN_GATE) {
           double a = clampd(s / C);
           double m = log(a) - log(1.0 - a);
           c->B[v] = (c->B[v] + m) - k->M[v];
@@ -898,7 +901,7 @@ int orc_features(orc_ctx* c, float* X, uint32_t* vox, int64_t cap, int64_t* n_ou
       const uint32_t v = k->pair_vox[q];
       const double C = k->pair_c[q];
       pass[v] += C;
-      if (C > 0.1) {
+      if (C > CUM_LEN_GATE) {
         const double m = k->M[v];
         msum[v] += m;
         mmax[v] = nc[v] ? (m > mmax[v] ? m : mmax[v]) : m;
@@ -984,7 +987,7 @@ int orc_warm_start(orc_ctx* c, const uint32_t* vox, const double* pred, int64_t 
   double* delta = (double*)calloc((size_t)nv, sizeof(double));
   for (int ci = 0; ci < c->ncams; ++ci)
     for (int64_t q = 0; q < c->cams[ci].npairs; ++q)
-      if (c->cams[ci].pair_c[q] > 0.1) nc[c->cams[ci].pair_vox[q]] += 1;
+      if (c->cams[ci].pair_c[q] > CUM_LEN_GATE) nc[c->cams[ci].pair_vox[q]] += 1;
   for (int64_t i = 0; i < n; ++i) {
     const uint32_t v = vox[i];
     if ((int64_t)v >= nv || nc[v] == 0) {
@@ -1001,7 +1004,7 @@ int orc_warm_start(orc_ctx* c, const uint32_t* vox, const double* pred, int64_t 
   for (int ci = 0; ci < c->ncams; ++ci) {
     ocam* k = &c->cams[ci];
     for (int64_t q = 0; q < k->npairs; ++q)
-      if (k->pair_c[q] > 0.1) k->M[k->pair_vox[q]] += delta[k->pair_vox[q]];
+      if (k->pair_c[q] > CUM_LEN_GATE) k->M[k->pair_vox[q]] += delta[k->pair_vox[q]];
   }
   free(nc);
   free(delta);
