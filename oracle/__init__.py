@@ -51,7 +51,7 @@ class OrcParams(ctypes.Structure):
         ("res", ctypes.c_float), ("prior", ctypes.c_float), ("dims", ctypes.c_float * 3),
         ("lookup_n", ctypes.c_int), ("sigma_poly", ctypes.c_float * 3), ("bias_poly", ctypes.c_float * 3),
         ("activation", ctypes.c_int), ("mask_848_border", ctypes.c_int),
-        ("sigma_lookup", ctypes.c_void_p), ("bias_lookup", ctypes.c_void_p),
+        ("sigma_lookup", ctypes.c_void_p), ("bias_lookup", ctypes.c_void_p), ("brick_local_vis", ctypes.c_int),
     ]
 
 
@@ -84,6 +84,9 @@ def _bind(L):
     L.orc_ray_messages.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_double, P, P, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
     L.orc_ray_messages.restype = ctypes.c_double
+    L.orc_ray_messages_ex.argtypes = [ctypes.c_int, P, P, P, P, P, ctypes.c_double, P, P, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, P]
+    L.orc_ray_messages_ex.restype = ctypes.c_double
     L.orc_ray_messages_bruteforce.argtypes = [ctypes.c_int, P, P, ctypes.c_double, P]
     L.orc_depth_psi.argtypes = [ctypes.c_double, ctypes.c_double, P, P, ctypes.c_int]
     L.orc_depth_psi.restype = ctypes.c_double
@@ -150,10 +153,13 @@ class Oracle:
     """FP64 scalar BP over the same keyframes the GPU path sees."""
 
     def __init__(self, K, rows, cols, res, prior, dims, sigma_poly=None, bias_poly=None, lookup_n=3,
-                 activation=1, mask_848_border=1, library=None, sigma_lookup=None, bias_lookup=None):
+                 activation=1, mask_848_border=1, library=None, sigma_lookup=None, bias_lookup=None,
+                 brick_local_vis=0):
         """sigma_lookup / bias_lookup: optional per-cell tables of
         ceil(rows/20) * ceil(cols/20) * lookup_n coefficients (then every
-        keyframe must be rows x cols)."""
+        keyframe must be rows x cols).  brick_local_vis: 1 for the
+        reference's pre pass, whose visibility restarts at every brick until
+        a tau survives FP32 flush-to-zero; 0 carries it along the ray."""
         p = OrcParams()
         p.K[:] = [float(v) for v in np.asarray(K, dtype=np.float32).reshape(9)]
         p.rows, p.cols = int(rows), int(cols)
@@ -166,6 +172,7 @@ class Oracle:
         p.bias_poly[:] = [float(v) for v in list(bp) + [0.0] * (3 - len(bp))]
         p.activation = int(activation)
         p.mask_848_border = int(mask_848_border)
+        p.brick_local_vis = int(brick_local_vis)
         if sigma_lookup is not None and bias_lookup is not None:
             tab_sig = np.ascontiguousarray(sigma_lookup, dtype=np.float32).ravel()
             tab_bia = np.ascontiguousarray(bias_lookup, dtype=np.float32).ravel()
@@ -310,16 +317,19 @@ class Oracle:
 
 
 def ray_messages(nin, length, s, d, sigma_poly, bias_poly=(0.0, 0.0, 0.0), live=None, lookup_n=3,
-                 ran_out=False, psi_inf=1.0, mode=0):
-    """Outgoing messages p_i of one ray; returns (p, T)."""
+                 ran_out=False, psi_inf=1.0, mode=0, bstart=None):
+    """Outgoing messages p_i of one ray; returns (p, T).  bstart (0/1 per
+    record): brick entries for the reference's pre-pass visibility restart."""
     nin, length, s = _f64(nin), _f64(length), _f64(s)
     n = len(nin)
     live = np.ones(n, dtype=np.int32) if live is None else np.ascontiguousarray(live, dtype=np.int32)
     sg = _f64(list(sigma_poly) + [0.0] * (3 - len(sigma_poly)))
     bi = _f64(list(bias_poly) + [0.0] * (3 - len(bias_poly)))
     p = np.zeros(n, dtype=np.float64)
-    T = lib().orc_ray_messages(n, _ptr(nin), _ptr(length), _ptr(s), _ptr(live), float(d), _ptr(sg), _ptr(bi),
-                               int(lookup_n), int(bool(ran_out)), float(psi_inf), int(mode), _ptr(p))
+    bs = None if bstart is None else np.ascontiguousarray(bstart, dtype=np.int32)
+    T = lib().orc_ray_messages_ex(n, _ptr(nin), _ptr(length), _ptr(s), _ptr(live), None if bs is None else _ptr(bs),
+                                  float(d), _ptr(sg), _ptr(bi), int(lookup_n), int(bool(ran_out)), float(psi_inf),
+                                  int(mode), _ptr(p))
     return p, T
 
 

@@ -673,9 +673,36 @@ static double* scratch_buf(int n) {
   return buf;
 }
 
+/* The reference's arithmetic runs in FP32 with denormals flushed to zero
+ * (--use_fast_math, CMakeLists.txt:192): the pre pass's tau_i = (V_i (1 -
+ * n_i)) (0.398942 e^{-u^2/2} / (3 sigma)) (gvdbkernels.cu:57-72, 792-795) is
+ * exactly 0 as soon as one of its products falls below the smallest normal
+ * float, 2^-126.  This is that rule in FP64: tau_i counts as non-zero when
+ * a = 0.398942 e^{-u^2/2}, psi = a / (3 sigma), b = V (1 - n) and tau = b psi
+ * are all at least 2^-126. */
+#define FTZ_MIN 0x1p-126
+
+static int tau_nonzero_ftz(double sv, double d, const double sig[3], const double bia[3], int ln, double V,
+                           double n, double* tau) {
+  const double diff = fabs(d - (sv + poly(bia, ln, sv)));
+  const double sg3 = 3.0 * poly(sig, ln, sv);
+  const double u = diff / sg3;
+  const double a = 0.398942 * exp(-0.5 * u * u);
+  const double ps = a / sg3;
+  const double b = V * (1.0 - n);
+  *tau = b * ps;
+  return a >= FTZ_MIN && ps >= FTZ_MIN && b >= FTZ_MIN && *tau >= FTZ_MIN;
+}
+
 double orc_ray_messages(int n, const double* nin, const double* len, const double* s, const int* live, double d,
                         const double sig[3], const double bia[3], int lookup_n, int ran_out, double psi_inf,
                         int mode, double* p) {
+  return orc_ray_messages_ex(n, nin, len, s, live, NULL, d, sig, bia, lookup_n, ran_out, psi_inf, mode, p);
+}
+
+double orc_ray_messages_ex(int n, const double* nin, const double* len, const double* s, const int* live,
+                           const int* bstart, double d, const double sig[3], const double bia[3], int lookup_n,
+                           int ran_out, double psi_inf, int mode, double* p) {
   if (n <= 0) return 0.0;
   double* buf = scratch_buf(n);
   double *tau = buf, *vpsi = buf + n, *R = buf + 2 * n;
@@ -699,21 +726,52 @@ double orc_ray_messages(int n, const double* nin, const double* len, const doubl
     T += tau[i];
   }
   double tinf = 0.0;
+  int inf_term = 0;
   if (mode == 0) {
     /* gvdbkernels.cu:1196-1208: only when the walk ran out, and only if d lies beyond */
     if (ran_out) {
       double sn = s[n - 1];
-      if (d > sn + poly(bia, lookup_n, sn)) tinf = exp(L) * psi_inf;
+      inf_term = d > sn + poly(bia, lookup_n, sn);
     }
   } else {
-    tinf = exp(L) * psi_inf;
+    inf_term = 1;
   }
+  if (inf_term) tinf = exp(L) * psi_inf;
   T += tinf;
-  /* suffix sums R_i = sum_{j>i} tau_j + tinf */
-  double acc = tinf;
-  for (int i = n - 1; i >= 0; --i) {
-    R[i] = acc;
-    acc += tau[i];
+  if (mode == 0 && bstart) {
+    /* The reference's pre pass (gvdbkernels.cu:689-819) carries its
+     * visibility sum from brick to brick only once the accumulated tau is
+     * non-zero (the restore at :708-715 tests clr.x != 0): until then every
+     * brick starts again from V = 1, and T, the infinity term (:1196-1208)
+     * included, is summed with these brick-local visibilities.  The post
+     * pass above carries along the whole ray (:844-856) and takes R_i =
+     * T - sum_{j<=i} tau_j from this T. */
+    double Lp = 0.0, Tp = 0.0;
+    int nz = 0;
+    for (int i = 0; i < n; ++i) {
+      if (!live[i]) continue;
+      if (bstart[i] && !nz) Lp = 0.0;
+      Lp += len[i] * log(nin[i]);
+      double tp;
+      if (tau_nonzero_ftz(s[i], d, sig, bia, lookup_n, exp(Lp), nin[i], &tp)) {
+        Tp += tp;
+        nz = 1;
+      }
+    }
+    if (inf_term) Tp += exp(Lp) * psi_inf;
+    T = Tp;
+    double pre = 0.0;
+    for (int i = 0; i < n; ++i) {
+      pre += tau[i];
+      R[i] = T - pre;
+    }
+  } else {
+    /* suffix sums R_i = sum_{j>i} tau_j + tinf */
+    double acc = tinf;
+    for (int i = n - 1; i >= 0; --i) {
+      R[i] = acc;
+      acc += tau[i];
+    }
   }
   double P = 0.0;
   for (int i = 0; i < n; ++i) {
@@ -817,6 +875,7 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
   double* nin = (double*)malloc(sizeof(double) * (size_t)maxn);
   double* pout = (double*)malloc(sizeof(double) * (size_t)maxn);
   int* live = (int*)malloc(sizeof(int) * (size_t)maxn);
+  int* bst = (int*)malloc(sizeof(int) * (size_t)maxn);
   double res = 0.0;
   int it;
   for (it = 0; it < max_iters; ++it) {
@@ -833,10 +892,13 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
           uint32_t vox = k->rb.rec[2 * (off + i)];
           live[i] = vox != GAP;
           nin[i] = live[i] ? clampd(1.0 / (1.0 + exp(c->B[vox] - k->M[vox]))) : 1.0;
+          /* the record enters a brick: first of the ray, after a gap, or in another slot */
+          bst[i] = live[i] && (i == 0 || !live[i - 1] || (k->rb.rec[2 * (off + i - 1)] >> 9) != (vox >> 9));
         }
         double sg[3], bi[3];
         cell_polys(c, rr[5] & ~RAN_OUT, sg, bi);
-        orc_ray_messages(n, nin, k->rb.len + off, k->rb.zex + off, live, (double)d, sg, bi, ln, ro, 1.0, 0, pout);
+        orc_ray_messages_ex(n, nin, k->rb.len + off, k->rb.zex + off, live, c->p.brick_local_vis ? bst : NULL,
+                            (double)d, sg, bi, ln, ro, 1.0, 0, pout);
         for (int i = 0; i < n; ++i)
           if (live[i]) S[k->rb.rec[2 * (off + i)]] += pout[i] * k->rb.len[off + i];
       }
@@ -870,6 +932,7 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
   free(nin);
   free(pout);
   free(live);
+  free(bst);
   if (residual) *residual = res;
   return it;
 }

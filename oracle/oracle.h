@@ -44,6 +44,10 @@ typedef struct orc_params {
    * flipped image (gvdbkernels.cu:46-55); every keyframe must then be rows x cols */
   const float* sigma_lookup;
   const float* bias_lookup;
+  /* 1: the pre pass restarts visibility at every brick until a tau survives
+   * FP32 flush-to-zero, as the reference's (gvdbkernels.cu:708-715);
+   * 0: visibility carried along the whole ray in both passes */
+  int brick_local_vis;
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;
@@ -96,6 +100,14 @@ int orc_get_cum_lengths(orc_ctx* c, int cam, double* out);
 double orc_ray_messages(int n, const double* nin, const double* len, const double* s, const int* live, double d,
                         const double sig[3], const double bia[3], int lookup_n, int ran_out, double psi_inf,
                         int mode, double* p);
+/* The same with the reference's pre-pass visibility restart (mode 0 only):
+ * bstart[i] = 1 where record i enters a brick (the first record of the
+ * ray, the record after a gap, a record whose voxel lies in another brick
+ * than the previous record's).  NULL: visibility carried along the whole ray
+ * in both passes (orc_ray_messages). */
+double orc_ray_messages_ex(int n, const double* nin, const double* len, const double* s, const int* live,
+                           const int* bstart, double d, const double sig[3], const double bia[3], int lookup_n,
+                           int ran_out, double psi_inf, int mode, double* p);
 /* Brute force over all 2^n occupancy configurations of the textbook factor. */
 int orc_ray_messages_bruteforce(int n, const double* nin, const double* psi, double psi_inf, double* p);
 double orc_depth_psi(double s, double d, const double sig[3], const double bia[3], int lookup_n);
