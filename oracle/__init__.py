@@ -31,7 +31,7 @@ SOURCES = ["oracle.c", "octomap_eq.c"]
 def build(verbose=False):
     """Compile liboracle.so with gcc (IEEE double, no FMA contraction)."""
     tmp = LIB + ".tmp"
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math",
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
            *[os.path.join(HERE, s) for s in SOURCES], "-o", tmp, "-lm"]
     if verbose:
         print(" ".join(cmd))
@@ -76,6 +76,10 @@ def _bind(L):
     L.orc_perform_inference.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, f64p,
                                         ctypes.POINTER(ctypes.c_int64)]
     L.orc_continue_inference.argtypes = [P, ctypes.c_int, ctypes.c_double, f64p]
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_set_threads.restype = None
+    L.orc_get_threads.argtypes = []
+    L.orc_get_threads.restype = ctypes.c_int
     L.orc_features.argtypes = [P, P, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.orc_warm_start.argtypes = [P, P, P, ctypes.c_int64]
     L.orc_get_alphas.argtypes = [P, P]
@@ -133,7 +137,7 @@ def build_variant(replacements, out_dir, name="variant"):
     so_path = os.path.join(out_dir, name + ".so")
     with open(c_path, "w") as f:
         f.write(src)
-    cmd = ["gcc", "-O1", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-I", HERE,
+    cmd = ["gcc", "-O1", "-fPIC", "-shared", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-I", HERE,
            c_path, os.path.join(HERE, "octomap_eq.c"), "-o", so_path, "-lm"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
@@ -314,6 +318,15 @@ class Oracle:
         if self._L.orc_get_cum_lengths(self._h, int(cam), _ptr(out)) != 0:
             raise RuntimeError("orc_get_cum_lengths failed")
         return out
+
+
+def set_threads(n):
+    """Threads of the oracle's BP loop (0: OpenMP's default)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().orc_get_threads())
 
 
 def ray_messages(nin, length, s, d, sigma_poly, bias_poly=(0.0, 0.0, 0.0), live=None, lookup_n=3,

@@ -15,6 +15,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define FIX 16
 #define FIXS 65536.0
@@ -858,55 +861,95 @@ int orc_continue_inference(orc_ctx* c, int max_iters, double tol, double* residu
   return bp_iterate(c, max_iters, tol, residual, NULL);
 }
 
+/* Threads of the BP loop (0: OpenMP's default, i.e. OMP_NUM_THREADS or
+ * every core).  The rays of a camera pass are split into one contiguous block
+ * per thread; each thread sums its rays' p * l per voxel in an array of its
+ * own, and the per-pair sums S add the threads' arrays in thread order, so a
+ * result depends on the thread count only through FP64 rounding and is
+ * reproducible for a given count. */
+static int g_threads = 0;
+
+void orc_set_threads(int n) { g_threads = n < 0 ? 0 : n; }
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* Messages of rays [r0, r1) of camera k into St (p * l summed per voxel). */
+static void camera_rays(const orc_ctx* c, const ocam* k, int64_t r0, int64_t r1, double* St, double* nin,
+                        double* pout, int* live, int* bst) {
+  const int ln = c->p.lookup_n;
+  for (int64_t r = r0; r < r1; ++r) {
+    const uint32_t* rr = &k->rays[6 * r];
+    const int64_t off = rr[0];
+    const int n = (int)rr[1];
+    float d;
+    memcpy(&d, &rr[4], 4);
+    const int ro = (rr[5] & RAN_OUT) != 0;
+    for (int i = 0; i < n; ++i) {
+      uint32_t vox = k->rb.rec[2 * (off + i)];
+      live[i] = vox != GAP;
+      nin[i] = live[i] ? clampd(1.0 / (1.0 + exp(c->B[vox] - k->M[vox]))) : 1.0;
+      /* the record enters a brick: first of the ray, after a gap, or in another slot */
+      bst[i] = live[i] && (i == 0 || !live[i - 1] || (k->rb.rec[2 * (off + i - 1)] >> 9) != (vox >> 9));
+    }
+    double sg[3], bi[3];
+    cell_polys(c, rr[5] & ~RAN_OUT, sg, bi);
+    orc_ray_messages_ex(n, nin, k->rb.len + off, k->rb.zex + off, live, c->p.brick_local_vis ? bst : NULL,
+                        (double)d, sg, bi, ln, ro, 1.0, 0, pout);
+    for (int i = 0; i < n; ++i)
+      if (live[i]) St[k->rb.rec[2 * (off + i)]] += pout[i] * k->rb.len[off + i];
+  }
+}
+
 /* BP iterations from the current beliefs and messages (no reset). */
 static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, int64_t* updates_per_iter) {
-  const int ln = c->p.lookup_n;
   const int64_t nv = c->nslots * 512;
+  const int nt = orc_get_threads();
   int64_t upd = 0;
   for (int ci = 0; ci < c->ncams; ++ci)
     for (int64_t i = 0; i < c->cams[ci].rb.n; ++i) upd += c->cams[ci].rb.rec[2 * i] != GAP;
   if (updates_per_iter) *updates_per_iter = upd;
   for (int64_t v = 0; v < nv; ++v) c->A[v] = marginal(c->B[v]);
-  double* S = (double*)calloc((size_t)nv, sizeof(double));
+  /* per thread: the voxel sums and the per-ray work arrays */
+  double* Sall = (double*)calloc((size_t)nv * (size_t)nt + 1, sizeof(double));
   int maxn = 1;
   for (int ci = 0; ci < c->ncams; ++ci)
     for (int64_t r = 0; r < c->cams[ci].nrays; ++r)
       if ((int)c->cams[ci].rays[6 * r + 1] > maxn) maxn = (int)c->cams[ci].rays[6 * r + 1];
-  double* nin = (double*)malloc(sizeof(double) * (size_t)maxn);
-  double* pout = (double*)malloc(sizeof(double) * (size_t)maxn);
-  int* live = (int*)malloc(sizeof(int) * (size_t)maxn);
-  int* bst = (int*)malloc(sizeof(int) * (size_t)maxn);
+  double* dbuf = (double*)malloc(sizeof(double) * 2 * (size_t)maxn * (size_t)nt);
+  int* ibuf = (int*)malloc(sizeof(int) * 2 * (size_t)maxn * (size_t)nt);
   double res = 0.0;
   int it;
   for (it = 0; it < max_iters; ++it) {
     for (int ci = 0; ci < c->ncams; ++ci) {
       ocam* k = &c->cams[ci];
-      for (int64_t r = 0; r < k->nrays; ++r) {
-        const uint32_t* rr = &k->rays[6 * r];
-        const int64_t off = rr[0];
-        const int n = (int)rr[1];
-        float d;
-        memcpy(&d, &rr[4], 4);
-        const int ro = (rr[5] & RAN_OUT) != 0;
-        for (int i = 0; i < n; ++i) {
-          uint32_t vox = k->rb.rec[2 * (off + i)];
-          live[i] = vox != GAP;
-          nin[i] = live[i] ? clampd(1.0 / (1.0 + exp(c->B[vox] - k->M[vox]))) : 1.0;
-          /* the record enters a brick: first of the ray, after a gap, or in another slot */
-          bst[i] = live[i] && (i == 0 || !live[i - 1] || (k->rb.rec[2 * (off + i - 1)] >> 9) != (vox >> 9));
-        }
-        double sg[3], bi[3];
-        cell_polys(c, rr[5] & ~RAN_OUT, sg, bi);
-        orc_ray_messages_ex(n, nin, k->rb.len + off, k->rb.zex + off, live, c->p.brick_local_vis ? bst : NULL,
-                            (double)d, sg, bi, ln, ro, 1.0, 0, pout);
-        for (int i = 0; i < n; ++i)
-          if (live[i]) S[k->rb.rec[2 * (off + i)]] += pout[i] * k->rb.len[off + i];
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+      {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        const int64_t r0 = k->nrays * t / nt, r1 = k->nrays * (t + 1) / nt;
+        double* nin = dbuf + (size_t)t * 2 * maxn;
+        int* live = ibuf + (size_t)t * 2 * maxn;
+        camera_rays(c, k, r0, r1, Sall + (size_t)t * nv, nin, nin + maxn, live, live + maxn);
       }
       for (int64_t q = 0; q < k->npairs; ++q) {
         const uint32_t v = k->pair_vox[q];
         const double C = k->pair_c[q];
-        const double s = S[v];
-        S[v] = 0.0;
+        double s = 0.0;
+        for (int t = 0; t < nt; ++t) {  /* thread order */
+          s += Sall[(size_t)t * nv + v];
+          Sall[(size_t)t * nv + v] = 0.0;
+        }
         if (C > CUM_LEN_GATE) {
           double a = clampd(s / C);
           double m = log(a) - log(1.0 - a);
@@ -928,11 +971,9 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
       break;
     }
   }
-  free(S);
-  free(nin);
-  free(pout);
-  free(live);
-  free(bst);
+  free(Sall);
+  free(dbuf);
+  free(ibuf);
   if (residual) *residual = res;
   return it;
 }

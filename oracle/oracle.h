@@ -75,6 +75,10 @@ int orc_get_traversal(orc_ctx* c, int cam, int64_t* n_rays, int64_t* n_records, 
  * iterations run; *residual gets the last max |delta marginal|. */
 int orc_perform_inference(orc_ctx* c, const int* ids, int n_ids, int max_iters, double tol, double* residual,
                           int64_t* updates_per_iter);
+/* Threads of the BP loop (0: OpenMP's default).  Results depend on the
+ * count only through the FP64 summation order of the per-thread sums. */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 /* Iterations from the current beliefs and messages, without the per-call
  * message reset (the warm-started continuation). */
 int orc_continue_inference(orc_ctx* c, int max_iters, double tol, double* residual);

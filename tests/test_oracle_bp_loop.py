@@ -25,7 +25,7 @@ MUTANTS = {
                            ["one_camera_one_iteration"]),
     "belief_update_without_minus_Mc": ([("c->B[v] = (c->B[v] + m) - k->M[v];", "c->B[v] = c->B[v] + m;")],
                                        ["fixed_point_of_one_camera"]),
-    "gate_dropped": ([("S[v] = 0.0;\n        if (C > CUM_LEN_GATE) {", "S[v] = 0.0;\n        if (C > 0.0) {")],
+    "gate_dropped": ([("        }\n        if (C > CUM_LEN_GATE) {", "        }\n        if (C > 0.0) {")],
                      ["cum_length_gate"]),
     "jacobi_order": ([("clampd(1.0 / (1.0 + exp(c->B[vox] - k->M[vox])))",
                        "clampd(1.0 / (1.0 + exp((log(c->A[vox]) - log(1.0 - c->A[vox])) - k->M[vox])))")],
