@@ -265,6 +265,22 @@ int mrf_mlp_grad(mrf_ctx* ctx, const float* params, const float* X, const float*
 int mrf_mlp_train(mrf_ctx* ctx, float* params, const float* X, const float* t, int64_t n, const mrf_train_opts* o,
                   double* epoch_loss);
 
+/* mrf_mlp_grad without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving or training loops, harnesses that forbid hipMalloc).
+ * params [MRF_MLP_PARAMS], X [n x 32], t [n], grad [MRF_MLP_PARAMS] and loss
+ * [1] (may be NULL) are DEVICE pointers of the context's GPU; the result is
+ * bit-identical to mrf_mlp_grad on the same values.  The per-wave partial
+ * gradients live in a fixed workspace of the library (12.4 MiB per GPU, held
+ * until the process ends), so n <= MRF_MLP_NOALLOC_MAX_SAMPLES (else
+ * MRF_E_LIMIT).  The work is enqueued on `stream` (a hipStream_t; NULL: the
+ * context's stream) and not waited for.  Every call on one GPU shares that
+ * workspace: calls that overlap on one GPU (two streams, two contexts) race. */
+#define MRF_MLP_NOALLOC_MAX_SAMPLES 16384
+int mrf_mlp_grad_noalloc(mrf_ctx* ctx, const float* params, const float* X, const float* t, int64_t n, float* grad,
+                         double* loss, void* stream);
+
+
 /* Reference marginals (rows = num_bricks, cols = 512, the layout of
  * mrf_get_alphas; call after mrf_prepare) for measuring convergence to a
  * known map, e.g. an FP64 or tightly converged solution: after every

@@ -30,8 +30,10 @@ EXPORTS = [
     "mrf_set_eval_thresholds",
     "mrf_set_launch_mode", "mrf_train_opts_default", "mrf_perform_inference_warm", "mrf_features",
     "mrf_warm_predictions", "mrf_mlp_forward", "mrf_mlp_grad", "mrf_mlp_train", "mrf_set_reference", "mrf_get_trace",
+    "mrf_mlp_grad_noalloc",
 ]
 MLP_FEATURES, MLP_PARAMS = 32, 6337
+MLP_NOALLOC_MAX_SAMPLES = 16384
 
 
 class MrfParams(ctypes.Structure):
@@ -137,6 +139,7 @@ def load():
     L.mrf_mlp_forward.argtypes = [P, P, P, ctypes.c_int64, P]
     L.mrf_mlp_grad.argtypes = [P, P, P, P, ctypes.c_int64, P, ctypes.POINTER(ctypes.c_double)]
     L.mrf_mlp_train.argtypes = [P, P, P, P, ctypes.c_int64, ctypes.POINTER(MrfTrainOpts), P]
+    L.mrf_mlp_grad_noalloc.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P]
     L.mrf_last_error.argtypes = [P]
     L.mrf_last_error.restype = ctypes.c_char_p
     _lib = L

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -158,6 +159,7 @@ struct mrf_ctx {
   double ref_stop = 0.0;
   unsigned long long* d_rerr = nullptr;
   std::vector<double> trace_resid, trace_err;  // per iteration of the last call
+  void* scratch = nullptr;  // this device's copy of the fixed workspace (mrf_mlp_grad_noalloc)
   float* d_mlp = nullptr;
   float *d_snapB = nullptr, *d_snapM = nullptr;
   size_t cap_snapB = 0, cap_snapM = 0;
@@ -169,6 +171,39 @@ struct mrf_ctx {
 };
 
 namespace {
+
+// Fixed workspace of mrf_mlp_grad_noalloc, the entry point that allocates
+// nothing: the per-wave partial gradients (kMlpParams floats per wave) and
+// loss sums (one double per wave) of k_mlp_grad for at most
+// kNoAllocMaxSamples samples, i.e. mlp_grad_waves(kNoAllocMaxSamples) = 512
+// waves.  Every GPU the library runs on holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device (mrf_ctx::scratch), never through the array's name.
+constexpr uint32_t kNoAllocMaxSamples = 16384;
+constexpr size_t kNoAllocWaves = 512;
+constexpr size_t kNoAllocPartBytes = kNoAllocWaves * (size_t)kMlpParams * sizeof(float);
+constexpr size_t kNoAllocBytes = kNoAllocPartBytes + kNoAllocWaves * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of the workspace, looked up once per device with
+// that device current (mrf_create runs on the device its context uses).
+hipError_t scratch_address(int dev, void** out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 int fail(mrf_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -1105,6 +1140,14 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
     delete c;
     return MRF_E_HIP;
   }
+  {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || scratch_address(dev, &c->scratch) != hipSuccess) {
+      (void)hipGetLastError();
+      mrf_destroy(c);
+      return MRF_E_HIP;
+    }
+  }
   c->h_slot.assign((size_t)c->nbricks, -1);
   if (const char* env = getenv("MRF_NO_GRAPH"))
     if (env[0] && env[0] != '0') c->launch_mode = MRF_LAUNCH_TIMED;
@@ -1506,6 +1549,28 @@ int mrf_mlp_grad(mrf_ctx* c, const float* params, const float* X, const float* t
   HIPCHK(c, hipMemcpyAsync(&l, c->d_tLoss, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (loss) *loss = l;
+  return MRF_OK;
+}
+
+int mrf_mlp_grad_noalloc(mrf_ctx* c, const float* params, const float* X, const float* t, int64_t n, float* grad,
+                         double* loss, void* stream) {
+  if (!c || !params || !X || !t || !grad || n <= 0) return MRF_E_INVALID;
+  if (n > (int64_t)kNoAllocMaxSamples)
+    return fail(c, MRF_E_LIMIT, "mlp_grad_noalloc: more than " + std::to_string(kNoAllocMaxSamples) +
+                                    " samples (size of the fixed workspace)");
+  const uint32_t nw = mlp_grad_waves((uint32_t)n);
+  if (nw > kNoAllocWaves) return fail(c, MRF_E_LIMIT, "mlp_grad_noalloc: workspace too small");
+  if (!c->scratch) return fail(c, MRF_E_STATE, "mlp_grad_noalloc: no workspace on this device");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  // every wave's partials are written by k_mlp_grad before k_mlp_reduce reads
+  // them, so nothing of the workspace needs clearing; the loss output is
+  // cleared by a kernel (no memset node under graph capture)
+  float* part = (float*)c->scratch;
+  double* lpart = (double*)((unsigned char*)c->scratch + kNoAllocPartBytes);
+  if (loss) launch_zero_words(loss, sizeof(double), s);
+  launch_mlp_grad(params, X, t, nullptr, 0, (uint32_t)n, 1.0f / (float)n, part, lpart, s);
+  launch_mlp_reduce(part, lpart, nw, grad, loss, 1.0 / (double)n, s);
+  HIPCHK(c, hipGetLastError());
   return MRF_OK;
 }
 
