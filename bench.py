@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="real")
     ap.add_argument("--msg", default="f32", choices=["f32", "f16", "bf16", "mixed-f16", "mixed-bf16"])
+    ap.add_argument("--vis", default="default", choices=["default", "carry", "brick-local"],
+                    help="pre-pass visibility: the library's default, carried along the ray, or restarted "
+                         "at every brick until a tau survives flush-to-zero (the reference's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tol", type=float, default=3e-5, help="residual of the reference run")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged run (profiling)")
@@ -52,23 +55,52 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(wl):
+def host_info():
+    """CPU model and core counts of the host running the CPU legs."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(wl, vis):
     """FP64 scalar oracle on keyframes 0 and 6 of the same workload, every 4th
-    pixel row; one BP iteration, timed on one host core."""
+    pixel row; one BP iteration, timed on one host thread and on all the
+    threads OpenMP gives it (OMP_NUM_THREADS, else every core)."""
     import oracle
 
     p, sub = _sample(wl)
     o = oracle.Oracle(p.K, p.rows, p.cols, p.res, p.prior, p.dims, p.const_sigmasq_poly, p.const_bias_poly,
-                      lookup_n=p.lookup_n, activation=p.activation)
+                      lookup_n=p.lookup_n, activation=p.activation, brick_local_vis=vis)
     for pose, K, d in sub:
         o.add_camera(pose, d, K)
     o.prepare()
-    t0 = time.perf_counter()
-    st = o.perform_inference(1)
-    dt = time.perf_counter() - t0
-    return {"value": st["updates_per_iter"] / dt, "unit": "ray-msg updates/s", "cores": 1, "kind": "port",
+    rates = {}
+    for nt in (1, 0):
+        oracle.set_threads(nt)
+        threads = oracle.get_threads()
+        t0 = time.perf_counter()
+        st = o.perform_inference(1)
+        rates[threads if nt else "all"] = (st["updates_per_iter"] / (time.perf_counter() - t0), threads)
+    oracle.set_threads(0)
+    v1, _ = rates[1]
+    va, nta = rates["all"]
+    return {"value": va, "unit": "ray-msg updates/s", "cores": nta, "kind": "port", "value_1_core": v1,
+            "host": host_info(),
             "sample": f"FP64 oracle BP, 1 iteration over keyframes 0 and 6 of '{wl}' with every 4th pixel row "
-                      f"({st['updates_per_iter']} updates, {dt:.2f} s)"}
+                      f"({st['updates_per_iter']} updates; {st['updates_per_iter'] / v1:.2f} s on 1 thread, "
+                      f"{st['updates_per_iter'] / va:.2f} s on {nta})"}
 
 
 def _sample(wl, keyframes=(0, 6), row_step=4):
@@ -177,6 +209,8 @@ def main():
            "mixed-bf16": _lib.MSG_MIXED_BF16}[a.msg]
     t0 = time.perf_counter()
     p, frames, _ = synth.make_workload(a.workload, msg_dtype=msg)
+    if a.vis != "default":
+        p.brick_local_vis = _lib.VIS_CARRY if a.vis == "carry" else _lib.VIS_BRICK_LOCAL
     t_gen = time.perf_counter() - t0
     print(f"[bench] generated {len(frames)} keyframes in {t_gen:.1f}s", file=sys.stderr, flush=True)
 
@@ -224,9 +258,33 @@ def main():
     elif a.launch != "timed":
         sys.exit("--no-compare needs --launch timed: the kernel durations come from the timed mode")
     timed = st if a.launch == "timed" else st_other
+    # the reference configuration's call: perform_inference with max_iters 3
+    # (coffin_world_640.yaml:24), messages reset at the call; graph launches
+    g.set_launch_mode(modes["graph"])
+    calls3 = []
+    for _ in range(3):
+        t3 = time.perf_counter()
+        s3 = g.perform_inference(max_iters=3)
+        calls3.append((time.perf_counter() - t3) * 1e3)
+    call3_ms = sorted(calls3)[1]
     g.close()  # one context at a time: the convergence runs below build their own
     if ref is not None:
         ttc = time_to_converged(p, frames, ref, a.legs.split(","))
+    resid_leg = None
+    if not a.no_converge:
+        # the residual criterion of the 640x480 configuration: from keyframes
+        # added (geometry included) until the max marginal change of an
+        # iteration is below 1e-4 (FP64 reduction on the device)
+        g = build()
+        t0 = time.perf_counter()
+        g.prepare()
+        t_geom = (time.perf_counter() - t0) * 1e3
+        sr = g.perform_inference(max_iters=400, tol=1e-4)
+        g.close()
+        resid_leg = {"tol": 1e-4, "iters": sr["iters"], "reached": bool(sr["converged"]),
+                     "ms": t_geom + sr["t_total_ms"], "geometry_ms": t_geom}
+        print(f"[bench] time to residual 1e-4: {sr['iters']} iters {resid_leg['ms']:.1f} ms", file=sys.stderr,
+              flush=True)
     n_upd = st["updates_per_iter"]
     ncam = len(frames)
     counts = {}
@@ -237,20 +295,42 @@ def main():
     value = n_upd * a.steps / dt
     launches = a.steps * ncam
     avg_launch_ms = timed["t_ray_kernel_ms"] / launches
-    # ray kernel, algorithmic bytes: every incidence reads its 8 B record and
-    # its pair's voxel->ray message; a window incidence also reads its 4 B
-    # segment position and writes its ray->voxel message.  Front records of
-    # chunks whose sums stop changing are read in the first two iterations of
-    # a call only; the average over the K timed launches of a camera counts them so.
+    # ray kernels, algorithmic bytes: every incidence reads its 8 B record and
+    # its pair's voxel->ray message; a window incidence also writes its
+    # ray->voxel message (its 4 B segment position is read by k_seg_sum and
+    # charged there).  Front records of chunks whose sums stop changing are
+    # read in the first two iterations of a call only; the average over the K
+    # timed launches of a camera counts them so.
     in_b = 4 if msg in (_lib.MSG_F32, _lib.MSG_MIXED_F16, _lib.MSG_MIXED_BF16) else 2
     out_b = 4 if msg == _lib.MSG_F32 else 2
     n_win = st["window_updates_per_iter"]
     n_static = st["static_front_updates"]
-    front_b, window_b = 8 + in_b, 8 + in_b + 4 + out_b
+    front_b, window_b = 8 + in_b, 8 + in_b + out_b
     front_reads = (n_upd - n_win - n_static) + n_static * min(a.steps, 2) / a.steps
     bytes_per_launch = (front_reads * front_b + n_win * window_b) / ncam
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(a)
+    # per kernel of a camera pass, each charged with the bytes it moves itself
+    # (the same per-update figures; k_seg_sum reads each window position's
+    # 4 B segment index and its message and writes one 4 B sum per window
+    # pair (upper bound: one per pair); k_seg_update reads 28 B and writes
+    # 8 B per window pair)
+    npairs = st["pairs_per_iter"]
+    per_kernel = {}
+    kb = {
+        "k_front_sums": (timed["t_front_ms"], front_reads * front_b),
+        "k_ray_window": (timed["t_window_ms"], n_win * (8 + in_b + out_b)),
+        "k_seg_sum": (timed["t_seg_sum_ms"], n_win * (4 + out_b) + 4.0 * min(npairs, n_win)),
+        "k_seg_update": (timed["t_seg_update_ms"], None),
+        "k_pair_msg": (timed["t_pair_msg_ms"], None),
+    }
+    for name, (tms, alg) in kb.items():
+        ms = tms / launches
+        ent = {"avg_launch_ms": ms}
+        if alg is not None:
+            gbs = alg / ncam / (ms * 1e-3) / 1e9
+            ent.update({"alg_bytes_per_launch": alg / ncam, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS})
+        per_kernel[name] = ent
     other_s = f"{dt_other * 1e3:.1f} ms" if dt_other is not None else "not run"
     print(f"[bench] {a.steps} steps ({a.launch}) in {dt * 1e3:.1f} ms, ({other}) {other_s}; "
           f"ray kernels {timed['t_ray_kernel_ms']:.1f} ms, epilogue {timed['t_epilogue_ms']:.1f} ms",
@@ -258,7 +338,7 @@ def main():
 
     cpu = comp = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(a.workload)
+        cpu = cpu_baseline(a.workload, p.brick_local_vis)
         print(f"[bench] cpu baseline {cpu['value']:.3e} updates/s", file=sys.stderr, flush=True)
         comp = cpu_comparator(a.workload)
         print(f"[bench] cpu comparator {comp['value']:.3e} rays/s", file=sys.stderr, flush=True)
@@ -291,7 +371,15 @@ def main():
             "autotune_ms": st["t_autotune_ms"],
             "launch": a.launch,
             f"ms_per_step_{other}": dt_other * 1e3 / a.steps if dt_other is not None else None,
+            "pre_pass_visibility": "brick-local" if p.brick_local_vis == _lib.VIS_BRICK_LOCAL else "carry",
         },
+        "window_updates_per_s": n_win * a.steps / dt,
+        "static_front_share": n_static / n_upd if n_upd else 0.0,
+        "call_3_iterations": {"ms": call3_ms, "updates_per_s": 3 * n_upd / (call3_ms * 1e-3),
+                              "note": "perform_inference(max_iters=3) from its per-call message reset, graph "
+                                      "launches, median of 3 calls: every pair's ln n is recomputed in the first "
+                                      "two iterations of a call"},
+        "time_to_residual": resid_leg,
         "time_to_converged": ttc,
         "reference_run": {"iters": conv["iters"], "residual": conv["residual"], "tol": a.tol,
                           "ms": conv["t_total_ms"]},
@@ -310,7 +398,8 @@ def main():
                                f"{'timed steps' if a.launch == 'timed' else 'comparison steps'}",
             "alg_bytes_per_launch": bytes_per_launch,
             "alg_bytes_per_front_update": front_b,
-            "alg_bytes_per_window_update": window_b,
+            "alg_bytes_per_window_update": 8 + in_b + out_b,
+            "per_kernel": per_kernel,
         },
         "cpu_baseline": cpu,
         "cpu_comparator": comp,

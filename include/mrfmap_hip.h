@@ -64,6 +64,17 @@ extern "C" {
 #define MRF_MSG_MIXED_F16 3  /* ln n FP32, p*l FP16 */
 #define MRF_MSG_MIXED_BF16 4 /* ln n FP32, p*l BF16 */
 
+/* Visibility of the pre pass.  The reference's pre pass runs once per active
+ * brick and restores its visibility carry only when the accumulated tau is
+ * non-zero (gvdbkernels.cu:708-715); its FP32 arithmetic flushes denormals
+ * (--use_fast_math), so in front of a surface, where every tau underflows,
+ * each brick starts again from V = 1, and T and the infinity term are summed
+ * with these brick-local visibilities.  The post pass always carries its
+ * visibility along the whole ray (gvdbkernels.cu:844-856). */
+#define MRF_VIS_CARRY 0       /* the pre pass carries its visibility along the whole ray, as the post pass */
+#define MRF_VIS_BRICK_LOCAL 1 /* the reference's: restart at every brick entry until a tau reaches 2^-126
+                                 (the default of mrf_params_default) */
+
 typedef struct mrf_ctx mrf_ctx;
 
 /* Mirrors gvdb_params_t (include/mrfmap/GVDBParams.h:11-215). */
@@ -83,6 +94,7 @@ typedef struct mrf_params {
   int activation;              /* MRF_ACT_* */
   int msg_dtype;               /* MRF_MSG_* */
   int mask_848_border;         /* blank 24/20 px borders of 848x480 frames (GVDBImage.cpp:35-40) */
+  int brick_local_vis;         /* MRF_VIS_*: visibility of the pre pass (the sum T of a ray's taus) */
 } mrf_params;
 
 /* Per-call statistics of mrf_perform_inference. */
@@ -120,6 +132,11 @@ typedef struct mrf_stats {
   int64_t front_slots;         /* record slots of the sliced front regions (each slice padded to its
                                   longest ray), summed over the cameras */
   int64_t window_slots;        /* the same for the window regions */
+  /* summed device times of each kernel of the camera passes, -1 unless the
+   * launch mode is MRF_LAUNCH_TIMED: voxel->ray messages (k_pair_msg), front
+   * visibility sums (k_front_sums), window passes (k_ray_window), segment
+   * sums (k_seg_sum), per-pair update (k_seg_update) */
+  double t_pair_msg_ms, t_front_ms, t_window_ms, t_seg_sum_ms, t_seg_update_ms;
 } mrf_stats;
 
 /* gvdb_params_t() defaults (GVDBParams.h:32-55). */
@@ -279,7 +296,6 @@ int mrf_mlp_train(mrf_ctx* ctx, float* params, const float* X, const float* t, i
 #define MRF_MLP_NOALLOC_MAX_SAMPLES 16384
 int mrf_mlp_grad_noalloc(mrf_ctx* ctx, const float* params, const float* X, const float* t, int64_t n, float* grad,
                          double* loss, void* stream);
-
 
 /* Reference marginals (rows = num_bricks, cols = 512, the layout of
  * mrf_get_alphas; call after mrf_prepare) for measuring convergence to a

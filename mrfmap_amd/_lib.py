@@ -16,6 +16,7 @@ ERRORS = {-1: "MRF_E_INVALID", -2: "MRF_E_HIP", -3: "MRF_E_NOMEM", -4: "MRF_E_SH
 ROW_MAJOR, COL_MAJOR = 0, 1
 ACT_ENDPOINT, ACT_FULL_RAY = 0, 1
 MSG_F32, MSG_F16, MSG_BF16, MSG_MIXED_F16, MSG_MIXED_BF16 = 0, 1, 2, 3, 4
+VIS_CARRY, VIS_BRICK_LOCAL = 0, 1
 # ray-kernel schedules (records loaded ahead in the front loop, in the window passes)
 LAUNCH_GRAPH, LAUNCH_TIMED = 0, 1
 RAY_SCHEDULES = {0: (8, 4), 1: (8, 8), 2: (16, 4), 3: (16, 8), 4: (4, 4), 5: (16, 16)}
@@ -44,7 +45,7 @@ class MrfParams(ctypes.Structure):
         ("lookup_n", ctypes.c_int), ("const_sigmasq_poly", ctypes.c_float * 3),
         ("const_bias_poly", ctypes.c_float * 3), ("depth_sigma_lookup", ctypes.c_void_p),
         ("depth_bias_lookup", ctypes.c_void_p), ("activation", ctypes.c_int), ("msg_dtype", ctypes.c_int),
-        ("mask_848_border", ctypes.c_int),
+        ("mask_848_border", ctypes.c_int), ("brick_local_vis", ctypes.c_int),
     ]
 
 
@@ -59,6 +60,8 @@ class MrfStats(ctypes.Structure):
         ("warm_voxels", ctypes.c_int64), ("t_warm_ms", ctypes.c_double), ("ref_error", ctypes.c_double),
         ("static_front_updates", ctypes.c_int64),
         ("front_slots", ctypes.c_int64), ("window_slots", ctypes.c_int64),
+        ("t_pair_msg_ms", ctypes.c_double), ("t_front_ms", ctypes.c_double), ("t_window_ms", ctypes.c_double),
+        ("t_seg_sum_ms", ctypes.c_double), ("t_seg_update_ms", ctypes.c_double),
     ]
 
     def as_dict(self):

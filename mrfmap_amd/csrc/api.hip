@@ -69,6 +69,8 @@ struct Camera {
   uint32_t* d_chunk_off = nullptr; // nslices + 1: first front chunk of each slice
   uint4* d_chunk_desc = nullptr;
   uint32_t nchunks = 0;
+  uint64_t* d_bstart = nullptr;    // MRF_VIS_BRICK_LOCAL: brick-entry word per window row of every slice
+  uint8_t* d_tail = nullptr;       // MRF_VIS_BRICK_LOCAL: per lane, front records in the first window brick
 };
 
 }  // namespace
@@ -455,6 +457,8 @@ void free_camera_lists(Camera& cam) {
   dfree(&cam.d_tile_h0);
   dfree(&cam.d_chunk_off);
   dfree(&cam.d_chunk_desc);
+  dfree(&cam.d_bstart);
+  dfree(&cam.d_tail);
   cam.nchunks = 0;
   cam.nrays = 0;
   cam.nslices = 0;
@@ -535,6 +539,16 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if ((rc = dalloc(c, &cam.d_inc, nrec))) return rc;
   launch_reslice(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
                  (uint32_t)nrec_f, cam.d_inc, s);
+  if (c->p.brick_local_vis == MRF_VIS_BRICK_LOCAL) {
+    // brick entries of the window records, while the records hold voxel ids;
+    // rows past the last slice read as zero words (the window kernel's
+    // batches may look up to 16 rows past a slice's longest ray)
+    const size_t wrows = (size_t)(nrec_w / kSlice);
+    if ((rc = dalloc(c, &cam.d_bstart, wrows + 32)) || (rc = dalloc(c, &cam.d_tail, npix))) return rc;
+    launch_zero_words(cam.d_bstart + wrows, 32 * sizeof(uint64_t), s);
+    launch_brick_starts(cam.d_front, cam.d_lanes, cam.d_len_w, ntiles, cam.d_inc, (uint32_t)nrec_f, cam.d_bstart,
+                        cam.d_tail, s);
+  }
   // front chunks: ceil(len_f / kChunk) per slice that has window records
   if ((rc = dalloc(c, &cam.d_chunk_off, (size_t)ntiles + 1))) return rc;
   launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
@@ -799,6 +813,9 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.lookup_n = c->p.lookup_n;
   a.msg_type = c->p.msg_dtype;
   a.ray_sched = c->ray_sched < 0 ? kSchedF8W4 : c->ray_sched;
+  a.brick_local = c->p.brick_local_vis == MRF_VIS_BRICK_LOCAL;
+  a.bstart = cam.d_bstart;
+  a.tail = cam.d_tail;
   return a;
 }
 
@@ -899,21 +916,38 @@ int evaluate(mrf_ctx* c, const CamGeom& cg, const float* h_depth, const float* d
 }
 
 // One BP iteration on c->stream: every camera's pass (Gauss-Seidel), then
-// the FP64 residual into d_total[1].  ev (3 per camera), when given, times
-// each camera's ray kernels and epilogue; a captured sequence records no
-// events (the runtime cannot time events recorded by graph nodes).
+// the FP64 residual into d_total[1].  ev (kCamEvents per camera), when given,
+// times each of a camera's kernels (pair messages, front sums, window,
+// segment sums, update); a captured sequence records no events (the runtime
+// cannot time events recorded by graph nodes).
+constexpr int kCamEvents = 6;
+
 hipError_t enqueue_iteration(mrf_ctx* c, const hipEvent_t* ev) {
   const size_t ncam = c->cams.size();
   const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
   hipError_t e;
   for (size_t ci = 0; ci < ncam; ++ci) {
     const CamPass a = cam_pass(c, ci);
+    const hipEvent_t* ec = ev ? ev + kCamEvents * ci : nullptr;
+    if (ec && (e = hipEventRecord(ec[0], c->stream)) != hipSuccess) return e;
     launch_pair_msgs(a, c->stream);
-    if (ev && (e = hipEventRecord(ev[3 * ci + 0], c->stream)) != hipSuccess) return e;
-    launch_ray_messages(a, c->stream);
-    if (ev && (e = hipEventRecord(ev[3 * ci + 1], c->stream)) != hipSuccess) return e;
-    launch_epilogue(a, c->stream);
-    if (ev && (e = hipEventRecord(ev[3 * ci + 2], c->stream)) != hipSuccess) return e;
+    if (ec && (e = hipEventRecord(ec[1], c->stream)) != hipSuccess) return e;
+    if (ec) {
+      launch_ray_messages(a, c->stream, 1);
+      if ((e = hipEventRecord(ec[2], c->stream)) != hipSuccess) return e;
+      launch_ray_messages(a, c->stream, 2);
+    } else {
+      launch_ray_messages(a, c->stream);
+    }
+    if (ec && (e = hipEventRecord(ec[3], c->stream)) != hipSuccess) return e;
+    if (ec) {
+      launch_epilogue(a, c->stream, 1);
+      if ((e = hipEventRecord(ec[4], c->stream)) != hipSuccess) return e;
+      launch_epilogue(a, c->stream, 2);
+    } else {
+      launch_epilogue(a, c->stream);
+    }
+    if (ec && (e = hipEventRecord(ec[5], c->stream)) != hipSuccess) return e;
   }
   launch_zero_words(c->d_total + 1, 8, c->stream);  // a kernel, not a memset: the sequence may be captured
   launch_residual(c->d_B, c->d_A, nvox, (unsigned long long*)(c->d_total + 1), c->stream);
@@ -1094,13 +1128,15 @@ void mrf_params_default(mrf_params* p) {
   p->activation = MRF_ACT_ENDPOINT;
   p->msg_dtype = MRF_MSG_F32;
   p->mask_848_border = 1;
+  p->brick_local_vis = MRF_VIS_BRICK_LOCAL;  // the reference's pre pass
 }
 
 int mrf_create(const mrf_params* p, mrf_ctx** out) {
   if (!p || !out) return MRF_E_INVALID;
   *out = nullptr;
   if (!(p->res > 0) || !(p->prior > 0 && p->prior < 1) || (p->lookup_n != 2 && p->lookup_n != 3) ||
-      p->activation < 0 || p->activation > 1 || p->msg_dtype < 0 || p->msg_dtype > 4)
+      p->activation < 0 || p->activation > 1 || p->msg_dtype < 0 || p->msg_dtype > 4 ||
+      (p->brick_local_vis != MRF_VIS_CARRY && p->brick_local_vis != MRF_VIS_BRICK_LOCAL))
     return MRF_E_INVALID;
   int cnt = 0;
   if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) {
@@ -1324,8 +1360,8 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   // the iteration is replayed from a captured graph unless per-kernel timing is asked for
   if (c->launch_mode == MRF_LAUNCH_GRAPH) ensure_graph(c);
   const bool graph = c->launch_mode == MRF_LAUNCH_GRAPH && c->gexec != nullptr;
-  // events: [0, 1] around the iteration, then 3 per camera unless graph-launched
-  std::vector<hipEvent_t> ev(graph ? 2 : 2 + 3 * ncam, nullptr);
+  // events: [0, 1] around the iteration, then kCamEvents per camera unless graph-launched
+  std::vector<hipEvent_t> ev(graph ? 2 : 2 + kCamEvents * ncam, nullptr);
   struct Events {
     std::vector<hipEvent_t>& v;
     ~Events() {
@@ -1335,6 +1371,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   } ev_guard{ev};
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
   double tray = 0, tepi = 0, titer = 0;
+  double tk[kCamEvents - 1] = {0, 0, 0, 0, 0};  // pair messages, front sums, window, segment sums, update
   int it = 0;
   double resid = 0;
   bool conv = false;
@@ -1373,9 +1410,14 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
     float ms = 0;
     if (!graph) {
       for (size_t ci = 0; ci < ncam; ++ci) {
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[2 + 3 * ci + 0], ev[2 + 3 * ci + 1]));
+        const hipEvent_t* ec = ev.data() + 2 + kCamEvents * ci;
+        for (int k = 0; k + 1 < kCamEvents; ++k) {
+          HIPCHK(c, hipEventElapsedTime(&ms, ec[k], ec[k + 1]));
+          tk[k] += ms;
+        }
+        HIPCHK(c, hipEventElapsedTime(&ms, ec[1], ec[3]));
         tray += ms;
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[2 + 3 * ci + 1], ev[2 + 3 * ci + 2]));
+        HIPCHK(c, hipEventElapsedTime(&ms, ec[3], ec[5]));
         tepi += ms;
       }
     }
@@ -1428,6 +1470,11 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   s.residual = resid;
   s.t_ray_kernel_ms = graph ? -1.0 : tray;
   s.t_epilogue_ms = graph ? -1.0 : tepi;
+  s.t_pair_msg_ms = graph ? -1.0 : tk[0];
+  s.t_front_ms = graph ? -1.0 : tk[1];
+  s.t_window_ms = graph ? -1.0 : tk[2];
+  s.t_seg_sum_ms = graph ? -1.0 : tk[3];
+  s.t_seg_update_ms = graph ? -1.0 : tk[4];
   s.t_iters_ms = titer;
   s.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   s.ray_sched = c->ray_sched;

@@ -235,6 +235,52 @@ __global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, R
   lanes[ray] = r;
 }
 
+// Brick entries of the window records, one wavefront per slice, while the
+// records still hold voxel ids (slot * 512 + voxel in the brick): record k
+// of lane l enters a brick when it is a voxel record and the previous record
+// of its ray is a gap record, lies in another slot, or does not exist.  The
+// 64 lanes' flags of window row k form one 64-bit word.  tail[lane]: when
+// the first window record continues the brick of the last front record, the
+// number of front records since that brick's entry (at most 22, the longest
+// crossing of a brick), else 0.
+__global__ __launch_bounds__(256) void k_brick_starts(const FrontRec* __restrict__ front,
+                                                      const RayRec* __restrict__ lanes,
+                                                      const uint32_t* __restrict__ len_w, uint32_t nslices,
+                                                      const uint2* __restrict__ inc, uint32_t base_w,
+                                                      uint64_t* __restrict__ bstart, uint8_t* __restrict__ tail) {
+  const uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (t >= nslices) return;  // uniform per wavefront
+  const uint32_t l = threadIdx.x & 63u, ray = t * kSlice + l;
+  const FrontRec f = front[ray];
+  const RayRec r = lanes[ray];
+  const uint32_t sw = __builtin_amdgcn_readfirstlane(r.off_w - l);  // the slice's window region
+  const uint32_t row0 = (sw - base_w) / kSlice;
+  const uint32_t lw = len_w[t];
+  const uint32_t last = f.nf ? inc[f.off + (size_t)(f.nf - 1) * kSlice].x : kGapVox;
+  uint32_t prev = last;
+  for (uint32_t k = 0; k < lw; ++k) {  // padding records are gap records
+    const uint32_t cur = inc[sw + (size_t)k * kSlice + l].x;
+    const bool bs = cur != kGapVox && (prev == kGapVox || (prev >> kBrickLog * 3) != (cur >> kBrickLog * 3));
+    const unsigned long long m = __ballot(bs);
+    if (l == 0) bstart[row0 + k] = m;
+    prev = cur;
+  }
+  uint32_t tl = 0;
+  if (f.nf && r.nw) {
+    const uint32_t w0 = inc[sw + l].x;
+    if (w0 != kGapVox && last != kGapVox && (w0 >> kBrickLog * 3) == (last >> kBrickLog * 3)) {
+      uint32_t k = f.nf - 1;
+      while (k > 0) {
+        const uint32_t pv = inc[f.off + (size_t)(k - 1) * kSlice].x;
+        if (pv == kGapVox || (pv >> kBrickLog * 3) != (w0 >> kBrickLog * 3)) break;
+        --k;
+      }
+      tl = f.nf - k;
+    }
+  }
+  tail[ray] = (uint8_t)tl;
+}
+
 // Front chunks: a slice with window records needs ceil(len_f / kChunk) of
 // them, a slice without none.
 // (cnt[nslices] = 0, so that an exclusive scan of nslices + 1 counts ends
@@ -481,6 +527,56 @@ __device__ __forceinline__ void win_pass1(const WinRay& r, uint32_t k0, uint32_t
   }
 }
 
+// Pass 1 of the reference's pre pass with its flush-to-zero arithmetic
+// (MRF_VIS_BRICK_LOCAL): a tau is non-zero there only when a = 0.398942
+// e^{-u^2/2}, psi = a / (3 sigma), b = V (1 - n) and b psi are all at least
+// the smallest normal float (gvdbkernels.cu:57-72, 792-795).  Until the
+// first such tau the visibility sum restarts at every brick entry (bit l of
+// the row's word in bsw; the restore at gvdbkernels.cu:708-715 tests the
+// accumulated tau), and only such taus enter T.  The terms are those of
+// win_terms, so a ray whose restarts change nothing gets the T of pass 1.
+constexpr float kFtzMin = 0x1p-126f;
+
+template <int UW>
+__device__ __forceinline__ void win_pass1_bl(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
+                                             const float (&x)[UW], const uint64_t* __restrict__ bsw, uint32_t l,
+                                             Kahan& kL, Kahan& kS, Kahan& kT, bool& nz) {
+#pragma unroll
+  for (int u = 0; u < UW; ++u) {
+    const bool valid = k0 + u < nw;
+    const bool restart = valid && !nz && ((bsw[k0 + u] >> l) & 1ull) != 0ull;
+    kL.s = restart ? 0.0f : kL.s;
+    kL.c = restart ? 0.0f : kL.c;
+    const float lnn = x[u];
+    const float a = ln[u] * lnn;
+    const float s = r.z0 + kS.at(ln[u]) * r.inv;
+    const float diff = fabsf(r.d - (s + r.bia(s)));
+    const float inv = __builtin_amdgcn_rcpf(3.0f * r.sig(s));
+    const float uu = diff * inv;
+    const float ap = 0.398942f * __expf(-0.5f * uu * uu);
+    const float ps = ap * inv;  // depth_psi's product order
+    const float V = __expf(kL.at(a));
+    const float nn = __expf(lnn);
+    const float b = V * (1.0f - nn);
+    const bool counted = valid && lnn != 0.0f && ap >= kFtzMin && ps >= kFtzMin && b >= kFtzMin &&
+                         b * ps >= kFtzMin;
+    kahan_add_if(kT, counted, (V * ps) * (1.0f - nn));
+    nz = nz || counted;
+    kahan_add_if(kL, valid, a);
+    kahan_add_if(kS, valid, ln[u]);
+  }
+}
+
+template <int UW, bool BL>
+__device__ __forceinline__ void win_pass1_any(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
+                                              const float (&x)[UW], const uint64_t* __restrict__ bsw, uint32_t l,
+                                              Kahan& kL, Kahan& kS, Kahan& kT, bool& nz) {
+  if constexpr (BL)
+    win_pass1_bl<UW>(r, k0, nw, ln, x, bsw, l, kL, kS, kT, nz);
+  else
+    win_pass1<UW>(r, k0, nw, ln, x, kL, kS, kT);
+}
+
 template <int MO, int UW>
 __device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
                                           const float (&x)[UW], float T, Rsrc ro, uint32_t lane_off, Kahan& kL,
@@ -505,7 +601,7 @@ __device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t
   }
 }
 
-template <int MI, int MO, int UW>
+template <int MI, int MO, int UW, bool BL>
 __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ lanes, const uint32_t* __restrict__ len_w,
                                                     uint32_t nslices, const uint32_t* __restrict__ chunk_off,
                                                     const float* __restrict__ partial, const uint2* __restrict__ inc,
@@ -513,7 +609,9 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
                                                     const typename Msg<MI>::T* __restrict__ Xi, uint32_t npairs,
                                                     typename Msg<MO>::T* __restrict__ Xo,
                                                     const float* __restrict__ sigt, const float* __restrict__ biat,
-                                                    int lookup_n) {
+                                                    int lookup_n, const FrontRec* __restrict__ front,
+                                                    const uint64_t* __restrict__ bstart,
+                                                    const uint8_t* __restrict__ tail) {
   typedef typename Msg<MI>::T TI;
   typedef typename Msg<MO>::T TO;
   // one wave (one slice) per block, so that a finished wave frees its slot at
@@ -560,17 +658,34 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   uint2 e[UW];
   // pass 1
   Kahan kT = {0.f, 0.f};
+  bool nz = false;
+  const uint64_t* bsw = nullptr;
+  if constexpr (BL) {
+    // the pre pass enters the window with the visibility sum of the front
+    // records in the first window record's brick (every front tau is 0);
+    // bit l of word k of bsw: window record k of this lane enters a brick
+    // (the slice's rows of the word array, padded past the last slice)
+    const FrontRec fr = front[ray];
+    const uint32_t tl = tail[ray];
+    Kahan kp = {0.f, 0.f};
+    for (uint32_t k = fr.nf - tl; k < fr.nf; ++k) {
+      const uint2 et = inc[fr.off + (size_t)k * kSlice];
+      kp.add(__uint_as_float(et.y) * Msg<MI>::load(Xi, et.x));
+    }
+    kL = kp;
+    bsw = bstart + (sw - base_w) / kSlice;
+  }
   win_load<UW>(rr, lr, 0, e);
   win_issue<MI, UW>(rx, e, lA, xA);
   win_load<UW>(rr, lr, UW, e);
   for (uint32_t k0 = 0;; k0 += 2 * UW) {  // bounded by the slice's longest ray: uniform control flow
     win_issue<MI, UW>(rx, e, lB, xB);
     win_load<UW>(rr, lr, k0 + 2 * UW, e);
-    win_pass1<UW>(r, k0, nw, lA, xA, kL, kS, kT);
+    win_pass1_any<UW, BL>(r, k0, nw, lA, xA, bsw, l, kL, kS, kT, nz);
     if (k0 + UW >= lws) break;
     win_issue<MI, UW>(rx, e, lA, xA);
     win_load<UW>(rr, lr, k0 + 3 * UW, e);
-    win_pass1<UW>(r, k0 + UW, nw, lB, xB, kL, kS, kT);
+    win_pass1_any<UW, BL>(r, k0 + UW, nw, lB, xB, bsw, l, kL, kS, kT, nz);
     if (k0 + 2 * UW >= lws) break;
   }
   float T = kT.at(0.0f);
@@ -1001,63 +1116,70 @@ void launch_pair_msgs(const CamPass& a, hipStream_t s) {
 }
 
 template <int MI, int MO, int UF, int UW>
-void launch_ray_t(const CamPass& a, hipStream_t s) {
+void launch_ray_t(const CamPass& a, hipStream_t s, int part) {
   const typename Msg<MI>::T* Xi = (const typename Msg<MI>::T*)a.apair;
-  if (a.nchunks)
+  if (a.nchunks && part != 2)
     hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s,
                        a.fspan, a.chunk_desc, a.nchunks, a.dyn_desc, a.ndyn_chunks, a.inc, Xi, a.partial, a.msg_all);
+  if (part == 1) return;
   const uint32_t blocks = (a.nslices * kSlice + 63) / 64;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
-  hipLaunchKernelGGL((k_ray_window<MI, MO, UW>), dim3(nb), dim3(64), 0, s, a.lanes, a.len_w, a.nslices, a.chunk_off,
-                     a.partial, a.inc, a.base_w, Xi, a.npairs, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia, a.lookup_n);
+  if (a.brick_local)
+    hipLaunchKernelGGL((k_ray_window<MI, MO, UW, true>), dim3(nb), dim3(64), 0, s, a.lanes, a.len_w, a.nslices,
+                       a.chunk_off, a.partial, a.inc, a.base_w, Xi, a.npairs, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia,
+                       a.lookup_n, a.front, a.bstart, a.tail);
+  else
+    hipLaunchKernelGGL((k_ray_window<MI, MO, UW, false>), dim3(nb), dim3(64), 0, s, a.lanes, a.len_w, a.nslices,
+                       a.chunk_off, a.partial, a.inc, a.base_w, Xi, a.npairs, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia,
+                       a.lookup_n, a.front, a.bstart, a.tail);
 }
 
 template <int MI, int MO>
-void launch_ray_sched(const CamPass& a, hipStream_t s) {
+void launch_ray_sched(const CamPass& a, hipStream_t s, int part) {
   switch (a.ray_sched) {
-    case kSchedF8W8: launch_ray_t<MI, MO, 8, 8>(a, s); break;
-    case kSchedF16W4: launch_ray_t<MI, MO, 16, 4>(a, s); break;
-    case kSchedF16W8: launch_ray_t<MI, MO, 16, 8>(a, s); break;
-    case kSchedF4W4: launch_ray_t<MI, MO, 4, 4>(a, s); break;
-    case kSchedF16W16: launch_ray_t<MI, MO, 16, 16>(a, s); break;
-    default: launch_ray_t<MI, MO, 8, 4>(a, s);
+    case kSchedF8W8: launch_ray_t<MI, MO, 8, 8>(a, s, part); break;
+    case kSchedF16W4: launch_ray_t<MI, MO, 16, 4>(a, s, part); break;
+    case kSchedF16W8: launch_ray_t<MI, MO, 16, 8>(a, s, part); break;
+    case kSchedF4W4: launch_ray_t<MI, MO, 4, 4>(a, s, part); break;
+    case kSchedF16W16: launch_ray_t<MI, MO, 16, 16>(a, s, part); break;
+    default: launch_ray_t<MI, MO, 8, 4>(a, s, part);
   }
 }
 
-void launch_ray_messages(const CamPass& a, hipStream_t s) {
+void launch_ray_messages(const CamPass& a, hipStream_t s, int part) {
   if (a.nslices == 0 || a.nwin == 0) return;
   switch (a.msg_type) {
-    case kMsgF16: launch_ray_sched<kMsgF16, kMsgF16>(a, s); break;
-    case kMsgBF16: launch_ray_sched<kMsgBF16, kMsgBF16>(a, s); break;
-    case kMsgMixedF16: launch_ray_sched<kMsgF32, kMsgF16>(a, s); break;
-    case kMsgMixedBF16: launch_ray_sched<kMsgF32, kMsgBF16>(a, s); break;
-    default: launch_ray_sched<kMsgF32, kMsgF32>(a, s);
+    case kMsgF16: launch_ray_sched<kMsgF16, kMsgF16>(a, s, part); break;
+    case kMsgBF16: launch_ray_sched<kMsgBF16, kMsgBF16>(a, s, part); break;
+    case kMsgMixedF16: launch_ray_sched<kMsgF32, kMsgF16>(a, s, part); break;
+    case kMsgMixedBF16: launch_ray_sched<kMsgF32, kMsgBF16>(a, s, part); break;
+    default: launch_ray_sched<kMsgF32, kMsgF32>(a, s, part);
   }
 }
 
 
-void launch_epilogue(const CamPass& a, hipStream_t s) {
+void launch_epilogue(const CamPass& a, hipStream_t s, int part) {
   if (a.npairs == 0) return;
-  float out_scale = 1.0f;
-  if (a.nwin) {
+  const int mo = msg_out_type(a.msg_type);
+  const float out_scale = mo == kMsgF16 ? Msg<kMsgF16>::kOut : (mo == kMsgBF16 ? Msg<kMsgBF16>::kOut : 1.0f);
+  if (a.nwin && part != 2) {
     const uint32_t ntiles = (a.nwin + kTile - 1) / kTile;
     dim3 grid(grid_for(ntiles, 4)), block(256);
-    switch (msg_out_type(a.msg_type)) {
+    switch (mo) {
       case kMsgF16:
         hipLaunchKernelGGL(k_seg_sum<kMsgF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const __half*)a.Xo,
                            a.win_idx, a.S, a.pf_v, a.pl_v, a.mid);
-        out_scale = Msg<kMsgF16>::kOut;
         break;
       case kMsgBF16:
         hipLaunchKernelGGL(k_seg_sum<kMsgBF16>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const uint16_t*)a.Xo,
                            a.win_idx, a.S, a.pf_v, a.pl_v, a.mid);
-        out_scale = Msg<kMsgBF16>::kOut;
         break;
       default:
         hipLaunchKernelGGL(k_seg_sum<kMsgF32>, grid, block, 0, s, a.heads, a.tile_h0, a.nwin, (const float*)a.Xo,
                            a.win_idx, a.S, a.pf_v, a.pl_v, a.mid);
     }
   }
+  if (part == 1) return;
   // the segments that cross tiles are joined by k_seg_update
   hipLaunchKernelGGL(k_seg_update, dim3(pair_grid(a)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf, a.nwp,
                      a.npairs, a.wseg_off, a.S, a.pl_v, a.pf_v, a.mid, out_scale, a.B, a.M, a.upd, a.all_pairs);
@@ -1076,6 +1198,13 @@ void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* 
   if (npix == 0) return;
   hipLaunchKernelGGL(k_reslice, dim3(grid_for(npix, 256)), dim3(256), 0, s, front, lanes, npix, sell, off_f, len_f,
                      off_w, len_w, base_w, inc);
+}
+
+void launch_brick_starts(const FrontRec* front, const RayRec* lanes, const uint32_t* len_w, uint32_t nslices,
+                         const uint2* inc, uint32_t base_w, uint64_t* bstart, uint8_t* tail, hipStream_t s) {
+  if (nslices == 0) return;
+  hipLaunchKernelGGL(k_brick_starts, dim3(grid_for(nslices, 4)), dim3(256), 0, s, front, lanes, len_w, nslices, inc,
+                     base_w, bstart, tail);
 }
 
 void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t nslices, uint32_t* cnt, hipStream_t s) {

@@ -141,6 +141,13 @@ struct CamPass {
   int lookup_n;
   int msg_type;
   int ray_sched;           // RaySched
+  // pre-pass visibility restarted at brick entries until a tau survives
+  // flush-to-zero (MRF_VIS_BRICK_LOCAL): per window row of every slice the
+  // lanes whose record enters a brick, and per lane the front records that
+  // share the first window record's brick
+  bool brick_local;
+  const uint64_t* bstart;
+  const uint8_t* tail;
 };
 // front/window split of an emitted camera (all records of a ray in its
 // front part): sets nf, nw, the front length sums and per-pixel counts
@@ -150,6 +157,13 @@ void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* se
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
                     const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
                     hipStream_t s);
+// brick entries of every window record (records still hold voxel ids):
+// bstart[(region row of the slice) + k] bit l = record k of lane l enters a
+// brick (the ray's first record, the record after a gap, or a voxel in
+// another brick than the previous record's); tail[lane] = front records at
+// the end of the front that lie in the first window record's brick
+void launch_brick_starts(const FrontRec* front, const RayRec* lanes, const uint32_t* len_w, uint32_t nslices,
+                         const uint2* inc, uint32_t base_w, uint64_t* bstart, uint8_t* tail, hipStream_t s);
 // front chunks per slice (cnt has nslices + 1 entries, the last 0), the
 // descriptor of every chunk from their offsets, and the per-lane front spans
 void launch_chunk_counts(const uint32_t* len_f, const uint32_t* len_w, uint32_t nslices, uint32_t* cnt, hipStream_t s);
@@ -158,13 +172,15 @@ void launch_front_span(const FrontRec* front, const RayRec* lanes, uint32_t n, u
 // out[j] = desc[idx[j]]
 void launch_gather_desc(const uint4* desc, const uint32_t* idx, uint32_t n, uint4* out, hipStream_t s);
 void launch_pair_msgs(const CamPass& a, hipStream_t s);
-void launch_ray_messages(const CamPass& a, hipStream_t s);
+// part: 0 both kernels, 1 the front sums only, 2 the window kernel only
+void launch_ray_messages(const CamPass& a, hipStream_t s, int part = 0);
 // per front chunk of the camera: 0 if all its pairs are front-only pairs whose pflag (front-only
 // pair q at pflag[q - nwp]: at a voxel of W) is 0 (static; its records are added to *nstatic), 1
 // otherwise
 void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, unsigned long long* nstatic,
                          hipStream_t s);
-void launch_epilogue(const CamPass& a, hipStream_t s);
+// part: 0 both kernels, 1 the segment sums only, 2 the per-pair update only
+void launch_epilogue(const CamPass& a, hipStream_t s, int part = 0);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
 void launch_residual(const float* B, float* Aprev, size_t nvox, unsigned long long* d_max, hipStream_t s);
 void launch_ref_error(const float* A, const uint8_t* upd, const float* ref, size_t n, unsigned long long* d_max,

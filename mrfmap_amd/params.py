@@ -45,6 +45,11 @@ class gvdb_params:  # noqa: N801  (reference class name)
         self.activation = _lib.ACT_ENDPOINT
         self.msg_dtype = _lib.MSG_F32
         self.mask_848_border = True
+        # pre-pass visibility: _lib.VIS_BRICK_LOCAL (default) restarts it at
+        # every brick entry until a tau survives FP32 flush-to-zero, as the
+        # reference's pre pass does (gvdbkernels.cu:708-715); _lib.VIS_CARRY
+        # carries it along the ray, as the post pass does
+        self.brick_local_vis = _lib.VIS_BRICK_LOCAL
 
     def lookup_cells(self):
         return math.ceil(self.cols / LOOKUP_PX) * math.ceil(self.rows / LOOKUP_PX)
@@ -128,6 +133,7 @@ class gvdb_params:  # noqa: N801  (reference class name)
         p.activation = int(self.activation)
         p.msg_dtype = int(self.msg_dtype)
         p.mask_848_border = int(bool(self.mask_848_border))
+        p.brick_local_vis = int(getattr(self, "brick_local_vis", _lib.VIS_BRICK_LOCAL))
         return p
 
     def __repr__(self):

@@ -271,6 +271,7 @@ typedef struct {
   int64_t npairs;
   uint32_t* pair_vox;
   double* pair_c;
+  uint32_t* rpair; /* per record: its pair (GAP for gap records) */
   double* M;
 } ocam;
 
@@ -345,8 +346,10 @@ static void free_cam_lists(ocam* k) {
   k->nrays = 0;
   free(k->pair_vox);
   free(k->pair_c);
+  free(k->rpair);
   k->pair_vox = NULL;
   k->pair_c = NULL;
+  k->rpair = NULL;
   k->npairs = 0;
 }
 
@@ -560,6 +563,7 @@ int orc_prepare(orc_ctx* c) {
   bm.slot = c->slot;
   bm.mark = NULL;
   double* csum = (double*)calloc((size_t)c->nslots * 512, sizeof(double));
+  uint32_t* vpair = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(c->nslots * 512 + 1));
   uint32_t* touched = NULL;
   int64_t ntouched = 0, captouched = 0;
   for (int ci = 0; ci < c->ncams; ++ci) {
@@ -625,12 +629,19 @@ int orc_prepare(orc_ctx* c) {
       k->pair_vox[i] = touched[i];
       k->pair_c[i] = csum[touched[i]];
       csum[touched[i]] = 0.0;
+      vpair[touched[i]] = (uint32_t)i;
+    }
+    k->rpair = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(k->rb.n ? k->rb.n : 1));
+    for (int64_t i = 0; i < k->rb.n; ++i) {
+      const uint32_t vox = k->rb.rec[2 * i];
+      k->rpair[i] = vox == GAP ? GAP : vpair[vox];
     }
     free(k->M);
     k->M = (double*)calloc((size_t)c->nslots * 512, sizeof(double));
   }
   free(touched);
   free(csum);
+  free(vpair);
   free(sv.s);
   c->dirty = 0;
   return 0;
@@ -863,10 +874,10 @@ int orc_continue_inference(orc_ctx* c, int max_iters, double tol, double* residu
 
 /* Threads of the BP loop (0: OpenMP's default, i.e. OMP_NUM_THREADS or
  * every core).  The rays of a camera pass are split into one contiguous block
- * per thread; each thread sums its rays' p * l per voxel in an array of its
- * own, and the per-pair sums S add the threads' arrays in thread order, so a
- * result depends on the thread count only through FP64 rounding and is
- * reproducible for a given count. */
+ * per thread; each thread sums its rays' p * l per (camera, voxel) pair in an
+ * array of its own, and the per-pair sums S add the threads' arrays in
+ * thread order, so a result depends on the thread count only through FP64
+ * rounding and is reproducible for a given count. */
 static int g_threads = 0;
 
 void orc_set_threads(int n) { g_threads = n < 0 ? 0 : n; }
@@ -879,7 +890,7 @@ int orc_get_threads(void) {
 #endif
 }
 
-/* Messages of rays [r0, r1) of camera k into St (p * l summed per voxel). */
+/* Messages of rays [r0, r1) of camera k into St (p * l summed per pair). */
 static void camera_rays(const orc_ctx* c, const ocam* k, int64_t r0, int64_t r1, double* St, double* nin,
                         double* pout, int* live, int* bst) {
   const int ln = c->p.lookup_n;
@@ -902,7 +913,7 @@ static void camera_rays(const orc_ctx* c, const ocam* k, int64_t r0, int64_t r1,
     orc_ray_messages_ex(n, nin, k->rb.len + off, k->rb.zex + off, live, c->p.brick_local_vis ? bst : NULL,
                         (double)d, sg, bi, ln, ro, 1.0, 0, pout);
     for (int i = 0; i < n; ++i)
-      if (live[i]) St[k->rb.rec[2 * (off + i)]] += pout[i] * k->rb.len[off + i];
+      if (live[i]) St[k->rpair[off + i]] += pout[i] * k->rb.len[off + i];
   }
 }
 
@@ -915,8 +926,11 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
     for (int64_t i = 0; i < c->cams[ci].rb.n; ++i) upd += c->cams[ci].rb.rec[2 * i] != GAP;
   if (updates_per_iter) *updates_per_iter = upd;
   for (int64_t v = 0; v < nv; ++v) c->A[v] = marginal(c->B[v]);
-  /* per thread: the voxel sums and the per-ray work arrays */
-  double* Sall = (double*)calloc((size_t)nv * (size_t)nt + 1, sizeof(double));
+  /* per thread: the pair sums and the per-ray work arrays */
+  int64_t np = 1;
+  for (int ci = 0; ci < c->ncams; ++ci)
+    if (c->cams[ci].npairs > np) np = c->cams[ci].npairs;
+  double* Sall = (double*)calloc((size_t)np * (size_t)nt, sizeof(double));
   int maxn = 1;
   for (int ci = 0; ci < c->ncams; ++ci)
     for (int64_t r = 0; r < c->cams[ci].nrays; ++r)
@@ -940,15 +954,19 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
         const int64_t r0 = k->nrays * t / nt, r1 = k->nrays * (t + 1) / nt;
         double* nin = dbuf + (size_t)t * 2 * maxn;
         int* live = ibuf + (size_t)t * 2 * maxn;
-        camera_rays(c, k, r0, r1, Sall + (size_t)t * nv, nin, nin + maxn, live, live + maxn);
+        camera_rays(c, k, r0, r1, Sall + (size_t)t * np, nin, nin + maxn, live, live + maxn);
       }
+      /* a camera's pairs are distinct voxels: its updates are independent */
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
       for (int64_t q = 0; q < k->npairs; ++q) {
         const uint32_t v = k->pair_vox[q];
         const double C = k->pair_c[q];
         double s = 0.0;
         for (int t = 0; t < nt; ++t) {  /* thread order */
-          s += Sall[(size_t)t * nv + v];
-          Sall[(size_t)t * nv + v] = 0.0;
+          s += Sall[(size_t)t * np + q];
+          Sall[(size_t)t * np + q] = 0.0;
         }
         if (C > CUM_LEN_GATE) {
           double a = clampd(s / C);
@@ -960,6 +978,9 @@ static int bp_iterate(orc_ctx* c, int max_iters, double tol, double* residual, i
       }
     }
     res = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static) reduction(max : res)
+#endif
     for (int64_t v = 0; v < nv; ++v) {
       double a = marginal(c->B[v]);
       double dl = fabs(a - c->A[v]);

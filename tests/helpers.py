@@ -18,7 +18,8 @@ def oracle_for(p, frames, activation=None):
         tables = {"sigma_lookup": p.depth_sigma_lookup, "bias_lookup": p.depth_bias_lookup}
     o = oracle.Oracle(p.K, p.rows, p.cols, p.res, p.prior, p.dims, p.const_sigmasq_poly, p.const_bias_poly,
                       lookup_n=p.lookup_n, activation=p.activation if activation is None else activation,
-                      mask_848_border=int(p.mask_848_border), **tables)
+                      mask_848_border=int(p.mask_848_border),
+                      brick_local_vis=int(p.brick_local_vis), **tables)
     for pose, K, depth in frames:
         o.add_camera(pose, depth, K)
     return o
@@ -33,3 +34,23 @@ def gpu_for(p, frames):
 
 def small(**kw):
     return synth.make_workload("small", **kw)
+
+
+def coffin_frames(activation, brick_local_vis=0, root=None):
+    """The reference's own inputs: keyframes 0 and 6 of data/ (640x480 planar
+    depth, NaN in the top rows) with the parameters of
+    config/coffin_world_640.yaml (res 0.02, K 381.36, sigma 0.05, prior 0.1,
+    dims 4x4x3, max_iters 3), each pose composed as pose * cam_in_body
+    (testMRFMapViewer.cpp:55)."""
+    import os
+
+    from mrfmap_amd import gvdb_params
+
+    gold = os.path.join(root or os.path.dirname(os.path.abspath(__file__)), "golden")
+    p = gvdb_params().load_from_file(os.path.join(gold, "coffin_world_640.yaml"))
+    p.activation = activation
+    p.brick_local_vis = brick_local_vis
+    z = np.load(os.path.join(gold, "coffin_frames.npz"))
+    frames = [((z["body_pose"][i] @ z["cam_in_body"]).astype(np.float32), None, z["depth"][i])
+              for i in range(len(z["frames"]))]
+    return p, frames

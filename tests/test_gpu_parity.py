@@ -393,7 +393,7 @@ def test_loaded_map_evaluation_matches_oracle():
     vals = np.full(len(pts), 0.95, np.float32)
     est = GVDBMapLikelihoodEstimator(params=p)
     o = oracle.Oracle(p.K, p.rows, p.cols, p.res, p.prior, p.dims, p.const_sigmasq_poly, p.const_bias_poly,
-                      lookup_n=p.lookup_n, activation=p.activation)
+                      lookup_n=p.lookup_n, activation=p.activation, brick_local_vis=p.brick_local_vis)
     est.load_map(pts, vals)
     o.load_map(pts, vals)
     assert est.num_bricks() == o.num_bricks()
@@ -595,3 +595,64 @@ def test_per_cell_sensor_model_matches_oracle(ln):
     err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
     print(f"per-cell sensor model, lookup_n={ln}: {n} records, max-abs marginal error {err.max():.3e}")
     assert err.max() <= MARG_TOL
+
+
+@pytest.mark.parametrize("vis", [_lib.VIS_CARRY, _lib.VIS_BRICK_LOCAL])
+@pytest.mark.parametrize("activation", [_lib.ACT_ENDPOINT, _lib.ACT_FULL_RAY])
+def test_reference_frames_match_oracle(activation, vis):
+    """The reference's recorded keyframes 0 and 6 (data/depth_k.bin,
+    pose_k.bin composed with cam_in_body) under coffin_world_640.yaml's
+    parameters (res 0.02, K 381.36, sigma 0.05, prior 0.1, 3 iterations),
+    with endpoint activation (the reference's) and full-ray activation, and
+    with both pre-pass visibilities: traversal bit-exact, marginals after the
+    config's 3 iterations within 1e-3 of the FP64 oracle."""
+    from helpers import coffin_frames
+
+    p, frames = coffin_frames(activation, vis)
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    n = _compare_traversal(g, o, len(frames))
+    assert n > (5_000_000 if activation == _lib.ACT_ENDPOINT else 50_000_000)
+    st = g.perform_inference(max_iters=p.max_iters)
+    so = o.perform_inference(p.max_iters)
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    a = o.alphas()
+    err = np.abs(g.get_alphas().astype(np.float64) - a)
+    print(f"reference frames, activation {activation}, vis {vis}: {n} records, {g.num_bricks()} bricks, "
+          f"max marginal {a.max():.3f}, max-abs marginal error {err.max():.3e}")
+    assert err.max() <= MARG_TOL
+
+
+@pytest.mark.parametrize("vis", [_lib.VIS_BRICK_LOCAL, _lib.VIS_CARRY])
+@pytest.mark.parametrize("activation", [_lib.ACT_FULL_RAY, _lib.ACT_ENDPOINT])
+def test_pre_pass_visibility_small_matches_oracle(activation, vis):
+    """Both pre-pass visibilities on the small workload -- the reference's
+    (restarted at every brick entry until a tau survives FP32 flush-to-zero,
+    gvdbkernels.cu:708-715; the default) and the carried one: after one
+    iteration within 1e-5 of the oracle, after five more within 1e-3; the
+    two maps differ."""
+    p, frames, _ = synth.make_workload("small", activation=activation)
+    p.brick_local_vis = vis
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.perform_inference(max_iters=1)
+    o.perform_inference(1)
+    e1 = np.abs(g.get_alphas().astype(np.float64) - o.alphas()).max()
+    g.perform_inference(max_iters=5)
+    o.perform_inference(5)
+    a = o.alphas()
+    e5 = np.abs(g.get_alphas().astype(np.float64) - a).max()
+    pc = p.copy()
+    pc.brick_local_vis = _lib.VIS_CARRY if vis == _lib.VIS_BRICK_LOCAL else _lib.VIS_BRICK_LOCAL
+    oc = oracle_for(pc, frames)
+    oc.perform_inference(1)
+    oc.perform_inference(5)
+    d = np.abs(oc.alphas() - a).max()
+    print(f"pre-pass visibility {vis}, activation {activation}: 1 iteration {e1:.3e}, 1+5 iterations {e5:.3e}; "
+          f"the two visibilities' maps differ by {d:.3f}")
+    assert e1 <= 1e-5
+    assert e5 <= MARG_TOL
+    assert d > 1e-2
