@@ -54,3 +54,11 @@ def coffin_frames(activation, brick_local_vis=0, root=None):
     frames = [((z["body_pose"][i] @ z["cam_in_body"]).astype(np.float32), None, z["depth"][i])
               for i in range(len(z["frames"]))]
     return p, frames
+
+
+def c2_anchor_workload(step):
+    """The keyframes of the FP64 anchor (tests/golden/c2_anchor.npz): the
+    640x480 / 1 cm / 13-keyframe workload with every step-th pixel row and
+    column kept."""
+    p, frames, _ = synth.make_workload("real", frames=13)
+    return p, [(pose, K, decimate(d, step)) for pose, K, d in frames]

@@ -250,17 +250,19 @@ def test_bp_bitwise_reproducible():
     np.testing.assert_array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
-# 16-bit storage of the per-incidence messages: after one BP iteration the
-# error reflects the storage precision alone (FP16 2^-11, BF16 2^-8 relative);
-# from the second iteration on the reference's message normaliser
-# p = mu1 / (mu1 + mu0 + 1e-7) amplifies belief perturbations ~50-170x (even
-# FP32 round-off grows from 4e-7 to 6e-5), so after 5 iterations the errors
-# are reported (mixed-precision workload) and bounded against regressions.
+# 16-bit storage of the ray->voxel messages on the small workload: bounds a
+# margin above the errors measured against the FP64 oracle (max-abs after 1
+# iteration; max-abs and RMS after 5).  FP16 measured 1.5e-3 / 3.4e-3 /
+# 3.9e-5, BF16 1.1e-2 / 8.3e-3 / 1.5e-4, the same with the per-pair values
+# kept in FP32 (mixed); FP32 storage stays within 5e-5.  The error comes from
+# the stored p * l: the marginal depends on the camera sum S / C through
+# logit, which multiplies a relative error of S by 1 / (1 - S / C), so 16-bit
+# storage does not meet 1e-3 where a voxel's messages are near 1.
 MIXED = {
-    _lib.MSG_F16: dict(it1=1e-3, max5=5e-2, rms5=1e-3),
-    _lib.MSG_BF16: dict(it1=5e-3, max5=1.0, rms5=1e-2),
-    _lib.MSG_MIXED_F16: dict(it1=1e-3, max5=5e-2, rms5=1e-3),
-    _lib.MSG_MIXED_BF16: dict(it1=5e-3, max5=1.0, rms5=1e-2),
+    _lib.MSG_F16: dict(it1=2.5e-3, max5=6e-3, rms5=8e-5),
+    _lib.MSG_BF16: dict(it1=2e-2, max5=1.6e-2, rms5=3e-4),
+    _lib.MSG_MIXED_F16: dict(it1=2.5e-3, max5=6e-3, rms5=8e-5),
+    _lib.MSG_MIXED_BF16: dict(it1=2e-2, max5=1.6e-2, rms5=3e-4),
 }
 
 
@@ -268,6 +270,7 @@ MIXED = {
 def test_bp_mixed_precision_small(msg):
     bound = MIXED[msg]
     p, frames, _ = synth.make_workload("small", msg_dtype=msg)
+    errs = {}
     for iters in (1, 5):
         g = gpu_for(p, frames)
         o = oracle_for(p, frames)
@@ -275,11 +278,10 @@ def test_bp_mixed_precision_small(msg):
         o.perform_inference(iters)
         err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
         rms = float(np.sqrt((err ** 2).mean()))
+        errs[iters] = (float(err.max()), rms)
         print(f"msg {msg} iters {iters}: max-abs {err.max():.3e} rms {rms:.3e}")
-        if iters == 1:
-            assert err.max() <= bound["it1"]
-        else:
-            assert err.max() <= bound["max5"] and rms <= bound["rms5"]
+    assert errs[1][0] <= bound["it1"]
+    assert errs[5][0] <= bound["max5"] and errs[5][1] <= bound["rms5"]
 
 
 @pytest.mark.parametrize("sched", sorted(_lib.RAY_SCHEDULES))
@@ -499,13 +501,18 @@ def test_empty_map_and_no_cameras():
     assert st["iters"] == 2 and st["updates_per_iter"] == 0
 
 
-def test_large_scene_limit_cycle_matches_oracle():
+@pytest.mark.parametrize("vis", [_lib.VIS_CARRY, _lib.VIS_BRICK_LOCAL])
+def test_large_scene_limit_cycle_matches_oracle(vis):
     """On the 10x10x3 m scene (16 keyframes of the batched generator, 4 cm
-    voxels, every 4th pixel row and column) Gauss-Seidel BP does not
-    converge: from the 8th iteration on some voxels alternate and the
-    residual stays at 0.199 in FP64 as in FP32.  The device follows the
-    oracle's residual iteration by iteration and ends on the same map."""
+    voxels, every 4th pixel row and column) Gauss-Seidel BP converges
+    slowly or not at all.  With the carried pre-pass visibility some voxels
+    alternate from the 8th iteration on and the residual stays at 0.199 in
+    FP64 as in FP32 (a limit cycle); with the reference's brick-local
+    visibility the residual falls slowly (0.30 after 12 iterations).  Either
+    way the device follows the oracle's residual iteration by iteration and
+    ends on the same map."""
     p, frames, _ = synth.make_workload("batched", frames=16, res=0.04)
+    p.brick_local_vis = vis
     frames = [(pose, K, decimate(depth, 4)) for pose, K, depth in frames]
     g = gpu_for(p, frames)
     g.perform_inference(max_iters=12)
@@ -517,7 +524,8 @@ def test_large_scene_limit_cycle_matches_oracle():
     print("residual gpu   ", " ".join(f"{x:.4f}" for x in rg))
     print("residual oracle", " ".join(f"{x:.4f}" for x in ro))
     np.testing.assert_allclose(rg, ro, rtol=0, atol=1e-3)
-    assert min(ro[-4:]) > 0.15 and max(ro[-4:]) - min(ro[-4:]) < 1e-3  # the cycle
+    if vis == _lib.VIS_CARRY:
+        assert min(ro[-4:]) > 0.15 and max(ro[-4:]) - min(ro[-4:]) < 1e-3  # the cycle
     err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
     print(f"limit cycle: max-abs marginal error {err.max():.3e}")
     assert err.max() <= MARG_TOL
@@ -656,3 +664,41 @@ def test_pre_pass_visibility_small_matches_oracle(activation, vis):
     assert e1 <= 1e-5
     assert e5 <= MARG_TOL
     assert d > 1e-2
+
+
+@pytest.mark.timeout(600)
+def test_converged_map_matches_fp64_anchor():
+    """The FP64 anchor (tests/golden/c2_anchor.npz, written by
+    tests/golden/make_c2_anchor.py): the 640x480 / 1 cm / 13-keyframe
+    workload with every 3rd pixel row and column, iterated by the FP64
+    oracle until the max marginal change of an iteration fell below 1e-6.
+    The GPU's FP32 BP, iterated from the prior until its own residual is
+    below 1e-5 (or 400 iterations), must land within 1e-3 of that map at
+    every voxel the anchor stores (marginal above its threshold, rounded to
+    1/65535) and below the anchor's bound plus 1e-3 everywhere else."""
+    import json
+    import os
+
+    from helpers import c2_anchor_workload
+
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_anchor.npz"))
+    meta = json.loads(str(z["meta"]))
+    p, frames = c2_anchor_workload(meta["step"])
+    assert int(p.brick_local_vis) == meta["brick_local_vis"]
+    g = gpu_for(p, frames)
+    g.prepare()
+    assert g.num_bricks() == meta["num_bricks"]
+    np.testing.assert_array_equal(g.get_occupied_bricks(), (z["bricks"].astype(np.float32) * 8 + 4))
+    st = g.perform_inference(max_iters=400, tol=1e-5)
+    a = g.get_alphas().astype(np.float64).ravel()
+    idx = np.cumsum(z["deltas"].astype(np.int64))
+    ref = z["vals"].astype(np.float64) / 65535.0
+    err = np.abs(a[idx] - ref)
+    rest = np.ones(a.size, bool)
+    rest[idx] = False
+    print(f"FP64 anchor (step {meta['step']}, {meta['iters']} FP64 iterations, residual {meta['residual']:.1e}): "
+          f"GPU {st['iters']} iterations, residual {st['residual']:.1e}; {len(idx)} stored voxels max-abs "
+          f"{err.max():.3e} rms {np.sqrt((err ** 2).mean()):.3e}; others max {a[rest].max():.4f} "
+          f"(anchor bound {meta['rest_max']:.4f})")
+    assert err.max() <= MARG_TOL
+    assert a[rest].max() <= meta["rest_max"] + MARG_TOL
