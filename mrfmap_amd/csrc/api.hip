@@ -57,7 +57,7 @@ struct Camera {
   uint64_t nupd = 0;               // live incidences
   uint64_t nwin = 0;               // live window incidences (window positions)
   uint64_t nlist = 0;              // records without the slice padding
-  uint32_t* d_win_idx = nullptr;   // per window position (voxel order): its run in the run array
+  uint32_t* d_win_idx = nullptr;   // per window position (voxel order): the record's slot in the window region
   uint32_t* d_pair_vox = nullptr;
   float* d_pair_c = nullptr;       // summed chord length per pair
   float* d_pair_sf = nullptr;      // summed constant front messages per pair
@@ -69,10 +69,6 @@ struct Camera {
   uint32_t* d_chunk_off = nullptr; // nslices + 1: first front chunk of each slice
   uint4* d_chunk_desc = nullptr;
   uint32_t nchunks = 0;
-  uint8_t* d_wperm = nullptr;      // per window record slot: the row's lanes in pair order
-  uint64_t* d_whm = nullptr;       // per window row: lanes (in pair order) where a run of equal pairs starts
-  uint32_t* d_roff = nullptr;      // per window row + 1: first run of the row in the run array
-  uint64_t nruns = 0;              // runs of equal pairs of live window records, over all rows (segment positions)
   uint64_t* d_bstart = nullptr;    // MRF_VIS_BRICK_LOCAL: brick-entry word per window row of every slice
   uint8_t* d_tail = nullptr;       // MRF_VIS_BRICK_LOCAL: per lane, front records in the first window brick
 };
@@ -463,10 +459,6 @@ void free_camera_lists(Camera& cam) {
   dfree(&cam.d_chunk_desc);
   dfree(&cam.d_bstart);
   dfree(&cam.d_tail);
-  dfree(&cam.d_wperm);
-  dfree(&cam.d_whm);
-  dfree(&cam.d_roff);
-  cam.nruns = 0;
   cam.nchunks = 0;
   cam.nrays = 0;
   cam.nslices = 0;
@@ -598,40 +590,27 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
       (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
     return rc;
   launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
-  // 5. per pair: C and the constant front messages; records point at their
-  //    pairs from here on.  Per window row: its lanes in pair order, the
-  //    starts of its runs of equal pairs and their count (into d_x0)
-  const int wbits = std::max(1, bit_length(nwp));
-  const uint64_t wrows = nrec_w / kSlice;
-  if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits)) || (rc = ensure_scan(c, wrows + 1))) return rc;
-  if ((rc = dalloc(c, &cam.d_wperm, std::max<uint64_t>(nrec_w, 1))) || (rc = dalloc(c, &cam.d_whm, wrows + 1)) ||
-      (rc = dalloc(c, &cam.d_roff, wrows + 1)))
-    return rc;
+  // 5. per pair: C and the constant front messages; records point at their pairs from here on
   launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, c->d_acc + 4 * (size_t)np,
-                      np, cam.d_pair_c, cam.d_pair_sf, cam.d_wperm, cam.d_whm, c->d_x0, s);
-  // 6. the runs of every window row in row order (run i of row w at
-  //    roff[w] + i; roff[wrows] = the total), stably sorted by pair: the
-  //    window segments are runs, each holding the sum of its records' messages
-  launch_exclusive_scan(c->d_x0, cam.d_roff, wrows, c->d_scan_tmp, c->d_total, s);
-  const uint64_t nruns = read_total(c);
-  launch_fill_u32(cam.d_roff + wrows, (uint32_t)nruns, 1, s);
-  cam.nruns = nruns;
-  if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_idx, nruns))) return rc;
-  HIPCHK(c, sort_window_runs(cam.d_inc + nrec_f, cam.d_wperm, cam.d_whm, cam.d_roff, wrows, (uint32_t)nruns, wbits,
-                             c->d_keys_in, c->d_keys_out, c->d_vals_in, c->d_vals_out, c->d_sort_tmp,
-                             c->sort_tmp_bytes, s));
+                      np, cam.d_pair_c, cam.d_pair_sf, s);
+  // 6. window segments: the window records stably sorted by pair
+  const int wbits = std::max(1, bit_length(nwp));
+  if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits))) return rc;
+  if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_idx, nwin))) return rc;
+  HIPCHK(c, sort_window(cam.d_inc + nrec_f, (uint32_t)nrec_w, nwp, wbits, c->d_keys_in, c->d_keys_out,
+                        c->d_vals_in, c->d_vals_out, c->d_sort_tmp, c->sort_tmp_bytes, s));
   uint32_t* whead = c->d_x0;
   uint32_t* hw = c->d_x1;
-  if (nruns) {
-    HIPCHK(c, hipMemcpyAsync(cam.d_win_idx, c->d_vals_out, nruns * 4, hipMemcpyDeviceToDevice, s));
-    launch_window_heads(c->d_keys_out, (uint32_t)nruns, whead, s);
-    if ((rc = flag_scan(c, whead, hw, nruns))) return rc;
+  if (nwin) {
+    HIPCHK(c, hipMemcpyAsync(cam.d_win_idx, c->d_vals_out, nwin * 4, hipMemcpyDeviceToDevice, s));
+    launch_window_heads(c->d_keys_out, (uint32_t)nwin, whead, s);
+    if ((rc = flag_scan(c, whead, hw, nwin))) return rc;
   }
-  launch_window_offsets(c->d_keys_out, whead, (uint32_t)nruns, nwp, cam.d_wseg_off, s);
+  launch_window_offsets(c->d_keys_out, whead, (uint32_t)nwin, nwp, cam.d_wseg_off, s);
   // 7. window segment head bitmask and per-tile head counts
-  const size_t nwords_w = nruns / 64 + 1, ntiles_w = nruns / kTile + 1;
+  const size_t nwords_w = nwin / 64 + 1, ntiles_w = nwin / kTile + 1;
   if ((rc = dalloc(c, &cam.d_heads, nwords_w)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles_w))) return rc;
-  launch_pack_flags(whead, hw, (uint32_t)nruns, cam.d_heads, cam.d_tile_h0, s);
+  launch_pack_flags(whead, hw, (uint32_t)nwin, cam.d_heads, cam.d_tile_h0, s);
   if (keep_msgs)
     launch_gather_pairs(cam.d_pair_vox, c->d_A, np, cam.d_M, s);
   else
@@ -730,9 +709,9 @@ int prepare(mrf_ctx* c) {
   // 4. per-pass scratch sized for the largest camera
   size_t xneed = 1, pneed = 1, tneed = 1;
   for (const Camera& cam : c->cams) {
-    xneed = std::max<size_t>(xneed, cam.nruns);  // one message sum per run
+    xneed = std::max<size_t>(xneed, cam.nrec - cam.base_w);  // window region, padding included
     pneed = std::max<size_t>(pneed, cam.npairs);
-    tneed = std::max<size_t>(tneed, (cam.nruns + kTile - 1) / kTile);
+    tneed = std::max<size_t>(tneed, (cam.nwin + kTile - 1) / kTile);
   }
   if (pneed > c->cap_pairs) {
     dfree(&c->d_Spair);
@@ -812,10 +791,7 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.pair_sf = cam.d_pair_sf;
   a.npairs = cam.npairs;
   a.nwp = cam.nwp;
-  a.nwin = (uint32_t)cam.nruns;
-  a.wperm = cam.d_wperm;
-  a.whm = cam.d_whm;
-  a.roff = cam.d_roff;
+  a.nwin = (uint32_t)cam.nwin;
   a.wseg_off = cam.d_wseg_off;
   a.heads = cam.d_heads;
   a.tile_h0 = cam.d_tile_h0;

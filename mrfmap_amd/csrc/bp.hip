@@ -25,9 +25,8 @@ namespace {
 // ---------------------------------------------------------------- storage
 template <int MT>
 struct Msg;
-// kOut: factor applied to ray->voxel messages before storage.  FP16 stores
-// sums of up to 64 messages p*l <= sqrt(3) (one run of a window row):
-// 512 x 64 x 1.74 < 65504 keeps them finite and most of them normal.
+// kOut: factor applied to ray->voxel messages p*l before storage (FP16 keeps
+// them out of its subnormal range: p*l >= 1e-4 * l).
 template <>
 struct Msg<kMsgF32> {
   typedef float T;
@@ -38,7 +37,7 @@ struct Msg<kMsgF32> {
 template <>
 struct Msg<kMsgF16> {
   typedef __half T;
-  static constexpr float kOut = 512.0f;
+  static constexpr float kOut = 4096.0f;
   __device__ static float load(const T* p, uint32_t i) { return __half2float(p[i]); }
   __device__ static void store(T* p, uint32_t i, float v) { p[i] = __float2half_rn(v); }
 };
@@ -400,31 +399,6 @@ __global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ 
   if (l == 0) dyn_flag[w] = any ? 1u : 0u;
 }
 
-// ---------------------------------------------------------------- segmented scans
-template <int CTRL, int ROWS>
-__device__ __forceinline__ int dppi(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xF, true);
-}
-
-// (x, f) <- (x_left, f_left) (+) (x, f): add the left value unless a segment
-// starts at or before this lane.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step(float& x, int& f) {
-  const float y = dpp<CTRL, ROWS>(x);
-  const int g = dppi<CTRL, ROWS>(f);
-  x = f ? x : x + y;
-  f |= g;
-}
-
-__device__ __forceinline__ void wave_seg_scan(float& x, int& f) {
-  seg_step<0x111, 0xF>(x, f);
-  seg_step<0x112, 0xF>(x, f);
-  seg_step<0x114, 0xF>(x, f);
-  seg_step<0x118, 0xF>(x, f);
-  seg_step<0x142, 0xA>(x, f);
-  seg_step<0x143, 0xC>(x, f);
-}
-
 // ---------------------------------------------------------------- buffer access
 // Raw buffer loads and stores through a wave-uniform resource (base address,
 // size in bytes): the hardware's range check makes an access at or past the
@@ -603,41 +577,10 @@ __device__ __forceinline__ void win_pass1_any(const WinRay& r, uint32_t k0, uint
     win_pass1<UW>(r, k0, nw, ln, x, kL, kS, kT);
 }
 
-// Runs of a window row: the row's lanes in pair order (one byte per lane),
-// where runs of equal pairs start (wave-uniform), its first run and the
-// number of runs of live records.
-template <int UW>
-struct RowRuns {
-  uint32_t src[UW];
-  uint64_t hm[UW];
-  uint32_t r0[UW], nr[UW];
-};
-
-// the runs of rows k0 .. k0 + UW - 1 (perm bytes through the slice's
-// resource, row words as uniform loads; rows at or past the slice's longest
-// ray hold no run and are not read)
-template <int UW>
-__device__ __forceinline__ void win_runs(Rsrc rp, uint32_t l, uint32_t k0, uint32_t lws, const uint64_t* __restrict__ whm,
-                                         const uint32_t* __restrict__ roff, RowRuns<UW>& q) {
-#pragma unroll
-  for (int u = 0; u < UW; ++u) {
-    const uint32_t row = k0 + (uint32_t)u;
-    q.src[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(rp, (int)(row * kSlice + l), 0, 0);
-    const bool in = row < lws;
-    q.hm[u] = in ? whm[row] : 0ull;
-    q.r0[u] = in ? roff[row] : 0u;
-    q.nr[u] = in ? roff[row + 1] - q.r0[u] : 0u;
-  }
-}
-
-// Pass 2: every record's message p * l; the 64 lanes of a row then sum their
-// messages per pair (lanes brought into pair order by ds_bpermute, a
-// segmented scan over the runs) and the last lane of each run of live
-// records stores the run's sum at its place in the run array.
 template <int MO, int UW>
 __device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
-                                          const float (&x)[UW], const RowRuns<UW>& q, float T, Rsrc ro, uint32_t l,
-                                          Kahan& kL, Kahan& kS, Kahan& kP) {
+                                          const float (&x)[UW], float T, Rsrc ro, uint32_t lane_off, Kahan& kL,
+                                          Kahan& kS, Kahan& kP) {
   typedef typename Msg<MO>::T TO;
 #pragma unroll
   for (int u = 0; u < UW; ++u) {
@@ -651,16 +594,7 @@ __device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t
     const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
     float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
     p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
-    const float v = (valid && x[u] != 0.0f) ? p * ln[u] * Msg<MO>::kOut : 0.0f;
-    if (q.nr[u]) {  // wave-uniform
-      float w = __int_as_float(__builtin_amdgcn_ds_bpermute((int)(q.src[u] << 2), __float_as_int(v)));
-      int f = (int)((q.hm[u] >> l) & 1ull);
-      wave_seg_scan(w, f);
-      const bool tail = l == 63u || ((q.hm[u] >> (l + 1u)) & 1ull) != 0ull;
-      const uint64_t upto = l == 63u ? ~0ull : ((2ull << l) - 1ull);
-      const uint32_t ri = (uint32_t)__popcll(q.hm[u] & upto) - 1u;
-      if (tail && ri < q.nr[u]) MsgB<MO>::store(ro, (q.r0[u] + ri) * (uint32_t)sizeof(TO), w);
-    }
+    MsgB<MO>::store(ro, lane_off + (k0 + (uint32_t)u) * (kSlice * (uint32_t)sizeof(TO)), p * ln[u] * Msg<MO>::kOut);
     kahan_add_if(kL, valid, a);
     kahan_add_if(kS, valid, ln[u]);
     kahan_add_if(kP, valid, tau);
@@ -677,10 +611,7 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
                                                     const float* __restrict__ sigt, const float* __restrict__ biat,
                                                     int lookup_n, const FrontRec* __restrict__ front,
                                                     const uint64_t* __restrict__ bstart,
-                                                    const uint8_t* __restrict__ tail, uint32_t nruns,
-                                                    const uint8_t* __restrict__ wperm,
-                                                    const uint64_t* __restrict__ whm,
-                                                    const uint32_t* __restrict__ roff) {
+                                                    const uint8_t* __restrict__ tail) {
   typedef typename Msg<MI>::T TI;
   typedef typename Msg<MO>::T TO;
   // one wave (one slice) per block, so that a finished wave frees its slot at
@@ -694,7 +625,8 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   const uint32_t lw = len_w[t];
   if (lw == 0) return;
   const RayRec rec = lanes[ray];
-  const uint32_t nw = rec.nw;  // lanes without window records run along (the row sums cross lanes)
+  const uint32_t nw = rec.nw;
+  if (nw == 0) return;  // no lane-crossing operations below
 
   // the front's visibility sum: the partial sums of the slice's front chunks, loaded 8 at a time
   Kahan kL = {0.f, 0.f};
@@ -719,12 +651,8 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   const uint32_t lws = __builtin_amdgcn_readfirstlane(lw);
   const Rsrc rr = make_rsrc(inc + sw, lws * kSlice * 8u);
   const Rsrc rx = make_rsrc(Xi, npairs * (uint32_t)sizeof(TI));
-  const Rsrc ro = make_rsrc(Xo, nruns * (uint32_t)sizeof(TO));
-  const Rsrc rp = make_rsrc(wperm + (sw - base_w), lws * kSlice);
-  const uint32_t row0 = (sw - base_w) / kSlice;  // the slice's first window row
-  const uint64_t* __restrict__ whm_s = whm + row0;
-  const uint32_t* __restrict__ roff_s = roff + row0;
-  const uint32_t lr = l * 8u;
+  const Rsrc ro = make_rsrc(Xo + (sw - base_w), lws * kSlice * (uint32_t)sizeof(TO));
+  const uint32_t lr = l * 8u, lo = l * (uint32_t)sizeof(TO);
 
   float lA[UW], xA[UW], lB[UW], xB[UW];
   uint2 e[UW];
@@ -737,12 +665,22 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
     // records in the first window record's brick (every front tau is 0);
     // bit l of word k of bsw: window record k of this lane enters a brick
     // (the slice's rows of the word array, padded past the last slice)
+    // (at most 22 records, the longest crossing of a brick; 8 record loads,
+    // then their 8 gathers, in flight at a time, summed in record order)
     const FrontRec fr = front[ray];
     const uint32_t tl = tail[ray];
+    const uint2* __restrict__ tr = inc + fr.off + (size_t)(fr.nf - tl) * kSlice;
     Kahan kp = {0.f, 0.f};
-    for (uint32_t k = fr.nf - tl; k < fr.nf; ++k) {
-      const uint2 et = inc[fr.off + (size_t)k * kSlice];
-      kp.add(__uint_as_float(et.y) * Msg<MI>::load(Xi, et.x));
+    for (uint32_t b = 0; b < tl; b += 8) {
+      uint2 et[8];
+      float tx[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) et[i] = b + i < tl ? tr[(size_t)(b + i) * kSlice] : make_uint2(0u, 0u);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) tx[i] = b + i < tl ? Msg<MI>::load(Xi, et[i].x) : 0.0f;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (b + i < tl) kp.add(__uint_as_float(et[i].y) * tx[i]);
     }
     kL = kp;
     bsw = bstart + (sw - base_w) / kSlice;
@@ -763,25 +701,21 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   float T = kT.at(0.0f);
   if (rec.cell_flags & kInfTerm) T += __expf(kL.at(0.0f));  // decided at traversal
 
-  // pass 2; each batch's row runs are loaded with its message gathers
+  // pass 2
   kL = L0;
   kS = S0;
   Kahan kP = {0.f, 0.f};
-  RowRuns<UW> qA, qB;
   win_load<UW>(rr, lr, 0, e);
   win_issue<MI, UW>(rx, e, lA, xA);
-  win_runs<UW>(rp, l, 0, lws, whm_s, roff_s, qA);
   win_load<UW>(rr, lr, UW, e);
   for (uint32_t k0 = 0;; k0 += 2 * UW) {
     win_issue<MI, UW>(rx, e, lB, xB);
-    win_runs<UW>(rp, l, k0 + UW, lws, whm_s, roff_s, qB);
     win_load<UW>(rr, lr, k0 + 2 * UW, e);
-    win_pass2<MO, UW>(r, k0, nw, lA, xA, qA, T, ro, l, kL, kS, kP);
+    win_pass2<MO, UW>(r, k0, nw, lA, xA, T, ro, lo, kL, kS, kP);
     if (k0 + UW >= lws) break;
     win_issue<MI, UW>(rx, e, lA, xA);
-    win_runs<UW>(rp, l, k0 + 2 * UW, lws, whm_s, roff_s, qA);
     win_load<UW>(rr, lr, k0 + 3 * UW, e);
-    win_pass2<MO, UW>(r, k0 + UW, nw, lB, xB, qB, T, ro, l, kL, kS, kP);
+    win_pass2<MO, UW>(r, k0 + UW, nw, lB, xB, T, ro, lo, kL, kS, kP);
     if (k0 + 2 * UW >= lws) break;
   }
 }
@@ -795,6 +729,30 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
 // the part that started in an earlier tile goes to pf, the part that
 // continues into the next tile to pl, and a tile lying wholly inside one
 // segment to mid; k_seg_update joins them.
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dppi(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xF, true);
+}
+
+// (x, f) <- (x_left, f_left) (+) (x, f): add the left value unless a segment
+// starts at or before this lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step(float& x, int& f) {
+  const float y = dpp<CTRL, ROWS>(x);
+  const int g = dppi<CTRL, ROWS>(f);
+  x = f ? x : x + y;
+  f |= g;
+}
+
+__device__ __forceinline__ void wave_seg_scan(float& x, int& f) {
+  seg_step<0x111, 0xF>(x, f);
+  seg_step<0x112, 0xF>(x, f);
+  seg_step<0x114, 0xF>(x, f);
+  seg_step<0x118, 0xF>(x, f);
+  seg_step<0x142, 0xA>(x, f);
+  seg_step<0x143, 0xC>(x, f);
+}
 
 __device__ __forceinline__ int wave_scan_i(int x) {
   x += dppi<0x111, 0xF>(x);
@@ -1179,11 +1137,11 @@ void launch_ray_t(const CamPass& a, hipStream_t s, int part) {
   if (a.brick_local)
     hipLaunchKernelGGL((k_ray_window<MI, MO, UW, true>), dim3(nb), dim3(64), 0, s, a.lanes, a.len_w, a.nslices,
                        a.chunk_off, a.partial, a.inc, a.base_w, Xi, a.npairs, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia,
-                       a.lookup_n, a.front, a.bstart, a.tail, a.nwin, a.wperm, a.whm, a.roff);
+                       a.lookup_n, a.front, a.bstart, a.tail);
   else
     hipLaunchKernelGGL((k_ray_window<MI, MO, UW, false>), dim3(nb), dim3(64), 0, s, a.lanes, a.len_w, a.nslices,
                        a.chunk_off, a.partial, a.inc, a.base_w, Xi, a.npairs, (typename Msg<MO>::T*)a.Xo, a.sig, a.bia,
-                       a.lookup_n, a.front, a.bstart, a.tail, a.nwin, a.wperm, a.whm, a.roff);
+                       a.lookup_n, a.front, a.bstart, a.tail);
 }
 
 template <int MI, int MO>

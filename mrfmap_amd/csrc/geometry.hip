@@ -144,17 +144,10 @@ __device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t& x2, uint32
 // pair across its lanes (bitonic network), sums each run of equal pairs
 // (segmented scan) and issues one atomic per run instead of one per record.
 // Regions are whole rows of 64 records, so a row is either all front (its
-// lengths also go to the front sum) or all window.  For a window row the
-// sort also gives the row's runs, which the window kernel uses to sum its
-// messages per pair before storing them: wperm[row * 64 + s] = the lane
-// whose record is the s-th in pair order, whm[row] = the lanes (in pair
-// order) where a run of equal pairs starts (the gap and padding records, pair
-// 0xFFFFFFFF, form the last run), wnr[row] = the runs of live records.
+// lengths also go to the front sum) or all window.
 __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
                                                       const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_c,
-                                                      uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r,
-                                                      uint8_t* __restrict__ wperm, uint64_t* __restrict__ whm,
-                                                      uint32_t* __restrict__ wnr) {
+                                                      uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -171,24 +164,9 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
       q = (uint32_t)x;
       r = (uint32_t)((x - (double)q) * kFixLen);
     }
-    // a window row's lanes travel with their records in r's top byte (r < 2^24)
-    const bool win = base >= base_w;
-    if (win) r |= (uint32_t)lane << 24;
     bitonic_sort<64>(lane, key, q, r);
     const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
     const uint32_t next = (uint32_t)__shfl_down((int)key, 1, 64);
-    if (win) {
-      const uint64_t wi = base - base_w;
-      wperm[wi + lane] = (uint8_t)(r >> 24);
-      r &= 0xFFFFFFu;
-      const bool head = lane == 0 || prev != key;
-      const unsigned long long hm = __ballot(head);
-      const unsigned long long live = __ballot(head && key != 0xFFFFFFFFu);
-      if (lane == 0) {
-        whm[wi >> 6] = hm;
-        wnr[wi >> 6] = (uint32_t)__popcll(live);
-      }
-    }
     wave_seg_sum_u(q, r, lane == 0 || prev != key ? 1u : 0u);
     const bool tail = lane == 63 || next != key;
     if (tail && key != 0xFFFFFFFFu) {
@@ -221,25 +199,15 @@ __global__ void k_pair_finish(const uint32_t* __restrict__ acc_c, const uint32_t
   pair_sf[p] = (float)((double)kFEps * (f / kFixLen));
 }
 
-// The runs of live records of every window row in row order (one wavefront
-// per row): run i of row w holds pair runpair[roff[w] + i]; sort values are
-// the run indices.
-__global__ void __launch_bounds__(256) k_run_pairs(const uint2* __restrict__ recw, const uint8_t* __restrict__ wperm,
-                                                   const uint64_t* __restrict__ whm,
-                                                   const uint32_t* __restrict__ roff, uint64_t nrows,
-                                                   uint32_t* __restrict__ runpair, uint32_t* __restrict__ vals) {
-  const uint64_t row = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (row >= nrows) return;  // uniform per wavefront
-  const int s = threadIdx.x & 63;
-  const uint32_t src = wperm[row * 64 + (uint64_t)s];
-  const uint32_t key = recw[row * 64 + src].x;
-  const uint64_t hm = whm[row];
-  if (((hm >> s) & 1ull) && key != kGapVox) {
-    const uint64_t upto = s == 63 ? ~0ull : ((2ull << s) - 1ull);
-    const uint32_t j = roff[row] + (uint32_t)__popcll(hm & upto) - 1u;
-    runpair[j] = key;
-    vals[j] = j;
-  }
+// sort keys of the window region: the record's pair, nwp (sorts last) for
+// gap and padding records; values: the record's index in the region
+__global__ void k_window_keys(const uint2* __restrict__ recw, uint32_t n, uint32_t sentinel,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t q = recw[j].x;
+  keys[j] = q == kGapVox ? sentinel : q;
+  vals[j] = j;
 }
 
 __global__ void k_window_heads(const uint32_t* __restrict__ keys, uint32_t nwin, uint32_t* __restrict__ whead) {
@@ -307,25 +275,24 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
 
 void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
                          uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
-                         uint8_t* wperm, uint64_t* whm, uint32_t* wnr, hipStream_t s) {
+                         hipStream_t s) {
   if (np == 0) return;
   launch_zero_words(acc_c, (size_t)np * 8, s);
   launch_zero_words(acc_f, (size_t)np * 8, s);
   launch_zero_words(acc_r, (size_t)np * 8, s);
   hipLaunchKernelGGL(k_pair_lengths, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_c, acc_f,
-                     acc_r, wperm, whm, wnr);
+                     acc_r);
   hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_c, acc_f, acc_r, np, pair_c,
                      pair_sf);
 }
 
-hipError_t sort_window_runs(const uint2* recw, const uint8_t* wperm, const uint64_t* whm, const uint32_t* roff,
-                            uint64_t nrows, uint32_t nruns, int bits, uint32_t* keys_in, uint32_t* keys_out,
-                            uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, hipStream_t s) {
-  if (nruns == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_run_pairs, dim3(blocks(nrows * 64, 256)), dim3(256), 0, s, recw, wperm, whm, roff, nrows,
-                     keys_in, vals_in);
+hipError_t sort_window(const uint2* recw, uint32_t nwr, uint32_t nwp, int bits, uint32_t* keys_in,
+                       uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
+                       hipStream_t s) {
+  if (nwr == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_window_keys, dim3(blocks(nwr, 256)), dim3(256), 0, s, recw, nwr, nwp, keys_in, vals_in);
   size_t tb = temp_bytes;
-  return hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (int)nruns, 0, bits, s);
+  return hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (int)nwr, 0, bits, s);
 }
 
 void launch_window_heads(const uint32_t* keys, uint32_t nwin, uint32_t* whead, hipStream_t s) {

@@ -52,10 +52,14 @@ constexpr uint32_t kChunk = 16;  // front records per lane in one chunk of the f
 
 // A record belongs to the front while d - (s + bias(s)) > kFrontU * 3 sigma(s)
 // at its exit depth s: then psi(s) = 0.398942 exp(-u^2/2) / (3 sigma) has
-// u > 15 and exp(-112.5) is below the smallest FP32 subnormal, so psi, tau
-// and the message's dependence on the ray vanish exactly (the message is the
-// clamp floor F_EPS * l).
-constexpr float kFrontU = 15.0f;
+// u > 13.5, exp(-u^2/2) < 2.7e-40 is an FP32 subnormal and psi < 1e-37 for
+// any sigma above 1 mm.  The record's message is the clamp floor F_EPS * l
+// (mu1 = P + V psi lies 30 orders of magnitude below the normaliser's 1e-7),
+// and its tau, left out of T and of the later records' P, is below the FP32
+// rounding of any T that can lift a message above the floor.  With the
+// reference's flush-to-zero arithmetic such a tau is exactly 0 (its factor
+// 0.398942 exp(-u^2/2) is below 2^-126 from u = 13.15 on).
+constexpr float kFrontU = 13.5f;
 constexpr uint32_t kRanOut = 0x80000000u;  // ray left the active space before truncation
 // the ray ran out and its depth lies beyond the exit: the infinity term applies
 // (gvdbkernels.cu:1196-1208), decided in FP64 from the exact exit depth

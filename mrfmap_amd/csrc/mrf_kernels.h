@@ -73,18 +73,16 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
 // per pair: C (summed chord length) and F_EPS x the chord length of its
 // front records (index < base_w), exact fixed-point sums; acc_c / acc_f /
 // acc_r (C below 2^-24) hold 2 words per pair.  Rewrites every live record's
-// voxel to its pair (pid).  Per window row (the region from base_w on): the
-// lanes in pair order (wperm, one byte per record), the lanes where a run of
-// equal pairs starts (whm) and the runs of live records (wnr).
+// voxel to its pair (pid).
 void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
                          uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
-                         uint8_t* wperm, uint64_t* whm, uint32_t* wnr, hipStream_t s);
+                         hipStream_t s);
 size_t window_sort_temp_bytes(uint32_t n, int bits);
-// the runs of the window rows (run i of row w at roff[w] + i) stably sorted
-// by pair: keys_out = pairs, vals_out = run index
-hipError_t sort_window_runs(const uint2* recw, const uint8_t* wperm, const uint64_t* whm, const uint32_t* roff,
-                            uint64_t nrows, uint32_t nruns, int bits, uint32_t* keys_in, uint32_t* keys_out,
-                            uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes, hipStream_t s);
+// window region records (holding pairs) stably sorted by pair (gap /
+// padding records last): keys_out = pairs, vals_out = record index in the region
+hipError_t sort_window(const uint2* recw, uint32_t nwr, uint32_t nwp, int bits, uint32_t* keys_in,
+                       uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
+                       hipStream_t s);
 void launch_window_heads(const uint32_t* keys, uint32_t nwin, uint32_t* whead, hipStream_t s);
 // wseg_off[pair] = first window position of the pair, wseg_off[nwp] = nwin
 void launch_window_offsets(const uint32_t* keys, const uint32_t* whead, uint32_t nwin, uint32_t nwp,
@@ -104,7 +102,7 @@ struct CamPass {
   const uint32_t* len_w;   // window records per lane of each slice
   uint32_t nslices;
   const uint2* inc;        // sliced records {pair | kGapVox, chord length}: front region, then window region
-  const uint32_t* win_idx; // per window position (voxel order): its run
+  const uint32_t* win_idx; // per window position (voxel order): its record's slot in the window region
   uint32_t base_w;         // first record of the window region
   const uint32_t* chunk_off;   // nslices + 1: first front chunk of each slice
   const uint4* chunk_desc;     // per front chunk: {chunk, slice, first row, end row}
@@ -116,10 +114,7 @@ struct CamPass {
   const float* pair_sf;    // summed constant front messages per pair
   uint32_t npairs;
   uint32_t nwp;            // pairs with window incidences (numbered first)
-  uint32_t nwin;           // window positions: runs of equal pairs in the window rows
-  const uint8_t* wperm;    // per window record slot: the row's lanes in pair order
-  const uint64_t* whm;     // per window row: lanes (in pair order) where a run starts
-  const uint32_t* roff;    // per window row + 1: the row's first run
+  uint32_t nwin;           // window positions (live window incidences)
   const uint32_t* wseg_off;  // nwp + 1 window segment offsets
   const uint64_t* heads;   // window segment head bitmask
   const uint32_t* tile_h0; // window heads before each kTile tile
