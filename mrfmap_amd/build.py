@@ -76,8 +76,27 @@ def build_synth(verbose=False):
     return out
 
 
+def build_cpp_runner(verbose=False):
+    """tests/cpp/run_workload: a caller of the C++ surface (include/mrfmap_hip.hpp)
+    that the GPU tests run; linked against libmrfmap_hip.so beside this file."""
+    src = os.path.join(ROOT, "tests", "cpp", "run_workload.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "run_workload")
+    tmp = out + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), src, "-o", tmp, "-L", HERE,
+           "-lmrfmap_hip", "-Wl,-rpath,$ORIGIN/../../mrfmap_amd"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(verbose=False):
-    return [build_synth(verbose), build_hip(verbose)]
+    outs = [build_synth(verbose), build_hip(verbose)]
+    runner = build_cpp_runner(verbose)
+    return outs + ([runner] if runner else [])
 
 
 if __name__ == "__main__":

@@ -8,7 +8,7 @@
         Achievable HBM bandwidth of plain torch kernels on N GiB buffers:
         copy and add (read and write N GiB), fill (writes N GiB) and sum
         (reads N GiB), best of R timed repetitions (HIP events), in GB/s.
-    python -m mrfmap_amd.profiling summarize DIR [--calib CALDIR] [--kernels A,B]
+    python -m mrfmap_amd.profiling summarize DIR [--calib CALDIR] [--kernels A,B] [--first N]
         Markdown summary of rocprofv3 runs kept under DIR: trace/ (from
         --kernel-trace --stats), fetch/ and write/ (from --pmc FETCH_SIZE and
         --pmc WRITE_SIZE, csv output).  With --calib, CALDIR/fetch and
@@ -16,7 +16,9 @@
         of every kernel are then scaled by 1 GiB / (counter of the copy).
         Also prints one JSON line with the calibrated HBM bytes per launch of
         the kernel families in --kernels (default: the front-sum and window
-        kernels of one camera pass), summed.
+        kernels of one camera pass), summed.  With --first N, each kernel's
+        figures use its first N dispatches only (from the per-dispatch trace),
+        e.g. the 20 x 13 camera passes of the bench's first timed call.
 """
 import argparse
 import collections
@@ -51,22 +53,40 @@ def _find(d, suffix):
     return None
 
 
-def counters(d):
-    """{kernel short name: [values]} of the single counter collected under d."""
+def counters(d, first=None):
+    """{kernel short name: [values]} of the single counter collected under d
+    (with first: each kernel's first `first` dispatches)."""
     path = _find(d, "counter_collection.csv")
     if path is None:
         return {}, None
     out = collections.defaultdict(list)
     cname = None
-    for r in _read_csv(path):
-        out[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    for r in sorted(_read_csv(path), key=lambda r: int(r["Dispatch_Id"])):
+        v = out[short_name(r["Kernel_Name"])]
+        if not first or len(v) < first:
+            v.append(float(r["Counter_Value"]))
         cname = r["Counter_Name"]
     return out, cname
 
 
-def kernel_stats(d):
-    path = _find(d, "kernel_stats.csv")
-    return _read_csv(path) if path else []
+def kernel_stats(d, first=None):
+    """rocprofv3's per-kernel statistics; with first, recomputed from the
+    per-dispatch trace over each kernel's first `first` dispatches."""
+    if not first:
+        path = _find(d, "kernel_stats.csv")
+        return _read_csv(path) if path else []
+    path = _find(d, "kernel_trace.csv")
+    if path is None:
+        return []
+    durs = collections.OrderedDict()
+    for r in sorted(_read_csv(path), key=lambda r: int(r["Dispatch_Id"])):
+        v = durs.setdefault(r["Kernel_Name"], [])
+        if len(v) < first:
+            v.append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    total = sum(sum(v) for v in durs.values()) or 1
+    rows = [{"Name": k, "Calls": str(len(v)), "TotalDurationNs": str(sum(v)), "AverageNs": str(sum(v) / len(v)),
+             "Percentage": str(100.0 * sum(v) / total)} for k, v in durs.items()]
+    return sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))
 
 
 def calibration(caldir):
@@ -82,10 +102,10 @@ def calibration(caldir):
     return fac
 
 
-def summarize(d, caldir=None, kernels=("k_front_sums", "k_ray_window"), top=16):
+def summarize(d, caldir=None, kernels=("k_front_sums", "k_ray_window"), top=16, first=None):
     lines = []
     parts = []
-    stats = kernel_stats(os.path.join(d, "trace"))
+    stats = kernel_stats(os.path.join(d, "trace"), first)
     if stats:
         lines.append("| kernel | calls | avg ms | total ms | % |")
         lines.append("|---|---:|---:|---:|---:|")
@@ -93,8 +113,8 @@ def summarize(d, caldir=None, kernels=("k_front_sums", "k_ray_window"), top=16):
             lines.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
                          f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.1f} |")
     fac = calibration(caldir) if caldir else {}
-    fetch, _ = counters(os.path.join(d, "fetch"))
-    write, _ = counters(os.path.join(d, "write"))
+    fetch, _ = counters(os.path.join(d, "fetch"), first)
+    write, _ = counters(os.path.join(d, "write"), first)
     traffic = None
     if fetch or write:
         lines.append("")
@@ -179,6 +199,7 @@ def main(argv=None):
     s.add_argument("--calib")
     s.add_argument("--kernels", default="k_front_sums,k_ray_window",
                    help="comma-separated kernel families whose per-launch bytes are summed")
+    s.add_argument("--first", type=int, default=None, help="each kernel's first N dispatches only")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_once()
@@ -186,7 +207,7 @@ def main(argv=None):
     if a.cmd == "stream":
         stream(a.gib, a.reps)
         return 0
-    text, traffic = summarize(a.dir, a.calib, tuple(a.kernels.split(",")))
+    text, traffic = summarize(a.dir, a.calib, tuple(a.kernels.split(",")), first=a.first)
     print(text)
     if traffic:
         print()

@@ -3,6 +3,7 @@ converged maps, training on the GPU, held-out evaluation.
 
     python -m mrfmap_amd.warmstart train --out FILE [--small N] [--real N] [--batched N]
     python -m mrfmap_amd.warmstart eval [--weights FILE] [--workload small] [--seeds 2000-2019]
+    python -m mrfmap_amd.warmstart convergence [--weights FILE] [--workload real] [--iters 5,10,...]
 
 A training sample is one touched voxel of a synthetic map: its 32 features
 after ``warm_after`` plain BP iterations (mrf_features) and, as target, the logit of its
@@ -184,6 +185,67 @@ def evaluate(params, workload="small", seeds=HELDOUT_SEEDS, thresh=1e-3, msg=_li
     return rows
 
 
+def convergence(p, frames, ref, iters, params=None, warm_after=1, thresh=1e-3):
+    """Per voxel, the first of the sampled iteration counts ``iters`` from
+    which on its marginal stays within ``thresh`` of ``ref`` (a fresh context
+    per count: a call resets the camera messages, so k iterations of one call
+    are the state after k).  Returns (index into iters per voxel, len(iters)
+    for a voxel still outside at the last count; the max error per count)."""
+    E = []
+    for k in iters:
+        g = build(p, frames)
+        try:
+            if params is None:
+                g.perform_inference(max_iters=k)
+            else:
+                g.perform_inference_warm(params, max_iters=k, warm_after=warm_after, growth=0.0)
+            E.append(np.abs(g.get_alphas().ravel() - ref.ravel()))
+        finally:
+            g.close()
+    bad = np.stack(E) > thresh
+    # some[k]: outside at count k or later; the first count from which on a
+    # voxel stays inside is the number of such k
+    some = np.cumsum(bad[::-1], axis=0)[::-1] > 0
+    return some.sum(axis=0), [float(e.max()) for e in E]
+
+
+def convergence_report(workload="real", seed=None, iters=(5, 10, 20, 40, 60, 80, 100, 120, 140, 160, 200),
+                       params=None, warm_after=1, thresh=1e-3, log=print):
+    """Where the iterations to a map within thresh of the converged one go:
+    histograms of the per-voxel convergence count for plain and (with
+    params) warm-started BP, and, for the voxels that converge last, their
+    converged marginal and their distance to the nearest depth endpoint."""
+    kw = {} if seed is None else {"seed": seed}
+    p, frames, _ = synth.make_workload(workload, **kw)
+    ref, st = reference_map(p, frames)
+    log(f"[warmstart] reference: {st['iters']} iterations, residual {st['residual']:.2e}")
+    g = build(p, frames)
+    try:
+        g.perform_inference(max_iters=1)
+        X, vox = g.features()
+    finally:
+        g.close()
+    dist = np.full(ref.size, -1.0, np.float32)
+    dist[vox] = X[:, 4] * 4.0  # distance to the nearest endpoint voxel, capped at 4 voxels
+    out = {"workload": workload, "thresh": thresh, "iters": list(iters), "ref_iters": int(st["iters"]),
+           "ref_residual": float(st["residual"])}
+    runs = [("plain", None)] + ([("warm", params)] if params is not None else [])
+    for name, prm in runs:
+        k, emax = convergence(p, frames, ref, iters, params=prm, warm_after=warm_after, thresh=thresh)
+        hist = np.bincount(k, minlength=len(iters) + 1)
+        late = k >= np.searchsorted(iters, 40)  # converging after 40 iterations or not at all
+        rl = ref.ravel()[late]
+        dl = dist[late]
+        rep = {"hist": [int(x) for x in hist], "max_error": emax, "late_voxels": int(late.sum()),
+               "late_ref_marginal_quantiles": [float(x) for x in np.quantile(rl, [0.1, 0.5, 0.9])] if late.any() else [],
+               "late_by_distance": {str(d): int((dl == d).sum()) for d in (0.0, 1.0, 2.0, 3.0, 4.0)},
+               "late_untouched_by_features": int((dl < 0).sum())}
+        out[name] = rep
+        log(f"[warmstart] {name}: converged-by histogram {rep['hist']} (last bin: not within {thresh} at "
+            f"{iters[-1]}); max errors {['%.1e' % e for e in emax]}")
+    return out
+
+
 def _seeds(s):
     a, _, b = s.partition("-")
     return tuple(range(int(a), int(b) + 1)) if b else (int(a),)
@@ -203,6 +265,10 @@ def main(argv=None):
     t.add_argument("--cap-real", type=int, default=600000)
     t.add_argument("--eval-seeds", default="2000-2004")
     t.add_argument("--warm-after", type=int, default=1)
+    cv = sub.add_parser("convergence")
+    cv.add_argument("--weights", default=None)
+    cv.add_argument("--workload", default="real")
+    cv.add_argument("--iters", default="5,10,20,40,60,80,100,120,140,160,200")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=None)
     e.add_argument("--workload", default="small")
@@ -230,6 +296,12 @@ def main(argv=None):
         log(f"[warmstart] trained in {t_train:.1f} s -> {a.out}")
         rows = evaluate(params, "small", _seeds(a.eval_seeds), log=log, warm_after=a.warm_after)
         print(json.dumps({"eval_small": rows}))
+        return 0
+    if a.cmd == "convergence":
+        params, meta = load_weights(a.weights)
+        rep = convergence_report(a.workload, iters=tuple(int(x) for x in a.iters.split(",")), params=params,
+                                 warm_after=meta.get("warm_after", 1), log=log)
+        print(json.dumps(rep))
         return 0
     params, meta = load_weights(a.weights)
     rows = evaluate(params, a.workload, _seeds(a.seeds), msg=a.msg, log=log, warm_after=meta.get("warm_after", 1),

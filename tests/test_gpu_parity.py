@@ -72,6 +72,36 @@ def test_uneven_keyframes_match_oracle():
     assert err.max() <= MARG_TOL
 
 
+def test_uneven_keyframes_at_1cm_match_oracle():
+    """The batched workload at its own resolution (1 cm, 10x10x3 m, depth up
+    to 10 m, so rays of more than a thousand voxels): the first keyframe of
+    each image size among its 64 (848x480 with the border mask, 320x240,
+    640x480, each with its own intrinsics), every 8th pixel row and column.
+    Traversal bit-exact per camera, marginals after 3 iterations within
+    1e-3 of the FP64 oracle."""
+    p, frames, _ = synth.make_workload("batched")
+    first = {}
+    for i, (_, _, d) in enumerate(frames):
+        first.setdefault(d.shape, i)
+    assert len(first) == 3, first
+    frames = [(frames[i][0], frames[i][1], decimate(frames[i][2], 8)) for i in sorted(first.values())]
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    n = _compare_traversal(g, o, len(frames))
+    longest = max(int(o.traversal(c)[0][:, 1].max()) for c in range(len(frames)))
+    assert n > 5_000_000 and longest > 1000, (n, longest)
+    st = g.perform_inference(max_iters=3)
+    so = o.perform_inference(3)
+    assert st["updates_per_iter"] == so["updates_per_iter"]
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"uneven keyframes at 1 cm: {n} records, longest ray {longest} records, max-abs marginal error "
+          f"{err.max():.3e}")
+    assert err.max() <= MARG_TOL
+
+
 def test_traversal_bit_exact_real_shape():
     """640x480, f=525, 1 cm voxels: two keyframes, every 3rd pixel."""
     p, frames, _ = synth.make_workload("real", frames=13)
