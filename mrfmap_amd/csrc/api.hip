@@ -45,7 +45,7 @@ struct Camera {
   uint32_t nrays = 0;              // rays with at least one record
   uint2* d_inc = nullptr;          // sliced records {pair | gap, length}: front region, then window region
   uint32_t* d_walkn = nullptr;     // full-ray activation: records per tiled pixel, then has-records flags
-  uint8_t* d_lnn = nullptr;        // per pair: voxel -> ray message ln n (input-side message type)
+  uint8_t* d_lnn = nullptr;        // per pair: voxel -> ray message log2 n (input-side message type)
   uint32_t* d_dyn = nullptr;       // front-only pairs at voxels where some camera has window records
   uint32_t ndyn = 0;
   float* d_partial = nullptr;      // front partial sums (64 per chunk), kept between iterations
@@ -660,7 +660,7 @@ int prepare(mrf_ctx* c) {
   for (Camera& cam : c->cams)
     if ((rc = traverse_camera(c, cam))) return rc;
   // 3b. voxels where some camera has window records; per camera, its
-  //     front-only pairs there (their ln n keeps changing after the first
+  //     front-only pairs there (their log2 n keeps changing after the first
   //     iterations of a call, the others' stops)
   {
     const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
@@ -670,7 +670,7 @@ int prepare(mrf_ctx* c) {
     uint32_t* W = c->d_vflag;
     uint32_t* flags = W + nflag;   // per front-only pair: at a voxel of W
     uint32_t* pos = flags + nflag;
-    uint32_t* cflags = pos + nflag;  // per front chunk: some ln n of it keeps changing
+    uint32_t* cflags = pos + nflag;  // per front chunk: some log2 n of it keeps changing
     launch_zero_words(W, nvox * 4, c->stream);
     for (const Camera& cam : c->cams) launch_mark_window_voxels(cam.d_pair_vox, cam.nwp, W, c->stream);
     for (size_t ci = 0; ci < c->cams.size(); ++ci) {
@@ -686,7 +686,7 @@ int prepare(mrf_ctx* c) {
       if ((rc = dalloc(c, &cam.d_dyn, std::max<uint64_t>(ndyn, 1)))) return rc;
       cam.ndyn = (uint32_t)ndyn;
       launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
-      // the front chunks whose ln n keep changing, as a list
+      // the front chunks whose log2 n keep changing, as a list
       launch_zero_words(c->d_total + 2, 8, c->stream);
       launch_chunk_static(cam_pass(c, ci), flags, cflags, (unsigned long long*)(c->d_total + 2), c->stream);
       uint64_t n = 0;
@@ -1344,7 +1344,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
   // first iteration refreshes every pair's message
   for (uint32_t id : sel) launch_zero_words(c->cams[id].d_M, (size_t)c->cams[id].npairs * sizeof(float), c->stream);
   launch_fill_u32(c->d_all_pairs, 1u, 1, c->stream);      // every pair's update
-  launch_fill_u32(c->d_all_pairs + 1, 2u, 1, c->stream);  // every pair's ln n, two iterations
+  launch_fill_u32(c->d_all_pairs + 1, 2u, 1, c->stream);  // every pair's log2 n, two iterations
   launch_marginals(c->d_B, c->d_A, nvox, c->stream);
   for (const Camera& cam : c->cams) {
     s.updates_per_iter += (int64_t)cam.nupd;
@@ -1447,7 +1447,7 @@ static int run_inference(mrf_ctx* c, const uint32_t* ids, int n_ids, int max_ite
     } else if (warm_state == 1 && it > warm_after && growth > 0 && resid > growth * r_prev && resid > 0.05) {
       // the residual grows: back to the plain state before the warm start
       // (the restored front-only messages are their constants already; the
-      // stored ln n of the warm iterations are recomputed once)
+      // stored log2 n of the warm iterations are recomputed once)
       if ((rc = snapshot(c, false))) return rc;
       launch_marginals(c->d_B, c->d_A, nvox, c->stream);
       launch_fill_u32(c->d_all_pairs + 1, 1u, 1, c->stream);

@@ -1,8 +1,8 @@
 // Loopy-BP message kernels of the MRFMap inference path.
 //
-//  * k_pair_msg: voxel->ray message ln n of every (camera, voxel) pair
+//  * k_pair_msg: voxel->ray message log2 n of every (camera, voxel) pair
 //    (gvdbkernels.cu:764-773) into a compact per-pair array.
-//  * k_ray_messages: L lanes per ray over sliced-ELL pixel tiles; reads ln n
+//  * k_ray_messages: L lanes per ray over sliced-ELL pixel tiles; reads log2 n
 //    of each record's pair, writes p*l into the record's slot of the
 //    camera's voxel-sorted segment array.  Replaces
 //    the pre pass (rayStartComputingMessagesFromBrickPre,
@@ -85,12 +85,16 @@ struct Poly {
   __device__ __forceinline__ float operator()(float z) const { return fmaf(z, fmaf(z, c2, c1), c0); }
 };
 
+// 0.398942 e^{-u^2/2}, the exponential as a power of 2 with the constants
+// folded: e^{-u^2/2} = 2^{u^2 (-log2(e) / 2)}
+constexpr float kNegHalfLog2e = -0.72134752044448170f;
+__device__ __forceinline__ float gauss_a(float u) { return 0.398942f * __builtin_amdgcn_exp2f((u * u) * kNegHalfLog2e); }
+
 // psi(s; d) = N(d; s + bias(s), (3 sigma(s))^2) (depth_potential, gvdbkernels.cu:57-72).
 __device__ __forceinline__ float depth_psi(float s, float d, const Poly& sig, const Poly& bia) {
   const float diff = fabsf(d - (s + bia(s)));
   const float inv = __builtin_amdgcn_rcpf(3.0f * sig(s));
-  const float u = diff * inv;
-  return 0.398942f * __expf(-0.5f * u * u) * inv;
+  return gauss_a(diff * inv) * inv;
 }
 
 // Kahan-compensated running sum of the per-chunk totals along a ray.
@@ -123,8 +127,11 @@ __device__ __forceinline__ uint32_t pair_of(uint64_t word, uint32_t before, int 
   return before + (uint32_t)__popcll(word & upto) - 1u;
 }
 
-// Voxel -> ray message of every pair: ln n = ln clamp(1/(1+exp(B - M_c))),
-// camera c's own previous message excluded from the belief (gvdbkernels.cu:764-773).
+// Voxel -> ray message of every pair: n = clamp(1/(1+exp(B - M_c))), camera
+// c's own previous message excluded from the belief (gvdbkernels.cu:764-773),
+// stored as log2 n (the reference's ln n / ln 2): the ray kernels sum
+// l log2 n and take exp2 of the sums (one multiply fewer per exponential than
+// e^x = 2^(x log2 e)).
 template <int MT>
 __device__ __forceinline__ void pair_msg_one(uint32_t q, const uint32_t* __restrict__ pair_vox,
                                              const float* __restrict__ B, const float* __restrict__ M,
@@ -133,7 +140,7 @@ __device__ __forceinline__ void pair_msg_one(uint32_t q, const uint32_t* __restr
   const float qv = B[v] - M[q];
   float nn = 1.0f / (1.0f + __expf(qv));
   nn = fminf(fmaxf(nn, kFEps), 1.0f - kFEps);
-  Msg<MT>::store(apair, q, __logf(nn));
+  Msg<MT>::store(apair, q, __builtin_amdgcn_logf(nn));
 }
 
 // The window pairs [0, nwp) change every iteration; of the front-only pairs,
@@ -315,7 +322,7 @@ __global__ void k_gather_desc(const uint4* __restrict__ desc, const uint32_t* __
 }
 
 // Front sums.  Front records have psi(s) = 0, so all they contribute to a ray
-// is the visibility sum Lambda = sum l ln n (the reference's visibility
+// is the visibility sum Lambda = sum l log2 n (in units of ln 2; the reference's visibility
 // includes the voxel itself, gvdbkernels.cu:781-783); their messages are
 // the constant F_EPS * l, summed per pair once per traversal, and their
 // length sum is geometric (RayRec::ks/kc).  Chunk c of a slice is records
@@ -330,8 +337,8 @@ __global__ __launch_bounds__(256) void k_front_sums(const uint2* __restrict__ fs
                                                     const typename Msg<MI>::T* __restrict__ Xi,
                                                     float* __restrict__ partial,
                                                     const uint32_t* __restrict__ msg_all) {
-  // every chunk while the ln n of all pairs are recomputed, afterwards only
-  // the chunks with a pair whose ln n still changes (the others keep their
+  // every chunk while the log2 n of all pairs are recomputed, afterwards only
+  // the chunks with a pair whose log2 n still changes (the others keep their
   // partial sums): a wave per entry of the descriptor list (one wave-uniform
   // load per chunk), the waves past the list's end exit at once.  The grid
   // covers every chunk; the loop strides for a smaller one.
@@ -367,7 +374,7 @@ __global__ __launch_bounds__(256) void k_front_sums(const uint2* __restrict__ fs
 
 // Per front chunk (the layout of k_front_sums): 0 when every record of every
 // lane belongs to a front-only pair at a voxel that no camera reaches with
-// window records (pflag[q - nwp] == 0), i.e. when its ln n stop changing
+// window records (pflag[q - nwp] == 0), i.e. when its log2 n stop changing
 // after the first two iterations of a call and its partial sums can be
 // kept; 1 otherwise.
 __global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ fspan,
@@ -464,11 +471,12 @@ struct MsgB<kMsgBF16> {
 // of the slice's window region, and the pair messages.  Loads past the
 // region read {0, 0} (pair 0, length 0); a gap record's pair index is past
 // the message array, so its message reads 0, and since every pair's
-// ln n <= ln(1 - F_EPS) < 0, a record is live exactly when its message is
+// log2 n <= log2(1 - F_EPS) < 0, a record is live exactly when its message is
 // nonzero.  All lanes run to the slice's longest ray (uniform loop control);
-// slots past a lane's last record are computed and discarded by selects,
-// their stores land in the slice's padding or past the region (dropped),
-// and no kernel reads those slots.
+// slots past a lane's last record are computed too: they have length 0, so
+// the visibility and length sums add exact zeros, their taus are left out
+// of T and P by a select, their stores land in the slice's padding or past
+// the region (dropped), and no kernel reads those slots.
 // UW records of a lane from row k0 on (record k at byte lane_off + k * 64 * 8)
 template <int UW>
 __device__ __forceinline__ void win_load(Rsrc rr, uint32_t lane_off, uint32_t k0, uint2 (&e)[UW]) {
@@ -488,42 +496,34 @@ __device__ __forceinline__ void win_issue(Rsrc rx, const uint2 (&e)[UW], float (
   }
 }
 
-// Kahan add that keeps the state unchanged when !p
-__device__ __forceinline__ void kahan_add_if(Kahan& k, bool p, float x) {
-  const float y = x - k.c;
-  const float t = k.s + y;
-  const float c = (t - k.s) - y;
-  k.s = p ? t : k.s;
-  k.c = p ? c : k.c;
-}
-
 struct WinRay {
   float z0, inv, d;
   Poly sig, bia;
 };
 
-// terms of one record shared by both passes: a = l ln n, vpsi, n
+// terms of one record shared by both passes: a = l log2 n, vpsi, n
 __device__ __forceinline__ void win_terms(const WinRay& r, const Kahan& kL, const Kahan& kS, float ln, float lnn,
                                           float& a, float& vpsi, float& nn) {
   a = ln * lnn;
   const float s = r.z0 + kS.at(ln) * r.inv;
   const float ps = depth_psi(s, r.d, r.sig, r.bia);
-  // exp(.) ps is finite, so the product with 0 is exactly 0 (no branch)
-  vpsi = __expf(kL.at(a)) * ps * (lnn != 0.0f ? 1.0f : 0.0f);
-  nn = __expf(lnn);
+  // a gap or padding record (log2 n = 0) gets a vpsi but n = 1 exactly, so
+  // its tau vpsi (1 - n) is exactly 0; its own message is never read
+  vpsi = __builtin_amdgcn_exp2f(kL.at(a)) * ps;
+  nn = __builtin_amdgcn_exp2f(lnn);
 }
 
 template <int UW>
 __device__ __forceinline__ void win_pass1(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
-                                          const float (&x)[UW], Kahan& kL, Kahan& kS, Kahan& kT) {
+                                          const float (&x)[UW], Kahan& kL, Kahan& kS, float& T) {
 #pragma unroll
   for (int u = 0; u < UW; ++u) {
     const bool valid = k0 + u < nw;
     float a, vpsi, nn;
     win_terms(r, kL, kS, ln[u], x[u], a, vpsi, nn);
-    kahan_add_if(kT, valid, vpsi * (1.0f - nn));
-    kahan_add_if(kL, valid, a);
-    kahan_add_if(kS, valid, ln[u]);
+    T += valid ? vpsi * (1.0f - nn) : 0.0f;
+    kL.add(a);  // a = 0 and l = 0 past the lane's last record: no select needed
+    kS.add(ln[u]);
   }
 }
 
@@ -540,7 +540,7 @@ constexpr float kFtzMin = 0x1p-126f;
 template <int UW>
 __device__ __forceinline__ void win_pass1_bl(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
                                              const float (&x)[UW], const uint64_t* __restrict__ bsw, uint32_t l,
-                                             Kahan& kL, Kahan& kS, Kahan& kT, bool& nz) {
+                                             Kahan& kL, Kahan& kS, float& T, bool& nz) {
 #pragma unroll
   for (int u = 0; u < UW; ++u) {
     const bool valid = k0 + u < nw;
@@ -553,34 +553,34 @@ __device__ __forceinline__ void win_pass1_bl(const WinRay& r, uint32_t k0, uint3
     const float diff = fabsf(r.d - (s + r.bia(s)));
     const float inv = __builtin_amdgcn_rcpf(3.0f * r.sig(s));
     const float uu = diff * inv;
-    const float ap = 0.398942f * __expf(-0.5f * uu * uu);
+    const float ap = gauss_a(uu);
     const float ps = ap * inv;  // depth_psi's product order
-    const float V = __expf(kL.at(a));
-    const float nn = __expf(lnn);
+    const float V = __builtin_amdgcn_exp2f(kL.at(a));
+    const float nn = __builtin_amdgcn_exp2f(lnn);
     const float b = V * (1.0f - nn);
-    const bool counted = valid && lnn != 0.0f && ap >= kFtzMin && ps >= kFtzMin && b >= kFtzMin &&
-                         b * ps >= kFtzMin;
-    kahan_add_if(kT, counted, (V * ps) * (1.0f - nn));
+    // every factor at least 2^-126 (log2 n = 0, a gap or padding record, gives b = 0)
+    const bool counted = valid && fminf(fminf(ap, ps), fminf(b, b * ps)) >= kFtzMin;
+    T += counted ? (V * ps) * (1.0f - nn) : 0.0f;
     nz = nz || counted;
-    kahan_add_if(kL, valid, a);
-    kahan_add_if(kS, valid, ln[u]);
+    kL.add(a);  // a = 0 and l = 0 past the lane's last record: no select needed
+    kS.add(ln[u]);
   }
 }
 
 template <int UW, bool BL>
 __device__ __forceinline__ void win_pass1_any(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
                                               const float (&x)[UW], const uint64_t* __restrict__ bsw, uint32_t l,
-                                              Kahan& kL, Kahan& kS, Kahan& kT, bool& nz) {
+                                              Kahan& kL, Kahan& kS, float& T, bool& nz) {
   if constexpr (BL)
-    win_pass1_bl<UW>(r, k0, nw, ln, x, bsw, l, kL, kS, kT, nz);
+    win_pass1_bl<UW>(r, k0, nw, ln, x, bsw, l, kL, kS, T, nz);
   else
-    win_pass1<UW>(r, k0, nw, ln, x, kL, kS, kT);
+    win_pass1<UW>(r, k0, nw, ln, x, kL, kS, T);
 }
 
 template <int MO, int UW>
 __device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t nw, const float (&ln)[UW],
                                           const float (&x)[UW], float T, Rsrc ro, uint32_t lane_off, Kahan& kL,
-                                          Kahan& kS, Kahan& kP) {
+                                          Kahan& kS, float& P) {
   typedef typename Msg<MO>::T TO;
 #pragma unroll
   for (int u = 0; u < UW; ++u) {
@@ -588,16 +588,15 @@ __device__ __forceinline__ void win_pass2(const WinRay& r, uint32_t k0, uint32_t
     float a, vpsi, nn;
     win_terms(r, kL, kS, ln[u], x[u], a, vpsi, nn);
     const float tau = vpsi * (1.0f - nn);
-    const float P = kP.at(0.0f);
     const float R = (T - P) - tau;
     const float mu1 = P + vpsi;
     const float mu0 = P + R * __builtin_amdgcn_rcpf(nn);
     float p = mu1 * __builtin_amdgcn_rcpf(mu1 + mu0 + kMsgNormEps);
     p = fminf(fmaxf(p, kFEps), 1.0f - kFEps);
     MsgB<MO>::store(ro, lane_off + (k0 + (uint32_t)u) * (kSlice * (uint32_t)sizeof(TO)), p * ln[u] * Msg<MO>::kOut);
-    kahan_add_if(kL, valid, a);
-    kahan_add_if(kS, valid, ln[u]);
-    kahan_add_if(kP, valid, tau);
+    kL.add(a);  // a = 0 and l = 0 past the lane's last record: no select needed
+    kS.add(ln[u]);
+    P += valid ? tau : 0.0f;
   }
 }
 
@@ -657,7 +656,7 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   float lA[UW], xA[UW], lB[UW], xB[UW];
   uint2 e[UW];
   // pass 1
-  Kahan kT = {0.f, 0.f};
+  float T = 0.0f;  // sum of the taus: positive terms, a plain sum
   bool nz = false;
   const uint64_t* bsw = nullptr;
   if constexpr (BL) {
@@ -691,31 +690,30 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   for (uint32_t k0 = 0;; k0 += 2 * UW) {  // bounded by the slice's longest ray: uniform control flow
     win_issue<MI, UW>(rx, e, lB, xB);
     win_load<UW>(rr, lr, k0 + 2 * UW, e);
-    win_pass1_any<UW, BL>(r, k0, nw, lA, xA, bsw, l, kL, kS, kT, nz);
+    win_pass1_any<UW, BL>(r, k0, nw, lA, xA, bsw, l, kL, kS, T, nz);
     if (k0 + UW >= lws) break;
     win_issue<MI, UW>(rx, e, lA, xA);
     win_load<UW>(rr, lr, k0 + 3 * UW, e);
-    win_pass1_any<UW, BL>(r, k0 + UW, nw, lB, xB, bsw, l, kL, kS, kT, nz);
+    win_pass1_any<UW, BL>(r, k0 + UW, nw, lB, xB, bsw, l, kL, kS, T, nz);
     if (k0 + 2 * UW >= lws) break;
   }
-  float T = kT.at(0.0f);
-  if (rec.cell_flags & kInfTerm) T += __expf(kL.at(0.0f));  // decided at traversal
+  if (rec.cell_flags & kInfTerm) T += __builtin_amdgcn_exp2f(kL.at(0.0f));  // decided at traversal
 
   // pass 2
   kL = L0;
   kS = S0;
-  Kahan kP = {0.f, 0.f};
+  float P = 0.0f;  // prefix sum of the taus
   win_load<UW>(rr, lr, 0, e);
   win_issue<MI, UW>(rx, e, lA, xA);
   win_load<UW>(rr, lr, UW, e);
   for (uint32_t k0 = 0;; k0 += 2 * UW) {
     win_issue<MI, UW>(rx, e, lB, xB);
     win_load<UW>(rr, lr, k0 + 2 * UW, e);
-    win_pass2<MO, UW>(r, k0, nw, lA, xA, T, ro, lo, kL, kS, kP);
+    win_pass2<MO, UW>(r, k0, nw, lA, xA, T, ro, lo, kL, kS, P);
     if (k0 + UW >= lws) break;
     win_issue<MI, UW>(rx, e, lA, xA);
     win_load<UW>(rr, lr, k0 + 3 * UW, e);
-    win_pass2<MO, UW>(r, k0 + UW, nw, lB, xB, T, ro, lo, kL, kS, kP);
+    win_pass2<MO, UW>(r, k0 + UW, nw, lB, xB, T, ro, lo, kL, kS, P);
     if (k0 + 2 * UW >= lws) break;
   }
 }

@@ -5,8 +5,8 @@
 
 namespace mrf {
 
-// Storage of the per-incidence messages: voxel->ray ln n (input side) and
-// ray->voxel p*l (output side).  The mixed types keep ln n in FP32.
+// Storage of the per-incidence messages: voxel->ray log2 n (input side) and
+// ray->voxel p*l (output side).  The mixed types keep log2 n in FP32.
 enum MsgType { kMsgF32 = 0, kMsgF16 = 1, kMsgBF16 = 2, kMsgMixedF16 = 3, kMsgMixedBF16 = 4 };
 inline int msg_in_type(int t) { return t == kMsgF16 ? kMsgF16 : (t == kMsgBF16 ? kMsgBF16 : kMsgF32); }
 inline int msg_out_type(int t) {
@@ -118,11 +118,11 @@ struct CamPass {
   const uint32_t* wseg_off;  // nwp + 1 window segment offsets
   const uint64_t* heads;   // window segment head bitmask
   const uint32_t* tile_h0; // window heads before each kTile tile
-  void* apair;             // per-pair voxel -> ray message ln n (input side type), kept per camera
+  void* apair;             // per-pair voxel -> ray message log2 n (input side type), kept per camera
   const uint32_t* dyn;     // front-only pairs at voxels where some camera has window records
   uint32_t ndyn;
-  const uint32_t* msg_all; // device countdown: nonzero while every pair's ln n must be recomputed
-  const uint4* dyn_desc;        // descriptors of the front chunks whose ln n keep changing
+  const uint32_t* msg_all; // device countdown: nonzero while every pair's log2 n must be recomputed
+  const uint4* dyn_desc;        // descriptors of the front chunks whose log2 n keep changing
   uint32_t ndyn_chunks;
   float* S;                // per window pair: window segment sum
   float *pf_v, *pl_v, *mid;  // per-tile parts of the segments that cross tiles

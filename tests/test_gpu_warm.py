@@ -155,8 +155,12 @@ def test_trained_network_on_held_out_maps():
     """The committed network (mrfmap_amd/data/warm_mlp.npz, features after
     one iteration) on held-out maps of the small generator: iterations and
     time to a map within 1e-3 max-abs of plain BP's converged map, plain vs
-    warm-started.  The warm-started runs must reach that map; the gain is
-    reported (about 1x for this network on these maps)."""
+    warm-started.  The warm-started runs reach that map, with no gain: the
+    iteration counts measured plain 13 / 8 / 21 against warm 13 / 9 / 22 (the
+    iterations to 1e-3 are set by the few voxels next to surfaces that
+    converge last, which the per-voxel prediction does not move), so the
+    test asserts that the warm start costs at most two iterations and gains
+    at most one."""
     from mrfmap_amd import warmstart
 
     params, meta = warmstart.load_weights()
@@ -170,3 +174,4 @@ def test_trained_network_on_held_out_maps():
         print(f"seed {seed}: plain {plain['iters']} it {plain['ms']:.1f} ms, warm {warm['iters']} it "
               f"{warm['ms']:.1f} ms (features + MLP + update {warm['warm_ms']:.2f} ms)")
         assert plain["reached"] and warm["reached"] and warm["warm"] == 1
+        assert plain["iters"] - 1 <= warm["iters"] <= plain["iters"] + 2
