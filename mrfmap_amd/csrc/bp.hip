@@ -90,10 +90,12 @@ struct Poly {
 constexpr float kNegHalfLog2e = -0.72134752044448170f;
 __device__ __forceinline__ float gauss_a(float u) { return 0.398942f * __builtin_amdgcn_exp2f((u * u) * kNegHalfLog2e); }
 
-// psi(s; d) = N(d; s + bias(s), (3 sigma(s))^2) (depth_potential, gvdbkernels.cu:57-72).
-__device__ __forceinline__ float depth_psi(float s, float d, const Poly& sig, const Poly& bia) {
+// psi(s; d) = N(d; s + bias(s), (3 sigma(s))^2) (depth_potential,
+// gvdbkernels.cu:57-72), with the likelihood width 3 sigma(s) evaluated
+// from sigma's coefficients times 3 (sig3)
+__device__ __forceinline__ float depth_psi(float s, float d, const Poly& sig3, const Poly& bia) {
   const float diff = fabsf(d - (s + bia(s)));
-  const float inv = __builtin_amdgcn_rcpf(3.0f * sig(s));
+  const float inv = __builtin_amdgcn_rcpf(sig3(s));
   return gauss_a(diff * inv) * inv;
 }
 
@@ -498,15 +500,20 @@ __device__ __forceinline__ void win_issue(Rsrc rx, const uint2 (&e)[UW], float (
 
 struct WinRay {
   float z0, inv, d;
-  Poly sig, bia;
+  Poly sig3, bia;  // 3 sigma(z) (the likelihood width) and bias(z)
 };
+
+// depth at the exit of a record: z0 + (travelled chord length) * inv
+__device__ __forceinline__ float win_depth(const WinRay& r, const Kahan& kS, float ln) {
+  return fmaf(kS.at(ln), r.inv, r.z0);
+}
 
 // terms of one record shared by both passes: a = l log2 n, vpsi, n
 __device__ __forceinline__ void win_terms(const WinRay& r, const Kahan& kL, const Kahan& kS, float ln, float lnn,
                                           float& a, float& vpsi, float& nn) {
   a = ln * lnn;
-  const float s = r.z0 + kS.at(ln) * r.inv;
-  const float ps = depth_psi(s, r.d, r.sig, r.bia);
+  const float s = win_depth(r, kS, ln);
+  const float ps = depth_psi(s, r.d, r.sig3, r.bia);
   // a gap or padding record (log2 n = 0) gets a vpsi but n = 1 exactly, so
   // its tau vpsi (1 - n) is exactly 0; its own message is never read
   vpsi = __builtin_amdgcn_exp2f(kL.at(a)) * ps;
@@ -549,9 +556,9 @@ __device__ __forceinline__ void win_pass1_bl(const WinRay& r, uint32_t k0, uint3
     kL.c = restart ? 0.0f : kL.c;
     const float lnn = x[u];
     const float a = ln[u] * lnn;
-    const float s = r.z0 + kS.at(ln[u]) * r.inv;
+    const float s = win_depth(r, kS, ln[u]);
     const float diff = fabsf(r.d - (s + r.bia(s)));
-    const float inv = __builtin_amdgcn_rcpf(3.0f * r.sig(s));
+    const float inv = __builtin_amdgcn_rcpf(r.sig3(s));
     const float uu = diff * inv;
     const float ap = gauss_a(uu);
     const float ps = ap * inv;  // depth_psi's product order
@@ -640,7 +647,10 @@ __global__ __launch_bounds__(64) void k_ray_window(const RayRec* __restrict__ la
   }
   Kahan kS = {rec.ks, rec.kc};
   WinRay r;
-  load_cell(sigt, biat, lookup_n, rec.cell_flags & kCellMask, r.sig, r.bia);
+  load_cell(sigt, biat, lookup_n, rec.cell_flags & kCellMask, r.sig3, r.bia);
+  r.sig3.c0 *= 3.0f;
+  r.sig3.c1 *= 3.0f;
+  r.sig3.c2 *= 3.0f;
   r.z0 = rec.z0;
   r.inv = rec.inv_norm;
   r.d = rec.depth;
