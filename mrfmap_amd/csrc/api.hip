@@ -131,6 +131,8 @@ struct mrf_ctx {
   size_t fscan_bytes = 0;
   uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
   uint32_t* d_all_pairs = nullptr;  // [0] CamPass::all_pairs, [1] CamPass::msg_all
+  uint32_t* d_wlist = nullptr;      // voxels of some camera's window pairs (pool indices)
+  uint64_t nwlist = 0;
 
   std::vector<Camera> cams;
   bool geom_dirty = true;
@@ -673,6 +675,17 @@ int prepare(mrf_ctx* c) {
     uint32_t* cflags = pos + nflag;  // per front chunk: some log2 n of it keeps changing
     launch_zero_words(W, nvox * 4, c->stream);
     for (const Camera& cam : c->cams) launch_mark_window_voxels(cam.d_pair_vox, cam.nwp, W, c->stream);
+    // the voxels of W as a list: the only beliefs that change once the
+    // front-only pairs stop updating (the residual's range then)
+    uint64_t nw = 0;
+    if (nvox) {
+      if ((rc = flag_scan(c, W, pos, nvox))) return rc;
+      nw = read_total(c);
+    }
+    dfree(&c->d_wlist);
+    if ((rc = dalloc(c, &c->d_wlist, std::max<uint64_t>(nw, 1)))) return rc;
+    launch_dyn_compact(W, pos, (uint32_t)nvox, 0u, c->d_wlist, c->stream);
+    c->nwlist = nw;
     for (size_t ci = 0; ci < c->cams.size(); ++ci) {
       Camera& cam = c->cams[ci];
       const uint32_t nrest = cam.npairs - cam.nwp;
@@ -950,7 +963,8 @@ hipError_t enqueue_iteration(mrf_ctx* c, const hipEvent_t* ev) {
     if (ec && (e = hipEventRecord(ec[5], c->stream)) != hipSuccess) return e;
   }
   launch_zero_words(c->d_total + 1, 8, c->stream);  // a kernel, not a memset: the sequence may be captured
-  launch_residual(c->d_B, c->d_A, nvox, (unsigned long long*)(c->d_total + 1), c->stream);
+  launch_residual(c->d_B, c->d_A, nvox, c->d_wlist, c->nwlist, c->d_all_pairs, (unsigned long long*)(c->d_total + 1),
+                  c->stream);
   return hipGetLastError();
 }
 
@@ -1255,7 +1269,7 @@ void mrf_destroy(mrf_ctx* c) {
   drop_graph(c);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
   dfree(&c->d_scan_tmp); dfree(&c->d_total);
-  if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp); dfree(&c->d_all_pairs);
+  if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp); dfree(&c->d_all_pairs); dfree(&c->d_wlist);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -1017,11 +1017,19 @@ __global__ void k_marginals(const float* __restrict__ B, float* __restrict__ A, 
 
 // max |marginal(B) - Aprev| as an IEEE double reduced through the ordered bit
 // pattern of non-negative doubles; Aprev is refreshed in the same sweep.
+// Over every voxel while *all (front-only pairs update their beliefs: the
+// first iteration of a call), afterwards over the voxels of list (those of
+// some camera's window pairs, the only beliefs that still change).
 __global__ __launch_bounds__(256) void k_residual(const float* __restrict__ B, float* __restrict__ A, size_t n,
+                                                   const uint32_t* __restrict__ list, size_t nlist,
+                                                   const uint32_t* __restrict__ all,
                                                    unsigned long long* __restrict__ d_max) {
   __shared__ double sh[256];
   double m = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const bool every = *all != 0u;
+  const size_t ne = every ? n : nlist;
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < ne; j += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = every ? j : (size_t)list[j];
     const float a = marginal(B[i]);
     const double dlt = fabs((double)a - (double)A[i]);
     A[i] = a;
@@ -1252,11 +1260,12 @@ void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_marginals, dim3(grid_for(n, 256)), dim3(256), 0, s, B, A, n);
 }
 
-void launch_residual(const float* B, float* A, size_t n, unsigned long long* d_max, hipStream_t s) {
+void launch_residual(const float* B, float* A, size_t n, const uint32_t* list, size_t nlist, const uint32_t* all,
+                     unsigned long long* d_max, hipStream_t s) {
   if (n == 0) return;
   unsigned g = grid_for(n, 256);
   if (g > 4096) g = 4096;
-  hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, B, A, n, d_max);
+  hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, B, A, n, list, nlist, all, d_max);
 }
 
 void launch_ref_error(const float* A, const uint8_t* upd, const float* ref, size_t n, unsigned long long* d_max,
