@@ -51,6 +51,10 @@ struct Camera {
   float* d_partial = nullptr;      // front partial sums (64 per chunk), kept between iterations
   uint4* d_dyn_desc = nullptr;        // descriptors of the front chunks whose partial sums keep changing
   uint32_t ndyn_chunks = 0;
+  uint32_t* d_fdmask = nullptr;       // per front chunk and lane: its changing records (bits)
+  float* d_fstatic = nullptr;         // per front chunk and lane: sum of its other records
+  uint2* d_dfront = nullptr;          // the changing chunks' changing records, compacted per lane
+  uint2* d_dcr = nullptr;             // per changing chunk: {first row, rows} in d_dfront
   uint2* d_fspan = nullptr;           // per lane of every slice: first front record, front records to sum
   uint64_t nstatic = 0;            // live records in such chunks
   uint64_t nrec = 0, base_w = 0;   // padded records; first record of the window region
@@ -440,6 +444,10 @@ int ensure_sell0(mrf_ctx* c, size_t n) {
 void free_camera_lists(Camera& cam) {
   dfree(&cam.d_partial);
   dfree(&cam.d_dyn_desc);
+  dfree(&cam.d_fdmask);
+  dfree(&cam.d_fstatic);
+  dfree(&cam.d_dfront);
+  dfree(&cam.d_dcr);
   dfree(&cam.d_fspan);
   dfree(&cam.d_lnn);
   dfree(&cam.d_dyn);
@@ -699,9 +707,19 @@ int prepare(mrf_ctx* c) {
       if ((rc = dalloc(c, &cam.d_dyn, std::max<uint64_t>(ndyn, 1)))) return rc;
       cam.ndyn = (uint32_t)ndyn;
       launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
-      // the front chunks whose log2 n keep changing, as a list
+      // the front chunks whose log2 n keep changing, as a list; per chunk and
+      // lane the mask of its changing records, per chunk the rows of its
+      // compact block (into pos, free until the chunk scan below)
+      dfree(&cam.d_fdmask);
+      dfree(&cam.d_fstatic);
+      if ((rc = dalloc(c, &cam.d_fdmask, std::max<size_t>((size_t)cam.nchunks * kSlice, 1))) ||
+          (rc = dalloc(c, &cam.d_fstatic, std::max<size_t>((size_t)cam.nchunks * kSlice, 1))))
+        return rc;
+      if ((rc = ensure_sort(c, std::max<size_t>(cam.nchunks, 1), 1))) return rc;
+      uint32_t* crow = c->d_x0;  // sort scratch of at least nchunks words: rows per chunk
       launch_zero_words(c->d_total + 2, 8, c->stream);
-      launch_chunk_static(cam_pass(c, ci), flags, cflags, (unsigned long long*)(c->d_total + 2), c->stream);
+      launch_chunk_static(cam_pass(c, ci), flags, cflags, cam.d_fdmask, crow, (unsigned long long*)(c->d_total + 2),
+                          c->stream);
       uint64_t n = 0;
       HIPCHK(c, hipMemcpyAsync(&n, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -716,6 +734,20 @@ int prepare(mrf_ctx* c) {
       cam.ndyn_chunks = (uint32_t)ndc;
       launch_dyn_compact(cflags, pos, cam.nchunks, 0, flags, c->stream);  // chunk ids, over the used pair flags
       launch_gather_desc(cam.d_chunk_desc, flags, (uint32_t)ndc, cam.d_dyn_desc, c->stream);
+      // the compact blocks of the changing chunks: rows per entry, their
+      // exclusive scan (the first rows), the copies of the changing records
+      uint64_t nrows = 0;
+      if (ndc) {
+        launch_dyn_rows(cam.d_dyn_desc, (uint32_t)ndc, crow, cflags, c->stream);
+        if ((rc = flag_scan(c, cflags, pos, ndc))) return rc;
+        nrows = read_total(c);
+      }
+      dfree(&cam.d_dfront);
+      dfree(&cam.d_dcr);
+      if ((rc = dalloc(c, &cam.d_dfront, std::max<uint64_t>(nrows * kSlice, 1))) ||
+          (rc = dalloc(c, &cam.d_dcr, std::max<uint64_t>(ndc, 1))))
+        return rc;
+      launch_fill_dfront(cam_pass(c, ci), crow, pos, cam.d_dfront, cam.d_dcr, c->stream);
     }
     HIPCHK(c, hipGetLastError());
   }
@@ -799,6 +831,10 @@ CamPass cam_pass(mrf_ctx* c, size_t ci) {
   a.partial = cam.d_partial;
   a.dyn_desc = cam.d_dyn_desc;
   a.ndyn_chunks = cam.ndyn_chunks;
+  a.fdmask = cam.d_fdmask;
+  a.fstatic = cam.d_fstatic;
+  a.dfront = cam.d_dfront;
+  a.dcr = cam.d_dcr;
   a.pair_vox = cam.d_pair_vox;
   a.pair_c = cam.d_pair_c;
   a.pair_sf = cam.d_pair_sf;

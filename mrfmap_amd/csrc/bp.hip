@@ -332,58 +332,99 @@ __global__ void k_gather_desc(const uint4* __restrict__ desc, const uint32_t* __
 // so the work splits into near-equal pieces whatever the ray lengths; each
 // lane writes its Kahan-compensated partial sum.  Rays without window
 // records are skipped.
+//
+// While the log2 n of all pairs are recomputed (the first two iterations of
+// a call) every chunk is summed in record order, and the records whose log2 n
+// stops changing (bit clear in the chunk-lane mask fdmask) are also summed
+// on their own into fstatic.  Afterwards only the chunks with a changing
+// record are visited (dyn_desc), and each lane adds its changing records,
+// copied in order into a compact block of the chunk (dfront, rows of 64
+// lanes, {first row, rows} in dcr), to its kept static sum: static records
+// are not read again.  The two orders of summation differ only by FP32
+// rounding.
 template <int MI, int U>
 __global__ __launch_bounds__(256) void k_front_sums(const uint2* __restrict__ fspan, const uint4* __restrict__ desc,
                                                     uint32_t nchunks, const uint4* __restrict__ dyn_desc,
                                                     uint32_t ndyn_chunks, const uint2* __restrict__ inc,
                                                     const typename Msg<MI>::T* __restrict__ Xi,
                                                     float* __restrict__ partial,
-                                                    const uint32_t* __restrict__ msg_all) {
-  // every chunk while the log2 n of all pairs are recomputed, afterwards only
-  // the chunks with a pair whose log2 n still changes (the others keep their
-  // partial sums): a wave per entry of the descriptor list (one wave-uniform
-  // load per chunk), the waves past the list's end exit at once.  The grid
-  // covers every chunk; the loop strides for a smaller one.
+                                                    const uint32_t* __restrict__ msg_all,
+                                                    const uint32_t* __restrict__ fdmask, float* __restrict__ fstatic,
+                                                    const uint2* __restrict__ dfront, const uint2* __restrict__ dcr) {
+  // a wave per entry of the descriptor list (one wave-uniform load per
+  // chunk), the waves past the list's end exit at once.  The grid covers
+  // every chunk; the loop strides for a smaller one.
   const bool all = *msg_all != 0u;
-  const uint4* __restrict__ list = all ? desc : dyn_desc;
   const uint32_t n = all ? nchunks : ndyn_chunks;
   const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
   const uint32_t l = threadIdx.x & 63u;
   for (uint32_t i = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); i < n;
        i += nwaves) {
-    const uint4 d = list[i];  // {chunk, slice, first row, end row}
-    const uint2 fs = fspan[d.y * kSlice + l];
-    const uint32_t k1 = min(d.z + kChunk, fs.y);
-    const uint2* __restrict__ rf = inc + fs.x;
-    Kahan acc = {0.f, 0.f};
-    for (uint32_t kb = d.z; kb < d.w; kb += U) {
-      uint2 e[U];
-      float x[U];
+    if (all) {
+      const uint4 d = desc[i];  // {chunk, slice, first row, end row}
+      const uint2 fs = fspan[d.y * kSlice + l];
+      const uint32_t k1 = min(d.z + kChunk, fs.y);
+      const uint2* __restrict__ rf = inc + fs.x;
+      const uint32_t m = fdmask[(size_t)d.x * kSlice + l];
+      Kahan acc = {0.f, 0.f}, accs = {0.f, 0.f};
+      for (uint32_t kb = d.z; kb < d.w; kb += U) {
+        uint2 e[U];
+        float x[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t k = kb + u;
-        e[u] = k < k1 ? rf[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+        for (int u = 0; u < U; ++u) {
+          const uint32_t k = kb + u;
+          e[u] = k < k1 ? rf[(size_t)k * kSlice] : make_uint2(kGapVox, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = e[u].x != kGapVox ? Msg<MI>::load(Xi, e[u].x) : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t k = kb + u;
+          if (k < k1) {
+            const float v = __uint_as_float(e[u].y) * x[u];
+            acc.add(v);
+            if (((m >> (k - d.z)) & 1u) == 0u) accs.add(v);
+          }
+        }
       }
+      partial[(size_t)d.x * kSlice + l] = acc.at(0.0f);
+      fstatic[(size_t)d.x * kSlice + l] = accs.at(0.0f);
+    } else {
+      const uint32_t w = dyn_desc[i].x;  // the chunk
+      const uint2 cr = dcr[i];           // {first row, rows} of its compact block
+      const uint32_t cnt = (uint32_t)__popc(fdmask[(size_t)w * kSlice + l]);
+      const uint2* __restrict__ rf = dfront + (size_t)cr.x * kSlice + l;
+      Kahan acc = {fstatic[(size_t)w * kSlice + l], 0.f};
+      for (uint32_t rb = 0; rb < cr.y; rb += U) {
+        uint2 e[U];
+        float x[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) x[u] = e[u].x != kGapVox ? Msg<MI>::load(Xi, e[u].x) : 0.0f;
+        for (int u = 0; u < U; ++u) e[u] = rb + u < cnt ? rf[(size_t)(rb + u) * kSlice] : make_uint2(kGapVox, 0u);
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (kb + u < k1) acc.add(__uint_as_float(e[u].y) * x[u]);
+        for (int u = 0; u < U; ++u) x[u] = e[u].x != kGapVox ? Msg<MI>::load(Xi, e[u].x) : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (rb + u < cnt) acc.add(__uint_as_float(e[u].y) * x[u]);
+      }
+      partial[(size_t)w * kSlice + l] = acc.at(0.0f);
     }
-    partial[(size_t)d.x * kSlice + l] = acc.at(0.0f);
   }
 }
 
-// Per front chunk (the layout of k_front_sums): 0 when every record of every
-// lane belongs to a front-only pair at a voxel that no camera reaches with
-// window records (pflag[q - nwp] == 0), i.e. when its log2 n stop changing
-// after the first two iterations of a call and its partial sums can be
-// kept; 1 otherwise.
+// Per front chunk (the layout of k_front_sums): a record changes when it
+// belongs to a window pair or to a front-only pair at a voxel that some
+// camera reaches with window records (pflag[q - nwp] != 0); the others'
+// log2 n stop changing after the first two iterations of a call.  Per lane,
+// dmask has bit k - first row set for each changing record k; per chunk,
+// rows = the most changing records of a lane (the rows of its compact
+// block), 0 when no record changes (the chunk's partial sums can be kept,
+// and its live records are counted into nstatic).
 __global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ fspan,
                                                       const uint4* __restrict__ desc, uint32_t nchunks,
                                                       const uint2* __restrict__ inc,
                                                       uint32_t nwp, const uint32_t* __restrict__ pflag,
-                                                      uint32_t* __restrict__ dyn_flag,
+                                                      uint32_t* __restrict__ dyn_flag, uint32_t* __restrict__ dmask,
+                                                      uint32_t* __restrict__ rows,
                                                       unsigned long long* __restrict__ nstatic) {
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nchunks) return;  // uniform per wavefront
@@ -392,20 +433,55 @@ __global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ 
   const uint2 fs = fspan[d.y * kSlice + l];
   const uint32_t k0 = d.z, k1 = min(k0 + kChunk, fs.y);
   const uint2* __restrict__ rf = inc + fs.x;
-  bool dyn = false;
-  uint32_t live = 0;
+  uint32_t m = 0, live = 0;
   for (uint32_t k = k0; k < k1; ++k) {
     const uint32_t q = rf[(size_t)k * kSlice].x;
     if (q == kGapVox) continue;
     ++live;
-    if (q < nwp || pflag[q - nwp] != 0u) dyn = true;
+    if (q < nwp || pflag[q - nwp] != 0u) m |= 1u << (k - k0);
   }
-  const unsigned long long any = __ballot(dyn);
-  if (!any) {  // the records whose sums are kept: counted once per chunk
-    for (int m = 32; m > 0; m >>= 1) live += __shfl_xor(live, m, 64);
+  dmask[(size_t)w * kSlice + l] = m;
+  uint32_t nr = (uint32_t)__popc(m);
+  for (int o = 32; o > 0; o >>= 1) nr = max(nr, (uint32_t)__shfl_xor((int)nr, o, 64));
+  if (nr == 0) {  // the records whose sums are kept: counted once per chunk
+    for (int o = 32; o > 0; o >>= 1) live += __shfl_xor(live, o, 64);
     if (l == 0) atomicAdd(nstatic, (unsigned long long)live);
   }
-  if (l == 0) dyn_flag[w] = any ? 1u : 0u;
+  if (l == 0) {
+    dyn_flag[w] = nr ? 1u : 0u;
+    rows[w] = nr;
+  }
+}
+
+// rows of every changing chunk's compact block, in list order
+__global__ void k_dyn_rows(const uint4* __restrict__ dyn_desc, uint32_t n, const uint32_t* __restrict__ rows,
+                           uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = rows[dyn_desc[i].x];
+}
+
+// The compact blocks: per entry of the changing-chunk list (one wave), each
+// lane copies its changing records of the chunk in order to rows
+// first row + 0, 1, ... of the block; dcr[i] = {first row, rows}.
+__global__ __launch_bounds__(256) void k_fill_dfront(const uint2* __restrict__ fspan,
+                                                     const uint4* __restrict__ dyn_desc, uint32_t n,
+                                                     const uint2* __restrict__ inc,
+                                                     const uint32_t* __restrict__ dmask,
+                                                     const uint32_t* __restrict__ rows,
+                                                     const uint32_t* __restrict__ row_off, uint2* __restrict__ dfront,
+                                                     uint2* __restrict__ dcr) {
+  const uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;  // uniform per wavefront
+  const uint32_t l = threadIdx.x & 63u;
+  const uint4 d = dyn_desc[i];
+  const uint2 fs = fspan[d.y * kSlice + l];
+  const uint2* __restrict__ rf = inc + fs.x;
+  const uint32_t m = dmask[(size_t)d.x * kSlice + l];
+  const uint32_t r0 = row_off[i];
+  uint32_t r = 0;
+  for (uint32_t b = 0; b < kChunk; ++b)
+    if ((m >> b) & 1u) dfront[(size_t)(r0 + r++) * kSlice + l] = rf[(size_t)(d.z + b) * kSlice];
+  if (l == 0) dcr[i] = make_uint2(r0, rows[d.x]);
 }
 
 // ---------------------------------------------------------------- buffer access
@@ -1146,7 +1222,8 @@ void launch_ray_t(const CamPass& a, hipStream_t s, int part) {
   const typename Msg<MI>::T* Xi = (const typename Msg<MI>::T*)a.apair;
   if (a.nchunks && part != 2)
     hipLaunchKernelGGL((k_front_sums<MI, UF>), dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s,
-                       a.fspan, a.chunk_desc, a.nchunks, a.dyn_desc, a.ndyn_chunks, a.inc, Xi, a.partial, a.msg_all);
+                       a.fspan, a.chunk_desc, a.nchunks, a.dyn_desc, a.ndyn_chunks, a.inc, Xi, a.partial, a.msg_all,
+                       a.fdmask, a.fstatic, a.dfront, a.dcr);
   if (part == 1) return;
   const uint32_t blocks = (a.nslices * kSlice + 63) / 64;
   const unsigned nb = (blocks + 7) / 8 * 8;  // multiple of the 8 XCDs for the block remapping
@@ -1248,11 +1325,22 @@ void launch_gather_desc(const uint4* desc, const uint32_t* idx, uint32_t n, uint
   if (n) hipLaunchKernelGGL(k_gather_desc, dim3(grid_for(n, 256)), dim3(256), 0, s, desc, idx, n, out);
 }
 
-void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, unsigned long long* nstatic,
-                         hipStream_t s) {
+void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, uint32_t* dmask, uint32_t* rows,
+                         unsigned long long* nstatic, hipStream_t s) {
   if (a.nchunks == 0) return;
   hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.fspan, a.chunk_desc, a.nchunks,
-                     a.inc, a.nwp, pflag, dyn_flag, nstatic);
+                     a.inc, a.nwp, pflag, dyn_flag, dmask, rows, nstatic);
+}
+
+void launch_dyn_rows(const uint4* dyn_desc, uint32_t n, const uint32_t* rows, uint32_t* out, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_dyn_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, dyn_desc, n, rows, out);
+}
+
+void launch_fill_dfront(const CamPass& a, const uint32_t* rows, const uint32_t* row_off, uint2* dfront, uint2* dcr,
+                        hipStream_t s) {
+  if (a.ndyn_chunks == 0) return;
+  hipLaunchKernelGGL(k_fill_dfront, dim3(grid_for(a.ndyn_chunks, 4)), dim3(256), 0, s, a.fspan, a.dyn_desc,
+                     a.ndyn_chunks, a.inc, a.fdmask, rows, row_off, dfront, dcr);
 }
 
 void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {

@@ -124,6 +124,10 @@ struct CamPass {
   const uint32_t* msg_all; // device countdown: nonzero while every pair's log2 n must be recomputed
   const uint4* dyn_desc;        // descriptors of the front chunks whose log2 n keep changing
   uint32_t ndyn_chunks;
+  const uint32_t* fdmask;       // per front chunk and lane: bits of its records whose log2 n keeps changing
+  float* fstatic;               // per front chunk and lane: sum of the others (first iterations of a call)
+  const uint2* dfront;          // per changing chunk: its changing records, compacted per lane (rows of 64)
+  const uint2* dcr;             // per entry of dyn_desc: {first row, rows} of its block in dfront
   float* S;                // per window pair: window segment sum
   float *pf_v, *pl_v, *mid;  // per-tile parts of the segments that cross tiles
   float* B;
@@ -176,9 +180,15 @@ void launch_pair_msgs(const CamPass& a, hipStream_t s);
 void launch_ray_messages(const CamPass& a, hipStream_t s, int part = 0);
 // per front chunk of the camera: 0 if all its pairs are front-only pairs whose pflag (front-only
 // pair q at pflag[q - nwp]: at a voxel of W) is 0 (static; its records are added to *nstatic), 1
-// otherwise
-void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, unsigned long long* nstatic,
-                         hipStream_t s);
+// otherwise; per chunk and lane the mask of its changing records (dmask), per chunk the rows of
+// its compact block (the most changing records of a lane)
+void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, uint32_t* dmask, uint32_t* rows,
+                         unsigned long long* nstatic, hipStream_t s);
+// rows of the compact block of every changing chunk (dyn_desc order)
+void launch_dyn_rows(const uint4* dyn_desc, uint32_t n, const uint32_t* rows, uint32_t* out, hipStream_t s);
+// the compact blocks of the changing chunks (a.dyn_desc, a.fdmask) at row_off, and dcr
+void launch_fill_dfront(const CamPass& a, const uint32_t* rows, const uint32_t* row_off, uint2* dfront, uint2* dcr,
+                        hipStream_t s);
 // part: 0 both kernels, 1 the segment sums only, 2 the per-pair update only
 void launch_epilogue(const CamPass& a, hipStream_t s, int part = 0);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
