@@ -304,9 +304,10 @@ def main():
     # ray kernels, algorithmic bytes: every incidence reads its 8 B record and
     # its pair's voxel->ray message; a window incidence also writes its
     # ray->voxel message (its 4 B segment position is read by k_seg_sum and
-    # charged there).  Front records of chunks whose sums stop changing are
-    # read in the first two iterations of a call only; the average over the K
-    # timed launches of a camera counts them so.
+    # charged there).  Front records whose pair's message stops changing are
+    # read in the first two iterations of a call only (afterwards their sums
+    # are kept); the average over the K timed launches of a camera counts
+    # them so.
     in_b = 4 if msg in (_lib.MSG_F32, _lib.MSG_MIXED_F16, _lib.MSG_MIXED_BF16) else 2
     out_b = 4 if msg == _lib.MSG_F32 else 2
     n_win = st["window_updates_per_iter"]

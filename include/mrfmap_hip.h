@@ -125,10 +125,10 @@ typedef struct mrf_stats {
   double t_warm_ms;            /* features + MLP + warm-start update, wall time */
   double ref_error;            /* max-abs marginal difference to the mrf_set_reference map after the
                                   last iteration, -1 without a reference */
-  int64_t static_front_updates; /* front updates whose visibility sums stop changing after the first two
-                                   iterations of a call (every pair of their chunk is a front-only pair
-                                   at a voxel no camera reaches with window records); from the third
-                                   iteration on their stored sums are used instead of re-reading them */
+  int64_t static_front_updates; /* front updates whose visibility terms stop changing after the first two
+                                   iterations of a call (their pair is a front-only pair at a voxel no
+                                   camera reaches with window records); from the third iteration on
+                                   their stored sums are used instead of re-reading them */
   int64_t front_slots;         /* record slots of the sliced front regions (each slice padded to its
                                   longest ray), summed over the cameras */
   int64_t window_slots;        /* the same for the window regions */

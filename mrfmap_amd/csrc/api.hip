@@ -51,9 +51,9 @@ struct Camera {
   float* d_partial = nullptr;      // front partial sums (64 per chunk), kept between iterations
   uint4* d_dyn_desc = nullptr;        // descriptors of the front chunks whose partial sums keep changing
   uint32_t ndyn_chunks = 0;
-  uint32_t* d_fdmask = nullptr;       // per front chunk and lane: its changing records (bits)
-  float* d_fstatic = nullptr;         // per front chunk and lane: sum of its other records
-  uint2* d_dfront = nullptr;          // the changing chunks' changing records, compacted per lane
+  uint32_t* d_fdmask = nullptr;       // per front chunk and lane: its changing records (bits; in d_partial)
+  float* d_fstatic = nullptr;         // per front chunk and lane: sum of its other records (in d_partial)
+  uint2* d_dfront = nullptr;          // the changing chunks' changing records, compacted per lane (in d_dcr)
   uint2* d_dcr = nullptr;             // per changing chunk: {first row, rows} in d_dfront
   uint2* d_fspan = nullptr;           // per lane of every slice: first front record, front records to sum
   uint64_t nstatic = 0;            // live records in such chunks
@@ -135,8 +135,6 @@ struct mrf_ctx {
   size_t fscan_bytes = 0;
   uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
   uint32_t* d_all_pairs = nullptr;  // [0] CamPass::all_pairs, [1] CamPass::msg_all
-  uint32_t* d_wlist = nullptr;      // voxels of some camera's window pairs (pool indices)
-  uint64_t nwlist = 0;
 
   std::vector<Camera> cams;
   bool geom_dirty = true;
@@ -442,12 +440,12 @@ int ensure_sell0(mrf_ctx* c, size_t n) {
 }
 
 void free_camera_lists(Camera& cam) {
-  dfree(&cam.d_partial);
+  dfree(&cam.d_partial);  // holds d_fstatic and d_fdmask too
+  cam.d_fstatic = nullptr;
+  cam.d_fdmask = nullptr;
   dfree(&cam.d_dyn_desc);
-  dfree(&cam.d_fdmask);
-  dfree(&cam.d_fstatic);
-  dfree(&cam.d_dfront);
-  dfree(&cam.d_dcr);
+  dfree(&cam.d_dcr);  // holds d_dfront too
+  cam.d_dfront = nullptr;
   dfree(&cam.d_fspan);
   dfree(&cam.d_lnn);
   dfree(&cam.d_dyn);
@@ -564,9 +562,14 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
   launch_exclusive_scan(c->d_counts, cam.d_chunk_off, (size_t)ntiles + 1, c->d_scan_tmp, c->d_total, s);
   cam.nchunks = (uint32_t)read_total(c);
-  if ((rc = dalloc(c, &cam.d_chunk_desc, cam.nchunks)) || (rc = dalloc(c, &cam.d_partial, (size_t)cam.nchunks * kSlice)) ||
+  // one allocation for the partial sums, the static sums and the masks of
+  // changing records (per chunk and lane each)
+  const size_t nparts = (size_t)cam.nchunks * kSlice;
+  if ((rc = dalloc(c, &cam.d_chunk_desc, cam.nchunks)) || (rc = dalloc(c, &cam.d_partial, 3 * nparts)) ||
       (rc = dalloc(c, &cam.d_fspan, (size_t)ntiles * kSlice)))
     return rc;
+  cam.d_fstatic = cam.d_partial + nparts;
+  cam.d_fdmask = reinterpret_cast<uint32_t*>(cam.d_partial + 2 * nparts);
   launch_chunk_desc(cam.d_chunk_off, cam.d_len_f, ntiles, cam.d_chunk_desc, s);
   launch_front_span(cam.d_front, cam.d_lanes, ntiles * kSlice, cam.d_fspan, s);
   // 3. live incidences (records that are not gap records): all, window region
@@ -683,17 +686,6 @@ int prepare(mrf_ctx* c) {
     uint32_t* cflags = pos + nflag;  // per front chunk: some log2 n of it keeps changing
     launch_zero_words(W, nvox * 4, c->stream);
     for (const Camera& cam : c->cams) launch_mark_window_voxels(cam.d_pair_vox, cam.nwp, W, c->stream);
-    // the voxels of W as a list: the only beliefs that change once the
-    // front-only pairs stop updating (the residual's range then)
-    uint64_t nw = 0;
-    if (nvox) {
-      if ((rc = flag_scan(c, W, pos, nvox))) return rc;
-      nw = read_total(c);
-    }
-    dfree(&c->d_wlist);
-    if ((rc = dalloc(c, &c->d_wlist, std::max<uint64_t>(nw, 1)))) return rc;
-    launch_dyn_compact(W, pos, (uint32_t)nvox, 0u, c->d_wlist, c->stream);
-    c->nwlist = nw;
     for (size_t ci = 0; ci < c->cams.size(); ++ci) {
       Camera& cam = c->cams[ci];
       const uint32_t nrest = cam.npairs - cam.nwp;
@@ -710,11 +702,6 @@ int prepare(mrf_ctx* c) {
       // the front chunks whose log2 n keep changing, as a list; per chunk and
       // lane the mask of its changing records, per chunk the rows of its
       // compact block (into pos, free until the chunk scan below)
-      dfree(&cam.d_fdmask);
-      dfree(&cam.d_fstatic);
-      if ((rc = dalloc(c, &cam.d_fdmask, std::max<size_t>((size_t)cam.nchunks * kSlice, 1))) ||
-          (rc = dalloc(c, &cam.d_fstatic, std::max<size_t>((size_t)cam.nchunks * kSlice, 1))))
-        return rc;
       if ((rc = ensure_sort(c, std::max<size_t>(cam.nchunks, 1), 1))) return rc;
       uint32_t* crow = c->d_x0;  // sort scratch of at least nchunks words: rows per chunk
       launch_zero_words(c->d_total + 2, 8, c->stream);
@@ -742,11 +729,10 @@ int prepare(mrf_ctx* c) {
         if ((rc = flag_scan(c, cflags, pos, ndc))) return rc;
         nrows = read_total(c);
       }
-      dfree(&cam.d_dfront);
-      dfree(&cam.d_dcr);
-      if ((rc = dalloc(c, &cam.d_dfront, std::max<uint64_t>(nrows * kSlice, 1))) ||
-          (rc = dalloc(c, &cam.d_dcr, std::max<uint64_t>(ndc, 1))))
-        return rc;
+      dfree(&cam.d_dcr);  // one allocation: dcr, then the blocks
+      cam.d_dfront = nullptr;
+      if ((rc = dalloc(c, &cam.d_dcr, ndc + nrows * kSlice))) return rc;
+      cam.d_dfront = cam.d_dcr + ndc;
       launch_fill_dfront(cam_pass(c, ci), crow, pos, cam.d_dfront, cam.d_dcr, c->stream);
     }
     HIPCHK(c, hipGetLastError());
@@ -999,8 +985,7 @@ hipError_t enqueue_iteration(mrf_ctx* c, const hipEvent_t* ev) {
     if (ec && (e = hipEventRecord(ec[5], c->stream)) != hipSuccess) return e;
   }
   launch_zero_words(c->d_total + 1, 8, c->stream);  // a kernel, not a memset: the sequence may be captured
-  launch_residual(c->d_B, c->d_A, nvox, c->d_wlist, c->nwlist, c->d_all_pairs, (unsigned long long*)(c->d_total + 1),
-                  c->stream);
+  launch_residual(c->d_B, c->d_A, nvox, (unsigned long long*)(c->d_total + 1), c->stream);
   return hipGetLastError();
 }
 
@@ -1305,7 +1290,7 @@ void mrf_destroy(mrf_ctx* c) {
   drop_graph(c);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
   dfree(&c->d_scan_tmp); dfree(&c->d_total);
-  if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp); dfree(&c->d_all_pairs); dfree(&c->d_wlist);
+  if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp); dfree(&c->d_all_pairs);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

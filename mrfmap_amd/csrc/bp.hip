@@ -417,8 +417,9 @@ __global__ __launch_bounds__(256) void k_front_sums(const uint2* __restrict__ fs
 // log2 n stop changing after the first two iterations of a call.  Per lane,
 // dmask has bit k - first row set for each changing record k; per chunk,
 // rows = the most changing records of a lane (the rows of its compact
-// block), 0 when no record changes (the chunk's partial sums can be kept,
-// and its live records are counted into nstatic).
+// block), 0 when no record changes (the chunk's partial sums can be kept).
+// The live records that do not change, which the front sums stop reading
+// after the first two iterations of a call, are counted into nstatic.
 __global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ fspan,
                                                       const uint4* __restrict__ desc, uint32_t nchunks,
                                                       const uint2* __restrict__ inc,
@@ -442,11 +443,12 @@ __global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ 
   }
   dmask[(size_t)w * kSlice + l] = m;
   uint32_t nr = (uint32_t)__popc(m);
-  for (int o = 32; o > 0; o >>= 1) nr = max(nr, (uint32_t)__shfl_xor((int)nr, o, 64));
-  if (nr == 0) {  // the records whose sums are kept: counted once per chunk
-    for (int o = 32; o > 0; o >>= 1) live += __shfl_xor(live, o, 64);
-    if (l == 0) atomicAdd(nstatic, (unsigned long long)live);
+  uint32_t kept = live - nr;  // this lane's records whose sums are kept
+  for (int o = 32; o > 0; o >>= 1) {
+    nr = max(nr, (uint32_t)__shfl_xor((int)nr, o, 64));
+    kept += (uint32_t)__shfl_xor((int)kept, o, 64);
   }
+  if (l == 0 && kept) atomicAdd(nstatic, (unsigned long long)kept);
   if (l == 0) {
     dyn_flag[w] = nr ? 1u : 0u;
     rows[w] = nr;
@@ -1093,19 +1095,11 @@ __global__ void k_marginals(const float* __restrict__ B, float* __restrict__ A, 
 
 // max |marginal(B) - Aprev| as an IEEE double reduced through the ordered bit
 // pattern of non-negative doubles; Aprev is refreshed in the same sweep.
-// Over every voxel while *all (front-only pairs update their beliefs: the
-// first iteration of a call), afterwards over the voxels of list (those of
-// some camera's window pairs, the only beliefs that still change).
 __global__ __launch_bounds__(256) void k_residual(const float* __restrict__ B, float* __restrict__ A, size_t n,
-                                                   const uint32_t* __restrict__ list, size_t nlist,
-                                                   const uint32_t* __restrict__ all,
                                                    unsigned long long* __restrict__ d_max) {
   __shared__ double sh[256];
   double m = 0.0;
-  const bool every = *all != 0u;
-  const size_t ne = every ? n : nlist;
-  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < ne; j += (size_t)gridDim.x * blockDim.x) {
-    const size_t i = every ? j : (size_t)list[j];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const float a = marginal(B[i]);
     const double dlt = fabs((double)a - (double)A[i]);
     A[i] = a;
@@ -1348,12 +1342,11 @@ void launch_marginals(const float* B, float* A, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_marginals, dim3(grid_for(n, 256)), dim3(256), 0, s, B, A, n);
 }
 
-void launch_residual(const float* B, float* A, size_t n, const uint32_t* list, size_t nlist, const uint32_t* all,
-                     unsigned long long* d_max, hipStream_t s) {
+void launch_residual(const float* B, float* A, size_t n, unsigned long long* d_max, hipStream_t s) {
   if (n == 0) return;
   unsigned g = grid_for(n, 256);
   if (g > 4096) g = 4096;
-  hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, B, A, n, list, nlist, all, d_max);
+  hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, B, A, n, d_max);
 }
 
 void launch_ref_error(const float* A, const uint8_t* upd, const float* ref, size_t n, unsigned long long* d_max,

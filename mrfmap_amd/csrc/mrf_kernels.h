@@ -192,9 +192,7 @@ void launch_fill_dfront(const CamPass& a, const uint32_t* rows, const uint32_t* 
 // part: 0 both kernels, 1 the segment sums only, 2 the per-pair update only
 void launch_epilogue(const CamPass& a, hipStream_t s, int part = 0);
 void launch_marginals(const float* B, float* Aprev, size_t nvox, hipStream_t s);
-// over every voxel while *all != 0, else over the nlist voxels of list
-void launch_residual(const float* B, float* Aprev, size_t nvox, const uint32_t* list, size_t nlist,
-                     const uint32_t* all, unsigned long long* d_max, hipStream_t s);
+void launch_residual(const float* B, float* Aprev, size_t nvox, unsigned long long* d_max, hipStream_t s);
 void launch_ref_error(const float* A, const uint8_t* upd, const float* ref, size_t n, unsigned long long* d_max,
                       hipStream_t s);
 void launch_alphas(const float* B, const uint8_t* upd, float* out, size_t nvox, hipStream_t s);
