@@ -144,9 +144,11 @@ __device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t& x2, uint32
 // pair across its lanes (bitonic network), sums each run of equal pairs
 // (segmented scan) and issues one atomic per run instead of one per record.
 // Regions are whole rows of 64 records, so a row is either all front (its
-// lengths also go to the front sum) or all window.
+// lengths go to the front sum acc_f) or all window (acc_w); C is their sum
+// plus the parts below 2^-24 (acc_r), so a front run costs one atomic for
+// its 2^-24 units, not one for C and one for the front sum.
 __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
-                                                      const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_c,
+                                                      const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_w,
                                                       uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -170,9 +172,8 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
     wave_seg_sum_u(q, r, lane == 0 || prev != key ? 1u : 0u);
     const bool tail = lane == 63 || next != key;
     if (tail && key != 0xFFFFFFFFu) {
-      add_split(acc_c + 2 * (size_t)key, q);
+      add_split((base < base_w ? acc_f : acc_w) + 2 * (size_t)key, q);  // uniform: the row's region
       if (r) add_split(acc_r + 2 * (size_t)key, r);
-      if (base < base_w) add_split(acc_f + 2 * (size_t)key, q);
     }
   }
 }
@@ -180,15 +181,16 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
 // C in FP32 with the gate of the exact sum (C > 0.1f, gvdbkernels.cu:408):
 // where rounding to float would move C onto kCumLenMin from above, C takes
 // the next float up, so that the float gate falls as on the exact sum
-__global__ void k_pair_finish(const uint32_t* __restrict__ acc_c, const uint32_t* __restrict__ acc_f,
+__global__ void k_pair_finish(const uint32_t* __restrict__ acc_w, const uint32_t* __restrict__ acc_f,
                               const uint32_t* __restrict__ acc_r, uint32_t np, float* __restrict__ pair_c,
                               float* __restrict__ pair_sf) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const double c = (double)acc_c[2 * p + 1] * 4294967296.0 + (double)acc_c[2 * p];
+  // integers below 2^53: the doubles and their sum are exact
+  const double w = (double)acc_w[2 * p + 1] * 4294967296.0 + (double)acc_w[2 * p];
   const double cr = (double)acc_r[2 * p + 1] * 4294967296.0 + (double)acc_r[2 * p];
   const double f = (double)acc_f[2 * p + 1] * 4294967296.0 + (double)acc_f[2 * p];
-  const double cd = c / kFixLen + cr / (kFixLen * kFixLen);
+  const double cd = (w + f) / kFixLen + cr / (kFixLen * kFixLen);
   float cf = (float)cd;
   if (cd > (double)kCumLenGate) {
     if (!(cf > kCumLenMin)) cf = __uint_as_float(__float_as_uint(kCumLenMin) + 1u);
@@ -273,16 +275,16 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
                      pair_vox);
 }
 
-void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
+void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_w,
                          uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
                          hipStream_t s) {
   if (np == 0) return;
-  launch_zero_words(acc_c, (size_t)np * 8, s);
+  launch_zero_words(acc_w, (size_t)np * 8, s);
   launch_zero_words(acc_f, (size_t)np * 8, s);
   launch_zero_words(acc_r, (size_t)np * 8, s);
-  hipLaunchKernelGGL(k_pair_lengths, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_c, acc_f,
+  hipLaunchKernelGGL(k_pair_lengths, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w, acc_f,
                      acc_r);
-  hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_c, acc_f, acc_r, np, pair_c,
+  hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_w, acc_f, acc_r, np, pair_c,
                      pair_sf);
 }
 

@@ -71,10 +71,10 @@ void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t
 void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
                        uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s);
 // per pair: C (summed chord length) and F_EPS x the chord length of its
-// front records (index < base_w), exact fixed-point sums; acc_c / acc_f /
-// acc_r (C below 2^-24) hold 2 words per pair.  Rewrites every live record's
-// voxel to its pair (pid).
-void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_c,
+// front records (index < base_w), exact fixed-point sums; acc_w (window
+// records) / acc_f (front records) / acc_r (C below 2^-24) hold 2 words per
+// pair.  Rewrites every live record's voxel to its pair (pid).
+void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_w,
                          uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
                          hipStream_t s);
 size_t window_sort_temp_bytes(uint32_t n, int bits);
