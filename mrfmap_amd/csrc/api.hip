@@ -545,8 +545,20 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.nrays = (uint32_t)nrays;
   cam.nslices = ntiles;
   if ((rc = dalloc(c, &cam.d_inc, nrec))) return rc;
+  // 3. the records into their regions; the voxels of live records (any) and
+  //    of live window records (wany), and the counts of both
+  const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
+  if ((rc = ensure_vflags(c, std::max<uint64_t>(nvox, 1)))) return rc;
+  uint32_t* any = c->d_vflag;
+  uint32_t* wany = any + nvox;
+  uint32_t* pid = wany + nvox;  // rank of any, then the pair id
+  uint32_t* rank_w = pid + nvox;
+  launch_zero_words(any, (size_t)nvox * 8, s);  // any and wany
+  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 16, s));
   launch_reslice(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
-                 (uint32_t)nrec_f, cam.d_inc, s);
+                 (uint32_t)nrec_f, cam.d_inc, any, wany, (unsigned long long*)(c->d_total + 2), s);
+  uint64_t counts2[2] = {0, 0};  // live records, live window records
+  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 16, hipMemcpyDeviceToHost, s));
   if (c->p.brick_local_vis == MRF_VIS_BRICK_LOCAL) {
     // brick entries of the window records, while the records hold voxel ids;
     // rows past the last slice read as zero words (the window kernel's
@@ -572,19 +584,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.d_fdmask = reinterpret_cast<uint32_t*>(cam.d_partial + 2 * nparts);
   launch_chunk_desc(cam.d_chunk_off, cam.d_len_f, ntiles, cam.d_chunk_desc, s);
   launch_front_span(cam.d_front, cam.d_lanes, ntiles * kSlice, cam.d_fspan, s);
-  // 3. live incidences (records that are not gap records): all, window region
   // 4. (camera, voxel) pairs: window pairs first, each group in voxel order
-  const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
-  if ((rc = ensure_vflags(c, std::max<uint64_t>(nvox, 1)))) return rc;
-  uint32_t* any = c->d_vflag;
-  uint32_t* wany = any + nvox;
-  uint32_t* pid = wany + nvox;  // rank of any, then the pair id
-  uint32_t* rank_w = pid + nvox;
-  launch_zero_words(any, (size_t)nvox * 8, s);  // any and wany
-  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 16, s));
-  launch_vox_flags(cam.d_inc, nrec, nrec_f, any, wany, (unsigned long long*)(c->d_total + 2), s);
-  uint64_t counts2[2] = {0, 0};  // live records, live window records
-  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 16, hipMemcpyDeviceToHost, s));
   if ((rc = flag_scan(c, any, pid, nvox))) return rc;
   const uint64_t np64 = read_total(c);  // synchronises: counts2 is in
   // the ray kernels address a camera's per-pair messages with 32-bit byte offsets

@@ -198,15 +198,25 @@ __global__ __launch_bounds__(256) void k_split(FrontRec* __restrict__ front, Ray
     Poly sig, bia;
     load_cell(sigt, biat, lookup_n, r.cell_flags & kCellMask, sig, bia);
     const uint2* __restrict__ rl = sell + f.off;
-    for (uint32_t k = 0; k < n; ++k) {
-      const float ln = __uint_as_float(rl[(size_t)k * kSlice].y);
-      const float s = r.z0 + kS.at(ln) * r.inv_norm;
-      const float sg = sig(s);
-      if (!(sg > 0.0f && r.depth - (s + bia(s)) > kFrontU * (3.0f * sg))) {
-        nf = k;
-        break;
+    // lengths in batches of 8 (their loads in flight together)
+    bool open = true;
+    for (uint32_t k0 = 0; open && k0 < n; k0 += 8) {
+      float lb[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) lb[u] = k0 + u < n ? __uint_as_float(rl[(size_t)(k0 + u) * kSlice].y) : 0.0f;
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        if (!open || k0 + u >= n) break;
+        const float ln = lb[u];
+        const float s = r.z0 + kS.at(ln) * r.inv_norm;
+        const float sg = sig(s);
+        if (!(sg > 0.0f && r.depth - (s + bia(s)) > kFrontU * (3.0f * sg))) {
+          nf = k0 + u;
+          open = false;
+          break;
+        }
+        kS.add(ln);
       }
-      kS.add(ln);
     }
   }
   f.nf = nf;
@@ -221,27 +231,85 @@ __global__ __launch_bounds__(256) void k_split(FrontRec* __restrict__ front, Ray
 
 // Copy every ray's records into the front and window regions of the final
 // record buffer (sliced: lane l of slice t at region offset + k * 64 + l),
-// padding each lane to its slice's length with gap records.
+// padding each lane to its slice's length with gap records.  One wavefront
+// is one slice, so record k of the 64 rays moves together: per row the
+// voxel flags any[v] = 1 (every live record) and wany[v] = 1 (live window
+// records) are set, skipping a lane whose voxel equals that of the ray to
+// its left or above in the pixel tile (the flags say only whether), and
+// counts[0] / counts[1] gain the row's live / live window records.
 __global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, RayRec* __restrict__ lanes,
                                                  uint32_t npix, const uint2* __restrict__ sell,
                                                  const uint32_t* __restrict__ off_f, const uint32_t* __restrict__ len_f,
                                                  const uint32_t* __restrict__ off_w, const uint32_t* __restrict__ len_w,
-                                                 uint32_t base_w, uint2* __restrict__ inc) {
+                                                 uint32_t base_w, uint2* __restrict__ inc, uint32_t* __restrict__ any,
+                                                 uint32_t* __restrict__ wany, unsigned long long* __restrict__ counts) {
   const uint32_t ray = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ray >= npix) return;
+  if (ray >= npix) return;  // uniform per wavefront: npix is a multiple of 64
   const uint32_t t = ray >> 6, l = ray & 63u;
-  FrontRec f = front[ray];
-  RayRec r = lanes[ray];
+  const FrontRec f = front[ray];
+  const uint32_t rnw = lanes[ray].nw;
   const uint2* __restrict__ src = sell + f.off;
   const uint32_t fo = off_f[t] + l, wo = base_w + off_w[t] + l;
   const uint2 gap = make_uint2(kGapVox, 0u);
-  for (uint32_t k = 0; k < len_f[t]; ++k) inc[fo + (size_t)k * kSlice] = k < f.nf ? src[(size_t)k * kSlice] : gap;
-  for (uint32_t k = 0; k < len_w[t]; ++k)
-    inc[wo + (size_t)k * kSlice] = k < r.nw ? src[(size_t)(f.nf + k) * kSlice] : gap;
-  f.off = fo;
-  r.off_w = wo;
-  front[ray] = f;
-  lanes[ray] = r;
+  // a voxel this lane need not flag: the left or the upper ray has it too
+  auto dup = [&](uint32_t v) {
+    const uint32_t left = (uint32_t)__shfl_up((int)v, 1, 64), up = (uint32_t)__shfl_up((int)v, 8, 64);
+    return ((l & 7u) != 0u && left == v) || (l >= 8u && up == v);
+  };
+  // rows in batches of kB: the batch's loads are in flight together (the
+  // bounds lf, lw are uniform per wavefront, so every lane takes part in the
+  // shuffles)
+  constexpr uint32_t kB = 8;
+  uint64_t nl = 0, nw = 0;
+  const uint32_t lf = len_f[t], lw = len_w[t];
+  for (uint32_t k0 = 0; k0 < lf; k0 += kB) {
+    uint2 e[kB];
+#pragma unroll
+    for (uint32_t u = 0; u < kB; ++u) {
+      e[u] = gap;
+      if (k0 + u < f.nf) e[u] = src[(size_t)(k0 + u) * kSlice];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kB; ++u) {
+      if (k0 + u < lf) {
+        inc[fo + (size_t)(k0 + u) * kSlice] = e[u];
+        const bool live = e[u].x != kGapVox;
+        nl += (uint64_t)__popcll(__ballot(live));
+        const bool d = dup(e[u].x);
+        if (live && !d) any[e[u].x] = 1u;
+      }
+    }
+  }
+  const uint2* __restrict__ srcw = src + (size_t)f.nf * kSlice;
+  for (uint32_t k0 = 0; k0 < lw; k0 += kB) {
+    uint2 e[kB];
+#pragma unroll
+    for (uint32_t u = 0; u < kB; ++u) {
+      e[u] = gap;
+      if (k0 + u < rnw) e[u] = srcw[(size_t)(k0 + u) * kSlice];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kB; ++u) {
+      if (k0 + u < lw) {
+        inc[wo + (size_t)(k0 + u) * kSlice] = e[u];
+        const bool live = e[u].x != kGapVox;
+        const uint64_t n = (uint64_t)__popcll(__ballot(live));
+        nl += n;
+        nw += n;
+        const bool d = dup(e[u].x);
+        if (live && !d) {
+          any[e[u].x] = 1u;
+          wany[e[u].x] = 1u;
+        }
+      }
+    }
+  }
+  if (l == 0) {
+    if (nl) atomicAdd(&counts[0], (unsigned long long)nl);
+    if (nw) atomicAdd(&counts[1], (unsigned long long)nw);
+  }
+  front[ray].off = fo;
+  lanes[ray].off_w = wo;
 }
 
 // Brick entries of the window records, one wavefront per slice, while the
@@ -420,26 +488,37 @@ __global__ __launch_bounds__(256) void k_front_sums(const uint2* __restrict__ fs
 // block), 0 when no record changes (the chunk's partial sums can be kept).
 // The live records that do not change, which the front sums stop reading
 // after the first two iterations of a call, are counted into nstatic.
-__global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ fspan,
+__global__ __launch_bounds__(1024) void k_chunk_static(const uint2* __restrict__ fspan,
                                                       const uint4* __restrict__ desc, uint32_t nchunks,
                                                       const uint2* __restrict__ inc,
                                                       uint32_t nwp, const uint32_t* __restrict__ pflag,
                                                       uint32_t* __restrict__ dyn_flag, uint32_t* __restrict__ dmask,
                                                       uint32_t* __restrict__ rows,
                                                       unsigned long long* __restrict__ nstatic) {
+  __shared__ uint32_t block_kept[16];
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= nchunks) return;  // uniform per wavefront
   const uint32_t l = threadIdx.x & 63u;
+  if (w >= nchunks) {  // uniform per wavefront; still takes part in the block's sum
+    if (l == 0) block_kept[threadIdx.x >> 6] = 0u;
+    __syncthreads();
+    return;
+  }
   const uint4 d = desc[w];
   const uint2 fs = fspan[d.y * kSlice + l];
   const uint32_t k0 = d.z, k1 = min(k0 + kChunk, fs.y);
   const uint2* __restrict__ rf = inc + fs.x;
+  // all loads of the chunk first, then the flags of its front-only pairs
+  uint32_t q[kChunk], pf[kChunk];
+#pragma unroll
+  for (uint32_t b = 0; b < kChunk; ++b) q[b] = k0 + b < k1 ? rf[(size_t)(k0 + b) * kSlice].x : kGapVox;
+#pragma unroll
+  for (uint32_t b = 0; b < kChunk; ++b) pf[b] = q[b] != kGapVox && q[b] >= nwp ? pflag[q[b] - nwp] : 0u;
   uint32_t m = 0, live = 0;
-  for (uint32_t k = k0; k < k1; ++k) {
-    const uint32_t q = rf[(size_t)k * kSlice].x;
-    if (q == kGapVox) continue;
+#pragma unroll
+  for (uint32_t b = 0; b < kChunk; ++b) {
+    if (q[b] == kGapVox) continue;
     ++live;
-    if (q < nwp || pflag[q - nwp] != 0u) m |= 1u << (k - k0);
+    if (q[b] < nwp || pf[b] != 0u) m |= 1u << b;
   }
   dmask[(size_t)w * kSlice + l] = m;
   uint32_t nr = (uint32_t)__popc(m);
@@ -448,10 +527,17 @@ __global__ __launch_bounds__(256) void k_chunk_static(const uint2* __restrict__ 
     nr = max(nr, (uint32_t)__shfl_xor((int)nr, o, 64));
     kept += (uint32_t)__shfl_xor((int)kept, o, 64);
   }
-  if (l == 0 && kept) atomicAdd(nstatic, (unsigned long long)kept);
   if (l == 0) {
     dyn_flag[w] = nr ? 1u : 0u;
     rows[w] = nr;
+    block_kept[threadIdx.x >> 6] = kept;
+  }
+  // one atomic per block, not per chunk: they all go to one address
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t sum = 0;
+    for (uint32_t i = 0; i < (blockDim.x >> 6); ++i) sum += block_kept[i];
+    if (sum) atomicAdd(nstatic, (unsigned long long)sum);
   }
 }
 
@@ -480,9 +566,13 @@ __global__ __launch_bounds__(256) void k_fill_dfront(const uint2* __restrict__ f
   const uint2* __restrict__ rf = inc + fs.x;
   const uint32_t m = dmask[(size_t)d.x * kSlice + l];
   const uint32_t r0 = row_off[i];
+  uint2 e[kChunk];
+#pragma unroll
+  for (uint32_t b = 0; b < kChunk; ++b) e[b] = (m >> b) & 1u ? rf[(size_t)(d.z + b) * kSlice] : make_uint2(0u, 0u);
   uint32_t r = 0;
+#pragma unroll
   for (uint32_t b = 0; b < kChunk; ++b)
-    if ((m >> b) & 1u) dfront[(size_t)(r0 + r++) * kSlice + l] = rf[(size_t)(d.z + b) * kSlice];
+    if ((m >> b) & 1u) dfront[(size_t)(r0 + r++) * kSlice + l] = e[b];
   if (l == 0) dcr[i] = make_uint2(r0, rows[d.x]);
 }
 
@@ -1291,10 +1381,10 @@ void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* se
 
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
                     const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
-                    hipStream_t s) {
+                    uint32_t* any, uint32_t* wany, unsigned long long* counts, hipStream_t s) {
   if (npix == 0) return;
   hipLaunchKernelGGL(k_reslice, dim3(grid_for(npix, 256)), dim3(256), 0, s, front, lanes, npix, sell, off_f, len_f,
-                     off_w, len_w, base_w, inc);
+                     off_w, len_w, base_w, inc, any, wany, counts);
 }
 
 void launch_brick_starts(const FrontRec* front, const RayRec* lanes, const uint32_t* len_w, uint32_t nslices,
@@ -1322,7 +1412,7 @@ void launch_gather_desc(const uint4* desc, const uint32_t* idx, uint32_t n, uint
 void launch_chunk_static(const CamPass& a, const uint32_t* pflag, uint32_t* dyn_flag, uint32_t* dmask, uint32_t* rows,
                          unsigned long long* nstatic, hipStream_t s) {
   if (a.nchunks == 0) return;
-  hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 4)), dim3(256), 0, s, a.fspan, a.chunk_desc, a.nchunks,
+  hipLaunchKernelGGL(k_chunk_static, dim3(grid_for(a.nchunks, 16)), dim3(1024), 0, s, a.fspan, a.chunk_desc, a.nchunks,
                      a.inc, a.nwp, pflag, dyn_flag, dmask, rows, nstatic);
 }
 

@@ -40,29 +40,6 @@ __device__ __forceinline__ void add_split(uint32_t* p, uint32_t q) {
   if (old + q < old) atomicAdd(&p[1], 1u);
 }
 
-// any[v] = 1 for every voxel of a live record, wany[v] = 1 for a window
-// record; counts[0] += live records, counts[1] += live window records
-__global__ void k_vox_flags(const uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w, uint32_t* __restrict__ any,
-                            uint32_t* __restrict__ wany, unsigned long long* __restrict__ counts) {
-  // nrec is a multiple of 64 and the rows of 64 lie wholly in one region, so
-  // the loop is uniform per wave and the counts are wave-wide popcounts
-  uint64_t nl = 0, nw = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = rec[i].x;
-    const bool live = v != kGapVox;
-    const uint64_t n = (uint64_t)__popcll(__ballot(live));
-    nl += n;
-    if (i >= base_w) nw += n;
-    if (!live) continue;
-    any[v] = 1u;
-    if (i >= base_w) wany[v] = 1u;
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (nl) atomicAdd(&counts[0], (unsigned long long)nl);
-    if (nw) atomicAdd(&counts[1], (unsigned long long)nw);
-  }
-}
-
 // pid[v] (written over rank_all) and pair_vox[pid] of every touched voxel
 __global__ void k_pair_table(const uint32_t* __restrict__ any, const uint32_t* __restrict__ wany,
                              uint32_t* __restrict__ rank_all, const uint32_t* __restrict__ rank_w, uint64_t nvox,
@@ -152,13 +129,21 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
                                                       uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r) {
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t base = wave * 64; base < nrec; base += nwaves * 64) {  // nrec is a multiple of 64
+  const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * 64;
+  // software pipeline over the wave's rows (nrec is a multiple of 64): the
+  // record of the row after next and the pair of the next row are loaded
+  // while a row is sorted and summed
+  auto load = [&](uint64_t b) { return b < nrec ? rec[b + lane] : make_uint2(kGapVox, 0u); };
+  auto pair_of = [&](uint2 e) { return e.x != kGapVox ? pid[e.x] : 0xFFFFFFFFu; };
+  uint2 e = load(wave * 64), e1 = load(wave * 64 + stride);
+  uint32_t pk = pair_of(e);
+  for (uint64_t base = wave * 64; base < nrec; base += stride) {
     const uint64_t i = base + lane;
+    const uint2 e2 = load(base + 2 * stride);
+    const uint32_t pk1 = pair_of(e1);
     uint32_t key = 0xFFFFFFFFu, q = 0, r = 0;
-    const uint2 e = rec[i];
     if (e.x != kGapVox) {
-      key = pid[e.x];
+      key = pk;
       rec[i].x = key;
       // l 2^24 is exact in double; q its integer part (< 2^25), r the rest
       // in units of 2^-48 (< 2^24)
@@ -175,6 +160,9 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
       add_split((base < base_w ? acc_f : acc_w) + 2 * (size_t)key, q);  // uniform: the row's region
       if (r) add_split(acc_r + 2 * (size_t)key, r);
     }
+    e = e1;
+    e1 = e2;
+    pk = pk1;
   }
 }
 
@@ -260,12 +248,6 @@ size_t window_sort_temp_bytes(uint32_t n, int bits) {
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, bits);
   return bytes;
-}
-
-void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t* any, uint32_t* wany,
-                      unsigned long long* counts, hipStream_t s) {
-  if (nrec == 0) return;
-  hipLaunchKernelGGL(k_vox_flags, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, any, wany, counts);
 }
 
 void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,

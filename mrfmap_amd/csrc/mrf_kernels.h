@@ -61,10 +61,6 @@ int num_tiled_pixels(const CamGeom& c);
 
 // geometry.hip: (camera, voxel) pairs of a camera's incidences and the
 // window segment layout of the BP iteration.
-// any / wany: pool-sized flags of voxels touched by a live / window record;
-// counts[0] / counts[1] += the live / live window records
-void launch_vox_flags(const uint2* rec, uint64_t nrec, uint64_t base_w, uint32_t* any, uint32_t* wany,
-                      unsigned long long* counts, hipStream_t s);
 // from the exclusive scans rank_all (of any) and rank_w (of wany): pair id
 // of every touched voxel (window pairs first, each group in voxel order),
 // written over rank_all, and pair_vox[pair]
@@ -157,10 +153,12 @@ struct CamPass {
 // front part): sets nf, nw, the front length sums and per-pixel counts
 void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
                   int lookup_n, uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s);
-// copy into the final front/window regions; window region starts at base_w
+// copy into the final front/window regions; window region starts at base_w.
+// any / wany: pool-sized flags of voxels touched by a live / window record
+// (set, never cleared); counts[0] / counts[1] += the live / live window records
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
                     const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
-                    hipStream_t s);
+                    uint32_t* any, uint32_t* wany, unsigned long long* counts, hipStream_t s);
 // brick entries of every window record (records still hold voxel ids):
 // bstart[(region row of the slice) + k] bit l = record k of lane l enters a
 // brick (the ray's first record, the record after a gap, or a voxel in

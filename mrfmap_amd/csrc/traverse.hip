@@ -149,6 +149,7 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
 
   int ix[3], sg[3];
   Rat t[3];
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int64_t Oi = rs.Oi[a], Di = rs.Di[a];
     bool exact;
@@ -183,14 +184,23 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
     return slot[lin];
   };
   bslot = brick_lookup();
+  // The next crossing of each moving axis is t[a] = n_a / d_a with d_a fixed
+  // and n_a growing by 2^kFix per step, so the cross differences
+  // X_ab = n_a d_b - n_b d_a (t_a < t_b iff X_ab < 0, equal iff 0) are kept
+  // and updated by additions: the same exact comparisons as rat_lt, with no
+  // multiplications and no run-time indexing in the loop.
+  const bool m0 = sg[0] != 0, m1 = sg[1] != 0, m2 = sg[2] != 0;
+  const int64_t s0 = t[0].d << kFix, s1 = t[1].d << kFix, s2 = t[2].d << kFix;
+  int64_t n0 = t[0].n, n1 = t[1].n, n2 = t[2].n;
+  const int64_t d0 = t[0].d, d1 = t[1].d, d2 = t[2].d;
+  int64_t X01 = n0 * d1 - n1 * d0, X02 = n0 * d2 - n2 * d0, X12 = n1 * d2 - n2 * d1;
   const int max_steps = g.nx + g.ny + g.nz + 3;
   for (int step = 0; step < max_steps; ++step) {
-    int best = -1;
-    for (int a = 0; a < 3; ++a) {
-      if (sg[a] == 0) continue;
-      if (best < 0 || rat_lt(t[a], t[best])) best = a;
-    }
-    const Rat zn = t[best];
+    // the nearest crossing; the lowest axis among equals (strict comparisons)
+    int best = m0 ? 0 : (m1 ? 1 : 2);
+    if (m1 && best == 0 && X01 > 0) best = 1;
+    if (m2 && best != 2 && (best == 0 ? X02 > 0 : X12 > 0)) best = 2;
+    const Rat zn = {best == 0 ? n0 : (best == 1 ? n1 : n2), best == 0 ? d0 : (best == 1 ? d1 : d2)};
     // per-voxel depths only where records or visits need them; the marking and
     // counting walks convert the exit depth at brick exits alone
     constexpr bool kDepths = MODE == kEmit || MODE == kVisit;
@@ -228,12 +238,25 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
     } else if (any) {
       skipped = true;
     }
-    int nx[3] = {ix[0], ix[1], ix[2]};
-    for (int a = 0; a < 3; ++a) {
-      if (sg[a] != 0 && !rat_lt(zn, t[a])) {
-        nx[a] += sg[a];
-        t[a].n += ((int64_t)1 << kFix);
-      }
+    // every axis whose crossing equals the nearest one steps
+    const bool st0 = m0 && (best == 0 || (best == 1 ? X01 == 0 : X02 == 0));
+    const bool st1 = m1 && (best == 1 || (best == 0 ? X01 == 0 : X12 == 0));
+    const bool st2 = m2 && (best == 2 || (best == 0 ? X02 == 0 : X12 == 0));
+    const int nx[3] = {ix[0] + (st0 ? sg[0] : 0), ix[1] + (st1 ? sg[1] : 0), ix[2] + (st2 ? sg[2] : 0)};
+    if (st0) {
+      n0 += (int64_t)1 << kFix;
+      X01 += s1;
+      X02 += s2;
+    }
+    if (st1) {
+      n1 += (int64_t)1 << kFix;
+      X01 -= s0;
+      X12 += s2;
+    }
+    if (st2) {
+      n2 += (int64_t)1 << kFix;
+      X02 -= s0;
+      X12 -= s1;
     }
     const bool leaving = nx[0] < 0 || nx[0] >= N[0] || nx[1] < 0 || nx[1] >= N[1] || nx[2] < 0 ||
                          nx[2] >= N[2];
