@@ -44,7 +44,7 @@ struct Camera {
   uint32_t nslices = 0;
   uint32_t nrays = 0;              // rays with at least one record
   uint2* d_inc = nullptr;          // sliced records {pair | gap, length}: front region, then window region
-  uint32_t* d_walkn = nullptr;     // full-ray activation: records per tiled pixel, then has-records flags
+  uint32_t* d_walkn = nullptr;     // full-ray activation: bound of the records per tiled pixel
   uint8_t* d_lnn = nullptr;        // per pair: voxel -> ray message log2 n (input-side message type)
   uint32_t* d_dyn = nullptr;       // front-only pairs at voxels where some camera has window records
   uint32_t ndyn = 0;
@@ -133,7 +133,7 @@ struct mrf_ctx {
   size_t cap_scan = 0;
   void* d_fscan_tmp = nullptr;  // flag-scan scratch
   size_t fscan_bytes = 0;
-  uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2] counter
+  uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2..5] counters, [6] flag
   uint32_t* d_all_pairs = nullptr;  // [0] CamPass::all_pairs, [1] CamPass::msg_all
 
   std::vector<Camera> cams;
@@ -501,20 +501,16 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   hipStream_t s = c->stream;
   int rc;
   if ((rc = ensure_pixel_scratch(c, npix)) || (rc = ensure_scan(c, npix))) return rc;
-  // 1. every ray's records, in order, into one sliced region: per tile the
-  //    longest of its 64 rays (d_has), slice sizes (d_rayidx) -> offsets (d_offs)
-  if (cam.d_walkn) {
-    // counted by the activation walk (full-ray activation)
-    HIPCHK(c, hipMemcpyAsync(c->d_counts, cam.d_walkn, npix * 4, hipMemcpyDeviceToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(c->d_has, cam.d_walkn + npix, npix * 4, hipMemcpyDeviceToDevice, s));
-  } else {
-    launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, c->d_has, s);
+  // 1. every ray's records, in order, into one sliced staging region sized
+  //    by a bound of each ray's count (full-ray activation: the bound of the
+  //    activation walk; endpoint activation: the count): per tile the
+  //    largest of its 64 rays (d_has), slice sizes (d_rayidx) -> offsets (d_offs)
+  const uint32_t* per_ray = cam.d_walkn;
+  if (!per_ray) {
+    launch_count(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_counts, s);
+    per_ray = c->d_counts;
   }
-  launch_exclusive_scan(c->d_counts, c->d_offs, npix, c->d_scan_tmp, c->d_total, s);
-  const uint64_t nlist = read_total(c);
-  launch_exclusive_scan(c->d_has, c->d_rayidx, npix, c->d_scan_tmp, c->d_total, s);
-  const uint64_t nrays = read_total(c);
-  launch_slice_len(c->d_counts, ntiles, c->d_has, c->d_rayidx, s);
+  launch_slice_len(per_ray, ntiles, c->d_has, c->d_rayidx, s);
   launch_exclusive_scan(c->d_rayidx, c->d_offs, ntiles, c->d_scan_tmp, c->d_total, s);
   const uint64_t nrec0 = read_total(c);
   HIPCHK(c, hipGetLastError());
@@ -522,8 +518,10 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if ((rc = ensure_sell0(c, std::max<uint64_t>(nrec0, 1))) || (rc = dalloc(c, &cam.d_front, npix)) ||
       (rc = dalloc(c, &cam.d_lanes, npix)))
     return rc;
+  uint32_t* overflow = reinterpret_cast<uint32_t*>(c->d_total + 6);
+  launch_zero_words(overflow, 8, s);
   launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_has, c->d_sell0, cam.d_front,
-              cam.d_lanes, s);
+              cam.d_lanes, overflow, s);
   // 2. front/window split (counts into d_counts / d_rayidx), then the front
   //    region (sizes d_has -> offsets d_offs) and the window region (sizes
   //    d_counts -> offsets d_has) of the final record buffer
@@ -541,8 +539,6 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if (nrec >= 0x7FFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^31 incidence records in one camera");
   cam.nrec = nrec;
   cam.base_w = nrec_f;
-  cam.nlist = nlist;
-  cam.nrays = (uint32_t)nrays;
   cam.nslices = ntiles;
   if ((rc = dalloc(c, &cam.d_inc, nrec))) return rc;
   // 3. the records into their regions; the voxels of live records (any) and
@@ -554,11 +550,13 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   uint32_t* pid = wany + nvox;  // rank of any, then the pair id
   uint32_t* rank_w = pid + nvox;
   launch_zero_words(any, (size_t)nvox * 8, s);  // any and wany
-  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 16, s));
+  HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 32, s));
   launch_reslice(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
                  (uint32_t)nrec_f, cam.d_inc, any, wany, (unsigned long long*)(c->d_total + 2), s);
-  uint64_t counts2[2] = {0, 0};  // live records, live window records
-  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 16, hipMemcpyDeviceToHost, s));
+  // live records, live window records, records (gap records included), rays
+  // with records; the emission's overflow flag
+  uint64_t counts2[5] = {0, 0, 0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 40, hipMemcpyDeviceToHost, s));
   if (c->p.brick_local_vis == MRF_VIS_BRICK_LOCAL) {
     // brick entries of the window records, while the records hold voxel ids;
     // rows past the last slice read as zero words (the window kernel's
@@ -587,12 +585,15 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   // 4. (camera, voxel) pairs: window pairs first, each group in voxel order
   if ((rc = flag_scan(c, any, pid, nvox))) return rc;
   const uint64_t np64 = read_total(c);  // synchronises: counts2 is in
+  if (counts2[4]) return fail(c, MRF_E_LIMIT, "traversal: a ray has more records than its bound");
   // the ray kernels address a camera's per-pair messages with 32-bit byte offsets
   if (np64 >= (1ull << 30)) return fail(c, MRF_E_LIMIT, "more than 2^30 (camera, voxel) pairs in one camera");
   const uint32_t np = (uint32_t)np64;
   cam.nupd = counts2[0];
   const uint64_t nwin = counts2[1];
   cam.nwin = nwin;
+  cam.nlist = counts2[2];
+  cam.nrays = (uint32_t)counts2[3];
   if ((rc = flag_scan(c, wany, rank_w, nvox))) return rc;
   const uint32_t nwp = (uint32_t)read_total(c);
   cam.npairs = np;
@@ -647,9 +648,8 @@ int prepare(mrf_ctx* c) {
     if (c->p.activation == MRF_ACT_FULL_RAY) {
       const size_t npix = (size_t)num_tiled_pixels(cg);
       dfree(&cam.d_walkn);
-      if ((rc = dalloc(c, &cam.d_walkn, 2 * npix))) return rc;
-      launch_mark_rays(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_bflags, cam.d_walkn, cam.d_walkn + npix,
-                       c->stream);
+      if ((rc = dalloc(c, &cam.d_walkn, npix))) return rc;
+      launch_mark_rays(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_bflags, cam.d_walkn, c->stream);
     } else
       launch_mark_endpoints(cg, c->g, cam.d_depth, c->d_bflags, c->stream);
   }
@@ -1231,7 +1231,7 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
   }
   int rc;
   if ((rc = dalloc(c, &c->d_slot, (size_t)c->nbricks)) || (rc = dalloc(c, &c->d_bflags, (size_t)c->nbricks)) ||
-      (rc = dalloc(c, &c->d_total, 4)) || (rc = dalloc(c, &c->d_all_pairs, 2))) {
+      (rc = dalloc(c, &c->d_total, 8)) || (rc = dalloc(c, &c->d_all_pairs, 2))) {
     mrf_destroy(c);
     return rc;
   }

@@ -236,7 +236,9 @@ __global__ __launch_bounds__(256) void k_split(FrontRec* __restrict__ front, Ray
 // voxel flags any[v] = 1 (every live record) and wany[v] = 1 (live window
 // records) are set, skipping a lane whose voxel equals that of the ray to
 // its left or above in the pixel tile (the flags say only whether), and
-// counts[0] / counts[1] gain the row's live / live window records.
+// counts[0] / counts[1] gain the row's live / live window records; per
+// wavefront counts[2] gains its rays' records (gap records included) and
+// counts[3] its rays with records.
 __global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, RayRec* __restrict__ lanes,
                                                  uint32_t npix, const uint2* __restrict__ sell,
                                                  const uint32_t* __restrict__ off_f, const uint32_t* __restrict__ len_f,
@@ -304,9 +306,15 @@ __global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, R
       }
     }
   }
+  // the ray's records (gap records included) and whether it has any
+  uint32_t nr = f.nf + rnw;
+  const uint64_t nray = (uint64_t)__popcll(__ballot(nr != 0u));
+  for (int o = 32; o > 0; o >>= 1) nr += (uint32_t)__shfl_xor((int)nr, o, 64);
   if (l == 0) {
     if (nl) atomicAdd(&counts[0], (unsigned long long)nl);
     if (nw) atomicAdd(&counts[1], (unsigned long long)nw);
+    if (nr) atomicAdd(&counts[2], (unsigned long long)nr);
+    if (nray) atomicAdd(&counts[3], (unsigned long long)nray);
   }
   front[ray].off = fo;
   lanes[ray].off_w = wo;

@@ -21,20 +21,25 @@ enum RaySched { kSchedF8W4 = 0, kSchedF8W8 = 1, kSchedF16W4 = 2, kSchedF16W8 = 3
                 kNumSched = 6 };
 
 // traverse.hip
+// full-ray activation (bricks marked in bflags); bound: per tiled pixel, a
+// bound of the ray's record count (may be null)
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                      const float* bia, uint8_t* bflags, uint32_t* counts, uint32_t* has, hipStream_t s);
+                      const float* bia, uint8_t* bflags, uint32_t* bound, hipStream_t s);
 void launch_mark_endpoints(const CamGeom& c, const GridGeom& g, const float* depth, uint8_t* bflags,
                            hipStream_t s);
+// records per tiled pixel (endpoint activation: gap records included)
 void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                  const float* bia, const int32_t* slot, uint32_t* counts, uint32_t* has, hipStream_t s);
+                  const float* bia, const int32_t* slot, uint32_t* counts, hipStream_t s);
 // slice (8x8 pixel tile) lengths: max records over the tile's 64 rays
 void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_len, uint32_t* slice_rec,
                       hipStream_t s);
-// sliced-ELL emission: record k of lane l of slice t at slice_off[t] + k*64 + l,
-// padded to the slice length with {kGapVox, 0}; lanes[t*64+l] describes the ray
+// sliced-ELL emission: record k of lane l of slice t at slice_off[t] + k*64 + l
+// for k < front[t*64+l].nf (the ray's records, nothing past them is written);
+// lanes[t*64+l] describes the ray.  A ray with more records than its slice's
+// length slice_len[t] (sized by a count or a bound of it) sets *overflow.
 void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
                  const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
-                 FrontRec* front, RayRec* lanes, hipStream_t s);
+                 FrontRec* front, RayRec* lanes, uint32_t* overflow, hipStream_t s);
 
 // map evaluation of one camera's pixels (log-likelihood, expected depth,
 // accuracy and simple accuracy images in row-major pixel order, any may be
@@ -155,7 +160,9 @@ void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* se
                   int lookup_n, uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s);
 // copy into the final front/window regions; window region starts at base_w.
 // any / wany: pool-sized flags of voxels touched by a live / window record
-// (set, never cleared); counts[0] / counts[1] += the live / live window records
+// (set, never cleared); counts[0] / counts[1] += the live / live window
+// records, counts[2] += all records (gap records included), counts[3] += rays
+// with records
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
                     const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
                     uint32_t* any, uint32_t* wany, unsigned long long* counts, hipStream_t s);

@@ -102,7 +102,7 @@ __device__ __forceinline__ uint32_t lookup_offset(const CamGeom& c, int lookup_n
   return (uint32_t)((v / kLookupPx) * (c.lookup_u * lookup_n) + (x / kLookupPx) * lookup_n);
 }
 
-enum WalkMode { kMark = 0, kCount = 1, kEmit = 2, kVisit = 3 };
+enum WalkMode { kCount = 1, kEmit = 2, kVisit = 3 };
 
 struct WalkOut {
   uint32_t n;
@@ -111,24 +111,19 @@ struct WalkOut {
   double last_exit;  // exact exit depth of the last voxel record (kEmit, kVisit)
 };
 
-// Records of one ray are written out[0], out[kSlice], out[2 kSlice], ...
-// (record k of the 64 rays of a pixel tile is contiguous: sliced ELL).
-// kVisit walks to the end of the grid without truncation and hands every
-// voxel of an active brick to vis->voxel(id, ix, chord, z entry, z exit) and
-// every active brick's exit depth to vis->brick_end(z); a true return from
-// either ends the walk.
-template <int MODE, class V = int>
-__device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, const float* sig,
-                        const float* bia, const int32_t* __restrict__ slot,
-                        uint8_t* __restrict__ bflags, uint2* __restrict__ out, V* vis = nullptr) {
-  WalkOut wo = {0u, 0.0f, true};
+// Entry of a pixel ray into the voxel grid: the exact entry depth zin (at
+// least 0), the voxel there and the step sign of each axis; false when the
+// ray misses the grid.
+__device__ __forceinline__ bool ray_entry(const GridGeom& g, const RaySetup& rs, Rat& zin, int ix[3], int sg[3]) {
   const int N[3] = {g.nx, g.ny, g.nz};
-  Rat zin = {0, 1}, zout = {0, 1};
+  zin = {0, 1};
+  Rat zout = {0, 1};
   bool has_out = false;
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int64_t Oi = rs.Oi[a], Di = rs.Di[a], top = (int64_t)N[a] << kFix;
     if (Di == 0) {
-      if (Oi < 0 || Oi >= top) return wo;
+      if (Oi < 0 || Oi >= top) return false;
       continue;
     }
     Rat lo, hi;
@@ -145,10 +140,7 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
       has_out = true;
     }
   }
-  if (!has_out || !rat_lt(zin, zout)) return wo;
-
-  int ix[3], sg[3];
-  Rat t[3];
+  if (!has_out || !rat_lt(zin, zout)) return false;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int64_t Oi = rs.Oi[a], Di = rs.Di[a];
@@ -160,6 +152,30 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
     if (q > N[a] - 1) q = N[a] - 1;
     ix[a] = (int)q;
     sg[a] = Di > 0 ? 1 : (Di < 0 ? -1 : 0);
+  }
+  return true;
+}
+
+// Records of one ray are written out[0], out[kSlice], out[2 kSlice], ...
+// (record k of the 64 rays of a pixel tile is contiguous: sliced ELL), the
+// first cap of them (the count goes on past cap).
+// kVisit walks to the end of the grid without truncation and hands every
+// voxel of an active brick to vis->voxel(id, ix, chord, z entry, z exit) and
+// every active brick's exit depth to vis->brick_end(z); a true return from
+// either ends the walk.
+template <int MODE, class V = int>
+__device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, const float* sig,
+                        const float* bia, const int32_t* __restrict__ slot, uint2* __restrict__ out,
+                        uint32_t cap, V* vis = nullptr) {
+  WalkOut wo = {0u, 0.0f, true};
+  const int N[3] = {g.nx, g.ny, g.nz};
+  Rat zin;
+  int ix[3], sg[3];
+  if (!ray_entry(g, rs, zin, ix, sg)) return wo;
+  Rat t[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t Oi = rs.Oi[a], Di = rs.Di[a];
     if (Di > 0)
       t[a] = {(((int64_t)ix[a] + 1) << kFix) - Oi, Di};
     else if (Di < 0)
@@ -176,12 +192,7 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
   uint32_t cnt = 0;
   int32_t bslot = 0;
   auto brick_lookup = [&](void) -> int32_t {
-    const int lin = ((ix[0] >> kBrickLog) * g.by + (ix[1] >> kBrickLog)) * g.bz + (ix[2] >> kBrickLog);
-    if (MODE == kMark) {
-      bflags[lin] = 1;
-      return 0;
-    }
-    return slot[lin];
+    return slot[((ix[0] >> kBrickLog) * g.by + (ix[1] >> kBrickLog)) * g.bz + (ix[2] >> kBrickLog)];
   };
   bslot = brick_lookup();
   // The next crossing of each moving axis is t[a] = n_a / d_a with d_a fixed
@@ -201,40 +212,37 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
     if (m1 && best == 0 && X01 > 0) best = 1;
     if (m2 && best != 2 && (best == 0 ? X02 > 0 : X12 > 0)) best = 2;
     const Rat zn = {best == 0 ? n0 : (best == 1 ? n1 : n2), best == 0 ? d0 : (best == 1 ? d1 : d2)};
-    // per-voxel depths only where records or visits need them; the marking and
-    // counting walks convert the exit depth at brick exits alone
+    // per-voxel depths only where records or visits need them; the counting
+    // walk converts the exit depth at brick exits alone
     constexpr bool kDepths = MODE == kEmit || MODE == kVisit;
     double zn_d = 0.0;
     if constexpr (kDepths) zn_d = (double)zn.n / (double)zn.d;
     const bool active = bslot >= 0;
     if (active) {
-      if (MODE == kMark) ++cnt;  // every brick counts as active: no gaps, the count walk's n
-      if (MODE != kMark) {
-        if (any && skipped) {
-          if (MODE == kEmit) {
-            float gl = (float)((zc_d - last_exit) * rs.norm);
-            out[(size_t)cnt * kSlice] = make_uint2(kGapVox, __float_as_uint(gl));
-          }
-          ++cnt;
-          skipped = false;
-        }
-        if (!any) {
-          wo.z0 = (float)zc_d;
-          any = true;
-        }
-        if constexpr (MODE == kEmit || MODE == kVisit) {
-          const uint32_t local = (uint32_t)((((ix[0] & 7) << 3) | (ix[1] & 7)) << 3 | (ix[2] & 7));
-          const uint32_t vox = (uint32_t)bslot * kBrickVox + local;
-          float len = (float)((zn_d - zc_d) * rs.norm);
-          if constexpr (MODE == kEmit) {
-            out[(size_t)cnt * kSlice] = make_uint2(vox, __float_as_uint(len));
-          } else {
-            if (vis->voxel(vox, ix, len, zc_d, zn_d)) break;
-          }
+      if (any && skipped) {
+        if (MODE == kEmit) {
+          float gl = (float)((zc_d - last_exit) * rs.norm);
+          if (cnt < cap) out[(size_t)cnt * kSlice] = make_uint2(kGapVox, __float_as_uint(gl));
         }
         ++cnt;
-        last_exit = zn_d;
+        skipped = false;
       }
+      if (!any) {
+        wo.z0 = (float)zc_d;
+        any = true;
+      }
+      if constexpr (MODE == kEmit || MODE == kVisit) {
+        const uint32_t local = (uint32_t)((((ix[0] & 7) << 3) | (ix[1] & 7)) << 3 | (ix[2] & 7));
+        const uint32_t vox = (uint32_t)bslot * kBrickVox + local;
+        float len = (float)((zn_d - zc_d) * rs.norm);
+        if constexpr (MODE == kEmit) {
+          if (cnt < cap) out[(size_t)cnt * kSlice] = make_uint2(vox, __float_as_uint(len));
+        } else {
+          if (vis->voxel(vox, ix, len, zc_d, zn_d)) break;
+        }
+      }
+      ++cnt;
+      last_exit = zn_d;
     } else if (any) {
       skipped = true;
     }
@@ -323,24 +331,103 @@ __device__ __forceinline__ PixelCtx load_pixel(const CamGeom& c, const GridGeom&
   return p;
 }
 
+// Full-ray activation of one ray, walked over the brick grid: every brick
+// the voxel walk would cross up to its truncation is marked.  Brick faces
+// are voxel faces, so the bricks are met in the order of the voxel walk and
+// each brick exit is the same exact rational (n 2^kFix - Oi) / Di, with the
+// same tie rule (all tied axes step at once); the truncation test runs at
+// each brick exit as in walk().  The return value bounds the voxel walk's
+// record count: every voxel step advances at least one axis by one voxel,
+// each axis runs monotonically from the entry voxel to a voxel of the last
+// brick, so n <= 1 + sum over axes of |last - entry| with "last" the far
+// voxel of that brick along the axis (ties and a stop short of that voxel
+// only make n smaller).  0 for a ray that misses the grid.
+__device__ uint32_t mark_bricks(const GridGeom& g, const RaySetup& rs, double depth, const float* sig,
+                                const float* bia, uint8_t* __restrict__ bflags) {
+  Rat zin;
+  int ix[3], sg[3];
+  if (!ray_entry(g, rs, zin, ix, sg)) return 0;
+  const int NB[3] = {g.bx, g.by, g.bz};
+  int b[3] = {ix[0] >> kBrickLog, ix[1] >> kBrickLog, ix[2] >> kBrickLog};
+  // next brick-face crossing per moving axis: n_a / d_a, n_a growing by
+  // 2^(kFix + kBrickLog) per brick; cross differences as in walk()
+  int64_t n[3], d[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t Oi = rs.Oi[a], Di = rs.Di[a];
+    if (Di > 0) {
+      n[a] = (((int64_t)b[a] + 1) << (kFix + kBrickLog)) - Oi;
+      d[a] = Di;
+    } else if (Di < 0) {
+      n[a] = Oi - ((int64_t)b[a] << (kFix + kBrickLog));
+      d[a] = -Di;
+    } else {
+      n[a] = 1;
+      d[a] = 0;
+    }
+  }
+  const bool m0 = sg[0] != 0, m1 = sg[1] != 0, m2 = sg[2] != 0;
+  const int64_t s0 = d[0] << (kFix + kBrickLog), s1 = d[1] << (kFix + kBrickLog), s2 = d[2] << (kFix + kBrickLog);
+  int64_t X01 = n[0] * d[1] - n[1] * d[0], X02 = n[0] * d[2] - n[2] * d[0], X12 = n[1] * d[2] - n[2] * d[1];
+  const int lookup_n = g.lookup_n;
+  const int max_steps = g.bx + g.by + g.bz + 3;
+  for (int step = 0; step < max_steps; ++step) {
+    bflags[(b[0] * g.by + b[1]) * g.bz + b[2]] = 1;
+    int best = m0 ? 0 : (m1 ? 1 : 2);
+    if (m1 && best == 0 && X01 > 0) best = 1;
+    if (m2 && best != 2 && (best == 0 ? X02 > 0 : X12 > 0)) best = 2;
+    const int64_t zn_n = best == 0 ? n[0] : (best == 1 ? n[1] : n[2]);
+    const int64_t zn_dd = best == 0 ? d[0] : (best == 1 ? d[1] : d[2]);
+    const double zb = (double)zn_n / (double)zn_dd;
+    const double lhs = zb + poly_eval(bia, lookup_n, zb);
+    const double rhs = depth + 3.0 * poly_eval(sig, lookup_n, zb);
+    if (lhs >= rhs) break;
+    const bool st0 = m0 && (best == 0 || (best == 1 ? X01 == 0 : X02 == 0));
+    const bool st1 = m1 && (best == 1 || (best == 0 ? X01 == 0 : X12 == 0));
+    const bool st2 = m2 && (best == 2 || (best == 0 ? X02 == 0 : X12 == 0));
+    const int nb[3] = {b[0] + (st0 ? sg[0] : 0), b[1] + (st1 ? sg[1] : 0), b[2] + (st2 ? sg[2] : 0)};
+    if (nb[0] < 0 || nb[0] >= NB[0] || nb[1] < 0 || nb[1] >= NB[1] || nb[2] < 0 || nb[2] >= NB[2]) break;
+    if (st0) {
+      n[0] += (int64_t)1 << (kFix + kBrickLog);
+      X01 += s1;
+      X02 += s2;
+    }
+    if (st1) {
+      n[1] += (int64_t)1 << (kFix + kBrickLog);
+      X01 -= s0;
+      X12 += s2;
+    }
+    if (st2) {
+      n[2] += (int64_t)1 << (kFix + kBrickLog);
+      X02 -= s0;
+      X12 -= s1;
+    }
+    b[0] = nb[0];
+    b[1] = nb[1];
+    b[2] = nb[2];
+  }
+  uint32_t bound = 1;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (sg[a] > 0) bound += (uint32_t)((b[a] << kBrickLog) + (1 << kBrickLog) - 1 - ix[a]);
+    if (sg[a] < 0) bound += (uint32_t)(ix[a] - (b[a] << kBrickLog));
+  }
+  return bound;
+}
+
 // Full-ray activation: marks every brick a ray crosses up to its truncation.
 // A ray's own walk activates every brick before and at its stopping brick,
-// so its later list walk sees exactly these bricks, with no gaps: the
-// number of voxels stepped here is that walk's record count, written to
-// counts / has (per tiled pixel) when they are given.
+// so its later list walk sees exactly these bricks, with no gaps; bound (per
+// tiled pixel, when given) receives the bound of that walk's record count.
 __global__ void k_mark_rays(CamGeom c, GridGeom g, const float* __restrict__ depth,
                             const float* __restrict__ sig, const float* __restrict__ bia,
-                            uint8_t* __restrict__ bflags, uint32_t* __restrict__ counts,
-                            uint32_t* __restrict__ has) {
+                            uint8_t* __restrict__ bflags, uint32_t* __restrict__ bound) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= tiled_pixels(c)) return;
   PixelCtx p = load_pixel(c, g, depth, pix);
   uint32_t n = 0;
-  if (p.ok) n = walk<kMark>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, nullptr, bflags, nullptr).n;
-  if (counts) {
-    counts[pix] = n;
-    has[pix] = n > 0 ? 1u : 0u;
-  }
+  if (p.ok) n = mark_bricks(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, bflags);
+  if (bound) bound[pix] = n;
 }
 
 __global__ void k_mark_endpoints(CamGeom c, GridGeom g, const float* __restrict__ depth,
@@ -393,15 +480,13 @@ __global__ void k_feat_hits(CamGeom c, GridGeom g, const float* __restrict__ dep
 
 __global__ void k_count(CamGeom c, GridGeom g, const float* __restrict__ depth,
                         const float* __restrict__ sig, const float* __restrict__ bia,
-                        const int32_t* __restrict__ slot, uint32_t* __restrict__ counts,
-                        uint32_t* __restrict__ has) {
+                        const int32_t* __restrict__ slot, uint32_t* __restrict__ counts) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= tiled_pixels(c)) return;
   PixelCtx p = load_pixel(c, g, depth, pix);
   uint32_t n = 0;
-  if (p.ok) n = walk<kCount>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, nullptr).n;
+  if (p.ok) n = walk<kCount>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, 0u).n;
   counts[pix] = n;
-  has[pix] = n > 0 ? 1u : 0u;
 }
 
 // Records per slice (max over the tile's 64 rays) and the slice's padded size.
@@ -420,7 +505,8 @@ __global__ void k_slice_len(const uint32_t* __restrict__ counts, uint32_t ntiles
 __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, const float* __restrict__ sig,
                        const float* __restrict__ bia, const int32_t* __restrict__ slot,
                        const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ slice_len,
-                       uint2* __restrict__ sell, FrontRec* __restrict__ front, RayRec* __restrict__ lanes) {
+                       uint2* __restrict__ sell, FrontRec* __restrict__ front, RayRec* __restrict__ lanes,
+                       uint32_t* __restrict__ overflow) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= tiled_pixels(c)) return;
   const uint32_t tile = (uint32_t)pix >> 6, lane = (uint32_t)pix & 63u;
@@ -439,8 +525,13 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
   r.depth = 0.0f;
   r.cell_flags = 0;
   if (p.ok) {
-    WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, nullptr, out);
+    const uint32_t cap = slice_len[tile];
+    WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, out, cap);
     f.nf = wo.n;
+    if (wo.n > cap) {  // the slot's bound did not hold: an error, and the list stays in bounds
+      atomicOr(overflow, 1u);
+      f.nf = cap;
+    }
     r.z0 = wo.z0;
     r.inv_norm = (float)(1.0 / p.rs.norm);
     r.depth = p.d;
@@ -450,9 +541,7 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
     const bool inf = wo.ran_out && (double)p.d > wo.last_exit + poly_eval(bia + p.cell, g.lookup_n, wo.last_exit);
     r.cell_flags = p.cell | (wo.ran_out ? kRanOut : 0u) | (inf ? kInfTerm : 0u);
   }
-  // pad the lane to the slice length with no-op records
-  const uint32_t len = slice_len[tile];
-  for (uint32_t k = f.nf; k < len; ++k) out[(size_t)k * kSlice] = make_uint2(kGapVox, 0u);
+  // records past f.nf are never read: no padding
   front[pix] = f;
   lanes[pix] = r;
 }
@@ -605,7 +694,7 @@ __global__ __launch_bounds__(256) void k_eval(CamGeom c, GridGeom g, const float
     v.first = true;
     v.sdone = false;
     RaySetup rs;
-    if (setup_ray(c, g, x, y, rs)) walk<kVisit>(g, rs, 0.0, nullptr, nullptr, slot, nullptr, nullptr, &v);
+    if (setup_ray(c, g, x, y, rs)) walk<kVisit>(g, rs, 0.0, nullptr, nullptr, slot, nullptr, 0u, &v);
     ed = v.esum / (1.0f - v.prev_vis + kFEps);
     if (v.valid) {
       ll = logf(v.sum + v.prev_vis * (d >= kZMax ? 1.0f - kLikeEps : kLikeEps));
@@ -654,10 +743,9 @@ void launch_eval(const CamGeom& c, const GridGeom& g, const float* depth, const 
 }
 
 void launch_mark_rays(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                      const float* bia, uint8_t* bflags, uint32_t* counts, uint32_t* has, hipStream_t s) {
+                      const float* bia, uint8_t* bflags, uint32_t* bound, hipStream_t s) {
   const int n = tiled_pixels(c);
-  hipLaunchKernelGGL(k_mark_rays, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, bflags, counts,
-                     has);
+  hipLaunchKernelGGL(k_mark_rays, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, bflags, bound);
 }
 
 void launch_feat_hits(const CamGeom& c, const GridGeom& g, const float* depth, const int32_t* slot, uint32_t* hits,
@@ -673,10 +761,9 @@ void launch_mark_endpoints(const CamGeom& c, const GridGeom& g, const float* dep
 }
 
 void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig,
-                  const float* bia, const int32_t* slot, uint32_t* counts, uint32_t* has, hipStream_t s) {
+                  const float* bia, const int32_t* slot, uint32_t* counts, hipStream_t s) {
   const int n = tiled_pixels(c);
-  hipLaunchKernelGGL(k_count, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, counts,
-                     has);
+  hipLaunchKernelGGL(k_count, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, counts);
 }
 
 void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_len, uint32_t* slice_rec,
@@ -687,10 +774,10 @@ void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_l
 
 void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
                  const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
-                 FrontRec* front, RayRec* lanes, hipStream_t s) {
+                 FrontRec* front, RayRec* lanes, uint32_t* overflow, hipStream_t s) {
   const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_emit, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, slice_off,
-                     slice_len, sell, front, lanes);
+                     slice_len, sell, front, lanes, overflow);
 }
 
 }  // namespace mrf
