@@ -133,7 +133,7 @@ struct mrf_ctx {
   size_t cap_scan = 0;
   void* d_fscan_tmp = nullptr;  // flag-scan scratch
   size_t fscan_bytes = 0;
-  uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2..5] counters, [6] flag
+  uint64_t* d_total = nullptr;  // [0] scan total, [1] residual bits, [2..9] counters, flags and totals
   uint32_t* d_all_pairs = nullptr;  // [0] CamPass::all_pairs, [1] CamPass::msg_all
 
   std::vector<Camera> cams;
@@ -385,11 +385,12 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
 }
 
 // Exclusive scan of 0/1 flags; the total lands in d_total[0].
-int flag_scan(mrf_ctx* c, const uint32_t* in, uint32_t* out, size_t n) {
+int flag_scan(mrf_ctx* c, const uint32_t* in, uint32_t* out, size_t n, uint64_t* total = nullptr) {
+  if (!total) total = c->d_total;
   if (n >= 0x7FFFFFFFull) {  // beyond the single-pass scan's item count: the general scan
     int rc = ensure_scan(c, n);
     if (rc) return rc;
-    launch_exclusive_scan(in, out, n, c->d_scan_tmp, c->d_total, c->stream);
+    launch_exclusive_scan(in, out, n, c->d_scan_tmp, total, c->stream);
     return MRF_OK;
   }
   const size_t tb = flag_scan_temp_bytes(n);
@@ -403,7 +404,7 @@ int flag_scan(mrf_ctx* c, const uint32_t* in, uint32_t* out, size_t n) {
     }
     c->fscan_bytes = std::max<size_t>(tb, 256);
   }
-  HIPCHK(c, launch_flag_scan(in, out, n, c->d_fscan_tmp, c->fscan_bytes, c->d_total, c->stream));
+  HIPCHK(c, launch_flag_scan(in, out, n, c->d_fscan_tmp, c->fscan_bytes, total, c->stream));
   return MRF_OK;
 }
 
@@ -527,13 +528,23 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   //    d_counts -> offsets d_has) of the final record buffer
   launch_split(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_sig, c->d_bia, c->p.lookup_n, c->d_counts,
                c->d_rayidx, s);
-  if ((rc = dalloc(c, &cam.d_len_f, ntiles)) || (rc = dalloc(c, &cam.d_len_w, ntiles))) return rc;
+  if ((rc = dalloc(c, &cam.d_len_f, ntiles)) || (rc = dalloc(c, &cam.d_len_w, ntiles)) ||
+      (rc = dalloc(c, &cam.d_chunk_off, (size_t)ntiles + 1)))
+    return rc;
+  // ... and the front chunks, ceil(len_f / kChunk) per slice that has window
+  // records (sizes d_counts -> chunk_off); the three totals in one readback
+  uint64_t* tot3 = c->d_total + 2;
   launch_slice_len(c->d_counts, ntiles, cam.d_len_f, c->d_has, s);
-  launch_exclusive_scan(c->d_has, c->d_offs, ntiles, c->d_scan_tmp, c->d_total, s);
-  const uint64_t nrec_f = read_total(c);
+  launch_exclusive_scan(c->d_has, c->d_offs, ntiles, c->d_scan_tmp, tot3, s);
   launch_slice_len(c->d_rayidx, ntiles, cam.d_len_w, c->d_counts, s);
-  launch_exclusive_scan(c->d_counts, c->d_has, ntiles, c->d_scan_tmp, c->d_total, s);
-  const uint64_t nrec_w = read_total(c);
+  launch_exclusive_scan(c->d_counts, c->d_has, ntiles, c->d_scan_tmp, tot3 + 1, s);
+  launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
+  launch_exclusive_scan(c->d_counts, cam.d_chunk_off, (size_t)ntiles + 1, c->d_scan_tmp, tot3 + 2, s);
+  uint64_t t3[3] = {0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(t3, tot3, sizeof(t3), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  const uint64_t nrec_f = t3[0], nrec_w = t3[1];
+  cam.nchunks = (uint32_t)t3[2];
   const uint64_t nrec = nrec_f + nrec_w;
   HIPCHK(c, hipGetLastError());
   if (nrec >= 0x7FFFFFF0ull) return fail(c, MRF_E_LIMIT, "more than 2^31 incidence records in one camera");
@@ -553,10 +564,6 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 32, s));
   launch_reslice(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
                  (uint32_t)nrec_f, cam.d_inc, any, wany, (unsigned long long*)(c->d_total + 2), s);
-  // live records, live window records, records (gap records included), rays
-  // with records; the emission's overflow flag
-  uint64_t counts2[5] = {0, 0, 0, 0, 0};
-  HIPCHK(c, hipMemcpyAsync(counts2, c->d_total + 2, 40, hipMemcpyDeviceToHost, s));
   if (c->p.brick_local_vis == MRF_VIS_BRICK_LOCAL) {
     // brick entries of the window records, while the records hold voxel ids;
     // rows past the last slice read as zero words (the window kernel's
@@ -567,11 +574,6 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     launch_brick_starts(cam.d_front, cam.d_lanes, cam.d_len_w, ntiles, cam.d_inc, (uint32_t)nrec_f, cam.d_bstart,
                         cam.d_tail, s);
   }
-  // front chunks: ceil(len_f / kChunk) per slice that has window records
-  if ((rc = dalloc(c, &cam.d_chunk_off, (size_t)ntiles + 1))) return rc;
-  launch_chunk_counts(cam.d_len_f, cam.d_len_w, ntiles, c->d_counts, s);
-  launch_exclusive_scan(c->d_counts, cam.d_chunk_off, (size_t)ntiles + 1, c->d_scan_tmp, c->d_total, s);
-  cam.nchunks = (uint32_t)read_total(c);
   // one allocation for the partial sums, the static sums and the masks of
   // changing records (per chunk and lane each)
   const size_t nparts = (size_t)cam.nchunks * kSlice;
@@ -582,20 +584,26 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.d_fdmask = reinterpret_cast<uint32_t*>(cam.d_partial + 2 * nparts);
   launch_chunk_desc(cam.d_chunk_off, cam.d_len_f, ntiles, cam.d_chunk_desc, s);
   launch_front_span(cam.d_front, cam.d_lanes, ntiles * kSlice, cam.d_fspan, s);
-  // 4. (camera, voxel) pairs: window pairs first, each group in voxel order
-  if ((rc = flag_scan(c, any, pid, nvox))) return rc;
-  const uint64_t np64 = read_total(c);  // synchronises: counts2 is in
-  if (counts2[4]) return fail(c, MRF_E_LIMIT, "traversal: a ray has more records than its bound");
+  // 4. (camera, voxel) pairs: window pairs first, each group in voxel order.
+  //    One readback: [0..3] the reslice counters (live records, live window
+  //    records, records with gap records, rays with records), [4] the
+  //    emission's overflow flag, [6] / [7] the pairs / window pairs
+  if ((rc = flag_scan(c, any, pid, nvox, c->d_total + 8)) || (rc = flag_scan(c, wany, rank_w, nvox, c->d_total + 9)))
+    return rc;
+  uint64_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(t8, c->d_total + 2, sizeof(t8), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (t8[4]) return fail(c, MRF_E_LIMIT, "traversal: a ray has more records than its bound");
+  const uint64_t np64 = t8[6];
   // the ray kernels address a camera's per-pair messages with 32-bit byte offsets
   if (np64 >= (1ull << 30)) return fail(c, MRF_E_LIMIT, "more than 2^30 (camera, voxel) pairs in one camera");
   const uint32_t np = (uint32_t)np64;
-  cam.nupd = counts2[0];
-  const uint64_t nwin = counts2[1];
+  cam.nupd = t8[0];
+  const uint64_t nwin = t8[1];
   cam.nwin = nwin;
-  cam.nlist = counts2[2];
-  cam.nrays = (uint32_t)counts2[3];
-  if ((rc = flag_scan(c, wany, rank_w, nvox))) return rc;
-  const uint32_t nwp = (uint32_t)read_total(c);
+  cam.nlist = t8[2];
+  cam.nrays = (uint32_t)t8[3];
+  const uint32_t nwp = (uint32_t)t8[7];
   cam.npairs = np;
   cam.nwp = nwp;
   if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
@@ -689,33 +697,31 @@ int prepare(mrf_ctx* c) {
     for (size_t ci = 0; ci < c->cams.size(); ++ci) {
       Camera& cam = c->cams[ci];
       const uint32_t nrest = cam.npairs - cam.nwp;
-      uint64_t ndyn = 0;
-      if (nrest) {
-        launch_dyn_flags(cam.d_pair_vox + cam.nwp, nrest, W, flags, c->stream);
-        if ((rc = flag_scan(c, flags, pos, nrest))) return rc;
-        ndyn = read_total(c);
-      }
+      // the front-only pairs at voxels of W, as a list (sized by its bound);
+      // the front chunks whose log2 n keep changing: per chunk and lane the
+      // mask of its changing records, per chunk the rows of its compact block
+      // (crow) and a flag (cflags, scanned into pos); one readback of the
+      // kept records [2], the list's length [8] and the changing chunks [9]
       dfree(&cam.d_dyn);
-      if ((rc = dalloc(c, &cam.d_dyn, std::max<uint64_t>(ndyn, 1)))) return rc;
-      cam.ndyn = (uint32_t)ndyn;
-      launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
-      // the front chunks whose log2 n keep changing, as a list; per chunk and
-      // lane the mask of its changing records, per chunk the rows of its
-      // compact block (into pos, free until the chunk scan below)
+      if ((rc = dalloc(c, &cam.d_dyn, std::max<uint32_t>(nrest, 1)))) return rc;
       if ((rc = ensure_sort(c, std::max<size_t>(cam.nchunks, 1), 1))) return rc;
       uint32_t* crow = c->d_x0;  // sort scratch of at least nchunks words: rows per chunk
       launch_zero_words(c->d_total + 2, 8, c->stream);
+      launch_zero_words(c->d_total + 8, 16, c->stream);
+      if (nrest) {
+        launch_dyn_flags(cam.d_pair_vox + cam.nwp, nrest, W, flags, c->stream);
+        if ((rc = flag_scan(c, flags, pos, nrest, c->d_total + 8))) return rc;
+        launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
+      }
       launch_chunk_static(cam_pass(c, ci), flags, cflags, cam.d_fdmask, crow, (unsigned long long*)(c->d_total + 2),
                           c->stream);
-      uint64_t n = 0;
-      HIPCHK(c, hipMemcpyAsync(&n, c->d_total + 2, 8, hipMemcpyDeviceToHost, c->stream));
+      if (cam.nchunks && (rc = flag_scan(c, cflags, pos, cam.nchunks, c->d_total + 9))) return rc;
+      uint64_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      HIPCHK(c, hipMemcpyAsync(t8, c->d_total + 2, sizeof(t8), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      cam.nstatic = n;
-      uint64_t ndc = 0;
-      if (cam.nchunks) {
-        if ((rc = flag_scan(c, cflags, pos, cam.nchunks))) return rc;
-        ndc = read_total(c);
-      }
+      cam.nstatic = t8[0];
+      cam.ndyn = (uint32_t)t8[6];
+      const uint64_t ndc = t8[7];
       dfree(&cam.d_dyn_desc);
       if ((rc = dalloc(c, &cam.d_dyn_desc, std::max<uint64_t>(ndc, 1)))) return rc;
       cam.ndyn_chunks = (uint32_t)ndc;
@@ -1231,7 +1237,7 @@ int mrf_create(const mrf_params* p, mrf_ctx** out) {
   }
   int rc;
   if ((rc = dalloc(c, &c->d_slot, (size_t)c->nbricks)) || (rc = dalloc(c, &c->d_bflags, (size_t)c->nbricks)) ||
-      (rc = dalloc(c, &c->d_total, 8)) || (rc = dalloc(c, &c->d_all_pairs, 2))) {
+      (rc = dalloc(c, &c->d_total, 16)) || (rc = dalloc(c, &c->d_all_pairs, 2))) {
     mrf_destroy(c);
     return rc;
   }

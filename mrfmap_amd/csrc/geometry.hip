@@ -87,6 +87,29 @@ __device__ __forceinline__ void bitonic_sort(int lane, uint32_t& key, uint32_t& 
   bitonic_merge<K, K / 2>(lane, key, val, val2);
 }
 
+// Sort of one 32-bit value per lane over the 64 lanes, ascending; every
+// compare-exchange keeps the minimum in the lower lane.  Merging two sorted
+// halves of K lanes starts with the mirror partner l ^ (K - 1), then the
+// half-cleaners l ^ J, J = K/4 .. 1.
+template <int J>
+__device__ __forceinline__ void cx_min_low(int lane, int low_bit, uint32_t& v) {
+  const uint32_t o = xor_lane<J>(v);
+  v = (lane & low_bit) == 0 ? min(v, o) : max(v, o);
+}
+
+template <int K, int J>
+__device__ __forceinline__ void half_clean(int lane, uint32_t& v) {
+  cx_min_low<J>(lane, J, v);
+  if constexpr (J > 1) half_clean<K, J / 2>(lane, v);
+}
+
+template <int K>
+__device__ __forceinline__ void sort_u32(int lane, uint32_t& v) {
+  if constexpr (K > 2) sort_u32<K / 2>(lane, v);
+  cx_min_low<K - 1>(lane, K / 2, v);
+  if constexpr (K > 2) half_clean<K, K / 4>(lane, v);
+}
+
 template <int CTRL, int ROWS>
 __device__ __forceinline__ uint32_t dpp_u(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWS, 0xF, true);
@@ -124,6 +147,10 @@ __device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t& x2, uint32
 // lengths go to the front sum acc_f) or all window (acc_w); C is their sum
 // plus the parts below 2^-24 (acc_r), so a front run costs one atomic for
 // its 2^-24 units, not one for C and one for the front sum.
+// PACKED (fewer than 2^26 - 1 pairs): the sort moves one word per lane, the
+// pair and the lane it came from (pair 2^6 + lane, unique), and each lane then
+// fetches its lengths from that lane; else (pair, q, r) move together.
+template <bool PACKED>
 __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
                                                       const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_w,
                                                       uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r) {
@@ -151,7 +178,16 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
       q = (uint32_t)x;
       r = (uint32_t)((x - (double)q) * kFixLen);
     }
-    bitonic_sort<64>(lane, key, q, r);
+    if constexpr (PACKED) {
+      uint32_t v = key == 0xFFFFFFFFu ? 0xFFFFFFFFu : (key << 6) | (uint32_t)lane;
+      sort_u32<64>(lane, v);
+      const int src = (int)(v & 63u);
+      key = v == 0xFFFFFFFFu ? 0xFFFFFFFFu : v >> 6;
+      q = (uint32_t)__shfl((int)q, src, 64);
+      r = (uint32_t)__shfl((int)r, src, 64);
+    } else {
+      bitonic_sort<64>(lane, key, q, r);
+    }
     const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
     const uint32_t next = (uint32_t)__shfl_down((int)key, 1, 64);
     wave_seg_sum_u(q, r, lane == 0 || prev != key ? 1u : 0u);
@@ -264,8 +300,12 @@ void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint3
   launch_zero_words(acc_w, (size_t)np * 8, s);
   launch_zero_words(acc_f, (size_t)np * 8, s);
   launch_zero_words(acc_r, (size_t)np * 8, s);
-  hipLaunchKernelGGL(k_pair_lengths, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w, acc_f,
-                     acc_r);
+  if (np < (1u << 26) - 1u)
+    hipLaunchKernelGGL(k_pair_lengths<true>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w,
+                       acc_f, acc_r);
+  else
+    hipLaunchKernelGGL(k_pair_lengths<false>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w,
+                       acc_f, acc_r);
   hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_w, acc_f, acc_r, np, pair_c,
                      pair_sf);
 }
