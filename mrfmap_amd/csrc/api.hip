@@ -521,13 +521,12 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     return rc;
   uint32_t* overflow = reinterpret_cast<uint32_t*>(c->d_total + 6);
   launch_zero_words(overflow, 8, s);
+  // 2. the emission splits each ray into front and window (counts into
+  //    d_counts / d_rayidx); then the front region (sizes d_has -> offsets
+  //    d_offs) and the window region (sizes d_counts -> offsets d_has) of the
+  //    final record buffer
   launch_emit(cg, c->g, cam.d_depth, c->d_sig, c->d_bia, c->d_slot, c->d_offs, c->d_has, c->d_sell0, cam.d_front,
-              cam.d_lanes, overflow, s);
-  // 2. front/window split (counts into d_counts / d_rayidx), then the front
-  //    region (sizes d_has -> offsets d_offs) and the window region (sizes
-  //    d_counts -> offsets d_has) of the final record buffer
-  launch_split(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_sig, c->d_bia, c->p.lookup_n, c->d_counts,
-               c->d_rayidx, s);
+              cam.d_lanes, c->d_counts, c->d_rayidx, overflow, s);
   if ((rc = dalloc(c, &cam.d_len_f, ntiles)) || (rc = dalloc(c, &cam.d_len_w, ntiles)) ||
       (rc = dalloc(c, &cam.d_chunk_off, (size_t)ntiles + 1)))
     return rc;

@@ -80,10 +80,6 @@ __device__ __forceinline__ float lane63(float x) {
 // (get_depth_sigma / get_depth_bias, gvdbkernels.cu:46-55).
 // A linear cell has c2 = 0, for which fmaf(z, fmaf(z, 0, c1), c0) is
 // fmaf(z, c1, c0) exactly.
-struct Poly {
-  float c0, c1, c2;
-  __device__ __forceinline__ float operator()(float z) const { return fmaf(z, fmaf(z, c2, c1), c0); }
-};
 
 // 0.398942 e^{-u^2/2}, the exponential as a power of 2 with the constants
 // folded: e^{-u^2/2} = 2^{u^2 (-log2(e) / 2)}
@@ -98,19 +94,6 @@ __device__ __forceinline__ float depth_psi(float s, float d, const Poly& sig3, c
   const float inv = __builtin_amdgcn_rcpf(sig3(s));
   return gauss_a(diff * inv) * inv;
 }
-
-// Kahan-compensated running sum of the per-chunk totals along a ray.
-struct Kahan {
-  float s, c;
-  __device__ __forceinline__ void add(float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  // carry + in-chunk prefix, corrected by the compensation
-  __device__ __forceinline__ float at(float prefix) const { return s + (prefix - c); }
-};
 
 // ---------------------------------------------------------------- segments
 // A camera's incidences are stored pair-major: pair q owns positions
@@ -166,69 +149,6 @@ __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ p
 }
 
 // ---------------------------------------------------------------- ray kernel
-// Per-lane sensor model of a ray: lookup cell coefficients (get_depth_sigma /
-// get_depth_bias, gvdbkernels.cu:46-55).
-__device__ __forceinline__ void load_cell(const float* __restrict__ sigt, const float* __restrict__ biat, int lookup_n,
-                                          uint32_t cell, Poly& sig, Poly& bia) {
-  sig.c0 = sigt[cell];
-  sig.c1 = sigt[cell + 1];
-  sig.c2 = lookup_n == 3 ? sigt[cell + 2] : 0.0f;
-  bia.c0 = biat[cell];
-  bia.c1 = biat[cell + 1];
-  bia.c2 = lookup_n == 3 ? biat[cell + 2] : 0.0f;
-}
-
-// Front/window split of every ray of a camera (one thread per ray), with the
-// ray kernel's own FP32 depth sums: the front ends at the first record whose
-// exit depth s has d - (s + bias(s)) <= kFrontU * 3 sigma(s).  The emitted
-// ray holds all its records in the front part.  The Kahan state of the
-// front's length sum is kept: the window continues it.
-__global__ __launch_bounds__(256) void k_split(FrontRec* __restrict__ front, RayRec* __restrict__ lanes, uint32_t npix,
-                                               const uint2* __restrict__ sell, const float* __restrict__ sigt,
-                                               const float* __restrict__ biat, int lookup_n,
-                                               uint32_t* __restrict__ cnt_f, uint32_t* __restrict__ cnt_w) {
-  const uint32_t ray = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ray >= npix) return;
-  FrontRec f = front[ray];
-  RayRec r = lanes[ray];
-  const uint32_t n = f.nf;
-  uint32_t nf = n;
-  Kahan kS = {0.f, 0.f};
-  if (n) {
-    Poly sig, bia;
-    load_cell(sigt, biat, lookup_n, r.cell_flags & kCellMask, sig, bia);
-    const uint2* __restrict__ rl = sell + f.off;
-    // lengths in batches of 8 (their loads in flight together)
-    bool open = true;
-    for (uint32_t k0 = 0; open && k0 < n; k0 += 8) {
-      float lb[8];
-#pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) lb[u] = k0 + u < n ? __uint_as_float(rl[(size_t)(k0 + u) * kSlice].y) : 0.0f;
-#pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) {
-        if (!open || k0 + u >= n) break;
-        const float ln = lb[u];
-        const float s = r.z0 + kS.at(ln) * r.inv_norm;
-        const float sg = sig(s);
-        if (!(sg > 0.0f && r.depth - (s + bia(s)) > kFrontU * (3.0f * sg))) {
-          nf = k0 + u;
-          open = false;
-          break;
-        }
-        kS.add(ln);
-      }
-    }
-  }
-  f.nf = nf;
-  r.nw = n - nf;
-  r.ks = kS.s;
-  r.kc = kS.c;
-  front[ray] = f;
-  lanes[ray] = r;
-  cnt_f[ray] = nf;
-  cnt_w[ray] = n - nf;
-}
-
 // Copy every ray's records into the front and window regions of the final
 // record buffer (sliced: lane l of slice t at region offset + k * 64 + l),
 // padding each lane to its slice's length with gap records.  One wavefront
@@ -1378,13 +1298,6 @@ void launch_epilogue(const CamPass& a, hipStream_t s, int part) {
   // the segments that cross tiles are joined by k_seg_update
   hipLaunchKernelGGL(k_seg_update, dim3(pair_grid(a)), dim3(256), 0, s, a.pair_vox, a.pair_c, a.pair_sf, a.nwp,
                      a.npairs, a.wseg_off, a.S, a.pl_v, a.pf_v, a.mid, out_scale, a.B, a.M, a.upd, a.all_pairs);
-}
-
-void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
-                  int lookup_n, uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s) {
-  if (npix == 0) return;
-  hipLaunchKernelGGL(k_split, dim3(grid_for(npix, 256)), dim3(256), 0, s, front, lanes, npix, sell, sig, bia, lookup_n,
-                     cnt_f, cnt_w);
 }
 
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,

@@ -34,12 +34,6 @@ namespace {
 
 constexpr double kFixLen = 16777216.0;  // 2^24: chord lengths (< 2) in fixed point, a 64-record sum < 2^31
 
-// exact add of q to the 64-bit value {lo = p[0], hi = p[1]}
-__device__ __forceinline__ void add_split(uint32_t* p, uint32_t q) {
-  const uint32_t old = atomicAdd(&p[0], q);
-  if (old + q < old) atomicAdd(&p[1], 1u);
-}
-
 // pid[v] (written over rank_all) and pair_vox[pid] of every touched voxel
 __global__ void k_pair_table(const uint32_t* __restrict__ any, const uint32_t* __restrict__ wany,
                              uint32_t* __restrict__ rank_all, const uint32_t* __restrict__ rank_w, uint64_t nvox,
@@ -154,51 +148,73 @@ template <bool PACKED>
 __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
                                                       const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_w,
                                                       uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r) {
+  // batches of kR rows per wave (nrec is a multiple of 64): the batch's
+  // records and then their pairs are loaded together, with no branches (a
+  // row past the end reads row 0 as gap records); the rows are sorted and
+  // summed; then the runs' exact adds {low word += q, high word += 1 if it
+  // wrapped} are issued, and their carries taken once the old values of all
+  // the batch's adds are back
+  constexpr int kR = 8;
   const int lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t stride = (((uint64_t)gridDim.x * blockDim.x) >> 6) * 64;
-  // software pipeline over the wave's rows (nrec is a multiple of 64): the
-  // record of the row after next and the pair of the next row are loaded
-  // while a row is sorted and summed
-  auto load = [&](uint64_t b) { return b < nrec ? rec[b + lane] : make_uint2(kGapVox, 0u); };
-  auto pair_of = [&](uint2 e) { return e.x != kGapVox ? pid[e.x] : 0xFFFFFFFFu; };
-  uint2 e = load(wave * 64), e1 = load(wave * 64 + stride);
-  uint32_t pk = pair_of(e);
-  for (uint64_t base = wave * 64; base < nrec; base += stride) {
-    const uint64_t i = base + lane;
-    const uint2 e2 = load(base + 2 * stride);
-    const uint32_t pk1 = pair_of(e1);
-    uint32_t key = 0xFFFFFFFFu, q = 0, r = 0;
-    if (e.x != kGapVox) {
-      key = pk;
-      rec[i].x = key;
-      // l 2^24 is exact in double; q its integer part (< 2^25), r the rest
-      // in units of 2^-48 (< 2^24)
-      const double x = (double)__uint_as_float(e.y) * kFixLen;
-      q = (uint32_t)x;
-      r = (uint32_t)((x - (double)q) * kFixLen);
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t nrows = nrec / 64;
+  for (uint64_t row0 = wave * kR; row0 < nrows; row0 += nwaves * kR) {
+    uint2 e[kR];
+    uint32_t key[kR], q[kR], r[kR];
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+      const bool in = row0 + j < nrows;
+      e[j] = rec[(in ? row0 + j : 0) * 64 + lane];
+      if (!in) e[j].x = kGapVox;
     }
-    if constexpr (PACKED) {
-      uint32_t v = key == 0xFFFFFFFFu ? 0xFFFFFFFFu : (key << 6) | (uint32_t)lane;
-      sort_u32<64>(lane, v);
-      const int src = (int)(v & 63u);
-      key = v == 0xFFFFFFFFu ? 0xFFFFFFFFu : v >> 6;
-      q = (uint32_t)__shfl((int)q, src, 64);
-      r = (uint32_t)__shfl((int)r, src, 64);
-    } else {
-      bitonic_sort<64>(lane, key, q, r);
+#pragma unroll
+    for (int j = 0; j < kR; ++j) key[j] = pid[e[j].x != kGapVox ? e[j].x : 0u];
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+      q[j] = r[j] = 0;
+      if (e[j].x != kGapVox) {
+        rec[(row0 + j) * 64 + lane].x = key[j];
+        // l 2^24 is exact in double; q its integer part (< 2^25), r the rest
+        // in units of 2^-48 (< 2^24)
+        const double x = (double)__uint_as_float(e[j].y) * kFixLen;
+        q[j] = (uint32_t)x;
+        r[j] = (uint32_t)((x - (double)q[j]) * kFixLen);
+      } else {
+        key[j] = 0xFFFFFFFFu;
+      }
+      if constexpr (PACKED) {
+        uint32_t v = key[j] == 0xFFFFFFFFu ? 0xFFFFFFFFu : (key[j] << 6) | (uint32_t)lane;
+        sort_u32<64>(lane, v);
+        const int src = (int)(v & 63u);
+        key[j] = v == 0xFFFFFFFFu ? 0xFFFFFFFFu : v >> 6;
+        q[j] = (uint32_t)__shfl((int)q[j], src, 64);
+        r[j] = (uint32_t)__shfl((int)r[j], src, 64);
+      } else {
+        bitonic_sort<64>(lane, key[j], q[j], r[j]);
+      }
+      const uint32_t prev = (uint32_t)__shfl_up((int)key[j], 1, 64);
+      const uint32_t next = (uint32_t)__shfl_down((int)key[j], 1, 64);
+      wave_seg_sum_u(q[j], r[j], lane == 0 || prev != key[j] ? 1u : 0u);
+      // one add per run, by its last lane
+      if (!(lane == 63 || next != key[j])) key[j] = 0xFFFFFFFFu;
     }
-    const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
-    const uint32_t next = (uint32_t)__shfl_down((int)key, 1, 64);
-    wave_seg_sum_u(q, r, lane == 0 || prev != key ? 1u : 0u);
-    const bool tail = lane == 63 || next != key;
-    if (tail && key != 0xFFFFFFFFu) {
-      add_split((base < base_w ? acc_f : acc_w) + 2 * (size_t)key, q);  // uniform: the row's region
-      if (r) add_split(acc_r + 2 * (size_t)key, r);
+    uint32_t po[kR], por[kR];
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+      po[j] = por[j] = 0;
+      if (key[j] != 0xFFFFFFFFu) {
+        uint32_t* acc = ((row0 + j) * 64 < base_w ? acc_f : acc_w) + 2 * (size_t)key[j];  // the row's region
+        po[j] = atomicAdd(acc, q[j]);
+        if (r[j]) por[j] = atomicAdd(acc_r + 2 * (size_t)key[j], r[j]);
+      }
     }
-    e = e1;
-    e1 = e2;
-    pk = pk1;
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+      if (key[j] == 0xFFFFFFFFu) continue;
+      if (po[j] + q[j] < po[j]) atomicAdd(((row0 + j) * 64 < base_w ? acc_f : acc_w) + 2 * (size_t)key[j] + 1, 1u);
+      if (r[j] && por[j] + r[j] < por[j]) atomicAdd(acc_r + 2 * (size_t)key[j] + 1, 1u);
+    }
   }
 }
 

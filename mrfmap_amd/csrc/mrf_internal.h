@@ -7,6 +7,38 @@
 
 namespace mrf {
 
+// Sensor-model polynomial in FP32 (sigma(z) or bias(z), c2 = 0 for two
+// coefficients).
+struct Poly {
+  float c0, c1, c2;
+  __device__ __forceinline__ float operator()(float z) const { return fmaf(z, fmaf(z, c2, c1), c0); }
+};
+
+// Kahan-compensated running sum along a ray.
+struct Kahan {
+  float s, c;
+  __device__ __forceinline__ void add(float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  // carry + in-chunk prefix, corrected by the compensation
+  __device__ __forceinline__ float at(float prefix) const { return s + (prefix - c); }
+};
+
+// Per-lane sensor model of a ray: lookup cell coefficients (get_depth_sigma /
+// get_depth_bias, gvdbkernels.cu:46-55).
+__device__ __forceinline__ void load_cell(const float* __restrict__ sigt, const float* __restrict__ biat, int lookup_n,
+                                          uint32_t cell, Poly& sig, Poly& bia) {
+  sig.c0 = sigt[cell];
+  sig.c1 = sigt[cell + 1];
+  sig.c2 = lookup_n == 3 ? sigt[cell + 2] : 0.0f;
+  bia.c0 = biat[cell];
+  bia.c1 = biat[cell + 1];
+  bia.c2 = lookup_n == 3 ? biat[cell + 2] : 0.0f;
+}
+
 // Constants of the live reference path (include/mrfmap/GVDBCommon.cuh:3-11).
 constexpr float kFEps = 1e-4f;         // F_EPS: probability clamp
 constexpr float kZMax = 10.0f;         // Z_MAX

@@ -34,12 +34,16 @@ void launch_count(const CamGeom& c, const GridGeom& g, const float* depth, const
 void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_len, uint32_t* slice_rec,
                       hipStream_t s);
 // sliced-ELL emission: record k of lane l of slice t at slice_off[t] + k*64 + l
-// for k < front[t*64+l].nf (the ray's records, nothing past them is written);
-// lanes[t*64+l] describes the ray.  A ray with more records than its slice's
-// length slice_len[t] (sized by a count or a bound of it) sets *overflow.
+// for k below the ray's record count (nothing past it is written), split
+// into front and window: front[t*64+l].nf front records, then
+// lanes[t*64+l].nw window records (the ray's description, the Kahan state of
+// the front's length sum); cnt_f / cnt_w the same per tiled pixel.  A ray
+// with more records than its slice's length slice_len[t] (sized by a count
+// or a bound of it) sets *overflow.
 void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
                  const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
-                 FrontRec* front, RayRec* lanes, uint32_t* overflow, hipStream_t s);
+                 FrontRec* front, RayRec* lanes, uint32_t* cnt_f, uint32_t* cnt_w, uint32_t* overflow,
+                 hipStream_t s);
 
 // map evaluation of one camera's pixels (log-likelihood, expected depth,
 // accuracy and simple accuracy images in row-major pixel order, any may be
@@ -154,10 +158,6 @@ struct CamPass {
   const uint64_t* bstart;
   const uint8_t* tail;
 };
-// front/window split of an emitted camera (all records of a ray in its
-// front part): sets nf, nw, the front length sums and per-pixel counts
-void launch_split(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const float* sig, const float* bia,
-                  int lookup_n, uint32_t* cnt_f, uint32_t* cnt_w, hipStream_t s);
 // copy into the final front/window regions; window region starts at base_w.
 // any / wany: pool-sized flags of voxels touched by a live / window record
 // (set, never cleared); counts[0] / counts[1] += the live / live window

@@ -156,9 +156,34 @@ __device__ __forceinline__ bool ray_entry(const GridGeom& g, const RaySetup& rs,
   return true;
 }
 
+// Front/window split of one ray during its emission, with the ray kernel's
+// own FP32 depth sums: the front ends at the first record whose exit depth s
+// has d - (s + bias(s)) <= kFrontU * 3 sigma(s).  The Kahan state of the
+// front's length sum is kept: the window continues it.
+struct FrontSplit {
+  Poly sig, bia;
+  float z0, inv_norm, depth;
+  Kahan kS;
+  uint32_t nf;  // the first window record (valid once open is false)
+  bool open;
+  __device__ __forceinline__ void record(uint32_t k, float ln) {
+    if (!open) return;
+    const float s = z0 + kS.at(ln) * inv_norm;
+    const float sg = sig(s);
+    if (!(sg > 0.0f && depth - (s + bia(s)) > kFrontU * (3.0f * sg))) {
+      nf = k;
+      open = false;
+      return;
+    }
+    kS.add(ln);
+  }
+};
+
 // Records of one ray are written out[0], out[kSlice], out[2 kSlice], ...
 // (record k of the 64 rays of a pixel tile is contiguous: sliced ELL), the
-// first cap of them (the count goes on past cap).
+// first cap of them (the count goes on past cap); vis, when given, splits
+// them into front and window (vis->record(k, chord) per record in order,
+// vis->z0 the planar depth at the entry of the first).
 // kVisit walks to the end of the grid without truncation and hands every
 // voxel of an active brick to vis->voxel(id, ix, chord, z entry, z exit) and
 // every active brick's exit depth to vis->brick_end(z); a true return from
@@ -220,15 +245,19 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
     const bool active = bslot >= 0;
     if (active) {
       if (any && skipped) {
-        if (MODE == kEmit) {
+        if constexpr (MODE == kEmit) {
           float gl = (float)((zc_d - last_exit) * rs.norm);
           if (cnt < cap) out[(size_t)cnt * kSlice] = make_uint2(kGapVox, __float_as_uint(gl));
+          if (vis) vis->record(cnt, gl);
         }
         ++cnt;
         skipped = false;
       }
       if (!any) {
         wo.z0 = (float)zc_d;
+        if constexpr (MODE == kEmit) {
+          if (vis) vis->z0 = wo.z0;
+        }
         any = true;
       }
       if constexpr (MODE == kEmit || MODE == kVisit) {
@@ -237,6 +266,7 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
         float len = (float)((zn_d - zc_d) * rs.norm);
         if constexpr (MODE == kEmit) {
           if (cnt < cap) out[(size_t)cnt * kSlice] = make_uint2(vox, __float_as_uint(len));
+          if (vis) vis->record(cnt, len);
         } else {
           if (vis->voxel(vox, ix, len, zc_d, zn_d)) break;
         }
@@ -502,18 +532,21 @@ __global__ void k_slice_len(const uint32_t* __restrict__ counts, uint32_t ntiles
   }
 }
 
+// Emission of every ray's records into the staging lists, split into front
+// and window as they go: front[pix].nf front records, then lanes[pix].nw
+// window records; cnt_f / cnt_w per tiled pixel.
 __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, const float* __restrict__ sig,
                        const float* __restrict__ bia, const int32_t* __restrict__ slot,
                        const uint32_t* __restrict__ slice_off, const uint32_t* __restrict__ slice_len,
                        uint2* __restrict__ sell, FrontRec* __restrict__ front, RayRec* __restrict__ lanes,
-                       uint32_t* __restrict__ overflow) {
+                       uint32_t* __restrict__ cnt_f, uint32_t* __restrict__ cnt_w, uint32_t* __restrict__ overflow) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= tiled_pixels(c)) return;
   const uint32_t tile = (uint32_t)pix >> 6, lane = (uint32_t)pix & 63u;
   uint2* out = sell + slice_off[tile] + lane;
   PixelCtx p = load_pixel(c, g, depth, pix);
   FrontRec f;
-  f.off = slice_off[tile] + lane;  // all records in the front part until the split
+  f.off = slice_off[tile] + lane;  // front and window records follow each other until the reslice
   f.nf = 0;
   RayRec r;
   r.off_w = 0;
@@ -525,15 +558,28 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
   r.depth = 0.0f;
   r.cell_flags = 0;
   if (p.ok) {
+    FrontSplit sp;
+    load_cell(sig, bia, g.lookup_n, p.cell, sp.sig, sp.bia);
+    sp.inv_norm = (float)(1.0 / p.rs.norm);
+    sp.depth = p.d;
+    sp.z0 = 0.0f;
+    sp.kS = {0.0f, 0.0f};
+    sp.nf = 0;
+    sp.open = true;
     const uint32_t cap = slice_len[tile];
-    WalkOut wo = walk<kEmit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, out, cap);
-    f.nf = wo.n;
-    if (wo.n > cap) {  // the slot's bound did not hold: an error, and the list stays in bounds
+    WalkOut wo = walk<kEmit, FrontSplit>(g, p.rs, (double)p.d, sig + p.cell, bia + p.cell, slot, out, cap, &sp);
+    uint32_t n = wo.n;
+    if (n > cap) {  // the slot's bound did not hold: an error, and the lists stay in bounds
       atomicOr(overflow, 1u);
-      f.nf = cap;
+      n = cap;
     }
+    const uint32_t nf = sp.open ? n : min(sp.nf, n);
+    f.nf = nf;
+    r.nw = n - nf;
+    r.ks = sp.kS.s;
+    r.kc = sp.kS.c;
     r.z0 = wo.z0;
-    r.inv_norm = (float)(1.0 / p.rs.norm);
+    r.inv_norm = sp.inv_norm;
     r.depth = p.d;
     // the infinity term's test d > s_n + bias(s_n) at the exact exit depth of
     // the last record, in FP64 as the traversal itself: an FP32 running sum
@@ -541,9 +587,11 @@ __global__ void k_emit(CamGeom c, GridGeom g, const float* __restrict__ depth, c
     const bool inf = wo.ran_out && (double)p.d > wo.last_exit + poly_eval(bia + p.cell, g.lookup_n, wo.last_exit);
     r.cell_flags = p.cell | (wo.ran_out ? kRanOut : 0u) | (inf ? kInfTerm : 0u);
   }
-  // records past f.nf are never read: no padding
+  // records past the ray's own are never read: no padding
   front[pix] = f;
   lanes[pix] = r;
+  cnt_f[pix] = f.nf;
+  cnt_w[pix] = r.nw;
 }
 
 // ---------------------------------------------------------------- map evaluation
@@ -774,10 +822,11 @@ void launch_slice_len(const uint32_t* counts, uint32_t ntiles, uint32_t* slice_l
 
 void launch_emit(const CamGeom& c, const GridGeom& g, const float* depth, const float* sig, const float* bia,
                  const int32_t* slot, const uint32_t* slice_off, const uint32_t* slice_len, uint2* sell,
-                 FrontRec* front, RayRec* lanes, uint32_t* overflow, hipStream_t s) {
+                 FrontRec* front, RayRec* lanes, uint32_t* cnt_f, uint32_t* cnt_w, uint32_t* overflow,
+                 hipStream_t s) {
   const int n = tiled_pixels(c);
   hipLaunchKernelGGL(k_emit, dim3((n + 127) / 128), dim3(128), 0, s, c, g, depth, sig, bia, slot, slice_off,
-                     slice_len, sell, front, lanes, overflow);
+                     slice_len, sell, front, lanes, cnt_f, cnt_w, overflow);
 }
 
 }  // namespace mrf
