@@ -108,7 +108,7 @@ struct mrf_ctx {
   float *d_pf_v = nullptr, *d_pl_v = nullptr, *d_mid = nullptr;
   size_t cap_tiles = 0;
   // segment-build scratch (per record), per-pair scratch and the emitted lists
-  uint32_t *d_keys_in = nullptr, *d_keys_out = nullptr, *d_vals_in = nullptr, *d_vals_out = nullptr;
+  uint32_t *d_keys_in = nullptr, *d_keys_out = nullptr, *d_vals_in = nullptr;  // window sort (values out: win_idx)
   uint32_t *d_x0 = nullptr, *d_x1 = nullptr;  // window head flags and their scan
   size_t cap_sort = 0;
   uint32_t* d_vflag = nullptr;  // 4 pool-sized tables: touched, window-touched and their scans
@@ -359,13 +359,13 @@ int ensure_pool(mrf_ctx* c, size_t nslots) {
 int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   if (n > c->cap_sort) {
     n += n / 4;  // headroom: the next camera is likely a little larger, and a free waits for the device
-    dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
+    dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in);
     dfree(&c->d_x0); dfree(&c->d_x1);
     c->cap_sort = 0;
     int rc;
     const size_t nx = n + 1024;
     if ((rc = dalloc(c, &c->d_keys_in, nx)) || (rc = dalloc(c, &c->d_keys_out, nx)) ||
-        (rc = dalloc(c, &c->d_vals_in, nx)) || (rc = dalloc(c, &c->d_vals_out, nx)) ||
+        (rc = dalloc(c, &c->d_vals_in, nx)) ||
         (rc = dalloc(c, &c->d_x0, nx)) || (rc = dalloc(c, &c->d_x1, nx)))
       return rc;
     c->cap_sort = n;
@@ -559,7 +559,8 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   uint32_t* wany = any + nvox;
   uint32_t* pid = wany + nvox;  // rank of any, then the pair id
   uint32_t* rank_w = pid + nvox;
-  launch_zero_words(any, (size_t)nvox * 8, s);  // any and wany
+  // (any and wany are all zero here: cleared before the first camera, and
+  // behind by the pair table of each camera)
   HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 32, s));
   launch_reslice(cam.d_front, cam.d_lanes, (uint32_t)npix, c->d_sell0, c->d_offs, cam.d_len_f, c->d_has, cam.d_len_w,
                  (uint32_t)nrec_f, cam.d_inc, any, wany, (unsigned long long*)(c->d_total + 2), s);
@@ -611,19 +612,23 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
       (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
     return rc;
   launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
-  // 5. per pair: C and the constant front messages; records point at their pairs from here on
-  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, c->d_acc + 4 * (size_t)np,
-                      np, cam.d_pair_c, cam.d_pair_sf, s);
-  // 6. window segments: the window records stably sorted by pair
+  // 5. per pair: C and the constant front messages; records point at their
+  //    pairs from here on.  The same pass writes the window sort's input
   const int wbits = std::max(1, bit_length(nwp));
   if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits))) return rc;
-  if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) || (rc = dalloc(c, &cam.d_win_idx, nwin))) return rc;
-  HIPCHK(c, sort_window(cam.d_inc + nrec_f, (uint32_t)nrec_w, nwp, wbits, c->d_keys_in, c->d_keys_out,
-                        c->d_vals_in, c->d_vals_out, c->d_sort_tmp, c->sort_tmp_bytes, s));
+  // win_idx holds the sorted values of the whole window region; its first
+  // nwin (the live records) are the segment layout
+  if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) ||
+      (rc = dalloc(c, &cam.d_win_idx, std::max<uint64_t>(nrec_w, 1))))
+    return rc;
+  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, c->d_acc + 4 * (size_t)np,
+                      np, cam.d_pair_c, cam.d_pair_sf, c->d_keys_in, c->d_vals_in, nwp, s);
+  // 6. window segments: the window records stably sorted by pair
+  HIPCHK(c, sort_window((uint32_t)nrec_w, wbits, c->d_keys_in, c->d_keys_out, c->d_vals_in, cam.d_win_idx,
+                        c->d_sort_tmp, c->sort_tmp_bytes, s));
   uint32_t* whead = c->d_x0;
   uint32_t* hw = c->d_x1;
   if (nwin) {
-    HIPCHK(c, hipMemcpyAsync(cam.d_win_idx, c->d_vals_out, nwin * 4, hipMemcpyDeviceToDevice, s));
     launch_window_heads(c->d_keys_out, (uint32_t)nwin, whead, s);
     if ((rc = flag_scan(c, whead, hw, nwin))) return rc;
   }
@@ -676,7 +681,13 @@ int prepare(mrf_ctx* c) {
   if ((rc = ensure_pool(c, c->slot_lin.size()))) return rc;
   if (grew || true)
     HIPCHK(c, hipMemcpyAsync(c->d_slot, c->h_slot.data(), c->h_slot.size() * 4, hipMemcpyHostToDevice, c->stream));
-  // 3. traversal lists and pairs of every camera
+  // 3. traversal lists and pairs of every camera; the voxel flags of the
+  //    pair numbering start cleared (each camera clears its own behind)
+  {
+    const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
+    if ((rc = ensure_vflags(c, std::max<size_t>(nvox, 1)))) return rc;
+    launch_zero_words(c->d_vflag, nvox * 8, c->stream);
+  }
   for (Camera& cam : c->cams)
     if ((rc = traverse_camera(c, cam))) return rc;
   // 3b. voxels where some camera has window records; per camera, its
@@ -1282,7 +1293,7 @@ void mrf_destroy(mrf_ctx* c) {
   if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
   dfree(&c->d_Spair);
   dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
-  dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in); dfree(&c->d_vals_out);
+  dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in);
   dfree(&c->d_x0); dfree(&c->d_x1);
   dfree(&c->d_vflag); dfree(&c->d_acc); dfree(&c->d_sell0);
   dfree(&c->d_eval); dfree(&c->d_eval_part); dfree(&c->d_eval_counts);

@@ -34,16 +34,20 @@ namespace {
 
 constexpr double kFixLen = 16777216.0;  // 2^24: chord lengths (< 2) in fixed point, a 64-record sum < 2^31
 
-// pid[v] (written over rank_all) and pair_vox[pid] of every touched voxel
-__global__ void k_pair_table(const uint32_t* __restrict__ any, const uint32_t* __restrict__ wany,
+// pid[v] (written over rank_all) and pair_vox[pid] of every touched voxel;
+// the flags are cleared behind (all zero again for the next camera)
+__global__ void k_pair_table(uint32_t* __restrict__ any, uint32_t* __restrict__ wany,
                              uint32_t* __restrict__ rank_all, const uint32_t* __restrict__ rank_w, uint64_t nvox,
                              uint32_t nwp, uint32_t* __restrict__ pair_vox) {
   const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nvox || !any[v]) return;
   const uint32_t rw = rank_w[v];
-  const uint32_t p = wany[v] ? rw : nwp + (rank_all[v] - rw);
+  const bool w = wany[v] != 0u;
+  const uint32_t p = w ? rw : nwp + (rank_all[v] - rw);
   rank_all[v] = p;
   pair_vox[p] = (uint32_t)v;
+  any[v] = 0u;
+  if (w) wany[v] = 0u;
 }
 
 // value of lane ^ J; J < 32 stays inside the 32-lane half (ds_swizzle, bitmask mode)
@@ -144,10 +148,15 @@ __device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t& x2, uint32
 // PACKED (fewer than 2^26 - 1 pairs): the sort moves one word per lane, the
 // pair and the lane it came from (pair 2^6 + lane, unique), and each lane then
 // fetches its lengths from that lane; else (pair, q, r) move together.
+// The same pass writes the window region's sort input: per window record j
+// (index base_w + j) key its pair (sentinel for a gap or padding record,
+// sorting last) and value j.
 template <bool PACKED>
 __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
                                                       const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_w,
-                                                      uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r) {
+                                                      uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r,
+                                                      uint32_t* __restrict__ wkeys, uint32_t* __restrict__ wvals,
+                                                      uint32_t sentinel) {
   // batches of kR rows per wave (nrec is a multiple of 64): the batch's
   // records and then their pairs are loaded together, with no branches (a
   // row past the end reads row 0 as gap records); the rows are sorted and
@@ -182,6 +191,11 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
         r[j] = (uint32_t)((x - (double)q[j]) * kFixLen);
       } else {
         key[j] = 0xFFFFFFFFu;
+      }
+      if ((row0 + j) * 64 >= base_w && row0 + j < nrows) {  // uniform per row
+        const uint64_t jw = (row0 + j) * 64 - base_w + lane;
+        wkeys[jw] = key[j] == 0xFFFFFFFFu ? sentinel : key[j];
+        wvals[jw] = (uint32_t)jw;
       }
       if constexpr (PACKED) {
         uint32_t v = key[j] == 0xFFFFFFFFu ? 0xFFFFFFFFu : (key[j] << 6) | (uint32_t)lane;
@@ -241,17 +255,6 @@ __global__ void k_pair_finish(const uint32_t* __restrict__ acc_w, const uint32_t
   pair_sf[p] = (float)((double)kFEps * (f / kFixLen));
 }
 
-// sort keys of the window region: the record's pair, nwp (sorts last) for
-// gap and padding records; values: the record's index in the region
-__global__ void k_window_keys(const uint2* __restrict__ recw, uint32_t n, uint32_t sentinel,
-                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t q = recw[j].x;
-  keys[j] = q == kGapVox ? sentinel : q;
-  vals[j] = j;
-}
-
 __global__ void k_window_heads(const uint32_t* __restrict__ keys, uint32_t nwin, uint32_t* __restrict__ whead) {
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nwin) return;
@@ -302,7 +305,7 @@ size_t window_sort_temp_bytes(uint32_t n, int bits) {
   return bytes;
 }
 
-void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
+void launch_pair_table(uint32_t* any, uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
                        uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s) {
   if (nvox == 0) return;
   hipLaunchKernelGGL(k_pair_table, dim3(blocks(nvox, 256)), dim3(256), 0, s, any, wany, rank_all, rank_w, nvox, nwp,
@@ -311,26 +314,24 @@ void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank
 
 void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_w,
                          uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
-                         hipStream_t s) {
+                         uint32_t* wkeys, uint32_t* wvals, uint32_t sentinel, hipStream_t s) {
   if (np == 0) return;
   launch_zero_words(acc_w, (size_t)np * 8, s);
   launch_zero_words(acc_f, (size_t)np * 8, s);
   launch_zero_words(acc_r, (size_t)np * 8, s);
   if (np < (1u << 26) - 1u)
     hipLaunchKernelGGL(k_pair_lengths<true>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w,
-                       acc_f, acc_r);
+                       acc_f, acc_r, wkeys, wvals, sentinel);
   else
     hipLaunchKernelGGL(k_pair_lengths<false>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w,
-                       acc_f, acc_r);
+                       acc_f, acc_r, wkeys, wvals, sentinel);
   hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_w, acc_f, acc_r, np, pair_c,
                      pair_sf);
 }
 
-hipError_t sort_window(const uint2* recw, uint32_t nwr, uint32_t nwp, int bits, uint32_t* keys_in,
-                       uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
-                       hipStream_t s) {
+hipError_t sort_window(uint32_t nwr, int bits, uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in,
+                       uint32_t* vals_out, void* temp, size_t temp_bytes, hipStream_t s) {
   if (nwr == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_window_keys, dim3(blocks(nwr, 256)), dim3(256), 0, s, recw, nwr, nwp, keys_in, vals_in);
   size_t tb = temp_bytes;
   return hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_in, keys_out, vals_in, vals_out, (int)nwr, 0, bits, s);
 }

@@ -72,22 +72,24 @@ int num_tiled_pixels(const CamGeom& c);
 // window segment layout of the BP iteration.
 // from the exclusive scans rank_all (of any) and rank_w (of wany): pair id
 // of every touched voxel (window pairs first, each group in voxel order),
-// written over rank_all, and pair_vox[pair]
-void launch_pair_table(const uint32_t* any, const uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
+// written over rank_all, and pair_vox[pair]; clears any and wany behind
+void launch_pair_table(uint32_t* any, uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
                        uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s);
 // per pair: C (summed chord length) and F_EPS x the chord length of its
 // front records (index < base_w), exact fixed-point sums; acc_w (window
 // records) / acc_f (front records) / acc_r (C below 2^-24) hold 2 words per
-// pair.  Rewrites every live record's voxel to its pair (pid).
+// pair.  Rewrites every live record's voxel to its pair (pid), and writes the
+// window sort input: wkeys[j] = pair of window record j (sentinel for a gap
+// or padding record), wvals[j] = j.
 void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_w,
                          uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
-                         hipStream_t s);
+                         uint32_t* wkeys, uint32_t* wvals, uint32_t sentinel, hipStream_t s);
 size_t window_sort_temp_bytes(uint32_t n, int bits);
-// window region records (holding pairs) stably sorted by pair (gap /
-// padding records last): keys_out = pairs, vals_out = record index in the region
-hipError_t sort_window(const uint2* recw, uint32_t nwr, uint32_t nwp, int bits, uint32_t* keys_in,
-                       uint32_t* keys_out, uint32_t* vals_in, uint32_t* vals_out, void* temp, size_t temp_bytes,
-                       hipStream_t s);
+// window region records stably sorted by pair (gap / padding records, with
+// the sentinel key, last): keys_out = pairs, vals_out = record index in the
+// region
+hipError_t sort_window(uint32_t nwr, int bits, uint32_t* keys_in, uint32_t* keys_out, uint32_t* vals_in,
+                       uint32_t* vals_out, void* temp, size_t temp_bytes, hipStream_t s);
 void launch_window_heads(const uint32_t* keys, uint32_t nwin, uint32_t* whead, hipStream_t s);
 // wseg_off[pair] = first window position of the pair, wseg_off[nwp] = nwin
 void launch_window_offsets(const uint32_t* keys, const uint32_t* whead, uint32_t nwin, uint32_t nwp,
