@@ -19,6 +19,10 @@
         kernels of one camera pass), summed.  With --first N, each kernel's
         figures use its first N dispatches only (from the per-dispatch trace),
         e.g. the 20 x 13 camera passes of the bench's first timed call.
+    python -m mrfmap_amd.profiling geometry DIR
+        Markdown table of the geometry build in the kernel trace under DIR:
+        every dispatch before the first front-sum kernel (activation, lists,
+        pairs, window segments), summed per kernel.
 """
 import argparse
 import collections
@@ -87,6 +91,36 @@ def kernel_stats(d, first=None):
     rows = [{"Name": k, "Calls": str(len(v)), "TotalDurationNs": str(sum(v)), "AverageNs": str(sum(v) / len(v)),
              "Percentage": str(100.0 * sum(v) / total)} for k, v in durs.items()]
     return sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))
+
+
+def geometry(d, top=24):
+    """Per-kernel totals of the dispatches before the first k_front_sums (the
+    geometry build of the traced run), as markdown."""
+    path = _find(d, "kernel_trace.csv")
+    if path is None:
+        return "no kernel trace under " + d
+    rows = sorted(_read_csv(path), key=lambda r: int(r["Start_Timestamp"]))
+    first = next((i for i, r in enumerate(rows) if "k_front_sums" in r["Kernel_Name"]), len(rows))
+    rows = rows[:first]
+    if not rows:
+        return "no geometry dispatches"
+    agg = collections.OrderedDict()
+    for r in rows:
+        n = short_name(r["Kernel_Name"])
+        n = re.sub(r"rocprim::trampoline_kernel<.*", "rocprim (radix sort and scans)", n)
+        n = re.sub(r"rocprim::init_lookback.*", "rocprim (radix sort and scans)", n)
+        n = re.sub(r"rocprim::.*", "rocprim (radix sort and scans)", n)
+        a = agg.setdefault(n, [0, 0])
+        a[0] += 1
+        a[1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    total = sum(v[1] for v in agg.values())
+    span = int(rows[-1]["End_Timestamp"]) - int(rows[0]["Start_Timestamp"])
+    lines = [f"kernel time {total / 1e6:.2f} ms over {len(rows)} dispatches "
+             f"(first to last dispatch {span / 1e6:.2f} ms under the tracer)", "",
+             "| kernel | calls | total ms | avg ms |", "|---|---:|---:|---:|"]
+    for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        lines.append(f"| {n} | {c} | {t / 1e6:.2f} | {t / c / 1e6:.3f} |")
+    return "\n".join(lines)
 
 
 def calibration(caldir):
@@ -200,12 +234,17 @@ def main(argv=None):
     s.add_argument("--kernels", default="k_front_sums,k_ray_window",
                    help="comma-separated kernel families whose per-launch bytes are summed")
     s.add_argument("--first", type=int, default=None, help="each kernel's first N dispatches only")
+    g = sub.add_parser("geometry")
+    g.add_argument("dir")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_once()
         return 0
     if a.cmd == "stream":
         stream(a.gib, a.reps)
+        return 0
+    if a.cmd == "geometry":
+        print(geometry(a.dir))
         return 0
     text, traffic = summarize(a.dir, a.calib, tuple(a.kernels.split(",")), first=a.first)
     print(text)
