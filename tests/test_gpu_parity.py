@@ -511,6 +511,42 @@ def test_edge_case_keyframes_match_oracle():
     assert err.max() <= MARG_TOL
 
 
+def test_full_ray_activation_on_tied_rays():
+    """Full-ray activation walks the brick grid and sizes each ray's list by a
+    bound of its voxel count.  Cameras centred on brick corners with
+    axis-aligned rotations: rays along an axis (two axes never move), through
+    voxel edges and brick corners at every step (two or three axes tied), at
+    half slopes (ties every other step), truncated inside the grid or leaving
+    it through a face: bricks, lists and marginals as the oracle's."""
+    p, _, _ = synth.make_workload("small")
+    n = 129
+    K = np.array([[64, 0, 64.0], [0, 64, 64.0], [0, 0, 1]], np.float32)
+    depth = np.full((n, n), 2.0, np.float32)
+    depth[:, 96:] = 9.0  # these rays leave the grid
+    off = np.array([p.dims[0] / p.res / 2, p.dims[1] / p.res / 2, 0.0])  # world -> voxel offset
+
+    def pose(R, corner):
+        T = np.eye(4, dtype=np.float32)
+        T[:3, :3] = R
+        T[:3, 3] = (np.asarray(corner, np.float64) - off) * p.res
+        return T
+
+    Rx = np.array([[1, 0, 0], [0, 0, -1], [0, 1, 0]], np.float32)  # optical axis along -y
+    frames = [(pose(np.eye(3), (48, 48, 16)), K, depth), (pose(Rx, (40, 64, 24)), K, depth)]
+    g = gpu_for(p, frames)
+    o = oracle_for(p, frames)
+    g.prepare()
+    o.prepare()
+    assert g.num_bricks() == o.num_bricks()
+    np.testing.assert_array_equal(g.get_occupied_bricks(), (o.brick_coords() * 8 + 4).astype(np.float32))
+    assert _compare_traversal(g, o, len(frames)) > 100_000
+    g.perform_inference(max_iters=3)
+    o.perform_inference(3)
+    err = np.abs(g.get_alphas().astype(np.float64) - o.alphas())
+    print(f"tied rays: {g.num_bricks()} bricks, max-abs marginal error {err.max():.3e}")
+    assert err.max() <= MARG_TOL
+
+
 def test_empty_map_and_no_cameras():
     """Only keyframes without valid depth: no bricks, empty readback, no
     features, a warm start over no voxels; a context without keyframes runs
