@@ -327,6 +327,15 @@ int mrf_get_traversal(mrf_ctx* ctx, uint32_t cam_id, int64_t* n_rays, int64_t* n
 
 const char* mrf_last_error(mrf_ctx* ctx);
 
+/* Returns to the HIP runtime the large device blocks (32 MiB and up) that
+ * the library keeps after contexts free them, for reuse by later contexts
+ * (the runtime takes seconds to provide tens of GB again after such a
+ * free).  No counterpart in the reference; for callers that share the
+ * device with other allocators.  The cache holds at most 3/4 of the
+ * device's memory and empties itself when an allocation fails;
+ * MRF_NO_DEVICE_CACHE=1 in the environment switches it off. */
+int mrf_release_device_cache(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ EXPORTS = [
     "mrf_set_eval_thresholds",
     "mrf_set_launch_mode", "mrf_train_opts_default", "mrf_perform_inference_warm", "mrf_features",
     "mrf_warm_predictions", "mrf_mlp_forward", "mrf_mlp_grad", "mrf_mlp_train", "mrf_set_reference", "mrf_get_trace",
-    "mrf_mlp_grad_noalloc",
+    "mrf_mlp_grad_noalloc", "mrf_release_device_cache",
 ]
 MLP_FEATURES, MLP_PARAMS = 32, 6337
 MLP_NOALLOC_MAX_SAMPLES = 16384
@@ -145,8 +145,15 @@ def load():
     L.mrf_mlp_grad_noalloc.argtypes = [P, P, P, P, ctypes.c_int64, P, P, P]
     L.mrf_last_error.argtypes = [P]
     L.mrf_last_error.restype = ctypes.c_char_p
+    L.mrf_release_device_cache.argtypes = []
     _lib = L
     return L
+
+
+def release_device_cache():
+    """Give the large device blocks the library keeps after freeing them back
+    to the HIP runtime (mrf_release_device_cache)."""
+    check(load().mrf_release_device_cache())
 
 
 def check(rc, h=None):

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <unordered_map>
 #include <chrono>
 #include <random>
 #include <cmath>
@@ -224,11 +226,111 @@ int fail(mrf_ctx* c, int code, const std::string& msg) {
       return fail(ctx, MRF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
   } while (0)
 
+// Process-wide cache of large device blocks.  A context's lists take tens
+// of GB (85 GB for 64 keyframes); when they are freed and a new context
+// allocates as much again, the runtime takes seconds to provide the memory
+// (a 64-keyframe build: 0.86 s in a fresh process, 4.3-4.5 s after a
+// context of that size was closed, whose frees took 2.4 s).  Blocks of at
+// least kCacheMin bytes are therefore kept on free (after the device has
+// finished with them) and handed out again for requests of up to their size
+// and at least 4/5 of it.  The cache keeps at most 3/4 of the device's
+// memory, gives everything back when an allocation fails, and is emptied by
+// mrf_release_device_cache(); MRF_NO_DEVICE_CACHE=1 switches it off.
+struct BlockCache {
+  std::mutex mu;
+  std::unordered_map<void*, std::pair<size_t, int>> live;  // large blocks in use: bytes, device
+  std::multimap<std::pair<int, size_t>, void*> idle;        // (device, bytes) -> kept block
+  size_t kept = 0;
+};
+constexpr size_t kCacheMin = size_t(32) << 20;
+
+BlockCache& block_cache() {
+  static BlockCache* bc = new BlockCache;  // never destroyed: frees may run during static destruction
+  return *bc;
+}
+
+bool cache_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("MRF_NO_DEVICE_CACHE");
+    return !(e && e[0] && e[0] != '0');
+  }();
+  return on;
+}
+
+// the cached blocks of device dev (all devices for dev < 0) back to the runtime; mu held
+void release_idle(BlockCache& bc, int dev) {
+  for (auto it = bc.idle.begin(); it != bc.idle.end();) {
+    if (dev >= 0 && it->first.first != dev) {
+      ++it;
+      continue;
+    }
+    (void)hipFree(it->second);
+    bc.kept -= it->first.second;
+    it = bc.idle.erase(it);
+  }
+}
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes < kCacheMin || !cache_enabled()) return hipMalloc(p, bytes);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  BlockCache& bc = block_cache();
+  std::lock_guard<std::mutex> lock(bc.mu);
+  auto it = bc.idle.lower_bound({dev, bytes});
+  if (it != bc.idle.end() && it->first.first == dev && it->first.second - it->first.second / 5 <= bytes) {
+    *p = it->second;
+    bc.live[*p] = {it->first.second, dev};
+    bc.kept -= it->first.second;
+    bc.idle.erase(it);
+    return hipSuccess;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && !bc.idle.empty()) {  // give the kept blocks back and try again
+    (void)hipGetLastError();
+    release_idle(bc, dev);
+    e = hipMalloc(p, bytes);
+  }
+  if (e == hipSuccess) bc.live[*p] = {bytes, dev};
+  return e;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  BlockCache& bc = block_cache();
+  {
+    std::lock_guard<std::mutex> lock(bc.mu);
+    auto it = bc.live.find(p);
+    if (it != bc.live.end()) {
+      const size_t bytes = it->second.first;
+      const int dev = it->second.second;
+      bc.live.erase(it);
+      static size_t cap = 0;
+      if (!cap) {
+        size_t free_b = 0, total_b = 0;
+        cap = hipMemGetInfo(&free_b, &total_b) == hipSuccess ? total_b / 4 * 3 : size_t(64) << 30;
+      }
+      if (bc.kept + bytes <= cap) {
+        // the block's last uses end before another allocation can take it
+        int cur = dev;
+        (void)hipGetDevice(&cur);
+        if (cur != dev) (void)hipSetDevice(dev);
+        (void)hipDeviceSynchronize();
+        if (cur != dev) (void)hipSetDevice(cur);
+        bc.idle.insert({{dev, bytes}, p});
+        bc.kept += bytes;
+        return;
+      }
+    }
+  }
+  (void)hipFree(p);
+}
+
 template <typename T>
 int dalloc(mrf_ctx* c, T** p, size_t count) {
   *p = nullptr;
   if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  hipError_t e = dev_malloc((void**)p, count * sizeof(T));
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
@@ -239,7 +341,7 @@ int dalloc(mrf_ctx* c, T** p, size_t count) {
 
 template <typename T>
 void dfree(T** p) {
-  if (*p) (void)hipFree((void*)*p);
+  dev_free((void*)*p);
   *p = nullptr;
 }
 
@@ -372,10 +474,10 @@ int ensure_sort(mrf_ctx* c, size_t n, int bits) {
   }
   const size_t tb = window_sort_temp_bytes((uint32_t)n, bits);
   if (tb > c->sort_tmp_bytes) {
-    if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
+    dev_free(c->d_sort_tmp);
     c->d_sort_tmp = nullptr;
     c->sort_tmp_bytes = 0;
-    if (hipMalloc(&c->d_sort_tmp, tb) != hipSuccess) {
+    if (dev_malloc(&c->d_sort_tmp, tb) != hipSuccess) {
       (void)hipGetLastError();
       return fail(c, MRF_E_NOMEM, "sort scratch allocation failed");
     }
@@ -395,10 +497,10 @@ int flag_scan(mrf_ctx* c, const uint32_t* in, uint32_t* out, size_t n, uint64_t*
   }
   const size_t tb = flag_scan_temp_bytes(n);
   if (tb > c->fscan_bytes || !c->d_fscan_tmp) {
-    if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp);
+    dev_free(c->d_fscan_tmp);
     c->d_fscan_tmp = nullptr;
     c->fscan_bytes = 0;
-    if (hipMalloc(&c->d_fscan_tmp, std::max<size_t>(tb, 256)) != hipSuccess) {
+    if (dev_malloc(&c->d_fscan_tmp, std::max<size_t>(tb, 256)) != hipSuccess) {
       (void)hipGetLastError();
       return fail(c, MRF_E_NOMEM, "flag-scan scratch allocation failed");
     }
@@ -775,10 +877,10 @@ int prepare(mrf_ctx* c) {
   }
   const size_t xout = xneed * msg_bytes(msg_out_type(c->p.msg_dtype));
   if (xout > c->Xo_bytes) {
-    if (c->d_Xo) (void)hipFree(c->d_Xo);
+    dev_free(c->d_Xo);
     c->d_Xo = nullptr;
     c->Xo_bytes = 0;
-    if (hipMalloc(&c->d_Xo, xout) != hipSuccess) {
+    if (dev_malloc(&c->d_Xo, xout) != hipSuccess) {
       (void)hipGetLastError();
       return fail(c, MRF_E_NOMEM, "segment array allocation failed");
     }
@@ -1289,8 +1391,8 @@ void mrf_destroy(mrf_ctx* c) {
   }
   dfree(&c->d_sig); dfree(&c->d_bia); dfree(&c->d_slot); dfree(&c->d_bflags);
   dfree(&c->d_B); dfree(&c->d_A); dfree(&c->d_upd); dfree(&c->d_out);
-  if (c->d_Xo) (void)hipFree(c->d_Xo);
-  if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
+  dev_free(c->d_Xo);
+  dev_free(c->d_sort_tmp);
   dfree(&c->d_Spair);
   dfree(&c->d_pf_v); dfree(&c->d_pl_v); dfree(&c->d_mid);
   dfree(&c->d_keys_in); dfree(&c->d_keys_out); dfree(&c->d_vals_in);
@@ -1306,7 +1408,7 @@ void mrf_destroy(mrf_ctx* c) {
   drop_graph(c);
   dfree(&c->d_counts); dfree(&c->d_has); dfree(&c->d_offs); dfree(&c->d_rayidx);
   dfree(&c->d_scan_tmp); dfree(&c->d_total);
-  if (c->d_fscan_tmp) (void)hipFree(c->d_fscan_tmp); dfree(&c->d_all_pairs);
+  dev_free(c->d_fscan_tmp); dfree(&c->d_all_pairs);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1974,6 +2076,13 @@ int mrf_set_eval_thresholds(mrf_ctx* c, float occ_thresh, float acc_thresh) {
   if (!c) return MRF_E_INVALID;
   c->p.occ_thresh = occ_thresh;
   c->p.acc_thresh = acc_thresh;
+  return MRF_OK;
+}
+
+int mrf_release_device_cache(void) {
+  BlockCache& bc = block_cache();
+  std::lock_guard<std::mutex> lock(bc.mu);
+  release_idle(bc, -1);
   return MRF_OK;
 }
 

@@ -547,6 +547,25 @@ def test_full_ray_activation_on_tied_rays():
     assert err.max() <= MARG_TOL
 
 
+def test_reused_device_blocks_give_the_same_map():
+    """Contexts built one after another reuse the large device blocks (lists of
+    1.2 GB per 640x480 keyframe at 1 cm) that the previous one freed, stale
+    contents included: the maps are bitwise equal, and again after the kept
+    blocks are given back to the runtime."""
+    p, frames, _ = synth.make_workload("real", frames=2)
+    maps = []
+    for k in range(3):
+        if k == 2:
+            _lib.release_device_cache()
+        g = gpu_for(p, frames)
+        g.perform_inference(max_iters=3)
+        maps.append(g.get_alphas().copy())
+        g.close()
+    assert maps[0].shape[0] > 0
+    np.testing.assert_array_equal(maps[1], maps[0])
+    np.testing.assert_array_equal(maps[2], maps[0])
+
+
 def test_empty_map_and_no_cameras():
     """Only keyframes without valid depth: no bricks, empty readback, no
     features, a warm start over no voxels; a context without keyframes runs
