@@ -510,6 +510,24 @@ int flag_scan(mrf_ctx* c, const uint32_t* in, uint32_t* out, size_t n, uint64_t*
   return MRF_OK;
 }
 
+// The same over byte flags (n below 2^31 - 1).
+int flag_scan_u8(mrf_ctx* c, const uint8_t* in, uint32_t* out, size_t n, uint64_t* total) {
+  if (n >= 0x7FFFFFFFull) return fail(c, MRF_E_LIMIT, "more than 2^31 voxels in the brick pool");
+  const size_t tb = flag_scan_u8_temp_bytes(n);
+  if (tb > c->fscan_bytes || !c->d_fscan_tmp) {
+    dev_free(c->d_fscan_tmp);
+    c->d_fscan_tmp = nullptr;
+    c->fscan_bytes = 0;
+    if (dev_malloc(&c->d_fscan_tmp, std::max<size_t>(tb, 256)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, MRF_E_NOMEM, "flag-scan scratch allocation failed");
+    }
+    c->fscan_bytes = std::max<size_t>(tb, 256);
+  }
+  HIPCHK(c, launch_flag_scan_u8(in, out, n, c->d_fscan_tmp, c->fscan_bytes, total, c->stream));
+  return MRF_OK;
+}
+
 int ensure_vflags(mrf_ctx* c, size_t nvox) {
   if (nvox <= c->cap_vflag && c->d_vflag) return MRF_OK;
   dfree(&c->d_vflag);
@@ -657,9 +675,9 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   //    of live window records (wany), and the counts of both
   const uint64_t nvox = (uint64_t)c->slot_lin.size() * kBrickVox;
   if ((rc = ensure_vflags(c, std::max<uint64_t>(nvox, 1)))) return rc;
-  uint32_t* any = c->d_vflag;
-  uint32_t* wany = any + nvox;
-  uint32_t* pid = wany + nvox;  // rank of any, then the pair id
+  uint8_t* any = reinterpret_cast<uint8_t*>(c->d_vflag);  // byte flags
+  uint8_t* wany = any + nvox;
+  uint32_t* pid = c->d_vflag + (2 * nvox + 3) / 4;  // rank of any, then the pair id
   uint32_t* rank_w = pid + nvox;
   // (any and wany are all zero here: cleared before the first camera, and
   // behind by the pair table of each camera)
@@ -690,7 +708,8 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   //    One readback: [0..3] the reslice counters (live records, live window
   //    records, records with gap records, rays with records), [4] the
   //    emission's overflow flag, [6] / [7] the pairs / window pairs
-  if ((rc = flag_scan(c, any, pid, nvox, c->d_total + 8)) || (rc = flag_scan(c, wany, rank_w, nvox, c->d_total + 9)))
+  if ((rc = flag_scan_u8(c, any, pid, nvox, c->d_total + 8)) ||
+      (rc = flag_scan_u8(c, wany, rank_w, nvox, c->d_total + 9)))
     return rc;
   uint64_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIPCHK(c, hipMemcpyAsync(t8, c->d_total + 2, sizeof(t8), hipMemcpyDeviceToHost, s));
@@ -788,7 +807,7 @@ int prepare(mrf_ctx* c) {
   {
     const size_t nvox = c->slot_lin.size() * (size_t)kBrickVox;
     if ((rc = ensure_vflags(c, std::max<size_t>(nvox, 1)))) return rc;
-    launch_zero_words(c->d_vflag, nvox * 8, c->stream);
+    launch_zero_words(c->d_vflag, (2 * nvox + 3) / 4 * 4, c->stream);  // the byte flags any, wany
   }
   for (Camera& cam : c->cams)
     if ((rc = traverse_camera(c, cam))) return rc;

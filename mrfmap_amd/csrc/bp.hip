@@ -163,8 +163,8 @@ __global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, R
                                                  uint32_t npix, const uint2* __restrict__ sell,
                                                  const uint32_t* __restrict__ off_f, const uint32_t* __restrict__ len_f,
                                                  const uint32_t* __restrict__ off_w, const uint32_t* __restrict__ len_w,
-                                                 uint32_t base_w, uint2* __restrict__ inc, uint32_t* __restrict__ any,
-                                                 uint32_t* __restrict__ wany, unsigned long long* __restrict__ counts) {
+                                                 uint32_t base_w, uint2* __restrict__ inc, uint8_t* __restrict__ any,
+                                                 uint8_t* __restrict__ wany, unsigned long long* __restrict__ counts) {
   const uint32_t ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= npix) return;  // uniform per wavefront: npix is a multiple of 64
   const uint32_t t = ray >> 6, l = ray & 63u;
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, R
         const bool live = e[u].x != kGapVox;
         nl += (uint64_t)__popcll(__ballot(live));
         const bool d = dup(e[u].x);
-        if (live && !d) any[e[u].x] = 1u;
+        if (live && !d) any[e[u].x] = 1;
       }
     }
   }
@@ -220,8 +220,8 @@ __global__ __launch_bounds__(256) void k_reslice(FrontRec* __restrict__ front, R
         nw += n;
         const bool d = dup(e[u].x);
         if (live && !d) {
-          any[e[u].x] = 1u;
-          wany[e[u].x] = 1u;
+          any[e[u].x] = 1;
+          wany[e[u].x] = 1;
         }
       }
     }
@@ -1302,7 +1302,7 @@ void launch_epilogue(const CamPass& a, hipStream_t s, int part) {
 
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
                     const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
-                    uint32_t* any, uint32_t* wany, unsigned long long* counts, hipStream_t s) {
+                    uint8_t* any, uint8_t* wany, unsigned long long* counts, hipStream_t s) {
   if (npix == 0) return;
   hipLaunchKernelGGL(k_reslice, dim3(grid_for(npix, 256)), dim3(256), 0, s, front, lanes, npix, sell, off_f, len_f,
                      off_w, len_w, base_w, inc, any, wany, counts);

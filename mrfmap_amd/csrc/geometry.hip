@@ -36,18 +36,18 @@ constexpr double kFixLen = 16777216.0;  // 2^24: chord lengths (< 2) in fixed po
 
 // pid[v] (written over rank_all) and pair_vox[pid] of every touched voxel;
 // the flags are cleared behind (all zero again for the next camera)
-__global__ void k_pair_table(uint32_t* __restrict__ any, uint32_t* __restrict__ wany,
+__global__ void k_pair_table(uint8_t* __restrict__ any, uint8_t* __restrict__ wany,
                              uint32_t* __restrict__ rank_all, const uint32_t* __restrict__ rank_w, uint64_t nvox,
                              uint32_t nwp, uint32_t* __restrict__ pair_vox) {
   const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nvox || !any[v]) return;
   const uint32_t rw = rank_w[v];
-  const bool w = wany[v] != 0u;
+  const bool w = wany[v] != 0;
   const uint32_t p = w ? rw : nwp + (rank_all[v] - rw);
   rank_all[v] = p;
   pair_vox[p] = (uint32_t)v;
-  any[v] = 0u;
-  if (w) wany[v] = 0u;
+  any[v] = 0;
+  if (w) wany[v] = 0;
 }
 
 // value of lane ^ J; J < 32 stays inside the 32-lane half (ds_swizzle, bitmask mode)
@@ -182,8 +182,11 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
 #pragma unroll
     for (int j = 0; j < kR; ++j) {
       q[j] = r[j] = 0;
+      // every record of the row is written back (gap records unchanged), so
+      // the row's lines are written whole
+      if (row0 + j < nrows)  // uniform per row
+        rec[(row0 + j) * 64 + lane] = e[j].x != kGapVox ? make_uint2(key[j], e[j].y) : e[j];
       if (e[j].x != kGapVox) {
-        rec[(row0 + j) * 64 + lane].x = key[j];
         // l 2^24 is exact in double; q its integer part (< 2^25), r the rest
         // in units of 2^-48 (< 2^24)
         const double x = (double)__uint_as_float(e[j].y) * kFixLen;
@@ -305,7 +308,7 @@ size_t window_sort_temp_bytes(uint32_t n, int bits) {
   return bytes;
 }
 
-void launch_pair_table(uint32_t* any, uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
+void launch_pair_table(uint8_t* any, uint8_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
                        uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s) {
   if (nvox == 0) return;
   hipLaunchKernelGGL(k_pair_table, dim3(blocks(nvox, 256)), dim3(256), 0, s, any, wany, rank_all, rank_w, nvox, nwp,

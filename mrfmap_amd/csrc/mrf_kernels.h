@@ -62,6 +62,10 @@ void launch_eval(const CamGeom& c, const GridGeom& g, const float* depth, const 
 size_t flag_scan_temp_bytes(size_t n);
 hipError_t launch_flag_scan(const uint32_t* in, uint32_t* out, size_t n, void* temp, size_t temp_bytes,
                             uint64_t* d_total, hipStream_t s);
+// the same over byte flags (temp: flag_scan_u8_temp_bytes(n) bytes)
+size_t flag_scan_u8_temp_bytes(size_t n);
+hipError_t launch_flag_scan_u8(const uint8_t* in, uint32_t* out, size_t n, void* temp, size_t temp_bytes,
+                               uint64_t* d_total, hipStream_t s);
 size_t scan_tmp_words(size_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint64_t* tmp, uint64_t* d_total,
                            hipStream_t s);
@@ -73,7 +77,7 @@ int num_tiled_pixels(const CamGeom& c);
 // from the exclusive scans rank_all (of any) and rank_w (of wany): pair id
 // of every touched voxel (window pairs first, each group in voxel order),
 // written over rank_all, and pair_vox[pair]; clears any and wany behind
-void launch_pair_table(uint32_t* any, uint32_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
+void launch_pair_table(uint8_t* any, uint8_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
                        uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s);
 // per pair: C (summed chord length) and F_EPS x the chord length of its
 // front records (index < base_w), exact fixed-point sums; acc_w (window
@@ -161,13 +165,13 @@ struct CamPass {
   const uint8_t* tail;
 };
 // copy into the final front/window regions; window region starts at base_w.
-// any / wany: pool-sized flags of voxels touched by a live / window record
-// (set, never cleared); counts[0] / counts[1] += the live / live window
-// records, counts[2] += all records (gap records included), counts[3] += rays
-// with records
+// any / wany: pool-sized byte flags of voxels touched by a live / window
+// record (set, never cleared); counts[0] / counts[1] += the live / live
+// window records, counts[2] += all records (gap records included),
+// counts[3] += rays with records
 void launch_reslice(FrontRec* front, RayRec* lanes, uint32_t npix, const uint2* sell, const uint32_t* off_f,
                     const uint32_t* len_f, const uint32_t* off_w, const uint32_t* len_w, uint32_t base_w, uint2* inc,
-                    uint32_t* any, uint32_t* wany, unsigned long long* counts, hipStream_t s);
+                    uint8_t* any, uint8_t* wany, unsigned long long* counts, hipStream_t s);
 // brick entries of every window record (records still hold voxel ids):
 // bstart[(region row of the slice) + k] bit l = record k of lane l enters a
 // brick (the ray's first record, the record after a gap, or a voxel in
