@@ -510,21 +510,21 @@ int flag_scan(mrf_ctx* c, const uint32_t* in, uint32_t* out, size_t n, uint64_t*
   return MRF_OK;
 }
 
-// The same over byte flags (n below 2^31 - 1).
-int flag_scan_u8(mrf_ctx* c, const uint8_t* in, uint32_t* out, size_t n, uint64_t* total) {
-  if (n >= 0x7FFFFFFFull) return fail(c, MRF_E_LIMIT, "more than 2^31 voxels in the brick pool");
-  const size_t tb = flag_scan_u8_temp_bytes(n);
+// Exclusive scan of n uint64 values into out, total into *total.
+int u64_scan(mrf_ctx* c, const uint64_t* in, uint64_t* out, size_t n, uint64_t* total) {
+  if (n >= 0x7FFFFFFFull) return fail(c, MRF_E_LIMIT, "more than 2^31 bricks in the pool");
+  const size_t tb = u64_scan_temp_bytes(n);
   if (tb > c->fscan_bytes || !c->d_fscan_tmp) {
     dev_free(c->d_fscan_tmp);
     c->d_fscan_tmp = nullptr;
     c->fscan_bytes = 0;
     if (dev_malloc(&c->d_fscan_tmp, std::max<size_t>(tb, 256)) != hipSuccess) {
       (void)hipGetLastError();
-      return fail(c, MRF_E_NOMEM, "flag-scan scratch allocation failed");
+      return fail(c, MRF_E_NOMEM, "scan scratch allocation failed");
     }
     c->fscan_bytes = std::max<size_t>(tb, 256);
   }
-  HIPCHK(c, launch_flag_scan_u8(in, out, n, c->d_fscan_tmp, c->fscan_bytes, total, c->stream));
+  HIPCHK(c, launch_u64_scan(in, out, n, c->d_fscan_tmp, c->fscan_bytes, total, c->stream));
   return MRF_OK;
 }
 
@@ -677,8 +677,11 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if ((rc = ensure_vflags(c, std::max<uint64_t>(nvox, 1)))) return rc;
   uint8_t* any = reinterpret_cast<uint8_t*>(c->d_vflag);  // byte flags
   uint8_t* wany = any + nvox;
-  uint32_t* pid = c->d_vflag + (2 * nvox + 3) / 4;  // rank of any, then the pair id
-  uint32_t* rank_w = pid + nvox;
+  uint32_t* pid = c->d_vflag + (2 * nvox + 3) / 4;  // the pair id of every touched voxel
+  // per brick of the pool: counts of the two flags, then their exclusive scan
+  const uint32_t nbr = (uint32_t)(nvox / kBrickVox);
+  uint64_t* bcnt = reinterpret_cast<uint64_t*>(pid + nvox);
+  uint64_t* boff = bcnt + nbr;
   // (any and wany are all zero here: cleared before the first camera, and
   // behind by the pair table of each camera)
   HIPCHK(c, hipMemsetAsync(c->d_total + 2, 0, 32, s));
@@ -707,15 +710,14 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   // 4. (camera, voxel) pairs: window pairs first, each group in voxel order.
   //    One readback: [0..3] the reslice counters (live records, live window
   //    records, records with gap records, rays with records), [4] the
-  //    emission's overflow flag, [6] / [7] the pairs / window pairs
-  if ((rc = flag_scan_u8(c, any, pid, nvox, c->d_total + 8)) ||
-      (rc = flag_scan_u8(c, wany, rank_w, nvox, c->d_total + 9)))
-    return rc;
+  //    emission's overflow flag, [6] the pairs | the window pairs << 32
+  launch_brick_flag_counts(any, wany, nbr, bcnt, s);
+  if ((rc = u64_scan(c, bcnt, boff, nbr, c->d_total + 8))) return rc;
   uint64_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIPCHK(c, hipMemcpyAsync(t8, c->d_total + 2, sizeof(t8), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   if (t8[4]) return fail(c, MRF_E_LIMIT, "traversal: a ray has more records than its bound");
-  const uint64_t np64 = t8[6];
+  const uint64_t np64 = t8[6] & 0xFFFFFFFFull;
   // the ray kernels address a camera's per-pair messages with 32-bit byte offsets
   if (np64 >= (1ull << 30)) return fail(c, MRF_E_LIMIT, "more than 2^30 (camera, voxel) pairs in one camera");
   const uint32_t np = (uint32_t)np64;
@@ -724,7 +726,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   cam.nwin = nwin;
   cam.nlist = t8[2];
   cam.nrays = (uint32_t)t8[3];
-  const uint32_t nwp = (uint32_t)t8[7];
+  const uint32_t nwp = (uint32_t)(t8[6] >> 32);
   cam.npairs = np;
   cam.nwp = nwp;
   if ((rc = dalloc(c, &cam.d_pair_vox, np)) || (rc = dalloc(c, &cam.d_pair_c, np)) ||
@@ -732,7 +734,7 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
       (rc = dalloc(c, &cam.d_lnn, (size_t)std::max<uint32_t>(np, 1) * msg_bytes(msg_in_type(c->p.msg_dtype)))) ||
       (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
     return rc;
-  launch_pair_table(any, wany, pid, rank_w, nvox, nwp, cam.d_pair_vox, s);
+  launch_pair_table(any, wany, boff, nbr, nwp, pid, cam.d_pair_vox, s);
   // 5. per pair: C and the constant front messages; records point at their
   //    pairs from here on.  The same pass writes the window sort's input
   const int wbits = std::max(1, bit_length(nwp));

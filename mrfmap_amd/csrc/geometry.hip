@@ -34,20 +34,68 @@ namespace {
 
 constexpr double kFixLen = 16777216.0;  // 2^24: chord lengths (< 2) in fixed point, a 64-record sum < 2^31
 
-// pid[v] (written over rank_all) and pair_vox[pid] of every touched voxel;
-// the flags are cleared behind (all zero again for the next camera)
-__global__ void k_pair_table(uint8_t* __restrict__ any, uint8_t* __restrict__ wany,
-                             uint32_t* __restrict__ rank_all, const uint32_t* __restrict__ rank_w, uint64_t nvox,
-                             uint32_t nwp, uint32_t* __restrict__ pair_vox) {
-  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvox || !any[v]) return;
-  const uint32_t rw = rank_w[v];
-  const bool w = wany[v] != 0;
-  const uint32_t p = w ? rw : nwp + (rank_all[v] - rw);
-  rank_all[v] = p;
-  pair_vox[p] = (uint32_t)v;
-  any[v] = 0;
-  if (w) wany[v] = 0;
+// Pair numbering by brick (one wavefront per brick of the pool, 8 voxels
+// per lane, the byte flags read 8 at a time): a voxel's rank among the
+// touched voxels in voxel order is its brick's offset plus the touched
+// voxels before it in the brick.
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+  for (int o = 32; o > 0; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_brick_flag_counts(const uint8_t* __restrict__ any,
+                                                           const uint8_t* __restrict__ wany, uint32_t nbricks,
+                                                           uint64_t* __restrict__ counts) {
+  const uint32_t b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= nbricks) return;  // uniform per wavefront
+  const uint32_t l = threadIdx.x & 63u;
+  const uint64_t a = reinterpret_cast<const uint64_t*>(any)[(size_t)b * 64 + l];
+  const uint64_t w = reinterpret_cast<const uint64_t*>(wany)[(size_t)b * 64 + l];
+  const uint32_t na = wave_sum_u32((uint32_t)__popcll(a)), nw = wave_sum_u32((uint32_t)__popcll(w));
+  if (l == 0) counts[b] = (uint64_t)na | ((uint64_t)nw << 32);
+}
+
+// pid[v] and pair_vox[pid] of every touched voxel (window pairs first, the
+// ranks of wany; then the others, nwp + the ranks of any minus those of
+// wany); the flags are cleared behind (all zero again for the next camera)
+__global__ __launch_bounds__(256) void k_pair_table(uint8_t* __restrict__ any, uint8_t* __restrict__ wany,
+                                                    const uint64_t* __restrict__ boff, uint32_t nbricks,
+                                                    uint32_t nwp, uint32_t* __restrict__ pid,
+                                                    uint32_t* __restrict__ pair_vox) {
+  const uint32_t b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= nbricks) return;  // uniform per wavefront
+  const uint32_t l = threadIdx.x & 63u;
+  uint64_t* A = reinterpret_cast<uint64_t*>(any) + (size_t)b * 64 + l;
+  uint64_t* Wf = reinterpret_cast<uint64_t*>(wany) + (size_t)b * 64 + l;
+  const uint64_t a = *A, w = *Wf;
+  if (__ballot(a != 0) == 0) return;  // no touched voxel in the brick (uniform)
+  // exclusive prefix of the lanes' counts
+  const uint32_t na = (uint32_t)__popcll(a), nw = (uint32_t)__popcll(w);
+  uint32_t ia = na, iw = nw;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t ua = (uint32_t)__shfl_up((int)ia, o, 64), uw = (uint32_t)__shfl_up((int)iw, o, 64);
+    if (l >= (uint32_t)o) {
+      ia += ua;
+      iw += uw;
+    }
+  }
+  const uint64_t o = boff[b];
+  uint32_t ra = (uint32_t)o + ia - na, rw = (uint32_t)(o >> 32) + iw - nw;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const bool fa = ((a >> (8 * k)) & 1u) != 0, fw = ((w >> (8 * k)) & 1u) != 0;
+    if (fa) {
+      const uint32_t v = b * (uint32_t)kBrickVox + l * 8u + (uint32_t)k;
+      const uint32_t p = fw ? rw : nwp + (ra - rw);
+      pid[v] = p;
+      pair_vox[p] = v;
+    }
+    ra += fa ? 1u : 0u;
+    rw += fw ? 1u : 0u;
+  }
+  if (a) *A = 0;
+  if (w) *Wf = 0;
 }
 
 // value of lane ^ J; J < 32 stays inside the 32-lane half (ds_swizzle, bitmask mode)
@@ -308,10 +356,16 @@ size_t window_sort_temp_bytes(uint32_t n, int bits) {
   return bytes;
 }
 
-void launch_pair_table(uint8_t* any, uint8_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
-                       uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s) {
-  if (nvox == 0) return;
-  hipLaunchKernelGGL(k_pair_table, dim3(blocks(nvox, 256)), dim3(256), 0, s, any, wany, rank_all, rank_w, nvox, nwp,
+void launch_brick_flag_counts(const uint8_t* any, const uint8_t* wany, uint32_t nbricks, uint64_t* counts,
+                              hipStream_t s) {
+  if (nbricks == 0) return;
+  hipLaunchKernelGGL(k_brick_flag_counts, dim3(blocks(nbricks, 4)), dim3(256), 0, s, any, wany, nbricks, counts);
+}
+
+void launch_pair_table(uint8_t* any, uint8_t* wany, const uint64_t* boff, uint32_t nbricks, uint32_t nwp,
+                       uint32_t* pid, uint32_t* pair_vox, hipStream_t s) {
+  if (nbricks == 0) return;
+  hipLaunchKernelGGL(k_pair_table, dim3(blocks(nbricks, 4)), dim3(256), 0, s, any, wany, boff, nbricks, nwp, pid,
                      pair_vox);
 }
 

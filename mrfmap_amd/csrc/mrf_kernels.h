@@ -62,10 +62,12 @@ void launch_eval(const CamGeom& c, const GridGeom& g, const float* depth, const 
 size_t flag_scan_temp_bytes(size_t n);
 hipError_t launch_flag_scan(const uint32_t* in, uint32_t* out, size_t n, void* temp, size_t temp_bytes,
                             uint64_t* d_total, hipStream_t s);
-// the same over byte flags (temp: flag_scan_u8_temp_bytes(n) bytes)
-size_t flag_scan_u8_temp_bytes(size_t n);
-hipError_t launch_flag_scan_u8(const uint8_t* in, uint32_t* out, size_t n, void* temp, size_t temp_bytes,
-                               uint64_t* d_total, hipStream_t s);
+// exclusive scan of n uint64 values (n < 2^31; sums below 2^64), total into
+// *d_total; temp holds u64_scan_temp_bytes(n) bytes
+size_t u64_scan_temp_bytes(size_t n);
+hipError_t launch_u64_scan(const uint64_t* in, uint64_t* out, size_t n, void* temp, size_t temp_bytes,
+                           uint64_t* d_total, hipStream_t s);
+
 size_t scan_tmp_words(size_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint64_t* tmp, uint64_t* d_total,
                            hipStream_t s);
@@ -74,11 +76,15 @@ int num_tiled_pixels(const CamGeom& c);
 
 // geometry.hip: (camera, voxel) pairs of a camera's incidences and the
 // window segment layout of the BP iteration.
-// from the exclusive scans rank_all (of any) and rank_w (of wany): pair id
-// of every touched voxel (window pairs first, each group in voxel order),
-// written over rank_all, and pair_vox[pair]; clears any and wany behind
-void launch_pair_table(uint8_t* any, uint8_t* wany, uint32_t* rank_all, const uint32_t* rank_w,
-                       uint64_t nvox, uint32_t nwp, uint32_t* pair_vox, hipStream_t s);
+// per brick of the pool: the voxels with any / wany set, packed as
+// count_any | count_wany << 32 (the flags are bytes of 0 or 1)
+void launch_brick_flag_counts(const uint8_t* any, const uint8_t* wany, uint32_t nbricks, uint64_t* counts,
+                              hipStream_t s);
+// from the exclusive scan boff of those counts: pid[v] and pair_vox[pid] of
+// every touched voxel v (window pairs first, each group in voxel order;
+// other entries of pid are left as they are); clears any and wany behind
+void launch_pair_table(uint8_t* any, uint8_t* wany, const uint64_t* boff, uint32_t nbricks, uint32_t nwp,
+                       uint32_t* pid, uint32_t* pair_vox, hipStream_t s);
 // per pair: C (summed chord length) and F_EPS x the chord length of its
 // front records (index < base_w), exact fixed-point sums; acc_w (window
 // records) / acc_f (front records) / acc_r (C below 2^-24) hold 2 words per

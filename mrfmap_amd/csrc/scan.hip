@@ -3,11 +3,12 @@
 //  * launch_exclusive_scan: three launches (block totals, one-block scan of
 //    the totals, block-local scan + offset) with a 64-bit total, for counts
 //    whose sum may exceed 32 bits (list offsets; the callers check limits).
-//  * launch_flag_scan / launch_flag_scan_u8: 0/1 flags (uint32 or byte
-//    words), whose sum is at most n, so a 32-bit scan is exact: hipCUB's
-//    single-pass scan and a one-thread total; used for the brick-pool-sized
-//    flag arrays, where the three-launch scan's one-block middle step
-//    dominates.
+//  * launch_flag_scan: 0/1 flags, whose sum is at most n, so a 32-bit scan is
+//    exact: hipCUB's single-pass scan and a one-thread total; used for the
+//    brick-pool-sized flag arrays, where the three-launch scan's one-block
+//    middle step dominates.
+//  * launch_u64_scan: the same single-pass scan over uint64 values (the
+//    per-brick counts of two flags, packed).
 #include <hipcub/hipcub.hpp>
 
 #include "mrf_kernels.h"
@@ -86,17 +87,15 @@ __global__ __launch_bounds__(kThreads) void k_apply(const uint32_t* __restrict__
   }
 }
 
-template <typename F>
-__global__ void k_flag_total(const F* __restrict__ in, const uint32_t* __restrict__ out, size_t n,
+__global__ void k_flag_total(const uint32_t* __restrict__ in, const uint32_t* __restrict__ out, size_t n,
                              uint64_t* __restrict__ d_total) {
   if (threadIdx.x == 0) *d_total = (uint64_t)out[n - 1] + in[n - 1];
 }
 
-// byte flags read as uint32 values (the scan's sums are uint32)
-struct ByteToU32 {
-  __host__ __device__ uint32_t operator()(uint8_t x) const { return x; }
-};
-using ByteFlags = hipcub::TransformInputIterator<uint32_t, ByteToU32, const uint8_t*>;
+__global__ void k_u64_total(const uint64_t* __restrict__ in, const uint64_t* __restrict__ out, size_t n,
+                            uint64_t* __restrict__ d_total) {
+  if (threadIdx.x == 0) *d_total = out[n - 1] + in[n - 1];
+}
 
 }  // namespace
 
@@ -115,27 +114,26 @@ hipError_t launch_flag_scan(const uint32_t* in, uint32_t* out, size_t n, void* t
   size_t tb = temp_bytes;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, tb, in, out, (int)n, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_flag_total<uint32_t>, dim3(1), dim3(64), 0, s, in, out, n, d_total);
+  hipLaunchKernelGGL(k_flag_total, dim3(1), dim3(64), 0, s, in, out, n, d_total);
   return hipGetLastError();
 }
 
-size_t flag_scan_u8_temp_bytes(size_t n) {
+size_t u64_scan_temp_bytes(size_t n) {
   size_t bytes = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ByteFlags(nullptr, ByteToU32()), (uint32_t*)nullptr,
-                                         (int)n);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr, (int)n);
   return bytes;
 }
 
-hipError_t launch_flag_scan_u8(const uint8_t* in, uint32_t* out, size_t n, void* temp, size_t temp_bytes,
-                               uint64_t* d_total, hipStream_t s) {
+hipError_t launch_u64_scan(const uint64_t* in, uint64_t* out, size_t n, void* temp, size_t temp_bytes,
+                           uint64_t* d_total, hipStream_t s) {
   if (n == 0) {
     launch_zero_words(d_total, sizeof(uint64_t), s);
     return hipGetLastError();
   }
   size_t tb = temp_bytes;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, tb, ByteFlags(in, ByteToU32()), out, (int)n, s);
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, tb, in, out, (int)n, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_flag_total<uint8_t>, dim3(1), dim3(64), 0, s, in, out, n, d_total);
+  hipLaunchKernelGGL(k_u64_total, dim3(1), dim3(64), 0, s, in, out, n, d_total);
   return hipGetLastError();
 }
 
