@@ -49,6 +49,9 @@ class GVDBInference:
         self._cam_shapes = {}
 
     def close(self):
+        """Destroy the context.  Its large device blocks stay with the library
+        for the next context; ``mrfmap_amd.release_device_cache()`` gives them
+        back to the HIP runtime."""
         if getattr(self, "_h", None):
             self._L.mrf_destroy(self._h)
             self._h = None
