@@ -735,8 +735,8 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
       (rc = ensure_acc(c, std::max<uint32_t>(np, 1))))
     return rc;
   launch_pair_table(any, wany, boff, nbr, nwp, pid, cam.d_pair_vox, s);
-  // 5. per pair: C and the constant front messages; records point at their
-  //    pairs from here on.  The same pass writes the window sort's input
+  // 5. per pair: the front parts of C; records point at their pairs from
+  //    here on.  The same pass writes the window sort's input
   const int wbits = std::max(1, bit_length(nwp));
   if ((rc = ensure_sort(c, std::max<uint64_t>(nrec_w, 1), wbits))) return rc;
   // win_idx holds the sorted values of the whole window region; its first
@@ -744,8 +744,10 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
   if ((rc = dalloc(c, &cam.d_wseg_off, (size_t)nwp + 1)) ||
       (rc = dalloc(c, &cam.d_win_idx, std::max<uint64_t>(nrec_w, 1))))
     return rc;
-  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, c->d_acc, c->d_acc + 2 * (size_t)np, c->d_acc + 4 * (size_t)np,
-                      np, cam.d_pair_c, cam.d_pair_sf, c->d_keys_in, c->d_vals_in, nwp, s);
+  uint32_t* acc_w = c->d_acc;
+  uint32_t* acc_f = c->d_acc + 2 * (size_t)np;
+  uint32_t* acc_r = c->d_acc + 4 * (size_t)np;
+  launch_pair_lengths(cam.d_inc, nrec, nrec_f, pid, acc_f, acc_r, np, c->d_keys_in, c->d_vals_in, nwp, s);
   // 6. window segments: the window records stably sorted by pair
   HIPCHK(c, sort_window((uint32_t)nrec_w, wbits, c->d_keys_in, c->d_keys_out, c->d_vals_in, cam.d_win_idx,
                         c->d_sort_tmp, c->sort_tmp_bytes, s));
@@ -756,6 +758,9 @@ int traverse_camera(mrf_ctx* c, Camera& cam) {
     if ((rc = flag_scan(c, whead, hw, nwin))) return rc;
   }
   launch_window_offsets(c->d_keys_out, whead, (uint32_t)nwin, nwp, cam.d_wseg_off, s);
+  // C with its window part summed per segment, and the front messages
+  launch_pair_finish(cam.d_inc + nrec_f, c->d_keys_out, cam.d_win_idx, (uint32_t)nwin, nwp, acc_w, acc_f, acc_r, np,
+                     cam.d_pair_c, cam.d_pair_sf, s);
   // 7. window segment head bitmask and per-tile head counts
   const size_t nwords_w = nwin / 64 + 1, ntiles_w = nwin / kTile + 1;
   if ((rc = dalloc(c, &cam.d_heads, nwords_w)) || (rc = dalloc(c, &cam.d_tile_h0, ntiles_w))) return rc;

@@ -2,10 +2,11 @@
 // once per traversal, not per BP iteration).
 //
 // Pairs.  Every voxel the camera's live records touch is one (camera, voxel)
-// pair.  Two voxel flags over the brick pool -- touched by any record, by a
-// window record -- and their exclusive scans number the pairs: the pairs
-// with window records first, then the others, each group in voxel order, so
-// that window segment m belongs to pair m.
+// pair.  Two byte flags over the brick pool -- touched by any record, by a
+// window record -- number the pairs by their ranks (counts per brick, a scan
+// over bricks, ranks inside the brick): the pairs with window records first,
+// then the others, each group in voxel order, so that window segment m
+// belongs to pair m.
 //
 // Per pair, the summed chord length C (what render_cumulative_length_kernel
 // accumulates with float atomics in every camera pass,
@@ -22,8 +23,9 @@
 // Window segments.  The window records are stably radix-sorted by pair, so
 // each pair's records stay in record order; the BP iteration sums each
 // segment in that order, gathered through win_idx (no atomics,
-// bit-reproducible).  The pass that sums C also rewrites every live record
-// to hold its pair instead of its voxel.
+// bit-reproducible).  The pass that sums the front part of C also rewrites
+// every live record to hold its pair instead of its voxel and writes the
+// sort's input.
 #include <hipcub/hipcub.hpp>
 
 #include "mrf_kernels.h"
@@ -189,10 +191,11 @@ __device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t& x2, uint32
 // record's voxel to its pair, sorts the 64 (pair, fixed-point length) by
 // pair across its lanes (bitonic network), sums each run of equal pairs
 // (segmented scan) and issues one atomic per run instead of one per record.
-// Regions are whole rows of 64 records, so a row is either all front (its
-// lengths go to the front sum acc_f) or all window (acc_w); C is their sum
-// plus the parts below 2^-24 (acc_r), so a front run costs one atomic for
-// its 2^-24 units, not one for C and one for the front sum.
+// Regions are whole rows of 64 records, so a row is either all front or all
+// window.  Front runs go to the front sum acc_f and their parts below 2^-24
+// to acc_r; a window row only has its records rewritten: the window part of
+// C is summed after the window sort, where each pair's records are already
+// contiguous (k_window_lengths).  C is the window part plus acc_f plus acc_r.
 // PACKED (fewer than 2^26 - 1 pairs): the sort moves one word per lane, the
 // pair and the lane it came from (pair 2^6 + lane, unique), and each lane then
 // fetches its lengths from that lane; else (pair, q, r) move together.
@@ -201,7 +204,7 @@ __device__ __forceinline__ void wave_seg_sum_u(uint32_t& x, uint32_t& x2, uint32
 // sorting last) and value j.
 template <bool PACKED>
 __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, uint64_t nrec, uint64_t base_w,
-                                                      const uint32_t* __restrict__ pid, uint32_t* __restrict__ acc_w,
+                                                      const uint32_t* __restrict__ pid,
                                                       uint32_t* __restrict__ acc_f, uint32_t* __restrict__ acc_r,
                                                       uint32_t* __restrict__ wkeys, uint32_t* __restrict__ wvals,
                                                       uint32_t sentinel) {
@@ -243,10 +246,14 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
       } else {
         key[j] = 0xFFFFFFFFu;
       }
-      if ((row0 + j) * 64 >= base_w && row0 + j < nrows) {  // uniform per row
-        const uint64_t jw = (row0 + j) * 64 - base_w + lane;
-        wkeys[jw] = key[j] == 0xFFFFFFFFu ? sentinel : key[j];
-        wvals[jw] = (uint32_t)jw;
+      if ((row0 + j) * 64 >= base_w) {  // a window row (uniform): the sort input only
+        if (row0 + j < nrows) {
+          const uint64_t jw = (row0 + j) * 64 - base_w + lane;
+          wkeys[jw] = key[j] == 0xFFFFFFFFu ? sentinel : key[j];
+          wvals[jw] = (uint32_t)jw;
+        }
+        key[j] = 0xFFFFFFFFu;
+        continue;
       }
       if constexpr (PACKED) {
         uint32_t v = key[j] == 0xFFFFFFFFu ? 0xFFFFFFFFu : (key[j] << 6) | (uint32_t)lane;
@@ -269,16 +276,53 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
     for (int j = 0; j < kR; ++j) {
       po[j] = por[j] = 0;
       if (key[j] != 0xFFFFFFFFu) {
-        uint32_t* acc = ((row0 + j) * 64 < base_w ? acc_f : acc_w) + 2 * (size_t)key[j];  // the row's region
-        po[j] = atomicAdd(acc, q[j]);
+        po[j] = atomicAdd(acc_f + 2 * (size_t)key[j], q[j]);
         if (r[j]) por[j] = atomicAdd(acc_r + 2 * (size_t)key[j], r[j]);
       }
     }
 #pragma unroll
     for (int j = 0; j < kR; ++j) {
       if (key[j] == 0xFFFFFFFFu) continue;
-      if (po[j] + q[j] < po[j]) atomicAdd(((row0 + j) * 64 < base_w ? acc_f : acc_w) + 2 * (size_t)key[j] + 1, 1u);
+      if (po[j] + q[j] < po[j]) atomicAdd(acc_f + 2 * (size_t)key[j] + 1, 1u);
       if (r[j] && por[j] + r[j] < por[j]) atomicAdd(acc_r + 2 * (size_t)key[j] + 1, 1u);
+    }
+  }
+}
+
+// The window part of C of every window pair, from the sorted window records
+// (position m holds pair keys[m], record win_idx[m] of the window region): a
+// wave takes 64 consecutive positions, computes the same exact fixed-point
+// parts as k_pair_lengths, sums each run of equal pairs (segmented scan; the
+// runs are already contiguous) and adds it with one exact add per run into
+// acc_w (2^-24 units) and acc_r (below): one add per segment and wave it
+// touches, instead of one per run of every window row.
+__global__ __launch_bounds__(256) void k_window_lengths(const uint2* __restrict__ recw,
+                                                        const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ win_idx, uint32_t nwin,
+                                                        uint32_t* __restrict__ acc_w, uint32_t* __restrict__ acc_r) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t m0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
+  if (m0 >= nwin) return;  // uniform per wavefront
+  const uint32_t m = m0 + (uint32_t)lane;
+  const bool valid = m < nwin;
+  uint32_t key = 0xFFFFFFFFu, q = 0, r = 0;
+  if (valid) {
+    key = keys[m];
+    const double x = (double)__uint_as_float(recw[win_idx[m]].y) * kFixLen;
+    q = (uint32_t)x;
+    r = (uint32_t)((x - (double)q) * kFixLen);
+  }
+  const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
+  const uint32_t next = (uint32_t)__shfl_down((int)key, 1, 64);
+  wave_seg_sum_u(q, r, lane == 0 || prev != key ? 1u : 0u);
+  if ((lane == 63 || next != key) && key != 0xFFFFFFFFu) {
+    uint32_t* pw = acc_w + 2 * (size_t)key;
+    const uint32_t o = atomicAdd(pw, q);
+    if (o + q < o) atomicAdd(pw + 1, 1u);
+    if (r) {
+      uint32_t* pr = acc_r + 2 * (size_t)key;
+      const uint32_t orr = atomicAdd(pr, r);
+      if (orr + r < orr) atomicAdd(pr + 1, 1u);
     }
   }
 }
@@ -287,12 +331,13 @@ __global__ void __launch_bounds__(256) k_pair_lengths(uint2* __restrict__ rec, u
 // where rounding to float would move C onto kCumLenMin from above, C takes
 // the next float up, so that the float gate falls as on the exact sum
 __global__ void k_pair_finish(const uint32_t* __restrict__ acc_w, const uint32_t* __restrict__ acc_f,
-                              const uint32_t* __restrict__ acc_r, uint32_t np, float* __restrict__ pair_c,
-                              float* __restrict__ pair_sf) {
+                              const uint32_t* __restrict__ acc_r, uint32_t np, uint32_t nwp,
+                              float* __restrict__ pair_c, float* __restrict__ pair_sf) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  // integers below 2^53: the doubles and their sum are exact
-  const double w = (double)acc_w[2 * p + 1] * 4294967296.0 + (double)acc_w[2 * p];
+  // integers below 2^53: the doubles and their sum are exact; a front-only
+  // pair has no window part
+  const double w = p < nwp ? (double)acc_w[2 * p + 1] * 4294967296.0 + (double)acc_w[2 * p] : 0.0;
   const double cr = (double)acc_r[2 * p + 1] * 4294967296.0 + (double)acc_r[2 * p];
   const double f = (double)acc_f[2 * p + 1] * 4294967296.0 + (double)acc_f[2 * p];
   const double cd = (w + f) / kFixLen + cr / (kFixLen * kFixLen);
@@ -369,20 +414,29 @@ void launch_pair_table(uint8_t* any, uint8_t* wany, const uint64_t* boff, uint32
                      pair_vox);
 }
 
-void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_w,
-                         uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
-                         uint32_t* wkeys, uint32_t* wvals, uint32_t sentinel, hipStream_t s) {
+void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_f,
+                         uint32_t* acc_r, uint32_t np, uint32_t* wkeys, uint32_t* wvals, uint32_t sentinel,
+                         hipStream_t s) {
   if (np == 0) return;
-  launch_zero_words(acc_w, (size_t)np * 8, s);
   launch_zero_words(acc_f, (size_t)np * 8, s);
   launch_zero_words(acc_r, (size_t)np * 8, s);
   if (np < (1u << 26) - 1u)
-    hipLaunchKernelGGL(k_pair_lengths<true>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w,
-                       acc_f, acc_r, wkeys, wvals, sentinel);
+    hipLaunchKernelGGL(k_pair_lengths<true>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_f,
+                       acc_r, wkeys, wvals, sentinel);
   else
-    hipLaunchKernelGGL(k_pair_lengths<false>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid, acc_w,
+    hipLaunchKernelGGL(k_pair_lengths<false>, dim3(stride_blocks(nrec)), dim3(256), 0, s, rec, nrec, base_w, pid,
                        acc_f, acc_r, wkeys, wvals, sentinel);
-  hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_w, acc_f, acc_r, np, pair_c,
+}
+
+void launch_pair_finish(const uint2* recw, const uint32_t* keys, const uint32_t* win_idx, uint32_t nwin,
+                        uint32_t nwp, uint32_t* acc_w, const uint32_t* acc_f, uint32_t* acc_r, uint32_t np,
+                        float* pair_c, float* pair_sf, hipStream_t s) {
+  if (np == 0) return;
+  launch_zero_words(acc_w, (size_t)nwp * 8, s);
+  if (nwin)
+    hipLaunchKernelGGL(k_window_lengths, dim3(blocks(nwin, 256)), dim3(256), 0, s, recw, keys, win_idx, nwin, acc_w,
+                       acc_r);
+  hipLaunchKernelGGL(k_pair_finish, dim3(blocks(np, 256)), dim3(256), 0, s, acc_w, acc_f, acc_r, np, nwp, pair_c,
                      pair_sf);
 }
 

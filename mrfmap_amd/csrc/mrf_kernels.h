@@ -85,15 +85,22 @@ void launch_brick_flag_counts(const uint8_t* any, const uint8_t* wany, uint32_t 
 // other entries of pid are left as they are); clears any and wany behind
 void launch_pair_table(uint8_t* any, uint8_t* wany, const uint64_t* boff, uint32_t nbricks, uint32_t nwp,
                        uint32_t* pid, uint32_t* pair_vox, hipStream_t s);
-// per pair: C (summed chord length) and F_EPS x the chord length of its
-// front records (index < base_w), exact fixed-point sums; acc_w (window
-// records) / acc_f (front records) / acc_r (C below 2^-24) hold 2 words per
-// pair.  Rewrites every live record's voxel to its pair (pid), and writes the
-// window sort input: wkeys[j] = pair of window record j (sentinel for a gap
-// or padding record), wvals[j] = j.
-void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_w,
-                         uint32_t* acc_f, uint32_t* acc_r, uint32_t np, float* pair_c, float* pair_sf,
-                         uint32_t* wkeys, uint32_t* wvals, uint32_t sentinel, hipStream_t s);
+// per pair: the exact fixed-point sums of its front records' chord lengths
+// (acc_f, 2 words per pair) and of the parts below 2^-24 of its front
+// records' lengths (acc_r).  Rewrites every live record's voxel to its pair
+// (pid), and writes the window sort input: wkeys[j] = pair of window record
+// j (sentinel for a gap or padding record), wvals[j] = j.
+void launch_pair_lengths(uint2* rec, uint64_t nrec, uint64_t base_w, const uint32_t* pid, uint32_t* acc_f,
+                         uint32_t* acc_r, uint32_t np, uint32_t* wkeys, uint32_t* wvals, uint32_t sentinel,
+                         hipStream_t s);
+// after the window sort (sorted pair keys and win_idx of the nwin live
+// window records): the window part of every window pair (acc_w; the parts
+// below 2^-24 into acc_r), then per pair C (summed chord length, exact, FP32
+// with the exact sum's gate) and F_EPS x the chord length of its front
+// records
+void launch_pair_finish(const uint2* recw, const uint32_t* keys, const uint32_t* win_idx, uint32_t nwin,
+                        uint32_t nwp, uint32_t* acc_w, const uint32_t* acc_f, uint32_t* acc_r, uint32_t np,
+                        float* pair_c, float* pair_sf, hipStream_t s);
 size_t window_sort_temp_bytes(uint32_t n, int bits);
 // window region records stably sorted by pair (gap / padding records, with
 // the sentinel key, last): keys_out = pairs, vals_out = record index in the
