@@ -391,6 +391,9 @@ def main():
         "reference_run": {"iters": conv["iters"], "residual": conv["residual"], "tol": a.tol,
                           "ms": conv["t_total_ms"]},
         "geometry_ms": conv["t_geometry_ms"],
+        "geometry_note": "geometry_ms is the first build of the process: it also pays the HIP runtime's first use "
+                         "and the first provisioning of the list memory; the build of a fresh context later in "
+                         "the process is time_to_residual.geometry_ms",
         "roofline": {
             "kernel": "k_front_sums + k_ray_window (one camera pass)",
             "bound": "hbm",
