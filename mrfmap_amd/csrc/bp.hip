@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ p
   }
 }
 
-// ---------------------------------------------------------------- ray kernel
+// ---------------------------------------------------------------- list layout
 // Copy every ray's records into the front and window regions of the final
 // record buffer (sliced: lane l of slice t at region offset + k * 64 + l),
 // padding each lane to its slice's length with gap records.  One wavefront
@@ -319,6 +319,7 @@ __global__ void k_gather_desc(const uint4* __restrict__ desc, const uint32_t* __
   if (j < n) out[j] = desc[idx[j]];
 }
 
+// ---------------------------------------------------------------- ray kernels
 // Front sums.  Front records have psi(s) = 0, so all they contribute to a ray
 // is the visibility sum Lambda = sum l log2 n (in units of ln 2; the reference's visibility
 // includes the voxel itself, gvdbkernels.cu:781-783); their messages are
