@@ -229,19 +229,22 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
   const int64_t s0 = t[0].d << kFix, s1 = t[1].d << kFix, s2 = t[2].d << kFix;
   int64_t n0 = t[0].n, n1 = t[1].n, n2 = t[2].n;
   const int64_t d0 = t[0].d, d1 = t[1].d, d2 = t[2].d;
+  const double dd0 = (double)d0, dd1 = (double)d1, dd2 = (double)d2;  // exact (below 2^53)
   int64_t X01 = n0 * d1 - n1 * d0, X02 = n0 * d2 - n2 * d0, X12 = n1 * d2 - n2 * d1;
   const int max_steps = g.nx + g.ny + g.nz + 3;
   for (int step = 0; step < max_steps; ++step) {
-    // the nearest crossing; the lowest axis among equals (strict comparisons)
-    int best = m0 ? 0 : (m1 ? 1 : 2);
-    if (m1 && best == 0 && X01 > 0) best = 1;
-    if (m2 && best != 2 && (best == 0 ? X02 > 0 : X12 > 0)) best = 2;
-    const Rat zn = {best == 0 ? n0 : (best == 1 ? n1 : n2), best == 0 ? d0 : (best == 1 ? d1 : d2)};
+    // the nearest crossing; the lowest axis among equals (strict
+    // comparisons), in selects rather than branches: axis 1 beats axis 0
+    // (or axis 0 does not move), axis 2 beats the better of the two
+    const bool b1 = m1 & (!m0 | (X01 > 0));
+    const bool b2 = m2 & (b1 ? (X12 > 0) : (!m0 | (X02 > 0)));
+    const int best = b2 ? 2 : (b1 ? 1 : 0);
+    const Rat zn = {b2 ? n2 : (b1 ? n1 : n0), b2 ? d2 : (b1 ? d1 : d0)};
     // per-voxel depths only where records or visits need them; the counting
     // walk converts the exit depth at brick exits alone
     constexpr bool kDepths = MODE == kEmit || MODE == kVisit;
     double zn_d = 0.0;
-    if constexpr (kDepths) zn_d = (double)zn.n / (double)zn.d;
+    if constexpr (kDepths) zn_d = (double)zn.n / (b2 ? dd2 : (b1 ? dd1 : dd0));
     const bool active = bslot >= 0;
     if (active) {
       if (any && skipped) {
@@ -276,26 +279,18 @@ __device__ WalkOut walk(const GridGeom& g, const RaySetup& rs, double depth, con
     } else if (any) {
       skipped = true;
     }
-    // every axis whose crossing equals the nearest one steps
-    const bool st0 = m0 && (best == 0 || (best == 1 ? X01 == 0 : X02 == 0));
-    const bool st1 = m1 && (best == 1 || (best == 0 ? X01 == 0 : X12 == 0));
-    const bool st2 = m2 && (best == 2 || (best == 0 ? X02 == 0 : X12 == 0));
+    // every axis whose crossing equals the nearest one steps (selects, no branches)
+    const bool st0 = m0 & ((best == 0) | ((best == 1) & (X01 == 0)) | ((best == 2) & (X02 == 0)));
+    const bool st1 = m1 & ((best == 1) | ((best == 0) & (X01 == 0)) | ((best == 2) & (X12 == 0)));
+    const bool st2 = m2 & ((best == 2) | ((best == 0) & (X02 == 0)) | ((best == 1) & (X12 == 0)));
     const int nx[3] = {ix[0] + (st0 ? sg[0] : 0), ix[1] + (st1 ? sg[1] : 0), ix[2] + (st2 ? sg[2] : 0)};
-    if (st0) {
-      n0 += (int64_t)1 << kFix;
-      X01 += s1;
-      X02 += s2;
-    }
-    if (st1) {
-      n1 += (int64_t)1 << kFix;
-      X01 -= s0;
-      X12 += s2;
-    }
-    if (st2) {
-      n2 += (int64_t)1 << kFix;
-      X02 -= s0;
-      X12 -= s1;
-    }
+    const int64_t one = (int64_t)1 << kFix;
+    n0 += st0 ? one : 0;
+    n1 += st1 ? one : 0;
+    n2 += st2 ? one : 0;
+    X01 += (st0 ? s1 : 0) - (st1 ? s0 : 0);
+    X02 += (st0 ? s2 : 0) - (st2 ? s0 : 0);
+    X12 += (st1 ? s2 : 0) - (st2 ? s1 : 0);
     const bool leaving = nx[0] < 0 || nx[0] >= N[0] || nx[1] < 0 || nx[1] >= N[1] || nx[2] < 0 ||
                          nx[2] >= N[2];
     const bool brick_change = leaving || ((nx[0] >> kBrickLog) != (ix[0] >> kBrickLog)) ||
