@@ -517,7 +517,8 @@ def test_full_ray_activation_on_tied_rays():
     axis-aligned rotations: rays along an axis (two axes never move), through
     voxel edges and brick corners at every step (two or three axes tied), at
     half slopes (ties every other step), truncated inside the grid or leaving
-    it through a face: bricks, lists and marginals as the oracle's."""
+    it through a face, and rays that enter the grid from a camera outside it:
+    bricks, lists and marginals as the oracle's."""
     p, _, _ = synth.make_workload("small")
     n = 129
     K = np.array([[64, 0, 64.0], [0, 64, 64.0], [0, 0, 1]], np.float32)
@@ -532,7 +533,14 @@ def test_full_ray_activation_on_tied_rays():
         return T
 
     Rx = np.array([[1, 0, 0], [0, 0, -1], [0, 1, 0]], np.float32)  # optical axis along -y
-    frames = [(pose(np.eye(3), (48, 48, 16)), K, depth), (pose(Rx, (40, 64, 24)), K, depth)]
+    # the third camera sits below the grid: its rays enter through the z = 0
+    # face; its depth puts its surface on the first camera's (z = 66 voxels),
+    # since contradicting surfaces make the FP32 and FP64 maps drift apart
+    # over the iterations (9.7e-4 after three with the first camera's depth)
+    depth3 = np.full((n, n), 3.28, np.float32)
+    depth3[:, 96:] = 9.0
+    frames = [(pose(np.eye(3), (48, 48, 16)), K, depth), (pose(Rx, (40, 64, 24)), K, depth),
+              (pose(np.eye(3), (56, 40, -16)), K, depth3)]
     g = gpu_for(p, frames)
     o = oracle_for(p, frames)
     g.prepare()
