@@ -39,6 +39,10 @@ inline void check(int rc, mrf_ctx* c, const char* where) {
   }
 }
 
+/* Gives the large device blocks that destroyed contexts left with the
+ * library back to the HIP runtime (mrf_release_device_cache). */
+inline void release_device_cache() { check(mrf_release_device_cache(), nullptr, "mrf_release_device_cache"); }
+
 /* gvdb_params_t with the reference defaults (GVDBParams.h:32-55). */
 struct Params : mrf_params {
   Params() { mrf_params_default(this); }

@@ -73,13 +73,21 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "input longer than its header says\n");
       return 2;
     }
-    mrfmap_hip::GVDBInference inf(p);
-    for (int c = 0; c < ncams; ++c)
-      inf.add_camera_with_depth(poses[c].data(), Ks[c].data(), depths[c].data(), rows, cols, MRF_ROW_MAJOR);
-    const mrf_stats st = inf.perform_inference();
-    const std::vector<float> a = inf.get_alphas();
-    const std::vector<float> centres = inf.get_occupied_bricks();
-    const int64_t nb = inf.num_bricks();
+    mrf_stats st;
+    std::vector<float> a, centres;
+    int64_t nb = 0;
+    uint32_t ncams_ctx = 0;
+    {
+      mrfmap_hip::GVDBInference inf(p);
+      for (int c = 0; c < ncams; ++c)
+        inf.add_camera_with_depth(poses[c].data(), Ks[c].data(), depths[c].data(), rows, cols, MRF_ROW_MAJOR);
+      st = inf.perform_inference();
+      a = inf.get_alphas();
+      centres = inf.get_occupied_bricks();
+      nb = inf.num_bricks();
+      ncams_ctx = inf.num_cameras();
+    }
+    mrfmap_hip::release_device_cache();  // the context's kept device blocks back to the runtime
     FILE* out = std::fopen(argv[2], "wb");
     if (!out) return 2;
     const int32_t iters = st.iters;
@@ -87,7 +95,7 @@ int main(int argc, char** argv) {
                     std::fwrite(centres.data(), 4, centres.size(), out) == centres.size() &&
                     std::fwrite(&iters, 4, 1, out) == 1;
     std::fclose(out);
-    std::printf("cameras %u bricks %lld iterations %d\n", inf.num_cameras(), (long long)nb, st.iters);
+    std::printf("cameras %u bricks %lld iterations %d\n", ncams_ctx, (long long)nb, st.iters);
     return ok ? 0 : 2;
   } catch (const mrfmap_hip::Error& e) {
     std::fprintf(stderr, "%s\n", e.what());
