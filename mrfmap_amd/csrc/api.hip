@@ -798,17 +798,16 @@ int prepare(mrf_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipGetLastError());
   // 2. slots for new bricks, ascending linear brick index
-  bool grew = false;
   for (int lin = 0; lin < c->nbricks; ++lin) {
     if (flags[lin] && c->h_slot[lin] < 0) {
       c->h_slot[lin] = (int32_t)c->slot_lin.size();
       c->slot_lin.push_back(lin);
-      grew = true;
     }
   }
   if ((rc = ensure_pool(c, c->slot_lin.size()))) return rc;
-  if (grew || true)
-    HIPCHK(c, hipMemcpyAsync(c->d_slot, c->h_slot.data(), c->h_slot.size() * 4, hipMemcpyHostToDevice, c->stream));
+  // the directory is uploaded by every build, grown or not: the device copy
+  // is not initialised when the context is created (4 B per brick of the grid)
+  HIPCHK(c, hipMemcpyAsync(c->d_slot, c->h_slot.data(), c->h_slot.size() * 4, hipMemcpyHostToDevice, c->stream));
   // 3. traversal lists and pairs of every camera; the voxel flags of the
   //    pair numbering start cleared (each camera clears its own behind)
   {
