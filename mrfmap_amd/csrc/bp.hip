@@ -995,36 +995,25 @@ __global__ __launch_bounds__(256) void k_seg_sum(const uint64_t* __restrict__ he
 // a = clamp(S/C), m = ln a - ln(1-a), B += m - M_c, M_c = m, for C > 0.1f
 // (compute_gvdb_message_to_factors, gvdbkernels.cu:408, 457-479).
 // S = (sum of the pair's constant front messages) + (its window segment
-// sum); window pairs are numbered first.
-// m = logit(clamp(S / C)), B += m - M_c, M_c = m for pair q
-// Window segment sum of window pair q.  A segment inside one tile of the
-// segment kernel has its sum in S; one that crosses tiles is joined from
-// its parts: the head tile's pl, the tiles inside it (mid), the tail tile's pf.
-__device__ __forceinline__ float window_sum(uint32_t q, const uint32_t* __restrict__ wseg_off,
-                                            const float* __restrict__ S, const float* __restrict__ pl_v,
-                                            const float* __restrict__ pf_v, const float* __restrict__ mid) {
-  const uint32_t w = wseg_off[q] / kTile, wt = (wseg_off[q + 1] - 1u) / kTile;
-  if (w == wt) return S[q];
-  float acc = 0.0f;
-  for (uint32_t t = w + 1; t < wt; ++t) acc += mid[t];
-  return (pl_v[w] + acc) + pf_v[wt];
-}
+// sum); window pairs are numbered first.  A window segment inside one tile
+// of the segment kernel has its sum in S; one that crosses tiles is joined
+// from its parts: the head tile's pl, the tiles inside it (mid), the tail
+// tile's pf.
 
-// Sw: the window segment sum (window pairs) or 0 (front-only pairs)
-__device__ __forceinline__ void seg_update_one(uint32_t q, const uint32_t* __restrict__ pair_vox,
-                                               const float* __restrict__ pair_c, const float* __restrict__ pair_sf,
-                                               float Sw, float* __restrict__ B, float* __restrict__ M,
-                                               uint8_t* __restrict__ upd, bool mark) {
-  const float C = pair_c[q];
+// The update of pair q from its loaded operands: C, the front sum sf, the
+// window sum Sw (0 for a front-only pair), the voxel v, its belief Bv and
+// the pair's previous message mo.  Every operand is loaded before this test
+// (the belief right after the voxel), so that a pair costs two memory round
+// trips, not four.
+__device__ __forceinline__ void apply_pair(uint32_t q, float C, float sf, float Sw, uint32_t v, float Bv, float mo,
+                                           float* __restrict__ B, float* __restrict__ M,
+                                           uint8_t* __restrict__ upd, bool mark) {
   if (!(C > kCumLenMin)) return;
-  const uint32_t v = pair_vox[q];
-  const float Sq = pair_sf[q] + Sw;
-  float a = Sq / C;
+  float a = (sf + Sw) / C;
   a = fminf(fmaxf(a, kFEps), 1.0f - kFEps);
   const float m = __logf(a) - __logf(1.0f - a);
-  const float mo = M[q];
   M[q] = m;
-  B[v] = (B[v] + m) - mo;
+  B[v] = (Bv + m) - mo;
   if (mark) upd[v] = 1;
 }
 
@@ -1045,11 +1034,27 @@ __global__ __launch_bounds__(256) void k_seg_update(const uint32_t* __restrict__
                                                     uint8_t* __restrict__ upd, const uint32_t* __restrict__ all_pairs) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   const bool first = *all_pairs != 0u;
-  for (uint32_t q = tid; q < nwp; q += stride)
-    seg_update_one(q, pair_vox, pair_c, pair_sf, window_sum(q, wseg_off, S, pl_v, pf_v, mid) / out_scale, B, M, upd,
-                   first);
+  for (uint32_t q = tid; q < nwp; q += stride) {
+    const uint32_t o0 = wseg_off[q], o1 = wseg_off[q + 1];
+    const float Sq = S[q];  // stale for a segment that crosses tiles: not used then
+    const float C = pair_c[q], sf = pair_sf[q], mo = M[q];
+    const uint32_t v = pair_vox[q];
+    const float Bv = B[v];
+    const uint32_t w = o0 / kTile, wt = (o1 - 1u) / kTile;
+    float Sw = Sq;
+    if (w != wt) {
+      float acc = 0.0f;
+      for (uint32_t t = w + 1; t < wt; ++t) acc += mid[t];
+      Sw = (pl_v[w] + acc) + pf_v[wt];
+    }
+    apply_pair(q, C, sf, Sw / out_scale, v, Bv, mo, B, M, upd, first);
+  }
   if (first)
-    for (uint32_t q = nwp + tid; q < npairs; q += stride) seg_update_one(q, pair_vox, pair_c, pair_sf, 0.0f, B, M, upd, true);
+    for (uint32_t q = nwp + tid; q < npairs; q += stride) {
+      const float C = pair_c[q], sf = pair_sf[q], mo = M[q];
+      const uint32_t v = pair_vox[q];
+      apply_pair(q, C, sf, 0.0f, v, B[v], mo, B, M, upd, true);
+    }
 }
 
 // Beliefs of an external map (distinct voxels): B = the given log-odds,
