@@ -48,7 +48,7 @@ struct Camera {
   uint2* d_inc = nullptr;          // sliced records {pair | gap, length}: front region, then window region
   uint32_t* d_walkn = nullptr;     // full-ray activation: bound of the records per tiled pixel
   uint8_t* d_lnn = nullptr;        // per pair: voxel -> ray message log2 n (input-side message type)
-  uint32_t* d_dyn = nullptr;       // front-only pairs at voxels where some camera has window records
+  uint2* d_dyn = nullptr;          // front-only pairs at voxels where some camera has window records: {pair, voxel}
   uint32_t ndyn = 0;
   float* d_partial = nullptr;      // front partial sums (64 per chunk), kept between iterations
   uint4* d_dyn_desc = nullptr;        // descriptors of the front chunks whose partial sums keep changing
@@ -848,7 +848,7 @@ int prepare(mrf_ctx* c) {
       if (nrest) {
         launch_dyn_flags(cam.d_pair_vox + cam.nwp, nrest, W, flags, c->stream);
         if ((rc = flag_scan(c, flags, pos, nrest, c->d_total + 8))) return rc;
-        launch_dyn_compact(flags, pos, nrest, cam.nwp, cam.d_dyn, c->stream);
+        launch_dyn_pairs(flags, pos, nrest, cam.nwp, cam.d_pair_vox, cam.d_dyn, c->stream);
       }
       launch_chunk_static(cam_pass(c, ci), flags, cflags, cam.d_fdmask, crow, (unsigned long long*)(c->d_total + 2),
                           c->stream);

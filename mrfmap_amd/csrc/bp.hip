@@ -118,24 +118,30 @@ __device__ __forceinline__ uint32_t pair_of(uint64_t word, uint32_t before, int 
 // l log2 n and take exp2 of the sums (one multiply fewer per exponential than
 // e^x = 2^(x log2 e)).
 template <int MT>
-__device__ __forceinline__ void pair_msg_one(uint32_t q, const uint32_t* __restrict__ pair_vox,
-                                             const float* __restrict__ B, const float* __restrict__ M,
-                                             typename Msg<MT>::T* __restrict__ apair) {
-  const uint32_t v = pair_vox[q];
+__device__ __forceinline__ void pair_msg_qv(uint32_t q, uint32_t v, const float* __restrict__ B,
+                                            const float* __restrict__ M, typename Msg<MT>::T* __restrict__ apair) {
   const float qv = B[v] - M[q];
   float nn = 1.0f / (1.0f + __expf(qv));
   nn = fminf(fmaxf(nn, kFEps), 1.0f - kFEps);
   Msg<MT>::store(apair, q, __builtin_amdgcn_logf(nn));
 }
 
+template <int MT>
+__device__ __forceinline__ void pair_msg_one(uint32_t q, const uint32_t* __restrict__ pair_vox,
+                                             const float* __restrict__ B, const float* __restrict__ M,
+                                             typename Msg<MT>::T* __restrict__ apair) {
+  pair_msg_qv<MT>(q, pair_vox[q], B, M, apair);
+}
+
 // The window pairs [0, nwp) change every iteration; of the front-only pairs,
 // all while *msg_all (the first iterations of a call), afterwards only the
 // list dyn (front-only pairs at voxels that some camera reaches with window
-// records).  One launch: the window pairs one per thread, the others
+// records, each with its voxel, so that its message and belief are loaded
+// together).  One launch: the window pairs one per thread, the others
 // strided over the grid.
 template <int MT>
 __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ pair_vox, uint32_t nwp,
-                                                  uint32_t npairs, const uint32_t* __restrict__ dyn, uint32_t ndyn,
+                                                  uint32_t npairs, const uint2* __restrict__ dyn, uint32_t ndyn,
                                                   const float* __restrict__ B, const float* __restrict__ M,
                                                   typename Msg<MT>::T* __restrict__ apair,
                                                   const uint32_t* __restrict__ msg_all) {
@@ -144,7 +150,10 @@ __global__ __launch_bounds__(256) void k_pair_msg(const uint32_t* __restrict__ p
   if (*msg_all) {
     for (uint32_t q = nwp + tid; q < npairs; q += stride) pair_msg_one<MT>(q, pair_vox, B, M, apair);
   } else {
-    for (uint32_t k = tid; k < ndyn; k += stride) pair_msg_one<MT>(dyn[k], pair_vox, B, M, apair);
+    for (uint32_t k = tid; k < ndyn; k += stride) {
+      const uint2 e = dyn[k];  // {pair, voxel}
+      pair_msg_qv<MT>(e.x, e.y, B, M, apair);
+    }
   }
 }
 
@@ -1191,6 +1200,12 @@ __global__ void k_dyn_compact(const uint32_t* __restrict__ flags, const uint32_t
   if (k < n && flags[k]) dyn[pos[k]] = q0 + k;
 }
 
+__global__ void k_dyn_pairs(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
+                            uint32_t q0, const uint32_t* __restrict__ pair_vox, uint2* __restrict__ dyn) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n && flags[k]) dyn[pos[k]] = make_uint2(q0 + k, pair_vox[q0 + k]);
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1428,6 +1443,11 @@ void launch_dyn_flags(const uint32_t* pair_vox, uint32_t n, const uint32_t* W, u
 void launch_dyn_compact(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t q0, uint32_t* dyn,
                         hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_dyn_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, pos, n, q0, dyn);
+}
+
+void launch_dyn_pairs(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t q0, const uint32_t* pair_vox,
+                      uint2* dyn, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_dyn_pairs, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, pos, n, q0, pair_vox, dyn);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {

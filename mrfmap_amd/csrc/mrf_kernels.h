@@ -143,7 +143,7 @@ struct CamPass {
   const uint64_t* heads;   // window segment head bitmask
   const uint32_t* tile_h0; // window heads before each kTile tile
   void* apair;             // per-pair voxel -> ray message log2 n (input side type), kept per camera
-  const uint32_t* dyn;     // front-only pairs at voxels where some camera has window records
+  const uint2* dyn;        // front-only pairs at voxels where some camera has window records: {pair, voxel}
   uint32_t ndyn;
   const uint32_t* msg_all; // device countdown: nonzero while every pair's log2 n must be recomputed
   const uint4* dyn_desc;        // descriptors of the front chunks whose log2 n keep changing
@@ -237,8 +237,12 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_countdown(uint32_t* p, hipStream_t s);  // *p -= 1 unless 0
 void launch_mark_window_voxels(const uint32_t* pair_vox, uint32_t nwp, uint32_t* W, hipStream_t s);
 void launch_dyn_flags(const uint32_t* pair_vox, uint32_t n, const uint32_t* W, uint32_t* flags, hipStream_t s);
+// the flagged indices q0 + k at dyn[pos[k]]
 void launch_dyn_compact(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t q0, uint32_t* dyn,
                         hipStream_t s);
+// the flagged pairs q0 + k as {pair, its voxel} at dyn[pos[k]]
+void launch_dyn_pairs(const uint32_t* flags, const uint32_t* pos, uint32_t n, uint32_t q0, const uint32_t* pair_vox,
+                      uint2* dyn, hipStream_t s);
 void launch_zero_words(void* p, size_t bytes, hipStream_t s);
 
 // ---- learned warm start (warm.hip): features, MLP, warm-start update
